@@ -1,0 +1,258 @@
+"""dcsr_amd -- MI355X-native Dynamic-CSR engine (Python side).
+
+The compute path is the in-tree HIP library ``dcsr_amd/libdcsr.so`` (gfx950),
+reached through its C ABI (``include/dcsr.h``) with ctypes.  PyTorch only
+provides device memory and the current HIP stream.  There is no CPU fallback:
+if the library is missing, every call raises.
+
+``DcsrMatrix`` mirrors the reference's ``dcsr_matrix`` container and the
+``device::`` entry points that operate on it (reference include/load.h,
+include/spmv.h): ``Initialize_Matrix``, ``LoadMatrix`` (COO / CSR -> DCSR),
+``to_csr`` (DCSR -> CSR), ``UpdateMatrix``, ``SortMatrixRow`` (defrag),
+``BinRows`` and ``spmv``.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import re
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libdcsr.so")
+HEADER_PATH = os.path.join(os.path.dirname(_HERE), "include", "dcsr.h")
+
+DCSR_OK = 0
+STATUS_NAMES = {
+    0: "DCSR_OK", 1: "DCSR_ERR_INVALID", 2: "DCSR_ERR_BAD_INDEX", 3: "DCSR_ERR_POOL_EXHAUSTED",
+    4: "DCSR_ERR_SEGMENT_OVERFLOW", 5: "DCSR_ERR_HIP", 6: "DCSR_ERR_OOM", 7: "DCSR_ERR_CAPACITY",
+    8: "DCSR_ERR_UNSUPPORTED",
+}
+UPDATE_NOSYNC = 0x1
+UPDATE_NOGROW = 0x2
+EXPORT_CANONICAL = 0x1
+
+
+class DcsrError(RuntimeError):
+    def __init__(self, code, what=""):
+        self.code = int(code)
+        super().__init__(f"{what}: {STATUS_NAMES.get(self.code, self.code)}")
+
+
+class DcsrConfig(ctypes.Structure):
+    _fields_ = [("pool_factor", ctypes.c_double), ("min_pool", ctypes.c_int64),
+                ("allow_grow", ctypes.c_int32), ("reserved", ctypes.c_int32)]
+
+
+class DcsrInfo(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_int64) for n in (
+        "rows", "cols", "nnz", "pool_cap", "pool_used", "garbage", "hub_rows",
+        "last_inserted", "last_added", "last_deleted", "last_touched_rows")] + \
+        [("canonical", ctypes.c_int32), ("max_segments", ctypes.c_int32)]
+
+    def as_dict(self):
+        return {n: getattr(self, n) for n, _ in self._fields_}
+
+
+_lib = None
+_P = ctypes.c_void_p
+_I64 = ctypes.c_int64
+_U32 = ctypes.c_uint32
+_F = ctypes.c_float
+_H = ctypes.c_void_p
+
+
+def header_functions():
+    """Names of the functions declared in include/dcsr.h."""
+    with open(HEADER_PATH) as f:
+        txt = f.read()
+    return sorted(set(re.findall(r"^\s*(?:const\s+)?[a-z_0-9]+\s*\*?\s*(dcsr_[a-z0-9_]+)\s*\(", txt, re.M)))
+
+
+def lib():
+    """Load the in-tree HIP library (raises if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(f"{LIB_PATH} is missing: run python -m dcsr_amd.build (no CPU fallback exists)")
+    L = ctypes.CDLL(LIB_PATH)
+    sig = {
+        "dcsr_status_string": (ctypes.c_char_p, [ctypes.c_int]),
+        "dcsr_get_status": (ctypes.c_int, [_H]),
+        "dcsr_create": (ctypes.c_int, [ctypes.POINTER(_H), _I64, _I64, ctypes.POINTER(DcsrConfig)]),
+        "dcsr_destroy": (ctypes.c_int, [_H]),
+        "dcsr_initialize": (ctypes.c_int, [_H, ctypes.c_int32, _P]),
+        "dcsr_load_coo": (ctypes.c_int, [_H, _P, _P, _P, _I64, _P]),
+        "dcsr_load_csr": (ctypes.c_int, [_H, _P, _P, _P, _P]),
+        "dcsr_export_csr": (ctypes.c_int, [_H, _P, _P, _P, _I64, ctypes.POINTER(_I64), _U32, _P]),
+        "dcsr_update": (ctypes.c_int, [_H, _P, _P, _P, _I64, _U32, _P]),
+        "dcsr_compact": (ctypes.c_int, [_H, _P]),
+        "dcsr_rebin": (ctypes.c_int, [_H, _P]),
+        "dcsr_spmv": (ctypes.c_int, [_H, _P, _P, _F, _F, _P]),
+        "dcsr_query": (ctypes.c_int, [_H, ctypes.POINTER(DcsrInfo)]),
+        "dcsr_csr_spmv": (ctypes.c_int, [_I64, _P, _P, _P, _P, _P, _P]),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(L, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = L
+    return L
+
+
+def check(code, what=""):
+    if code != DCSR_OK:
+        raise DcsrError(code, what)
+
+
+def _torch():
+    import torch
+    return torch
+
+
+def _stream():
+    torch = _torch()
+    return ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+
+def _ptr(t):
+    return ctypes.c_void_p(t.data_ptr()) if t is not None and t.numel() else ctypes.c_void_p(0)
+
+
+def _dev(a, dtype):
+    """Host array / tensor -> contiguous CUDA tensor of dtype."""
+    torch = _torch()
+    if isinstance(a, torch.Tensor):
+        t = a.to(device="cuda", dtype=dtype)
+    else:
+        import numpy as np
+        t = torch.from_numpy(np.ascontiguousarray(a)).to(device="cuda", dtype=dtype)
+    return t.contiguous()
+
+
+class DcsrMatrix:
+    """Owning handle of one device-resident Dynamic-CSR matrix.
+
+    Mirrors ``dcsr_matrix<int, float, device_memory, BINS>`` (reference
+    include/dcsr_matrix.h:8-109) with its ``device::`` operations."""
+
+    def __init__(self, num_rows, num_cols, pool_factor=2.0, min_pool=1024, allow_grow=True):
+        torch = _torch()
+        torch.cuda.init()
+        self.num_rows = int(num_rows)
+        self.num_cols = int(num_cols)
+        cfg = DcsrConfig(pool_factor, min_pool, 1 if allow_grow else 0, 0)
+        h = _H()
+        check(lib().dcsr_create(ctypes.byref(h), self.num_rows, self.num_cols, ctypes.byref(cfg)), "dcsr_create")
+        self._h = h
+
+    # -- lifetime ----------------------------------------------------------
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().dcsr_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def handle(self):
+        return self._h
+
+    # -- reference API mirror ---------------------------------------------
+    def Initialize_Matrix(self, bin_length):
+        """device::Initialize_Matrix (reference include/load.h:8-22)."""
+        check(lib().dcsr_initialize(self._h, int(bin_length), _stream()), "dcsr_initialize")
+
+    def LoadMatrix(self, rows, cols, vals):
+        """device::LoadMatrix(dcsr&, rows, cols, vals, N) -- COO -> DCSR
+        (reference include/load.h:27-59).  Rows need not be sorted."""
+        torch = _torch()
+        r, c, v = _dev(rows, torch.int32), _dev(cols, torch.int32), _dev(vals, torch.float32)
+        if not (r.numel() == c.numel() == v.numel()):
+            raise ValueError("rows/cols/vals length mismatch")
+        check(lib().dcsr_load_coo(self._h, _ptr(r), _ptr(c), _ptr(v), r.numel(), _stream()), "dcsr_load_coo")
+
+    def load_csr(self, row_offsets, cols, vals):
+        """device::LoadMatrix(dcsr&, const csr_matrix&) (reference include/load.h:207-227)."""
+        torch = _torch()
+        o, c, v = _dev(row_offsets, torch.int32), _dev(cols, torch.int32), _dev(vals, torch.float32)
+        if o.numel() != self.num_rows + 1:
+            raise ValueError("row_offsets must have rows+1 entries")
+        check(lib().dcsr_load_csr(self._h, _ptr(o), _ptr(c), _ptr(v), _stream()), "dcsr_load_csr")
+
+    def to_csr(self, canonical=False):
+        """device::LoadMatrix(csr_matrix&, const dcsr&) -- DCSR -> CSR in
+        natural row order with columns ascending (reference include/load.h:232-253)."""
+        torch = _torch()
+        info = self.info()
+        cap = max(info["nnz"], 1)
+        off = torch.empty(self.num_rows + 1, dtype=torch.int32, device="cuda")
+        col = torch.empty(cap, dtype=torch.int32, device="cuda")
+        val = torch.empty(cap, dtype=torch.float32, device="cuda")
+        nnz = _I64(0)
+        check(lib().dcsr_export_csr(self._h, _ptr(off), _ptr(col), _ptr(val), cap, ctypes.byref(nnz),
+                                    EXPORT_CANONICAL if canonical else 0, _stream()), "dcsr_export_csr")
+        return off, col[: nnz.value], val[: nnz.value]
+
+    def UpdateMatrix(self, rows, cols, vals, flags=0):
+        """device::UpdateMatrix (reference include/load.h:258-325), with
+        insert-or-add semantics; rows need not be sorted."""
+        torch = _torch()
+        r, c, v = _dev(rows, torch.int32), _dev(cols, torch.int32), _dev(vals, torch.float32)
+        if not (r.numel() == c.numel() == v.numel()):
+            raise ValueError("rows/cols/vals length mismatch")
+        check(lib().dcsr_update(self._h, _ptr(r), _ptr(c), _ptr(v), r.numel(), int(flags), _stream()),
+              "dcsr_update")
+
+    def SortMatrixRow(self):
+        """device::SortMatrixRow -- defragmentation (reference include/load.h:404-581)."""
+        check(lib().dcsr_compact(self._h, _stream()), "dcsr_compact")
+
+    def BinRows(self):
+        """device::BinRows (reference include/load.h:583-610)."""
+        check(lib().dcsr_rebin(self._h, _stream()), "dcsr_rebin")
+
+    def spmv(self, x, y=None, alpha=1.0, beta=0.0):
+        """device::spmv(const dcsr&, x, y) (reference include/spmv.h:121-248):
+        y = alpha * A x + beta * y."""
+        torch = _torch()
+        x = _dev(x, torch.float32)
+        if x.numel() != self.num_cols:
+            raise ValueError("x must have num_cols entries")
+        if y is None:
+            y = torch.empty(self.num_rows, dtype=torch.float32, device="cuda")
+            beta = 0.0
+        elif y.numel() != self.num_rows or y.dtype != torch.float32 or not y.is_cuda or not y.is_contiguous():
+            raise ValueError("y must be a contiguous float32 CUDA tensor with num_rows entries")
+        check(lib().dcsr_spmv(self._h, _ptr(x), _ptr(y), float(alpha), float(beta), _stream()), "dcsr_spmv")
+        return y
+
+    def info(self):
+        inf = DcsrInfo()
+        check(lib().dcsr_query(self._h, ctypes.byref(inf)), "dcsr_query")
+        return inf.as_dict()
+
+    def status(self):
+        return lib().dcsr_get_status(self._h)
+
+    @property
+    def nnz(self):
+        return self.info()["nnz"]
+
+
+def csr_spmv(row_offsets, cols, vals, x, y=None):
+    """Plain CSR SpMV comparator (reference include/spmv.h:310-332)."""
+    torch = _torch()
+    o = _dev(row_offsets, torch.int32)
+    c = _dev(cols, torch.int32)
+    v = _dev(vals, torch.float32)
+    x = _dev(x, torch.float32)
+    R = o.numel() - 1
+    if y is None:
+        y = torch.empty(R, dtype=torch.float32, device="cuda")
+    check(lib().dcsr_csr_spmv(R, _ptr(o), _ptr(c), _ptr(v), _ptr(x), _ptr(y), _stream()), "dcsr_csr_spmv")
+    return y

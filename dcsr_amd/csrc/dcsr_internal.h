@@ -1,0 +1,123 @@
+// dcsr_internal.h -- handle layout, device state and shared host helpers.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <stddef.h>
+#include "../../include/dcsr.h"
+
+// Sticky device-side status bits.
+#define ST_BAD_INDEX 0x1u
+#define ST_POOL_EXHAUSTED 0x2u
+#define ST_SEGMENT_OVERFLOW 0x4u
+
+// Rows longer than this are "hubs": SpMV splits them over whole workgroups.
+#define DCSR_HUB_LEN 4096
+// Entries per hub chunk (one 256-thread workgroup).
+#define DCSR_HUB_CHUNK 8192
+
+// Pool chunk granularity: chunk starts stay 16-byte aligned.
+#define DCSR_CHUNK_ALIGN 4u
+
+// Device-resident counters of one handle.  Written by kernels, read back by
+// the host at the documented sync points.
+struct DevState {
+    unsigned long long bump;        // next free pool slot
+    unsigned long long bump_saved;  // bump before the current update's claims
+    long long nnz;                  // live entries
+    long long garbage;              // entries in retired chunks
+    long long d_nnz;                // current update: inserted - deleted
+    long long d_garbage;            // current update: retired capacity
+    long long n_ins, n_add, n_del;  // current update counters
+    long long d_hubs;               // current update: hub-row count delta
+    long long d_hub_chunks;         // current update: hub chunk delta
+    unsigned int status;            // sticky ST_* bits
+    unsigned int go;                // 1 if the current update may commit
+    unsigned int nuniq;             // unique coordinates in the current batch
+    unsigned int ntouched;          // rows touched by the current batch
+    unsigned int hub_count;         // entries in the hub list
+    unsigned int hub_chunks;        // total hub chunks
+    unsigned int nrewrite_old;      // old entries moved by the current update
+    unsigned int ubad;              // current batch holds an out-of-range index
+    unsigned long long need;        // current update: claimed capacity
+    unsigned int hub_touch;         // current update: rewritten rows that were/are hubs
+    unsigned int pad2;
+};
+
+// Grow-only device scratch buffers owned by a handle.
+enum WsSlot {
+    WS_KEY_A, WS_KEY_B, WS_VAL_A, WS_VAL_B, WS_FLAGS, WS_SCAN, WS_PART, WS_HIST,
+    WS_UCOL, WS_UVAL, WS_UT, WS_UPOS, WS_UCODE, WS_UPRE,
+    WS_TROW, WS_TUBEG, WS_TINS, WS_TDEL, WS_TOLDS, WS_TOLDL, WS_TDEST, WS_TNEWL, WS_TMODE, WS_TOLDPRE,
+    WS_HUB_ROWS, WS_HUB_PRE, WS_HUB_PART, WS_HUB_DONE,
+    WS_CANON, WS_TMP0, WS_TMP1, WS_TMP2,
+    WS_COUNT
+};
+
+struct Workspace {
+    void *ptr[WS_COUNT];
+    size_t bytes[WS_COUNT];
+};
+
+struct dcsr_matrix_s {
+    int64_t rows, cols;
+    int rbits, cbits;        // key = (row << cbits) | col
+    // row table
+    uint32_t *row_start;
+    int32_t *row_len;
+    int32_t *row_cap;
+    // pool
+    int32_t *pool_col;
+    float *pool_val;
+    int64_t pool_cap;
+    // device/host state
+    DevState *ds;            // device
+    DevState *hs;            // pinned host mirror
+    int64_t nnz_h, bump_h, garbage_h, hub_rows_h, hub_chunks_h;
+    bool canonical;
+    bool hubs_dirty;
+    bool state_known;        // host mirrors valid (no NOSYNC update since last sync)
+    dcsr_config cfg;
+    Workspace ws;
+    hipStream_t last_stream;
+    int64_t last_ins, last_add, last_del, last_touched;
+};
+
+// ---------------------------------------------------------------------------
+// host helpers (defined in matrix.hip)
+// ---------------------------------------------------------------------------
+#define DCSR_HIP(call)                                                     \
+    do {                                                                   \
+        hipError_t _e = (call);                                            \
+        if (_e != hipSuccess) return dcsr_hip_fail(_e, #call, __FILE__, __LINE__); \
+    } while (0)
+
+#define DCSR_TRY(call)                    \
+    do {                                  \
+        dcsr_status _s = (call);          \
+        if (_s != DCSR_OK) return _s;     \
+    } while (0)
+
+dcsr_status dcsr_hip_fail(hipError_t e, const char *what, const char *file, int line);
+dcsr_status ws_get(dcsr_handle h, WsSlot slot, size_t bytes, void **out);
+dcsr_status pull_state(dcsr_handle h, hipStream_t s);   // copy DevState to host + sync
+dcsr_status ensure_hub_list(dcsr_handle h, hipStream_t s);
+dcsr_status pool_reserve(dcsr_handle h, int64_t min_free, hipStream_t s, const int32_t *reserve_len);
+dcsr_status canonicalize(dcsr_handle h, hipStream_t s);
+
+static inline int bits_for(int64_t n)
+{
+    int b = 1;
+    while ((int64_t(1) << b) < n) ++b;
+    return b;
+}
+
+static inline unsigned grid_for(int64_t n, int per_block)
+{
+    int64_t g = (n + per_block - 1) / per_block;
+    return (unsigned)(g < 1 ? 1 : g);
+}
+
+__host__ __device__ static inline uint32_t chunk_round(uint32_t len)
+{
+    return (len + DCSR_CHUNK_ALIGN - 1u) & ~(DCSR_CHUNK_ALIGN - 1u);
+}

@@ -1,0 +1,778 @@
+// matrix.hip -- handle lifetime, COO/CSR -> DCSR load, DCSR -> CSR export,
+// compaction (defrag), pool growth, hub-row list, status plumbing.
+//
+// Reference counterparts: dcsr_matrix::resize (include/dcsr_matrix.h:41-108),
+// device::Initialize_Matrix (include/load.h:8-22), device::LoadMatrix
+// (include/load.h:27-59, 207-253), device::SortMatrixRow (include/load.h:404-581),
+// device::BinRows (include/load.h:583-610).
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+#include "dcsr_internal.h"
+#include "prims.h"
+#include "segments.h"
+
+// ---------------------------------------------------------------------------
+// kernels
+// ---------------------------------------------------------------------------
+namespace {
+
+constexpr int TB = 256;
+
+__global__ void k_fill_i32(int32_t *p, int64_t n, int32_t v)
+{
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        p[i] = v;
+}
+
+// Empty rows with capacity L: start = r*L (InitializeMatrix_dcsr,
+// include/sparse_update.inl:1155-1176).
+__global__ void k_init_rows(int64_t R, uint32_t L, uint32_t *start, int32_t *len, int32_t *cap, DevState *ds)
+{
+    int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r < R) {
+        start[r] = (uint32_t)(r * L);
+        len[r] = 0;
+        cap[r] = (int32_t)L;
+    }
+    if (r == 0) {
+        ds->bump = (unsigned long long)R * L;
+        ds->nnz = 0;
+        ds->garbage = 0;
+        ds->status = 0;
+        ds->hub_count = 0;
+        ds->hub_chunks = 0;
+    }
+}
+
+// Pack (row, col) into a sort key; invalid indices become ~0 and raise
+// ST_BAD_INDEX.
+__global__ void k_pack_keys(const int32_t *__restrict__ rows, const int32_t *__restrict__ cols,
+                            const float *__restrict__ vals, int64_t n, int64_t R, int64_t C, int cbits,
+                            uint64_t *__restrict__ key, uint32_t *__restrict__ val, DevState *ds)
+{
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    bool bad = false;
+    if (i < n) {
+        int32_t r = rows[i], c = cols[i];
+        bad = (r < 0) | (r >= R) | (c < 0) | (c >= C);
+        key[i] = bad ? ~0ull : (((uint64_t)(uint32_t)r << cbits) | (uint32_t)c);
+        val[i] = __float_as_uint(vals[i]);
+    }
+    if (__ballot(bad) && lane_id() == 0) atomicOr(&ds->status, ST_BAD_INDEX);
+}
+
+// Expand CSR offsets into row indices (one thread per entry, binary search).
+__global__ void k_expand_rows(const int32_t *__restrict__ off, int64_t R, int64_t n, int32_t *__restrict__ rows)
+{
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    int64_t lo = 0, hi = R;  // largest r with off[r] <= i
+    while (hi - lo > 1) {
+        int64_t mid = (lo + hi) >> 1;
+        if (off[mid] <= i) lo = mid; else hi = mid;
+    }
+    rows[i] = (int32_t)lo;
+}
+
+__global__ void k_check_offsets(const int32_t *__restrict__ off, int64_t R, DevState *ds)
+{
+    int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    bool bad = false;
+    if (r < R) bad = off[r + 1] < off[r];
+    if (r == 0) bad |= off[0] != 0;
+    if (__ballot(bad) && lane_id() == 0) atomicOr(&ds->status, ST_BAD_INDEX);
+}
+
+// After the sort: per row, its length and first sorted position.  Also
+// detects duplicate coordinates and explicit zeros (non-canonical input).
+__global__ void k_row_runs(const uint64_t *__restrict__ key, const uint32_t *__restrict__ val, int64_t n, int cbits,
+                           int32_t *__restrict__ row_len, uint32_t *__restrict__ first, uint32_t *flag_noncanon)
+{
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    bool nc = false;
+    if (i < n) {
+        uint64_t k = key[i];
+        uint32_t r = (uint32_t)(k >> cbits);
+        bool head = (i == 0) || ((uint32_t)(key[i - 1] >> cbits) != r);
+        nc = (i > 0 && key[i - 1] == k) || (__uint_as_float(val[i]) == 0.0f);
+        if (head) {
+            uint64_t nextkey = (uint64_t)(r + 1) << cbits;
+            int64_t lo = i, hi = n;  // first index with key >= nextkey
+            while (lo < hi) {
+                int64_t mid = (lo + hi) >> 1;
+                if (key[mid] < nextkey) lo = mid + 1; else hi = mid;
+            }
+            row_len[r] = (int32_t)(lo - i);
+            first[r] = (uint32_t)i;
+        }
+    }
+    if (__ballot(nc) && lane_id() == 0) atomicOr(flag_noncanon, 1u);
+}
+
+// Write sorted entries into the pool at row_start[r] + (i - first[r]).
+__global__ void k_fill_pool(const uint64_t *__restrict__ key, const uint32_t *__restrict__ val, int64_t n, int cbits,
+                            const uint32_t *__restrict__ start, const uint32_t *__restrict__ first,
+                            int32_t *__restrict__ pcol, float *__restrict__ pval)
+{
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    uint64_t k = key[i];
+    uint32_t r = (uint32_t)(k >> cbits);
+    uint64_t p = (uint64_t)start[r] + (uint64_t)(i - first[r]);
+    pcol[p] = (int32_t)(k & ((1ull << cbits) - 1ull));
+    pval[p] = __uint_as_float(val[i]);
+}
+
+__global__ void k_set_loaded(int64_t R, const uint32_t *__restrict__ start, const int32_t *__restrict__ len,
+                             int32_t *__restrict__ cap, DevState *ds, uint64_t nnz)
+{
+    int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r < R) cap[r] = len[r];
+    if (r == 0) {
+        ds->bump = nnz;
+        ds->nnz = (long long)nnz;
+        ds->garbage = 0;
+        ds->hub_count = 0;
+        ds->hub_chunks = 0;
+    }
+}
+
+// Flat copy of every row into destination offsets (compaction / export).
+// dst_off[r] is the row's new start; src rows are (start, len).
+__global__ __launch_bounds__(TB) void k_copy_rows(const uint32_t *__restrict__ dst_off, uint32_t R, uint32_t total,
+                                                  uint32_t per_wave, const uint32_t *__restrict__ src_start,
+                                                  const int32_t *__restrict__ src_col, const float *__restrict__ src_val,
+                                                  int32_t *__restrict__ dst_col, float *__restrict__ dst_val)
+{
+    __shared__ uint32_t s_off[TB / 64][65];
+    const int w = threadIdx.x >> 6;
+    const uint64_t wave = (uint64_t)blockIdx.x * (TB / 64) + w;
+    const uint32_t dev_total = dst_off[R];  // the offsets decide, not the caller's count
+    if (total > dev_total) total = dev_total;
+    uint64_t e0 = wave * per_wave;
+    if (e0 >= total) return;
+    uint64_t e1 = e0 + per_wave < total ? e0 + per_wave : total;
+    wave_visit(dst_off, R, (uint32_t)e0, (uint32_t)e1, s_off[w], [&](bool valid, uint32_t r, uint32_t k, uint32_t e) {
+        if (valid) {
+            uint64_t sp = (uint64_t)src_start[r] + k;
+            dst_col[e] = src_col[sp];
+            dst_val[e] = src_val[sp];
+        }
+    });
+}
+
+__global__ void k_len_u32(const int32_t *__restrict__ len, int64_t R, uint32_t *__restrict__ out)
+{
+    int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r < R) out[r] = (uint32_t)len[r];
+}
+
+__global__ void k_u32_to_i32_offsets(const uint32_t *__restrict__ in, int64_t n, int32_t *__restrict__ out)
+{
+    int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r < n) out[r] = (int32_t)in[r];
+}
+
+// New row table after compaction: start = off[r], cap = len.
+__global__ void k_set_compacted(int64_t R, const uint32_t *__restrict__ off, uint32_t *__restrict__ start,
+                                const int32_t *__restrict__ len, int32_t *__restrict__ cap, DevState *ds,
+                                const uint32_t *__restrict__ total)
+{
+    int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r < R) {
+        start[r] = off[r];
+        cap[r] = len[r];
+    }
+    if (r == 0) {
+        ds->bump = total[0];
+        ds->nnz = total[0];
+        ds->garbage = 0;
+    }
+}
+
+// Canonical form of a natural-order CSR: emit one entry per run of equal
+// columns with value (0 + a1) + a2 + ... (stored order) when it is not 0.
+// Pass 1 flags heads that emit.
+__global__ void k_canon_flags(const int32_t *__restrict__ off_i, int64_t R, const int32_t *__restrict__ col,
+                              const float *__restrict__ val, int64_t n, uint32_t *__restrict__ emit,
+                              float *__restrict__ sum)
+{
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    // row of entry i
+    int64_t lo = 0, hi = R;
+    while (hi - lo > 1) {
+        int64_t mid = (lo + hi) >> 1;
+        if (off_i[mid] <= i) lo = mid; else hi = mid;
+    }
+    int64_t rs = off_i[lo], re = off_i[lo + 1];
+    bool head = (i == rs) || (col[i - 1] != col[i]);
+    uint32_t e = 0;
+    if (head) {
+        float s = 0.0f + val[i];
+        for (int64_t m = i + 1; m < re && col[m] == col[i]; ++m) s = s + val[m];
+        e = (s != 0.0f) ? 1u : 0u;
+        sum[i] = s;
+    }
+    emit[i] = e;
+}
+
+__global__ void k_canon_write(const int32_t *__restrict__ col, int64_t n, const uint32_t *__restrict__ emit_flag,
+                              const uint32_t *__restrict__ pos, const float *__restrict__ sum,
+                              int32_t *__restrict__ ocol, float *__restrict__ oval)
+{
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    if (emit_flag[i]) {
+        ocol[pos[i]] = col[i];
+        oval[pos[i]] = sum[i];
+    }
+}
+
+__global__ void k_canon_offsets(const int32_t *__restrict__ off_in, int64_t R, const uint32_t *__restrict__ pos,
+                                const uint32_t *__restrict__ total, int32_t *__restrict__ off_out, int64_t n)
+{
+    int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r <= R) {
+        int64_t o = off_in[r];
+        off_out[r] = (int32_t)(o < n ? pos[o] : total[0]);
+    }
+}
+
+// Hub-row list: rows with len > DCSR_HUB_LEN.
+__global__ void k_hub_reset(DevState *ds)
+{
+    ds->hub_count = 0;
+    ds->hub_chunks = 0;
+}
+
+__global__ void k_hub_collect(int64_t R, const int32_t *__restrict__ len, uint32_t *__restrict__ hub_rows,
+                              uint32_t cap, DevState *ds)
+{
+    int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    bool hub = r < R && len[r] > DCSR_HUB_LEN;
+    uint64_t m = __ballot(hub);
+    if (!m) return;
+    uint32_t base = 0;
+    int leader = __ffsll((long long)m) - 1;
+    if (lane_id() == leader) base = atomicAdd(&ds->hub_count, (uint32_t)__popcll(m));
+    base = __shfl(base, leader);
+    if (hub) {
+        uint32_t slot = base + (uint32_t)__popcll(m & lanemask_lt());
+        if (slot < cap) hub_rows[slot] = (uint32_t)r;
+    }
+}
+
+// Single workgroup: exclusive chunk prefix over the hub list (+ total).
+__global__ __launch_bounds__(1024) void k_hub_prefix(const int32_t *__restrict__ len,
+                                                     const uint32_t *__restrict__ hub_rows,
+                                                     uint32_t *__restrict__ hub_pre, uint32_t *__restrict__ hub_done,
+                                                     DevState *ds)
+{
+    __shared__ uint32_t lds16[16];
+    __shared__ uint32_t carry;
+    if (threadIdx.x == 0) carry = 0;
+    __syncthreads();
+    const uint32_t cnt = ds->hub_count;
+    const int w = threadIdx.x >> 6;
+    for (uint32_t base = 0; base < cnt; base += 1024) {
+        uint32_t h = base + threadIdx.x;
+        uint32_t c = 0;
+        if (h < cnt) {
+            uint32_t L = (uint32_t)len[hub_rows[h]];
+            c = (L + DCSR_HUB_CHUNK - 1) / DCSR_HUB_CHUNK;
+            hub_done[h] = 0;
+        }
+        uint32_t inc = wave_incl_scan_u32(c);
+        if (lane_id() == 63) lds16[w] = inc;
+        __syncthreads();
+        uint32_t pre = 0, tot = 0;
+        for (int i = 0; i < 16; ++i) {
+            if (i < w) pre += lds16[i];
+            tot += lds16[i];
+        }
+        if (h < cnt) hub_pre[h] = carry + pre + inc - c;
+        __syncthreads();
+        if (threadIdx.x == 0) carry += tot;
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        hub_pre[cnt] = carry;
+        ds->hub_chunks = carry;
+    }
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// host side
+// ---------------------------------------------------------------------------
+dcsr_status dcsr_hip_fail(hipError_t e, const char *what, const char *file, int line)
+{
+    if (getenv("DCSR_VERBOSE"))
+        fprintf(stderr, "[dcsr] HIP error %d (%s) in %s at %s:%d\n", (int)e, hipGetErrorString(e), what, file, line);
+    return e == hipErrorOutOfMemory ? DCSR_ERR_OOM : DCSR_ERR_HIP;
+}
+
+const char *dcsr_status_string(dcsr_status s)
+{
+    switch (s) {
+    case DCSR_OK: return "ok";
+    case DCSR_ERR_INVALID: return "invalid argument";
+    case DCSR_ERR_BAD_INDEX: return "row or column index out of range";
+    case DCSR_ERR_POOL_EXHAUSTED: return "entry pool exhausted";
+    case DCSR_ERR_SEGMENT_OVERFLOW: return "row segment overflow";
+    case DCSR_ERR_HIP: return "HIP runtime error";
+    case DCSR_ERR_OOM: return "device out of memory";
+    case DCSR_ERR_CAPACITY: return "output buffer too small";
+    case DCSR_ERR_UNSUPPORTED: return "unsupported";
+    }
+    return "unknown";
+}
+
+dcsr_status ws_get(dcsr_handle h, WsSlot slot, size_t bytes, void **out)
+{
+    if (bytes == 0) bytes = 16;
+    if (h->ws.bytes[slot] < bytes) {
+        if (h->ws.ptr[slot]) DCSR_HIP(hipFree(h->ws.ptr[slot]));
+        h->ws.ptr[slot] = nullptr;
+        h->ws.bytes[slot] = 0;
+        size_t nb = bytes + bytes / 4;  // headroom for growing batches
+        DCSR_HIP(hipMalloc(&h->ws.ptr[slot], nb));
+        h->ws.bytes[slot] = nb;
+    }
+    *out = h->ws.ptr[slot];
+    return DCSR_OK;
+}
+
+dcsr_status pull_state(dcsr_handle h, hipStream_t s)
+{
+    DCSR_HIP(hipMemcpyAsync(h->hs, h->ds, sizeof(DevState), hipMemcpyDeviceToHost, s));
+    DCSR_HIP(hipStreamSynchronize(s));
+    h->nnz_h = h->hs->nnz;
+    h->bump_h = (int64_t)h->hs->bump;
+    h->garbage_h = h->hs->garbage;
+    h->state_known = true;
+    return DCSR_OK;
+}
+
+static dcsr_status status_from_bits(unsigned bits)
+{
+    if (bits & ST_BAD_INDEX) return DCSR_ERR_BAD_INDEX;
+    if (bits & ST_SEGMENT_OVERFLOW) return DCSR_ERR_SEGMENT_OVERFLOW;
+    if (bits & ST_POOL_EXHAUSTED) return DCSR_ERR_POOL_EXHAUSTED;
+    return DCSR_OK;
+}
+
+__global__ void k_clear_status(DevState *ds) { ds->status = 0; }
+
+dcsr_status dcsr_get_status(dcsr_handle h)
+{
+    if (!h) return DCSR_ERR_INVALID;
+    DCSR_TRY(pull_state(h, h->last_stream));
+    unsigned bits = h->hs->status;
+    k_clear_status<<<1, 1, 0, h->last_stream>>>(h->ds);
+    DCSR_HIP(hipGetLastError());
+    return status_from_bits(bits);
+}
+
+static dcsr_status alloc_rows(dcsr_handle h)
+{
+    size_t R = (size_t)(h->rows > 0 ? h->rows : 1);
+    DCSR_HIP(hipMalloc(&h->row_start, R * sizeof(uint32_t)));
+    DCSR_HIP(hipMalloc(&h->row_len, R * sizeof(int32_t)));
+    DCSR_HIP(hipMalloc(&h->row_cap, R * sizeof(int32_t)));
+    return DCSR_OK;
+}
+
+static dcsr_status alloc_pool(dcsr_handle h, int64_t cap)
+{
+    if (cap < h->cfg.min_pool) cap = h->cfg.min_pool;
+    if (cap > (int64_t)0xfffffff0ll) cap = (int64_t)0xfffffff0ll;
+    if (h->pool_col) DCSR_HIP(hipFree(h->pool_col));
+    if (h->pool_val) DCSR_HIP(hipFree(h->pool_val));
+    h->pool_col = nullptr;
+    h->pool_val = nullptr;
+    DCSR_HIP(hipMalloc(&h->pool_col, (size_t)cap * sizeof(int32_t)));
+    DCSR_HIP(hipMalloc(&h->pool_val, (size_t)cap * sizeof(float)));
+    h->pool_cap = cap;
+    return DCSR_OK;
+}
+
+dcsr_status dcsr_create(dcsr_handle *out, int64_t rows, int64_t cols, const dcsr_config *cfg)
+{
+    if (!out || rows < 0 || cols < 0 || rows >= (1ll << 31) || cols >= (1ll << 31)) return DCSR_ERR_INVALID;
+    dcsr_handle h = (dcsr_handle)calloc(1, sizeof(struct dcsr_matrix_s));
+    if (!h) return DCSR_ERR_OOM;
+    h->rows = rows;
+    h->cols = cols;
+    h->rbits = bits_for(rows);
+    h->cbits = bits_for(cols);
+    h->cfg.pool_factor = 2.0;
+    h->cfg.min_pool = 1024;
+    h->cfg.allow_grow = 1;
+    if (cfg) {
+        if (cfg->pool_factor >= 1.0) h->cfg.pool_factor = cfg->pool_factor;
+        if (cfg->min_pool > 0) h->cfg.min_pool = cfg->min_pool;
+        h->cfg.allow_grow = cfg->allow_grow;
+    }
+    h->canonical = true;
+    h->hubs_dirty = false;
+    h->state_known = true;
+    dcsr_status st = alloc_rows(h);
+    if (st == DCSR_OK) st = alloc_pool(h, h->cfg.min_pool);
+    if (st == DCSR_OK && hipMalloc(&h->ds, sizeof(DevState)) != hipSuccess) st = DCSR_ERR_OOM;
+    if (st == DCSR_OK && hipHostMalloc(&h->hs, sizeof(DevState), hipHostMallocDefault) != hipSuccess) st = DCSR_ERR_OOM;
+    if (st == DCSR_OK && hipMemset(h->ds, 0, sizeof(DevState)) != hipSuccess) st = DCSR_ERR_HIP;
+    if (st == DCSR_OK) {
+        // all rows empty
+        k_init_rows<<<grid_for(rows, TB), TB>>>(rows, 0, h->row_start, h->row_len, h->row_cap, h->ds);
+        if (hipGetLastError() != hipSuccess || hipDeviceSynchronize() != hipSuccess) st = DCSR_ERR_HIP;
+    }
+    if (st != DCSR_OK) {
+        dcsr_destroy(h);
+        return st;
+    }
+    *out = h;
+    return DCSR_OK;
+}
+
+dcsr_status dcsr_destroy(dcsr_handle h)
+{
+    if (!h) return DCSR_ERR_INVALID;
+    if (h->last_stream) (void)hipStreamSynchronize(h->last_stream);
+    else (void)hipDeviceSynchronize();
+    (void)hipFree(h->row_start);
+    (void)hipFree(h->row_len);
+    (void)hipFree(h->row_cap);
+    (void)hipFree(h->pool_col);
+    (void)hipFree(h->pool_val);
+    (void)hipFree(h->ds);
+    if (h->hs) (void)hipHostFree(h->hs);
+    for (int i = 0; i < WS_COUNT; ++i)
+        if (h->ws.ptr[i]) (void)hipFree(h->ws.ptr[i]);
+    free(h);
+    return DCSR_OK;
+}
+
+dcsr_status dcsr_initialize(dcsr_handle h, int32_t bin_length, dcsr_stream_t stream)
+{
+    if (!h || bin_length < 0) return DCSR_ERR_INVALID;
+    hipStream_t s = (hipStream_t)stream;
+    h->last_stream = s;
+    int64_t need = h->rows * (int64_t)bin_length;
+    if (need > (int64_t)0xfffffff0ll) return DCSR_ERR_INVALID;
+    if (need > h->pool_cap) {
+        DCSR_HIP(hipStreamSynchronize(s));
+        DCSR_TRY(alloc_pool(h, (int64_t)(need * h->cfg.pool_factor)));
+    }
+    k_init_rows<<<grid_for(h->rows, TB), TB, 0, s>>>(h->rows, (uint32_t)bin_length, h->row_start, h->row_len,
+                                                       h->row_cap, h->ds);
+    DCSR_HIP(hipGetLastError());
+    h->nnz_h = 0;
+    h->bump_h = need;
+    h->garbage_h = 0;
+    h->hub_rows_h = 0;
+    h->hub_chunks_h = 0;
+    h->canonical = true;
+    h->hubs_dirty = true;
+    h->state_known = true;
+    return DCSR_OK;
+}
+
+// Load sorted-by-key triples into a fresh natural-order pool.
+static dcsr_status load_sorted(dcsr_handle h, const uint64_t *key, const uint32_t *val, int64_t n, hipStream_t s)
+{
+    const int64_t R = h->rows;
+    uint32_t *first, *scan_part, *flag;
+    DCSR_TRY(ws_get(h, WS_TMP0, (size_t)(R + 1) * sizeof(uint32_t), (void **)&first));
+    DCSR_TRY(ws_get(h, WS_PART, scan_part_words(R + 1, 1) * sizeof(uint32_t), (void **)&scan_part));
+    DCSR_TRY(ws_get(h, WS_TMP2, 64, (void **)&flag));
+    k_fill_i32<<<grid_for(R, TB * 4), TB, 0, s>>>(h->row_len, R, 0);
+    k_fill_i32<<<1, 64, 0, s>>>((int32_t *)flag, 2, 0);
+    k_row_runs<<<grid_for(n, TB), TB, 0, s>>>(key, val, n, h->cbits, h->row_len, first, flag);
+    // row starts = exclusive scan of lengths (natural order, no slack)
+    uint32_t *lens;
+    DCSR_TRY(ws_get(h, WS_TMP1, (size_t)(R + 1) * sizeof(uint32_t), (void **)&lens));
+    k_len_u32<<<grid_for(R, TB), TB, 0, s>>>(h->row_len, R, lens);
+    DCSR_HIP(scan_exclusive_u32(lens, h->row_start, R, 1, scan_part, flag + 1, s));
+    int64_t want = (int64_t)(n * h->cfg.pool_factor);
+    if (want < n) want = n;
+    if (want > h->pool_cap || n > h->pool_cap) {
+        DCSR_HIP(hipStreamSynchronize(s));
+        DCSR_TRY(alloc_pool(h, want));
+    }
+    k_fill_pool<<<grid_for(n, TB), TB, 0, s>>>(key, val, n, h->cbits, h->row_start, first, h->pool_col, h->pool_val);
+    k_set_loaded<<<grid_for(R > 0 ? R : 1, TB), TB, 0, s>>>(R, h->row_start, h->row_len, h->row_cap, h->ds,
+                                                            (uint64_t)n);
+    DCSR_HIP(hipGetLastError());
+    uint32_t hflag[2];
+    DCSR_HIP(hipMemcpyAsync(hflag, flag, sizeof(hflag), hipMemcpyDeviceToHost, s));
+    DCSR_TRY(pull_state(h, s));
+    h->canonical = hflag[0] == 0;
+    h->hubs_dirty = true;
+    h->hub_rows_h = -1;  // unknown until the hub list is rebuilt
+    return DCSR_OK;
+}
+
+static dcsr_status load_coo_impl(dcsr_handle h, const int32_t *rows, const int32_t *cols, const float *vals,
+                                 int64_t n, hipStream_t s)
+{
+    if (n >= (int64_t)0xfffffff0ll) return DCSR_ERR_INVALID;
+    uint64_t *k0, *k1;
+    uint32_t *v0, *v1, *hist, *part;
+    DCSR_TRY(ws_get(h, WS_KEY_A, (size_t)n * 8, (void **)&k0));
+    DCSR_TRY(ws_get(h, WS_KEY_B, (size_t)n * 8, (void **)&k1));
+    DCSR_TRY(ws_get(h, WS_VAL_A, (size_t)n * 4, (void **)&v0));
+    DCSR_TRY(ws_get(h, WS_VAL_B, (size_t)n * 4, (void **)&v1));
+    DCSR_TRY(ws_get(h, WS_HIST, radix_hist_words(n) * 4, (void **)&hist));
+    DCSR_TRY(ws_get(h, WS_SCAN, radix_part_words(n) * 4, (void **)&part));
+    k_clear_status<<<1, 1, 0, s>>>(h->ds);
+    if (n > 0)
+        k_pack_keys<<<grid_for(n, TB), TB, 0, s>>>(rows, cols, vals, n, h->rows, h->cols, h->cbits, k0, v0, h->ds);
+    DCSR_HIP(hipGetLastError());
+    // reject out-of-range indices before anything is modified
+    DCSR_TRY(pull_state(h, s));
+    if (h->hs->status & ST_BAD_INDEX) {
+        k_clear_status<<<1, 1, 0, s>>>(h->ds);
+        return DCSR_ERR_BAD_INDEX;
+    }
+    int which = 0;
+    DCSR_HIP(radix_sort_pairs(k0, v0, k1, v1, n, h->rbits + h->cbits, hist, part, &which, s));
+    return load_sorted(h, which ? k1 : k0, which ? v1 : v0, n, s);
+}
+
+dcsr_status dcsr_load_coo(dcsr_handle h, const int32_t *rows, const int32_t *cols, const float *vals, int64_t n,
+                          dcsr_stream_t stream)
+{
+    if (!h || n < 0 || (n > 0 && (!rows || !cols || !vals))) return DCSR_ERR_INVALID;
+    hipStream_t s = (hipStream_t)stream;
+    h->last_stream = s;
+    return load_coo_impl(h, rows, cols, vals, n, s);
+}
+
+dcsr_status dcsr_load_csr(dcsr_handle h, const int32_t *row_offsets, const int32_t *cols, const float *vals,
+                          dcsr_stream_t stream)
+{
+    if (!h || !row_offsets) return DCSR_ERR_INVALID;
+    hipStream_t s = (hipStream_t)stream;
+    h->last_stream = s;
+    int32_t nnz = 0;
+    DCSR_HIP(hipMemcpyAsync(&nnz, row_offsets + h->rows, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+    k_clear_status<<<1, 1, 0, s>>>(h->ds);
+    if (h->rows > 0) k_check_offsets<<<grid_for(h->rows, TB), TB, 0, s>>>(row_offsets, h->rows, h->ds);
+    DCSR_TRY(pull_state(h, s));
+    if (nnz < 0 || (h->hs->status & ST_BAD_INDEX)) return DCSR_ERR_INVALID;
+    if (nnz > 0 && (!cols || !vals)) return DCSR_ERR_INVALID;
+    int32_t *rows;
+    DCSR_TRY(ws_get(h, WS_CANON, (size_t)nnz * 4, (void **)&rows));
+    if (nnz > 0) k_expand_rows<<<grid_for(nnz, TB), TB, 0, s>>>(row_offsets, h->rows, nnz, rows);
+    return load_coo_impl(h, rows, cols, vals, nnz, s);
+}
+
+// Exclusive offsets of the current row lengths (u32, R+1 entries) into `off`.
+static dcsr_status row_offsets_u32(dcsr_handle h, uint32_t *off, hipStream_t s)
+{
+    uint32_t *lens, *part;
+    const int64_t R = h->rows;
+    DCSR_TRY(ws_get(h, WS_TMP1, (size_t)(R + 1) * 4, (void **)&lens));
+    DCSR_TRY(ws_get(h, WS_PART, scan_part_words(R + 1, 1) * 4, (void **)&part));
+    k_len_u32<<<grid_for(R, TB), TB, 0, s>>>(h->row_len, R, lens);
+    DCSR_HIP(scan_exclusive_u32(lens, off, R, 1, part, off + R, s));
+    return DCSR_OK;
+}
+
+static void launch_copy_rows(const uint32_t *dst_off, int64_t R, int64_t total, const uint32_t *src_start,
+                             const int32_t *scol, const float *sval, int32_t *dcol, float *dval, hipStream_t s)
+{
+    if (total <= 0 || R <= 0) return;
+    const uint32_t per_wave = 1024;
+    int64_t waves = (total + per_wave - 1) / per_wave;
+    unsigned blocks = (unsigned)((waves + 3) / 4);
+    k_copy_rows<<<blocks, TB, 0, s>>>(dst_off, (uint32_t)R, (uint32_t)total, per_wave, src_start, scol, sval, dcol,
+                                      dval);
+}
+
+dcsr_status dcsr_export_csr(dcsr_handle h, int32_t *row_offsets, int32_t *cols, float *vals, int64_t cap,
+                            int64_t *nnz_out, uint32_t flags, dcsr_stream_t stream)
+{
+    if (!h || !row_offsets || !nnz_out) return DCSR_ERR_INVALID;
+    hipStream_t s = (hipStream_t)stream;
+    h->last_stream = s;
+    const int64_t R = h->rows;
+    if (!h->state_known) DCSR_TRY(pull_state(h, s));
+    const int64_t nnz = h->nnz_h;
+    bool canon = (flags & DCSR_EXPORT_CANONICAL) && !h->canonical;
+    if (!canon && cap < nnz) return DCSR_ERR_CAPACITY;
+    if (nnz > 0 && (!cols || !vals)) return DCSR_ERR_INVALID;
+    uint32_t *off;
+    DCSR_TRY(ws_get(h, WS_TMP0, (size_t)(R + 1) * 4, (void **)&off));
+    DCSR_TRY(row_offsets_u32(h, off, s));
+    if (!canon) {
+        k_u32_to_i32_offsets<<<grid_for(R + 1, TB), TB, 0, s>>>(off, R + 1, row_offsets);
+        launch_copy_rows(off, R, nnz, h->row_start, h->pool_col, h->pool_val, cols, vals, s);
+        DCSR_HIP(hipGetLastError());
+        DCSR_HIP(hipStreamSynchronize(s));
+        *nnz_out = nnz;
+        return DCSR_OK;
+    }
+    // canonical export of a matrix with duplicates / zeros
+    int32_t *tcol, *toff;
+    float *tval, *tsum;
+    uint32_t *emit, *pos, *part, *tot;
+    DCSR_TRY(ws_get(h, WS_UCOL, (size_t)nnz * 4 + 4, (void **)&tcol));
+    DCSR_TRY(ws_get(h, WS_UVAL, (size_t)nnz * 4 + 4, (void **)&tval));
+    DCSR_TRY(ws_get(h, WS_UPOS, (size_t)nnz * 4 + 4, (void **)&tsum));
+    DCSR_TRY(ws_get(h, WS_UCODE, (size_t)nnz * 4 + 4, (void **)&emit));
+    DCSR_TRY(ws_get(h, WS_UPRE, (size_t)nnz * 4 + 4, (void **)&pos));
+    DCSR_TRY(ws_get(h, WS_TOLDPRE, (size_t)(R + 1) * 4, (void **)&toff));
+    DCSR_TRY(ws_get(h, WS_SCAN, scan_part_words(nnz + 1, 1) * 4, (void **)&part));
+    DCSR_TRY(ws_get(h, WS_TMP2, 64, (void **)&tot));
+    k_u32_to_i32_offsets<<<grid_for(R + 1, TB), TB, 0, s>>>(off, R + 1, toff);
+    launch_copy_rows(off, R, nnz, h->row_start, h->pool_col, h->pool_val, tcol, tval, s);
+    if (nnz > 0) {
+        k_canon_flags<<<grid_for(nnz, TB), TB, 0, s>>>(toff, R, tcol, tval, nnz, emit, tsum);
+        DCSR_HIP(scan_exclusive_u32(emit, pos, nnz, 1, part, tot, s));
+    } else {
+        DCSR_HIP(scan_exclusive_u32(emit, pos, 0, 1, part, tot, s));
+    }
+    uint32_t total = 0;
+    DCSR_HIP(hipMemcpyAsync(&total, tot, 4, hipMemcpyDeviceToHost, s));
+    DCSR_HIP(hipStreamSynchronize(s));
+    if ((int64_t)total > cap) return DCSR_ERR_CAPACITY;
+    if (nnz > 0) k_canon_write<<<grid_for(nnz, TB), TB, 0, s>>>(tcol, nnz, emit, pos, tsum, cols, vals);
+    k_canon_offsets<<<grid_for(R + 1, TB), TB, 0, s>>>(toff, R, pos, tot, row_offsets, nnz);
+    DCSR_HIP(hipGetLastError());
+    DCSR_HIP(hipStreamSynchronize(s));
+    *nnz_out = total;
+    return DCSR_OK;
+}
+
+// Rewrite every row contiguously into a pool of capacity new_cap (>= nnz).
+static dcsr_status compact_into(dcsr_handle h, int64_t new_cap, hipStream_t s)
+{
+    const int64_t R = h->rows;
+    if (!h->state_known) DCSR_TRY(pull_state(h, s));
+    const int64_t nnz = h->nnz_h;
+    if (new_cap < nnz) new_cap = nnz;
+    if (new_cap < h->cfg.min_pool) new_cap = h->cfg.min_pool;
+    if (new_cap > (int64_t)0xfffffff0ll) return DCSR_ERR_POOL_EXHAUSTED;
+    uint32_t *off;
+    DCSR_TRY(ws_get(h, WS_TMP0, (size_t)(R + 1) * 4, (void **)&off));
+    DCSR_TRY(row_offsets_u32(h, off, s));
+    int32_t *ncol = nullptr;
+    float *nval = nullptr;
+    DCSR_HIP(hipMalloc(&ncol, (size_t)new_cap * 4));
+    if (hipMalloc(&nval, (size_t)new_cap * 4) != hipSuccess) {
+        (void)hipFree(ncol);
+        return DCSR_ERR_OOM;
+    }
+    launch_copy_rows(off, R, nnz, h->row_start, h->pool_col, h->pool_val, ncol, nval, s);
+    if (R > 0) k_set_compacted<<<grid_for(R, TB), TB, 0, s>>>(R, off, h->row_start, h->row_len, h->row_cap, h->ds, off + R);
+    DCSR_HIP(hipGetLastError());
+    DCSR_HIP(hipStreamSynchronize(s));
+    (void)hipFree(h->pool_col);
+    (void)hipFree(h->pool_val);
+    h->pool_col = ncol;
+    h->pool_val = nval;
+    h->pool_cap = new_cap;
+    DCSR_TRY(pull_state(h, s));
+    return DCSR_OK;
+}
+
+dcsr_status dcsr_compact(dcsr_handle h, dcsr_stream_t stream)
+{
+    if (!h) return DCSR_ERR_INVALID;
+    hipStream_t s = (hipStream_t)stream;
+    h->last_stream = s;
+    if (!h->state_known) DCSR_TRY(pull_state(h, s));
+    // keep the capacity, release retired chunks
+    return compact_into(h, h->pool_cap, s);
+}
+
+// Make room for at least min_free entries past the bump pointer.
+dcsr_status pool_reserve(dcsr_handle h, int64_t min_free, hipStream_t s, const int32_t *)
+{
+    if (!h->state_known) DCSR_TRY(pull_state(h, s));
+    if (h->pool_cap - h->bump_h >= min_free) return DCSR_OK;
+    int64_t cap = h->pool_cap;
+    if (cap - h->nnz_h < min_free + min_free / 4 || h->garbage_h < (h->nnz_h >> 3)) {
+        int64_t want = (int64_t)((h->nnz_h + min_free) * h->cfg.pool_factor);
+        cap = want > cap ? want : cap;
+    }
+    return compact_into(h, cap, s);
+}
+
+// Rewrite a non-canonical matrix (duplicates / explicit zeros from the load)
+// into canonical rows, as the first host-path add does for every row
+// (cusp/system/detail/sequential/elementwise.h:44-131 visits all rows of A).
+dcsr_status canonicalize(dcsr_handle h, hipStream_t s)
+{
+    if (h->canonical) return DCSR_OK;
+    const int64_t R = h->rows;
+    int64_t nnz = h->nnz_h;
+    int32_t *off, *ocol;
+    float *oval;
+    DCSR_HIP(hipMalloc(&off, (size_t)(R + 1) * 4));
+    DCSR_HIP(hipMalloc(&ocol, (size_t)(nnz > 0 ? nnz : 1) * 4));
+    DCSR_HIP(hipMalloc(&oval, (size_t)(nnz > 0 ? nnz : 1) * 4));
+    int64_t cn = 0;
+    dcsr_status st = dcsr_export_csr(h, off, ocol, oval, nnz, &cn, DCSR_EXPORT_CANONICAL, s);
+    if (st == DCSR_OK) st = dcsr_load_csr(h, off, ocol, oval, s);
+    (void)hipFree(off);
+    (void)hipFree(ocol);
+    (void)hipFree(oval);
+    if (st == DCSR_OK) h->canonical = true;
+    return st;
+}
+
+dcsr_status ensure_hub_list(dcsr_handle h, hipStream_t s)
+{
+    if (!h->hubs_dirty) return DCSR_OK;
+    const int64_t R = h->rows;
+    uint32_t cap = (uint32_t)(h->pool_cap / DCSR_HUB_LEN + 1);
+    uint32_t *rows_, *pre, *done;
+    DCSR_TRY(ws_get(h, WS_HUB_ROWS, (size_t)cap * 4, (void **)&rows_));
+    DCSR_TRY(ws_get(h, WS_HUB_PRE, (size_t)(cap + 1) * 4, (void **)&pre));
+    DCSR_TRY(ws_get(h, WS_HUB_DONE, (size_t)cap * 4, (void **)&done));
+    k_hub_reset<<<1, 1, 0, s>>>(h->ds);
+    if (R > 0) k_hub_collect<<<grid_for(R, TB), TB, 0, s>>>(R, h->row_len, rows_, cap, h->ds);
+    k_hub_prefix<<<1, 1024, 0, s>>>(h->row_len, rows_, pre, done, h->ds);
+    DCSR_HIP(hipGetLastError());
+    DCSR_TRY(pull_state(h, s));
+    h->hub_rows_h = h->hs->hub_count;
+    h->hub_chunks_h = h->hs->hub_chunks;
+    h->hubs_dirty = false;
+    return DCSR_OK;
+}
+
+dcsr_status dcsr_rebin(dcsr_handle h, dcsr_stream_t stream)
+{
+    if (!h) return DCSR_ERR_INVALID;
+    hipStream_t s = (hipStream_t)stream;
+    h->last_stream = s;
+    h->hubs_dirty = true;
+    return ensure_hub_list(h, s);
+}
+
+dcsr_status dcsr_query(dcsr_handle h, dcsr_info *info)
+{
+    if (!h || !info) return DCSR_ERR_INVALID;
+    DCSR_TRY(pull_state(h, h->last_stream));
+    memset(info, 0, sizeof(*info));
+    info->rows = h->rows;
+    info->cols = h->cols;
+    info->nnz = h->nnz_h;
+    info->pool_cap = h->pool_cap;
+    info->pool_used = h->bump_h;
+    info->garbage = h->garbage_h;
+    info->hub_rows = h->hubs_dirty ? -1 : h->hub_rows_h;
+    info->last_inserted = h->last_ins;
+    info->last_added = h->last_add;
+    info->last_deleted = h->last_del;
+    info->last_touched_rows = h->last_touched;
+    info->canonical = h->canonical ? 1 : 0;
+    info->max_segments = 1;
+    return DCSR_OK;
+}

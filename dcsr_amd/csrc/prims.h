@@ -1,0 +1,21 @@
+// prims.h -- device-wide primitives (host launchers).  Implemented in
+// prims.hip with wave64 DPP scans and ballot-based digit ranking.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+// Exclusive prefix sum of NC interleaved uint32 components (in[i*NC + c]),
+// n elements, in place allowed.  `total` (device, NC words) receives the sums.
+// `part` is scratch of scan_part_words(n, NC) words.
+size_t scan_part_words(int64_t n, int nc);
+hipError_t scan_exclusive_u32(const uint32_t *in, uint32_t *out, int64_t n, int nc, uint32_t *part,
+                              uint32_t *total, hipStream_t s);
+
+// Stable LSD radix sort of (key, value) pairs on key bits [0, nbits).
+// Ping-pongs between (k0, v0) and (k1, v1); returns in *which the buffer
+// index (0 or 1) holding the result.  `hist` is scratch of
+// radix_hist_words(n) words, `part` scratch of radix_part_words(n) words.
+size_t radix_hist_words(int64_t n);
+size_t radix_part_words(int64_t n);
+hipError_t radix_sort_pairs(uint64_t *k0, uint32_t *v0, uint64_t *k1, uint32_t *v1, int64_t n, int nbits,
+                            uint32_t *hist, uint32_t *part, int *which, hipStream_t s);

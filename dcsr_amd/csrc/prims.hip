@@ -1,0 +1,292 @@
+// prims.hip -- device-wide scan and stable LSD radix sort for gfx950.
+//
+// Replace the thrust/cub primitives the reference calls on its hot path
+// (thrust::exclusive_scan include/load.h:418, thrust::sort_by_key
+// include/load.h:602-603, cusp::indices_to_offsets, cub::BlockRadixSort
+// include/spmm.inl:489-490) with wave64 code: DPP scans and ballot-based
+// digit ranking (8 ballots per item give the lanes sharing a digit).
+#include "prims.h"
+#include "wave.h"
+
+namespace {
+
+__global__ void k_fill_u32(uint32_t *p, int64_t n, uint32_t v)
+{
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        p[i] = v;
+}
+
+constexpr int SCAN_THREADS = 256;
+constexpr int SCAN_IPT = 8;
+constexpr int SCAN_TILE = SCAN_THREADS * SCAN_IPT;
+
+// Block-wide exclusive scan of one uint32 per thread (256 threads).
+// Returns the exclusive prefix; *block_total receives the block sum.
+__device__ __forceinline__ uint32_t block_excl_scan_256(uint32_t v, uint32_t *lds4, uint32_t *block_total)
+{
+    const int w = threadIdx.x >> 6;
+    uint32_t inc = wave_incl_scan_u32(v);
+    if (lane_id() == 63) lds4[w] = inc;
+    __syncthreads();
+    uint32_t carry = 0, tot = 0;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        uint32_t t = lds4[i];
+        if (i < w) carry += t;
+        tot += t;
+    }
+    __syncthreads();
+    *block_total = tot;
+    return carry + inc - v;
+}
+
+template <int NC>
+__global__ __launch_bounds__(SCAN_THREADS) void k_scan_reduce(const uint32_t *__restrict__ in, int64_t n,
+                                                              uint32_t *__restrict__ part)
+{
+    __shared__ uint32_t lds4[4];
+    const int64_t base = (int64_t)blockIdx.x * SCAN_TILE;
+#pragma unroll
+    for (int c = 0; c < NC; ++c) {
+        uint32_t acc = 0;
+#pragma unroll
+        for (int i = 0; i < SCAN_IPT; ++i) {
+            int64_t idx = base + i * SCAN_THREADS + threadIdx.x;
+            if (idx < n) acc += in[idx * NC + c];
+        }
+        uint32_t tot;
+        block_excl_scan_256(acc, lds4, &tot);
+        if (threadIdx.x == 0) part[(int64_t)blockIdx.x * NC + c] = tot;
+    }
+}
+
+// Single workgroup: exclusive scan of the per-tile partials, in place.
+template <int NC>
+__global__ __launch_bounds__(1024) void k_scan_partials(uint32_t *__restrict__ part, int64_t nb,
+                                                        uint32_t *__restrict__ total)
+{
+    __shared__ uint32_t lds16[16];
+    __shared__ uint32_t carry_s[NC];
+    if (threadIdx.x < NC) carry_s[threadIdx.x] = 0;
+    __syncthreads();
+    const int w = threadIdx.x >> 6;
+    for (int64_t base = 0; base < nb; base += 1024) {
+        int64_t idx = base + threadIdx.x;
+#pragma unroll
+        for (int c = 0; c < NC; ++c) {
+            uint32_t v = idx < nb ? part[idx * NC + c] : 0u;
+            uint32_t inc = wave_incl_scan_u32(v);
+            if (lane_id() == 63) lds16[w] = inc;
+            __syncthreads();
+            uint32_t pre = 0, tot = 0;
+            for (int i = 0; i < 16; ++i) {
+                uint32_t t = lds16[i];
+                if (i < w) pre += t;
+                tot += t;
+            }
+            uint32_t carry = carry_s[c];
+            if (idx < nb) part[idx * NC + c] = carry + pre + inc - v;
+            __syncthreads();
+            if (threadIdx.x == 0) carry_s[c] = carry + tot;
+            __syncthreads();
+        }
+    }
+    if (threadIdx.x < NC && total) total[threadIdx.x] = carry_s[threadIdx.x];
+}
+
+template <int NC>
+__global__ __launch_bounds__(SCAN_THREADS) void k_scan_down(const uint32_t *__restrict__ in,
+                                                            uint32_t *__restrict__ out, int64_t n,
+                                                            const uint32_t *__restrict__ part)
+{
+    __shared__ uint32_t tile[SCAN_TILE];
+    __shared__ uint32_t lds4[4];
+    const int64_t base = (int64_t)blockIdx.x * SCAN_TILE;
+#pragma unroll
+    for (int c = 0; c < NC; ++c) {
+        // striped coalesced load
+#pragma unroll
+        for (int i = 0; i < SCAN_IPT; ++i) {
+            int64_t idx = base + i * SCAN_THREADS + threadIdx.x;
+            tile[i * SCAN_THREADS + threadIdx.x] = idx < n ? in[idx * NC + c] : 0u;
+        }
+        __syncthreads();
+        // blocked scan: thread t owns items [t*IPT, t*IPT+IPT)
+        uint32_t loc[SCAN_IPT];
+        uint32_t s = 0;
+#pragma unroll
+        for (int i = 0; i < SCAN_IPT; ++i) {
+            loc[i] = s;
+            s += tile[threadIdx.x * SCAN_IPT + i];
+        }
+        uint32_t tot;
+        uint32_t pre = block_excl_scan_256(s, lds4, &tot) + part[(int64_t)blockIdx.x * NC + c];
+#pragma unroll
+        for (int i = 0; i < SCAN_IPT; ++i) tile[threadIdx.x * SCAN_IPT + i] = pre + loc[i];
+        __syncthreads();
+#pragma unroll
+        for (int i = 0; i < SCAN_IPT; ++i) {
+            int64_t idx = base + i * SCAN_THREADS + threadIdx.x;
+            if (idx < n) out[idx * NC + c] = tile[i * SCAN_THREADS + threadIdx.x];
+        }
+        __syncthreads();
+    }
+}
+
+// ---------------------------------------------------------------------------
+// radix sort
+// ---------------------------------------------------------------------------
+constexpr int RS_THREADS = 256;
+constexpr int RS_WAVES = RS_THREADS / 64;
+constexpr int RS_IPT = 8;                        // items per lane
+constexpr int RS_WAVE_ITEMS = RS_IPT * 64;       // contiguous items per wave
+constexpr int RS_TILE = RS_WAVES * RS_WAVE_ITEMS; // 2048
+
+__global__ __launch_bounds__(RS_THREADS) void k_rs_hist(const uint64_t *__restrict__ keys, int64_t n, int shift,
+                                                        uint32_t *__restrict__ hist, int64_t nblocks)
+{
+    __shared__ uint32_t cnt[256];
+    cnt[threadIdx.x] = 0;
+    __syncthreads();
+    const int64_t base = (int64_t)blockIdx.x * RS_TILE;
+    for (int i = threadIdx.x; i < RS_TILE; i += RS_THREADS) {
+        int64_t idx = base + i;
+        if (idx < n) atomicAdd(&cnt[(uint32_t)(keys[idx] >> shift) & 255u], 1u);
+    }
+    __syncthreads();
+    hist[(int64_t)threadIdx.x * nblocks + blockIdx.x] = cnt[threadIdx.x];
+}
+
+__global__ __launch_bounds__(RS_THREADS) void k_rs_scatter(const uint64_t *__restrict__ kin,
+                                                           const uint32_t *__restrict__ vin,
+                                                           uint64_t *__restrict__ kout, uint32_t *__restrict__ vout,
+                                                           int64_t n, int shift,
+                                                           const uint32_t *__restrict__ offs, int64_t nblocks)
+{
+    __shared__ uint64_t skey[RS_TILE];
+    __shared__ uint32_t sval[RS_TILE];
+    __shared__ uint32_t wcnt[RS_WAVES][256];
+    __shared__ uint32_t loff[256];
+    __shared__ uint32_t goff[256];
+    __shared__ uint32_t lds4[4];
+
+    const int w = threadIdx.x >> 6;
+    const int lane = lane_id();
+    for (int i = threadIdx.x; i < RS_WAVES * 256; i += RS_THREADS) (&wcnt[0][0])[i] = 0;
+    __syncthreads();
+
+    const int64_t base = (int64_t)blockIdx.x * RS_TILE;
+    const int64_t wbase = base + (int64_t)w * RS_WAVE_ITEMS;
+    uint64_t key[RS_IPT];
+    uint32_t val[RS_IPT];
+    uint32_t rnk[RS_IPT];
+    const uint64_t lt = lanemask_lt();
+#pragma unroll
+    for (int j = 0; j < RS_IPT; ++j) {
+        int64_t idx = wbase + j * 64 + lane;
+        bool ok = idx < n;
+        key[j] = ok ? kin[idx] : ~0ull;
+        val[j] = ok ? vin[idx] : 0u;
+        uint32_t d = (uint32_t)(key[j] >> shift) & 255u;
+        uint64_t peers = wave_match_digit8(d, ~0ull);
+        int leader = __ffsll((long long)peers) - 1;
+        uint32_t b = 0;
+        if (lane == leader) {
+            b = wcnt[w][d];
+            wcnt[w][d] = b + (uint32_t)__popcll(peers);
+        }
+        b = __shfl(b, leader);
+        rnk[j] = b + (uint32_t)__popcll(peers & lt);
+    }
+    __syncthreads();
+    {
+        const int d = threadIdx.x;  // one digit per thread
+        uint32_t c[RS_WAVES];
+        uint32_t s = 0;
+#pragma unroll
+        for (int i = 0; i < RS_WAVES; ++i) {
+            c[i] = wcnt[i][d];
+            wcnt[i][d] = s;
+            s += c[i];
+        }
+        uint32_t tot;
+        loff[d] = block_excl_scan_256(s, lds4, &tot);
+        goff[d] = offs[(int64_t)d * nblocks + blockIdx.x];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < RS_IPT; ++j) {
+        uint32_t d = (uint32_t)(key[j] >> shift) & 255u;
+        uint32_t p = loff[d] + wcnt[w][d] + rnk[j];
+        skey[p] = key[j];
+        sval[p] = val[j];
+    }
+    __syncthreads();
+    const int64_t valid = (n - base) < RS_TILE ? (n - base) : RS_TILE;
+    for (int p = threadIdx.x; p < valid; p += RS_THREADS) {
+        uint64_t k = skey[p];
+        uint32_t d = (uint32_t)(k >> shift) & 255u;
+        int64_t o = (int64_t)goff[d] + (p - (int64_t)loff[d]);
+        kout[o] = k;
+        vout[o] = sval[p];
+    }
+}
+
+}  // namespace
+
+size_t scan_part_words(int64_t n, int nc)
+{
+    return (size_t)((n + SCAN_TILE - 1) / SCAN_TILE + 1) * nc;
+}
+
+hipError_t scan_exclusive_u32(const uint32_t *in, uint32_t *out, int64_t n, int nc, uint32_t *part,
+                              uint32_t *total, hipStream_t s)
+{
+    if (n <= 0) {
+        if (total) k_fill_u32<<<1, 64, 0, s>>>(total, nc, 0u);
+        return hipGetLastError();
+    }
+    int64_t nb = (n + SCAN_TILE - 1) / SCAN_TILE;
+    if (nc == 1) {
+        k_scan_reduce<1><<<(unsigned)nb, SCAN_THREADS, 0, s>>>(in, n, part);
+        k_scan_partials<1><<<1, 1024, 0, s>>>(part, nb, total);
+        k_scan_down<1><<<(unsigned)nb, SCAN_THREADS, 0, s>>>(in, out, n, part);
+    } else if (nc == 2) {
+        k_scan_reduce<2><<<(unsigned)nb, SCAN_THREADS, 0, s>>>(in, n, part);
+        k_scan_partials<2><<<1, 1024, 0, s>>>(part, nb, total);
+        k_scan_down<2><<<(unsigned)nb, SCAN_THREADS, 0, s>>>(in, out, n, part);
+    } else {
+        return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+size_t radix_hist_words(int64_t n)
+{
+    int64_t nb = (n + RS_TILE - 1) / RS_TILE;
+    return (size_t)(nb < 1 ? 1 : nb) * 256;
+}
+
+size_t radix_part_words(int64_t n) { return scan_part_words((int64_t)radix_hist_words(n), 1); }
+
+hipError_t radix_sort_pairs(uint64_t *k0, uint32_t *v0, uint64_t *k1, uint32_t *v1, int64_t n, int nbits,
+                            uint32_t *hist, uint32_t *part, int *which, hipStream_t s)
+{
+    *which = 0;
+    if (n <= 1) return hipSuccess;
+    int64_t nb = (n + RS_TILE - 1) / RS_TILE;
+    uint64_t *ki = k0, *ko = k1;
+    uint32_t *vi = v0, *vo = v1;
+    int cur = 0;
+    for (int shift = 0; shift < nbits; shift += 8) {
+        k_rs_hist<<<(unsigned)nb, RS_THREADS, 0, s>>>(ki, n, shift, hist, nb);
+        hipError_t e = scan_exclusive_u32(hist, hist, nb * 256, 1, part, nullptr, s);
+        if (e != hipSuccess) return e;
+        k_rs_scatter<<<(unsigned)nb, RS_THREADS, 0, s>>>(ki, vi, ko, vo, n, shift, hist, nb);
+        uint64_t *tk = ki; ki = ko; ko = tk;
+        uint32_t *tv = vi; vi = vo; vo = tv;
+        cur ^= 1;
+    }
+    *which = cur;
+    return hipGetLastError();
+}

@@ -1,0 +1,65 @@
+// segments.h -- device helpers for work that is flat over the entries of many
+// variable-length segments (rows).  A wave owns a fixed range of entries, so
+// a row of millions of entries is spread over many waves and short rows are
+// packed 64-at-a-time: the load balance does not depend on the row-length
+// distribution (power-law hubs included).
+#pragma once
+#include "wave.h"
+
+// Largest j in [0, 64) with s[j] <= e (s non-decreasing, s[0] <= e).
+__device__ __forceinline__ int find_seg64(const uint32_t *s, uint32_t e)
+{
+    int lo = 0, hi = 64;
+#pragma unroll
+    for (int it = 0; it < 6; ++it) {
+        int mid = (lo + hi) >> 1;
+        if (s[mid] <= e) lo = mid; else hi = mid;
+    }
+    return lo;
+}
+
+// Largest j in [0, n) with off[j] <= e (global array, off[0] == 0 <= e).
+__device__ __forceinline__ uint32_t find_seg_global(const uint32_t *off, uint32_t n, uint32_t e)
+{
+    uint32_t lo = 0, hi = n;
+    while (hi - lo > 1) {
+        uint32_t mid = lo + ((hi - lo) >> 1);
+        if (off[mid] <= e) lo = mid; else hi = mid;
+    }
+    return lo;
+}
+
+// Visit entries [e0, e1) of the flat space given by the exclusive offsets
+// off[0..nseg] (off[nseg] = total).  For every entry, calls
+//   f(seg, k, e)   with k = e - off[seg]   (all lanes of the wave call
+// into the loop together; lanes without an entry get valid == false).
+// s_off: 65 words of LDS owned by this wave.
+template <class F>
+__device__ __forceinline__ void wave_visit(const uint32_t *__restrict__ off, uint32_t nseg, uint32_t e0, uint32_t e1,
+                                           uint32_t *s_off, F &&f)
+{
+    const int lane = lane_id();
+    if (e0 >= e1) return;
+    uint32_t j = 0;
+    if (lane == 0) j = find_seg_global(off, nseg, e0);
+    j = wave_read_lane_u32(j, 0);
+    uint32_t e = e0;
+    while (e < e1) {
+        // window: segments j .. j+63, boundaries off[j .. j+64]
+        uint32_t idx = j + lane;
+        s_off[lane] = idx <= nseg ? off[idx] : 0xffffffffu;
+        if (lane == 0) s_off[64] = (j + 64 <= nseg) ? off[j + 64] : 0xffffffffu;
+        __builtin_amdgcn_wave_barrier();
+        uint32_t wend = s_off[64];
+        if (wend > e1) wend = e1;
+        for (uint32_t b = e; b < wend; b += 64) {
+            uint32_t ee = b + lane;
+            bool valid = ee < wend;
+            int sj = valid ? find_seg64(s_off, ee) : 0;
+            f(valid, j + sj, valid ? ee - s_off[sj] : 0u, ee);
+        }
+        e = wend;
+        j += 64;
+        __builtin_amdgcn_wave_barrier();
+    }
+}

@@ -1,0 +1,158 @@
+/*
+ * dcsr.h -- C ABI of the MI355X Dynamic-CSR engine (libdcsr.so, gfx950).
+ *
+ * Drop-in boundary for the reference's header-only C++ API in namespace
+ * `device` (include/load.h, include/spmv.h, include/spmm.h of the reference).
+ * Each entry point names the reference function it replaces.
+ *
+ * Conventions
+ *   - Matrices are opaque handles; the library owns every device array of a
+ *     handle (reference: dcsr_matrix owns row_sizes/row_offsets/pool,
+ *     include/dcsr_matrix.h:8-39).
+ *   - Index type int32, value type float32: the reference's instantiation
+ *     (src/Fill_Test.inl:11-16 with PRECISION 32).
+ *   - Every array argument is a DEVICE pointer unless stated otherwise.
+ *   - `stream` is a hipStream_t (NULL = the null stream).  Calls are
+ *     stream-ordered.  Entry points documented as "syncs" wait for the stream
+ *     to read a device-side count back (pool accounting, output sizes).
+ *   - Every entry point returns a dcsr_status; the reference returns void and
+ *     prints errors (include/sparse.h:24-30).  Errors never leave a matrix
+ *     half-updated: a rejected update is a no-op.
+ *   - Threading: a handle may be used by one host thread at a time.
+ *
+ * Storage model: every row is one column-sorted segment
+ * [start, start+len) inside a pooled (col, val) store with capacity cap >= len;
+ * structural updates write the new row to a chunk claimed from the pool's
+ * bump region and retire the old chunk; compaction rewrites all rows
+ * contiguously in natural row order.
+ */
+#ifndef DCSR_H
+#define DCSR_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct dcsr_matrix_s *dcsr_handle;
+typedef void *dcsr_stream_t; /* hipStream_t */
+
+typedef enum dcsr_status {
+    DCSR_OK = 0,
+    DCSR_ERR_INVALID = 1,        /* bad argument (null handle, negative size, size mismatch) */
+    DCSR_ERR_BAD_INDEX = 2,      /* a row/column index outside the matrix */
+    DCSR_ERR_POOL_EXHAUSTED = 3, /* the update needs more pool than reserved and growth is disabled */
+    DCSR_ERR_SEGMENT_OVERFLOW = 4, /* a row would exceed the per-row limit (2^31-1) */
+    DCSR_ERR_HIP = 5,            /* a HIP runtime call failed */
+    DCSR_ERR_OOM = 6,            /* device allocation failed */
+    DCSR_ERR_CAPACITY = 7,       /* caller-provided output buffer too small */
+    DCSR_ERR_UNSUPPORTED = 8
+} dcsr_status;
+
+/* Flags for dcsr_update. */
+#define DCSR_UPDATE_NOSYNC   0x1u /* do not read the allocation check back; on
+                                     pool exhaustion the batch becomes a no-op
+                                     and DCSR_ERR_POOL_EXHAUSTED is reported by
+                                     the next dcsr_get_status */
+#define DCSR_UPDATE_NOGROW   0x2u /* never grow/compact automatically */
+/* Flags for dcsr_export_csr. */
+#define DCSR_EXPORT_CANONICAL 0x1u /* merge duplicate columns and drop exact
+                                      zeros (only differs before the first
+                                      update of a matrix loaded with duplicates
+                                      or explicit zeros) */
+
+typedef struct dcsr_config {
+    double pool_factor;  /* pool capacity = pool_factor * nnz at load (default 2.0) */
+    int64_t min_pool;    /* minimum pool entries (default 1024) */
+    int32_t allow_grow;  /* grow the pool when an update needs more (default 1) */
+    int32_t reserved;
+} dcsr_config;
+
+typedef struct dcsr_info {
+    int64_t rows, cols;
+    int64_t nnz;          /* live entries */
+    int64_t pool_cap;     /* pool capacity (entries) */
+    int64_t pool_used;    /* bump pointer: entries handed out since the last compaction */
+    int64_t garbage;      /* entries in retired chunks (reclaimed by compaction) */
+    int64_t hub_rows;     /* rows longer than the hub threshold */
+    int64_t last_inserted; /* last update: new coordinates stored */
+    int64_t last_added;    /* last update: existing coordinates updated in place */
+    int64_t last_deleted;  /* last update: coordinates removed (sum == 0) */
+    int64_t last_touched_rows;
+    int32_t canonical;    /* 1 if rows hold no duplicate columns and no exact zeros */
+    int32_t max_segments; /* segments per row: always 1 in this layout */
+} dcsr_info;
+
+const char *dcsr_status_string(dcsr_status s);
+/* Sticky device-side status of the handle (syncs the handle's last stream),
+ * then clears it. */
+dcsr_status dcsr_get_status(dcsr_handle h);
+
+/* Container setup.  Replaces dcsr_matrix::resize (include/dcsr_matrix.h:41-73,
+ * 76-108) and the owning constructor/destructor (:33-39).  cfg may be NULL. */
+dcsr_status dcsr_create(dcsr_handle *out, int64_t rows, int64_t cols, const dcsr_config *cfg);
+dcsr_status dcsr_destroy(dcsr_handle h);
+
+/* Uniform preallocation: every row empty with capacity bin_length.
+ * Replaces device::Initialize_Matrix -> InitializeMatrix_dcsr
+ * (include/load.h:8-22, include/sparse_update.inl:1155-1176). */
+dcsr_status dcsr_initialize(dcsr_handle h, int32_t bin_length, dcsr_stream_t stream);
+
+/* COO -> DCSR.  Triples may be unsorted; entries of a row keep their input
+ * order among equal columns (duplicates and explicit zeros are kept, as the
+ * host COO->CSR conversion keeps them).  Syncs.
+ * Replaces device::LoadMatrix(dcsr&, rows, cols, vals, N) (include/load.h:27-59),
+ * which additionally required row-sorted input. */
+dcsr_status dcsr_load_coo(dcsr_handle h, const int32_t *rows, const int32_t *cols,
+                          const float *vals, int64_t n, dcsr_stream_t stream);
+
+/* CSR -> DCSR.  Syncs.  Replaces device::LoadMatrix(dcsr&, const csr_matrix&)
+ * (include/load.h:207-227). */
+dcsr_status dcsr_load_csr(dcsr_handle h, const int32_t *row_offsets, const int32_t *cols,
+                          const float *vals, dcsr_stream_t stream);
+
+/* DCSR -> CSR in natural row order, columns ascending inside every row.
+ * Output arrays hold `cap` entries; *nnz_out receives the entry count (host
+ * pointer).  Syncs.  Replaces device::LoadMatrix(csr_matrix&, const dcsr&)
+ * (include/load.h:232-253), which was only correct for a freshly compacted
+ * matrix. */
+dcsr_status dcsr_export_csr(dcsr_handle h, int32_t *row_offsets, int32_t *cols, float *vals,
+                            int64_t cap, int64_t *nnz_out, uint32_t flags, dcsr_stream_t stream);
+
+/* In-place insert-or-add of a batch of n (row, col, val) triples in any order.
+ * Result equals the host path "sort batch by row, COO->CSR, cusp::add(A, B)":
+ * duplicate batch coordinates are summed in batch order, the sum is added to
+ * the stored value, and an exact-zero result removes the coordinate.
+ * Replaces device::UpdateMatrix(dcsr&, rows, cols, vals, permuted)
+ * (include/load.h:258-325), which appended without merging.
+ * Syncs once unless DCSR_UPDATE_NOSYNC. */
+dcsr_status dcsr_update(dcsr_handle h, const int32_t *rows, const int32_t *cols,
+                        const float *vals, int64_t n, uint32_t flags, dcsr_stream_t stream);
+
+/* Defragmentation: rewrite every row contiguously in natural row order and
+ * release retired chunks.  Replaces device::SortMatrixRow (include/load.h:404-581). */
+dcsr_status dcsr_compact(dcsr_handle h, dcsr_stream_t stream);
+
+/* Rebuild the row-length classification used by SpMV/SpMM (hub-row list).
+ * Replaces device::BinRows -> SetBins (include/load.h:583-610); called
+ * implicitly by the compute entry points after any structural change. */
+dcsr_status dcsr_rebin(dcsr_handle h, dcsr_stream_t stream);
+
+/* y = alpha * A x + beta * y  (beta == 0: y is overwritten, never read).
+ * Replaces device::spmv(const dcsr&, x, y, streams) (include/spmv.h:121-248). */
+dcsr_status dcsr_spmv(dcsr_handle h, const float *x, float *y, float alpha, float beta,
+                      dcsr_stream_t stream);
+
+/* Counters and sizes of the handle (syncs). */
+dcsr_status dcsr_query(dcsr_handle h, dcsr_info *info);
+
+/* Plain CSR SpMV comparator, y = A x.  Replaces device::spmv(const csr&, x, y)
+ * -> spmv_csr_vector_kernel (include/spmv.h:310-332, include/spmv.inl:708-781). */
+dcsr_status dcsr_csr_spmv(int64_t rows, const int32_t *row_offsets, const int32_t *cols,
+                          const float *vals, const float *x, float *y, dcsr_stream_t stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* DCSR_H */

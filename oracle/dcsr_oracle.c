@@ -1,0 +1,297 @@
+/*
+ * dcsr_oracle.c -- CPU restatement of the reference's host (cusp 0.5.1 /
+ * thrust 1.8.1 "sequential" backend) path for the Dynamic-CSR operations.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This file is the parity checker: it is loaded by
+ * tests/, by __graft_entry__.smoke() and by bench.py's cpu_baseline leg, and
+ * never by the product library (dcsr_amd/csrc), which has no CPU fallback.
+ *
+ * Every function restates one reference routine, cited file:line (paths are
+ * relative to the reference checkout).  Semantics that matter for bit-exact
+ * structure are reproduced exactly: FP32 accumulation order, exact-zero drop,
+ * stable ordering, and the output order inside each row.  Build with
+ * -ffp-contract=off (no fused multiply-add), as the reference host path
+ * compiled for x86-64 has no FMA.
+ *
+ * Pinning (how each function is checked, see tests/test_oracle_*.py):
+ *   orc_sort_by_row        vs oracle/_ref (cusp::sort_by_row) + brute force
+ *   orc_indices_to_offsets vs oracle/_ref + closed form (bincount/cumsum)
+ *   orc_csr_add            vs oracle/_ref (cusp::add) bit-exact incl. order,
+ *                          vs a dictionary-of-keys brute force on small inputs
+ *   orc_csr_spmv           vs oracle/_ref bit-exact, vs scipy FP64 within tol
+ *   orc_spgemm_*           vs oracle/_ref bit-exact incl. order, vs scipy
+ *                          structure on data without cancellation
+ *   orc_csr_to_hyb         vs oracle/_ref (generic csr->hyb conversion)
+ *   orc_update             composition of the above (sort, offsets, add)
+ *   orc_spmm_dense         no reference counterpart: defined as k column-wise
+ *                          orc_csr_spmv calls; pinned against orc_csr_spmv and
+ *                          scipy FP64.
+ */
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+
+typedef int32_t idx_t;
+
+/* cusp::sort_by_row -> cusp::counting_sort_by_key (stable counting sort on the
+ * row key carrying a permutation), then gather of (col, val) through the
+ * permutation.  cusp/detail/sort.inl:39-70, sequential/sort.h:80-124.
+ * min_row is forced to 0 (sort.h:100) and max_row = max element. */
+int orc_sort_by_row(int64_t n, idx_t *rows, idx_t *cols, float *vals)
+{
+    if (n <= 0) return 0;
+    idx_t maxr = rows[0];
+    for (int64_t i = 0; i < n; ++i) {
+        if (rows[i] < 0) return -1;
+        if (rows[i] > maxr) maxr = rows[i];
+    }
+    size_t nb = (size_t)maxr + 2;
+    size_t *counts = (size_t *)calloc(nb, sizeof(size_t));
+    idx_t *tk = (idx_t *)malloc(n * sizeof(idx_t));
+    idx_t *tc = (idx_t *)malloc(n * sizeof(idx_t));
+    float *tv = (float *)malloc(n * sizeof(float));
+    if (!counts || !tk || !tc || !tv) { free(counts); free(tk); free(tc); free(tv); return -2; }
+    memcpy(tk, rows, n * sizeof(idx_t));
+    memcpy(tc, cols, n * sizeof(idx_t));
+    memcpy(tv, vals, n * sizeof(float));
+    for (int64_t i = 0; i < n; ++i) counts[tk[i] + 1]++;
+    for (size_t i = 0; i + 1 < nb; ++i) counts[i + 1] += counts[i];
+    for (int64_t i = 0; i < n; ++i) {
+        size_t p = counts[tk[i]]++;
+        rows[p] = tk[i];
+        cols[p] = tc[i];
+        vals[p] = tv[i];
+    }
+    free(counts); free(tk); free(tc); free(tv);
+    return 0;
+}
+
+/* cusp::indices_to_offsets = thrust::lower_bound of 0..R over the (sorted)
+ * index array.  cusp/detail/format_utils.inl:188-200. */
+void orc_indices_to_offsets(int64_t n, const idx_t *rows, int64_t R, idx_t *offsets)
+{
+    for (int64_t r = 0; r <= R; ++r) {
+        int64_t lo = 0, hi = n;
+        while (lo < hi) {
+            int64_t mid = lo + (hi - lo) / 2;
+            if (rows[mid] < r) lo = mid + 1; else hi = mid;
+        }
+        offsets[r] = (idx_t)lo;
+    }
+}
+
+/* cusp::add on host CSR -> sequential elementwise with thrust::plus.
+ * cusp/detail/elementwise.inl:107-116, sequential/elementwise.h:44-131.
+ * Per row: A entries accumulate into A_row[j] (from 0, stored order), B
+ * entries into B_row[j]; each touched column is visited through the linked
+ * list (most recently first-touched column first) and emitted when
+ * A_row[j] + B_row[j] != 0.  Returns nnz of C. */
+int64_t orc_csr_add(int64_t R, int64_t C,
+                    const idx_t *Aoff, const idx_t *Acol, const float *Aval,
+                    const idx_t *Boff, const idx_t *Bcol, const float *Bval,
+                    idx_t *Coff, idx_t *Ccol, float *Cval)
+{
+    idx_t *next = (idx_t *)malloc((C > 0 ? C : 1) * sizeof(idx_t));
+    float *arow = (float *)calloc(C > 0 ? C : 1, sizeof(float));
+    float *brow = (float *)calloc(C > 0 ? C : 1, sizeof(float));
+    if (!next || !arow || !brow) { free(next); free(arow); free(brow); return -1; }
+    for (int64_t j = 0; j < C; ++j) next[j] = -1;
+    int64_t nnz = 0;
+    Coff[0] = 0;
+    for (int64_t i = 0; i < R; ++i) {
+        idx_t head = -2, length = 0;
+        for (idx_t jj = Aoff[i]; jj < Aoff[i + 1]; ++jj) {
+            idx_t j = Acol[jj];
+            arow[j] += Aval[jj];
+            if (next[j] == -1) { next[j] = head; head = j; length++; }
+        }
+        for (idx_t jj = Boff[i]; jj < Boff[i + 1]; ++jj) {
+            idx_t j = Bcol[jj];
+            brow[j] += Bval[jj];
+            if (next[j] == -1) { next[j] = head; head = j; length++; }
+        }
+        for (idx_t jj = 0; jj < length; ++jj) {
+            float result = arow[head] + brow[head];
+            if (result != 0.0f) {
+                Ccol[nnz] = head;
+                Cval[nnz] = result;
+                nnz++;
+            }
+            idx_t prev = head;
+            head = next[head];
+            next[prev] = -1;
+            arow[prev] = 0.0f;
+            brow[prev] = 0.0f;
+        }
+        Coff[i + 1] = (idx_t)nnz;
+    }
+    free(next); free(arow); free(brow);
+    return nnz;
+}
+
+/* cusp::multiply(csr, x, y) on host -> sequential csr_spmv with
+ * initialize = zero, combine = multiplies, reduce = plus.
+ * sequential/multiply/csr_spmv.h:41-73, generic/multiply.inl:70-72. */
+void orc_csr_spmv(int64_t R, const idx_t *off, const idx_t *col, const float *val,
+                  const float *x, float *y)
+{
+    for (int64_t i = 0; i < R; ++i) {
+        float acc = 0.0f;
+        for (idx_t jj = off[i]; jj < off[i + 1]; ++jj) {
+            float p = val[jj] * x[col[jj]];
+            acc = acc + p;
+        }
+        y[i] = acc;
+    }
+}
+
+/* Dense multi-RHS product Y = A X (X row-major C x k with leading dim ldx,
+ * Y row-major R x k with ldy).  No reference counterpart (the reference has
+ * SpMV only); defined as k independent orc_csr_spmv column products. */
+void orc_spmm_dense(int64_t R, const idx_t *off, const idx_t *col, const float *val,
+                    int64_t k, const float *X, int64_t ldx, float *Y, int64_t ldy)
+{
+    for (int64_t i = 0; i < R; ++i) {
+        for (int64_t c = 0; c < k; ++c) {
+            float acc = 0.0f;
+            for (idx_t jj = off[i]; jj < off[i + 1]; ++jj) {
+                float p = val[jj] * X[(int64_t)col[jj] * ldx + c];
+                acc = acc + p;
+            }
+            Y[i * ldy + c] = acc;
+        }
+    }
+}
+
+/* Symbolic pass of host CSR SpGEMM: number of distinct (i,k) with a partial
+ * product, explicit zeros included.  sequential/multiply/csr_spgemm.h:38-73. */
+int64_t orc_spgemm_count(int64_t R, int64_t Cb,
+                         const idx_t *Aoff, const idx_t *Acol,
+                         const idx_t *Boff, const idx_t *Bcol)
+{
+    int64_t *mask = (int64_t *)malloc((Cb > 0 ? Cb : 1) * sizeof(int64_t));
+    if (!mask) return -1;
+    for (int64_t k = 0; k < Cb; ++k) mask[k] = -1;
+    int64_t nnz = 0;
+    for (int64_t i = 0; i < R; ++i)
+        for (idx_t jj = Aoff[i]; jj < Aoff[i + 1]; ++jj) {
+            idx_t j = Acol[jj];
+            for (idx_t kk = Boff[j]; kk < Boff[j + 1]; ++kk) {
+                idx_t k = Bcol[kk];
+                if (mask[k] != i) { mask[k] = i; nnz++; }
+            }
+        }
+    free(mask);
+    return nnz;
+}
+
+/* Numeric pass: Gustavson with a linked list; sums[k] += a_ij * b_jk in A-row
+ * then B-row stored order; exact zeros dropped; row output in reverse
+ * first-touch order.  sequential/multiply/csr_spgemm.h:75-150. */
+int64_t orc_spgemm(int64_t R, int64_t Cb,
+                   const idx_t *Aoff, const idx_t *Acol, const float *Aval,
+                   const idx_t *Boff, const idx_t *Bcol, const float *Bval,
+                   idx_t *Coff, idx_t *Ccol, float *Cval)
+{
+    idx_t *next = (idx_t *)malloc((Cb > 0 ? Cb : 1) * sizeof(idx_t));
+    float *sums = (float *)calloc(Cb > 0 ? Cb : 1, sizeof(float));
+    if (!next || !sums) { free(next); free(sums); return -1; }
+    for (int64_t k = 0; k < Cb; ++k) next[k] = -1;
+    int64_t nnz = 0;
+    Coff[0] = 0;
+    for (int64_t i = 0; i < R; ++i) {
+        idx_t head = -2, length = 0;
+        for (idx_t jj = Aoff[i]; jj < Aoff[i + 1]; ++jj) {
+            idx_t j = Acol[jj];
+            float v = Aval[jj];
+            for (idx_t kk = Boff[j]; kk < Boff[j + 1]; ++kk) {
+                idx_t k = Bcol[kk];
+                float p = v * Bval[kk];
+                sums[k] = sums[k] + p;
+                if (next[k] == -1) { next[k] = head; head = k; length++; }
+            }
+        }
+        for (idx_t jj = 0; jj < length; ++jj) {
+            if (sums[head] != 0.0f) {
+                Ccol[nnz] = head;
+                Cval[nnz] = sums[head];
+                nnz++;
+            }
+            idx_t t = head;
+            head = next[head];
+            next[t] = -1;
+            sums[t] = 0.0f;
+        }
+        Coff[i + 1] = (idx_t)nnz;
+    }
+    free(next); free(sums);
+    return nnz;
+}
+
+/* Host CSR -> HYB with a given ELL width: the first `width` entries of each
+ * row (stored order) go to the column-major ELL block (index k*pitch + row,
+ * padding column -1 / value 0), the rest go to the COO part in CSR order.
+ * generic/conversions/csr_to_other.h:226-303.  Returns the COO entry count. */
+int64_t orc_csr_to_hyb(int64_t R, const idx_t *off, const idx_t *col, const float *val,
+                       int64_t width, int64_t pitch,
+                       idx_t *ell_col, float *ell_val,
+                       idx_t *coo_row, idx_t *coo_col, float *coo_val)
+{
+    for (int64_t t = 0; t < width * pitch; ++t) { ell_col[t] = -1; ell_val[t] = 0.0f; }
+    int64_t nc = 0;
+    for (int64_t i = 0; i < R; ++i) {
+        for (idx_t jj = off[i]; jj < off[i + 1]; ++jj) {
+            int64_t k = jj - off[i];
+            if (k < width) {
+                ell_col[k * pitch + i] = col[jj];
+                ell_val[k * pitch + i] = val[jj];
+            } else {
+                coo_row[nc] = (idx_t)i;
+                coo_col[nc] = col[jj];
+                coo_val[nc] = val[jj];
+                nc++;
+            }
+        }
+    }
+    return nc;
+}
+
+/* COO -> CSR on host: cusp::sort_by_row (the drivers sort before loading,
+ * src/Fill_Test.inl:90) then generic coo->csr conversion =
+ * indices_to_offsets + copies (generic/conversions/coo_to_other.h:118-125).
+ * Duplicates and explicit zeros are kept.  rows/cols/vals are sorted in place;
+ * the resulting CSR column/value arrays are cols/vals. */
+int orc_coo_to_csr(int64_t R, int64_t n, idx_t *rows, idx_t *cols, float *vals, idx_t *off)
+{
+    int rc = orc_sort_by_row(n, rows, cols, vals);
+    if (rc) return rc;
+    orc_indices_to_offsets(n, rows, R, off);
+    return 0;
+}
+
+/* One in-place update batch on the CPU path: the batch triples are sorted by
+ * row (cusp::sort_by_row), converted to CSR B, and C = cusp::add(A, B).  This
+ * is the reference driver's CSR update step (src/Fill_Test.inl:228,240-258)
+ * on host matrices.  The batch arrays are copied (the caller's are left
+ * untouched).  C arrays must hold nnz(A) + n entries.  Returns nnz(C). */
+int64_t orc_update(int64_t R, int64_t C,
+                   const idx_t *Aoff, const idx_t *Acol, const float *Aval,
+                   int64_t n, const idx_t *brow, const idx_t *bcol, const float *bval,
+                   idx_t *Coff, idx_t *Ccol, float *Cval)
+{
+    idx_t *r = (idx_t *)malloc((n > 0 ? n : 1) * sizeof(idx_t));
+    idx_t *c = (idx_t *)malloc((n > 0 ? n : 1) * sizeof(idx_t));
+    float *v = (float *)malloc((n > 0 ? n : 1) * sizeof(float));
+    idx_t *boff = (idx_t *)malloc((R + 1) * sizeof(idx_t));
+    if (!r || !c || !v || !boff) { free(r); free(c); free(v); free(boff); return -1; }
+    if (n > 0) {
+        memcpy(r, brow, n * sizeof(idx_t));
+        memcpy(c, bcol, n * sizeof(idx_t));
+        memcpy(v, bval, n * sizeof(float));
+    }
+    int64_t nnz;
+    if (orc_coo_to_csr(R, n, r, c, v, boff) != 0) nnz = -2;
+    else nnz = orc_csr_add(R, C, Aoff, Acol, Aval, boff, c, v, Coff, Ccol, Cval);
+    free(r); free(c); free(v); free(boff);
+    return nnz;
+}

@@ -1,0 +1,222 @@
+"""GPU parity: load / export / insert-or-add update / compaction / SpMV of the
+HIP engine against the oracle (cusp host-path semantics) and the reference's
+own fixtures.  Structure and update values are compared bit-exactly; SpMV
+values within 1e-5 of FP64 (conditioned: |y - y64| <= 1e-5 (|A||x|)_i)."""
+import os
+import sys
+
+import numpy as np
+import pytest
+
+import oracle as O
+from dcsr_amd import workloads as W
+from tests.helpers import assert_same_structure, fp64_spmv_check, random_coo, to_host_csr
+
+pytestmark = pytest.mark.gpu
+GOLD = os.path.join(os.path.dirname(__file__), "golden")
+
+
+def say(msg):
+    print(f"[gpu-test] {msg}", flush=True)
+    sys.stdout.flush()
+
+
+def gold_csr(d, prefix, R, C):
+    return O.Csr(R, C, d[f"{prefix}_off"], d[f"{prefix}_col"], d[f"{prefix}_val"])
+
+
+@pytest.fixture(scope="module")
+def torch():
+    import torch as t
+    assert t.cuda.is_available()
+    import dcsr_amd
+    dcsr_amd.lib()
+    return t
+
+
+def spmv_host(mat, x):
+    import torch
+    y = mat.spmv(torch.from_numpy(x).cuda())
+    torch.cuda.synchronize()
+    return y.cpu().numpy()
+
+
+def test_banded_flow_matches_reference_fixtures(torch):
+    from dcsr_amd import DcsrMatrix
+    d = np.load(os.path.join(GOLD, "banded256.npz"))
+    R = int(d["R"])
+    say("banded256 load")
+    m = DcsrMatrix(R, R)
+    m.LoadMatrix(d["in_row"], d["in_col"], d["in_val"])
+    assert_same_structure(to_host_csr(m), gold_csr(d, "load", R, R))
+    for i in range(4):
+        say(f"banded256 update {i}")
+        m.UpdateMatrix(d[f"b{i}_row"], d[f"b{i}_col"], d[f"b{i}_val"])
+        assert_same_structure(to_host_csr(m), gold_csr(d, f"upd{i}", R, R))
+    say("banded256 spmv")
+    A = gold_csr(d, "upd3", R, R)
+    y = spmv_host(m, d["x"])
+    fp64_spmv_check(A, d["x"], y)
+    say("banded256 compact")
+    m.SortMatrixRow()
+    assert m.info()["garbage"] == 0
+    assert_same_structure(to_host_csr(m), A)
+    fp64_spmv_check(A, d["x"], spmv_host(m, d["x"]))
+    m.close()
+
+
+def test_edge_fixture_duplicates_zeros_deletes(torch):
+    from dcsr_amd import DcsrMatrix
+    d = np.load(os.path.join(GOLD, "edge.npz"))
+    R, C = int(d["R"]), int(d["C"])
+    say("edge load")
+    m = DcsrMatrix(R, C)
+    m.LoadMatrix(d["in_row"], d["in_col"], d["in_val"])
+    info = m.info()
+    assert info["canonical"] == 0 and info["nnz"] == d["in_row"].size
+    assert_same_structure(to_host_csr(m), gold_csr(d, "load", R, C))
+    say("edge canonical export")
+    assert_same_structure(to_host_csr(m, canonical=True), gold_csr(d, "canon", R, C))
+    say("edge update with deletes")
+    m.UpdateMatrix(d["b0_row"], d["b0_col"], d["b0_val"])
+    assert_same_structure(to_host_csr(m), gold_csr(d, "upd0", R, C))
+    assert m.info()["canonical"] == 1
+    y = spmv_host(m, d["x"])
+    np.testing.assert_array_equal(y, d["y"])  # small integers: every order is exact
+    m.close()
+
+
+@pytest.mark.parametrize("R,C,n,seed", [(1, 1, 3, 0), (33, 65, 500, 1), (1000, 700, 30000, 2), (5000, 5000, 200000, 3)])
+def test_random_update_chain(torch, R, C, n, seed):
+    from dcsr_amd import DcsrMatrix
+    rng = np.random.default_rng(seed)
+    r, c, v = random_coo(rng, R, C, n, 0.0, 0.0, "sym")
+    say(f"random chain R={R} C={C} n={n}")
+    m = DcsrMatrix(R, C, pool_factor=1.2)
+    m.LoadMatrix(r, c, v)
+    A = O.coo_to_csr(R, C, r, c, v)
+    for k in range(4):
+        br, bc, bv = random_coo(rng, R, C, max(1, n // 2), 0.2, 0.02, "small_int" if k % 2 else "sym")
+        if k == 2 and A.nnz:
+            rows_of = np.repeat(np.arange(R), np.diff(A.off))
+            pick = rng.integers(0, A.nnz, max(1, A.nnz // 5))
+            br = np.concatenate([br, rows_of[pick]]).astype(np.int32)
+            bc = np.concatenate([bc, A.col[pick]]).astype(np.int32)
+            bv = np.concatenate([bv, -A.val[pick]]).astype(np.float32)
+        say(f"  update {k} ({br.size} triples)")
+        m.UpdateMatrix(br, bc, bv)
+        A = O.update(A, br, bc, bv)
+        assert_same_structure(to_host_csr(m), A)
+        info = m.info()
+        assert info["nnz"] == A.nnz
+        x = rng.uniform(-1, 1, C).astype(np.float32)
+        fp64_spmv_check(A, x, spmv_host(m, x))
+    say("  compact")
+    m.SortMatrixRow()
+    assert_same_structure(to_host_csr(m), A)
+    m.close()
+
+
+def test_powerlaw_with_hub_rows(torch):
+    """Rows longer than the hub threshold go through the split SpMV path."""
+    from dcsr_amd import DcsrMatrix
+    say("powerlaw hubs")
+    R = 20000
+    r, c, v = W.powerlaw(R, 400000, max_degree=19000, seed=5, signed=True)
+    m = DcsrMatrix(R, R)
+    m.LoadMatrix(r, c, v)
+    A = O.coo_to_csr(R, R, r, c, v)
+    assert np.diff(A.off).max() > 4096
+    rng = np.random.default_rng(9)
+    x = rng.uniform(-1, 1, R).astype(np.float32)
+    y = spmv_host(m, x)
+    fp64_spmv_check(A, x, y)
+    y2 = spmv_host(m, x)
+    np.testing.assert_array_equal(y, y2)  # deterministic
+    br, bc, bv = W.batch_mixed(rng, r, c, R, R, 100000, signed=True)
+    m.UpdateMatrix(br, bc, bv)
+    A = O.update(A, br, bc, bv)
+    assert_same_structure(to_host_csr(m), A)
+    fp64_spmv_check(A, x, spmv_host(m, x))
+    m.close()
+
+
+def test_alpha_beta_and_csr_comparator(torch):
+    from dcsr_amd import DcsrMatrix, csr_spmv
+    rng = np.random.default_rng(11)
+    R, C = 3000, 2000
+    r, c, v = random_coo(rng, R, C, 60000, 0.0, 0.0, "sym")
+    m = DcsrMatrix(R, C)
+    m.LoadMatrix(r, c, v)
+    A = O.coo_to_csr(R, C, r, c, v)
+    x = rng.uniform(-1, 1, C).astype(np.float32)
+    y0 = rng.uniform(-1, 1, R).astype(np.float32)
+    say("alpha/beta spmv")
+    y = torch.from_numpy(y0.copy()).cuda()
+    m.spmv(torch.from_numpy(x).cuda(), y, alpha=2.0, beta=-0.5)
+    torch.cuda.synchronize()
+    want = 2.0 * (A.to_scipy() @ x.astype(np.float64)) - 0.5 * y0
+    scale = 2.0 * (abs(A.to_scipy()) @ np.abs(x.astype(np.float64))) + 0.5 * np.abs(y0)
+    assert (np.abs(y.cpu().numpy() - want) <= 1e-5 * scale + 1e-30).all()
+    say("csr comparator")
+    yc = csr_spmv(A.off, A.col, A.val, x)
+    torch.cuda.synchronize()
+    fp64_spmv_check(A, x, yc.cpu().numpy())
+    m.close()
+
+
+def test_errors_leave_matrix_unchanged(torch):
+    from dcsr_amd import DcsrError, DcsrMatrix, UPDATE_NOGROW
+    rng = np.random.default_rng(12)
+    R = 200
+    keys = rng.choice(R * R, 2000, replace=False)
+    r, c = (keys // R).astype(np.int32), (keys % R).astype(np.int32)
+    v = rng.uniform(-1, 1, keys.size).astype(np.float32)
+    m = DcsrMatrix(R, R, pool_factor=1.0, min_pool=16, allow_grow=False)
+    m.LoadMatrix(r, c, v)
+    A = O.coo_to_csr(R, R, r, c, v)
+    say("bad index update")
+    with pytest.raises(DcsrError) as e:
+        m.UpdateMatrix(np.array([1, R]), np.array([1, 1]), np.array([1.0, 1.0], np.float32))
+    assert e.value.code == 2
+    assert_same_structure(to_host_csr(m), A)
+    say("pool exhaustion without growth")
+    br, bc, bv = random_coo(rng, R, R, 4000, 0.0, 0.0, "sym")
+    with pytest.raises(DcsrError) as e:
+        m.UpdateMatrix(br, bc, bv, flags=UPDATE_NOGROW)
+    assert e.value.code == 3
+    assert_same_structure(to_host_csr(m), A)
+    say("empty batch")
+    m.UpdateMatrix(np.zeros(0, np.int32), np.zeros(0, np.int32), np.zeros(0, np.float32))
+    assert_same_structure(to_host_csr(m), A)
+    say("bad load index")
+    with pytest.raises(DcsrError):
+        m.LoadMatrix(np.array([0, -1]), np.array([0, 0]), np.ones(2, np.float32))
+    m.close()
+    say("growth on demand")
+    m = DcsrMatrix(R, R, pool_factor=1.0, min_pool=16)
+    m.LoadMatrix(r, c, v)
+    m.UpdateMatrix(br, bc, bv)
+    assert_same_structure(to_host_csr(m), O.update(A, br, bc, bv))
+    m.close()
+
+
+def test_csr_load_and_initialize(torch):
+    from dcsr_amd import DcsrMatrix
+    rng = np.random.default_rng(13)
+    R, C = 400, 300
+    r, c, v = random_coo(rng, R, C, 5000, 0.1, 0.0, "sym")
+    A = O.coo_to_csr(R, C, r, c, v)
+    say("load_csr")
+    m = DcsrMatrix(R, C)
+    m.load_csr(A.off, A.col, A.val)
+    assert_same_structure(to_host_csr(m), A)
+    say("initialize + update")
+    m2 = DcsrMatrix(R, C)
+    m2.Initialize_Matrix(8)
+    assert m2.info()["nnz"] == 0
+    m2.UpdateMatrix(r, c, v)
+    E = O.Csr(R, C, np.zeros(R + 1, np.int32), [], [])
+    assert_same_structure(to_host_csr(m2), O.update(E, r, c, v))
+    m.close()
+    m2.close()

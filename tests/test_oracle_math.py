@@ -1,0 +1,165 @@
+"""Pins of the oracle that do not depend on the reference build: brute force
+on small inputs, closed forms and an independent library (scipy, FP64).
+Each check is chosen so that a plausible mistake in the restatement (wrong
+accumulation order, missing zero drop, unstable sort, transposed operand)
+fails it."""
+import numpy as np
+import pytest
+
+import oracle as O
+from tests.helpers import fp64_spmv_check, random_coo
+
+
+def brute_sort_by_row(r, c, v):
+    order = sorted(range(len(r)), key=lambda i: (int(r[i]), i))  # stable by row
+    return r[order], c[order], v[order]
+
+
+def test_sort_by_row_is_stable_counting_sort():
+    rng = np.random.default_rng(1)
+    r, c, v = random_coo(rng, 20, 20, 400, 0.3, 0.0, "sym")
+    got = O.sort_by_row(r, c, v)
+    want = brute_sort_by_row(r, c, v)
+    for g, w in zip(got, want):
+        np.testing.assert_array_equal(g, w)
+
+
+def test_indices_to_offsets_closed_form():
+    rng = np.random.default_rng(2)
+    R = 50
+    r = np.sort(rng.integers(0, R, 1000)).astype(np.int32)
+    A = O.coo_to_csr(R, 10, r, np.zeros_like(r), np.ones(r.size, np.float32))
+    want = np.concatenate([[0], np.cumsum(np.bincount(r, minlength=R))])
+    np.testing.assert_array_equal(A.off, want)
+
+
+def brute_add(A, B):
+    """Dictionary-of-keys A + B with float32 left-to-right sums per
+    coordinate, A part and B part accumulated separately, exact zero drop."""
+    out = []
+    for i in range(A.rows):
+        sa, sb, order = {}, {}, []
+        for M, acc in ((A, sa), (B, sb)):
+            cols, vals = M.row(i)
+            for j, x in zip(cols.tolist(), vals.tolist()):
+                if j not in sa and j not in sb:
+                    order.append(j)
+                acc[j] = np.float32(acc.get(j, np.float32(0)) + np.float32(x))
+        row = {}
+        for j in order:
+            s = np.float32(sa.get(j, np.float32(0)) + sb.get(j, np.float32(0)))
+            if s != 0:
+                row[j] = s
+        out.append(row)
+    return out
+
+
+@pytest.mark.parametrize("seed", [0, 1, 2])
+def test_add_matches_brute_force(seed):
+    rng = np.random.default_rng(seed)
+    R, C = 30, 40
+    A = O.coo_to_csr(R, C, *random_coo(rng, R, C, 300, 0.3, 0.1, "small_int"))
+    B = O.coo_to_csr(R, C, *random_coo(rng, R, C, 200, 0.3, 0.1, "small_int"))
+    got = O.csr_add(A, B)
+    want = brute_add(A, B)
+    for i in range(R):
+        cols, vals = got.row(i)
+        assert dict(zip(cols.tolist(), vals.tolist())) == {k: float(v) for k, v in want[i].items()}
+        assert len(set(cols.tolist())) == cols.size  # unique columns
+    # exact zeros are dropped and the stored order is reverse first touch
+    assert not (got.val == 0).any()
+
+
+def test_add_sums_batch_duplicates_before_adding():
+    # a + (b1 + b2) differs from (a + b1) + b2 in float32 for these values
+    a, b1, b2 = np.float32(1.0), np.float32(1e8), np.float32(-1e8)
+    A = O.Csr(1, 4, [0, 1], [2], [a])
+    B = O.Csr(1, 4, [0, 2], [2, 2], [b1, b2])
+    got = O.csr_add(A, B)
+    assert got.val[0] == a + (b1 + b2) == np.float32(1.0)
+    assert got.val[0] != (a + b1) + b2
+
+
+def test_update_order_of_rows_is_reverse_first_touch():
+    A = O.Csr(1, 8, [0, 3], [5, 1, 3], [1, 1, 1])
+    got = O.update(A, np.array([0, 0]), np.array([7, 0]), np.array([2, 2], np.float32))
+    np.testing.assert_array_equal(got.col, [0, 7, 3, 1, 5])
+
+
+@pytest.mark.parametrize("seed", [0, 1])
+def test_spmv_against_fp64(seed):
+    rng = np.random.default_rng(seed)
+    R, C = 500, 400
+    A = O.coo_to_csr(R, C, *random_coo(rng, R, C, 20000, 0.0, 0.0, "sym"))
+    x = rng.uniform(-1, 1, C).astype(np.float32)
+    y = O.spmv(A, x)
+    fp64_spmv_check(A, x, y, rtol=1e-5)
+    # the check is sharp enough to reject a transposed operand
+    Rs = 300
+    B = O.coo_to_csr(Rs, Rs, *random_coo(rng, Rs, Rs, 5000, 0.0, 0.0, "sym"))
+    xs = rng.uniform(-1, 1, Rs).astype(np.float32)
+    yt = (B.to_scipy().T @ xs.astype(np.float64)).astype(np.float32)
+    with pytest.raises(AssertionError):
+        fp64_spmv_check(B, xs, yt, rtol=1e-5)
+
+
+def test_spmv_sequential_order():
+    # ((0 + 1e8*1) + (-1e8*1)) + 1*1 = 1 ; a tree order would give 0 or 1 too,
+    # so use three terms where order matters
+    A = O.Csr(1, 3, [0, 3], [0, 1, 2], [1e8, 1.0, -1e8])
+    y = O.spmv(A, np.ones(3, np.float32))
+    assert y[0] == np.float32(np.float32(np.float32(1e8) + 1.0) - np.float32(1e8))
+
+
+def test_spmm_dense_is_columnwise_spmv():
+    rng = np.random.default_rng(5)
+    R, C, k = 60, 50, 4
+    A = O.coo_to_csr(R, C, *random_coo(rng, R, C, 700, 0.1, 0.0, "sym"))
+    X = rng.uniform(-1, 1, (C, k)).astype(np.float32)
+    Y = O.spmm_dense(A, X)
+    for j in range(k):
+        np.testing.assert_array_equal(Y[:, j], O.spmv(A, X[:, j].copy()))
+
+
+def test_spgemm_structure_matches_scipy_without_cancellation():
+    rng = np.random.default_rng(6)
+    R = 120
+    A = O.coo_to_csr(R, R, *random_coo(rng, R, R, 900, 0.0, 0.0, "pos"))
+    G = O.spgemm(A, A)
+    S = (A.to_scipy() @ A.to_scipy()).tocsr()
+    S.sort_indices()
+    Gc = G.canonical()
+    np.testing.assert_array_equal(Gc.off, S.indptr)
+    np.testing.assert_array_equal(Gc.col, S.indices)
+    np.testing.assert_allclose(Gc.val, S.data, rtol=1e-5)
+    assert O.spgemm_count(A, A) == S.nnz
+
+
+def test_spgemm_drops_exact_zeros():
+    # row 0 of A*B: 1*1 + 1*(-1) = 0 at column 0 -> dropped
+    A = O.Csr(1, 2, [0, 2], [0, 1], [1, 1])
+    B = O.Csr(2, 2, [0, 1, 3], [0, 0, 1], [1, -1, 2])
+    G = O.spgemm(A, B)
+    np.testing.assert_array_equal(G.col, [1])
+    np.testing.assert_array_equal(G.val, [2])
+    assert O.spgemm_count(A, B) == 2
+
+
+def test_hyb_brute_force():
+    rng = np.random.default_rng(8)
+    R = 40
+    A = O.coo_to_csr(R, R, *random_coo(rng, R, R, 300, 0.0, 0.0, "sym"))
+    w = 3
+    ec, ev, cr, cc, cv, pitch = O.csr_to_hyb(A, w)
+    assert pitch % 32 == 0 and pitch >= R
+    coo = []
+    for i in range(R):
+        cols, vals = A.row(i)
+        for k, (j, x) in enumerate(zip(cols, vals)):
+            if k < w:
+                assert ec[k * pitch + i] == j and ev[k * pitch + i] == x
+            else:
+                coo.append((i, j, x))
+        for k in range(len(cols), w):
+            assert ec[k * pitch + i] == -1
+    assert [(int(a), int(b), float(c)) for a, b, c in zip(cr, cc, cv)] == [(a, int(b), float(c)) for a, b, c in coo]
