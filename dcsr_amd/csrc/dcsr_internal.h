@@ -10,10 +10,11 @@
 #define ST_POOL_EXHAUSTED 0x2u
 #define ST_SEGMENT_OVERFLOW 0x4u
 
-// Rows longer than this are "hubs": SpMV splits them over whole workgroups.
+// Rows longer than this are counted as power-law hubs in dcsr_info.
 #define DCSR_HUB_LEN 4096
-// Entries per hub chunk (one 256-thread workgroup).
 #define DCSR_HUB_CHUNK 8192
+// Nonzeros owned by one wave in the flat SpMV / SpMM kernels.
+#define DCSR_SPMV_PW 256u
 
 // Pool chunk granularity: chunk starts stay 16-byte aligned.
 #define DCSR_CHUNK_ALIGN 4u
@@ -30,17 +31,17 @@ struct DevState {
     long long n_ins, n_add, n_del;  // current update counters
     long long d_hubs;               // current update: hub-row count delta
     long long d_hub_chunks;         // current update: hub chunk delta
+    unsigned long long need;        // current update: claimed capacity
     unsigned int status;            // sticky ST_* bits
     unsigned int go;                // 1 if the current update may commit
-    unsigned int nuniq;             // unique coordinates in the current batch
-    unsigned int ntouched;          // rows touched by the current batch
+    unsigned int nuniq;             // \ distinct coordinates in the batch
+    unsigned int ntouched;          // /  rows touched (adjacent: one 2-word scan total)
+    unsigned int claim_total;       // \ pool slots claimed by rewritten rows
+    unsigned int nrewrite_old;      // /  old entries moved (adjacent: one 2-word scan total)
     unsigned int hub_count;         // entries in the hub list
     unsigned int hub_chunks;        // total hub chunks
-    unsigned int nrewrite_old;      // old entries moved by the current update
     unsigned int ubad;              // current batch holds an out-of-range index
-    unsigned long long need;        // current update: claimed capacity
     unsigned int hub_touch;         // current update: rewritten rows that were/are hubs
-    unsigned int pad2;
 };
 
 // Grow-only device scratch buffers owned by a handle.
@@ -49,6 +50,7 @@ enum WsSlot {
     WS_UCOL, WS_UVAL, WS_UT, WS_UPOS, WS_UCODE, WS_UPRE,
     WS_TROW, WS_TUBEG, WS_TINS, WS_TDEL, WS_TOLDS, WS_TOLDL, WS_TDEST, WS_TNEWL, WS_TMODE, WS_TOLDPRE,
     WS_HUB_ROWS, WS_HUB_PRE, WS_HUB_PART, WS_HUB_DONE,
+    WS_PPROBE, WS_PPLAN, WS_TPAIR, WS_TPAIRPRE, WS_FOFF, WS_CARRY_ROW, WS_CARRY_VAL,
     WS_CANON, WS_TMP0, WS_TMP1, WS_TMP2,
     WS_COUNT
 };
@@ -69,12 +71,17 @@ struct dcsr_matrix_s {
     int32_t *pool_col;
     float *pool_val;
     int64_t pool_cap;
+    // second pool of the same layout: compaction copies into it and swaps
+    int32_t *spare_col;
+    float *spare_val;
+    int64_t spare_cap;
     // device/host state
     DevState *ds;            // device
     DevState *hs;            // pinned host mirror
     int64_t nnz_h, bump_h, garbage_h, hub_rows_h, hub_chunks_h;
     bool canonical;
     bool hubs_dirty;
+    bool flat_dirty;         // flat row offsets (SpMV binning) need a rebuild
     bool state_known;        // host mirrors valid (no NOSYNC update since last sync)
     dcsr_config cfg;
     Workspace ws;
@@ -100,7 +107,7 @@ struct dcsr_matrix_s {
 dcsr_status dcsr_hip_fail(hipError_t e, const char *what, const char *file, int line);
 dcsr_status ws_get(dcsr_handle h, WsSlot slot, size_t bytes, void **out);
 dcsr_status pull_state(dcsr_handle h, hipStream_t s);   // copy DevState to host + sync
-dcsr_status ensure_hub_list(dcsr_handle h, hipStream_t s);
+dcsr_status ensure_flat(dcsr_handle h, hipStream_t s);  // spmv.hip
 dcsr_status pool_reserve(dcsr_handle h, int64_t min_free, hipStream_t s, const int32_t *reserve_len);
 dcsr_status canonicalize(dcsr_handle h, hipStream_t s);
 

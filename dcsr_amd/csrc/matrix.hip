@@ -240,68 +240,16 @@ __global__ void k_canon_offsets(const int32_t *__restrict__ off_in, int64_t R, c
     }
 }
 
-// Hub-row list: rows with len > DCSR_HUB_LEN.
-__global__ void k_hub_reset(DevState *ds)
-{
-    ds->hub_count = 0;
-    ds->hub_chunks = 0;
-}
-
-__global__ void k_hub_collect(int64_t R, const int32_t *__restrict__ len, uint32_t *__restrict__ hub_rows,
-                              uint32_t cap, DevState *ds)
+// Number of rows longer than DCSR_HUB_LEN (reported by dcsr_query).
+__global__ void k_count_hubs(int64_t R, const int32_t *__restrict__ len, DevState *ds)
 {
     int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     bool hub = r < R && len[r] > DCSR_HUB_LEN;
     uint64_t m = __ballot(hub);
-    if (!m) return;
-    uint32_t base = 0;
-    int leader = __ffsll((long long)m) - 1;
-    if (lane_id() == leader) base = atomicAdd(&ds->hub_count, (uint32_t)__popcll(m));
-    base = __shfl(base, leader);
-    if (hub) {
-        uint32_t slot = base + (uint32_t)__popcll(m & lanemask_lt());
-        if (slot < cap) hub_rows[slot] = (uint32_t)r;
-    }
+    if (m && lane_id() == 0) atomicAdd(&ds->hub_count, (uint32_t)__popcll(m));
 }
 
-// Single workgroup: exclusive chunk prefix over the hub list (+ total).
-__global__ __launch_bounds__(1024) void k_hub_prefix(const int32_t *__restrict__ len,
-                                                     const uint32_t *__restrict__ hub_rows,
-                                                     uint32_t *__restrict__ hub_pre, uint32_t *__restrict__ hub_done,
-                                                     DevState *ds)
-{
-    __shared__ uint32_t lds16[16];
-    __shared__ uint32_t carry;
-    if (threadIdx.x == 0) carry = 0;
-    __syncthreads();
-    const uint32_t cnt = ds->hub_count;
-    const int w = threadIdx.x >> 6;
-    for (uint32_t base = 0; base < cnt; base += 1024) {
-        uint32_t h = base + threadIdx.x;
-        uint32_t c = 0;
-        if (h < cnt) {
-            uint32_t L = (uint32_t)len[hub_rows[h]];
-            c = (L + DCSR_HUB_CHUNK - 1) / DCSR_HUB_CHUNK;
-            hub_done[h] = 0;
-        }
-        uint32_t inc = wave_incl_scan_u32(c);
-        if (lane_id() == 63) lds16[w] = inc;
-        __syncthreads();
-        uint32_t pre = 0, tot = 0;
-        for (int i = 0; i < 16; ++i) {
-            if (i < w) pre += lds16[i];
-            tot += lds16[i];
-        }
-        if (h < cnt) hub_pre[h] = carry + pre + inc - c;
-        __syncthreads();
-        if (threadIdx.x == 0) carry += tot;
-        __syncthreads();
-    }
-    if (threadIdx.x == 0) {
-        hub_pre[cnt] = carry;
-        ds->hub_chunks = carry;
-    }
-}
+__global__ void k_zero_hubs(DevState *ds) { ds->hub_count = 0; }
 
 }  // namespace
 
@@ -419,6 +367,7 @@ dcsr_status dcsr_create(dcsr_handle *out, int64_t rows, int64_t cols, const dcsr
     }
     h->canonical = true;
     h->hubs_dirty = false;
+    h->flat_dirty = true;
     h->state_known = true;
     dcsr_status st = alloc_rows(h);
     if (st == DCSR_OK) st = alloc_pool(h, h->cfg.min_pool);
@@ -448,6 +397,8 @@ dcsr_status dcsr_destroy(dcsr_handle h)
     (void)hipFree(h->row_cap);
     (void)hipFree(h->pool_col);
     (void)hipFree(h->pool_val);
+    (void)hipFree(h->spare_col);
+    (void)hipFree(h->spare_val);
     (void)hipFree(h->ds);
     if (h->hs) (void)hipHostFree(h->hs);
     for (int i = 0; i < WS_COUNT; ++i)
@@ -477,6 +428,7 @@ dcsr_status dcsr_initialize(dcsr_handle h, int32_t bin_length, dcsr_stream_t str
     h->hub_chunks_h = 0;
     h->canonical = true;
     h->hubs_dirty = true;
+    h->flat_dirty = true;
     h->state_known = true;
     return DCSR_OK;
 }
@@ -503,6 +455,19 @@ static dcsr_status load_sorted(dcsr_handle h, const uint64_t *key, const uint32_
         DCSR_HIP(hipStreamSynchronize(s));
         DCSR_TRY(alloc_pool(h, want));
     }
+    // the spare pool (compaction target) is sized with the main pool so that
+    // compactions during updates allocate nothing
+    if (h->spare_cap < h->pool_cap) {
+        DCSR_HIP(hipStreamSynchronize(s));
+        if (h->spare_col) (void)hipFree(h->spare_col);
+        if (h->spare_val) (void)hipFree(h->spare_val);
+        h->spare_col = nullptr;
+        h->spare_val = nullptr;
+        h->spare_cap = 0;
+        DCSR_HIP(hipMalloc(&h->spare_col, (size_t)h->pool_cap * 4));
+        DCSR_HIP(hipMalloc(&h->spare_val, (size_t)h->pool_cap * 4));
+        h->spare_cap = h->pool_cap;
+    }
     k_fill_pool<<<grid_for(n, TB), TB, 0, s>>>(key, val, n, h->cbits, h->row_start, first, h->pool_col, h->pool_val);
     k_set_loaded<<<grid_for(R > 0 ? R : 1, TB), TB, 0, s>>>(R, h->row_start, h->row_len, h->row_cap, h->ds,
                                                             (uint64_t)n);
@@ -512,6 +477,7 @@ static dcsr_status load_sorted(dcsr_handle h, const uint64_t *key, const uint32_
     DCSR_TRY(pull_state(h, s));
     h->canonical = hflag[0] == 0;
     h->hubs_dirty = true;
+    h->flat_dirty = true;
     h->hub_rows_h = -1;  // unknown until the hub list is rebuilt
     return DCSR_OK;
 }
@@ -574,12 +540,11 @@ dcsr_status dcsr_load_csr(dcsr_handle h, const int32_t *row_offsets, const int32
 // Exclusive offsets of the current row lengths (u32, R+1 entries) into `off`.
 static dcsr_status row_offsets_u32(dcsr_handle h, uint32_t *off, hipStream_t s)
 {
-    uint32_t *lens, *part;
+    uint32_t *part;
     const int64_t R = h->rows;
-    DCSR_TRY(ws_get(h, WS_TMP1, (size_t)(R + 1) * 4, (void **)&lens));
     DCSR_TRY(ws_get(h, WS_PART, scan_part_words(R + 1, 1) * 4, (void **)&part));
-    k_len_u32<<<grid_for(R, TB), TB, 0, s>>>(h->row_len, R, lens);
-    DCSR_HIP(scan_exclusive_u32(lens, off, R, 1, part, off + R, s));
+    // lengths are non-negative int32: scan them as uint32
+    DCSR_HIP(scan_exclusive_u32((const uint32_t *)h->row_len, off, R, 1, part, off + R, s));
     return DCSR_OK;
 }
 
@@ -650,6 +615,9 @@ dcsr_status dcsr_export_csr(dcsr_handle h, int32_t *row_offsets, int32_t *cols, 
 }
 
 // Rewrite every row contiguously into a pool of capacity new_cap (>= nnz).
+// The destination is the handle's spare pool (allocated only when it is too
+// small); the old pool becomes the spare, so steady-state compaction makes no
+// allocation and no host synchronization beyond the state read-back.
 static dcsr_status compact_into(dcsr_handle h, int64_t new_cap, hipStream_t s)
 {
     const int64_t R = h->rows;
@@ -661,23 +629,32 @@ static dcsr_status compact_into(dcsr_handle h, int64_t new_cap, hipStream_t s)
     uint32_t *off;
     DCSR_TRY(ws_get(h, WS_TMP0, (size_t)(R + 1) * 4, (void **)&off));
     DCSR_TRY(row_offsets_u32(h, off, s));
-    int32_t *ncol = nullptr;
-    float *nval = nullptr;
-    DCSR_HIP(hipMalloc(&ncol, (size_t)new_cap * 4));
-    if (hipMalloc(&nval, (size_t)new_cap * 4) != hipSuccess) {
-        (void)hipFree(ncol);
-        return DCSR_ERR_OOM;
+    if (h->spare_cap < new_cap) {
+        DCSR_HIP(hipStreamSynchronize(s));
+        if (h->spare_col) (void)hipFree(h->spare_col);
+        if (h->spare_val) (void)hipFree(h->spare_val);
+        h->spare_col = nullptr;
+        h->spare_val = nullptr;
+        h->spare_cap = 0;
+        DCSR_HIP(hipMalloc(&h->spare_col, (size_t)new_cap * 4));
+        DCSR_HIP(hipMalloc(&h->spare_val, (size_t)new_cap * 4));
+        h->spare_cap = new_cap;
     }
-    launch_copy_rows(off, R, nnz, h->row_start, h->pool_col, h->pool_val, ncol, nval, s);
+    launch_copy_rows(off, R, nnz, h->row_start, h->pool_col, h->pool_val, h->spare_col, h->spare_val, s);
     if (R > 0) k_set_compacted<<<grid_for(R, TB), TB, 0, s>>>(R, off, h->row_start, h->row_len, h->row_cap, h->ds, off + R);
+    else k_set_compacted<<<1, 1, 0, s>>>(0, off, h->row_start, h->row_len, h->row_cap, h->ds, off);
     DCSR_HIP(hipGetLastError());
-    DCSR_HIP(hipStreamSynchronize(s));
-    (void)hipFree(h->pool_col);
-    (void)hipFree(h->pool_val);
-    h->pool_col = ncol;
-    h->pool_val = nval;
-    h->pool_cap = new_cap;
-    DCSR_TRY(pull_state(h, s));
+    int32_t *oc = h->pool_col;
+    float *ov = h->pool_val;
+    int64_t ocap = h->pool_cap;
+    h->pool_col = h->spare_col;
+    h->pool_val = h->spare_val;
+    h->pool_cap = h->spare_cap;
+    h->spare_col = oc;
+    h->spare_val = ov;
+    h->spare_cap = ocap;
+    h->bump_h = nnz;
+    h->garbage_h = 0;
     return DCSR_OK;
 }
 
@@ -691,15 +668,19 @@ dcsr_status dcsr_compact(dcsr_handle h, dcsr_stream_t stream)
     return compact_into(h, h->pool_cap, s);
 }
 
-// Make room for at least min_free entries past the bump pointer.
+// Make room for at least min_free entries past the bump pointer: compact
+// (releasing retired chunks) and grow geometrically when live data plus the
+// request would fill more than 2/3 of the pool.
 dcsr_status pool_reserve(dcsr_handle h, int64_t min_free, hipStream_t s, const int32_t *)
 {
     if (!h->state_known) DCSR_TRY(pull_state(h, s));
     if (h->pool_cap - h->bump_h >= min_free) return DCSR_OK;
     int64_t cap = h->pool_cap;
-    if (cap - h->nnz_h < min_free + min_free / 4 || h->garbage_h < (h->nnz_h >> 3)) {
-        int64_t want = (int64_t)((h->nnz_h + min_free) * h->cfg.pool_factor);
-        cap = want > cap ? want : cap;
+    int64_t want = h->nnz_h + min_free;
+    if (want + want / 2 > cap) {
+        int64_t grown = (int64_t)(want * h->cfg.pool_factor);
+        if (grown < 2 * cap) grown = 2 * cap;
+        cap = grown;
     }
     return compact_into(h, cap, s);
 }
@@ -727,33 +708,20 @@ dcsr_status canonicalize(dcsr_handle h, hipStream_t s)
     return st;
 }
 
-dcsr_status ensure_hub_list(dcsr_handle h, hipStream_t s)
-{
-    if (!h->hubs_dirty) return DCSR_OK;
-    const int64_t R = h->rows;
-    uint32_t cap = (uint32_t)(h->pool_cap / DCSR_HUB_LEN + 1);
-    uint32_t *rows_, *pre, *done;
-    DCSR_TRY(ws_get(h, WS_HUB_ROWS, (size_t)cap * 4, (void **)&rows_));
-    DCSR_TRY(ws_get(h, WS_HUB_PRE, (size_t)(cap + 1) * 4, (void **)&pre));
-    DCSR_TRY(ws_get(h, WS_HUB_DONE, (size_t)cap * 4, (void **)&done));
-    k_hub_reset<<<1, 1, 0, s>>>(h->ds);
-    if (R > 0) k_hub_collect<<<grid_for(R, TB), TB, 0, s>>>(R, h->row_len, rows_, cap, h->ds);
-    k_hub_prefix<<<1, 1024, 0, s>>>(h->row_len, rows_, pre, done, h->ds);
-    DCSR_HIP(hipGetLastError());
-    DCSR_TRY(pull_state(h, s));
-    h->hub_rows_h = h->hs->hub_count;
-    h->hub_chunks_h = h->hs->hub_chunks;
-    h->hubs_dirty = false;
-    return DCSR_OK;
-}
-
 dcsr_status dcsr_rebin(dcsr_handle h, dcsr_stream_t stream)
 {
     if (!h) return DCSR_ERR_INVALID;
     hipStream_t s = (hipStream_t)stream;
     h->last_stream = s;
-    h->hubs_dirty = true;
-    return ensure_hub_list(h, s);
+    h->flat_dirty = true;
+    DCSR_TRY(ensure_flat(h, s));
+    k_zero_hubs<<<1, 1, 0, s>>>(h->ds);
+    if (h->rows > 0) k_count_hubs<<<grid_for(h->rows, TB), TB, 0, s>>>(h->rows, h->row_len, h->ds);
+    DCSR_HIP(hipGetLastError());
+    DCSR_TRY(pull_state(h, s));
+    h->hub_rows_h = h->hs->hub_count;
+    h->hubs_dirty = false;
+    return DCSR_OK;
 }
 
 dcsr_status dcsr_query(dcsr_handle h, dcsr_info *info)

@@ -18,37 +18,50 @@ __device__ __forceinline__ int find_seg64(const uint32_t *s, uint32_t e)
     return lo;
 }
 
-// Largest j in [0, n) with off[j] <= e (global array, off[0] == 0 <= e).
-__device__ __forceinline__ uint32_t find_seg_global(const uint32_t *off, uint32_t n, uint32_t e)
+// Largest j in [0, n) with off[j*S] <= e (global array, non-decreasing,
+// off[0] <= e).  64-ary search by the whole wave: every round loads 64
+// probes at once and a ballot picks the sub-range, so the number of dependent
+// memory round trips is ceil(log64(n)) instead of log2(n).  All lanes must
+// call it; every lane gets the result.
+template <int S = 1>
+__device__ __forceinline__ uint32_t wave_find_last_le(const uint32_t *off, uint32_t n, uint32_t e)
 {
-    uint32_t lo = 0, hi = n;
-    while (hi - lo > 1) {
-        uint32_t mid = lo + ((hi - lo) >> 1);
-        if (off[mid] <= e) lo = mid; else hi = mid;
+    const int lane = lane_id();
+    uint32_t lo = 0, hi = n;  // answer in [lo, hi)
+    while (hi - lo > 64) {
+        const uint32_t step = (hi - lo + 63) / 64;
+        const uint32_t idx = lo + (uint32_t)lane * step;
+        const bool ok = idx < hi && off[(uint64_t)idx * S] <= e;
+        const uint64_t m = __ballot(ok);  // prefix of lanes (monotone), lane 0 always set
+        const uint32_t k = 63u - (uint32_t)__clzll((long long)m);
+        lo = lo + k * step;
+        const uint32_t nh = lo + step;
+        hi = nh < hi ? nh : hi;
     }
-    return lo;
+    const uint32_t idx = lo + (uint32_t)lane;
+    const bool ok = idx < hi && off[(uint64_t)idx * S] <= e;
+    const uint64_t m = __ballot(ok);
+    return lo + (63u - (uint32_t)__clzll((long long)m));
 }
 
 // Visit entries [e0, e1) of the flat space given by the exclusive offsets
-// off[0..nseg] (off[nseg] = total).  For every entry, calls
+// off[0..nseg] (element j at off[j*S]; off[nseg*S] = total).  For every entry, calls
 //   f(seg, k, e)   with k = e - off[seg]   (all lanes of the wave call
 // into the loop together; lanes without an entry get valid == false).
 // s_off: 65 words of LDS owned by this wave.
-template <class F>
+template <int S = 1, class F>
 __device__ __forceinline__ void wave_visit(const uint32_t *__restrict__ off, uint32_t nseg, uint32_t e0, uint32_t e1,
                                            uint32_t *s_off, F &&f)
 {
     const int lane = lane_id();
     if (e0 >= e1) return;
-    uint32_t j = 0;
-    if (lane == 0) j = find_seg_global(off, nseg, e0);
-    j = wave_read_lane_u32(j, 0);
+    uint32_t j = wave_find_last_le<S>(off, nseg, e0);
     uint32_t e = e0;
     while (e < e1) {
         // window: segments j .. j+63, boundaries off[j .. j+64]
         uint32_t idx = j + lane;
-        s_off[lane] = idx <= nseg ? off[idx] : 0xffffffffu;
-        if (lane == 0) s_off[64] = (j + 64 <= nseg) ? off[j + 64] : 0xffffffffu;
+        s_off[lane] = idx <= nseg ? off[(uint64_t)idx * S] : 0xffffffffu;
+        if (lane == 0) s_off[64] = (j + 64 <= nseg) ? off[(uint64_t)(j + 64) * S] : 0xffffffffu;
         __builtin_amdgcn_wave_barrier();
         uint32_t wend = s_off[64];
         if (wend > e1) wend = e1;

@@ -13,11 +13,12 @@
 //   3. probe every distinct coordinate in its (column-sorted) row: existing
 //      -> new = old + sum, removed when == 0; missing -> inserted when != 0;
 //   4. plan per touched row: rows whose structure is unchanged get their
-//      values written in place; the others claim a new chunk from the pool's
-//      bump region (one wave-aggregated atomicAdd per 64 rows: DPP prefix sum
-//      of the chunk sizes, lane 63 claims the total) and retire the old one;
-//   5. a one-thread check rejects the whole batch (no-op) on a bad index or
-//      pool exhaustion, so a batch is either fully applied or not at all;
+//      values written in place; the others claim a new chunk of the pool's
+//      bump region.  Claims are slot offsets from a device-wide prefix sum
+//      (DPP wave scans), so no two rows contend on an atomic;
+//   5. a one-block check reduces the per-block counters, rejects the whole
+//      batch (no-op) on a bad index or pool exhaustion, and advances the bump
+//      pointer: a batch is either fully applied or not at all;
 //   6. move surviving old entries and write inserted ones at their merged
 //      positions, then commit the row table.
 #include "dcsr_internal.h"
@@ -29,6 +30,7 @@ namespace {
 constexpr int TB = 256;
 enum { C_NOP = 0, C_ADD = 1, C_DEL = 2, C_INS = 3 };
 enum { M_VALUES = 0, M_REWRITE = 1 };
+constexpr int NPLAN = 5;  // per-block plan counters
 
 __global__ void k_upd_begin(DevState *ds)
 {
@@ -38,18 +40,6 @@ __global__ void k_upd_begin(DevState *ds)
     ds->d_hubs = 0;
     ds->d_hub_chunks = 0;
     ds->ubad = 0;
-    ds->go = 0;
-    ds->need = 0;
-    ds->hub_touch = 0;
-    ds->bump_saved = ds->bump;
-}
-
-__global__ void k_upd_replan_reset(DevState *ds)
-{
-    ds->d_nnz = 0;
-    ds->d_garbage = 0;
-    ds->d_hubs = 0;
-    ds->d_hub_chunks = 0;
     ds->go = 0;
     ds->need = 0;
     ds->hub_touch = 0;
@@ -112,8 +102,27 @@ __global__ void k_upd_compact(const uint64_t *__restrict__ key, const uint32_t *
     }
 }
 
-// Probe each distinct coordinate in its row.
-__global__ __launch_bounds__(TB) void k_upd_probe(int64_t n, const DevState *__restrict__ ds_ro,
+// Sum of one 32-bit counter over the 256 threads of a block (fixed order).
+__device__ __forceinline__ uint32_t block_sum_u32(uint32_t v, uint32_t *lds4)
+{
+    uint32_t w = wave_sum_u32(v);
+    if (lane_id() == 0) lds4[threadIdx.x >> 6] = w;
+    __syncthreads();
+    uint32_t t = lds4[0] + lds4[1] + lds4[2] + lds4[3];
+    __syncthreads();
+    return t;
+}
+
+__device__ __forceinline__ long long wave_sum_i64(long long v)
+{
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o);
+    return v;
+}
+
+// Probe each distinct coordinate in its row.  Per-block counters go to
+// pprobe[block][0..2] (reduced by the check kernel).
+__global__ __launch_bounds__(TB) void k_upd_probe(int64_t n, const DevState *__restrict__ ds,
                                                   const uint32_t *__restrict__ row_start,
                                                   const int32_t *__restrict__ row_len,
                                                   const int32_t *__restrict__ pcol, const float *__restrict__ pval,
@@ -122,27 +131,28 @@ __global__ __launch_bounds__(TB) void k_upd_probe(int64_t n, const DevState *__r
                                                   uint32_t *__restrict__ tubeg, float *__restrict__ uval,
                                                   uint32_t *__restrict__ upos, uint32_t *__restrict__ ucode,
                                                   uint32_t *__restrict__ uflag, uint32_t *__restrict__ tins,
-                                                  uint32_t *__restrict__ tdel, DevState *ds)
+                                                  uint32_t *__restrict__ tdel, uint32_t *__restrict__ pprobe)
 {
+    __shared__ uint32_t lds4[4];
     int64_t u = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const uint32_t nu = ds_ro->nuniq;
+    const uint32_t nu = ds->nuniq;
     uint32_t code = C_NOP;
     if (u < nu) {
         uint32_t t = ut[u];
         uint32_t r = trow[t];
-        uint32_t st = row_start[r];
+        uint64_t st = row_start[r];
         int32_t L = row_len[r];
         int32_t c = ucol[u];
         int32_t lo = 0, hi = L;  // lower bound of c in the row
         while (lo < hi) {
             int32_t mid = (lo + hi) >> 1;
-            if (pcol[(uint64_t)st + mid] < c) lo = mid + 1; else hi = mid;
+            if (pcol[st + mid] < c) lo = mid + 1; else hi = mid;
         }
-        bool hit = lo < L && pcol[(uint64_t)st + lo] == c;
+        bool hit = lo < L && pcol[st + lo] == c;
         float s = usum[u];
         float nv;
         if (hit) {
-            nv = pval[(uint64_t)st + lo] + s;
+            nv = pval[st + lo] + s;
             code = (nv != 0.0f) ? C_ADD : C_DEL;
         } else {
             nv = s;
@@ -153,35 +163,37 @@ __global__ __launch_bounds__(TB) void k_upd_probe(int64_t n, const DevState *__r
         ucode[u] = code;
         if (code == C_INS) atomicAdd(&tins[t], 1u);
         if (code == C_DEL) atomicAdd(&tdel[t], 1u);
-        if (u == 0) tubeg[ds_ro->ntouched] = nu;
+        if (u == 0) tubeg[ds->ntouched] = nu;
     }
     if (u <= n) {  // index n is the scan's end sentinel
         uflag[2 * u] = code == C_INS ? 1u : 0u;
         uflag[2 * u + 1] = code == C_DEL ? 1u : 0u;
     }
-    uint32_t ni = wave_sum_u32(code == C_INS), na = wave_sum_u32(code == C_ADD), nd = wave_sum_u32(code == C_DEL);
-    if (lane_id() == 0) {
-        if (ni) atomicAdd((unsigned long long *)&ds->n_ins, (unsigned long long)ni);
-        if (na) atomicAdd((unsigned long long *)&ds->n_add, (unsigned long long)na);
-        if (nd) atomicAdd((unsigned long long *)&ds->n_del, (unsigned long long)nd);
+    uint32_t ni = block_sum_u32(code == C_INS, lds4);
+    uint32_t na = block_sum_u32(code == C_ADD, lds4);
+    uint32_t nd = block_sum_u32(code == C_DEL, lds4);
+    if (threadIdx.x == 0) {
+        pprobe[blockIdx.x * 4 + 0] = ni;
+        pprobe[blockIdx.x * 4 + 1] = na;
+        pprobe[blockIdx.x * 4 + 2] = nd;
     }
 }
 
-// Per touched row: new length, mode, chunk claim.
-__global__ __launch_bounds__(TB) void k_upd_plan(int64_t n, const uint32_t *__restrict__ trow,
-                                                 const uint32_t *__restrict__ tins, const uint32_t *__restrict__ tdel,
-                                                 const uint32_t *__restrict__ row_start,
-                                                 const int32_t *__restrict__ row_len,
-                                                 const int32_t *__restrict__ row_cap, uint32_t *__restrict__ tolds,
-                                                 uint32_t *__restrict__ toldl, uint32_t *__restrict__ tdest,
-                                                 uint32_t *__restrict__ tnewl, uint32_t *__restrict__ tmode,
-                                                 uint32_t *__restrict__ tmoved, DevState *ds)
+// Per touched row: new length, mode, chunk claim size.  tpair[2t] = claim,
+// tpair[2t+1] = old entries to move; a 2-word scan turns them into offsets.
+__global__ __launch_bounds__(TB) void k_upd_plan(int64_t n, const DevState *__restrict__ ds,
+                                                 const uint32_t *__restrict__ trow, const uint32_t *__restrict__ tins,
+                                                 const uint32_t *__restrict__ tdel, const int32_t *__restrict__ row_len,
+                                                 const int32_t *__restrict__ row_cap, uint32_t *__restrict__ tnewl,
+                                                 uint32_t *__restrict__ tmode, uint32_t *__restrict__ tpair,
+                                                 long long *__restrict__ pplan)
 {
+    __shared__ long long lds[4][NPLAN];
     int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const uint32_t nt = ds->ntouched;
-    uint32_t claim = 0, mode = M_VALUES, L = 0, nl = 0, cap = 0, r = 0;
+    uint32_t claim = 0, mode = M_VALUES, L = 0, nl = 0, cap = 0;
     if (t < nt) {
-        r = trow[t];
+        uint32_t r = trow[t];
         L = (uint32_t)row_len[r];
         cap = (uint32_t)row_cap[r];
         uint32_t ni = tins[t], nd = tdel[t];
@@ -190,110 +202,185 @@ __global__ __launch_bounds__(TB) void k_upd_plan(int64_t n, const uint32_t *__re
             mode = M_REWRITE;
             claim = chunk_round(nl);
         }
-    }
-    // wave-aggregated claim of the bump region
-    uint32_t inc = wave_incl_scan_u32(claim);
-    uint32_t wtot = wave_read_lane_u32(inc, 63);
-    unsigned long long base = 0;
-    if (lane_id() == 63 && wtot) base = atomicAdd(&ds->bump, (unsigned long long)wtot);
-    base = __shfl(base, 63);
-    if (t < nt) {
-        tolds[t] = row_start[r];
-        toldl[t] = L;
-        tdest[t] = (uint32_t)(base + inc - claim);
         tnewl[t] = nl;
         tmode[t] = mode;
     }
-    if (t <= n) tmoved[t] = (t < nt && mode == M_REWRITE) ? L : 0u;  // index n: end sentinel
-    long long dn = (long long)nl - (long long)L;
-    long long dg = mode == M_REWRITE ? (long long)cap : 0;
-    int hub_now = nl > DCSR_HUB_LEN, hub_was = L > DCSR_HUB_LEN;
-    int dh = mode == M_REWRITE ? hub_now - hub_was : 0;
-    int touch = mode == M_REWRITE && (hub_now | hub_was);
-    long long dc = 0;
-    if (mode == M_REWRITE) {
-        long long cn = hub_now ? (nl + DCSR_HUB_CHUNK - 1) / DCSR_HUB_CHUNK : 0;
-        long long cw = hub_was ? (L + DCSR_HUB_CHUNK - 1) / DCSR_HUB_CHUNK : 0;
-        dc = cn - cw;
+    if (t <= n) {  // index n is the scan's end sentinel
+        tpair[2 * t] = claim;
+        tpair[2 * t + 1] = mode == M_REWRITE ? L : 0u;
     }
-    // per-wave sums of the deltas (two's complement, |sum| < 2^31 per wave)
-    int sdn = (int)wave_sum_u32((uint32_t)(int)dn);
-    int sdg = (int)wave_sum_u32((uint32_t)dg);
-    int sdh = (int)wave_sum_u32((uint32_t)dh);
-    int sdc = (int)wave_sum_u32((uint32_t)dc);
-    uint32_t stt = wave_sum_u32((uint32_t)touch);
-    if (lane_id() == 0) {
-        if (sdn) atomicAdd((unsigned long long *)&ds->d_nnz, (unsigned long long)(long long)sdn);
-        if (sdg) atomicAdd((unsigned long long *)&ds->d_garbage, (unsigned long long)(long long)sdg);
-        if (sdh) atomicAdd((unsigned long long *)&ds->d_hubs, (unsigned long long)(long long)sdh);
-        if (sdc) atomicAdd((unsigned long long *)&ds->d_hub_chunks, (unsigned long long)(long long)sdc);
-        if (stt) atomicAdd(&ds->hub_touch, stt);
+    long long v[NPLAN] = {0, 0, 0, 0, 0};
+    if (mode == M_REWRITE) {
+        int hub_now = nl > DCSR_HUB_LEN, hub_was = L > DCSR_HUB_LEN;
+        v[0] = (long long)nl - (long long)L;
+        v[1] = (long long)cap;
+        v[2] = hub_now - hub_was;
+        v[3] = (hub_now ? ((long long)nl + DCSR_HUB_CHUNK - 1) / DCSR_HUB_CHUNK : 0) -
+               (hub_was ? ((long long)L + DCSR_HUB_CHUNK - 1) / DCSR_HUB_CHUNK : 0);
+        v[4] = hub_now | hub_was;
+    }
+#pragma unroll
+    for (int i = 0; i < NPLAN; ++i) {
+        long long w = wave_sum_i64(v[i]);
+        if (lane_id() == 0) lds[threadIdx.x >> 6][i] = w;
+    }
+    __syncthreads();
+    if (threadIdx.x < NPLAN) {
+        int i = threadIdx.x;
+        pplan[(int64_t)blockIdx.x * NPLAN + i] = ((lds[0][i] + lds[1][i]) + (lds[2][i] + lds[3][i]));
     }
 }
 
-__global__ void k_upd_check(DevState *ds, unsigned long long pool_cap)
+// One workgroup: reduce the per-block counters, decide, advance the bump.
+__global__ __launch_bounds__(1024) void k_upd_check(DevState *ds, unsigned long long pool_cap,
+                                                    const uint32_t *__restrict__ pprobe, int nbprobe,
+                                                    const long long *__restrict__ pplan, int nbplan)
 {
-    unsigned long long claimed = ds->bump - ds->bump_saved;
-    ds->need = claimed;
-    if (ds->ubad) {
-        ds->go = 0;
+    __shared__ long long red[16][8];
+    long long acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    for (int b = threadIdx.x; b < nbprobe; b += 1024) {
+        acc[0] += pprobe[b * 4 + 0];
+        acc[1] += pprobe[b * 4 + 1];
+        acc[2] += pprobe[b * 4 + 2];
+    }
+    for (int b = threadIdx.x; b < nbplan; b += 1024) {
+#pragma unroll
+        for (int i = 0; i < NPLAN; ++i) acc[3 + i] += pplan[(int64_t)b * NPLAN + i];
+    }
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+        long long w = wave_sum_i64(acc[i]);
+        if (lane_id() == 0) red[threadIdx.x >> 6][i] = w;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        long long tot[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+        for (int wv = 0; wv < 16; ++wv)
+            for (int i = 0; i < 8; ++i) tot[i] += red[wv][i];
+        ds->n_ins = tot[0];
+        ds->n_add = tot[1];
+        ds->n_del = tot[2];
+        ds->d_nnz = tot[3];
+        ds->d_garbage = tot[4];
+        ds->d_hubs = tot[5];
+        ds->d_hub_chunks = tot[6];
+        ds->hub_touch = (unsigned)tot[7];
+        unsigned long long need = ds->claim_total;
+        ds->need = need;
         ds->bump = ds->bump_saved;
-        ds->status |= ST_BAD_INDEX;
-    } else if (ds->bump > pool_cap) {
-        ds->go = 0;
-        ds->bump = ds->bump_saved;
-        ds->status |= ST_POOL_EXHAUSTED;
-    } else {
-        ds->go = 1;
+        if (ds->ubad) {
+            ds->go = 0;
+            ds->status |= ST_BAD_INDEX;
+        } else if (ds->bump_saved + need > pool_cap) {
+            ds->go = 0;
+            ds->status |= ST_POOL_EXHAUSTED;
+        } else {
+            ds->go = 1;
+            ds->bump = ds->bump_saved + need;
+        }
     }
 }
 
 // Move surviving old entries of rewritten rows to their merged positions.
-__global__ __launch_bounds__(TB) void k_upd_move(const DevState *__restrict__ ds, const uint32_t *__restrict__ tmoff,
-                                                 const uint32_t *__restrict__ tolds,
-                                                 const uint32_t *__restrict__ tdest,
+// tpre[2t] = claim offset of row t (relative to bump_saved), tpre[2t+1] =
+// offset of row t's old entries in the flat move space.  Each wave owns a
+// fixed range of that space; per-row constants of a 64-row window are staged
+// in LDS and two 64-entry chunks are in flight per iteration.
+constexpr int MV_ILP = 2;
+__global__ __launch_bounds__(TB) void k_upd_move(const DevState *__restrict__ ds, const uint32_t *__restrict__ tpre,
+                                                 const uint32_t *__restrict__ trow,
+                                                 const uint32_t *__restrict__ row_start,
                                                  const uint32_t *__restrict__ tubeg, const int32_t *__restrict__ ucol,
                                                  const float *__restrict__ uval, const uint32_t *__restrict__ ucode,
                                                  const uint32_t *__restrict__ upre, int32_t *__restrict__ pcol,
                                                  float *__restrict__ pval, uint32_t per_wave)
 {
     __shared__ uint32_t s_off[TB / 64][65];
+    __shared__ uint32_t s_src[TB / 64][64];
+    __shared__ uint32_t s_ub[TB / 64][65];
+    __shared__ uint32_t s_dst[TB / 64][64];
+    __shared__ uint32_t s_ib[TB / 64][64];
+    __shared__ uint32_t s_db[TB / 64][64];
     if (!ds->go) return;
     const uint32_t total = ds->nrewrite_old;
     const uint32_t nt = ds->ntouched;
+    const uint64_t base = ds->bump_saved;
     const int w = threadIdx.x >> 6;
+    const int lane = lane_id();
     const uint64_t nwaves = (uint64_t)gridDim.x * (TB / 64);
     for (uint64_t wave = (uint64_t)blockIdx.x * (TB / 64) + w; wave * per_wave < total; wave += nwaves) {
-        uint64_t e0 = wave * per_wave;
-        uint64_t e1 = e0 + per_wave < total ? e0 + per_wave : total;
-        wave_visit(tmoff, nt, (uint32_t)e0, (uint32_t)e1, s_off[w], [&](bool valid, uint32_t t, uint32_t k, uint32_t) {
-            if (!valid) return;
-            uint64_t sp = (uint64_t)tolds[t] + k;
-            int32_t c = pcol[sp];
-            float v = pval[sp];
-            uint32_t ub = tubeg[t], ue = tubeg[t + 1];
-            uint32_t lo = ub, hi = ue;
-            while (lo < hi) {
-                uint32_t mid = (lo + hi) >> 1;
-                if (ucol[mid] < c) lo = mid + 1; else hi = mid;
+        const uint32_t e0 = (uint32_t)(wave * per_wave);
+        const uint32_t e1 = total - e0 < per_wave ? total : e0 + per_wave;
+        uint32_t j = wave_find_last_le<2>(tpre + 1, nt, e0);
+        uint32_t e = e0;
+        while (e < e1) {
+            const uint32_t t = j + lane;
+            s_off[w][lane] = t <= nt ? tpre[2 * (uint64_t)t + 1] : 0xffffffffu;
+            if (lane == 0) s_off[w][64] = (j + 64 <= nt) ? tpre[2 * (uint64_t)(j + 64) + 1] : 0xffffffffu;
+            if (t < nt) {
+                const uint32_t ub = tubeg[t];
+                s_src[w][lane] = row_start[trow[t]];
+                s_ub[w][lane] = ub;
+                s_dst[w][lane] = tpre[2 * t];
+                s_ib[w][lane] = upre[2 * ub];
+                s_db[w][lane] = upre[2 * ub + 1];
+                if (lane == 63) s_ub[w][64] = tubeg[t + 1];
+            } else if (t == nt) {
+                s_ub[w][lane] = tubeg[t];
             }
-            if (lo < ue && ucol[lo] == c) {
-                if (ucode[lo] == C_DEL) return;
-                v = uval[lo];
+            __builtin_amdgcn_wave_barrier();
+            uint32_t wend = s_off[w][64];
+            if (wend > e1) wend = e1;
+            for (uint32_t b = e; b < wend; b += 64 * MV_ILP) {
+                uint32_t jj[MV_ILP], k[MV_ILP];
+                int32_t c[MV_ILP];
+                float v[MV_ILP];
+#pragma unroll
+                for (int q = 0; q < MV_ILP; ++q) {
+                    const uint32_t ee = b + q * 64 + lane;
+                    jj[q] = 64;
+                    c[q] = 0;
+                    v[q] = 0.0f;
+                    if (ee < wend) {
+                        jj[q] = (uint32_t)find_seg64(s_off[w], ee);
+                        k[q] = ee - s_off[w][jj[q]];
+                        const uint64_t sp = (uint64_t)s_src[w][jj[q]] + k[q];
+                        c[q] = pcol[sp];
+                        v[q] = pval[sp];
+                    }
+                }
+#pragma unroll
+                for (int q = 0; q < MV_ILP; ++q) {
+                    if (jj[q] == 64) continue;
+                    const uint32_t ub = s_ub[w][jj[q]], ue = s_ub[w][jj[q] + 1];
+                    uint32_t lo = ub, hi = ue;
+                    while (lo < hi) {
+                        uint32_t mid = (lo + hi) >> 1;
+                        if (ucol[mid] < c[q]) lo = mid + 1; else hi = mid;
+                    }
+                    float vv = v[q];
+                    if (lo < ue && ucol[lo] == c[q]) {
+                        if (ucode[lo] == C_DEL) continue;
+                        vv = uval[lo];
+                    }
+                    const int64_t shift = (int64_t)(upre[2 * lo] - s_ib[w][jj[q]]) -
+                                          (int64_t)(upre[2 * lo + 1] - s_db[w][jj[q]]);
+                    const uint64_t d = base + s_dst[w][jj[q]] + (uint64_t)((int64_t)k[q] + shift);
+                    pcol[d] = c[q];
+                    pval[d] = vv;
+                }
             }
-            int64_t shift = (int64_t)(upre[2 * lo] - upre[2 * ub]) - (int64_t)(upre[2 * lo + 1] - upre[2 * ub + 1]);
-            uint64_t d = (uint64_t)((int64_t)tdest[t] + (int64_t)k + shift);
-            pcol[d] = c;
-            pval[d] = v;
-        });
+            __builtin_amdgcn_wave_barrier();
+            e = wend;
+            j += 64;
+        }
     }
 }
 
 // Inserted entries, in-place value updates, row-table commit.
 __global__ __launch_bounds__(TB) void k_upd_write(DevState *ds, const uint32_t *__restrict__ ut,
                                                   const uint32_t *__restrict__ trow, const uint32_t *__restrict__ tubeg,
-                                                  const uint32_t *__restrict__ tolds,
-                                                  const uint32_t *__restrict__ tdest,
+                                                  const uint32_t *__restrict__ tpre,
                                                   const uint32_t *__restrict__ tnewl,
                                                   const uint32_t *__restrict__ tmode, const int32_t *__restrict__ ucol,
                                                   const float *__restrict__ uval, const uint32_t *__restrict__ upos,
@@ -310,28 +397,34 @@ __global__ __launch_bounds__(TB) void k_upd_write(DevState *ds, const uint32_t *
         ds->garbage += ds->d_garbage;
     }
     if (u >= nu) return;
-    uint32_t t = ut[u];
-    uint32_t code = ucode[u];
-    uint32_t mode = tmode[t];
-    uint32_t ub = tubeg[t];
+    const uint32_t t = ut[u];
+    const uint32_t code = ucode[u];
+    const uint32_t mode = tmode[t];
+    const uint32_t r = trow[t];
     if (mode == M_VALUES) {
-        if (code == C_ADD) pval[(uint64_t)tolds[t] + upos[u]] = uval[u];
+        if (code == C_ADD) pval[(uint64_t)row_start[r] + upos[u]] = uval[u];
         return;
     }
+    const uint32_t ub = tubeg[t];
+    const uint64_t dest = ds->bump_saved + tpre[2 * t];
     if (code == C_INS) {
         int64_t shift = (int64_t)(upre[2 * u] - upre[2 * ub]) - (int64_t)(upre[2 * u + 1] - upre[2 * ub + 1]);
-        uint64_t d = (uint64_t)((int64_t)tdest[t] + (int64_t)upos[u] + shift);
+        uint64_t d = dest + (uint64_t)((int64_t)upos[u] + shift);
         pcol[d] = ucol[u];
         pval[d] = uval[u];
     }
     if (u == ub) {
-        uint32_t r = trow[t];
         uint32_t nl = tnewl[t];
-        row_start[r] = tdest[t];
+        row_start[r] = (uint32_t)dest;
         row_len[r] = (int32_t)nl;
         row_cap[r] = (int32_t)chunk_round(nl);
     }
 }
+
+__global__ void k_upd_validate_begin(DevState *ds) { ds->ubad = 0; }
+
+// After a compaction inside an update: claims restart from the new bump.
+__global__ void k_upd_rebase(DevState *ds) { ds->bump_saved = ds->bump; }
 
 struct UpdBufs {
     uint64_t *k0, *k1;
@@ -339,10 +432,11 @@ struct UpdBufs {
     int32_t *ucol;
     float *usum, *uval;
     uint32_t *ut, *upos, *ucode, *uflag, *upre;
-    uint32_t *trow, *tubeg, *tins, *tdel, *tolds, *toldl, *tdest, *tnewl, *tmode, *tmoved, *tmoff;
+    uint32_t *trow, *tubeg, *tins, *tdel, *tnewl, *tmode, *tpair, *tpre, *pprobe;
+    long long *pplan;
 };
 
-static dcsr_status upd_bufs(dcsr_handle h, int64_t n, UpdBufs &b)
+dcsr_status upd_bufs(dcsr_handle h, int64_t n, UpdBufs &b)
 {
     size_t n4 = (size_t)(n + 2) * 4, n8 = (size_t)(n + 2) * 8;
     DCSR_TRY(ws_get(h, WS_KEY_A, n8, (void **)&b.k0));
@@ -366,27 +460,27 @@ static dcsr_status upd_bufs(dcsr_handle h, int64_t n, UpdBufs &b)
     DCSR_TRY(ws_get(h, WS_TUBEG, n4 + 4, (void **)&b.tubeg));
     DCSR_TRY(ws_get(h, WS_TINS, n4, (void **)&b.tins));
     DCSR_TRY(ws_get(h, WS_TDEL, n4, (void **)&b.tdel));
-    DCSR_TRY(ws_get(h, WS_TOLDS, n4, (void **)&b.tolds));
-    DCSR_TRY(ws_get(h, WS_TOLDL, n4, (void **)&b.toldl));
-    DCSR_TRY(ws_get(h, WS_TDEST, n4, (void **)&b.tdest));
     DCSR_TRY(ws_get(h, WS_TNEWL, n4, (void **)&b.tnewl));
     DCSR_TRY(ws_get(h, WS_TMODE, n4, (void **)&b.tmode));
-    DCSR_TRY(ws_get(h, WS_TOLDPRE, n4 + 4, (void **)&b.tmoff));
-    DCSR_TRY(ws_get(h, WS_CANON, n4 + 4, (void **)&b.tmoved));
+    DCSR_TRY(ws_get(h, WS_TPAIR, n8, (void **)&b.tpair));
+    DCSR_TRY(ws_get(h, WS_TPAIRPRE, n8, (void **)&b.tpre));
     DCSR_TRY(ws_get(h, WS_TMP2, n4, (void **)&b.usum));
+    DCSR_TRY(ws_get(h, WS_PPROBE, (size_t)grid_for(n + 1, TB) * 16, (void **)&b.pprobe));
+    DCSR_TRY(ws_get(h, WS_PPLAN, (size_t)grid_for(n + 1, TB) * NPLAN * 8, (void **)&b.pplan));
     return DCSR_OK;
 }
 
-}  // namespace
-
-static dcsr_status plan_and_check(dcsr_handle h, int64_t n, UpdBufs &b, hipStream_t s)
+dcsr_status plan_and_check(dcsr_handle h, int64_t n, UpdBufs &b, hipStream_t s)
 {
-    k_upd_plan<<<grid_for(n + 1, TB), TB, 0, s>>>(n, b.trow, b.tins, b.tdel, h->row_start, h->row_len, h->row_cap, b.tolds,
-                                               b.toldl, b.tdest, b.tnewl, b.tmode, b.tmoved, h->ds);
-    DCSR_HIP(scan_exclusive_u32(b.tmoved, b.tmoff, n + 1, 1, b.part, &h->ds->nrewrite_old, s));
-    k_upd_check<<<1, 1, 0, s>>>(h->ds, (unsigned long long)h->pool_cap);
+    const unsigned nb = grid_for(n + 1, TB);
+    k_upd_plan<<<nb, TB, 0, s>>>(n, h->ds, b.trow, b.tins, b.tdel, h->row_len, h->row_cap, b.tnewl, b.tmode, b.tpair,
+                                 b.pplan);
+    DCSR_HIP(scan_exclusive_u32(b.tpair, b.tpre, n + 1, 2, b.part, &h->ds->claim_total, s));
+    k_upd_check<<<1, 1024, 0, s>>>(h->ds, (unsigned long long)h->pool_cap, b.pprobe, (int)nb, b.pplan, (int)nb);
     return hipGetLastError() == hipSuccess ? DCSR_OK : DCSR_ERR_HIP;
 }
+
+}  // namespace
 
 dcsr_status dcsr_update(dcsr_handle h, const int32_t *rows, const int32_t *cols, const float *vals, int64_t n,
                         uint32_t flags, dcsr_stream_t stream)
@@ -402,7 +496,7 @@ dcsr_status dcsr_update(dcsr_handle h, const int32_t *rows, const int32_t *cols,
             uint32_t *v0;
             DCSR_TRY(ws_get(h, WS_KEY_A, (size_t)n * 8, (void **)&k0));
             DCSR_TRY(ws_get(h, WS_VAL_A, (size_t)n * 4, (void **)&v0));
-            k_upd_begin<<<1, 1, 0, s>>>(h->ds);
+            k_upd_validate_begin<<<1, 1, 0, s>>>(h->ds);
             k_upd_pack<<<grid_for(n, TB), TB, 0, s>>>(rows, cols, vals, n, h->rows, h->cols, h->cbits, k0, v0, h->ds);
             DCSR_HIP(hipGetLastError());
             DCSR_TRY(pull_state(h, s));
@@ -429,9 +523,9 @@ dcsr_status dcsr_update(dcsr_handle h, const int32_t *rows, const int32_t *cols,
     DCSR_HIP(scan_exclusive_u32(b.flags, b.pre, n, 2, b.part, &h->ds->nuniq, s));
     k_upd_compact<<<grid_for(n, TB), TB, 0, s>>>(key, val, n, h->cbits, b.flags, b.pre, b.ucol, b.usum, b.ut, b.trow,
                                                   b.tubeg, b.tins, b.tdel);
-    k_upd_probe<<<grid_for(n + 1, TB), TB, 0, s>>>(n, h->ds, h->row_start, h->row_len, h->pool_col, h->pool_val, b.ucol,
-                                                b.usum, b.ut, b.trow, b.tubeg, b.uval, b.upos, b.ucode, b.uflag, b.tins,
-                                                b.tdel, h->ds);
+    k_upd_probe<<<grid_for(n + 1, TB), TB, 0, s>>>(n, h->ds, h->row_start, h->row_len, h->pool_col, h->pool_val,
+                                                    b.ucol, b.usum, b.ut, b.trow, b.tubeg, b.uval, b.upos, b.ucode,
+                                                    b.uflag, b.tins, b.tdel, b.pprobe);
     DCSR_HIP(scan_exclusive_u32(b.uflag, b.upre, n + 1, 2, b.part, nullptr, s));
     DCSR_TRY(plan_and_check(h, n, b, s));
 
@@ -440,15 +534,13 @@ dcsr_status dcsr_update(dcsr_handle h, const int32_t *rows, const int32_t *cols,
         DCSR_TRY(pull_state(h, s));
         if (!h->hs->go) {
             unsigned bits = h->hs->status;
-            if (h->hs->ubad) {
-                return DCSR_ERR_BAD_INDEX;
-            }
+            if (h->hs->ubad) return DCSR_ERR_BAD_INDEX;
             if ((bits & ST_POOL_EXHAUSTED) && h->cfg.allow_grow && !(flags & DCSR_UPDATE_NOGROW)) {
                 // make room, then plan again (row contents are unchanged, only
                 // their locations move)
                 int64_t need = (int64_t)h->hs->need;
                 DCSR_TRY(pool_reserve(h, need + need / 8 + 64, s, nullptr));
-                k_upd_replan_reset<<<1, 1, 0, s>>>(h->ds);
+                k_upd_rebase<<<1, 1, 0, s>>>(h->ds);
                 DCSR_TRY(plan_and_check(h, n, b, s));
                 DCSR_TRY(pull_state(h, s));
                 if (!h->hs->go) return DCSR_ERR_POOL_EXHAUSTED;
@@ -459,19 +551,20 @@ dcsr_status dcsr_update(dcsr_handle h, const int32_t *rows, const int32_t *cols,
     }
     // persistent move kernel: any grid size is correct
     {
-        const uint32_t per_wave = 1024;
+        const uint32_t per_wave = 256;
         int64_t bound = nosync ? h->pool_cap : (int64_t)h->hs->nrewrite_old;
         int64_t waves = (bound + per_wave - 1) / per_wave;
         int64_t blocks = (waves + 3) / 4;
-        if (blocks > 4096) blocks = 4096;
+        if (blocks > 65536) blocks = 65536;
         if (blocks < 1) blocks = 1;
-        k_upd_move<<<(unsigned)blocks, TB, 0, s>>>(h->ds, b.tmoff, b.tolds, b.tdest, b.tubeg, b.ucol, b.uval, b.ucode,
-                                                   b.upre, h->pool_col, h->pool_val, per_wave);
+        k_upd_move<<<(unsigned)blocks, TB, 0, s>>>(h->ds, b.tpre, b.trow, h->row_start, b.tubeg, b.ucol, b.uval,
+                                                   b.ucode, b.upre, h->pool_col, h->pool_val, per_wave);
     }
-    k_upd_write<<<grid_for(n, TB), TB, 0, s>>>(h->ds, b.ut, b.trow, b.tubeg, b.tolds, b.tdest, b.tnewl, b.tmode, b.ucol,
-                                                b.uval, b.upos, b.ucode, b.upre, h->pool_col, h->pool_val, h->row_start,
+    k_upd_write<<<grid_for(n, TB), TB, 0, s>>>(h->ds, b.ut, b.trow, b.tubeg, b.tpre, b.tnewl, b.tmode, b.ucol, b.uval,
+                                                b.upos, b.ucode, b.upre, h->pool_col, h->pool_val, h->row_start,
                                                 h->row_len, h->row_cap);
     DCSR_HIP(hipGetLastError());
+    h->flat_dirty = true;
     if (nosync) {
         h->state_known = false;
         h->hubs_dirty = true;

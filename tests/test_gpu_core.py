@@ -220,3 +220,42 @@ def test_csr_load_and_initialize(torch):
     assert_same_structure(to_host_csr(m2), O.update(E, r, c, v))
     m.close()
     m2.close()
+
+
+def test_spmv_empty_rows_and_long_rows(torch):
+    """Flat SpMV edge cases: empty matrix, leading/trailing empty rows, a row
+    spanning many waves (carry chain), beta applied to empty rows."""
+    from dcsr_amd import DcsrMatrix
+    rng = np.random.default_rng(21)
+    say("empty matrix spmv")
+    R, C = 5000, 3000
+    m = DcsrMatrix(R, C)
+    x = rng.uniform(-1, 1, C).astype(np.float32)
+    y0 = rng.uniform(-1, 1, R).astype(np.float32)
+    y = torch.from_numpy(y0.copy()).cuda()
+    m.spmv(torch.from_numpy(x).cuda(), y, alpha=1.0, beta=0.5)
+    torch.cuda.synchronize()
+    np.testing.assert_array_equal(y.cpu().numpy(), (0.5 * y0).astype(np.float32))
+    say("sparse rows + one 50k-entry row")
+    rows = [np.full(2900, 4000), rng.integers(100, 200, 500), np.full(2500, 4001)]
+    cols = [rng.choice(C, 2900, replace=False), rng.integers(0, C, 500), rng.choice(C, 2500, replace=False)]
+    r = np.concatenate(rows).astype(np.int32)
+    c = np.concatenate(cols).astype(np.int32)
+    v = rng.uniform(-1, 1, r.size).astype(np.float32)
+    # a very long row: 50k entries with repeated columns (duplicates kept)
+    r = np.concatenate([r, np.full(50000, 2500, np.int32)])
+    c = np.concatenate([c, rng.integers(0, C, 50000).astype(np.int32)])
+    v = np.concatenate([v, rng.uniform(-1, 1, 50000).astype(np.float32)])
+    m.LoadMatrix(r, c, v)
+    A = O.coo_to_csr(R, C, r, c, v)
+    yv = spmv_host(m, x)
+    fp64_spmv_check(A, x, yv)
+    assert (yv[np.diff(A.off) == 0] == 0).all()
+    y = torch.from_numpy(y0.copy()).cuda()
+    m.spmv(torch.from_numpy(x).cuda(), y, alpha=-1.5, beta=2.0)
+    torch.cuda.synchronize()
+    want = -1.5 * (A.to_scipy() @ x.astype(np.float64)) + 2.0 * y0
+    scale = 1.5 * (abs(A.to_scipy()) @ np.abs(x.astype(np.float64))) + 2.0 * np.abs(y0)
+    assert (np.abs(y.cpu().numpy() - want) <= 1e-5 * scale + 1e-30).all()
+    np.testing.assert_array_equal(spmv_host(m, x), yv)  # reproducible
+    m.close()
