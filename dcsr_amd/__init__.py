@@ -89,6 +89,7 @@ def lib():
         "dcsr_compact": (ctypes.c_int, [_H, _P]),
         "dcsr_rebin": (ctypes.c_int, [_H, _P]),
         "dcsr_spmv": (ctypes.c_int, [_H, _P, _P, _F, _F, _P]),
+        "dcsr_spmm_dense": (ctypes.c_int, [_H, _P, _I64, _I64, _P, _I64, _F, _F, _P]),
         "dcsr_query": (ctypes.c_int, [_H, ctypes.POINTER(DcsrInfo)]),
         "dcsr_csr_spmv": (ctypes.c_int, [_I64, _P, _P, _P, _P, _P, _P]),
     }
@@ -230,6 +231,23 @@ class DcsrMatrix:
             raise ValueError("y must be a contiguous float32 CUDA tensor with num_rows entries")
         check(lib().dcsr_spmv(self._h, _ptr(x), _ptr(y), float(alpha), float(beta), _stream()), "dcsr_spmv")
         return y
+
+    def spmm(self, X, Y=None, alpha=1.0, beta=0.0):
+        """Y = alpha * A X + beta * Y for a dense row-major X (num_cols x k)."""
+        torch = _torch()
+        X = X if isinstance(X, torch.Tensor) else torch.from_numpy(X)
+        X = X.to(device="cuda", dtype=torch.float32)
+        if X.dim() != 2 or X.shape[0] != self.num_cols or X.stride(1) != 1:
+            raise ValueError("X must be a (num_cols, k) row-major float32 tensor")
+        k = X.shape[1]
+        if Y is None:
+            Y = torch.empty((self.num_rows, k), dtype=torch.float32, device="cuda")
+            beta = 0.0
+        elif Y.shape != (self.num_rows, k) or Y.dtype != torch.float32 or Y.stride(1) != 1 or not Y.is_cuda:
+            raise ValueError("Y must be a (num_rows, k) row-major float32 CUDA tensor")
+        check(lib().dcsr_spmm_dense(self._h, _ptr(X), k, X.stride(0), _ptr(Y), Y.stride(0), float(alpha),
+                                    float(beta), _stream()), "dcsr_spmm_dense")
+        return Y
 
     def info(self):
         inf = DcsrInfo()

@@ -50,7 +50,7 @@ enum WsSlot {
     WS_UCOL, WS_UVAL, WS_UT, WS_UPOS, WS_UCODE, WS_UPRE,
     WS_TROW, WS_TUBEG, WS_TINS, WS_TDEL, WS_TOLDS, WS_TOLDL, WS_TDEST, WS_TNEWL, WS_TMODE, WS_TOLDPRE,
     WS_HUB_ROWS, WS_HUB_PRE, WS_HUB_PART, WS_HUB_DONE,
-    WS_PPROBE, WS_PPLAN, WS_TPAIR, WS_TPAIRPRE, WS_FOFF, WS_CARRY_ROW, WS_CARRY_VAL,
+    WS_PPROBE, WS_PPLAN, WS_TPAIR, WS_TPAIRPRE, WS_FOFF, WS_ROWMAP, WS_CARRY_ROW, WS_CARRY_VAL, WS_SPMM_CROW, WS_SPMM_CVAL,
     WS_CANON, WS_TMP0, WS_TMP1, WS_TMP2,
     WS_COUNT
 };

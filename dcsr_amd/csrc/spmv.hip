@@ -26,22 +26,45 @@ namespace {
 constexpr int TB = 256;
 constexpr int ILP = 4;
 constexpr uint32_t NONE = 0xffffffffu;
+constexpr uint32_t GRAN = 64;  // row-sample granule (nonzeros)
 
-__device__ __forceinline__ float combine(float alpha, float acc, float beta, const float *y, uint32_t r)
+// rowmap[i] = first row not entirely before flat position i*GRAN, i.e. the
+// smallest r with foff[r+1] >= i*GRAN.  Lets any GRAN-aligned range find its
+// first row with one load.
+__global__ void k_rowmap(uint32_t R, const uint32_t *__restrict__ foff, uint32_t *__restrict__ rowmap, uint32_t cap)
 {
-    return beta == 0.0f ? alpha * acc : alpha * acc + beta * y[r];
+    const uint32_t total = foff[R];
+    const uint32_t ng = total / GRAN + 1;
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= ng || i >= cap) return;
+    const uint32_t e = i * GRAN;
+    uint32_t lo = 0, hi = R;  // smallest r in [0, R) with foff[r+1] >= e
+    while (lo < hi) {
+        uint32_t mid = (lo + hi) >> 1;
+        if (foff[mid + 1] >= e) hi = mid; else lo = mid + 1;
+    }
+    rowmap[i] = lo;
 }
 
-__global__ __launch_bounds__(TB) void k_spmv_flat(uint32_t R, const uint32_t *__restrict__ foff,
-                                                  const uint32_t *__restrict__ row_start,
-                                                  const int32_t *__restrict__ col, const float *__restrict__ val,
-                                                  const float *__restrict__ x, float *__restrict__ y, float alpha,
-                                                  float beta, uint32_t per_wave, uint32_t *__restrict__ carry_row,
-                                                  float *__restrict__ carry_val)
+__device__ __forceinline__ float combine(float alpha, float acc, float beta, const float *y, uint64_t i)
+{
+    return beta == 0.0f ? alpha * acc : alpha * acc + beta * y[i];
+}
+
+// Flat kernel for a column block of width KB (1: SpMV, 2 or 4: narrow SpMM).
+// X is row-major with leading dimension ldx, Y with ldy (SpMV: both 1).
+template <int KB>
+__global__ __launch_bounds__(TB) void k_flat(uint32_t R, const uint32_t *__restrict__ foff,
+                                             const uint32_t *__restrict__ rowmap,
+                                             const uint32_t *__restrict__ row_start, const int32_t *__restrict__ col,
+                                             const float *__restrict__ val, const float *__restrict__ X, int64_t ldx,
+                                             float *__restrict__ Y, int64_t ldy, float alpha, float beta,
+                                             uint32_t per_wave, uint32_t *__restrict__ carry_row,
+                                             float *__restrict__ carry_val)
 {
     __shared__ uint32_t s_off[TB / 64][65];
     __shared__ uint32_t s_st[TB / 64][64];
-    __shared__ float s_acc[TB / 64][64];
+    __shared__ float s_acc[TB / 64][KB][64];
     const int w = threadIdx.x >> 6;
     const int lane = lane_id();
     const uint32_t wave = blockIdx.x * (TB / 64) + w;
@@ -53,31 +76,31 @@ __global__ __launch_bounds__(TB) void k_spmv_flat(uint32_t R, const uint32_t *__
     const uint32_t e1 = (total - e0) < per_wave ? total : e0 + per_wave;
     const bool last = wave == lastw;
 
-    // first row not entirely before e0 (smallest r with foff[r+1] >= e0) =
-    // largest i in [0, R] with foff[i] < e0, or 0 when e0 == 0
-    uint32_t j = e0 == 0 ? 0u : wave_find_last_le(foff, R + 1, e0 - 1);
+    // first row not entirely before e0 (per_wave is a multiple of GRAN)
+    uint32_t j = rowmap[e0 / GRAN];
     while (j < R) {
         const uint32_t r = j + lane;
         s_off[w][lane] = r <= R ? foff[r] : total;
         if (lane == 0) s_off[w][64] = (j + 64 <= R) ? foff[j + 64] : total;
         s_st[w][lane] = r < R ? row_start[r] : 0u;
-        s_acc[w][lane] = 0.0f;
+#pragma unroll
+        for (int q = 0; q < KB; ++q) s_acc[w][q][lane] = 0.0f;
         __builtin_amdgcn_wave_barrier();
         const uint32_t wb = s_off[w][0] > e0 ? s_off[w][0] : e0;
         const uint32_t wend = s_off[w][64] < e1 ? s_off[w][64] : e1;
         for (uint32_t b = wb; b < wend; b += 64 * ILP) {
-            uint32_t jj[ILP], e[ILP];
+            uint32_t jj[ILP];
             uint64_t pos[ILP];
             int32_t c[ILP];
-            float v[ILP], xv[ILP];
+            float v[ILP], xv[ILP][KB];
 #pragma unroll
             for (int u = 0; u < ILP; ++u) {
-                e[u] = b + u * 64 + lane;
+                const uint32_t e = b + u * 64 + lane;
                 jj[u] = 64;
                 pos[u] = 0;
-                if (e[u] < wend) {
-                    jj[u] = (uint32_t)find_seg64(s_off[w], e[u]);
-                    pos[u] = (uint64_t)s_st[w][jj[u]] + (e[u] - s_off[w][jj[u]]);
+                if (e < wend) {
+                    jj[u] = (uint32_t)find_seg64(s_off[w], e);
+                    pos[u] = (uint64_t)s_st[w][jj[u]] + (e - s_off[w][jj[u]]);
                 }
             }
 #pragma unroll
@@ -90,28 +113,41 @@ __global__ __launch_bounds__(TB) void k_spmv_flat(uint32_t R, const uint32_t *__
                 }
             }
 #pragma unroll
-            for (int u = 0; u < ILP; ++u) xv[u] = jj[u] < 64 ? x[c[u]] : 0.0f;
+            for (int u = 0; u < ILP; ++u)
+#pragma unroll
+                for (int q = 0; q < KB; ++q) xv[u][q] = jj[u] < 64 ? X[(int64_t)c[u] * ldx + q] : 0.0f;
 #pragma unroll
             for (int u = 0; u < ILP; ++u) {
-                float p = v[u] * xv[u];
-                p = wave_seg_incl_scan_f32(p, jj[u]);
-                uint32_t jn = __shfl_down(jj[u], 1);
-                if (jj[u] < 64 && (lane == 63 || jn != jj[u])) s_acc[w][jj[u]] += p;
+                const uint32_t jn = __shfl_down(jj[u], 1);
+                const bool tail = jj[u] < 64 && (lane == 63 || jn != jj[u]);
+#pragma unroll
+                for (int q = 0; q < KB; ++q) {
+                    float p = v[u] * xv[u][q];
+                    p = wave_seg_incl_scan_f32(p, jj[u]);
+                    if (tail) s_acc[w][q][jj[u]] += p;
+                }
                 __builtin_amdgcn_wave_barrier();
             }
         }
         // flush the window's rows
         if (r < R) {
             const uint32_t rs = s_off[w][lane], re = s_off[w][lane + 1];
-            const float a = s_acc[w][lane];
             if (re > rs) {
-                if (rs >= e0 && rs < e1) y[r] = combine(alpha, a, beta, y, r);
-                else if (rs < e0 && re > e0) {
+                if (rs >= e0 && rs < e1) {
+#pragma unroll
+                    for (int q = 0; q < KB; ++q)
+                        Y[(uint64_t)r * ldy + q] = combine(alpha, s_acc[w][q][lane], beta, Y, (uint64_t)r * ldy + q);
+                } else if (rs < e0 && re > e0) {
                     carry_row[wave] = r;
-                    carry_val[wave] = a;
+#pragma unroll
+                    for (int q = 0; q < KB; ++q) carry_val[(uint64_t)wave * KB + q] = s_acc[w][q][lane];
                 }
             } else if ((rs >= e0 && rs < e1) || (last && rs == total)) {
-                y[r] = beta == 0.0f ? 0.0f : beta * y[r];
+#pragma unroll
+                for (int q = 0; q < KB; ++q) {
+                    const uint64_t i = (uint64_t)r * ldy + q;
+                    Y[i] = beta == 0.0f ? 0.0f : beta * Y[i];
+                }
             }
         }
         const uint32_t next_start = s_off[w][64];
@@ -121,25 +157,101 @@ __global__ __launch_bounds__(TB) void k_spmv_flat(uint32_t R, const uint32_t *__
     }
 }
 
-// Each carry chain (consecutive waves carrying into the same row) is summed
+// Wide column blocks (KB = 8, 16, 32): a wave holds 64/KB lane groups; group
+// g owns GRAN consecutive nonzeros of the flat order and walks them row by
+// row, each of its KB lanes accumulating one column of Y (the (col, val)
+// loads are broadcasts, the X-row gathers are contiguous KB*4 bytes), so no
+// cross-lane reduction is needed.  Rows that started in an earlier group are
+// left as carries, like the flat kernel.
+template <int KB>
+__global__ __launch_bounds__(TB) void k_spmm_groups(uint32_t R, const uint32_t *__restrict__ foff,
+                                                    const uint32_t *__restrict__ rowmap,
+                                                    const uint32_t *__restrict__ row_start,
+                                                    const int32_t *__restrict__ col, const float *__restrict__ val,
+                                                    const float *__restrict__ X, int64_t ldx, float *__restrict__ Y,
+                                                    int64_t ldy, float alpha, float beta,
+                                                    uint32_t *__restrict__ carry_row, float *__restrict__ carry_val)
+{
+    constexpr int G = 64 / KB;
+    constexpr int UN = 4;
+    const int lane = lane_id();
+    const int cl = lane % KB;
+    const uint32_t gid = (blockIdx.x * (TB / 64) + (threadIdx.x >> 6)) * G + lane / KB;
+    const uint32_t total = foff[R];
+    const uint32_t lastg = total ? (total - 1) / GRAN : 0;
+    if (cl == 0) carry_row[gid] = NONE;
+    if (gid > lastg) return;
+    const uint32_t g0 = gid * GRAN;
+    const uint32_t g1 = (total - g0) < GRAN ? total : g0 + GRAN;
+    const bool last = gid == lastg;
+    for (uint32_t r = rowmap[gid]; r < R; ++r) {
+        const uint32_t rs = foff[r], re = foff[r + 1];
+        if (rs >= g1 && !(last && rs == total)) break;
+        const uint64_t yi = (uint64_t)r * ldy + cl;
+        if (re > rs) {
+            const uint32_t a = rs > g0 ? rs : g0;
+            const uint32_t b = re < g1 ? re : g1;
+            if (a >= b) continue;
+            const uint64_t st = (uint64_t)row_start[r] - rs;
+            float acc = 0.0f;
+            uint32_t e = a;
+            for (; e + UN <= b; e += UN) {
+                int32_t c[UN];
+                float v[UN], xv[UN];
+#pragma unroll
+                for (int u = 0; u < UN; ++u) {
+                    c[u] = col[st + e + u];
+                    v[u] = val[st + e + u];
+                }
+#pragma unroll
+                for (int u = 0; u < UN; ++u) xv[u] = X[(int64_t)c[u] * ldx + cl];
+#pragma unroll
+                for (int u = 0; u < UN; ++u) acc = acc + v[u] * xv[u];
+            }
+            for (; e < b; ++e) acc = acc + val[st + e] * X[(int64_t)col[st + e] * ldx + cl];
+            if (rs >= g0) {
+                Y[yi] = combine(alpha, acc, beta, Y, yi);
+            } else {
+                if (cl == 0) carry_row[gid] = r;
+                carry_val[(uint64_t)gid * KB + cl] = acc;
+            }
+        } else if ((rs >= g0 && rs < g1) || (last && rs == total)) {
+            Y[yi] = beta == 0.0f ? 0.0f : beta * Y[yi];
+        }
+    }
+}
+
+// Each carry chain (consecutive units carrying into the same row) is summed
 // by one wave in a fixed order and added to the row's head value.
-__global__ __launch_bounds__(TB) void k_spmv_carry(uint32_t nwaves, const uint32_t *__restrict__ carry_row,
-                                                   const float *__restrict__ carry_val, float *__restrict__ y,
-                                                   float alpha)
+template <int KB>
+__global__ __launch_bounds__(TB) void k_carry(uint32_t nunits, const uint32_t *__restrict__ carry_row,
+                                              const float *__restrict__ carry_val, float *__restrict__ Y,
+                                              int64_t ldy, float alpha)
 {
     const uint32_t w = blockIdx.x * (TB / 64) + (threadIdx.x >> 6);
-    if (w >= nwaves) return;
+    if (w >= nunits) return;
     const uint32_t r = carry_row[w];
     if (r == NONE || (w > 0 && carry_row[w - 1] == r)) return;
     const int lane = lane_id();
-    float s = 0.0f;
+    float s[KB];
+#pragma unroll
+    for (int q = 0; q < KB; ++q) s[q] = 0.0f;
     for (uint32_t k = w + lane;; k += 64) {
-        bool in = k < nwaves && carry_row[k] == r;
-        if (in) s = s + carry_val[k];
+        const bool in = k < nunits && carry_row[k] == r;
+        if (in) {
+#pragma unroll
+            for (int q = 0; q < KB; ++q) s[q] = s[q] + carry_val[(uint64_t)k * KB + q];
+        }
         if (!__ballot(in)) break;
     }
-    s = wave_sum_f32(s);
-    if (lane == 0) y[r] = y[r] + alpha * s;
+#pragma unroll
+    for (int q = 0; q < KB; ++q) {
+        const float t = wave_sum_f32(s[q]);
+        if (lane == 0) {
+            const uint64_t i = (uint64_t)r * ldy + q;
+            Y[i] = Y[i] + alpha * t;
+        }
+    }
 }
 
 // Plain CSR comparator: one wave per 64 consecutive rows (the wave packs the
@@ -196,8 +308,66 @@ dcsr_status ensure_flat(dcsr_handle h, hipStream_t s)
     DCSR_TRY(ws_get(h, WS_PART, scan_part_words(R + 1, 1) * 4, (void **)&part));
     // row lengths are non-negative int32: scan them as uint32 in place
     DCSR_HIP(scan_exclusive_u32((const uint32_t *)h->row_len, foff, R, 1, part, foff + R, s));
+    if (!h->state_known) DCSR_TRY(pull_state(h, s));
+    const int64_t ng = h->nnz_h / GRAN + 1;
+    uint32_t *rowmap;
+    DCSR_TRY(ws_get(h, WS_ROWMAP, (size_t)ng * 4, (void **)&rowmap));
+    k_rowmap<<<grid_for(ng, TB), TB, 0, s>>>((uint32_t)R, foff, rowmap, (uint32_t)ng);
+    DCSR_HIP(hipGetLastError());
     h->flat_dirty = false;
     return DCSR_OK;
+}
+
+template <int KB>
+static dcsr_status launch_flat(dcsr_handle h, const float *X, int64_t ldx, float *Y, int64_t ldy, float alpha,
+                               float beta, hipStream_t s)
+{
+    const uint32_t R = (uint32_t)h->rows;
+    const uint32_t pw = DCSR_SPMV_PW;
+    int64_t nwaves = (h->nnz_h + pw - 1) / pw;
+    if (nwaves < 1) nwaves = 1;
+    const unsigned blocks = (unsigned)((nwaves + 3) / 4);
+    nwaves = (int64_t)blocks * 4;
+    uint32_t *crow;
+    float *cval;
+    DCSR_TRY(ws_get(h, WS_CARRY_ROW, (size_t)nwaves * 4, (void **)&crow));
+    DCSR_TRY(ws_get(h, WS_CARRY_VAL, (size_t)nwaves * 4 * KB, (void **)&cval));
+    const uint32_t *foff = (const uint32_t *)h->ws.ptr[WS_FOFF];
+    const uint32_t *rowmap = (const uint32_t *)h->ws.ptr[WS_ROWMAP];
+    k_flat<KB><<<blocks, TB, 0, s>>>(R, foff, rowmap, h->row_start, h->pool_col, h->pool_val, X, ldx, Y, ldy, alpha,
+                                     beta, pw, crow, cval);
+    k_carry<KB><<<blocks, TB, 0, s>>>((uint32_t)nwaves, crow, cval, Y, ldy, alpha);
+    DCSR_HIP(hipGetLastError());
+    return DCSR_OK;
+}
+
+template <int KB>
+static dcsr_status launch_groups(dcsr_handle h, const float *X, int64_t ldx, float *Y, int64_t ldy, float alpha,
+                                 float beta, hipStream_t s)
+{
+    const uint32_t R = (uint32_t)h->rows;
+    constexpr int G = 64 / KB;
+    int64_t ngroups = h->nnz_h / GRAN + 1;
+    const unsigned blocks = (unsigned)((ngroups + G * 4 - 1) / (G * 4));
+    ngroups = (int64_t)blocks * 4 * G;
+    uint32_t *crow;
+    float *cval;
+    DCSR_TRY(ws_get(h, WS_SPMM_CROW, (size_t)ngroups * 4, (void **)&crow));
+    DCSR_TRY(ws_get(h, WS_SPMM_CVAL, (size_t)ngroups * 4 * KB, (void **)&cval));
+    const uint32_t *foff = (const uint32_t *)h->ws.ptr[WS_FOFF];
+    const uint32_t *rowmap = (const uint32_t *)h->ws.ptr[WS_ROWMAP];
+    k_spmm_groups<KB><<<blocks, TB, 0, s>>>(R, foff, rowmap, h->row_start, h->pool_col, h->pool_val, X, ldx, Y, ldy,
+                                            alpha, beta, crow, cval);
+    k_carry<KB><<<grid_for(ngroups, 4), TB, 0, s>>>((uint32_t)ngroups, crow, cval, Y, ldy, alpha);
+    DCSR_HIP(hipGetLastError());
+    return DCSR_OK;
+}
+
+static dcsr_status prepare_compute(dcsr_handle h, hipStream_t s)
+{
+    h->last_stream = s;
+    if (!h->state_known) DCSR_TRY(pull_state(h, s));
+    return ensure_flat(h, s);
 }
 
 dcsr_status dcsr_spmv(dcsr_handle h, const float *x, float *y, float alpha, float beta, dcsr_stream_t stream)
@@ -206,23 +376,31 @@ dcsr_status dcsr_spmv(dcsr_handle h, const float *x, float *y, float alpha, floa
     hipStream_t s = (hipStream_t)stream;
     h->last_stream = s;
     if (h->rows == 0) return DCSR_OK;
-    if (!h->state_known) DCSR_TRY(pull_state(h, s));
-    DCSR_TRY(ensure_flat(h, s));
-    const uint32_t R = (uint32_t)h->rows;
-    const uint32_t pw = DCSR_SPMV_PW;
-    int64_t nwaves = (h->nnz_h + pw - 1) / pw;
-    if (nwaves < 1) nwaves = 1;
-    unsigned blocks = (unsigned)((nwaves + 3) / 4);
-    nwaves = (int64_t)blocks * 4;
-    uint32_t *crow;
-    float *cval;
-    DCSR_TRY(ws_get(h, WS_CARRY_ROW, (size_t)nwaves * 4, (void **)&crow));
-    DCSR_TRY(ws_get(h, WS_CARRY_VAL, (size_t)nwaves * 4, (void **)&cval));
-    const uint32_t *foff = (const uint32_t *)h->ws.ptr[WS_FOFF];
-    k_spmv_flat<<<blocks, TB, 0, s>>>(R, foff, h->row_start, h->pool_col, h->pool_val, x, y, alpha, beta, pw, crow,
-                                      cval);
-    k_spmv_carry<<<blocks, TB, 0, s>>>((uint32_t)nwaves, crow, cval, y, alpha);
-    DCSR_HIP(hipGetLastError());
+    DCSR_TRY(prepare_compute(h, s));
+    return launch_flat<1>(h, x, 1, y, 1, alpha, beta, s);
+}
+
+dcsr_status dcsr_spmm_dense(dcsr_handle h, const float *X, int64_t k, int64_t ldx, float *Y, int64_t ldy, float alpha,
+                            float beta, dcsr_stream_t stream)
+{
+    if (!h || k < 1 || ldx < k || ldy < k) return DCSR_ERR_INVALID;
+    if (h->rows > 0 && (!X || !Y)) return DCSR_ERR_INVALID;
+    hipStream_t s = (hipStream_t)stream;
+    h->last_stream = s;
+    if (h->rows == 0) return DCSR_OK;
+    DCSR_TRY(prepare_compute(h, s));
+    // column blocks of width 32, 16, 8, 4, 2, 1
+    for (int64_t cb = 0; cb < k;) {
+        const int64_t rem = k - cb;
+        const float *Xb = X + cb;
+        float *Yb = Y + cb;
+        if (rem >= 32) { DCSR_TRY(launch_groups<32>(h, Xb, ldx, Yb, ldy, alpha, beta, s)); cb += 32; }
+        else if (rem >= 16) { DCSR_TRY(launch_groups<16>(h, Xb, ldx, Yb, ldy, alpha, beta, s)); cb += 16; }
+        else if (rem >= 8) { DCSR_TRY(launch_groups<8>(h, Xb, ldx, Yb, ldy, alpha, beta, s)); cb += 8; }
+        else if (rem >= 4) { DCSR_TRY(launch_flat<4>(h, Xb, ldx, Yb, ldy, alpha, beta, s)); cb += 4; }
+        else if (rem >= 2) { DCSR_TRY(launch_flat<2>(h, Xb, ldx, Yb, ldy, alpha, beta, s)); cb += 2; }
+        else { DCSR_TRY(launch_flat<1>(h, Xb, ldx, Yb, ldy, alpha, beta, s)); cb += 1; }
+    }
     return DCSR_OK;
 }
 

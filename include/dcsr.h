@@ -144,6 +144,14 @@ dcsr_status dcsr_rebin(dcsr_handle h, dcsr_stream_t stream);
 dcsr_status dcsr_spmv(dcsr_handle h, const float *x, float *y, float alpha, float beta,
                       dcsr_stream_t stream);
 
+/* Y = alpha * A X + beta * Y with a dense multi-column right-hand side:
+ * X is cols x k row-major with leading dimension ldx >= k, Y is rows x k
+ * row-major with ldy >= k (beta == 0: Y is overwritten, never read).
+ * No reference counterpart (the reference has SpMV only); each column equals
+ * dcsr_spmv on that column up to FP32 summation order. */
+dcsr_status dcsr_spmm_dense(dcsr_handle h, const float *X, int64_t k, int64_t ldx, float *Y, int64_t ldy,
+                            float alpha, float beta, dcsr_stream_t stream);
+
 /* Counters and sizes of the handle (syncs). */
 dcsr_status dcsr_query(dcsr_handle h, dcsr_info *info);
 
