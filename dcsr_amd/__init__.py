@@ -90,6 +90,10 @@ def lib():
         "dcsr_rebin": (ctypes.c_int, [_H, _P]),
         "dcsr_spmv": (ctypes.c_int, [_H, _P, _P, _F, _F, _P]),
         "dcsr_spmm_dense": (ctypes.c_int, [_H, _P, _I64, _I64, _P, _I64, _F, _F, _P]),
+        "dcsr_spgemm": (ctypes.c_int, [_H, _H, _H, _P]),
+        "dcsr_export_hyb": (ctypes.c_int, [_H, ctypes.c_int32, _I64, _P, _P, _P, _P, _P, _I64,
+                                           ctypes.POINTER(_I64), _P]),
+        "dcsr_load_hyb": (ctypes.c_int, [_H, ctypes.c_int32, _I64, _P, _P, _P, _P, _P, _I64, _P]),
         "dcsr_query": (ctypes.c_int, [_H, ctypes.POINTER(DcsrInfo)]),
         "dcsr_csr_spmv": (ctypes.c_int, [_I64, _P, _P, _P, _P, _P, _P]),
     }
@@ -249,6 +253,33 @@ class DcsrMatrix:
                                     float(beta), _stream()), "dcsr_spmm_dense")
         return Y
 
+    def to_hyb(self, width, pitch=None):
+        """DCSR -> HYB: (ell_col, ell_val) of shape (width, pitch) column-major
+        slots and COO (row, col, val) overflow; pitch defaults to rows rounded
+        up to 32, as cusp's hyb_matrix."""
+        torch = _torch()
+        pitch = ((self.num_rows + 31) // 32) * 32 if pitch is None else int(pitch)
+        nnz = self.info()["nnz"]
+        ec = torch.empty(max(width * pitch, 1), dtype=torch.int32, device="cuda")
+        ev = torch.empty(max(width * pitch, 1), dtype=torch.float32, device="cuda")
+        cap = max(nnz, 1)
+        cr = torch.empty(cap, dtype=torch.int32, device="cuda")
+        cc = torch.empty(cap, dtype=torch.int32, device="cuda")
+        cv = torch.empty(cap, dtype=torch.float32, device="cuda")
+        n = _I64(0)
+        check(lib().dcsr_export_hyb(self._h, int(width), pitch, _ptr(ec), _ptr(ev), _ptr(cr), _ptr(cc), _ptr(cv),
+                                    cap, ctypes.byref(n), _stream()), "dcsr_export_hyb")
+        k = n.value
+        return ec[: width * pitch], ev[: width * pitch], cr[:k], cc[:k], cv[:k], pitch
+
+    def load_hyb(self, width, pitch, ell_col, ell_val, coo_row, coo_col, coo_val):
+        """HYB -> DCSR (ELL slots with column != -1, then the COO part)."""
+        torch = _torch()
+        ec, ev = _dev(ell_col, torch.int32), _dev(ell_val, torch.float32)
+        cr, cc, cv = _dev(coo_row, torch.int32), _dev(coo_col, torch.int32), _dev(coo_val, torch.float32)
+        check(lib().dcsr_load_hyb(self._h, int(width), int(pitch), _ptr(ec), _ptr(ev), _ptr(cr), _ptr(cc), _ptr(cv),
+                                  cr.numel(), _stream()), "dcsr_load_hyb")
+
     def info(self):
         inf = DcsrInfo()
         check(lib().dcsr_query(self._h, ctypes.byref(inf)), "dcsr_query")
@@ -260,6 +291,16 @@ class DcsrMatrix:
     @property
     def nnz(self):
         return self.info()["nnz"]
+
+
+def spgemm(A: DcsrMatrix, B: DcsrMatrix, C: DcsrMatrix = None) -> DcsrMatrix:
+    """C = A * B (device::spmm, reference include/spmm.h:30-391)."""
+    if A.num_cols != B.num_rows:
+        raise ValueError("inner dimensions differ")
+    if C is None:
+        C = DcsrMatrix(A.num_rows, B.num_cols)
+    check(lib().dcsr_spgemm(A.handle, B.handle, C.handle, _stream()), "dcsr_spgemm")
+    return C
 
 
 def csr_spmv(row_offsets, cols, vals, x, y=None):

@@ -51,6 +51,8 @@ enum WsSlot {
     WS_TROW, WS_TUBEG, WS_TINS, WS_TDEL, WS_TOLDS, WS_TOLDL, WS_TDEST, WS_TNEWL, WS_TMODE, WS_TOLDPRE,
     WS_HUB_ROWS, WS_HUB_PRE, WS_HUB_PART, WS_HUB_DONE,
     WS_PPROBE, WS_PPLAN, WS_TPAIR, WS_TPAIRPRE, WS_FOFF, WS_ROWMAP, WS_CARRY_ROW, WS_CARRY_VAL, WS_SPMM_CROW, WS_SPMM_CVAL,
+    WS_SG_P, WS_SG_POFF, WS_SG_CNT, WS_SG_COFF, WS_SG_LIST, WS_SG_STATE, WS_SG_BSUM, WS_SG_TCOL, WS_SG_TVAL,
+    WS_SG_HPRE, WS_SG_EA, WS_SG_APOS, WS_SG_KEY0, WS_SG_KEY1, WS_SG_VAL0, WS_SG_VAL1, WS_SG_FLAG, WS_SG_EPOS,
     WS_CANON, WS_TMP0, WS_TMP1, WS_TMP2,
     WS_COUNT
 };
@@ -110,6 +112,10 @@ dcsr_status pull_state(dcsr_handle h, hipStream_t s);   // copy DevState to host
 dcsr_status ensure_flat(dcsr_handle h, hipStream_t s);  // spmv.hip
 dcsr_status pool_reserve(dcsr_handle h, int64_t min_free, hipStream_t s, const int32_t *reserve_len);
 dcsr_status canonicalize(dcsr_handle h, hipStream_t s);
+dcsr_status alloc_pool(dcsr_handle h, int64_t cap);
+// Flat copy of R rows (src_start[r], length dst_off[r+1]-dst_off[r]) to dst_off.
+void launch_copy_rows(const uint32_t *dst_off, int64_t R, int64_t total, const uint32_t *src_start,
+                      const int32_t *scol, const float *sval, int32_t *dcol, float *dval, hipStream_t s);
 
 static inline int bits_for(int64_t n)
 {

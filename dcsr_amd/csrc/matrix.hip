@@ -334,7 +334,7 @@ static dcsr_status alloc_rows(dcsr_handle h)
     return DCSR_OK;
 }
 
-static dcsr_status alloc_pool(dcsr_handle h, int64_t cap)
+dcsr_status alloc_pool(dcsr_handle h, int64_t cap)
 {
     if (cap < h->cfg.min_pool) cap = h->cfg.min_pool;
     if (cap > (int64_t)0xfffffff0ll) cap = (int64_t)0xfffffff0ll;
@@ -509,6 +509,12 @@ static dcsr_status load_coo_impl(dcsr_handle h, const int32_t *rows, const int32
     return load_sorted(h, which ? k1 : k0, which ? v1 : v0, n, s);
 }
 
+dcsr_status load_coo_internal(dcsr_handle h, const int32_t *rows, const int32_t *cols, const float *vals, int64_t n,
+                              hipStream_t s)
+{
+    return load_coo_impl(h, rows, cols, vals, n, s);
+}
+
 dcsr_status dcsr_load_coo(dcsr_handle h, const int32_t *rows, const int32_t *cols, const float *vals, int64_t n,
                           dcsr_stream_t stream)
 {
@@ -548,7 +554,7 @@ static dcsr_status row_offsets_u32(dcsr_handle h, uint32_t *off, hipStream_t s)
     return DCSR_OK;
 }
 
-static void launch_copy_rows(const uint32_t *dst_off, int64_t R, int64_t total, const uint32_t *src_start,
+void launch_copy_rows(const uint32_t *dst_off, int64_t R, int64_t total, const uint32_t *src_start,
                              const int32_t *scol, const float *sval, int32_t *dcol, float *dval, hipStream_t s)
 {
     if (total <= 0 || R <= 0) return;

@@ -1,0 +1,581 @@
+// spgemm.hip -- C = A * B for DCSR operands.
+//
+// Replaces device::spmm(const dcsr& A, const dcsr& B, dcsr& C, streams)
+// (include/spmm.h:30-391) and its binned ESC kernels (include/spmm.inl:361-668,
+// 951-1061).  Result semantics are those of the host path
+// (cusp/system/detail/sequential/multiply/csr_spgemm.h:75-150): the structure
+// holds every (i, k) with a partial product whose FP32 sum, accumulated from
+// 0 in A-row order then B-row order, is not exactly 0.
+//
+// Rows are bucketed by their partial-product count P_i:
+//   P_i <= 1024  one wave per row, LDS ESC: the row's partials are expanded
+//                with their generation index (A-entry order, then B-row
+//                order), bitonic-sorted in LDS by (column, generation), and
+//                each run of equal columns is summed left-to-right;
+//   P_i <= 4096  the same with one 256-thread workgroup per row;
+//   P_i >  4096  (power-law hub rows, which the reference skips) a global
+//                ESC: expand, stable LSD radix sort by (row, column), reduce.
+// Output rows come out column-sorted with exact zeros removed, and the
+// accumulation order is fixed, so C is deterministic.
+#include "dcsr_internal.h"
+#include "prims.h"
+#include "segments.h"
+
+namespace {
+
+constexpr int TB = 256;
+constexpr uint32_t WAVE_MAX = 1024;   // partials per row for the wave kernel
+constexpr uint32_t BLOCK_MAX = 4096;  // partials per row for the workgroup kernel
+constexpr int NCLS = 3;
+
+struct SgState {
+    unsigned long long total;  // sum of P_i (64-bit)
+    unsigned int count[NCLS];  // rows per class
+    unsigned int pad;
+};
+
+__device__ __forceinline__ uint32_t next_pow2(uint32_t v)
+{
+    v--;
+    v |= v >> 1;
+    v |= v >> 2;
+    v |= v >> 4;
+    v |= v >> 8;
+    v |= v >> 16;
+    return v + 1;
+}
+
+// P_i = sum over the entries of A row i of len(B row col).  One wave per 64
+// consecutive rows; entries folded per row with a DPP segmented scan.
+__global__ __launch_bounds__(TB) void k_sg_count(uint32_t R, const uint32_t *__restrict__ a_start,
+                                                 const int32_t *__restrict__ a_len, const int32_t *__restrict__ a_col,
+                                                 const int32_t *__restrict__ b_len, uint32_t *__restrict__ P)
+{
+    __shared__ uint32_t s_ex[TB / 64][64];
+    __shared__ uint32_t s_st[TB / 64][64];
+    __shared__ uint32_t s_acc[TB / 64][64];
+    const int w = threadIdx.x >> 6;
+    const int lane = lane_id();
+    const uint32_t r = (blockIdx.x * (TB / 64) + w) * 64u + lane;
+    uint32_t st = 0, L = 0;
+    if (r < R) {
+        st = a_start[r];
+        L = (uint32_t)a_len[r];
+    }
+    const uint32_t inc = wave_incl_scan_u32(L);
+    const uint32_t tot = wave_read_lane_u32(inc, 63);
+    s_ex[w][lane] = inc - L;
+    s_st[w][lane] = st;
+    s_acc[w][lane] = 0;
+    __builtin_amdgcn_wave_barrier();
+    for (uint32_t b = 0; b < tot; b += 64) {
+        const uint32_t e = b + lane;
+        uint32_t j = 64, p = 0;
+        if (e < tot) {
+            j = (uint32_t)find_seg64(s_ex[w], e);
+            p = (uint32_t)b_len[a_col[(uint64_t)s_st[w][j] + (e - s_ex[w][j])]];
+        }
+        p = wave_seg_incl_scan_u32(p, j);
+        const uint32_t jn = __shfl_down(j, 1);
+        if (j < 64 && (lane == 63 || jn != j)) s_acc[w][j] += p;
+        __builtin_amdgcn_wave_barrier();
+    }
+    if (r < R) P[r] = s_acc[w][lane];
+}
+
+// Row classes (wave-aggregated list appends) and per-block 64-bit P sums.
+__global__ __launch_bounds__(TB) void k_sg_classify(uint32_t R, const uint32_t *__restrict__ P,
+                                                    uint32_t *__restrict__ lists, uint32_t list_stride,
+                                                    SgState *st, uint32_t *__restrict__ cnt,
+                                                    unsigned long long *__restrict__ bsum)
+{
+    __shared__ unsigned long long ws[TB / 64];
+    const uint32_t r = blockIdx.x * TB + threadIdx.x;
+    uint32_t p = 0;
+    int cls = -1;
+    if (r < R) {
+        p = P[r];
+        cls = p == 0 ? -1 : (p <= WAVE_MAX ? 0 : (p <= BLOCK_MAX ? 1 : 2));
+        if (p == 0) cnt[r] = 0;
+    }
+#pragma unroll
+    for (int c = 0; c < NCLS; ++c) {
+        const uint64_t m = __ballot(cls == c);
+        if (!m) continue;
+        const int leader = __ffsll((long long)m) - 1;
+        uint32_t base = 0;
+        if (lane_id() == leader) base = atomicAdd(&st->count[c], (uint32_t)__popcll(m));
+        base = __shfl(base, leader);
+        if (cls == c) lists[(uint64_t)c * list_stride + base + (uint32_t)__popcll(m & lanemask_lt())] = r;
+    }
+    unsigned long long v = p;
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o);
+    if (lane_id() == 0) ws[threadIdx.x >> 6] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) bsum[blockIdx.x] = (ws[0] + ws[1]) + (ws[2] + ws[3]);
+}
+
+__global__ void k_sg_state_zero(SgState *st)
+{
+    st->total = 0;
+    for (int c = 0; c < NCLS; ++c) st->count[c] = 0;
+}
+
+__global__ __launch_bounds__(1024) void k_sg_total(SgState *st, const unsigned long long *__restrict__ bsum, int nb)
+{
+    __shared__ unsigned long long red[16];
+    unsigned long long v = 0;
+    for (int b = threadIdx.x; b < nb; b += 1024) v += bsum[b];
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o);
+    if (lane_id() == 0) red[threadIdx.x >> 6] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned long long t = 0;
+        for (int i = 0; i < 16; ++i) t += red[i];
+        st->total = t;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Shared row routine: expand, sort, reduce.  NT threads cooperate (64 for the
+// wave kernel, 256 for the workgroup kernel); GB = generation-index bits.
+// ---------------------------------------------------------------------------
+template <int NT>
+__device__ __forceinline__ void team_sync()
+{
+    if (NT == 64) __builtin_amdgcn_wave_barrier();
+    else __syncthreads();
+}
+
+// Exclusive scan of one uint32 per team thread (team = NT threads).
+template <int NT>
+__device__ __forceinline__ uint32_t team_excl_scan(uint32_t v, uint32_t *lds_w, uint32_t *total)
+{
+    const uint32_t inc = wave_incl_scan_u32(v);
+    if (NT == 64) {
+        *total = wave_read_lane_u32(inc, 63);
+        return inc - v;
+    }
+    const int w = (threadIdx.x % NT) >> 6;
+    if (lane_id() == 63) lds_w[w] = inc;
+    __syncthreads();
+    uint32_t carry = 0, tot = 0;
+#pragma unroll
+    for (int i = 0; i < NT / 64; ++i) {
+        const uint32_t t = lds_w[i];
+        if (i < w) carry += t;
+        tot += t;
+    }
+    __syncthreads();
+    *total = tot;
+    return carry + inc - v;
+}
+
+// Largest j in [0, NT) with s[j] <= e.
+template <int NT>
+__device__ __forceinline__ uint32_t find_seg_team(const uint32_t *s, uint32_t e)
+{
+    uint32_t lo = 0, hi = NT;
+#pragma unroll
+    for (int it = 0; (1 << it) < NT; ++it) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (s[mid] <= e) lo = mid; else hi = mid;
+    }
+    return lo;
+}
+
+template <int NT, int GB>
+__device__ void sg_row(uint32_t row, uint32_t P, uint64_t out_base, const uint32_t *__restrict__ a_start,
+                       const int32_t *__restrict__ a_len, const int32_t *__restrict__ a_col,
+                       const float *__restrict__ a_val, const uint32_t *__restrict__ b_start,
+                       const int32_t *__restrict__ b_len, const int32_t *__restrict__ b_col,
+                       const float *__restrict__ b_val, int32_t *__restrict__ tcol, float *__restrict__ tval,
+                       uint32_t *__restrict__ cnt, uint64_t *key, float *val, uint32_t *s_ex, uint32_t *s_sb,
+                       float *s_va, uint32_t *lds_w)
+{
+    const uint32_t t = threadIdx.x % NT;
+    const uint32_t ast = a_start[row];
+    const uint32_t la = (uint32_t)a_len[row];
+    // 1. expand in generation order
+    uint32_t gbase = 0;
+    for (uint32_t a0 = 0; a0 < la; a0 += NT) {
+        const uint32_t a = a0 + t;
+        uint32_t lb = 0, sb = 0;
+        float va = 0.0f;
+        if (a < la) {
+            const int32_t j = a_col[(uint64_t)ast + a];
+            va = a_val[(uint64_t)ast + a];
+            lb = (uint32_t)b_len[j];
+            sb = b_start[j];
+        }
+        uint32_t tot;
+        const uint32_t ex = team_excl_scan<NT>(lb, lds_w, &tot);
+        s_ex[t] = ex;
+        s_sb[t] = sb;
+        s_va[t] = va;
+        team_sync<NT>();
+        for (uint32_t e0 = 0; e0 < tot; e0 += NT) {
+            const uint32_t pe = e0 + t;
+            if (pe < tot) {
+                const uint32_t q = find_seg_team<NT>(s_ex, pe);
+                const uint64_t bp = (uint64_t)s_sb[q] + (pe - s_ex[q]);
+                const uint32_t g = gbase + pe;
+                key[g] = ((uint64_t)(uint32_t)b_col[bp] << GB) | g;
+                val[g] = s_va[q] * b_val[bp];
+            }
+        }
+        gbase += tot;
+        team_sync<NT>();
+    }
+    // 2. bitonic sort of n2 keys (padding sorts last)
+    const uint32_t n2 = next_pow2(P < 64 ? 64 : P);
+    for (uint32_t p = P + t; p < n2; p += NT) key[p] = ~0ull;
+    team_sync<NT>();
+    for (uint32_t k = 2; k <= n2; k <<= 1) {
+        for (uint32_t js = k >> 1; js > 0; js >>= 1) {
+            for (uint32_t c = t; c < n2 / 2; c += NT) {
+                const uint32_t i1 = (c / js) * 2 * js + (c % js);
+                const uint32_t i2 = i1 + js;
+                const bool asc = (i1 & k) == 0;
+                const uint64_t ka = key[i1], kb = key[i2];
+                if ((ka > kb) == asc) {
+                    key[i1] = kb;
+                    key[i2] = ka;
+                    const float va = val[i1];
+                    val[i1] = val[i2];
+                    val[i2] = va;
+                }
+            }
+            team_sync<NT>();
+        }
+    }
+    // 3. runs of equal columns, summed left to right from 0; drop exact zeros
+    uint32_t outc = 0;
+    for (uint32_t p0 = 0; p0 < P; p0 += NT) {
+        const uint32_t p = p0 + t;
+        bool emit = false;
+        float s = 0.0f;
+        uint32_t c = 0;
+        if (p < P) {
+            c = (uint32_t)(key[p] >> GB);
+            const bool head = p == 0 || (uint32_t)(key[p - 1] >> GB) != c;
+            if (head) {
+                s = 0.0f + val[p];
+                for (uint32_t q = p + 1; q < P && (uint32_t)(key[q] >> GB) == c; ++q) s = s + val[q];
+                emit = s != 0.0f;
+            }
+        }
+        uint32_t tot;
+        const uint32_t pos = team_excl_scan<NT>(emit ? 1u : 0u, lds_w, &tot);
+        if (emit) {
+            tcol[out_base + outc + pos] = (int32_t)c;
+            tval[out_base + outc + pos] = s;
+        }
+        outc += tot;
+    }
+    if (t == 0) cnt[row] = outc;
+    team_sync<NT>();
+}
+
+__global__ __launch_bounds__(TB) void k_sg_wave(const uint32_t *__restrict__ list, const SgState *__restrict__ st,
+                                                const uint32_t *__restrict__ P, const uint32_t *__restrict__ Poff,
+                                                const uint32_t *__restrict__ a_start, const int32_t *__restrict__ a_len,
+                                                const int32_t *__restrict__ a_col, const float *__restrict__ a_val,
+                                                const uint32_t *__restrict__ b_start, const int32_t *__restrict__ b_len,
+                                                const int32_t *__restrict__ b_col, const float *__restrict__ b_val,
+                                                int32_t *__restrict__ tcol, float *__restrict__ tval,
+                                                uint32_t *__restrict__ cnt)
+{
+    __shared__ uint64_t key[TB / 64][WAVE_MAX];
+    __shared__ float val[TB / 64][WAVE_MAX];
+    __shared__ uint32_t s_ex[TB / 64][64], s_sb[TB / 64][64];
+    __shared__ float s_va[TB / 64][64];
+    const int w = threadIdx.x >> 6;
+    const uint32_t n = st->count[0];
+    const uint32_t nw = gridDim.x * (TB / 64);
+    for (uint32_t i = blockIdx.x * (TB / 64) + w; i < n; i += nw) {
+        const uint32_t row = list[i];
+        sg_row<64, 10>(row, P[row], Poff[row], a_start, a_len, a_col, a_val, b_start, b_len, b_col, b_val, tcol, tval,
+                       cnt, key[w], val[w], s_ex[w], s_sb[w], s_va[w], nullptr);
+    }
+}
+
+__global__ __launch_bounds__(TB) void k_sg_block(const uint32_t *__restrict__ list, const SgState *__restrict__ st,
+                                                 const uint32_t *__restrict__ P, const uint32_t *__restrict__ Poff,
+                                                 const uint32_t *__restrict__ a_start, const int32_t *__restrict__ a_len,
+                                                 const int32_t *__restrict__ a_col, const float *__restrict__ a_val,
+                                                 const uint32_t *__restrict__ b_start, const int32_t *__restrict__ b_len,
+                                                 const int32_t *__restrict__ b_col, const float *__restrict__ b_val,
+                                                 int32_t *__restrict__ tcol, float *__restrict__ tval,
+                                                 uint32_t *__restrict__ cnt)
+{
+    __shared__ uint64_t key[BLOCK_MAX];
+    __shared__ float val[BLOCK_MAX];
+    __shared__ uint32_t s_ex[TB], s_sb[TB];
+    __shared__ float s_va[TB];
+    __shared__ uint32_t lds_w[TB / 64];
+    const uint32_t n = st->count[1];
+    for (uint32_t i = blockIdx.x; i < n; i += gridDim.x) {
+        const uint32_t row = list[i];
+        sg_row<TB, 12>(row, P[row], Poff[row], a_start, a_len, a_col, a_val, b_start, b_len, b_col, b_val, tcol, tval,
+                       cnt, key, val, s_ex, s_sb, s_va, lds_w);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// global path for hub rows (P_i > BLOCK_MAX)
+// ---------------------------------------------------------------------------
+// pre[2h] = len(A row), pre[2h+1] = P for hub row h (index hn: zeros)
+__global__ void k_sg_hprep(const uint32_t *__restrict__ hlist, uint32_t hn, const int32_t *__restrict__ a_len,
+                           const uint32_t *__restrict__ P, uint32_t *__restrict__ pre)
+{
+    const uint32_t h = blockIdx.x * blockDim.x + threadIdx.x;
+    if (h > hn) return;
+    pre[2 * h] = h < hn ? (uint32_t)a_len[hlist[h]] : 0u;
+    pre[2 * h + 1] = h < hn ? P[hlist[h]] : 0u;
+}
+
+struct EA {
+    uint32_t h;
+    uint32_t bst;
+    float va;
+};
+
+// One record per A entry of every hub row (flat over hpre[2h]).
+__global__ __launch_bounds__(TB) void k_sg_hentries(const uint32_t *__restrict__ hlist, uint32_t hn,
+                                                    const uint32_t *__restrict__ hpre, uint32_t total,
+                                                    const uint32_t *__restrict__ a_start,
+                                                    const int32_t *__restrict__ a_col, const float *__restrict__ a_val,
+                                                    const uint32_t *__restrict__ b_start,
+                                                    const int32_t *__restrict__ b_len, EA *__restrict__ ea,
+                                                    uint32_t *__restrict__ blen)
+{
+    __shared__ uint32_t s_off[TB / 64][65];
+    const int w = threadIdx.x >> 6;
+    const uint64_t wave = (uint64_t)blockIdx.x * (TB / 64) + w;
+    const uint32_t per = 256;
+    const uint64_t e0 = wave * per;
+    if (e0 >= total) return;
+    const uint64_t e1 = e0 + per < total ? e0 + per : total;
+    wave_visit<2>(hpre, hn, (uint32_t)e0, (uint32_t)e1, s_off[w], [&](bool valid, uint32_t h, uint32_t k, uint32_t e) {
+        if (!valid) return;
+        const uint64_t ap = (uint64_t)a_start[hlist[h]] + k;
+        const int32_t j = a_col[ap];
+        EA r;
+        r.h = h;
+        r.bst = b_start[j];
+        r.va = a_val[ap];
+        ea[e] = r;
+        blen[e] = (uint32_t)b_len[j];
+    });
+}
+
+// Partials of hub rows in generation order: key = (h, col), value a*b.
+__global__ __launch_bounds__(TB) void k_sg_hexpand(const uint32_t *__restrict__ apos, uint32_t nent, uint32_t total,
+                                                   const EA *__restrict__ ea, const int32_t *__restrict__ b_col,
+                                                   const float *__restrict__ b_val, int cbits,
+                                                   uint64_t *__restrict__ key, uint32_t *__restrict__ val)
+{
+    __shared__ uint32_t s_off[TB / 64][65];
+    const int w = threadIdx.x >> 6;
+    const uint64_t wave = (uint64_t)blockIdx.x * (TB / 64) + w;
+    const uint32_t per = 256;
+    const uint64_t e0 = wave * per;
+    if (e0 >= total) return;
+    const uint64_t e1 = e0 + per < total ? e0 + per : total;
+    wave_visit(apos, nent, (uint32_t)e0, (uint32_t)e1, s_off[w], [&](bool valid, uint32_t q, uint32_t k, uint32_t e) {
+        if (!valid) return;
+        const EA r = ea[q];
+        const uint64_t bp = (uint64_t)r.bst + k;
+        key[e] = ((uint64_t)r.h << cbits) | (uint32_t)b_col[bp];
+        val[e] = __float_as_uint(r.va * b_val[bp]);
+    });
+}
+
+// Run heads of the sorted hub partials: left-to-right sums, emit flags.
+__global__ void k_sg_hreduce(const uint64_t *__restrict__ key, const uint32_t *__restrict__ val, uint32_t n,
+                             uint32_t *__restrict__ emit, float *__restrict__ sum)
+{
+    const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p > n) return;
+    uint32_t e = 0;
+    if (p < n) {
+        const uint64_t k = key[p];
+        if (p == 0 || key[p - 1] != k) {
+            float s = 0.0f + __uint_as_float(val[p]);
+            for (uint32_t q = p + 1; q < n && key[q] == k; ++q) s = s + __uint_as_float(val[q]);
+            e = s != 0.0f ? 1u : 0u;
+            sum[p] = s;
+        }
+    }
+    emit[p] = e;
+}
+
+__global__ void k_sg_hwrite(const uint64_t *__restrict__ key, uint32_t n, int cbits, const uint32_t *__restrict__ emit,
+                            const uint32_t *__restrict__ epos, const float *__restrict__ sum,
+                            const uint32_t *__restrict__ hlist, const uint32_t *__restrict__ hpre,
+                            const uint32_t *__restrict__ Poff, int32_t *__restrict__ tcol, float *__restrict__ tval)
+{
+    const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n || !emit[p]) return;
+    const uint64_t k = key[p];
+    const uint32_t h = (uint32_t)(k >> cbits);
+    const uint32_t first = epos[hpre[2 * h + 1]];  // emits before this hub row
+    const uint64_t o = (uint64_t)Poff[hlist[h]] + (epos[p] - first);
+    tcol[o] = (int32_t)(k & ((1ull << cbits) - 1ull));
+    tval[o] = sum[p];
+}
+
+__global__ void k_sg_hcount(const uint32_t *__restrict__ hlist, uint32_t hn, const uint32_t *__restrict__ hpre,
+                            const uint32_t *__restrict__ epos, uint32_t *__restrict__ cnt)
+{
+    const uint32_t h = blockIdx.x * blockDim.x + threadIdx.x;
+    if (h >= hn) return;
+    cnt[hlist[h]] = epos[hpre[2 * (h + 1) + 1]] - epos[hpre[2 * h + 1]];
+}
+
+// C row table from the output counts.
+__global__ void k_sg_set_rows(uint32_t R, const uint32_t *__restrict__ coff, const uint32_t *__restrict__ cnt,
+                              uint32_t *__restrict__ start, int32_t *__restrict__ len, int32_t *__restrict__ cap,
+                              DevState *ds)
+{
+    const uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r < R) {
+        start[r] = coff[r];
+        len[r] = (int32_t)cnt[r];
+        cap[r] = (int32_t)cnt[r];
+    }
+    if (r == 0) {
+        const uint32_t tot = coff[R];
+        ds->bump = tot;
+        ds->nnz = tot;
+        ds->garbage = 0;
+        ds->status = 0;
+    }
+}
+
+}  // namespace
+
+dcsr_status dcsr_spgemm(dcsr_handle A, dcsr_handle B, dcsr_handle C, dcsr_stream_t stream)
+{
+    if (!A || !B || !C || C == A || C == B) return DCSR_ERR_INVALID;
+    if (A->cols != B->rows || C->rows != A->rows || C->cols != B->cols) return DCSR_ERR_INVALID;
+    hipStream_t s = (hipStream_t)stream;
+    A->last_stream = B->last_stream = C->last_stream = s;
+    if (!A->state_known) DCSR_TRY(pull_state(A, s));
+    if (!B->state_known) DCSR_TRY(pull_state(B, s));
+    const uint32_t R = (uint32_t)A->rows;
+
+    uint32_t *P, *Poff, *cnt, *coff, *lists, *part;
+    SgState *st;
+    unsigned long long *bsum;
+    const unsigned nbR = grid_for(R, TB);
+    DCSR_TRY(ws_get(C, WS_SG_P, (size_t)(R + 1) * 4, (void **)&P));
+    DCSR_TRY(ws_get(C, WS_SG_POFF, (size_t)(R + 1) * 4, (void **)&Poff));
+    DCSR_TRY(ws_get(C, WS_SG_CNT, (size_t)(R + 1) * 4, (void **)&cnt));
+    DCSR_TRY(ws_get(C, WS_SG_COFF, (size_t)(R + 1) * 4, (void **)&coff));
+    DCSR_TRY(ws_get(C, WS_SG_LIST, (size_t)(R + 1) * 4 * NCLS, (void **)&lists));
+    DCSR_TRY(ws_get(C, WS_SG_STATE, sizeof(SgState), (void **)&st));
+    DCSR_TRY(ws_get(C, WS_SG_BSUM, (size_t)nbR * 8, (void **)&bsum));
+    DCSR_TRY(ws_get(C, WS_PART, scan_part_words(R + 1, 2) * 4, (void **)&part));
+
+    // 1. partial counts, classes, offsets
+    k_sg_state_zero<<<1, 1, 0, s>>>(st);
+    if (R > 0) {
+        k_sg_count<<<grid_for(R, TB), TB, 0, s>>>(R, A->row_start, A->row_len, A->pool_col, B->row_len, P);
+        k_sg_classify<<<nbR, TB, 0, s>>>(R, P, lists, R + 1, st, cnt, bsum);
+        k_sg_total<<<1, 1024, 0, s>>>(st, bsum, (int)nbR);
+    }
+    DCSR_HIP(scan_exclusive_u32(P, Poff, R, 1, part, Poff + R, s));
+    SgState hst;
+    DCSR_HIP(hipMemcpyAsync(&hst, st, sizeof(SgState), hipMemcpyDeviceToHost, s));
+    DCSR_HIP(hipStreamSynchronize(s));
+    if (hst.total >= 0xfffffff0ull) return DCSR_ERR_UNSUPPORTED;
+    const uint32_t Ptot = (uint32_t)hst.total;
+
+    int32_t *tcol;
+    float *tval;
+    DCSR_TRY(ws_get(C, WS_SG_TCOL, (size_t)(Ptot + 1) * 4, (void **)&tcol));
+    DCSR_TRY(ws_get(C, WS_SG_TVAL, (size_t)(Ptot + 1) * 4, (void **)&tval));
+
+    // 2. numeric phases
+    if (hst.count[0]) {
+        const unsigned g = (unsigned)((hst.count[0] + 3) / 4 < 8192 ? (hst.count[0] + 3) / 4 : 8192);
+        k_sg_wave<<<g, TB, 0, s>>>(lists, st, P, Poff, A->row_start, A->row_len, A->pool_col, A->pool_val,
+                                   B->row_start, B->row_len, B->pool_col, B->pool_val, tcol, tval, cnt);
+    }
+    if (hst.count[1]) {
+        const unsigned g = hst.count[1] < 4096 ? hst.count[1] : 4096;
+        k_sg_block<<<g, TB, 0, s>>>(lists + (R + 1), st, P, Poff, A->row_start, A->row_len, A->pool_col, A->pool_val,
+                                    B->row_start, B->row_len, B->pool_col, B->pool_val, tcol, tval, cnt);
+    }
+    if (hst.count[2]) {
+        const uint32_t hn = hst.count[2];
+        const uint32_t *hlist = lists + 2 * (uint64_t)(R + 1);
+        uint32_t *hpre_in, *hpre, *blen, *apos, *emit, *epos, *v0, *v1;
+        uint64_t *k0, *k1;
+        EA *ea;
+        DCSR_TRY(ws_get(C, WS_SG_HPRE, (size_t)(hn + 1) * 16, (void **)&hpre_in));
+        hpre = hpre_in + 2 * (hn + 1);
+        k_sg_hprep<<<grid_for(hn + 1, TB), TB, 0, s>>>(hlist, hn, A->row_len, P, hpre_in);
+        uint32_t htot[2];
+        uint32_t *htot_d;
+        DCSR_TRY(ws_get(C, WS_TMP2, 64, (void **)&htot_d));
+        DCSR_HIP(scan_exclusive_u32(hpre_in, hpre, hn + 1, 2, part, htot_d, s));
+        DCSR_HIP(hipMemcpyAsync(htot, htot_d, 8, hipMemcpyDeviceToHost, s));
+        DCSR_HIP(hipStreamSynchronize(s));
+        const uint32_t nent = htot[0], np = htot[1];
+        DCSR_TRY(ws_get(C, WS_SG_EA, (size_t)(nent + 1) * sizeof(EA), (void **)&ea));
+        DCSR_TRY(ws_get(C, WS_SG_APOS, (size_t)(nent + 2) * 8, (void **)&blen));
+        apos = blen + (nent + 1);
+        // hpre (the exclusive prefix), not hpre_in (the lengths): the segment
+        // search needs monotone offsets
+        k_sg_hentries<<<grid_for(nent, 4 * 256) + 1, TB, 0, s>>>(hlist, hn, hpre, nent, A->row_start, A->pool_col,
+                                                               A->pool_val, B->row_start, B->row_len, ea, blen);
+        DCSR_TRY(ws_get(C, WS_SCAN, scan_part_words(nent + 1, 1) * 4 + radix_part_words(np) * 4 + 64,
+                        (void **)&part));
+        DCSR_HIP(scan_exclusive_u32(blen, apos, nent, 1, part, apos + nent, s));
+        DCSR_TRY(ws_get(C, WS_SG_KEY0, (size_t)(np + 1) * 8, (void **)&k0));
+        DCSR_TRY(ws_get(C, WS_SG_KEY1, (size_t)(np + 1) * 8, (void **)&k1));
+        DCSR_TRY(ws_get(C, WS_SG_VAL0, (size_t)(np + 1) * 4, (void **)&v0));
+        DCSR_TRY(ws_get(C, WS_SG_VAL1, (size_t)(np + 1) * 4, (void **)&v1));
+        k_sg_hexpand<<<grid_for(np, 4 * 256) + 1, TB, 0, s>>>(apos, nent, np, ea, B->pool_col, B->pool_val, C->cbits,
+                                                              k0, v0);
+        uint32_t *hist;
+        DCSR_TRY(ws_get(C, WS_HIST, radix_hist_words(np) * 4, (void **)&hist));
+        int which = 0;
+        DCSR_HIP(radix_sort_pairs(k0, v0, k1, v1, np, bits_for(hn) + C->cbits, hist, part, &which, s));
+        uint64_t *key = which ? k1 : k0;
+        uint32_t *val = which ? v1 : v0;
+        float *sum;
+        DCSR_TRY(ws_get(C, WS_SG_FLAG, (size_t)(np + 2) * 8, (void **)&emit));
+        sum = (float *)(emit + (np + 2));
+        DCSR_TRY(ws_get(C, WS_SG_EPOS, (size_t)(np + 2) * 4, (void **)&epos));
+        k_sg_hreduce<<<grid_for(np + 1, TB), TB, 0, s>>>(key, val, np, emit, sum);
+        DCSR_HIP(scan_exclusive_u32(emit, epos, np + 1, 1, part, nullptr, s));
+        k_sg_hwrite<<<grid_for(np, TB), TB, 0, s>>>(key, np, C->cbits, emit, epos, sum, hlist, hpre, Poff, tcol, tval);
+        k_sg_hcount<<<grid_for(hn, TB), TB, 0, s>>>(hlist, hn, hpre, epos, cnt);
+    }
+    DCSR_HIP(hipGetLastError());
+
+    // 3. assemble C in natural row order
+    DCSR_TRY(ws_get(C, WS_PART, scan_part_words(R + 1, 1) * 4, (void **)&part));
+    DCSR_HIP(scan_exclusive_u32(cnt, coff, R, 1, part, coff + R, s));
+    uint32_t nnzC = 0;
+    DCSR_HIP(hipMemcpyAsync(&nnzC, coff + R, 4, hipMemcpyDeviceToHost, s));
+    DCSR_HIP(hipStreamSynchronize(s));
+    if ((int64_t)nnzC > C->pool_cap) {
+        int64_t want = (int64_t)(nnzC * C->cfg.pool_factor);
+        DCSR_TRY(alloc_pool(C, want > (int64_t)nnzC ? want : (int64_t)nnzC));
+    }
+    launch_copy_rows(coff, R, nnzC, Poff, tcol, tval, C->pool_col, C->pool_val, s);
+    k_sg_set_rows<<<grid_for(R + 1, TB), TB, 0, s>>>(R, coff, cnt, C->row_start, C->row_len, C->row_cap, C->ds);
+    DCSR_HIP(hipGetLastError());
+    DCSR_TRY(pull_state(C, s));
+    C->canonical = true;
+    C->flat_dirty = true;
+    C->hubs_dirty = true;
+    return DCSR_OK;
+}

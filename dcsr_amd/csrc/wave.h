@@ -73,6 +73,19 @@ __device__ __forceinline__ float wave_seg_incl_scan_f32(float v, uint32_t seg)
     return v;
 }
 
+// Segmented inclusive uint32 sum (segments as in wave_seg_incl_scan_f32).
+__device__ __forceinline__ uint32_t wave_seg_incl_scan_u32(uint32_t v, uint32_t seg)
+{
+    uint32_t t, s;
+    t = dpp_u32<DPP_ROW_SHR(1)>(0u, v); s = dpp_u32<DPP_ROW_SHR(1)>(0xffffffffu, seg); v += (s == seg) ? t : 0u;
+    t = dpp_u32<DPP_ROW_SHR(2)>(0u, v); s = dpp_u32<DPP_ROW_SHR(2)>(0xffffffffu, seg); v += (s == seg) ? t : 0u;
+    t = dpp_u32<DPP_ROW_SHR(4)>(0u, v); s = dpp_u32<DPP_ROW_SHR(4)>(0xffffffffu, seg); v += (s == seg) ? t : 0u;
+    t = dpp_u32<DPP_ROW_SHR(8)>(0u, v); s = dpp_u32<DPP_ROW_SHR(8)>(0xffffffffu, seg); v += (s == seg) ? t : 0u;
+    t = dpp_u32<DPP_ROW_BCAST15, 0xa>(0u, v); s = dpp_u32<DPP_ROW_BCAST15, 0xa>(0xffffffffu, seg); v += (s == seg) ? t : 0u;
+    t = dpp_u32<DPP_ROW_BCAST31, 0xc>(0u, v); s = dpp_u32<DPP_ROW_BCAST31, 0xc>(0xffffffffu, seg); v += (s == seg) ? t : 0u;
+    return v;
+}
+
 // Lanes holding the same 8-bit digit (the active-lane mask is `active`).
 __device__ __forceinline__ uint64_t wave_match_digit8(uint32_t d, uint64_t active)
 {

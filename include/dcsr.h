@@ -152,6 +152,35 @@ dcsr_status dcsr_spmv(dcsr_handle h, const float *x, float *y, float alpha, floa
 dcsr_status dcsr_spmm_dense(dcsr_handle h, const float *X, int64_t k, int64_t ldx, float *Y, int64_t ldy,
                             float alpha, float beta, dcsr_stream_t stream);
 
+/* C = A * B (SpGEMM).  C must be a distinct handle with C.rows == A.rows and
+ * C.cols == B.cols (A.cols == B.rows); its previous contents are replaced.
+ * Structure: every (i, k) with a partial product whose FP32 sum, accumulated
+ * from 0 in A-row order then B-row order, is not exactly zero (host path:
+ * cusp/system/detail/sequential/multiply/csr_spgemm.h:75-150); rows come out
+ * column-sorted.  Rows with more than 4096 partial products, which the
+ * reference skips, are included.  Syncs.
+ * Replaces device::spmm(const dcsr& A, const dcsr& B, dcsr& C, streams)
+ * (include/spmm.h:30-391). */
+dcsr_status dcsr_spgemm(dcsr_handle A, dcsr_handle B, dcsr_handle C, dcsr_stream_t stream);
+
+/* DCSR -> HYB (ELL + COO).  The ELL block holds `width` slots per row in
+ * column-major order (slot k of row r at k * pitch + r, pitch >= rows; padding
+ * column -1, value 0); entries past `width` go to the COO arrays (capacity
+ * coo_cap) in row order; *coo_n (host) receives their count.  Rows are taken
+ * with columns ascending, as the host CSR -> HYB conversion with the same
+ * width (cusp/system/detail/generic/conversions/csr_to_other.h:226-303).
+ * Syncs.  The reference converts only COO/CSR -> HYB
+ * (include/load.h:61-106, 174-205); HYB <-> DCSR went through CSR. */
+dcsr_status dcsr_export_hyb(dcsr_handle h, int32_t width, int64_t pitch, int32_t *ell_col, float *ell_val,
+                            int32_t *coo_row, int32_t *coo_col, float *coo_val, int64_t coo_cap, int64_t *coo_n,
+                            dcsr_stream_t stream);
+
+/* HYB -> DCSR: every ELL slot with a column != -1 (slot order), then the COO
+ * part, loaded as by dcsr_load_coo (duplicates kept).  Syncs. */
+dcsr_status dcsr_load_hyb(dcsr_handle h, int32_t width, int64_t pitch, const int32_t *ell_col, const float *ell_val,
+                          const int32_t *coo_row, const int32_t *coo_col, const float *coo_val, int64_t coo_n,
+                          dcsr_stream_t stream);
+
 /* Counters and sizes of the handle (syncs). */
 dcsr_status dcsr_query(dcsr_handle h, dcsr_info *info);
 
