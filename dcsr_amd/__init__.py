@@ -43,6 +43,16 @@ class DcsrConfig(ctypes.Structure):
                 ("allow_grow", ctypes.c_int32), ("reserved", ctypes.c_int32)]
 
 
+class DcsrRequest(ctypes.Structure):
+    _fields_ = [("kind", ctypes.c_int32), ("k", ctypes.c_int32), ("n", ctypes.c_int64),
+                ("rows", ctypes.c_void_p), ("cols", ctypes.c_void_p), ("vals", ctypes.c_void_p),
+                ("x", ctypes.c_void_p), ("y", ctypes.c_void_p), ("ldx", ctypes.c_int64), ("ldy", ctypes.c_int64)]
+
+
+class DcsrServerStats(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_int64) for n in ("updates", "update_triples", "queries", "query_groups", "compactions")]
+
+
 class DcsrInfo(ctypes.Structure):
     _fields_ = [(n, ctypes.c_int64) for n in (
         "rows", "cols", "nnz", "pool_cap", "pool_used", "garbage", "hub_rows",
@@ -94,6 +104,22 @@ def lib():
         "dcsr_export_hyb": (ctypes.c_int, [_H, ctypes.c_int32, _I64, _P, _P, _P, _P, _P, _I64,
                                            ctypes.POINTER(_I64), _P]),
         "dcsr_load_hyb": (ctypes.c_int, [_H, ctypes.c_int32, _I64, _P, _P, _P, _P, _P, _I64, _P]),
+        "dcsr_batch_create": (ctypes.c_int, [ctypes.POINTER(_H), ctypes.c_int32, _P, _P, ctypes.POINTER(DcsrConfig)]),
+        "dcsr_batch_destroy": (ctypes.c_int, [_H]),
+        "dcsr_batch_handle": (_H, [_H]),
+        "dcsr_batch_info": (ctypes.c_int, [_H, _P, _P, ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_int32)]),
+        "dcsr_batch_load_coo": (ctypes.c_int, [_H, _P, _P, _P, _P, _I64, _P]),
+        "dcsr_batch_step": (ctypes.c_int, [_H, _P, _P, _P, _P, _I64, _P, _P, _U32, _P]),
+        "dcsr_batch_status": (ctypes.c_int, [_H, _P]),
+        "dcsr_batch_autotune": (ctypes.c_int, [_H, _P, _P, ctypes.POINTER(ctypes.c_int32),
+                                               ctypes.POINTER(ctypes.c_int32), _P]),
+        "dcsr_batch_set_shape": (ctypes.c_int, [_H, ctypes.c_int32, ctypes.c_int32]),
+        "dcsr_server_create": (ctypes.c_int, [ctypes.POINTER(_H), _H, ctypes.c_int32]),
+        "dcsr_server_submit": (ctypes.c_int, [_H, ctypes.POINTER(DcsrRequest), ctypes.POINTER(ctypes.c_uint64)]),
+        "dcsr_server_wait": (ctypes.c_int, [_H, ctypes.c_uint64, ctypes.POINTER(ctypes.c_double)]),
+        "dcsr_server_drain": (ctypes.c_int, [_H]),
+        "dcsr_server_get_stats": (ctypes.c_int, [_H, ctypes.POINTER(DcsrServerStats)]),
+        "dcsr_server_destroy": (ctypes.c_int, [_H]),
         "dcsr_query": (ctypes.c_int, [_H, ctypes.POINTER(DcsrInfo)]),
         "dcsr_csr_spmv": (ctypes.c_int, [_I64, _P, _P, _P, _P, _P, _P]),
     }
@@ -301,6 +327,182 @@ def spgemm(A: DcsrMatrix, B: DcsrMatrix, C: DcsrMatrix = None) -> DcsrMatrix:
         C = DcsrMatrix(A.num_rows, B.num_cols)
     check(lib().dcsr_spgemm(A.handle, B.handle, C.handle, _stream()), "dcsr_spgemm")
     return C
+
+
+BATCH_GRAPH = 0x1
+
+
+class DcsrBatch:
+    """Many small DCSR matrices stepped (update + SpMV) in one persistent
+    launch.  Matrix m is rows[m] x cols[m] (each <= 4096), local indices."""
+
+    def __init__(self, rows, cols, pool_factor=3.0):
+        import numpy as np
+        torch = _torch()
+        torch.cuda.init()
+        r = np.ascontiguousarray(rows, np.int32)
+        c = np.ascontiguousarray(cols, np.int32)
+        assert r.size == c.size
+        self.count = r.size
+        cfg = DcsrConfig(pool_factor, 1024, 1, 0)
+        h = _H()
+        check(lib().dcsr_batch_create(ctypes.byref(h), r.size, r.ctypes.data_as(_P), c.ctypes.data_as(_P),
+                                      ctypes.byref(cfg)), "dcsr_batch_create")
+        self._b = h
+        rb = np.zeros(r.size + 1, np.int64)
+        cb = np.zeros(r.size + 1, np.int64)
+        check(lib().dcsr_batch_info(h, rb.ctypes.data_as(_P), cb.ctypes.data_as(_P), None, None), "dcsr_batch_info")
+        self.row_base, self.col_base = rb, cb
+        self.total_rows, self.total_cols = int(rb[-1]), int(cb[-1])
+
+    def close(self):
+        if getattr(self, "_b", None):
+            lib().dcsr_batch_destroy(self._b)
+            self._b = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check_lengths(self, o, r, c, v):
+        if o.numel() != self.count + 1:
+            raise DcsrError(1, f"tri_off needs {self.count + 1} entries, got {o.numel()}")
+        if not (r.numel() == c.numel() == v.numel()):
+            raise DcsrError(1, "rows, cols and vals differ in length")
+
+    def load_coo(self, tri_off, rows, cols, vals):
+        torch = _torch()
+        o = _dev(tri_off, torch.int64)
+        r, c, v = _dev(rows, torch.int32), _dev(cols, torch.int32), _dev(vals, torch.float32)
+        self._check_lengths(o, r, c, v)
+        check(lib().dcsr_batch_load_coo(self._b, _ptr(o), _ptr(r), _ptr(c), _ptr(v), r.numel(), _stream()),
+              "dcsr_batch_load_coo")
+
+    def step(self, tri_off=None, rows=None, cols=None, vals=None, x=None, y=None, graph=False):
+        """Update every matrix with its triples, then y = A x per matrix."""
+        torch = _torch()
+        if tri_off is not None:
+            o = _dev(tri_off, torch.int64)
+            r, c, v = _dev(rows, torch.int32), _dev(cols, torch.int32), _dev(vals, torch.float32)
+            self._check_lengths(o, r, c, v)
+            n = r.numel()
+        else:
+            o = r = c = v = None
+            n = 0
+        if x is not None:
+            x = _dev(x, torch.float32)
+            if y is None:
+                y = torch.empty(self.total_rows, dtype=torch.float32, device="cuda")
+        check(lib().dcsr_batch_step(self._b, _ptr(o), _ptr(r), _ptr(c), _ptr(v), n, _ptr(x), _ptr(y),
+                                    BATCH_GRAPH if graph else 0, _stream()), "dcsr_batch_step")
+        self._keep = (o, r, c, v, x, y)  # inputs must outlive a captured graph
+        return y
+
+    def status(self):
+        import numpy as np
+        per = np.zeros(self.count, np.uint32)
+        code = lib().dcsr_batch_status(self._b, per.ctypes.data_as(_P))
+        return code, per
+
+    def autotune(self, x, y):
+        t, th = ctypes.c_int32(0), ctypes.c_int32(0)
+        check(lib().dcsr_batch_autotune(self._b, _ptr(x), _ptr(y), ctypes.byref(t), ctypes.byref(th), _stream()),
+              "dcsr_batch_autotune")
+        return t.value, th.value
+
+    def set_shape(self, team, thresh):
+        check(lib().dcsr_batch_set_shape(self._b, int(team), int(thresh)), "dcsr_batch_set_shape")
+
+    def to_csr(self):
+        """The whole block-diagonal batch as CSR (local column indices)."""
+        torch = _torch()
+        h = lib().dcsr_batch_handle(self._b)
+        inf = DcsrInfo()
+        check(lib().dcsr_query(h, ctypes.byref(inf)), "dcsr_query")
+        cap = max(inf.nnz, 1)
+        off = torch.empty(self.total_rows + 1, dtype=torch.int32, device="cuda")
+        col = torch.empty(cap, dtype=torch.int32, device="cuda")
+        val = torch.empty(cap, dtype=torch.float32, device="cuda")
+        nnz = _I64(0)
+        check(lib().dcsr_export_csr(h, _ptr(off), _ptr(col), _ptr(val), cap, ctypes.byref(nnz), 0, _stream()),
+              "dcsr_export_csr")
+        return off, col[: nnz.value], val[: nnz.value]
+
+
+REQ_UPDATE, REQ_SPMV, REQ_SPMM = 0, 1, 2
+
+
+class DcsrServer:
+    """Resident-matrix request loop with continuous batching (see
+    include/dcsr.h).  Request tensors are kept alive until completion."""
+
+    def __init__(self, matrix: DcsrMatrix, max_cols=32):
+        h = _H()
+        check(lib().dcsr_server_create(ctypes.byref(h), matrix.handle, int(max_cols)), "dcsr_server_create")
+        self._s = h
+        self.matrix = matrix
+        self._live = {}
+
+    def _submit(self, req, keep):
+        # the server reads on its own streams: inputs must be complete now
+        _torch().cuda.current_stream().synchronize()
+        t = ctypes.c_uint64(0)
+        check(lib().dcsr_server_submit(self._s, ctypes.byref(req), ctypes.byref(t)), "dcsr_server_submit")
+        self._live[t.value] = keep
+        return t.value
+
+    def update(self, rows, cols, vals):
+        torch = _torch()
+        r, c, v = _dev(rows, torch.int32), _dev(cols, torch.int32), _dev(vals, torch.float32)
+        req = DcsrRequest(REQ_UPDATE, 0, r.numel(), r.data_ptr() if r.numel() else None,
+                          c.data_ptr() if c.numel() else None, v.data_ptr() if v.numel() else None, None, None, 0, 0)
+        return self._submit(req, (r, c, v))
+
+    def spmv(self, x, y=None):
+        torch = _torch()
+        x = _dev(x, torch.float32)
+        if y is None:
+            y = torch.empty(self.matrix.num_rows, dtype=torch.float32, device="cuda")
+        req = DcsrRequest(REQ_SPMV, 1, 0, None, None, None, x.data_ptr(), y.data_ptr(), 1, 1)
+        return self._submit(req, (x, y)), y
+
+    def spmm(self, X, Y=None):
+        torch = _torch()
+        X = _dev(X, torch.float32)
+        k = X.shape[1]
+        if Y is None:
+            Y = torch.empty((self.matrix.num_rows, k), dtype=torch.float32, device="cuda")
+        req = DcsrRequest(REQ_SPMM, k, 0, None, None, None, X.data_ptr(), Y.data_ptr(), X.stride(0), Y.stride(0))
+        return self._submit(req, (X, Y)), Y
+
+    def wait(self, ticket):
+        lat = ctypes.c_double(0.0)
+        code = lib().dcsr_server_wait(self._s, ticket, ctypes.byref(lat))
+        self._live.pop(ticket, None)
+        check(code, "server request")
+        return lat.value
+
+    def drain(self):
+        check(lib().dcsr_server_drain(self._s), "dcsr_server_drain")
+
+    def stats(self):
+        st = DcsrServerStats()
+        check(lib().dcsr_server_get_stats(self._s, ctypes.byref(st)), "dcsr_server_get_stats")
+        return {n: getattr(st, n) for n, _ in st._fields_}
+
+    def close(self):
+        if getattr(self, "_s", None):
+            lib().dcsr_server_destroy(self._s)
+            self._s = None
+            self._live.clear()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 def csr_spmv(row_offsets, cols, vals, x, y=None):

@@ -113,6 +113,11 @@ dcsr_status ensure_flat(dcsr_handle h, hipStream_t s);  // spmv.hip
 dcsr_status pool_reserve(dcsr_handle h, int64_t min_free, hipStream_t s, const int32_t *reserve_len);
 dcsr_status canonicalize(dcsr_handle h, hipStream_t s);
 dcsr_status alloc_pool(dcsr_handle h, int64_t cap);
+// update.hip: the two phases of dcsr_update (prepare only reads the matrix)
+dcsr_status update_prepare(dcsr_handle h, const int32_t *rows, const int32_t *cols, const float *vals, int64_t n,
+                           hipStream_t s, bool *go, int64_t *need);
+dcsr_status update_replan(dcsr_handle h, int64_t n, hipStream_t s, bool *go);
+dcsr_status update_commit(dcsr_handle h, int64_t n, hipStream_t s, bool nosync);
 // Flat copy of R rows (src_start[r], length dst_off[r+1]-dst_off[r]) to dst_off.
 void launch_copy_rows(const uint32_t *dst_off, int64_t R, int64_t total, const uint32_t *src_start,
                       const int32_t *scol, const float *sval, int32_t *dcol, float *dval, hipStream_t s);

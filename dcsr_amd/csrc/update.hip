@@ -482,13 +482,14 @@ dcsr_status plan_and_check(dcsr_handle h, int64_t n, UpdBufs &b, hipStream_t s)
 
 }  // namespace
 
-dcsr_status dcsr_update(dcsr_handle h, const int32_t *rows, const int32_t *cols, const float *vals, int64_t n,
-                        uint32_t flags, dcsr_stream_t stream)
+// Phase 1 (reads the matrix, writes only workspace): validation, sort,
+// merge plan and pool check.  *go (host) receives whether the batch may
+// commit; on pool exhaustion *need receives the claimed slot count.
+dcsr_status update_prepare(dcsr_handle h, const int32_t *rows, const int32_t *cols, const float *vals, int64_t n,
+                           hipStream_t s, bool *go, int64_t *need)
 {
-    if (!h || n < 0 || (n > 0 && (!rows || !cols || !vals))) return DCSR_ERR_INVALID;
-    if (n >= (int64_t)0x7ffffff0ll) return DCSR_ERR_INVALID;
-    hipStream_t s = (hipStream_t)stream;
-    h->last_stream = s;
+    *go = false;
+    *need = 0;
     if (!h->canonical) {
         // validate first: a rejected batch must not canonicalize the matrix
         if (n > 0) {
@@ -511,7 +512,6 @@ dcsr_status dcsr_update(dcsr_handle h, const int32_t *rows, const int32_t *cols,
     }
     UpdBufs b;
     DCSR_TRY(upd_bufs(h, n, b));
-
     const int nbits = h->rbits + h->cbits + 1;
     k_upd_begin<<<1, 1, 0, s>>>(h->ds);
     k_upd_pack<<<grid_for(n, TB), TB, 0, s>>>(rows, cols, vals, n, h->rows, h->cols, h->cbits, b.k0, b.v0, h->ds);
@@ -528,28 +528,32 @@ dcsr_status dcsr_update(dcsr_handle h, const int32_t *rows, const int32_t *cols,
                                                     b.uflag, b.tins, b.tdel, b.pprobe);
     DCSR_HIP(scan_exclusive_u32(b.uflag, b.upre, n + 1, 2, b.part, nullptr, s));
     DCSR_TRY(plan_and_check(h, n, b, s));
+    DCSR_TRY(pull_state(h, s));
+    if (h->hs->ubad) return DCSR_ERR_BAD_INDEX;
+    *go = h->hs->go != 0;
+    *need = (int64_t)h->hs->need;
+    return DCSR_OK;
+}
 
-    const bool nosync = (flags & DCSR_UPDATE_NOSYNC) != 0;
-    if (!nosync) {
-        DCSR_TRY(pull_state(h, s));
-        if (!h->hs->go) {
-            unsigned bits = h->hs->status;
-            if (h->hs->ubad) return DCSR_ERR_BAD_INDEX;
-            if ((bits & ST_POOL_EXHAUSTED) && h->cfg.allow_grow && !(flags & DCSR_UPDATE_NOGROW)) {
-                // make room, then plan again (row contents are unchanged, only
-                // their locations move)
-                int64_t need = (int64_t)h->hs->need;
-                DCSR_TRY(pool_reserve(h, need + need / 8 + 64, s, nullptr));
-                k_upd_rebase<<<1, 1, 0, s>>>(h->ds);
-                DCSR_TRY(plan_and_check(h, n, b, s));
-                DCSR_TRY(pull_state(h, s));
-                if (!h->hs->go) return DCSR_ERR_POOL_EXHAUSTED;
-            } else {
-                return DCSR_ERR_POOL_EXHAUSTED;
-            }
-        }
-    }
-    // persistent move kernel: any grid size is correct
+// After pool growth/compaction between prepare and commit: plan again (row
+// contents are unchanged, only their locations moved).
+dcsr_status update_replan(dcsr_handle h, int64_t n, hipStream_t s, bool *go)
+{
+    UpdBufs b;
+    DCSR_TRY(upd_bufs(h, n, b));
+    k_upd_rebase<<<1, 1, 0, s>>>(h->ds);
+    DCSR_TRY(plan_and_check(h, n, b, s));
+    DCSR_TRY(pull_state(h, s));
+    *go = h->hs->go != 0;
+    return DCSR_OK;
+}
+
+// Phase 2 (writes the matrix): moves, inserts, in-place values, row-table
+// commit.  Launches only; the host mirrors are set from the prepare check.
+dcsr_status update_commit(dcsr_handle h, int64_t n, hipStream_t s, bool nosync)
+{
+    UpdBufs b;
+    DCSR_TRY(upd_bufs(h, n, b));
     {
         const uint32_t per_wave = 256;
         int64_t bound = nosync ? h->pool_cap : (int64_t)h->hs->nrewrite_old;
@@ -570,7 +574,6 @@ dcsr_status dcsr_update(dcsr_handle h, const int32_t *rows, const int32_t *cols,
         h->hubs_dirty = true;
         return DCSR_OK;
     }
-    // host mirrors from the check point
     const DevState &st = *h->hs;
     h->nnz_h = st.nnz + st.d_nnz;
     h->bump_h = (int64_t)st.bump;
@@ -582,4 +585,57 @@ dcsr_status dcsr_update(dcsr_handle h, const int32_t *rows, const int32_t *cols,
     if (st.hub_touch) h->hubs_dirty = true;
     h->state_known = true;
     return DCSR_OK;
+}
+
+static dcsr_status update_nosync(dcsr_handle h, const int32_t *rows, const int32_t *cols, const float *vals, int64_t n,
+                                 hipStream_t s)
+{
+    if (!h->canonical) {
+        bool go;
+        int64_t need;
+        return update_prepare(h, rows, cols, vals, n, s, &go, &need) != DCSR_OK
+                   ? DCSR_ERR_BAD_INDEX
+                   : (go ? update_commit(h, n, s, false) : DCSR_ERR_POOL_EXHAUSTED);
+    }
+    if (n == 0) return DCSR_OK;
+    UpdBufs b;
+    DCSR_TRY(upd_bufs(h, n, b));
+    const int nbits = h->rbits + h->cbits + 1;
+    k_upd_begin<<<1, 1, 0, s>>>(h->ds);
+    k_upd_pack<<<grid_for(n, TB), TB, 0, s>>>(rows, cols, vals, n, h->rows, h->cols, h->cbits, b.k0, b.v0, h->ds);
+    int which = 0;
+    DCSR_HIP(radix_sort_pairs(b.k0, b.v0, b.k1, b.v1, n, nbits, b.hist, b.part, &which, s));
+    uint64_t *key = which ? b.k1 : b.k0;
+    uint32_t *val = which ? b.v1 : b.v0;
+    k_upd_flags<<<grid_for(n, TB), TB, 0, s>>>(key, n, h->cbits, b.flags);
+    DCSR_HIP(scan_exclusive_u32(b.flags, b.pre, n, 2, b.part, &h->ds->nuniq, s));
+    k_upd_compact<<<grid_for(n, TB), TB, 0, s>>>(key, val, n, h->cbits, b.flags, b.pre, b.ucol, b.usum, b.ut, b.trow,
+                                                  b.tubeg, b.tins, b.tdel);
+    k_upd_probe<<<grid_for(n + 1, TB), TB, 0, s>>>(n, h->ds, h->row_start, h->row_len, h->pool_col, h->pool_val,
+                                                    b.ucol, b.usum, b.ut, b.trow, b.tubeg, b.uval, b.upos, b.ucode,
+                                                    b.uflag, b.tins, b.tdel, b.pprobe);
+    DCSR_HIP(scan_exclusive_u32(b.uflag, b.upre, n + 1, 2, b.part, nullptr, s));
+    DCSR_TRY(plan_and_check(h, n, b, s));
+    return update_commit(h, n, s, true);
+}
+
+dcsr_status dcsr_update(dcsr_handle h, const int32_t *rows, const int32_t *cols, const float *vals, int64_t n,
+                        uint32_t flags, dcsr_stream_t stream)
+{
+    if (!h || n < 0 || (n > 0 && (!rows || !cols || !vals))) return DCSR_ERR_INVALID;
+    if (n >= (int64_t)0x7ffffff0ll) return DCSR_ERR_INVALID;
+    hipStream_t s = (hipStream_t)stream;
+    h->last_stream = s;
+    if (flags & DCSR_UPDATE_NOSYNC) return update_nosync(h, rows, cols, vals, n, s);
+    bool go = false;
+    int64_t need = 0;
+    DCSR_TRY(update_prepare(h, rows, cols, vals, n, s, &go, &need));
+    if (n == 0) return DCSR_OK;
+    if (!go) {
+        if (!h->cfg.allow_grow || (flags & DCSR_UPDATE_NOGROW)) return DCSR_ERR_POOL_EXHAUSTED;
+        DCSR_TRY(pool_reserve(h, need + need / 8 + 64, s, nullptr));
+        DCSR_TRY(update_replan(h, n, s, &go));
+        if (!go) return DCSR_ERR_POOL_EXHAUSTED;
+    }
+    return update_commit(h, n, s, false);
 }

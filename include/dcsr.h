@@ -181,6 +181,87 @@ dcsr_status dcsr_load_hyb(dcsr_handle h, int32_t width, int64_t pitch, const int
                           const int32_t *coo_row, const int32_t *coo_col, const float *coo_val, int64_t coo_n,
                           dcsr_stream_t stream);
 
+/* ---------------------------------------------------------------------------
+ * Batches of small matrices (no reference counterpart: the reference keeps
+ * one process-global bin table, include/Matrix_Test.h:57, and so handles one
+ * matrix at a time).  Matrix m has rows[m] x cols[m] (each <= 4096) and uses
+ * local indices.  A step applies every matrix's own insert-or-add batch (at
+ * most 4096 triples per matrix; semantics of dcsr_update) and then its SpMV,
+ * all in ONE persistent launch; with DCSR_BATCH_GRAPH the launch is captured
+ * once into a hipGraph and replayed on later calls with the same arguments.
+ * Triples of all matrices are concatenated; tri_off (device, count+1) gives
+ * each matrix's range.  x / y concatenate the matrices' vectors in order.
+ * ------------------------------------------------------------------------- */
+typedef struct dcsr_batch_s *dcsr_batch;
+#define DCSR_BATCH_GRAPH 0x1u
+
+dcsr_status dcsr_batch_create(dcsr_batch *out, int32_t count, const int32_t *rows, const int32_t *cols,
+                              const dcsr_config *cfg); /* rows/cols: host arrays */
+dcsr_status dcsr_batch_destroy(dcsr_batch b);
+/* The block-diagonal handle holding the batch (matrix m = global rows
+ * [row_base[m], row_base[m+1]) with local columns): export / query only. */
+dcsr_handle dcsr_batch_handle(dcsr_batch b);
+/* Host arrays of count+1 offsets; current SpMV shape (lanes per row, long-row
+ * threshold).  Any pointer may be NULL. */
+dcsr_status dcsr_batch_info(dcsr_batch b, int64_t *row_base, int64_t *col_base, int32_t *team, int32_t *thresh);
+dcsr_status dcsr_batch_load_coo(dcsr_batch b, const int64_t *tri_off, const int32_t *rows, const int32_t *cols,
+                                const float *vals, int64_t n, dcsr_stream_t stream);
+/* One step: update (skipped when tri_off is NULL or n == 0) then SpMV
+ * (skipped when x or y is NULL).  Syncs once to check pool headroom. */
+dcsr_status dcsr_batch_step(dcsr_batch b, const int64_t *tri_off, const int32_t *rows, const int32_t *cols,
+                            const float *vals, int64_t n, const float *x, float *y, uint32_t flags,
+                            dcsr_stream_t stream);
+/* OR of the per-matrix status of the last step (syncs); per_matrix (host,
+ * count entries) may be NULL. */
+dcsr_status dcsr_batch_status(dcsr_batch b, uint32_t *per_matrix);
+/* Autotuner: times the step's SpMV for lanes-per-row in {4,8,16,32} and
+ * long-row thresholds {64,256,1024} and keeps the fastest; choices are cached
+ * per shape class (matrix count, mean row length) for the process. */
+dcsr_status dcsr_batch_autotune(dcsr_batch b, const float *x, float *y, int32_t *team_out, int32_t *thresh_out,
+                                dcsr_stream_t stream);
+dcsr_status dcsr_batch_set_shape(dcsr_batch b, int32_t team, int32_t thresh);
+
+/* ---------------------------------------------------------------------------
+ * Serving loop (no reference counterpart).  A server owns a resident matrix
+ * (use only the server API on that handle while the server lives) and runs a
+ * scheduler thread with continuous batching: queries (SpMV / SpMM) queued
+ * together between two updates are answered by one multi-column SpMM on the
+ * same matrix version (at most max_cols columns per group); an update's
+ * prepare phase (reads the matrix only) overlaps the previous query group and
+ * its commit is ordered after it; every query sees exactly the updates
+ * submitted before it.  Updates that need more pool than is free wait for
+ * in-flight queries, then the pool is compacted / grown.  Request arrays are
+ * device pointers whose contents must be complete when submitted (the server
+ * reads them on its own streams) and that must stay valid until the request
+ * completes.
+ * ------------------------------------------------------------------------- */
+typedef struct dcsr_server_s *dcsr_server;
+enum { DCSR_REQ_UPDATE = 0, DCSR_REQ_SPMV = 1, DCSR_REQ_SPMM = 2 };
+typedef struct dcsr_request {
+    int32_t kind;          /* DCSR_REQ_* */
+    int32_t k;             /* SpMM columns */
+    int64_t n;             /* update: triples */
+    const int32_t *rows;   /* update */
+    const int32_t *cols;
+    const float *vals;
+    const float *x;        /* SpMV: x[cols]; SpMM: X (cols x k, row-major, ldx) */
+    float *y;              /* SpMV: y[rows]; SpMM: Y (rows x k, row-major, ldy) */
+    int64_t ldx, ldy;
+} dcsr_request;
+typedef struct dcsr_server_stats {
+    int64_t updates, update_triples, queries, query_groups, compactions;
+} dcsr_server_stats;
+
+dcsr_status dcsr_server_create(dcsr_server *out, dcsr_handle h, int32_t max_cols);
+/* Enqueue; *ticket identifies the request.  Never blocks on the GPU. */
+dcsr_status dcsr_server_submit(dcsr_server s, const dcsr_request *r, uint64_t *ticket);
+/* Block until the request finished; returns its status and (host pointer,
+ * may be NULL) the submit-to-completion latency in milliseconds. */
+dcsr_status dcsr_server_wait(dcsr_server s, uint64_t ticket, double *latency_ms);
+dcsr_status dcsr_server_drain(dcsr_server s);
+dcsr_status dcsr_server_get_stats(dcsr_server s, dcsr_server_stats *out);
+dcsr_status dcsr_server_destroy(dcsr_server s);
+
 /* Counters and sizes of the handle (syncs). */
 dcsr_status dcsr_query(dcsr_handle h, dcsr_info *info);
 
