@@ -1,22 +1,36 @@
 #!/usr/bin/env python3
-"""Benchmark: insert-or-add update batches + SpMV on a power-law DCSR matrix.
+"""Benchmarks of the Dynamic-CSR engine on one MI355X.  Prints ONE JSON line.
 
-Workload (one step = one 1,000,000-triple insert-or-add batch followed by one
-SpMV y = A x, on the same resident matrix):
+--workload powerlaw (default; the headline):
+  one step = one 1,000,000-triple insert-or-add batch followed by one SpMV
+  y = A x, on the same resident matrix:
   * matrix: 325,557 x 325,557, 3,200,000 nnz, Zipf(2.1) out-degrees capped at
     2,700, uniform columns without replacement, FP32 values in (0, 1], seed 1;
   * batches: 50% existing coordinates (uniform over stored entries, i.e. rows
     in proportion to degree), 50% new columns on degree-drawn rows, values in
     (0, 1]; all batches are generated up front and resident in HBM.
+  The default run applies 8 batches (the matrix grows to ~7.2M nnz) after W
+  warmup steps on the same handle, reloaded before the timed steps.
 
-The default run applies 8 batches (the matrix grows to ~7.2M nnz) after W
-warmup steps on a separate copy.  Prints ONE JSON line.
+--workload batched: 4,096 square matrices, rows log-uniform in [64, 4096],
+  Poisson(16) nnz per row; one step = every matrix applies its own
+  2,000-triple batch and computes its SpMV, in one persistent launch replayed
+  from a hipGraph.
 
-value      = triples applied per second over whole steps (update + SpMV)
-roofline   = the dominant kernel of the step, algorithmic bytes / measured
-             per-launch time (HIP events on the launch stream)
+--workload spgemm: 2,000,000 x 2,000,000, Poisson(8) nnz per row, N(0,1)
+  values; one 4M-triple update, then one step = C = A * A.
+
+--workload serving: resident power-law matrix; a Poisson stream of update
+  batches (log-uniform 1k..1M triples, 30% deletions by -current) and
+  SpMV/SpMM queries (k in {1, 4, 16, 32}); the server (continuous batching)
+  against the plain per-call path on the same stream.
+
+value        = whole-job throughput of the workload's step (unit in "unit")
+roofline     = the dominant kernel (or kernel pipeline) of the step:
+               algorithmic bytes / measured time (HIP events on the launch
+               stream), against 8 TB/s of HBM
 cpu_baseline = the reference host path (cusp sequential, oracle/_ref) or its
-             restatement (oracle/) timed on a bounded sample on this host.
+               restatement (oracle/) timed on a bounded sample on this host.
 """
 from __future__ import annotations
 
@@ -32,33 +46,75 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+METRIC = "inserted nnz/s, SpMV GB/s (% HBM peak), p99 latency on mixed update/SpMV stream"
 R_CFG = 325_557
 NNZ_CFG = 3_200_000
 MAXDEG = 2700
 BATCH = 1_000_000
+WORKLOADS = ("powerlaw", "batched", "spgemm", "serving")
+DEFAULT_STEPS = {"powerlaw": 8, "batched": 10, "spgemm": 3, "serving": 400}
 
 
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=8)
+    ap.add_argument("--steps", type=int, default=None, help="timed steps (default per workload)")
     ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--workload", choices=WORKLOADS, default="powerlaw")
     ap.add_argument("--batch", type=int, default=BATCH)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-sample-batches", type=int, default=8)
     ap.add_argument("--cpu-min-seconds", type=float, default=10.0)
-    return ap.parse_args()
+    ap.add_argument("--serving-rows", type=int, default=10_000_000)
+    ap.add_argument("--serving-nnz", type=int, default=160_000_000)
+    args = ap.parse_args()
+    if args.steps is None:
+        args.steps = DEFAULT_STEPS[args.workload]
+    return args
 
 
 def spmv_bytes(R, C, N):
     return 4 * (R + 1) + 8 * N + 4 * C + 4 * R
 
 
+def spmm_bytes(R, C, N, k):
+    return 4 * (R + 1) + 8 * N + 4 * k * C + 4 * k * R
+
+
 def update_bytes(b, n_ins, n_add, touched):
     return 12 * b + 8 * n_ins + 8 * n_add + 8 * touched
 
 
-def cpu_baseline(r, c, v, batches, x, nsample, min_seconds):
+def roofline(achieved_gbs, kernel, traffic=None):
+    return {"bound": "hbm", "achieved": round(achieved_gbs, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(achieved_gbs / HBM_PEAK_GBS, 4), "traffic": traffic, "kernel": kernel}
+
+
+def line(args, value, unit, ms_per_step, config, higher=True, **extra):
+    out = {
+        "metric": METRIC,
+        "value": value,
+        "unit": unit,
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": higher,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": config,
+    }
+    out.update(extra)
+    print(json.dumps(out), flush=True)
+
+
+# --------------------------------------------------------------------------
+# powerlaw: update + SpMV steps (headline)
+# --------------------------------------------------------------------------
+
+def cpu_baseline_powerlaw(r, c, v, batches, x, nsample, min_seconds):
     """Reference host path (or its restatement) on the first nsample batches,
     the sequence repeated from the initial matrix until min_seconds of work."""
     import oracle as O
@@ -91,17 +147,11 @@ def cpu_baseline(r, c, v, batches, x, nsample, min_seconds):
     }
 
 
-def main():
-    args = parse()
-    if args.gpus != 1:
-        print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); this engine runs on a single MI355X.",
-              file=sys.stderr)
-        sys.exit(2)
+def bench_powerlaw(args):
     import torch
     import dcsr_amd
     from dcsr_amd import workloads as W
 
-    dcsr_amd.lib()
     K, Wm = args.steps, args.warmup
     r, c, v = W.powerlaw(R_CFG, NNZ_CFG, 2.1, MAXDEG, seed=1)
     rng = np.random.default_rng(101)
@@ -139,67 +189,399 @@ def main():
         e1.record(stream)
         m.spmv(d_x, d_y)
         e2.record(stream)
-        inf = m.info()
-        stats.append(inf)
+        stats.append(m.info())
     torch.cuda.synchronize()
-    t1 = time.perf_counter()
-    elapsed = t1 - t0
+    elapsed = time.perf_counter() - t0
 
     upd_ms = np.array([e0.elapsed_time(e1) for e0, e1, _ in ev])
     spmv_ms = np.array([e1.elapsed_time(e2) for _, e1, e2 in ev])
     step_ms = np.array([e0.elapsed_time(e2) for e0, _, e2 in ev])
     trip = K * args.batch
     n_ins = sum(s["last_inserted"] for s in stats)
-    n_add = sum(s["last_added"] for s in stats)
-    touched = sum(s["last_touched_rows"] for s in stats)
     nnz_after = [s["nnz"] for s in stats]
     spmv_gb = [spmv_bytes(R_CFG, R_CFG, n) / 1e9 for n in nnz_after]
     spmv_gbs = float(np.mean([b / (t / 1e3) for b, t in zip(spmv_gb, spmv_ms)]))
     upd_algo = [update_bytes(args.batch, s["last_inserted"], s["last_added"], s["last_touched_rows"]) for s in stats]
     upd_gbs = float(np.mean([b / 1e9 / (t / 1e3) for b, t in zip(upd_algo, upd_ms)]))
 
-    # dominant piece of the step
     if upd_ms.sum() >= spmv_ms.sum():
-        dom = {"kernel": "update pipeline (dcsr_update)", "bound": "hbm", "achieved": upd_gbs}
-        traffic = None
+        roof = roofline(upd_gbs, "update pipeline (dcsr_update, all kernels of one call)")
     else:
-        dom = {"kernel": "k_spmv_rows (dcsr_spmv)", "bound": "hbm", "achieved": spmv_gbs}
-        traffic = None
-    roof = {"bound": dom["bound"], "achieved": round(dom["achieved"], 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(dom["achieved"] / HBM_PEAK_GBS, 4), "traffic": traffic, "kernel": dom["kernel"]}
+        roof = roofline(spmv_gbs, "flat SpMV (dcsr_spmv)")
+    roof["algorithmic_bytes_per_launch"] = float(np.mean(upd_algo))
+    roof["avg_launch_ms"] = float(upd_ms.mean())
 
     cpu = None
     if not args.no_cpu:
-        cpu = cpu_baseline(r, c, v, batches, x, min(args.cpu_sample_batches, len(batches)), args.cpu_min_seconds)
+        cpu = cpu_baseline_powerlaw(r, c, v, batches, x, min(args.cpu_sample_batches, len(batches)),
+                                    args.cpu_min_seconds)
+    line(args, trip / elapsed, "triples/s", elapsed * 1e3 / K,
+         {"workload": "powerlaw-update-spmv", "rows": R_CFG, "nnz0": NNZ_CFG, "zipf": 2.1, "max_degree": MAXDEG,
+          "batch_triples": args.batch, "nnz_final": nnz_after[-1]},
+         inserted_nnz_per_s=n_ins / elapsed,
+         update_triples_per_s_gpu=trip / (upd_ms.sum() / 1e3),
+         spmv_gbs=spmv_gbs,
+         spmv_hbm_frac=spmv_gbs / HBM_PEAK_GBS,
+         update_algo_gbs=upd_gbs,
+         p50_step_ms=float(np.percentile(step_ms, 50)),
+         p99_step_ms=float(np.percentile(step_ms, 99)),
+         mean_update_ms=float(upd_ms.mean()),
+         mean_spmv_ms=float(spmv_ms.mean()),
+         roofline=roof,
+         cpu_baseline=cpu)
 
-    out = {
-        "metric": "inserted nnz/s, SpMV GB/s (% HBM peak), p99 latency on mixed update/SpMV stream",
-        "value": trip / elapsed,
-        "unit": "triples/s",
-        "n_gpus": 1,
-        "steps": K,
-        "warmup": Wm,
-        "ms_per_step": elapsed * 1e3 / K,
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "f32",
-        "data": "synthetic",
-        "config": {"workload": "powerlaw-update-spmv", "rows": R_CFG, "nnz0": NNZ_CFG, "zipf": 2.1,
-                   "max_degree": MAXDEG, "batch_triples": args.batch, "nnz_final": nnz_after[-1]},
-        "inserted_nnz_per_s": n_ins / elapsed,
-        "update_triples_per_s_gpu": trip / (upd_ms.sum() / 1e3),
-        "spmv_gbs": spmv_gbs,
-        "spmv_hbm_frac": spmv_gbs / HBM_PEAK_GBS,
-        "update_algo_gbs": upd_gbs,
-        "p50_step_ms": float(np.percentile(step_ms, 50)),
-        "p99_step_ms": float(np.percentile(step_ms, 99)),
-        "mean_update_ms": float(upd_ms.mean()),
-        "mean_spmv_ms": float(spmv_ms.mean()),
-        "roofline": roof,
-        "cpu_baseline": cpu,
-    }
-    print(json.dumps(out))
+
+# --------------------------------------------------------------------------
+# batched: 4,096 small matrices, one graph-replayed persistent launch per step
+# --------------------------------------------------------------------------
+
+def bench_batched(args):
+    import torch
+    import dcsr_amd
+    from dcsr_amd import workloads as W
+
+    K, Wm = args.steps, args.warmup
+    count, per = 4096, 2000
+    mats = W.many_small(count=count, min_rows=64, max_rows=4096, lam=16.0, seed=4)
+    sizes = np.array([mm[0] for mm in mats], np.int32)
+    rng = np.random.default_rng(202)
+
+    def concat(parts):
+        off = np.zeros(len(parts) + 1, np.int64)
+        off[1:] = np.cumsum([p[0].size for p in parts])
+        return off, np.concatenate([p[0] for p in parts]), np.concatenate([p[1] for p in parts]), \
+            np.concatenate([p[2] for p in parts])
+
+    steps = [concat([W.batch_mixed(rng, mm[1], mm[2], mm[0], mm[0], per, signed=True) for mm in mats])
+             for _ in range(K + Wm)]
+    dev = torch.device("cuda:0")
+    b = dcsr_amd.DcsrBatch(sizes, sizes, pool_factor=8.0)
+    off0, r0, c0, v0 = concat([(mm[1], mm[2], mm[3]) for mm in mats])
+    b.load_coo(off0, r0, c0, v0)
+    nnz0 = int(r0.size)
+    d_steps = [tuple(torch.from_numpy(a).to(dev) for a in st) for st in steps]
+    x = torch.from_numpy(rng.uniform(-1, 1, b.total_cols).astype(np.float32)).to(dev)
+    y = torch.empty(b.total_rows, dtype=torch.float32, device=dev)
+    team, thresh = b.autotune(x, y)
+    for k in range(Wm):
+        b.step(*d_steps[K + k], x=x, y=y, graph=True)
+    torch.cuda.synchronize()
+    code, _ = b.status()
+    if code != 0:
+        raise RuntimeError(f"batched warmup step failed with status {code}")
+    stream = torch.cuda.current_stream()
+    ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(K)]
+    infos = []
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for k in range(K):
+        ev[k][0].record(stream)
+        b.step(*d_steps[k], x=x, y=y, graph=True)
+        ev[k][1].record(stream)
+        code, _ = b.status()
+        if code != 0:
+            raise RuntimeError(f"batched step {k} failed with status {code}")
+        infos.append(b.info())
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    step_ms = np.array([e0.elapsed_time(e1) for e0, e1 in ev])
+    trip = count * per
+    R = b.total_rows
+    algo = []
+    for inf in infos:
+        # SpMV bytes summed over matrices = formula on the block-diagonal whole
+        algo.append(update_bytes(trip, inf["last_inserted"], inf["last_added"], inf["last_touched_rows"])
+                    + spmv_bytes(R, b.total_cols, inf["nnz"]) + 4 * count)
+    gbs = float(np.mean([a / 1e9 / (t / 1e3) for a, t in zip(algo, step_ms)]))
+    b.close()
+    line(args, trip / elapsed, "triples/s", elapsed * 1e3 / K,
+         {"workload": "batched-4096-update-spmv", "matrices": count, "rows_total": int(R), "nnz0": nnz0,
+          "triples_per_matrix": per, "team": team, "thresh": thresh},
+         step_wall_ms=elapsed * 1e3 / K,
+         kernel_step_ms=float(step_ms.mean()),
+         aggregate_gbs=gbs,
+         roofline=roofline(gbs, "k_batch_step (persistent, graph replay)"),
+         cpu_baseline=None)
+
+
+# --------------------------------------------------------------------------
+# spgemm: 2M x 2M Poisson(8), one 4M-triple update, then C = A * A
+# --------------------------------------------------------------------------
+
+def bench_spgemm(args):
+    import torch
+    import dcsr_amd
+    from dcsr_amd import workloads as W
+
+    K, Wm = args.steps, args.warmup
+    R = 2_000_000
+    r, c, v = W.poisson(R, R, 8.0, seed=3, normal=True)
+    rng = np.random.default_rng(303)
+    br, bc, bv = W.batch_mixed(rng, r, c, R, R, 4_000_000, signed=True)
+    dev = torch.device("cuda:0")
+    A = dcsr_amd.DcsrMatrix(R, R, pool_factor=2.0)
+    A.LoadMatrix(*(torch.from_numpy(a).to(dev) for a in (r, c, v)))
+    A.UpdateMatrix(*(torch.from_numpy(a).to(dev) for a in (br, bc, bv)))
+    off, col, val = (t.cpu().numpy() for t in A.to_csr(canonical=True))
+    nA = int(col.size)
+    rowlen = np.diff(off.astype(np.int64))
+    P = int(np.dot(np.bincount(col, minlength=R).astype(np.int64), rowlen))
+    C = dcsr_amd.DcsrMatrix(R, R)
+    for _ in range(Wm):
+        dcsr_amd.spgemm(A, A, C)
+    torch.cuda.synchronize()
+    stream = torch.cuda.current_stream()
+    ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(K)]
+    t0 = time.perf_counter()
+    for k in range(K):
+        ev[k][0].record(stream)
+        dcsr_amd.spgemm(A, A, C)
+        ev[k][1].record(stream)
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    ms = np.array([e0.elapsed_time(e1) for e0, e1 in ev])
+    nC = C.info()["nnz"]
+    b_comp = 8 * nA + 8 * nA + 8 * nC + 12 * R
+    b_gather = 8 * nA + 8 * P + 8 * nC + 12 * R
+    t = float(ms.mean()) / 1e3
+    cpu = None
+    if not args.no_cpu:
+        import oracle as O
+        use_ref = O.ref_available()
+        Acsr = O.Csr(R, R, off.astype(np.int32), col.astype(np.int32), val.astype(np.float32))
+        rows_s = 50_000
+        tt, pp, reps = 0.0, 0, 0
+        while reps == 0 or tt < args.cpu_min_seconds:
+            r0 = (reps * rows_s) % (R - rows_s)
+            o = off[r0:r0 + rows_s + 1].astype(np.int64)
+            sub = O.Csr(rows_s, R, (o - o[0]).astype(np.int32), col[o[0]:o[-1]], val[o[0]:o[-1]])
+            pp += int(rowlen[col[o[0]:o[-1]]].sum())
+            t1 = time.perf_counter()
+            O.spgemm(sub, Acsr, use_ref=use_ref)
+            tt += time.perf_counter() - t1
+            reps += 1
+        cpu = {"value": 2 * pp / tt / 1e9, "unit": "GFLOP/s", "cores": 1, "kind": "reference" if use_ref else "port",
+               "sample": f"{reps} blocks of {rows_s} rows of A times A (host Gustavson SpGEMM, single thread, with its "
+                         f"symbolic count pass); {pp} partial products in {tt:.2f}s"}
+    line(args, 2 * P / t / 1e9, "GFLOP/s", elapsed * 1e3 / K,
+         {"workload": "spgemm-2M-poisson8", "rows": R, "nnz_A": nA, "partials": P, "nnz_C": int(nC),
+          "update_triples": 4_000_000},
+         spgemm_ms=float(ms.mean()),
+         b_comp_gbs=b_comp / t / 1e9,
+         b_gather_gbs=b_gather / t / 1e9,
+         roofline=roofline(b_comp / t / 1e9, "SpGEMM pipeline (dcsr_spgemm, all kernels of one call)"),
+         cpu_baseline=cpu)
+
+
+# --------------------------------------------------------------------------
+# serving: resident power-law matrix, Poisson request stream, server vs
+# the plain per-call path
+# --------------------------------------------------------------------------
+
+def serving_matrix(R, N, seed=5):
+    """Zipf(2.1) out-degrees scaled to N (capped at R/10), uniform columns;
+    coordinates made unique on the GPU.  Returns host (rows, cols, vals)."""
+    import torch
+    from dcsr_amd import workloads as W
+    rng = np.random.default_rng(seed)
+    d, _ = W.zipf_degrees(rng, R, N, 2.1, max(R // 10, 1))
+    g = torch.Generator(device="cuda")
+    g.manual_seed(seed)
+    rows = torch.repeat_interleave(torch.arange(R, device="cuda", dtype=torch.int64),
+                                   torch.from_numpy(d).to("cuda"))
+    cols = torch.randint(0, R, (rows.numel(),), device="cuda", generator=g, dtype=torch.int64)
+    key = torch.unique(rows * R + cols, sorted=True)
+    del rows, cols
+    r = (key // R).to(torch.int32).cpu().numpy()
+    c = (key % R).to(torch.int32).cpu().numpy()
+    del key
+    v = (1.0 - rng.random(r.size, dtype=np.float32)).astype(np.float32)
+    return r, c, v
+
+
+def serving_stream(rng, r, c, v, R, nreq, upd_frac=0.1):
+    """Request list: (kind, k or n, arrays).  Updates: log-uniform 1e3..1e6
+    triples, 30% deletions (-current of a coordinate no earlier request
+    touched), 35% adds to stored coordinates, 35% new coordinates on
+    degree-drawn rows.  Queries: k = 1, 4, 16, 32 with weights .55/.2/.15/.1."""
+    dirty = np.zeros(r.size, bool)
+    reqs = []
+    ks = np.array([1, 4, 16, 32])
+    for _ in range(nreq):
+        if rng.random() < upd_frac:
+            n = int(np.exp(rng.uniform(np.log(1e3), np.log(1e6))))
+            nd = int(0.3 * n)
+            na = int(0.35 * n)
+            nn = n - nd - na
+            pick = rng.integers(0, r.size, 2 * nd + 16)
+            pick = np.unique(pick[~dirty[pick]])[:nd]
+            nd = pick.size
+            add = rng.integers(0, r.size, na)
+            dirty[pick] = True
+            dirty[add] = True
+            newrow = r[rng.integers(0, r.size, nn)]
+            rows = np.concatenate([r[pick], r[add], newrow]).astype(np.int32)
+            cols = np.concatenate([c[pick], c[add], rng.integers(0, R, nn)]).astype(np.int32)
+            vals = np.concatenate([-v[pick], (1.0 - rng.random(na + nn, dtype=np.float32))]).astype(np.float32)
+            perm = rng.permutation(rows.size)
+            reqs.append(("update", rows.size, (rows[perm], cols[perm], vals[perm])))
+        else:
+            reqs.append(("query", int(rng.choice(ks, p=[0.55, 0.2, 0.15, 0.1])), None))
+    return reqs
+
+
+def bench_serving(args):
+    import torch
+    import dcsr_amd
+
+    R, N = args.serving_rows, args.serving_nnz
+    nreq = args.steps
+    t_gen = time.perf_counter()
+    r, c, v = serving_matrix(R, N)
+    rng = np.random.default_rng(55)
+    reqs = serving_stream(rng, r, c, v, R, nreq + args.warmup * 4)
+    warm, reqs = reqs[:args.warmup * 4], reqs[args.warmup * 4:]
+    dev = torch.device("cuda:0")
+    d_r, d_c, d_v = (torch.from_numpy(a).to(dev) for a in (r, c, v))
+    nnz0 = int(r.size)
+    del r, c, v
+    upd = {i: tuple(torch.from_numpy(a).to(dev) for a in q[2]) for i, q in enumerate(reqs) if q[0] == "update"}
+    wupd = {i: tuple(torch.from_numpy(a).to(dev) for a in q[2]) for i, q in enumerate(warm) if q[0] == "update"}
+    # query inputs are shared per k (read-only); outputs rotate over a ring
+    X = {k: torch.rand((R, k), generator=torch.Generator(device="cuda").manual_seed(k), device=dev)
+         for k in (1, 4, 16, 32)}
+    ring = 4
+    Y = {k: [torch.empty((R, k), device=dev) for _ in range(ring)] for k in (1, 4, 16, 32)}
+    t_gen = time.perf_counter() - t_gen
+
+    def fresh():
+        m = dcsr_amd.DcsrMatrix(R, R, pool_factor=1.5)
+        m.LoadMatrix(d_r, d_c, d_v)
+        torch.cuda.synchronize()
+        return m
+
+    def service_times():
+        """Per-request service time of the per-call path, run back to back."""
+        m = fresh()
+        t = []
+        for i, q in enumerate(reqs):
+            t0 = time.perf_counter()
+            if q[0] == "update":
+                m.UpdateMatrix(*upd[i])
+            elif q[1] == 1:
+                m.spmv(X[1][:, 0], Y[1][i % ring][:, 0])
+            else:
+                m.spmm(X[q[1]], Y[q[1]][i % ring])
+            torch.cuda.synchronize()
+            t.append(time.perf_counter() - t0)
+        m.close()
+        return np.array(t)
+
+    def arrivals(rate):
+        gaps = np.random.default_rng(77).exponential(1.0 / rate, len(reqs))
+        return np.cumsum(gaps)
+
+    def run_per_call(rate):
+        m = fresh()
+        arr = arrivals(rate)
+        lat = np.zeros(len(reqs))
+        t_start = time.perf_counter()
+        for i, q in enumerate(reqs):
+            now = time.perf_counter() - t_start
+            if now < arr[i]:
+                time.sleep(arr[i] - now)
+            if q[0] == "update":
+                m.UpdateMatrix(*upd[i])
+            elif q[1] == 1:
+                m.spmv(X[1][:, 0], Y[1][i % ring][:, 0])
+            else:
+                m.spmm(X[q[1]], Y[q[1]][i % ring])
+            torch.cuda.synchronize()
+            lat[i] = time.perf_counter() - t_start - arr[i]
+        span = time.perf_counter() - t_start
+        m.close()
+        return len(reqs) / span, lat * 1e3
+
+    def run_server(rate, warmup=False):
+        m = fresh()
+        srv = dcsr_amd.DcsrServer(m, max_cols=64)
+        if warmup:
+            for i, q in enumerate(warm):
+                if q[0] == "update":
+                    srv.wait(srv.update(*wupd[i]))
+                else:
+                    srv.wait(srv.spmm(X[q[1]], Y[q[1]][0])[0])
+        arr = arrivals(rate)
+        tickets = []
+        t_start = time.perf_counter()
+        for i, q in enumerate(reqs):
+            now = time.perf_counter() - t_start
+            if now < arr[i]:
+                time.sleep(arr[i] - now)
+            if q[0] == "update":
+                tickets.append(srv.update(*upd[i]))
+            elif q[1] == 1:
+                tickets.append(srv.spmv(X[1][:, 0], Y[1][i % ring][:, 0])[0])
+            else:
+                tickets.append(srv.spmm(X[q[1]], Y[q[1]][i % ring])[0])
+        lat = np.array([srv.wait(t) for t in tickets])
+        span = time.perf_counter() - t_start
+        st = srv.stats()
+        srv.close()
+        m.close()
+        return len(reqs) / span, lat, st
+
+    run_server(1e9, warmup=True)  # first-touch workspaces, server threads
+    svc = service_times()
+    cap_pc = 1.0 / svc.mean()  # per-call capacity (requests/s)
+    # per-call path at 90% of its capacity: its p99 is the latency bar
+    rate_pc = 0.9 * cap_pc
+    thr_pc, lat_pc = run_per_call(rate_pc)
+    p99_pc = float(np.percentile(lat_pc, 99))
+    # server: raise the offered rate until its p99 exceeds the per-call p99
+    best = None
+    sweep = []
+    for mult in (1, 2, 3, 4, 6, 8, 12):
+        thr, lat, st = run_server(rate_pc * mult)
+        p50, p99 = float(np.percentile(lat, 50)), float(np.percentile(lat, 99))
+        sweep.append({"offered_x": mult, "throughput": thr, "p50_ms": p50, "p99_ms": p99,
+                      "groups": st["query_groups"], "queries": st["queries"], "compactions": st["compactions"]})
+        if p99 <= p99_pc:
+            best = sweep[-1]
+        else:
+            break
+    if best is None:
+        best = sweep[0]
+    ratio = best["throughput"] / thr_pc
+    line(args, best["throughput"], "requests/s", 1e3 / best["throughput"],
+         {"workload": "serving-powerlaw", "rows": R, "nnz0": nnz0, "requests": len(reqs), "update_share": 0.1,
+          "update_triples": "log-uniform 1e3..1e6, 30% deletes", "query_k": [1, 4, 16, 32]},
+         p50_ms=best["p50_ms"], p99_ms=best["p99_ms"],
+         per_call={"throughput": thr_pc, "p50_ms": float(np.percentile(lat_pc, 50)), "p99_ms": p99_pc,
+                   "capacity": cap_pc},
+         throughput_vs_per_call_at_equal_p99=ratio,
+         sweep=sweep,
+         setup_s=t_gen,
+         roofline=None,
+         cpu_baseline=None)
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); this engine runs on a single MI355X.",
+              file=sys.stderr)
+        sys.exit(2)
+    import dcsr_amd
+    dcsr_amd.lib()
+    if args.workload == "powerlaw":
+        bench_powerlaw(args)
+    elif args.workload == "batched":
+        bench_batched(args)
+    elif args.workload == "spgemm":
+        bench_spgemm(args)
+    else:
+        bench_serving(args)
 
 
 if __name__ == "__main__":

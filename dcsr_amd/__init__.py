@@ -415,6 +415,13 @@ class DcsrBatch:
     def set_shape(self, team, thresh):
         check(lib().dcsr_batch_set_shape(self._b, int(team), int(thresh)), "dcsr_batch_set_shape")
 
+    def info(self):
+        """Counters of the whole batch (dcsr_query on its storage); the
+        last_* fields are those of the last step once status() has run."""
+        inf = DcsrInfo()
+        check(lib().dcsr_query(lib().dcsr_batch_handle(self._b), ctypes.byref(inf)), "dcsr_query")
+        return inf.as_dict()
+
     def to_csr(self):
         """The whole block-diagonal batch as CSR (local column indices)."""
         torch = _torch()

@@ -219,6 +219,7 @@ __device__ void batch_update(uint32_t m, const uint32_t *__restrict__ row_base, 
     __syncthreads();
     // U3: probe
     uint32_t *uip = sc + S_UIP, *udp = sc + S_UDP;
+    uint32_t my_adds = 0;
     for (uint32_t u = t; u <= nu; u += BT) {
         uint32_t code = C_NOP;
         if (u < nu) {
@@ -246,13 +247,15 @@ __device__ void batch_update(uint32_t m, const uint32_t *__restrict__ row_base, 
             ucode[u] = code;
             if (code == C_INS) atomicAdd(&tins[ut[u]], 1u);
             if (code == C_DEL) atomicAdd(&tdel[ut[u]], 1u);
+            my_adds += code == C_ADD ? 1u : 0u;
         }
         uip[u] = code == C_INS ? 1u : 0u;
         udp[u] = code == C_DEL ? 1u : 0u;
     }
     __syncthreads();
-    block_scan_inplace(uip, nu + 1, lw);
-    block_scan_inplace(udp, nu + 1, lw);
+    const uint32_t nadd = block_sum(my_adds, lw);
+    const uint32_t nins = block_scan_inplace(uip, nu + 1, lw);
+    const uint32_t ndel = block_scan_inplace(udp, nu + 1, lw);
     // U4: plan, claim
     uint32_t *tdst = sc + S_TDST, *tmp = sc + S_TMP, *tmode = sc + S_TMODE, *tnl = sc + S_TNL;
     long long dnnz = 0, dgar = 0;
@@ -291,6 +294,10 @@ __device__ void batch_update(uint32_t m, const uint32_t *__restrict__ row_base, 
         if (ok) {
             atomicAdd((unsigned long long *)&ds->nnz, (unsigned long long)(long long)(int32_t)dn_lo);
             atomicAdd((unsigned long long *)&ds->garbage, (unsigned long long)(long long)(int32_t)dg_lo);
+            atomicAdd((unsigned long long *)&ds->n_ins, (unsigned long long)nins);
+            atomicAdd((unsigned long long *)&ds->n_add, (unsigned long long)nadd);
+            atomicAdd((unsigned long long *)&ds->n_del, (unsigned long long)ndel);
+            atomicAdd(&ds->ntouched, nt);
         }
     }
     __syncthreads();
@@ -447,10 +454,14 @@ __global__ void k_batch_globalize(const int64_t *__restrict__ tri_off, uint32_t 
     if (bad) atomicOr(&ds->status, ST_BAD_INDEX);
 }
 
-__global__ void k_batch_zero_stat(uint32_t *mstat, uint32_t M)
+__global__ void k_batch_zero_stat(uint32_t *mstat, uint32_t M, DevState *ds)
 {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < M) mstat[i] = 0;
+    if (i == 0) {
+        ds->n_ins = ds->n_add = ds->n_del = 0;
+        ds->ntouched = 0;
+    }
 }
 
 }  // namespace
@@ -641,7 +652,7 @@ dcsr_status dcsr_batch_step(dcsr_batch b, const int64_t *tri_off, const int32_t 
     h->last_stream = s;
     if (!h->canonical) DCSR_TRY(canonicalize(h, s));
     if (do_update) DCSR_TRY(batch_reserve(b, n, s));
-    k_batch_zero_stat<<<grid_for(b->M, 256), 256, 0, s>>>(b->mstat, b->M);
+    k_batch_zero_stat<<<grid_for(b->M, 256), 256, 0, s>>>(b->mstat, b->M, h->ds);
     if (flags & DCSR_BATCH_GRAPH) {
         const void *key[13] = {tri_off, rows, cols, vals, x, y, h->pool_col, (const void *)(intptr_t)b->team,
                                (const void *)(intptr_t)b->thresh, (const void *)(intptr_t)(do_update * 2 + do_spmv),
@@ -675,13 +686,22 @@ dcsr_status dcsr_batch_step(dcsr_batch b, const int64_t *tri_off, const int32_t 
 }
 
 // Status of the last step: OR of the per-matrix bits (syncs); `per_matrix`
-// (host, M entries) receives them when not NULL.
+// (host, M entries) receives them when not NULL.  Also refreshes the handle's
+// counters (nnz, inserted / added / deleted entries and touched rows of the
+// step, summed over the matrices).
 dcsr_status dcsr_batch_status(dcsr_batch b, uint32_t *per_matrix)
 {
     if (!b) return DCSR_ERR_INVALID;
     std::vector<uint32_t> st(b->M);
     DCSR_HIP(hipMemcpyAsync(st.data(), b->mstat, b->M * 4, hipMemcpyDeviceToHost, b->h->last_stream));
     DCSR_HIP(hipStreamSynchronize(b->h->last_stream));
+    // the step's counters (summed over matrices) become the handle's
+    // "last update" figures reported by dcsr_query
+    DCSR_TRY(pull_state(b->h, b->h->last_stream));
+    b->h->last_ins = b->h->hs->n_ins;
+    b->h->last_add = b->h->hs->n_add;
+    b->h->last_del = b->h->hs->n_del;
+    b->h->last_touched = b->h->hs->ntouched;
     uint32_t any = 0;
     for (uint32_t i = 0; i < b->M; ++i) {
         any |= st[i];

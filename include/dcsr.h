@@ -134,9 +134,11 @@ dcsr_status dcsr_update(dcsr_handle h, const int32_t *rows, const int32_t *cols,
  * release retired chunks.  Replaces device::SortMatrixRow (include/load.h:404-581). */
 dcsr_status dcsr_compact(dcsr_handle h, dcsr_stream_t stream);
 
-/* Rebuild the row-length classification used by SpMV/SpMM (hub-row list).
- * Replaces device::BinRows -> SetBins (include/load.h:583-610); called
- * implicitly by the compute entry points after any structural change. */
+/* Rebuild what SpMV/SpMM/SpGEMM derive from the row lengths: the flat
+ * exclusive offsets of the live rows, the 64-nonzero granule -> row map and
+ * the count of hub rows (longer than 4096).  Replaces device::BinRows ->
+ * SetBins (include/load.h:583-610); the compute entry points call it
+ * implicitly after any structural change, so calling it is optional. */
 dcsr_status dcsr_rebin(dcsr_handle h, dcsr_stream_t stream);
 
 /* y = alpha * A x + beta * y  (beta == 0: y is overwritten, never read).
