@@ -1,0 +1,143 @@
+// lookback.h -- single-pass prefix machinery (decoupled look-back) for gfx950.
+//
+// A kernel that needs the exclusive prefix of per-tile aggregates takes its
+// tile index from a ticket counter (so every lower tile has already started
+// and will publish without waiting on anything), publishes its aggregate,
+// looks back over its predecessors' published words until it meets an
+// inclusive prefix, then publishes its own inclusive prefix.  A status word
+// packs (epoch, flag) in the high half and the 32-bit value in the low half,
+// so one 64-bit agent-scope atomic store publishes both and readers need no
+// fence.  Epochs are unique per launch (process-wide counter), so status
+// arrays are never cleared; words from older launches read as "not ready".
+//
+// Every wait is bounded: after LB_SPIN_LIMIT polls a tile gives up, counts
+// the fault in ctl[2] of its launch (never reset; 0 in a correct run) and
+// proceeds, so no launch can hang the device.
+//
+// Control words ctl[0..3] sit at the head of the launch's workspace:
+// ctl[0] ticket, ctl[1] finished workgroups, ctl[2] fault count.  The ticket
+// and finish counters are self-resetting (the last workgroup to finish a
+// launch zeroes them); they start at zero because workspace buffers are
+// zero-filled on allocation.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include "wave.h"
+
+#define LB_AGG 1u
+#define LB_INC 2u
+#define LB_SPIN_LIMIT (1 << 20)
+
+__device__ __forceinline__ uint32_t lb_tag(uint32_t epoch) { return (epoch & 0x3fffffffu) << 2; }
+
+__device__ __forceinline__ void lb_publish(uint64_t *p, uint32_t tag, uint32_t flag, uint32_t v)
+{
+    const uint64_t w = ((uint64_t)(tag | flag) << 32) | (uint64_t)v;
+    __hip_atomic_store(p, w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__device__ __forceinline__ uint64_t lb_load(const uint64_t *p)
+{
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__device__ __forceinline__ bool lb_ready(uint64_t w, uint32_t tag)
+{
+    const uint32_t hi = (uint32_t)(w >> 32);
+    return (hi & ~3u) == tag && (hi & 3u) != 0u;
+}
+
+__device__ __forceinline__ bool lb_inclusive(uint64_t w, uint32_t tag)
+{
+    const uint32_t hi = (uint32_t)(w >> 32);
+    return (hi & ~3u) == tag && (hi & 3u) == LB_INC;
+}
+
+// Ticket of this workgroup (thread 0 draws; broadcast through LDS).
+__device__ __forceinline__ uint32_t lb_ticket(unsigned int *ctl, uint32_t *lds_slot)
+{
+    if (threadIdx.x == 0) *lds_slot = atomicAdd(&ctl[0], 1u);
+    __syncthreads();
+    const uint32_t t = *lds_slot;
+    __syncthreads();
+    return t;
+}
+
+// Called by thread 0 of every workgroup once it has finished all its
+// reads and writes of the launch's shared state: the last one out resets
+// the counters for the next launch.
+__device__ __forceinline__ void lb_finish(unsigned int *ctl)
+{
+    __threadfence();
+    if (atomicAdd(&ctl[1], 1u) == gridDim.x - 1) {
+        ctl[0] = 0u;
+        ctl[1] = 0u;
+        __threadfence();
+    }
+}
+
+// Exclusive prefix of tile `tile` (> 0) over status words st[j * stride]
+// (j < tile), computed by one whole wave: 64 predecessors are examined per
+// round, nearest first.  Returns the same value in every lane.
+__device__ __forceinline__ uint32_t lb_wave_lookback(const uint64_t *st, int64_t stride, int64_t tile, uint32_t tag,
+                                                     unsigned int *ctl)
+{
+    const int lane = lane_id();
+    uint32_t prefix = 0;
+    int64_t j = tile - 1;
+    int spins = 0;
+    while (j >= 0) {
+        const int64_t idx = j - lane;
+        uint64_t w = 0;
+        for (;;) {
+            if (idx >= 0) w = lb_load(&st[idx * stride]);
+            const bool ok = idx < 0 || lb_ready(w, tag);
+            if (__all(ok)) break;
+            if (++spins > LB_SPIN_LIMIT) {
+                if (lane == 0) atomicAdd(&ctl[2], 1u);
+                break;
+            }
+            __builtin_amdgcn_s_sleep(1);
+        }
+        const bool inc = idx >= 0 && lb_inclusive(w, tag);
+        const uint64_t im = __ballot(inc);
+        uint32_t v = idx >= 0 ? (uint32_t)w : 0u;
+        if (im) {
+            const int L = __ffsll((long long)im) - 1;
+            if (lane > L) v = 0u;
+            prefix += wave_sum_u32(v);
+            break;
+        }
+        prefix += wave_sum_u32(v);
+        j -= 64;
+    }
+    return prefix;
+}
+
+// Exclusive prefix of tile `tile` (> 0) over st[j * stride], walked by ONE
+// thread (used when every thread of a workgroup resolves its own column,
+// e.g. one radix digit each).
+__device__ __forceinline__ uint32_t lb_thread_lookback(const uint64_t *st, int64_t stride, int64_t tile, uint32_t tag,
+                                                       unsigned int *ctl)
+{
+    uint32_t prefix = 0;
+    int spins = 0;
+    for (int64_t j = tile - 1; j >= 0; --j) {
+        uint64_t w;
+        for (;;) {
+            w = lb_load(&st[j * stride]);
+            if (lb_ready(w, tag)) break;
+            if (++spins > LB_SPIN_LIMIT) {
+                atomicAdd(&ctl[2], 1u);
+                return prefix;
+            }
+            __builtin_amdgcn_s_sleep(1);
+        }
+        prefix += (uint32_t)w;
+        if (lb_inclusive(w, tag)) break;
+    }
+    return prefix;
+}
+
+// Host: a fresh epoch for one launch (process-wide, thread-safe).
+uint32_t lb_next_epoch();

@@ -6,7 +6,8 @@
 
 // Exclusive prefix sum of NC interleaved uint32 components (in[i*NC + c]),
 // n elements, in place allowed.  `total` (device, NC words) receives the sums.
-// `part` is scratch of scan_part_words(n, NC) words.
+// `part` is scratch of scan_part_words(n, NC) words, zero-filled when first
+// allocated; calls that share a `part` buffer must be stream-ordered.
 size_t scan_part_words(int64_t n, int nc);
 hipError_t scan_exclusive_u32(const uint32_t *in, uint32_t *out, int64_t n, int nc, uint32_t *part,
                               uint32_t *total, hipStream_t s);
@@ -14,7 +15,9 @@ hipError_t scan_exclusive_u32(const uint32_t *in, uint32_t *out, int64_t n, int 
 // Stable LSD radix sort of (key, value) pairs on key bits [0, nbits).
 // Ping-pongs between (k0, v0) and (k1, v1); returns in *which the buffer
 // index (0 or 1) holding the result.  `hist` is scratch of
-// radix_hist_words(n) words, `part` scratch of radix_part_words(n) words.
+// radix_hist_words(n) words, zero-filled when first allocated (each sort
+// leaves it as it found it); calls sharing it must be stream-ordered.  `part`
+// is unused (kept for the call sites).
 size_t radix_hist_words(int64_t n);
 size_t radix_part_words(int64_t n);
 hipError_t radix_sort_pairs(uint64_t *k0, uint32_t *v0, uint64_t *k1, uint32_t *v1, int64_t n, int nbits,

@@ -5,6 +5,11 @@
 // include/load.h:602-603, cusp::indices_to_offsets, cub::BlockRadixSort
 // include/spmm.inl:489-490) with wave64 code: DPP scans and ballot-based
 // digit ranking (8 ballots per item give the lanes sharing a digit).
+// Both are single-pass over their tiles: one launch per scan and one per
+// radix pass (plus one histogram launch per sort), with cross-tile prefixes
+// resolved by decoupled look-back (lookback.h).
+#include <atomic>
+#include "lookback.h"
 #include "prims.h"
 #include "wave.h"
 
@@ -40,97 +45,76 @@ __device__ __forceinline__ uint32_t block_excl_scan_256(uint32_t v, uint32_t *ld
     return carry + inc - v;
 }
 
+// Single-pass exclusive scan of NC interleaved components: tile = ticket,
+// local blocked scan, decoupled look-back per component, write.
+// part: [ctl 4 words][status: tiles x NC words of 64 bits].
 template <int NC>
-__global__ __launch_bounds__(SCAN_THREADS) void k_scan_reduce(const uint32_t *__restrict__ in, int64_t n,
-                                                              uint32_t *__restrict__ part)
-{
-    __shared__ uint32_t lds4[4];
-    const int64_t base = (int64_t)blockIdx.x * SCAN_TILE;
-#pragma unroll
-    for (int c = 0; c < NC; ++c) {
-        uint32_t acc = 0;
-#pragma unroll
-        for (int i = 0; i < SCAN_IPT; ++i) {
-            int64_t idx = base + i * SCAN_THREADS + threadIdx.x;
-            if (idx < n) acc += in[idx * NC + c];
-        }
-        uint32_t tot;
-        block_excl_scan_256(acc, lds4, &tot);
-        if (threadIdx.x == 0) part[(int64_t)blockIdx.x * NC + c] = tot;
-    }
-}
-
-// Single workgroup: exclusive scan of the per-tile partials, in place.
-template <int NC>
-__global__ __launch_bounds__(1024) void k_scan_partials(uint32_t *__restrict__ part, int64_t nb,
+__global__ __launch_bounds__(SCAN_THREADS) void k_scan1(const uint32_t *in, uint32_t *out, int64_t n,
+                                                        uint32_t *__restrict__ part, uint32_t epoch,
                                                         uint32_t *__restrict__ total)
 {
-    __shared__ uint32_t lds16[16];
-    __shared__ uint32_t carry_s[NC];
-    if (threadIdx.x < NC) carry_s[threadIdx.x] = 0;
-    __syncthreads();
-    const int w = threadIdx.x >> 6;
-    for (int64_t base = 0; base < nb; base += 1024) {
-        int64_t idx = base + threadIdx.x;
-#pragma unroll
-        for (int c = 0; c < NC; ++c) {
-            uint32_t v = idx < nb ? part[idx * NC + c] : 0u;
-            uint32_t inc = wave_incl_scan_u32(v);
-            if (lane_id() == 63) lds16[w] = inc;
-            __syncthreads();
-            uint32_t pre = 0, tot = 0;
-            for (int i = 0; i < 16; ++i) {
-                uint32_t t = lds16[i];
-                if (i < w) pre += t;
-                tot += t;
-            }
-            uint32_t carry = carry_s[c];
-            if (idx < nb) part[idx * NC + c] = carry + pre + inc - v;
-            __syncthreads();
-            if (threadIdx.x == 0) carry_s[c] = carry + tot;
-            __syncthreads();
-        }
-    }
-    if (threadIdx.x < NC && total) total[threadIdx.x] = carry_s[threadIdx.x];
-}
-
-template <int NC>
-__global__ __launch_bounds__(SCAN_THREADS) void k_scan_down(const uint32_t *__restrict__ in,
-                                                            uint32_t *__restrict__ out, int64_t n,
-                                                            const uint32_t *__restrict__ part)
-{
-    __shared__ uint32_t tile[SCAN_TILE];
+    __shared__ uint32_t tile_s[NC][SCAN_TILE];
     __shared__ uint32_t lds4[4];
-    const int64_t base = (int64_t)blockIdx.x * SCAN_TILE;
+    __shared__ uint32_t s_ticket;
+    __shared__ uint32_t s_agg[NC];
+    __shared__ uint32_t s_pre[NC];
+    unsigned int *ctl = part;
+    uint64_t *st = (uint64_t *)(part + 4);
+    const uint32_t tag = lb_tag(epoch);
+    const int64_t tile = lb_ticket(ctl, &s_ticket);
+    const int64_t ntiles = (n + SCAN_TILE - 1) / SCAN_TILE;
+    const int64_t base = tile * SCAN_TILE;
 #pragma unroll
     for (int c = 0; c < NC; ++c) {
-        // striped coalesced load
 #pragma unroll
         for (int i = 0; i < SCAN_IPT; ++i) {
-            int64_t idx = base + i * SCAN_THREADS + threadIdx.x;
-            tile[i * SCAN_THREADS + threadIdx.x] = idx < n ? in[idx * NC + c] : 0u;
+            const int64_t idx = base + i * SCAN_THREADS + threadIdx.x;
+            tile_s[c][i * SCAN_THREADS + threadIdx.x] = idx < n ? in[idx * NC + c] : 0u;
         }
-        __syncthreads();
-        // blocked scan: thread t owns items [t*IPT, t*IPT+IPT)
+    }
+    __syncthreads();
+#pragma unroll
+    for (int c = 0; c < NC; ++c) {
         uint32_t loc[SCAN_IPT];
-        uint32_t s = 0;
+        uint32_t sum = 0;
 #pragma unroll
         for (int i = 0; i < SCAN_IPT; ++i) {
-            loc[i] = s;
-            s += tile[threadIdx.x * SCAN_IPT + i];
+            loc[i] = sum;
+            sum += tile_s[c][threadIdx.x * SCAN_IPT + i];
         }
         uint32_t tot;
-        uint32_t pre = block_excl_scan_256(s, lds4, &tot) + part[(int64_t)blockIdx.x * NC + c];
+        const uint32_t pre = block_excl_scan_256(sum, lds4, &tot);
 #pragma unroll
-        for (int i = 0; i < SCAN_IPT; ++i) tile[threadIdx.x * SCAN_IPT + i] = pre + loc[i];
-        __syncthreads();
+        for (int i = 0; i < SCAN_IPT; ++i) tile_s[c][threadIdx.x * SCAN_IPT + i] = pre + loc[i];
+        if (threadIdx.x == 0) {
+            s_agg[c] = tot;
+            lb_publish(&st[tile * NC + c], tag, tile == 0 ? LB_INC : LB_AGG, tot);
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x < 64) {
+#pragma unroll
+        for (int c = 0; c < NC; ++c) {
+            const uint32_t p = tile == 0 ? 0u : lb_wave_lookback(st + c, NC, tile, tag, ctl);
+            if (threadIdx.x == 0) {
+                s_pre[c] = p;
+                if (tile > 0) lb_publish(&st[tile * NC + c], tag, LB_INC, p + s_agg[c]);
+                if (tile == ntiles - 1 && total) total[c] = p + s_agg[c];
+            }
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int c = 0; c < NC; ++c) {
+        const uint32_t p = s_pre[c];
 #pragma unroll
         for (int i = 0; i < SCAN_IPT; ++i) {
-            int64_t idx = base + i * SCAN_THREADS + threadIdx.x;
-            if (idx < n) out[idx * NC + c] = tile[i * SCAN_THREADS + threadIdx.x];
+            const int64_t idx = base + i * SCAN_THREADS + threadIdx.x;
+            if (idx < n) out[idx * NC + c] = p + tile_s[c][i * SCAN_THREADS + threadIdx.x];
         }
-        __syncthreads();
     }
+    __syncthreads();
+    if (threadIdx.x == 0) lb_finish(ctl);
 }
 
 // ---------------------------------------------------------------------------
@@ -142,26 +126,45 @@ constexpr int RS_IPT = 8;                        // items per lane
 constexpr int RS_WAVE_ITEMS = RS_IPT * 64;       // contiguous items per wave
 constexpr int RS_TILE = RS_WAVES * RS_WAVE_ITEMS; // 2048
 
-__global__ __launch_bounds__(RS_THREADS) void k_rs_hist(const uint64_t *__restrict__ keys, int64_t n, int shift,
-                                                        uint32_t *__restrict__ hist, int64_t nblocks)
+constexpr int RS_MAXPASS = 8;  // up to 64 key bits
+
+// Global digit histograms of every pass in one read of the keys (the count of
+// each digit value does not depend on the order the earlier passes leave).
+// ghist: [npass][256], zero on entry (the last pass of the previous sort
+// cleared it).
+__global__ __launch_bounds__(RS_THREADS) void k_rs_hist_all(const uint64_t *__restrict__ keys, int64_t n, int npass,
+                                                            uint32_t *__restrict__ ghist)
 {
-    __shared__ uint32_t cnt[256];
-    cnt[threadIdx.x] = 0;
+    __shared__ uint32_t cnt[RS_MAXPASS][256];
+    for (int i = threadIdx.x; i < RS_MAXPASS * 256; i += RS_THREADS) (&cnt[0][0])[i] = 0;
     __syncthreads();
     const int64_t base = (int64_t)blockIdx.x * RS_TILE;
     for (int i = threadIdx.x; i < RS_TILE; i += RS_THREADS) {
-        int64_t idx = base + i;
-        if (idx < n) atomicAdd(&cnt[(uint32_t)(keys[idx] >> shift) & 255u], 1u);
+        const int64_t idx = base + i;
+        if (idx < n) {
+            const uint64_t k = keys[idx];
+            for (int p = 0; p < npass; ++p) atomicAdd(&cnt[p][(uint32_t)(k >> (8 * p)) & 255u], 1u);
+        }
     }
     __syncthreads();
-    hist[(int64_t)threadIdx.x * nblocks + blockIdx.x] = cnt[threadIdx.x];
+    for (int p = 0; p < npass; ++p) {
+        const uint32_t c = cnt[p][threadIdx.x];
+        if (c) atomicAdd(&ghist[p * 256 + threadIdx.x], c);
+    }
 }
 
-__global__ __launch_bounds__(RS_THREADS) void k_rs_scatter(const uint64_t *__restrict__ kin,
-                                                           const uint32_t *__restrict__ vin,
-                                                           uint64_t *__restrict__ kout, uint32_t *__restrict__ vout,
-                                                           int64_t n, int shift,
-                                                           const uint32_t *__restrict__ offs, int64_t nblocks)
+// One stable scatter pass (onesweep): tile = ticket; rank the tile's keys by
+// digit inside the workgroup (wave ballots, wave-major order keeps
+// stability); per digit, publish the tile's count and look back over lower
+// tiles for the digit's running offset; the digit's base is the exclusive
+// scan of the global histogram.  The last workgroup of the last pass clears
+// ghist for the next sort.
+// ws: [ctl 4 words][ghist npass x 256][status: tiles x 256 words of 64 bits]
+__global__ __launch_bounds__(RS_THREADS) void k_rs_onesweep(const uint64_t *__restrict__ kin,
+                                                            const uint32_t *__restrict__ vin,
+                                                            uint64_t *__restrict__ kout, uint32_t *__restrict__ vout,
+                                                            int64_t n, int pass, int npass, uint32_t *__restrict__ ws,
+                                                            uint32_t epoch)
 {
     __shared__ uint64_t skey[RS_TILE];
     __shared__ uint32_t sval[RS_TILE];
@@ -169,13 +172,20 @@ __global__ __launch_bounds__(RS_THREADS) void k_rs_scatter(const uint64_t *__res
     __shared__ uint32_t loff[256];
     __shared__ uint32_t goff[256];
     __shared__ uint32_t lds4[4];
+    __shared__ uint32_t s_ticket;
 
+    unsigned int *ctl = ws;
+    uint32_t *ghist = ws + 4;
+    uint64_t *st = (uint64_t *)(ws + 4 + RS_MAXPASS * 256);
+    const uint32_t tag = lb_tag(epoch);
+    const int shift = 8 * pass;
     const int w = threadIdx.x >> 6;
     const int lane = lane_id();
+    const int64_t tile = lb_ticket(ctl, &s_ticket);
     for (int i = threadIdx.x; i < RS_WAVES * 256; i += RS_THREADS) (&wcnt[0][0])[i] = 0;
     __syncthreads();
 
-    const int64_t base = (int64_t)blockIdx.x * RS_TILE;
+    const int64_t base = tile * RS_TILE;
     const int64_t wbase = base + (int64_t)w * RS_WAVE_ITEMS;
     uint64_t key[RS_IPT];
     uint32_t val[RS_IPT];
@@ -183,13 +193,18 @@ __global__ __launch_bounds__(RS_THREADS) void k_rs_scatter(const uint64_t *__res
     const uint64_t lt = lanemask_lt();
 #pragma unroll
     for (int j = 0; j < RS_IPT; ++j) {
-        int64_t idx = wbase + j * 64 + lane;
-        bool ok = idx < n;
+        const int64_t idx = wbase + j * 64 + lane;
+        const bool ok = idx < n;
         key[j] = ok ? kin[idx] : ~0ull;
         val[j] = ok ? vin[idx] : 0u;
-        uint32_t d = (uint32_t)(key[j] >> shift) & 255u;
-        uint64_t peers = wave_match_digit8(d, ~0ull);
-        int leader = __ffsll((long long)peers) - 1;
+    }
+    // Items past n carry key ~0 (digit 255): only the last tile has them,
+    // they rank after every valid item of the tile, and no tile follows.
+#pragma unroll
+    for (int j = 0; j < RS_IPT; ++j) {
+        const uint32_t d = (uint32_t)(key[j] >> shift) & 255u;
+        const uint64_t peers = wave_match_digit8(d, ~0ull);
+        const int leader = __ffsll((long long)peers) - 1;
         uint32_t b = 0;
         if (lane == leader) {
             b = wcnt[w][d];
@@ -202,41 +217,66 @@ __global__ __launch_bounds__(RS_THREADS) void k_rs_scatter(const uint64_t *__res
     {
         const int d = threadIdx.x;  // one digit per thread
         uint32_t c[RS_WAVES];
-        uint32_t s = 0;
+        uint32_t sum = 0;
 #pragma unroll
         for (int i = 0; i < RS_WAVES; ++i) {
             c[i] = wcnt[i][d];
-            wcnt[i][d] = s;
-            s += c[i];
+            wcnt[i][d] = sum;
+            sum += c[i];
         }
         uint32_t tot;
-        loff[d] = block_excl_scan_256(s, lds4, &tot);
-        goff[d] = offs[(int64_t)d * nblocks + blockIdx.x];
+        loff[d] = block_excl_scan_256(sum, lds4, &tot);
+        // this tile's count of digit d, then its running offset
+        lb_publish(&st[tile * 256 + d], tag, tile == 0 ? LB_INC : LB_AGG, sum);
+        const uint32_t g = ghist[pass * 256 + d];
+        uint32_t gtot;
+        const uint32_t gbase = block_excl_scan_256(g, lds4, &gtot);
+        const uint32_t run = tile == 0 ? 0u : lb_thread_lookback(st + d, 256, tile, tag, ctl);
+        if (tile > 0) lb_publish(&st[tile * 256 + d], tag, LB_INC, run + sum);
+        goff[d] = gbase + run;
     }
     __syncthreads();
 #pragma unroll
     for (int j = 0; j < RS_IPT; ++j) {
-        uint32_t d = (uint32_t)(key[j] >> shift) & 255u;
-        uint32_t p = loff[d] + wcnt[w][d] + rnk[j];
+        const uint32_t d = (uint32_t)(key[j] >> shift) & 255u;
+        const uint32_t p = loff[d] + wcnt[w][d] + rnk[j];
         skey[p] = key[j];
         sval[p] = val[j];
     }
     __syncthreads();
     const int64_t valid = (n - base) < RS_TILE ? (n - base) : RS_TILE;
     for (int p = threadIdx.x; p < valid; p += RS_THREADS) {
-        uint64_t k = skey[p];
-        uint32_t d = (uint32_t)(k >> shift) & 255u;
-        int64_t o = (int64_t)goff[d] + (p - (int64_t)loff[d]);
+        const uint64_t k = skey[p];
+        const uint32_t d = (uint32_t)(k >> shift) & 255u;
+        const int64_t o = (int64_t)goff[d] + (p - (int64_t)loff[d]);
         kout[o] = k;
         vout[o] = sval[p];
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        __threadfence();
+        if (atomicAdd(&ctl[1], 1u) == gridDim.x - 1) {
+            if (pass == npass - 1)
+                for (int i = 0; i < npass * 256; ++i) ghist[i] = 0u;
+            ctl[0] = 0u;
+            ctl[1] = 0u;
+            __threadfence();
+        }
     }
 }
 
 }  // namespace
 
+uint32_t lb_next_epoch()
+{
+    static std::atomic<uint32_t> epoch{1};
+    return epoch.fetch_add(1u, std::memory_order_relaxed);
+}
+
 size_t scan_part_words(int64_t n, int nc)
 {
-    return (size_t)((n + SCAN_TILE - 1) / SCAN_TILE + 1) * nc;
+    const int64_t tiles = (n + SCAN_TILE - 1) / SCAN_TILE;
+    return (size_t)(4 + 2 * (tiles < 1 ? 1 : tiles) * nc);
 }
 
 hipError_t scan_exclusive_u32(const uint32_t *in, uint32_t *out, int64_t n, int nc, uint32_t *part,
@@ -246,43 +286,39 @@ hipError_t scan_exclusive_u32(const uint32_t *in, uint32_t *out, int64_t n, int 
         if (total) k_fill_u32<<<1, 64, 0, s>>>(total, nc, 0u);
         return hipGetLastError();
     }
-    int64_t nb = (n + SCAN_TILE - 1) / SCAN_TILE;
-    if (nc == 1) {
-        k_scan_reduce<1><<<(unsigned)nb, SCAN_THREADS, 0, s>>>(in, n, part);
-        k_scan_partials<1><<<1, 1024, 0, s>>>(part, nb, total);
-        k_scan_down<1><<<(unsigned)nb, SCAN_THREADS, 0, s>>>(in, out, n, part);
-    } else if (nc == 2) {
-        k_scan_reduce<2><<<(unsigned)nb, SCAN_THREADS, 0, s>>>(in, n, part);
-        k_scan_partials<2><<<1, 1024, 0, s>>>(part, nb, total);
-        k_scan_down<2><<<(unsigned)nb, SCAN_THREADS, 0, s>>>(in, out, n, part);
-    } else {
+    const int64_t nb = (n + SCAN_TILE - 1) / SCAN_TILE;
+    const uint32_t epoch = lb_next_epoch();
+    if (nc == 1)
+        k_scan1<1><<<(unsigned)nb, SCAN_THREADS, 0, s>>>(in, out, n, part, epoch, total);
+    else if (nc == 2)
+        k_scan1<2><<<(unsigned)nb, SCAN_THREADS, 0, s>>>(in, out, n, part, epoch, total);
+    else
         return hipErrorInvalidValue;
-    }
     return hipGetLastError();
 }
 
 size_t radix_hist_words(int64_t n)
 {
-    int64_t nb = (n + RS_TILE - 1) / RS_TILE;
-    return (size_t)(nb < 1 ? 1 : nb) * 256;
+    const int64_t nb = (n + RS_TILE - 1) / RS_TILE;
+    return (size_t)(4 + RS_MAXPASS * 256 + 2 * 256 * (nb < 1 ? 1 : nb));
 }
 
-size_t radix_part_words(int64_t n) { return scan_part_words((int64_t)radix_hist_words(n), 1); }
+size_t radix_part_words(int64_t) { return 4; }
 
 hipError_t radix_sort_pairs(uint64_t *k0, uint32_t *v0, uint64_t *k1, uint32_t *v1, int64_t n, int nbits,
-                            uint32_t *hist, uint32_t *part, int *which, hipStream_t s)
+                            uint32_t *hist, uint32_t *, int *which, hipStream_t s)
 {
     *which = 0;
     if (n <= 1) return hipSuccess;
-    int64_t nb = (n + RS_TILE - 1) / RS_TILE;
+    const int npass = (nbits + 7) / 8;
+    if (npass > RS_MAXPASS) return hipErrorInvalidValue;
+    const int64_t nb = (n + RS_TILE - 1) / RS_TILE;
+    k_rs_hist_all<<<(unsigned)nb, RS_THREADS, 0, s>>>(k0, n, npass, hist + 4);
     uint64_t *ki = k0, *ko = k1;
     uint32_t *vi = v0, *vo = v1;
     int cur = 0;
-    for (int shift = 0; shift < nbits; shift += 8) {
-        k_rs_hist<<<(unsigned)nb, RS_THREADS, 0, s>>>(ki, n, shift, hist, nb);
-        hipError_t e = scan_exclusive_u32(hist, hist, nb * 256, 1, part, nullptr, s);
-        if (e != hipSuccess) return e;
-        k_rs_scatter<<<(unsigned)nb, RS_THREADS, 0, s>>>(ki, vi, ko, vo, n, shift, hist, nb);
+    for (int pass = 0; pass < npass; ++pass) {
+        k_rs_onesweep<<<(unsigned)nb, RS_THREADS, 0, s>>>(ki, vi, ko, vo, n, pass, npass, hist, lb_next_epoch());
         uint64_t *tk = ki; ki = ko; ko = tk;
         uint32_t *tv = vi; vi = vo; vo = tv;
         cur ^= 1;
