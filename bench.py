@@ -20,6 +20,10 @@
 --workload spgemm: 2,000,000 x 2,000,000, Poisson(8) nnz per row, N(0,1)
   values; one 4M-triple update, then one step = C = A * A.
 
+--workload spmv: SpMV and SpMM (k = 4, 16, 32) on a resident 10M x 10M
+  power-law matrix with 160M nnz (HBM-resident: A alone is 1.3 GB, far
+  beyond the 256 MB last-level cache); one step = one SpMV.
+
 --workload serving: resident power-law matrix; a Poisson stream of update
   batches (log-uniform 1k..1M triples, 30% deletions by -current) and
   SpMV/SpMM queries (k in {1, 4, 16, 32}); the server (continuous batching)
@@ -51,8 +55,8 @@ R_CFG = 325_557
 NNZ_CFG = 3_200_000
 MAXDEG = 2700
 BATCH = 1_000_000
-WORKLOADS = ("powerlaw", "batched", "spgemm", "serving")
-DEFAULT_STEPS = {"powerlaw": 8, "batched": 10, "spgemm": 3, "serving": 400}
+WORKLOADS = ("powerlaw", "batched", "spgemm", "serving", "spmv")
+DEFAULT_STEPS = {"powerlaw": 8, "batched": 10, "spgemm": 3, "serving": 400, "spmv": 20}
 
 
 def parse():
@@ -65,6 +69,8 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-sample-batches", type=int, default=8)
     ap.add_argument("--cpu-min-seconds", type=float, default=10.0)
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run a plain 256 MiB device copy three times (calibrates HBM traffic counters)")
     ap.add_argument("--serving-rows", type=int, default=10_000_000)
     ap.add_argument("--serving-nnz", type=int, default=160_000_000)
     args = ap.parse_args()
@@ -566,12 +572,61 @@ def bench_serving(args):
          cpu_baseline=None)
 
 
+def bench_spmv(args):
+    import torch
+    import dcsr_amd
+
+    R, N = args.serving_rows, args.serving_nnz
+    r, c, v = serving_matrix(R, N)
+    dev = torch.device("cuda:0")
+    m = dcsr_amd.DcsrMatrix(R, R, pool_factor=1.0)
+    m.LoadMatrix(*(torch.from_numpy(a).to(dev) for a in (r, c, v)))
+    nnz = int(r.size)
+    del r, c, v
+    m.BinRows()
+    stream = torch.cuda.current_stream()
+    g = torch.Generator(device="cuda").manual_seed(9)
+    res = {}
+    for k in (1, 4, 16, 32):
+        X = torch.rand((R, k), generator=g, device=dev)
+        Y = torch.empty((R, k), device=dev)
+        for _ in range(args.warmup):
+            m.spmv(X[:, 0], Y[:, 0]) if k == 1 else m.spmm(X, Y)
+        reps = args.steps if k == 1 else max(args.steps // 2, 3)
+        ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(reps)]
+        for e0, e1 in ev:
+            e0.record(stream)
+            m.spmv(X[:, 0], Y[:, 0]) if k == 1 else m.spmm(X, Y)
+            e1.record(stream)
+        torch.cuda.synchronize()
+        ms = float(np.median([e0.elapsed_time(e1) for e0, e1 in ev]))
+        b = spmv_bytes(R, R, nnz) if k == 1 else spmm_bytes(R, R, nnz, k)
+        res[k] = {"ms": ms, "gbs": b / 1e9 / (ms / 1e3), "hbm_frac": b / 1e9 / (ms / 1e3) / HBM_PEAK_GBS}
+        del X, Y
+    gbs = res[1]["gbs"]
+    line(args, gbs, "GB/s", res[1]["ms"],
+         {"workload": "spmv-powerlaw-10M", "rows": R, "nnz": nnz, "zipf": 2.1},
+         spmv_hbm_frac=res[1]["hbm_frac"],
+         spmm={str(k): res[k] for k in (4, 16, 32)},
+         roofline=roofline(gbs, "flat SpMV (dcsr_spmv: flat kernel + carry)"),
+         cpu_baseline=None)
+
+
 def main():
     args = parse()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); this engine runs on a single MI355X.",
               file=sys.stderr)
         sys.exit(2)
+    if args.copy_calibration:
+        import torch
+        x = torch.rand(64 * 1024 * 1024, device="cuda")
+        y = torch.empty_like(x)
+        for _ in range(3):
+            y.copy_(x)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_calibration_bytes": x.numel() * 4, "launches": 3}))
+        return
     import dcsr_amd
     dcsr_amd.lib()
     if args.workload == "powerlaw":
@@ -580,6 +635,8 @@ def main():
         bench_batched(args)
     elif args.workload == "spgemm":
         bench_spgemm(args)
+    elif args.workload == "spmv":
+        bench_spmv(args)
     else:
         bench_serving(args)
 

@@ -85,6 +85,10 @@ def lib():
         return _lib
     if not os.path.exists(LIB_PATH):
         raise ImportError(f"{LIB_PATH} is missing: run python -m dcsr_amd.build (no CPU fallback exists)")
+    # torch ships its own HIP runtime under the same SONAME (libamdhip64.so.7):
+    # whichever is loaded first serves the whole process, so load torch's
+    # before this library pulls in the system one
+    import torch  # noqa: F401
     L = ctypes.CDLL(LIB_PATH)
     sig = {
         "dcsr_status_string": (ctypes.c_char_p, [ctypes.c_int]),

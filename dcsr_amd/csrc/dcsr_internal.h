@@ -34,14 +34,15 @@ struct DevState {
     unsigned long long need;        // current update: claimed capacity
     unsigned int status;            // sticky ST_* bits
     unsigned int go;                // 1 if the current update may commit
-    unsigned int nuniq;             // \ distinct coordinates in the batch
-    unsigned int ntouched;          // /  rows touched (adjacent: one 2-word scan total)
-    unsigned int claim_total;       // \ pool slots claimed by rewritten rows
-    unsigned int nrewrite_old;      // /  old entries moved (adjacent: one 2-word scan total)
+    unsigned int nuniq;             // distinct coordinates in the batch
+    unsigned int ntouched;          // rows touched
+    unsigned int claim_total;       // pool slots claimed by rewritten rows
+    unsigned int nrewrite_old;      // old entries moved
     unsigned int hub_count;         // entries in the hub list
     unsigned int hub_chunks;        // total hub chunks
     unsigned int ubad;              // current batch holds an out-of-range index
     unsigned int hub_touch;         // current update: rewritten rows that were/are hubs
+    unsigned int ubad_last;         // current update's batch held an out-of-range index
 };
 
 // Grow-only device scratch buffers owned by a handle.

@@ -139,5 +139,30 @@ __device__ __forceinline__ uint32_t lb_thread_lookback(const uint64_t *st, int64
     return prefix;
 }
 
+// Tile prefix of NC components in one call: thread 0 publishes the tile's
+// aggregates s_agg[0..NC) (LDS, written and synchronised by the caller),
+// wave 0 resolves the exclusive prefixes into s_pre[0..NC) (LDS) and
+// publishes the inclusive ones.  Ends with a __syncthreads.
+template <int NC>
+__device__ __forceinline__ void lb_tile_prefix(const uint32_t *s_agg, uint32_t *s_pre, uint64_t *st, int64_t tile,
+                                               uint32_t tag, unsigned int *ctl)
+{
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int c = 0; c < NC; ++c) lb_publish(&st[tile * NC + c], tag, tile == 0 ? LB_INC : LB_AGG, s_agg[c]);
+    }
+    if (threadIdx.x < 64) {
+#pragma unroll
+        for (int c = 0; c < NC; ++c) {
+            const uint32_t p = tile == 0 ? 0u : lb_wave_lookback(st + c, NC, tile, tag, ctl);
+            if (threadIdx.x == 0) {
+                s_pre[c] = p;
+                if (tile > 0) lb_publish(&st[tile * NC + c], tag, LB_INC, p + s_agg[c]);
+            }
+        }
+    }
+    __syncthreads();
+}
+
 // Host: a fresh epoch for one launch (process-wide, thread-safe).
 uint32_t lb_next_epoch();

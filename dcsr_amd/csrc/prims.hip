@@ -86,24 +86,11 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_scan1(const uint32_t *in, uint
         const uint32_t pre = block_excl_scan_256(sum, lds4, &tot);
 #pragma unroll
         for (int i = 0; i < SCAN_IPT; ++i) tile_s[c][threadIdx.x * SCAN_IPT + i] = pre + loc[i];
-        if (threadIdx.x == 0) {
-            s_agg[c] = tot;
-            lb_publish(&st[tile * NC + c], tag, tile == 0 ? LB_INC : LB_AGG, tot);
-        }
+        if (threadIdx.x == 0) s_agg[c] = tot;
     }
     __syncthreads();
-    if (threadIdx.x < 64) {
-#pragma unroll
-        for (int c = 0; c < NC; ++c) {
-            const uint32_t p = tile == 0 ? 0u : lb_wave_lookback(st + c, NC, tile, tag, ctl);
-            if (threadIdx.x == 0) {
-                s_pre[c] = p;
-                if (tile > 0) lb_publish(&st[tile * NC + c], tag, LB_INC, p + s_agg[c]);
-                if (tile == ntiles - 1 && total) total[c] = p + s_agg[c];
-            }
-        }
-    }
-    __syncthreads();
+    lb_tile_prefix<NC>(s_agg, s_pre, st, tile, tag, ctl);
+    if (threadIdx.x < NC && tile == ntiles - 1 && total) total[threadIdx.x] = s_pre[threadIdx.x] + s_agg[threadIdx.x];
 #pragma unroll
     for (int c = 0; c < NC; ++c) {
         const uint32_t p = s_pre[c];

@@ -22,6 +22,7 @@
 //   6. move surviving old entries and write inserted ones at their merged
 //      positions, then commit the row table.
 #include "dcsr_internal.h"
+#include "lookback.h"
 #include "prims.h"
 #include "segments.h"
 
@@ -32,18 +33,24 @@ enum { C_NOP = 0, C_ADD = 1, C_DEL = 2, C_INS = 3 };
 enum { M_VALUES = 0, M_REWRITE = 1 };
 constexpr int NPLAN = 5;  // per-block plan counters
 
-__global__ void k_upd_begin(DevState *ds)
+// Block-wide exclusive scan of one uint32 per thread (256 threads);
+// *block_total receives the block sum.
+__device__ __forceinline__ uint32_t block_excl_scan_256(uint32_t v, uint32_t *lds4, uint32_t *block_total)
 {
-    ds->d_nnz = 0;
-    ds->d_garbage = 0;
-    ds->n_ins = ds->n_add = ds->n_del = 0;
-    ds->d_hubs = 0;
-    ds->d_hub_chunks = 0;
-    ds->ubad = 0;
-    ds->go = 0;
-    ds->need = 0;
-    ds->hub_touch = 0;
-    ds->bump_saved = ds->bump;
+    const int w = threadIdx.x >> 6;
+    const uint32_t inc = wave_incl_scan_u32(v);
+    if (lane_id() == 63) lds4[w] = inc;
+    __syncthreads();
+    uint32_t carry = 0, tot = 0;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        const uint32_t t = lds4[i];
+        if (i < w) carry += t;
+        tot += t;
+    }
+    __syncthreads();
+    *block_total = tot;
+    return carry + inc - v;
 }
 
 // Invalid indices become ~0 keys: with one key bit above row|col they sort
@@ -63,43 +70,88 @@ __global__ void k_upd_pack(const int32_t *__restrict__ rows, const int32_t *__re
     if (__ballot(bad) && lane_id() == 0) atomicOr(&ds->ubad, 1u);
 }
 
-// flags[2i] = distinct-coordinate head, flags[2i+1] = row head.
-__global__ void k_upd_flags(const uint64_t *__restrict__ key, int64_t n, int cbits, uint32_t *__restrict__ flags)
+// Distinct coordinates and touched rows of the sorted batch, in one pass:
+// per item, head = first of its coordinate run, row head = first of its row;
+// the two counts are prefix-summed across tiles by look-back, and every head
+// writes its compacted record: ucol/usum/ut for coordinate u, trow/tubeg
+// (and zeroed tins/tdel) for touched row t.  Sums run left to right from 0 in
+// batch order, as the host path accumulates B_row[j].  The last tile stores
+// nuniq / ntouched and the tubeg[ntouched] = nuniq sentinel.
+constexpr int HD_IPT = 8;
+constexpr int HD_TILE = TB * HD_IPT;
+__global__ __launch_bounds__(TB) void k_upd_heads(const uint64_t *__restrict__ key, const uint32_t *__restrict__ val,
+                                                  int64_t n, int cbits, uint32_t *__restrict__ ws, uint32_t epoch,
+                                                  DevState *ds, int32_t *__restrict__ ucol, float *__restrict__ usum,
+                                                  uint32_t *__restrict__ ut, uint32_t *__restrict__ trow,
+                                                  uint32_t *__restrict__ tubeg, uint32_t *__restrict__ tins,
+                                                  uint32_t *__restrict__ tdel)
 {
-    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    uint64_t k = key[i];
-    bool valid = k != ~0ull;
-    bool head = valid && (i == 0 || key[i - 1] != k);
-    bool rhead = valid && (i == 0 || (key[i - 1] >> cbits) != (k >> cbits));
-    flags[2 * i] = head ? 1u : 0u;
-    flags[2 * i + 1] = rhead ? 1u : 0u;
-}
-
-// Distinct coordinates: ucol/usum/ut; touched rows: trow/tubeg.
-__global__ void k_upd_compact(const uint64_t *__restrict__ key, const uint32_t *__restrict__ val, int64_t n,
-                              int cbits, const uint32_t *__restrict__ flags, const uint32_t *__restrict__ pre,
-                              int32_t *__restrict__ ucol, float *__restrict__ usum, uint32_t *__restrict__ ut,
-                              uint32_t *__restrict__ trow, uint32_t *__restrict__ tubeg, uint32_t *__restrict__ tins,
-                              uint32_t *__restrict__ tdel)
-{
-    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n || !flags[2 * i]) return;
-    const uint64_t k = key[i];
-    const bool rhead = flags[2 * i + 1] != 0;
-    const uint32_t u = pre[2 * i];
-    const uint32_t t = pre[2 * i + 1] + (rhead ? 0u : 0xffffffffu);
-    float s = 0.0f + __uint_as_float(val[i]);
-    for (int64_t m = i + 1; m < n && key[m] == k; ++m) s = s + __uint_as_float(val[m]);
-    ucol[u] = (int32_t)(k & ((1ull << cbits) - 1ull));
-    usum[u] = s;
-    ut[u] = t;
-    if (rhead) {
-        trow[t] = (uint32_t)(k >> cbits);
-        tubeg[t] = u;
-        tins[t] = 0;
-        tdel[t] = 0;
+    __shared__ uint64_t sk[HD_TILE + 1];
+    __shared__ uint32_t lds4[4];
+    __shared__ uint32_t s_ticket, s_agg[2], s_pre[2];
+    unsigned int *ctl = ws;
+    uint64_t *st = (uint64_t *)(ws + 4);
+    const uint32_t tag = lb_tag(epoch);
+    const int64_t tile = lb_ticket(ctl, &s_ticket);
+    const int64_t ntiles = (n + HD_TILE - 1) / HD_TILE;
+    const int64_t base = tile * HD_TILE;
+    for (int i = threadIdx.x; i < HD_TILE; i += TB) {
+        const int64_t idx = base + i;
+        sk[1 + i] = idx < n ? key[idx] : ~0ull;
     }
+    // the key before the tile; ~0 before item 0 (differs from every valid key
+    // and, shifted, from every valid row)
+    if (threadIdx.x == 0) sk[0] = base > 0 ? key[base - 1] : ~0ull;
+    __syncthreads();
+    uint32_t hmask = 0, rmask = 0;
+#pragma unroll
+    for (int j = 0; j < HD_IPT; ++j) {
+        const int i = threadIdx.x * HD_IPT + j;
+        const uint64_t k = sk[1 + i], p = sk[i];
+        const bool valid = k != ~0ull;
+        if (valid && p != k) hmask |= 1u << j;
+        if (valid && (p >> cbits) != (k >> cbits)) rmask |= 1u << j;
+    }
+    uint32_t htot, rtot;
+    const uint32_t hex = block_excl_scan_256(__popc(hmask), lds4, &htot);
+    const uint32_t rex = block_excl_scan_256(__popc(rmask), lds4, &rtot);
+    if (threadIdx.x == 0) {
+        s_agg[0] = htot;
+        s_agg[1] = rtot;
+    }
+    __syncthreads();
+    lb_tile_prefix<2>(s_agg, s_pre, st, tile, tag, ctl);
+    uint32_t u = s_pre[0] + hex;
+    uint32_t t = s_pre[1] + rex;  // row heads before the thread's first item
+#pragma unroll
+    for (int j = 0; j < HD_IPT; ++j) {
+        if (!((hmask >> j) & 1u)) continue;
+        const bool rh = (rmask >> j) & 1u;
+        const int64_t idx = base + threadIdx.x * HD_IPT + j;
+        const uint64_t k = sk[1 + threadIdx.x * HD_IPT + j];
+        const uint32_t tr = rh ? t : t - 1u;
+        float s = 0.0f + __uint_as_float(val[idx]);
+        for (int64_t m = idx + 1; m < n && key[m] == k; ++m) s = s + __uint_as_float(val[m]);
+        ucol[u] = (int32_t)(k & ((1ull << cbits) - 1ull));
+        usum[u] = s;
+        ut[u] = tr;
+        if (rh) {
+            trow[tr] = (uint32_t)(k >> cbits);
+            tubeg[tr] = u;
+            tins[tr] = 0;
+            tdel[tr] = 0;
+            ++t;
+        }
+        ++u;
+    }
+    if (threadIdx.x == 0 && tile == ntiles - 1) {
+        const uint32_t nu = s_pre[0] + s_agg[0], nt = s_pre[1] + s_agg[1];
+        ds->nuniq = nu;
+        ds->ntouched = nt;
+        tubeg[nt] = nu;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) lb_finish(ctl);
 }
 
 // Sum of one 32-bit counter over the 256 threads of a block (fixed order).
@@ -120,39 +172,47 @@ __device__ __forceinline__ long long wave_sum_i64(long long v)
     return v;
 }
 
-// Probe each distinct coordinate in its row.  Per-block counters go to
-// pprobe[block][0..2] (reduced by the check kernel).
+// Probe each distinct coordinate in its row (one per thread, tile = 256
+// coordinates), then prefix-sum the insert / delete flags across tiles by
+// look-back into upre[2u], upre[2u+1] (u <= n; entry nuniq holds the totals).
+// Per-tile counters go to pprobe[tile][0..2] (reduced by the check kernel).
 __global__ __launch_bounds__(TB) void k_upd_probe(int64_t n, const DevState *__restrict__ ds,
                                                   const uint32_t *__restrict__ row_start,
                                                   const int32_t *__restrict__ row_len,
                                                   const int32_t *__restrict__ pcol, const float *__restrict__ pval,
                                                   const int32_t *__restrict__ ucol, const float *__restrict__ usum,
                                                   const uint32_t *__restrict__ ut, const uint32_t *__restrict__ trow,
-                                                  uint32_t *__restrict__ tubeg, float *__restrict__ uval,
-                                                  uint32_t *__restrict__ upos, uint32_t *__restrict__ ucode,
-                                                  uint32_t *__restrict__ uflag, uint32_t *__restrict__ tins,
-                                                  uint32_t *__restrict__ tdel, uint32_t *__restrict__ pprobe)
+                                                  float *__restrict__ uval, uint32_t *__restrict__ upos,
+                                                  uint32_t *__restrict__ ucode, uint32_t *__restrict__ upre,
+                                                  uint32_t *__restrict__ tins, uint32_t *__restrict__ tdel,
+                                                  uint32_t *__restrict__ pprobe, uint32_t *__restrict__ ws,
+                                                  uint32_t epoch)
 {
     __shared__ uint32_t lds4[4];
-    int64_t u = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    __shared__ uint32_t s_ticket, s_agg[2], s_pre[2];
+    unsigned int *ctl = ws;
+    uint64_t *st = (uint64_t *)(ws + 4);
+    const uint32_t tag = lb_tag(epoch);
+    const int64_t tile = lb_ticket(ctl, &s_ticket);
+    const int64_t u = tile * TB + threadIdx.x;
     const uint32_t nu = ds->nuniq;
     uint32_t code = C_NOP;
     if (u < nu) {
-        uint32_t t = ut[u];
-        uint32_t r = trow[t];
-        uint64_t st = row_start[r];
-        int32_t L = row_len[r];
-        int32_t c = ucol[u];
+        const uint32_t t = ut[u];
+        const uint32_t r = trow[t];
+        const uint64_t st0 = row_start[r];
+        const int32_t L = row_len[r];
+        const int32_t c = ucol[u];
         int32_t lo = 0, hi = L;  // lower bound of c in the row
         while (lo < hi) {
-            int32_t mid = (lo + hi) >> 1;
-            if (pcol[st + mid] < c) lo = mid + 1; else hi = mid;
+            const int32_t mid = (lo + hi) >> 1;
+            if (pcol[st0 + mid] < c) lo = mid + 1; else hi = mid;
         }
-        bool hit = lo < L && pcol[st + lo] == c;
-        float s = usum[u];
+        const bool hit = lo < L && pcol[st0 + lo] == c;
+        const float s = usum[u];
         float nv;
         if (hit) {
-            nv = pval[st + lo] + s;
+            nv = pval[st0 + lo] + s;
             code = (nv != 0.0f) ? C_ADD : C_DEL;
         } else {
             nv = s;
@@ -163,40 +223,58 @@ __global__ __launch_bounds__(TB) void k_upd_probe(int64_t n, const DevState *__r
         ucode[u] = code;
         if (code == C_INS) atomicAdd(&tins[t], 1u);
         if (code == C_DEL) atomicAdd(&tdel[t], 1u);
-        if (u == 0) tubeg[ds->ntouched] = nu;
     }
-    if (u <= n) {  // index n is the scan's end sentinel
-        uflag[2 * u] = code == C_INS ? 1u : 0u;
-        uflag[2 * u + 1] = code == C_DEL ? 1u : 0u;
-    }
-    uint32_t ni = block_sum_u32(code == C_INS, lds4);
-    uint32_t na = block_sum_u32(code == C_ADD, lds4);
-    uint32_t nd = block_sum_u32(code == C_DEL, lds4);
+    uint32_t itot, dtot;
+    const uint32_t iex = block_excl_scan_256(code == C_INS ? 1u : 0u, lds4, &itot);
+    const uint32_t dex = block_excl_scan_256(code == C_DEL ? 1u : 0u, lds4, &dtot);
     if (threadIdx.x == 0) {
-        pprobe[blockIdx.x * 4 + 0] = ni;
-        pprobe[blockIdx.x * 4 + 1] = na;
-        pprobe[blockIdx.x * 4 + 2] = nd;
+        s_agg[0] = itot;
+        s_agg[1] = dtot;
+    }
+    __syncthreads();
+    lb_tile_prefix<2>(s_agg, s_pre, st, tile, tag, ctl);
+    if (u <= n) {
+        upre[2 * u] = s_pre[0] + iex;
+        upre[2 * u + 1] = s_pre[1] + dex;
+    }
+    const uint32_t na = block_sum_u32(code == C_ADD, lds4);
+    if (threadIdx.x == 0) {
+        pprobe[tile * 4 + 0] = itot;
+        pprobe[tile * 4 + 1] = na;
+        pprobe[tile * 4 + 2] = dtot;
+        lb_finish(ctl);
     }
 }
 
-// Per touched row: new length, mode, chunk claim size.  tpair[2t] = claim,
-// tpair[2t+1] = old entries to move; a 2-word scan turns them into offsets.
-__global__ __launch_bounds__(TB) void k_upd_plan(int64_t n, const DevState *__restrict__ ds,
-                                                 const uint32_t *__restrict__ trow, const uint32_t *__restrict__ tins,
-                                                 const uint32_t *__restrict__ tdel, const int32_t *__restrict__ row_len,
+// Per touched row (one per thread, tile = 256 rows): new length, mode and
+// chunk claim, then the (claim, old entries to move) pairs prefix-summed
+// across tiles by look-back into tpre[2t], tpre[2t+1] (t <= n; entry
+// ntouched holds the totals, which the last tile also stores in
+// ds->claim_total / ds->nrewrite_old).  Per-tile counters go to pplan.
+__global__ __launch_bounds__(TB) void k_upd_plan(int64_t n, DevState *ds, const uint32_t *__restrict__ trow,
+                                                 const uint32_t *__restrict__ tins, const uint32_t *__restrict__ tdel,
+                                                 const int32_t *__restrict__ row_len,
                                                  const int32_t *__restrict__ row_cap, uint32_t *__restrict__ tnewl,
-                                                 uint32_t *__restrict__ tmode, uint32_t *__restrict__ tpair,
-                                                 long long *__restrict__ pplan)
+                                                 uint32_t *__restrict__ tmode, uint32_t *__restrict__ tpre,
+                                                 long long *__restrict__ pplan, uint32_t *__restrict__ ws,
+                                                 uint32_t epoch)
 {
     __shared__ long long lds[4][NPLAN];
-    int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    __shared__ uint32_t lds4[4];
+    __shared__ uint32_t s_ticket, s_agg[2], s_pre[2];
+    unsigned int *ctl = ws;
+    uint64_t *stw = (uint64_t *)(ws + 4);
+    const uint32_t tag = lb_tag(epoch);
+    const int64_t tile = lb_ticket(ctl, &s_ticket);
+    const int64_t ntiles = (n + 1 + TB - 1) / TB;
+    const int64_t t = tile * TB + threadIdx.x;
     const uint32_t nt = ds->ntouched;
     uint32_t claim = 0, mode = M_VALUES, L = 0, nl = 0, cap = 0;
     if (t < nt) {
-        uint32_t r = trow[t];
+        const uint32_t r = trow[t];
         L = (uint32_t)row_len[r];
         cap = (uint32_t)row_cap[r];
-        uint32_t ni = tins[t], nd = tdel[t];
+        const uint32_t ni = tins[t], nd = tdel[t];
         nl = L + ni - nd;
         if (ni | nd) {
             mode = M_REWRITE;
@@ -205,13 +283,27 @@ __global__ __launch_bounds__(TB) void k_upd_plan(int64_t n, const DevState *__re
         tnewl[t] = nl;
         tmode[t] = mode;
     }
-    if (t <= n) {  // index n is the scan's end sentinel
-        tpair[2 * t] = claim;
-        tpair[2 * t + 1] = mode == M_REWRITE ? L : 0u;
+    const uint32_t moved = mode == M_REWRITE ? L : 0u;
+    uint32_t ctot, mtot;
+    const uint32_t cex = block_excl_scan_256(claim, lds4, &ctot);
+    const uint32_t mex = block_excl_scan_256(moved, lds4, &mtot);
+    if (threadIdx.x == 0) {
+        s_agg[0] = ctot;
+        s_agg[1] = mtot;
+    }
+    __syncthreads();
+    lb_tile_prefix<2>(s_agg, s_pre, stw, tile, tag, ctl);
+    if (t <= n) {
+        tpre[2 * t] = s_pre[0] + cex;
+        tpre[2 * t + 1] = s_pre[1] + mex;
+    }
+    if (threadIdx.x == 0 && tile == ntiles - 1) {
+        ds->claim_total = s_pre[0] + s_agg[0];
+        ds->nrewrite_old = s_pre[1] + s_agg[1];
     }
     long long v[NPLAN] = {0, 0, 0, 0, 0};
     if (mode == M_REWRITE) {
-        int hub_now = nl > DCSR_HUB_LEN, hub_was = L > DCSR_HUB_LEN;
+        const int hub_now = nl > DCSR_HUB_LEN, hub_was = L > DCSR_HUB_LEN;
         v[0] = (long long)nl - (long long)L;
         v[1] = (long long)cap;
         v[2] = hub_now - hub_was;
@@ -221,14 +313,16 @@ __global__ __launch_bounds__(TB) void k_upd_plan(int64_t n, const DevState *__re
     }
 #pragma unroll
     for (int i = 0; i < NPLAN; ++i) {
-        long long w = wave_sum_i64(v[i]);
+        const long long w = wave_sum_i64(v[i]);
         if (lane_id() == 0) lds[threadIdx.x >> 6][i] = w;
     }
     __syncthreads();
     if (threadIdx.x < NPLAN) {
-        int i = threadIdx.x;
-        pplan[(int64_t)blockIdx.x * NPLAN + i] = ((lds[0][i] + lds[1][i]) + (lds[2][i] + lds[3][i]));
+        const int i = threadIdx.x;
+        pplan[tile * NPLAN + i] = ((lds[0][i] + lds[1][i]) + (lds[2][i] + lds[3][i]));
     }
+    __syncthreads();
+    if (threadIdx.x == 0) lb_finish(ctl);
 }
 
 // One workgroup: reduce the per-block counters, decide, advance the bump.
@@ -265,19 +359,24 @@ __global__ __launch_bounds__(1024) void k_upd_check(DevState *ds, unsigned long 
         ds->d_hubs = tot[5];
         ds->d_hub_chunks = tot[6];
         ds->hub_touch = (unsigned)tot[7];
-        unsigned long long need = ds->claim_total;
+        const unsigned long long need = ds->claim_total;
+        const unsigned long long base = ds->bump;  // claims start at the current bump
         ds->need = need;
-        ds->bump = ds->bump_saved;
-        if (ds->ubad) {
+        ds->bump_saved = base;
+        const unsigned bad = ds->ubad;
+        ds->ubad = 0;  // the next batch's pack starts from a clear flag
+        ds->ubad_last = bad;
+        if (bad) {
             ds->go = 0;
             ds->status |= ST_BAD_INDEX;
-        } else if (ds->bump_saved + need > pool_cap) {
+        } else if (base + need > pool_cap) {
             ds->go = 0;
             ds->status |= ST_POOL_EXHAUSTED;
         } else {
             ds->go = 1;
-            ds->bump = ds->bump_saved + need;
+            ds->bump = base + need;
         }
+
     }
 }
 
@@ -423,46 +522,40 @@ __global__ __launch_bounds__(TB) void k_upd_write(DevState *ds, const uint32_t *
 
 __global__ void k_upd_validate_begin(DevState *ds) { ds->ubad = 0; }
 
-// After a compaction inside an update: claims restart from the new bump.
-__global__ void k_upd_rebase(DevState *ds) { ds->bump_saved = ds->bump; }
-
 struct UpdBufs {
     uint64_t *k0, *k1;
-    uint32_t *v0, *v1, *hist, *part, *flags, *pre;
+    uint32_t *v0, *v1, *hist, *part;
     int32_t *ucol;
     float *usum, *uval;
-    uint32_t *ut, *upos, *ucode, *uflag, *upre;
-    uint32_t *trow, *tubeg, *tins, *tdel, *tnewl, *tmode, *tpair, *tpre, *pprobe;
+    uint32_t *ut, *upos, *ucode, *upre;
+    uint32_t *trow, *tubeg, *tins, *tdel, *tnewl, *tmode, *tpre, *pprobe;
     long long *pplan;
 };
 
 dcsr_status upd_bufs(dcsr_handle h, int64_t n, UpdBufs &b)
 {
-    size_t n4 = (size_t)(n + 2) * 4, n8 = (size_t)(n + 2) * 8;
+    const size_t n4 = (size_t)(n + 2) * 4, n8 = (size_t)(n + 2) * 8;
     DCSR_TRY(ws_get(h, WS_KEY_A, n8, (void **)&b.k0));
     DCSR_TRY(ws_get(h, WS_KEY_B, n8, (void **)&b.k1));
     DCSR_TRY(ws_get(h, WS_VAL_A, n4, (void **)&b.v0));
     DCSR_TRY(ws_get(h, WS_VAL_B, n4, (void **)&b.v1));
-    size_t partw = scan_part_words(n + 2, 2);
-    size_t rpw = radix_part_words(n);
+    // look-back state of the three fused kernels (the largest grid is
+    // (n+1)/256 tiles of 2 components)
+    const size_t partw = 4 + 2 * 2 * (size_t)grid_for(n + 1, TB);
     DCSR_TRY(ws_get(h, WS_HIST, radix_hist_words(n) * 4, (void **)&b.hist));
-    DCSR_TRY(ws_get(h, WS_SCAN, (partw > rpw ? partw : rpw) * 4, (void **)&b.part));
-    DCSR_TRY(ws_get(h, WS_FLAGS, n8, (void **)&b.flags));
-    DCSR_TRY(ws_get(h, WS_TMP0, n8, (void **)&b.pre));
+    DCSR_TRY(ws_get(h, WS_SCAN, partw * 4, (void **)&b.part));
     DCSR_TRY(ws_get(h, WS_UCOL, n4, (void **)&b.ucol));
     DCSR_TRY(ws_get(h, WS_UPOS, n4, (void **)&b.upos));
     DCSR_TRY(ws_get(h, WS_UVAL, n4, (void **)&b.uval));
     DCSR_TRY(ws_get(h, WS_UT, n4, (void **)&b.ut));
     DCSR_TRY(ws_get(h, WS_UCODE, n4, (void **)&b.ucode));
     DCSR_TRY(ws_get(h, WS_UPRE, n8, (void **)&b.upre));
-    DCSR_TRY(ws_get(h, WS_TMP1, n8, (void **)&b.uflag));
     DCSR_TRY(ws_get(h, WS_TROW, n4, (void **)&b.trow));
     DCSR_TRY(ws_get(h, WS_TUBEG, n4 + 4, (void **)&b.tubeg));
     DCSR_TRY(ws_get(h, WS_TINS, n4, (void **)&b.tins));
     DCSR_TRY(ws_get(h, WS_TDEL, n4, (void **)&b.tdel));
     DCSR_TRY(ws_get(h, WS_TNEWL, n4, (void **)&b.tnewl));
     DCSR_TRY(ws_get(h, WS_TMODE, n4, (void **)&b.tmode));
-    DCSR_TRY(ws_get(h, WS_TPAIR, n8, (void **)&b.tpair));
     DCSR_TRY(ws_get(h, WS_TPAIRPRE, n8, (void **)&b.tpre));
     DCSR_TRY(ws_get(h, WS_TMP2, n4, (void **)&b.usum));
     DCSR_TRY(ws_get(h, WS_PPROBE, (size_t)grid_for(n + 1, TB) * 16, (void **)&b.pprobe));
@@ -473,11 +566,31 @@ dcsr_status upd_bufs(dcsr_handle h, int64_t n, UpdBufs &b)
 dcsr_status plan_and_check(dcsr_handle h, int64_t n, UpdBufs &b, hipStream_t s)
 {
     const unsigned nb = grid_for(n + 1, TB);
-    k_upd_plan<<<nb, TB, 0, s>>>(n, h->ds, b.trow, b.tins, b.tdel, h->row_len, h->row_cap, b.tnewl, b.tmode, b.tpair,
-                                 b.pplan);
-    DCSR_HIP(scan_exclusive_u32(b.tpair, b.tpre, n + 1, 2, b.part, &h->ds->claim_total, s));
+    k_upd_plan<<<nb, TB, 0, s>>>(n, h->ds, b.trow, b.tins, b.tdel, h->row_len, h->row_cap, b.tnewl, b.tmode, b.tpre,
+                                 b.pplan, b.part, lb_next_epoch());
     k_upd_check<<<1, 1024, 0, s>>>(h->ds, (unsigned long long)h->pool_cap, b.pprobe, (int)nb, b.pplan, (int)nb);
     return hipGetLastError() == hipSuccess ? DCSR_OK : DCSR_ERR_HIP;
+}
+
+// Pack, sort, heads, probe, plan, check: the launches shared by the
+// synchronous and the NOSYNC paths (the matrix is only read).
+dcsr_status prepare_launch(dcsr_handle h, const int32_t *rows, const int32_t *cols, const float *vals, int64_t n,
+                           UpdBufs &b, hipStream_t s)
+{
+    const int nbits = h->rbits + h->cbits + 1;
+    k_upd_pack<<<grid_for(n, TB), TB, 0, s>>>(rows, cols, vals, n, h->rows, h->cols, h->cbits, b.k0, b.v0, h->ds);
+    int which = 0;
+    DCSR_HIP(radix_sort_pairs(b.k0, b.v0, b.k1, b.v1, n, nbits, b.hist, b.part, &which, s));
+    const uint64_t *key = which ? b.k1 : b.k0;
+    const uint32_t *val = which ? b.v1 : b.v0;
+    k_upd_heads<<<(unsigned)((n + HD_TILE - 1) / HD_TILE), TB, 0, s>>>(key, val, n, h->cbits, b.part, lb_next_epoch(),
+                                                                       h->ds, b.ucol, b.usum, b.ut, b.trow, b.tubeg,
+                                                                       b.tins, b.tdel);
+    k_upd_probe<<<grid_for(n + 1, TB), TB, 0, s>>>(n, h->ds, h->row_start, h->row_len, h->pool_col, h->pool_val,
+                                                    b.ucol, b.usum, b.ut, b.trow, b.uval, b.upos, b.ucode, b.upre,
+                                                    b.tins, b.tdel, b.pprobe, b.part, lb_next_epoch());
+    DCSR_HIP(hipGetLastError());
+    return plan_and_check(h, n, b, s);
 }
 
 }  // namespace
@@ -512,36 +625,21 @@ dcsr_status update_prepare(dcsr_handle h, const int32_t *rows, const int32_t *co
     }
     UpdBufs b;
     DCSR_TRY(upd_bufs(h, n, b));
-    const int nbits = h->rbits + h->cbits + 1;
-    k_upd_begin<<<1, 1, 0, s>>>(h->ds);
-    k_upd_pack<<<grid_for(n, TB), TB, 0, s>>>(rows, cols, vals, n, h->rows, h->cols, h->cbits, b.k0, b.v0, h->ds);
-    int which = 0;
-    DCSR_HIP(radix_sort_pairs(b.k0, b.v0, b.k1, b.v1, n, nbits, b.hist, b.part, &which, s));
-    uint64_t *key = which ? b.k1 : b.k0;
-    uint32_t *val = which ? b.v1 : b.v0;
-    k_upd_flags<<<grid_for(n, TB), TB, 0, s>>>(key, n, h->cbits, b.flags);
-    DCSR_HIP(scan_exclusive_u32(b.flags, b.pre, n, 2, b.part, &h->ds->nuniq, s));
-    k_upd_compact<<<grid_for(n, TB), TB, 0, s>>>(key, val, n, h->cbits, b.flags, b.pre, b.ucol, b.usum, b.ut, b.trow,
-                                                  b.tubeg, b.tins, b.tdel);
-    k_upd_probe<<<grid_for(n + 1, TB), TB, 0, s>>>(n, h->ds, h->row_start, h->row_len, h->pool_col, h->pool_val,
-                                                    b.ucol, b.usum, b.ut, b.trow, b.tubeg, b.uval, b.upos, b.ucode,
-                                                    b.uflag, b.tins, b.tdel, b.pprobe);
-    DCSR_HIP(scan_exclusive_u32(b.uflag, b.upre, n + 1, 2, b.part, nullptr, s));
-    DCSR_TRY(plan_and_check(h, n, b, s));
+    DCSR_TRY(prepare_launch(h, rows, cols, vals, n, b, s));
     DCSR_TRY(pull_state(h, s));
-    if (h->hs->ubad) return DCSR_ERR_BAD_INDEX;
+    if (h->hs->ubad_last) return DCSR_ERR_BAD_INDEX;
     *go = h->hs->go != 0;
     *need = (int64_t)h->hs->need;
     return DCSR_OK;
 }
 
 // After pool growth/compaction between prepare and commit: plan again (row
-// contents are unchanged, only their locations moved).
+// contents are unchanged, only their locations moved; the check takes the
+// claims from the current bump pointer).
 dcsr_status update_replan(dcsr_handle h, int64_t n, hipStream_t s, bool *go)
 {
     UpdBufs b;
     DCSR_TRY(upd_bufs(h, n, b));
-    k_upd_rebase<<<1, 1, 0, s>>>(h->ds);
     DCSR_TRY(plan_and_check(h, n, b, s));
     DCSR_TRY(pull_state(h, s));
     *go = h->hs->go != 0;
@@ -600,22 +698,7 @@ static dcsr_status update_nosync(dcsr_handle h, const int32_t *rows, const int32
     if (n == 0) return DCSR_OK;
     UpdBufs b;
     DCSR_TRY(upd_bufs(h, n, b));
-    const int nbits = h->rbits + h->cbits + 1;
-    k_upd_begin<<<1, 1, 0, s>>>(h->ds);
-    k_upd_pack<<<grid_for(n, TB), TB, 0, s>>>(rows, cols, vals, n, h->rows, h->cols, h->cbits, b.k0, b.v0, h->ds);
-    int which = 0;
-    DCSR_HIP(radix_sort_pairs(b.k0, b.v0, b.k1, b.v1, n, nbits, b.hist, b.part, &which, s));
-    uint64_t *key = which ? b.k1 : b.k0;
-    uint32_t *val = which ? b.v1 : b.v0;
-    k_upd_flags<<<grid_for(n, TB), TB, 0, s>>>(key, n, h->cbits, b.flags);
-    DCSR_HIP(scan_exclusive_u32(b.flags, b.pre, n, 2, b.part, &h->ds->nuniq, s));
-    k_upd_compact<<<grid_for(n, TB), TB, 0, s>>>(key, val, n, h->cbits, b.flags, b.pre, b.ucol, b.usum, b.ut, b.trow,
-                                                  b.tubeg, b.tins, b.tdel);
-    k_upd_probe<<<grid_for(n + 1, TB), TB, 0, s>>>(n, h->ds, h->row_start, h->row_len, h->pool_col, h->pool_val,
-                                                    b.ucol, b.usum, b.ut, b.trow, b.tubeg, b.uval, b.upos, b.ucode,
-                                                    b.uflag, b.tins, b.tdel, b.pprobe);
-    DCSR_HIP(scan_exclusive_u32(b.uflag, b.upre, n + 1, 2, b.part, nullptr, s));
-    DCSR_TRY(plan_and_check(h, n, b, s));
+    DCSR_TRY(prepare_launch(h, rows, cols, vals, n, b, s));
     return update_commit(h, n, s, true);
 }
 
