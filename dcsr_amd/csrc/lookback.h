@@ -76,22 +76,36 @@ __device__ __forceinline__ void lb_finish(unsigned int *ctl)
     }
 }
 
-// Exclusive prefix of tile `tile` (> 0) over status words st[j * stride]
-// (j < tile), computed by one whole wave: 64 predecessors are examined per
-// round, nearest first.  Returns the same value in every lane.
-__device__ __forceinline__ uint32_t lb_wave_lookback(const uint64_t *st, int64_t stride, int64_t tile, uint32_t tag,
-                                                     unsigned int *ctl)
+// Exclusive prefixes of tile `tile` (> 0) for NC components whose status
+// words sit at st[j * NC + c] (j < tile), computed by one whole wave: 64
+// predecessors are examined per round, nearest first, all components in the
+// same round (each stops at its own nearest inclusive word).  pre[c] gets
+// the same value in every lane.
+template <int NC>
+__device__ __forceinline__ void lb_wave_lookback(const uint64_t *st, int64_t tile, uint32_t tag, unsigned int *ctl,
+                                                 uint32_t *pre)
 {
     const int lane = lane_id();
-    uint32_t prefix = 0;
+    bool done[NC];
+#pragma unroll
+    for (int c = 0; c < NC; ++c) {
+        pre[c] = 0;
+        done[c] = false;
+    }
     int64_t j = tile - 1;
     int spins = 0;
     while (j >= 0) {
         const int64_t idx = j - lane;
-        uint64_t w = 0;
+        uint64_t w[NC];
+#pragma unroll
+        for (int c = 0; c < NC; ++c) w[c] = 0;
         for (;;) {
-            if (idx >= 0) w = lb_load(&st[idx * stride]);
-            const bool ok = idx < 0 || lb_ready(w, tag);
+            bool ok = true;
+#pragma unroll
+            for (int c = 0; c < NC; ++c) {
+                if (idx >= 0 && !done[c]) w[c] = lb_load(&st[idx * NC + c]);
+                ok = ok && (idx < 0 || done[c] || lb_ready(w[c], tag));
+            }
             if (__all(ok)) break;
             if (++spins > LB_SPIN_LIMIT) {
                 if (lane == 0) atomicAdd(&ctl[2], 1u);
@@ -99,44 +113,23 @@ __device__ __forceinline__ uint32_t lb_wave_lookback(const uint64_t *st, int64_t
             }
             __builtin_amdgcn_s_sleep(1);
         }
-        const bool inc = idx >= 0 && lb_inclusive(w, tag);
-        const uint64_t im = __ballot(inc);
-        uint32_t v = idx >= 0 ? (uint32_t)w : 0u;
-        if (im) {
-            const int L = __ffsll((long long)im) - 1;
-            if (lane > L) v = 0u;
-            prefix += wave_sum_u32(v);
-            break;
+        bool all_done = true;
+#pragma unroll
+        for (int c = 0; c < NC; ++c) {
+            if (done[c]) continue;
+            const uint64_t im = __ballot(idx >= 0 && lb_inclusive(w[c], tag));
+            uint32_t v = idx >= 0 ? (uint32_t)w[c] : 0u;
+            if (im) {
+                const int L = __ffsll((long long)im) - 1;
+                if (lane > L) v = 0u;
+                done[c] = true;
+            }
+            pre[c] += wave_sum_u32(v);
+            all_done = all_done && done[c];
         }
-        prefix += wave_sum_u32(v);
+        if (all_done) break;
         j -= 64;
     }
-    return prefix;
-}
-
-// Exclusive prefix of tile `tile` (> 0) over st[j * stride], walked by ONE
-// thread (used when every thread of a workgroup resolves its own column,
-// e.g. one radix digit each).
-__device__ __forceinline__ uint32_t lb_thread_lookback(const uint64_t *st, int64_t stride, int64_t tile, uint32_t tag,
-                                                       unsigned int *ctl)
-{
-    uint32_t prefix = 0;
-    int spins = 0;
-    for (int64_t j = tile - 1; j >= 0; --j) {
-        uint64_t w;
-        for (;;) {
-            w = lb_load(&st[j * stride]);
-            if (lb_ready(w, tag)) break;
-            if (++spins > LB_SPIN_LIMIT) {
-                atomicAdd(&ctl[2], 1u);
-                return prefix;
-            }
-            __builtin_amdgcn_s_sleep(1);
-        }
-        prefix += (uint32_t)w;
-        if (lb_inclusive(w, tag)) break;
-    }
-    return prefix;
 }
 
 // Tile prefix of NC components in one call: thread 0 publishes the tile's
@@ -152,12 +145,18 @@ __device__ __forceinline__ void lb_tile_prefix(const uint32_t *s_agg, uint32_t *
         for (int c = 0; c < NC; ++c) lb_publish(&st[tile * NC + c], tag, tile == 0 ? LB_INC : LB_AGG, s_agg[c]);
     }
     if (threadIdx.x < 64) {
+        uint32_t p[NC];
+        if (tile > 0) {
+            lb_wave_lookback<NC>(st, tile, tag, ctl, p);
+        } else {
 #pragma unroll
-        for (int c = 0; c < NC; ++c) {
-            const uint32_t p = tile == 0 ? 0u : lb_wave_lookback(st + c, NC, tile, tag, ctl);
-            if (threadIdx.x == 0) {
-                s_pre[c] = p;
-                if (tile > 0) lb_publish(&st[tile * NC + c], tag, LB_INC, p + s_agg[c]);
+            for (int c = 0; c < NC; ++c) p[c] = 0u;
+        }
+        if (threadIdx.x == 0) {
+#pragma unroll
+            for (int c = 0; c < NC; ++c) {
+                s_pre[c] = p[c];
+                if (tile > 0) lb_publish(&st[tile * NC + c], tag, LB_INC, p[c] + s_agg[c]);
             }
         }
     }

@@ -15,9 +15,8 @@ hipError_t scan_exclusive_u32(const uint32_t *in, uint32_t *out, int64_t n, int 
 // Stable LSD radix sort of (key, value) pairs on key bits [0, nbits).
 // Ping-pongs between (k0, v0) and (k1, v1); returns in *which the buffer
 // index (0 or 1) holding the result.  `hist` is scratch of
-// radix_hist_words(n) words, zero-filled when first allocated (each sort
-// leaves it as it found it); calls sharing it must be stream-ordered.  `part`
-// is unused (kept for the call sites).
+// radix_hist_words(n) words; `part` is scan scratch of radix_part_words(n)
+// words (zero-filled when first allocated, as for scan_exclusive_u32).
 size_t radix_hist_words(int64_t n);
 size_t radix_part_words(int64_t n);
 hipError_t radix_sort_pairs(uint64_t *k0, uint32_t *v0, uint64_t *k1, uint32_t *v1, int64_t n, int nbits,

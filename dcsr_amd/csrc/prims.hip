@@ -5,9 +5,12 @@
 // include/load.h:602-603, cusp::indices_to_offsets, cub::BlockRadixSort
 // include/spmm.inl:489-490) with wave64 code: DPP scans and ballot-based
 // digit ranking (8 ballots per item give the lanes sharing a digit).
-// Both are single-pass over their tiles: one launch per scan and one per
-// radix pass (plus one histogram launch per sort), with cross-tile prefixes
-// resolved by decoupled look-back (lookback.h).
+// Scans are single-pass (one launch; cross-tile prefixes by decoupled
+// look-back, lookback.h).  A radix pass is three launches: per-tile digit
+// counts, one scan of the count table, stable scatter.  (A onesweep pass
+// that resolves each digit's offset by look-back walks hundreds of tiles
+// serially per digit here, where every look-back poll is a cross-XCD
+// coherent load; the table scan is faster.)
 #include <atomic>
 #include "lookback.h"
 #include "prims.h"
@@ -113,45 +116,26 @@ constexpr int RS_IPT = 8;                        // items per lane
 constexpr int RS_WAVE_ITEMS = RS_IPT * 64;       // contiguous items per wave
 constexpr int RS_TILE = RS_WAVES * RS_WAVE_ITEMS; // 2048
 
-constexpr int RS_MAXPASS = 8;  // up to 64 key bits
-
-// Global digit histograms of every pass in one read of the keys (the count of
-// each digit value does not depend on the order the earlier passes leave).
-// ghist: [npass][256], zero on entry (the last pass of the previous sort
-// cleared it).
-__global__ __launch_bounds__(RS_THREADS) void k_rs_hist_all(const uint64_t *__restrict__ keys, int64_t n, int npass,
-                                                            uint32_t *__restrict__ ghist)
+__global__ __launch_bounds__(RS_THREADS) void k_rs_hist(const uint64_t *__restrict__ keys, int64_t n, int shift,
+                                                        uint32_t *__restrict__ hist, int64_t nblocks)
 {
-    __shared__ uint32_t cnt[RS_MAXPASS][256];
-    for (int i = threadIdx.x; i < RS_MAXPASS * 256; i += RS_THREADS) (&cnt[0][0])[i] = 0;
+    __shared__ uint32_t cnt[256];
+    cnt[threadIdx.x] = 0;
     __syncthreads();
     const int64_t base = (int64_t)blockIdx.x * RS_TILE;
     for (int i = threadIdx.x; i < RS_TILE; i += RS_THREADS) {
-        const int64_t idx = base + i;
-        if (idx < n) {
-            const uint64_t k = keys[idx];
-            for (int p = 0; p < npass; ++p) atomicAdd(&cnt[p][(uint32_t)(k >> (8 * p)) & 255u], 1u);
-        }
+        int64_t idx = base + i;
+        if (idx < n) atomicAdd(&cnt[(uint32_t)(keys[idx] >> shift) & 255u], 1u);
     }
     __syncthreads();
-    for (int p = 0; p < npass; ++p) {
-        const uint32_t c = cnt[p][threadIdx.x];
-        if (c) atomicAdd(&ghist[p * 256 + threadIdx.x], c);
-    }
+    hist[(int64_t)threadIdx.x * nblocks + blockIdx.x] = cnt[threadIdx.x];
 }
 
-// One stable scatter pass (onesweep): tile = ticket; rank the tile's keys by
-// digit inside the workgroup (wave ballots, wave-major order keeps
-// stability); per digit, publish the tile's count and look back over lower
-// tiles for the digit's running offset; the digit's base is the exclusive
-// scan of the global histogram.  The last workgroup of the last pass clears
-// ghist for the next sort.
-// ws: [ctl 4 words][ghist npass x 256][status: tiles x 256 words of 64 bits]
-__global__ __launch_bounds__(RS_THREADS) void k_rs_onesweep(const uint64_t *__restrict__ kin,
-                                                            const uint32_t *__restrict__ vin,
-                                                            uint64_t *__restrict__ kout, uint32_t *__restrict__ vout,
-                                                            int64_t n, int pass, int npass, uint32_t *__restrict__ ws,
-                                                            uint32_t epoch)
+__global__ __launch_bounds__(RS_THREADS) void k_rs_scatter(const uint64_t *__restrict__ kin,
+                                                           const uint32_t *__restrict__ vin,
+                                                           uint64_t *__restrict__ kout, uint32_t *__restrict__ vout,
+                                                           int64_t n, int shift,
+                                                           const uint32_t *__restrict__ offs, int64_t nblocks)
 {
     __shared__ uint64_t skey[RS_TILE];
     __shared__ uint32_t sval[RS_TILE];
@@ -159,20 +143,13 @@ __global__ __launch_bounds__(RS_THREADS) void k_rs_onesweep(const uint64_t *__re
     __shared__ uint32_t loff[256];
     __shared__ uint32_t goff[256];
     __shared__ uint32_t lds4[4];
-    __shared__ uint32_t s_ticket;
 
-    unsigned int *ctl = ws;
-    uint32_t *ghist = ws + 4;
-    uint64_t *st = (uint64_t *)(ws + 4 + RS_MAXPASS * 256);
-    const uint32_t tag = lb_tag(epoch);
-    const int shift = 8 * pass;
     const int w = threadIdx.x >> 6;
     const int lane = lane_id();
-    const int64_t tile = lb_ticket(ctl, &s_ticket);
     for (int i = threadIdx.x; i < RS_WAVES * 256; i += RS_THREADS) (&wcnt[0][0])[i] = 0;
     __syncthreads();
 
-    const int64_t base = tile * RS_TILE;
+    const int64_t base = (int64_t)blockIdx.x * RS_TILE;
     const int64_t wbase = base + (int64_t)w * RS_WAVE_ITEMS;
     uint64_t key[RS_IPT];
     uint32_t val[RS_IPT];
@@ -180,18 +157,13 @@ __global__ __launch_bounds__(RS_THREADS) void k_rs_onesweep(const uint64_t *__re
     const uint64_t lt = lanemask_lt();
 #pragma unroll
     for (int j = 0; j < RS_IPT; ++j) {
-        const int64_t idx = wbase + j * 64 + lane;
-        const bool ok = idx < n;
+        int64_t idx = wbase + j * 64 + lane;
+        bool ok = idx < n;
         key[j] = ok ? kin[idx] : ~0ull;
         val[j] = ok ? vin[idx] : 0u;
-    }
-    // Items past n carry key ~0 (digit 255): only the last tile has them,
-    // they rank after every valid item of the tile, and no tile follows.
-#pragma unroll
-    for (int j = 0; j < RS_IPT; ++j) {
-        const uint32_t d = (uint32_t)(key[j] >> shift) & 255u;
-        const uint64_t peers = wave_match_digit8(d, ~0ull);
-        const int leader = __ffsll((long long)peers) - 1;
+        uint32_t d = (uint32_t)(key[j] >> shift) & 255u;
+        uint64_t peers = wave_match_digit8(d, ~0ull);
+        int leader = __ffsll((long long)peers) - 1;
         uint32_t b = 0;
         if (lane == leader) {
             b = wcnt[w][d];
@@ -204,51 +176,33 @@ __global__ __launch_bounds__(RS_THREADS) void k_rs_onesweep(const uint64_t *__re
     {
         const int d = threadIdx.x;  // one digit per thread
         uint32_t c[RS_WAVES];
-        uint32_t sum = 0;
+        uint32_t s = 0;
 #pragma unroll
         for (int i = 0; i < RS_WAVES; ++i) {
             c[i] = wcnt[i][d];
-            wcnt[i][d] = sum;
-            sum += c[i];
+            wcnt[i][d] = s;
+            s += c[i];
         }
         uint32_t tot;
-        loff[d] = block_excl_scan_256(sum, lds4, &tot);
-        // this tile's count of digit d, then its running offset
-        lb_publish(&st[tile * 256 + d], tag, tile == 0 ? LB_INC : LB_AGG, sum);
-        const uint32_t g = ghist[pass * 256 + d];
-        uint32_t gtot;
-        const uint32_t gbase = block_excl_scan_256(g, lds4, &gtot);
-        const uint32_t run = tile == 0 ? 0u : lb_thread_lookback(st + d, 256, tile, tag, ctl);
-        if (tile > 0) lb_publish(&st[tile * 256 + d], tag, LB_INC, run + sum);
-        goff[d] = gbase + run;
+        loff[d] = block_excl_scan_256(s, lds4, &tot);
+        goff[d] = offs[(int64_t)d * nblocks + blockIdx.x];
     }
     __syncthreads();
 #pragma unroll
     for (int j = 0; j < RS_IPT; ++j) {
-        const uint32_t d = (uint32_t)(key[j] >> shift) & 255u;
-        const uint32_t p = loff[d] + wcnt[w][d] + rnk[j];
+        uint32_t d = (uint32_t)(key[j] >> shift) & 255u;
+        uint32_t p = loff[d] + wcnt[w][d] + rnk[j];
         skey[p] = key[j];
         sval[p] = val[j];
     }
     __syncthreads();
     const int64_t valid = (n - base) < RS_TILE ? (n - base) : RS_TILE;
     for (int p = threadIdx.x; p < valid; p += RS_THREADS) {
-        const uint64_t k = skey[p];
-        const uint32_t d = (uint32_t)(k >> shift) & 255u;
-        const int64_t o = (int64_t)goff[d] + (p - (int64_t)loff[d]);
+        uint64_t k = skey[p];
+        uint32_t d = (uint32_t)(k >> shift) & 255u;
+        int64_t o = (int64_t)goff[d] + (p - (int64_t)loff[d]);
         kout[o] = k;
         vout[o] = sval[p];
-    }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        __threadfence();
-        if (atomicAdd(&ctl[1], 1u) == gridDim.x - 1) {
-            if (pass == npass - 1)
-                for (int i = 0; i < npass * 256; ++i) ghist[i] = 0u;
-            ctl[0] = 0u;
-            ctl[1] = 0u;
-            __threadfence();
-        }
     }
 }
 
@@ -287,25 +241,27 @@ hipError_t scan_exclusive_u32(const uint32_t *in, uint32_t *out, int64_t n, int 
 size_t radix_hist_words(int64_t n)
 {
     const int64_t nb = (n + RS_TILE - 1) / RS_TILE;
-    return (size_t)(4 + RS_MAXPASS * 256 + 2 * 256 * (nb < 1 ? 1 : nb));
+    return (size_t)(nb < 1 ? 1 : nb) * 256;
 }
 
-size_t radix_part_words(int64_t) { return 4; }
+size_t radix_part_words(int64_t n) { return scan_part_words((int64_t)radix_hist_words(n), 1); }
 
+// Per pass: per-tile digit counts (column-major: digit-major, tile-minor),
+// one single-pass scan of that table, stable scatter.
 hipError_t radix_sort_pairs(uint64_t *k0, uint32_t *v0, uint64_t *k1, uint32_t *v1, int64_t n, int nbits,
-                            uint32_t *hist, uint32_t *, int *which, hipStream_t s)
+                            uint32_t *hist, uint32_t *part, int *which, hipStream_t s)
 {
     *which = 0;
     if (n <= 1) return hipSuccess;
-    const int npass = (nbits + 7) / 8;
-    if (npass > RS_MAXPASS) return hipErrorInvalidValue;
     const int64_t nb = (n + RS_TILE - 1) / RS_TILE;
-    k_rs_hist_all<<<(unsigned)nb, RS_THREADS, 0, s>>>(k0, n, npass, hist + 4);
     uint64_t *ki = k0, *ko = k1;
     uint32_t *vi = v0, *vo = v1;
     int cur = 0;
-    for (int pass = 0; pass < npass; ++pass) {
-        k_rs_onesweep<<<(unsigned)nb, RS_THREADS, 0, s>>>(ki, vi, ko, vo, n, pass, npass, hist, lb_next_epoch());
+    for (int shift = 0; shift < nbits; shift += 8) {
+        k_rs_hist<<<(unsigned)nb, RS_THREADS, 0, s>>>(ki, n, shift, hist, nb);
+        const hipError_t e = scan_exclusive_u32(hist, hist, nb * 256, 1, part, nullptr, s);
+        if (e != hipSuccess) return e;
+        k_rs_scatter<<<(unsigned)nb, RS_THREADS, 0, s>>>(ki, vi, ko, vo, n, shift, hist, nb);
         uint64_t *tk = ki; ki = ko; ko = tk;
         uint32_t *tv = vi; vi = vo; vo = tv;
         cur ^= 1;

@@ -534,8 +534,10 @@ dcsr_status dcsr_spgemm(dcsr_handle A, dcsr_handle B, dcsr_handle C, dcsr_stream
         // search needs monotone offsets
         k_sg_hentries<<<grid_for(nent, 4 * 256) + 1, TB, 0, s>>>(hlist, hn, hpre, nent, A->row_start, A->pool_col,
                                                                A->pool_val, B->row_start, B->row_len, ea, blen);
-        const size_t pw_a = scan_part_words(nent + 1, 1), pw_b = scan_part_words((int64_t)np + 1, 1);
-        DCSR_TRY(ws_get(C, WS_SCAN, (pw_a > pw_b ? pw_a : pw_b) * 4, (void **)&part));
+        size_t pw = scan_part_words(nent + 1, 1);
+        if (scan_part_words((int64_t)np + 1, 1) > pw) pw = scan_part_words((int64_t)np + 1, 1);
+        if (radix_part_words(np) > pw) pw = radix_part_words(np);
+        DCSR_TRY(ws_get(C, WS_SCAN, pw * 4, (void **)&part));
         DCSR_HIP(scan_exclusive_u32(blen, apos, nent, 1, part, apos + nent, s));
         DCSR_TRY(ws_get(C, WS_SG_KEY0, (size_t)(np + 1) * 8, (void **)&k0));
         DCSR_TRY(ws_get(C, WS_SG_KEY1, (size_t)(np + 1) * 8, (void **)&k1));

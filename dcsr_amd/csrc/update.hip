@@ -172,10 +172,14 @@ __device__ __forceinline__ long long wave_sum_i64(long long v)
     return v;
 }
 
-// Probe each distinct coordinate in its row (one per thread, tile = 256
-// coordinates), then prefix-sum the insert / delete flags across tiles by
-// look-back into upre[2u], upre[2u+1] (u <= n; entry nuniq holds the totals).
-// Per-tile counters go to pprobe[tile][0..2] (reduced by the check kernel).
+// Probe each distinct coordinate in its row, then prefix-sum the insert /
+// delete flags across tiles by look-back into upre[2u], upre[2u+1] (u <= n;
+// entry nuniq holds the totals).  A tile is 2048 coordinates, 8 per thread
+// (blocked); the eight binary searches of a thread advance in lock step so
+// their loads overlap.  Per-tile counters go to pprobe[tile][0..2] (reduced
+// by the check kernel).
+constexpr int PB_IPT = 8;
+constexpr int PB_TILE = TB * PB_IPT;
 __global__ __launch_bounds__(TB) void k_upd_probe(int64_t n, const DevState *__restrict__ ds,
                                                   const uint32_t *__restrict__ row_start,
                                                   const int32_t *__restrict__ row_len,
@@ -194,63 +198,125 @@ __global__ __launch_bounds__(TB) void k_upd_probe(int64_t n, const DevState *__r
     uint64_t *st = (uint64_t *)(ws + 4);
     const uint32_t tag = lb_tag(epoch);
     const int64_t tile = lb_ticket(ctl, &s_ticket);
-    const int64_t u = tile * TB + threadIdx.x;
+    const int64_t u0 = tile * PB_TILE + (int64_t)threadIdx.x * PB_IPT;
     const uint32_t nu = ds->nuniq;
-    uint32_t code = C_NOP;
-    if (u < nu) {
-        const uint32_t t = ut[u];
-        const uint32_t r = trow[t];
-        const uint64_t st0 = row_start[r];
-        const int32_t L = row_len[r];
-        const int32_t c = ucol[u];
-        int32_t lo = 0, hi = L;  // lower bound of c in the row
-        while (lo < hi) {
-            const int32_t mid = (lo + hi) >> 1;
-            if (pcol[st0 + mid] < c) lo = mid + 1; else hi = mid;
+    uint64_t rs[PB_IPT];
+    int32_t lo[PB_IPT], hi[PB_IPT], cc[PB_IPT], L[PB_IPT];
+    uint32_t tt[PB_IPT];
+#pragma unroll
+    for (int j = 0; j < PB_IPT; ++j) {
+        const int64_t u = u0 + j;
+        lo[j] = hi[j] = L[j] = cc[j] = 0;
+        tt[j] = 0;
+        rs[j] = 0;
+        if (u < nu) {
+            tt[j] = ut[u];
+            cc[j] = ucol[u];
         }
-        const bool hit = lo < L && pcol[st0 + lo] == c;
-        const float s = usum[u];
-        float nv;
-        if (hit) {
-            nv = pval[st0 + lo] + s;
-            code = (nv != 0.0f) ? C_ADD : C_DEL;
-        } else {
-            nv = s;
-            code = (s != 0.0f) ? C_INS : C_NOP;
+    }
+#pragma unroll
+    for (int j = 0; j < PB_IPT; ++j) {
+        if (u0 + j < nu) {
+            const uint32_t r = trow[tt[j]];
+            rs[j] = row_start[r];
+            L[j] = row_len[r];
+            hi[j] = L[j];
         }
-        uval[u] = nv;
-        upos[u] = (uint32_t)lo;
-        ucode[u] = code;
-        if (code == C_INS) atomicAdd(&tins[t], 1u);
-        if (code == C_DEL) atomicAdd(&tdel[t], 1u);
+    }
+    // lower bound of cc[j] in its row, eight searches in lock step
+    for (;;) {
+        bool any = false;
+        int32_t pc[PB_IPT];
+#pragma unroll
+        for (int j = 0; j < PB_IPT; ++j) {
+            pc[j] = 0;
+            if (lo[j] < hi[j]) {
+                pc[j] = pcol[rs[j] + (uint32_t)((lo[j] + hi[j]) >> 1)];
+                any = true;
+            }
+        }
+        if (!any) break;
+#pragma unroll
+        for (int j = 0; j < PB_IPT; ++j) {
+            if (lo[j] < hi[j]) {
+                const int32_t mid = (lo[j] + hi[j]) >> 1;
+                if (pc[j] < cc[j]) lo[j] = mid + 1; else hi[j] = mid;
+            }
+        }
+    }
+    uint32_t code[PB_IPT];
+    int32_t hc[PB_IPT];
+    float hv[PB_IPT];
+#pragma unroll
+    for (int j = 0; j < PB_IPT; ++j) {
+        hc[j] = -1;
+        hv[j] = 0.0f;
+        if (u0 + j < nu && lo[j] < L[j]) {
+            hc[j] = pcol[rs[j] + lo[j]];
+            hv[j] = pval[rs[j] + lo[j]];
+        }
+    }
+    uint32_t ni = 0, nd = 0, na = 0;
+#pragma unroll
+    for (int j = 0; j < PB_IPT; ++j) {
+        const int64_t u = u0 + j;
+        code[j] = C_NOP;
+        if (u < nu) {
+            const bool hit = hc[j] == cc[j];
+            const float s = usum[u];
+            float nv;
+            if (hit) {
+                nv = hv[j] + s;
+                code[j] = (nv != 0.0f) ? C_ADD : C_DEL;
+            } else {
+                nv = s;
+                code[j] = (s != 0.0f) ? C_INS : C_NOP;
+            }
+            uval[u] = nv;
+            upos[u] = (uint32_t)lo[j];
+            ucode[u] = code[j];
+            if (code[j] == C_INS) atomicAdd(&tins[tt[j]], 1u);
+            if (code[j] == C_DEL) atomicAdd(&tdel[tt[j]], 1u);
+        }
+        ni += code[j] == C_INS;
+        nd += code[j] == C_DEL;
+        na += code[j] == C_ADD;
     }
     uint32_t itot, dtot;
-    const uint32_t iex = block_excl_scan_256(code == C_INS ? 1u : 0u, lds4, &itot);
-    const uint32_t dex = block_excl_scan_256(code == C_DEL ? 1u : 0u, lds4, &dtot);
+    uint32_t iex = block_excl_scan_256(ni, lds4, &itot);
+    uint32_t dex = block_excl_scan_256(nd, lds4, &dtot);
     if (threadIdx.x == 0) {
         s_agg[0] = itot;
         s_agg[1] = dtot;
     }
     __syncthreads();
     lb_tile_prefix<2>(s_agg, s_pre, st, tile, tag, ctl);
-    if (u <= n) {
-        upre[2 * u] = s_pre[0] + iex;
-        upre[2 * u + 1] = s_pre[1] + dex;
+    iex += s_pre[0];
+    dex += s_pre[1];
+#pragma unroll
+    for (int j = 0; j < PB_IPT; ++j) {
+        const int64_t u = u0 + j;
+        if (u <= n) {
+            upre[2 * u] = iex;
+            upre[2 * u + 1] = dex;
+        }
+        iex += code[j] == C_INS;
+        dex += code[j] == C_DEL;
     }
-    const uint32_t na = block_sum_u32(code == C_ADD, lds4);
+    const uint32_t atot = block_sum_u32(na, lds4);
     if (threadIdx.x == 0) {
         pprobe[tile * 4 + 0] = itot;
-        pprobe[tile * 4 + 1] = na;
+        pprobe[tile * 4 + 1] = atot;
         pprobe[tile * 4 + 2] = dtot;
         lb_finish(ctl);
     }
 }
 
-// Per touched row (one per thread, tile = 256 rows): new length, mode and
-// chunk claim, then the (claim, old entries to move) pairs prefix-summed
-// across tiles by look-back into tpre[2t], tpre[2t+1] (t <= n; entry
-// ntouched holds the totals, which the last tile also stores in
-// ds->claim_total / ds->nrewrite_old).  Per-tile counters go to pplan.
+// Per touched row: new length, mode and chunk claim, then the (claim, old
+// entries to move) pairs prefix-summed across tiles by look-back into
+// tpre[2t], tpre[2t+1] (t <= n; entry ntouched holds the totals, which the
+// last tile also stores in ds->claim_total / ds->nrewrite_old).  A tile is
+// 2048 rows, 8 per thread (blocked).  Per-tile counters go to pplan.
 __global__ __launch_bounds__(TB) void k_upd_plan(int64_t n, DevState *ds, const uint32_t *__restrict__ trow,
                                                  const uint32_t *__restrict__ tins, const uint32_t *__restrict__ tdel,
                                                  const int32_t *__restrict__ row_len,
@@ -266,50 +332,65 @@ __global__ __launch_bounds__(TB) void k_upd_plan(int64_t n, DevState *ds, const 
     uint64_t *stw = (uint64_t *)(ws + 4);
     const uint32_t tag = lb_tag(epoch);
     const int64_t tile = lb_ticket(ctl, &s_ticket);
-    const int64_t ntiles = (n + 1 + TB - 1) / TB;
-    const int64_t t = tile * TB + threadIdx.x;
+    const int64_t ntiles = (n + 1 + PB_TILE - 1) / PB_TILE;
+    const int64_t t0 = tile * PB_TILE + (int64_t)threadIdx.x * PB_IPT;
     const uint32_t nt = ds->ntouched;
-    uint32_t claim = 0, mode = M_VALUES, L = 0, nl = 0, cap = 0;
-    if (t < nt) {
-        const uint32_t r = trow[t];
-        L = (uint32_t)row_len[r];
-        cap = (uint32_t)row_cap[r];
-        const uint32_t ni = tins[t], nd = tdel[t];
-        nl = L + ni - nd;
-        if (ni | nd) {
-            mode = M_REWRITE;
-            claim = chunk_round(nl);
+    uint32_t claim[PB_IPT], moved[PB_IPT];
+    uint32_t csum = 0, msum = 0;
+    long long v[NPLAN] = {0, 0, 0, 0, 0};
+#pragma unroll
+    for (int j = 0; j < PB_IPT; ++j) {
+        const int64_t t = t0 + j;
+        claim[j] = 0;
+        moved[j] = 0;
+        if (t < nt) {
+            const uint32_t r = trow[t];
+            const uint32_t L = (uint32_t)row_len[r];
+            const uint32_t cap = (uint32_t)row_cap[r];
+            const uint32_t ni = tins[t], nd = tdel[t];
+            const uint32_t nl = L + ni - nd;
+            const uint32_t mode = (ni | nd) ? M_REWRITE : M_VALUES;
+            if (mode == M_REWRITE) {
+                claim[j] = chunk_round(nl);
+                moved[j] = L;
+                const int hub_now = nl > DCSR_HUB_LEN, hub_was = L > DCSR_HUB_LEN;
+                v[0] += (long long)nl - (long long)L;
+                v[1] += (long long)cap;
+                v[2] += hub_now - hub_was;
+                v[3] += (hub_now ? ((long long)nl + DCSR_HUB_CHUNK - 1) / DCSR_HUB_CHUNK : 0) -
+                        (hub_was ? ((long long)L + DCSR_HUB_CHUNK - 1) / DCSR_HUB_CHUNK : 0);
+                v[4] += hub_now | hub_was;
+            }
+            tnewl[t] = nl;
+            tmode[t] = mode;
         }
-        tnewl[t] = nl;
-        tmode[t] = mode;
+        csum += claim[j];
+        msum += moved[j];
     }
-    const uint32_t moved = mode == M_REWRITE ? L : 0u;
     uint32_t ctot, mtot;
-    const uint32_t cex = block_excl_scan_256(claim, lds4, &ctot);
-    const uint32_t mex = block_excl_scan_256(moved, lds4, &mtot);
+    uint32_t cex = block_excl_scan_256(csum, lds4, &ctot);
+    uint32_t mex = block_excl_scan_256(msum, lds4, &mtot);
     if (threadIdx.x == 0) {
         s_agg[0] = ctot;
         s_agg[1] = mtot;
     }
     __syncthreads();
     lb_tile_prefix<2>(s_agg, s_pre, stw, tile, tag, ctl);
-    if (t <= n) {
-        tpre[2 * t] = s_pre[0] + cex;
-        tpre[2 * t + 1] = s_pre[1] + mex;
+    cex += s_pre[0];
+    mex += s_pre[1];
+#pragma unroll
+    for (int j = 0; j < PB_IPT; ++j) {
+        const int64_t t = t0 + j;
+        if (t <= n) {
+            tpre[2 * t] = cex;
+            tpre[2 * t + 1] = mex;
+        }
+        cex += claim[j];
+        mex += moved[j];
     }
     if (threadIdx.x == 0 && tile == ntiles - 1) {
         ds->claim_total = s_pre[0] + s_agg[0];
         ds->nrewrite_old = s_pre[1] + s_agg[1];
-    }
-    long long v[NPLAN] = {0, 0, 0, 0, 0};
-    if (mode == M_REWRITE) {
-        const int hub_now = nl > DCSR_HUB_LEN, hub_was = L > DCSR_HUB_LEN;
-        v[0] = (long long)nl - (long long)L;
-        v[1] = (long long)cap;
-        v[2] = hub_now - hub_was;
-        v[3] = (hub_now ? ((long long)nl + DCSR_HUB_CHUNK - 1) / DCSR_HUB_CHUNK : 0) -
-               (hub_was ? ((long long)L + DCSR_HUB_CHUNK - 1) / DCSR_HUB_CHUNK : 0);
-        v[4] = hub_now | hub_was;
     }
 #pragma unroll
     for (int i = 0; i < NPLAN; ++i) {
@@ -532,6 +613,9 @@ struct UpdBufs {
     long long *pplan;
 };
 
+// Tiles of the probe and plan kernels (entries 0..n).
+static inline unsigned ptiles(int64_t n) { return (unsigned)((n + 1 + PB_TILE - 1) / PB_TILE); }
+
 dcsr_status upd_bufs(dcsr_handle h, int64_t n, UpdBufs &b)
 {
     const size_t n4 = (size_t)(n + 2) * 4, n8 = (size_t)(n + 2) * 8;
@@ -539,9 +623,10 @@ dcsr_status upd_bufs(dcsr_handle h, int64_t n, UpdBufs &b)
     DCSR_TRY(ws_get(h, WS_KEY_B, n8, (void **)&b.k1));
     DCSR_TRY(ws_get(h, WS_VAL_A, n4, (void **)&b.v0));
     DCSR_TRY(ws_get(h, WS_VAL_B, n4, (void **)&b.v1));
-    // look-back state of the three fused kernels (the largest grid is
-    // (n+1)/256 tiles of 2 components)
-    const size_t partw = 4 + 2 * 2 * (size_t)grid_for(n + 1, TB);
+    // look-back state of the three fused kernels ((n+1)/2048 tiles of 2
+    // components) and scan scratch of the radix passes
+    size_t partw = 4 + 2 * 2 * (size_t)ptiles(n);
+    if (radix_part_words(n) > partw) partw = radix_part_words(n);
     DCSR_TRY(ws_get(h, WS_HIST, radix_hist_words(n) * 4, (void **)&b.hist));
     DCSR_TRY(ws_get(h, WS_SCAN, partw * 4, (void **)&b.part));
     DCSR_TRY(ws_get(h, WS_UCOL, n4, (void **)&b.ucol));
@@ -558,14 +643,14 @@ dcsr_status upd_bufs(dcsr_handle h, int64_t n, UpdBufs &b)
     DCSR_TRY(ws_get(h, WS_TMODE, n4, (void **)&b.tmode));
     DCSR_TRY(ws_get(h, WS_TPAIRPRE, n8, (void **)&b.tpre));
     DCSR_TRY(ws_get(h, WS_TMP2, n4, (void **)&b.usum));
-    DCSR_TRY(ws_get(h, WS_PPROBE, (size_t)grid_for(n + 1, TB) * 16, (void **)&b.pprobe));
-    DCSR_TRY(ws_get(h, WS_PPLAN, (size_t)grid_for(n + 1, TB) * NPLAN * 8, (void **)&b.pplan));
+    DCSR_TRY(ws_get(h, WS_PPROBE, (size_t)ptiles(n) * 16, (void **)&b.pprobe));
+    DCSR_TRY(ws_get(h, WS_PPLAN, (size_t)ptiles(n) * NPLAN * 8, (void **)&b.pplan));
     return DCSR_OK;
 }
 
 dcsr_status plan_and_check(dcsr_handle h, int64_t n, UpdBufs &b, hipStream_t s)
 {
-    const unsigned nb = grid_for(n + 1, TB);
+    const unsigned nb = ptiles(n);
     k_upd_plan<<<nb, TB, 0, s>>>(n, h->ds, b.trow, b.tins, b.tdel, h->row_len, h->row_cap, b.tnewl, b.tmode, b.tpre,
                                  b.pplan, b.part, lb_next_epoch());
     k_upd_check<<<1, 1024, 0, s>>>(h->ds, (unsigned long long)h->pool_cap, b.pprobe, (int)nb, b.pplan, (int)nb);
@@ -586,7 +671,7 @@ dcsr_status prepare_launch(dcsr_handle h, const int32_t *rows, const int32_t *co
     k_upd_heads<<<(unsigned)((n + HD_TILE - 1) / HD_TILE), TB, 0, s>>>(key, val, n, h->cbits, b.part, lb_next_epoch(),
                                                                        h->ds, b.ucol, b.usum, b.ut, b.trow, b.tubeg,
                                                                        b.tins, b.tdel);
-    k_upd_probe<<<grid_for(n + 1, TB), TB, 0, s>>>(n, h->ds, h->row_start, h->row_len, h->pool_col, h->pool_val,
+    k_upd_probe<<<ptiles(n), TB, 0, s>>>(n, h->ds, h->row_start, h->row_len, h->pool_col, h->pool_val,
                                                     b.ucol, b.usum, b.ut, b.trow, b.uval, b.upos, b.ucode, b.upre,
                                                     b.tins, b.tdel, b.pprobe, b.part, lb_next_epoch());
     DCSR_HIP(hipGetLastError());
