@@ -22,7 +22,6 @@
 //   6. move surviving old entries and write inserted ones at their merged
 //      positions, then commit the row table.
 #include "dcsr_internal.h"
-#include "lookback.h"
 #include "prims.h"
 #include "segments.h"
 
@@ -70,66 +69,148 @@ __global__ void k_upd_pack(const int32_t *__restrict__ rows, const int32_t *__re
     if (__ballot(bad) && lane_id() == 0) atomicOr(&ds->ubad, 1u);
 }
 
-// Distinct coordinates and touched rows of the sorted batch, in one pass:
-// per item, head = first of its coordinate run, row head = first of its row;
-// the two counts are prefix-summed across tiles by look-back, and every head
-// writes its compacted record: ucol/usum/ut for coordinate u, trow/tubeg
-// (and zeroed tins/tdel) for touched row t.  Sums run left to right from 0 in
-// batch order, as the host path accumulates B_row[j].  The last tile stores
-// nuniq / ntouched and the tubeg[ntouched] = nuniq sentinel.
-constexpr int HD_IPT = 8;
-constexpr int HD_TILE = TB * HD_IPT;
-__global__ __launch_bounds__(TB) void k_upd_heads(const uint64_t *__restrict__ key, const uint32_t *__restrict__ val,
-                                                  int64_t n, int cbits, uint32_t *__restrict__ ws, uint32_t epoch,
-                                                  DevState *ds, int32_t *__restrict__ ucol, float *__restrict__ usum,
-                                                  uint32_t *__restrict__ ut, uint32_t *__restrict__ trow,
-                                                  uint32_t *__restrict__ tubeg, uint32_t *__restrict__ tins,
-                                                  uint32_t *__restrict__ tdel)
+// Sum of one 32-bit counter over the 256 threads of a block (fixed order).
+__device__ __forceinline__ uint32_t block_sum_u32(uint32_t v, uint32_t *lds4)
 {
-    __shared__ uint64_t sk[HD_TILE + 1];
-    __shared__ uint32_t lds4[4];
-    __shared__ uint32_t s_ticket, s_agg[2], s_pre[2];
-    unsigned int *ctl = ws;
-    uint64_t *st = (uint64_t *)(ws + 4);
-    const uint32_t tag = lb_tag(epoch);
-    const int64_t tile = lb_ticket(ctl, &s_ticket);
-    const int64_t ntiles = (n + HD_TILE - 1) / HD_TILE;
-    const int64_t base = tile * HD_TILE;
-    for (int i = threadIdx.x; i < HD_TILE; i += TB) {
-        const int64_t idx = base + i;
-        sk[1 + i] = idx < n ? key[idx] : ~0ull;
-    }
-    // the key before the tile; ~0 before item 0 (differs from every valid key
-    // and, shifted, from every valid row)
-    if (threadIdx.x == 0) sk[0] = base > 0 ? key[base - 1] : ~0ull;
+    const uint32_t w = wave_sum_u32(v);
+    if (lane_id() == 0) lds4[threadIdx.x >> 6] = w;
     __syncthreads();
-    uint32_t hmask = 0, rmask = 0;
+    const uint32_t t = lds4[0] + lds4[1] + lds4[2] + lds4[3];
+    __syncthreads();
+    return t;
+}
+
+__device__ __forceinline__ long long wave_sum_i64(long long v)
+{
 #pragma unroll
-    for (int j = 0; j < HD_IPT; ++j) {
-        const int i = threadIdx.x * HD_IPT + j;
-        const uint64_t k = sk[1 + i], p = sk[i];
-        const bool valid = k != ~0ull;
-        if (valid && p != k) hmask |= 1u << j;
-        if (valid && (p >> cbits) != (k >> cbits)) rmask |= 1u << j;
-    }
-    uint32_t htot, rtot;
-    const uint32_t hex = block_excl_scan_256(__popc(hmask), lds4, &htot);
-    const uint32_t rex = block_excl_scan_256(__popc(rmask), lds4, &rtot);
+    for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o);
+    return v;
+}
+
+// ---------------------------------------------------------------------------
+// Tile tables.  The per-element passes work on tiles of 2048 entries (8 per
+// thread, blocked).  Each tile writes its counts to a table; the last
+// workgroup of the launch to finish (an atomic finish counter: nothing ever
+// waits) turns the table's columns into exclusive offsets, and a later
+// kernel adds them to the in-tile prefixes.
+// ---------------------------------------------------------------------------
+constexpr int IPT = 8;
+constexpr int TILE = TB * IPT;
+// batch tile table: heads / row heads (count, offset), claims / moved (sum, offset)
+enum { TT_H = 0, TT_R, TT_HX, TT_RX, TT_C, TT_M, TT_CX, TT_MX, TT_WORDS };
+// probe tile table: inserts, adds, deletes; insert / delete offsets
+enum { TP_I = 0, TP_A, TP_D, TP_IX, TP_DX, TP_WORDS = 8 };
+// finish counters (self-resetting)
+enum { FC_HEADS = 0, FC_PLAN = 1 };
+
+// True in every thread of the last workgroup of the launch to finish; it then
+// sees what every other workgroup wrote before finishing.
+__device__ __forceinline__ bool last_block(unsigned int *ctr, uint32_t *s_flag)
+{
+    __syncthreads();
     if (threadIdx.x == 0) {
-        s_agg[0] = htot;
-        s_agg[1] = rtot;
+        __threadfence();
+        const bool last = atomicAdd(ctr, 1u) == gridDim.x - 1;
+        if (last) atomicExch(ctr, 0u);
+        *s_flag = last ? 1u : 0u;
     }
     __syncthreads();
-    lb_tile_prefix<2>(s_agg, s_pre, st, tile, tag, ctl);
-    uint32_t u = s_pre[0] + hex;
-    uint32_t t = s_pre[1] + rex;  // row heads before the thread's first item
+    const bool last = *s_flag != 0u;
+    if (last) __threadfence();
+    return last;
+}
+
+// One workgroup: exclusive scan of column `src` of a table with `stride`
+// words per row (cnt rows) into column `dst`; returns the column total.
+__device__ uint32_t block_scan_column(uint32_t *a, int64_t cnt, int stride, int src, int dst, uint32_t *lds4)
+{
+    uint32_t carry = 0;
+    for (int64_t base = 0; base < cnt; base += TB) {
+        const int64_t i = base + threadIdx.x;
+        const uint32_t v = i < cnt ? a[i * stride + src] : 0u;
+        uint32_t tot;
+        const uint32_t ex = block_excl_scan_256(v, lds4, &tot);
+        if (i < cnt) a[i * stride + dst] = carry + ex;
+        carry += tot;
+    }
+    return carry;
+}
+
+// Heads of the sorted batch: an item is a coordinate head if it is the
+// first of its (row, col) run and a row head if the first of its row; items
+// past n and invalid keys (~0) are neither.
+struct HeadFlags {
+    uint32_t h, r;  // bit j: item j of the thread
+};
+__device__ __forceinline__ HeadFlags head_flags(const uint64_t *__restrict__ key, int64_t n, int64_t i0, int cbits)
+{
+    HeadFlags f{0u, 0u};
+    // ~0 before item 0: differs from every valid key and, shifted, from
+    // every valid row
+    uint64_t prev = (i0 > 0 && i0 - 1 < n) ? key[i0 - 1] : ~0ull;
 #pragma unroll
-    for (int j = 0; j < HD_IPT; ++j) {
-        if (!((hmask >> j) & 1u)) continue;
-        const bool rh = (rmask >> j) & 1u;
-        const int64_t idx = base + threadIdx.x * HD_IPT + j;
-        const uint64_t k = sk[1 + threadIdx.x * HD_IPT + j];
-        const uint32_t tr = rh ? t : t - 1u;
+    for (int j = 0; j < IPT; ++j) {
+        const int64_t i = i0 + j;
+        const uint64_t k = i < n ? key[i] : ~0ull;
+        const bool valid = k != ~0ull;
+        if (valid && prev != k) f.h |= 1u << j;
+        if (valid && (prev >> cbits) != (k >> cbits)) f.r |= 1u << j;
+        prev = k;
+    }
+    return f;
+}
+
+// Pass 1 of the heads: per-tile counts; the last workgroup makes them
+// offsets and stores nuniq / ntouched.
+__global__ __launch_bounds__(TB) void k_upd_heads_count(const uint64_t *__restrict__ key, int64_t n, int cbits,
+                                                        uint32_t *__restrict__ tt, unsigned int *fin, DevState *ds)
+{
+    __shared__ uint32_t lds4[4], s_flag;
+    const int64_t tile = blockIdx.x;
+    const HeadFlags f = head_flags(key, n, tile * TILE + (int64_t)threadIdx.x * IPT, cbits);
+    const uint32_t hc = block_sum_u32(__popc(f.h), lds4);
+    const uint32_t rc = block_sum_u32(__popc(f.r), lds4);
+    if (threadIdx.x == 0) {
+        tt[tile * TT_WORDS + TT_H] = hc;
+        tt[tile * TT_WORDS + TT_R] = rc;
+    }
+    if (last_block(&fin[FC_HEADS], &s_flag)) {
+        const uint32_t nu = block_scan_column(tt, gridDim.x, TT_WORDS, TT_H, TT_HX, lds4);
+        const uint32_t nt = block_scan_column(tt, gridDim.x, TT_WORDS, TT_R, TT_RX, lds4);
+        if (threadIdx.x == 0) {
+            ds->nuniq = nu;
+            ds->ntouched = nt;
+        }
+    }
+}
+
+// Pass 2 of the heads: every head writes its compacted record: ucol/usum/ut
+// for coordinate u, trow/tubeg (and zeroed tins/tdel) for touched row t.
+// Sums run left to right from 0 in batch order, as the host path
+// accumulates B_row[j].  Workgroup 0 stores the tubeg[ntouched] = nuniq
+// sentinel.
+__global__ __launch_bounds__(TB) void k_upd_heads_emit(const uint64_t *__restrict__ key,
+                                                       const uint32_t *__restrict__ val, int64_t n, int cbits,
+                                                       const uint32_t *__restrict__ tt, const DevState *ds,
+                                                       int32_t *__restrict__ ucol, float *__restrict__ usum,
+                                                       uint32_t *__restrict__ ut, uint32_t *__restrict__ trow,
+                                                       uint32_t *__restrict__ tubeg, uint32_t *__restrict__ tins,
+                                                       uint32_t *__restrict__ tdel)
+{
+    __shared__ uint32_t lds4[4];
+    const int64_t tile = blockIdx.x;
+    const int64_t i0 = tile * TILE + (int64_t)threadIdx.x * IPT;
+    const HeadFlags f = head_flags(key, n, i0, cbits);
+    uint32_t htot, rtot;
+    uint32_t u = block_excl_scan_256(__popc(f.h), lds4, &htot) + tt[tile * TT_WORDS + TT_HX];
+    uint32_t t = block_excl_scan_256(__popc(f.r), lds4, &rtot) + tt[tile * TT_WORDS + TT_RX];
+#pragma unroll
+    for (int j = 0; j < IPT; ++j) {
+        if (!((f.h >> j) & 1u)) continue;
+        const bool rh = (f.r >> j) & 1u;
+        const int64_t idx = i0 + j;
+        const uint64_t k = key[idx];
+        const uint32_t tr = rh ? t : t - 1u;  // a non-head-of-row item belongs to the last row head
         float s = 0.0f + __uint_as_float(val[idx]);
         for (int64_t m = idx + 1; m < n && key[m] == k; ++m) s = s + __uint_as_float(val[m]);
         ucol[u] = (int32_t)(k & ((1ull << cbits) - 1ull));
@@ -144,78 +225,42 @@ __global__ __launch_bounds__(TB) void k_upd_heads(const uint64_t *__restrict__ k
         }
         ++u;
     }
-    if (threadIdx.x == 0 && tile == ntiles - 1) {
-        const uint32_t nu = s_pre[0] + s_agg[0], nt = s_pre[1] + s_agg[1];
-        ds->nuniq = nu;
-        ds->ntouched = nt;
-        tubeg[nt] = nu;
-    }
-    __syncthreads();
-    if (threadIdx.x == 0) lb_finish(ctl);
+    if (tile == 0 && threadIdx.x == 0) tubeg[ds->ntouched] = ds->nuniq;
 }
 
-// Sum of one 32-bit counter over the 256 threads of a block (fixed order).
-__device__ __forceinline__ uint32_t block_sum_u32(uint32_t v, uint32_t *lds4)
-{
-    uint32_t w = wave_sum_u32(v);
-    if (lane_id() == 0) lds4[threadIdx.x >> 6] = w;
-    __syncthreads();
-    uint32_t t = lds4[0] + lds4[1] + lds4[2] + lds4[3];
-    __syncthreads();
-    return t;
-}
-
-__device__ __forceinline__ long long wave_sum_i64(long long v)
-{
-#pragma unroll
-    for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o);
-    return v;
-}
-
-// Probe each distinct coordinate in its row, then prefix-sum the insert /
-// delete flags across tiles by look-back into upre[2u], upre[2u+1] (u <= n;
-// entry nuniq holds the totals).  A tile is 2048 coordinates, 8 per thread
-// (blocked); the eight binary searches of a thread advance in lock step so
-// their loads overlap.  Per-tile counters go to pprobe[tile][0..2] (reduced
-// by the check kernel).
-constexpr int PB_IPT = 8;
-constexpr int PB_TILE = TB * PB_IPT;
-__global__ __launch_bounds__(TB) void k_upd_probe(int64_t n, const DevState *__restrict__ ds,
+// Probe every distinct coordinate in its row: existing -> old + sum (delete
+// when it cancels to 0.0), missing -> insert when the sum is not 0.0.  The
+// eight binary searches of a thread advance in lock step so their loads
+// overlap.  Per-tile counts go to the probe table.
+__global__ __launch_bounds__(TB) void k_upd_probe(const DevState *__restrict__ ds,
                                                   const uint32_t *__restrict__ row_start,
                                                   const int32_t *__restrict__ row_len,
                                                   const int32_t *__restrict__ pcol, const float *__restrict__ pval,
                                                   const int32_t *__restrict__ ucol, const float *__restrict__ usum,
                                                   const uint32_t *__restrict__ ut, const uint32_t *__restrict__ trow,
                                                   float *__restrict__ uval, uint32_t *__restrict__ upos,
-                                                  uint32_t *__restrict__ ucode, uint32_t *__restrict__ upre,
-                                                  uint32_t *__restrict__ tins, uint32_t *__restrict__ tdel,
-                                                  uint32_t *__restrict__ pprobe, uint32_t *__restrict__ ws,
-                                                  uint32_t epoch)
+                                                  uint32_t *__restrict__ ucode, uint32_t *__restrict__ tins,
+                                                  uint32_t *__restrict__ tdel, uint32_t *__restrict__ tp)
 {
     __shared__ uint32_t lds4[4];
-    __shared__ uint32_t s_ticket, s_agg[2], s_pre[2];
-    unsigned int *ctl = ws;
-    uint64_t *st = (uint64_t *)(ws + 4);
-    const uint32_t tag = lb_tag(epoch);
-    const int64_t tile = lb_ticket(ctl, &s_ticket);
-    const int64_t u0 = tile * PB_TILE + (int64_t)threadIdx.x * PB_IPT;
+    const int64_t tile = blockIdx.x;
+    const int64_t u0 = tile * TILE + (int64_t)threadIdx.x * IPT;
     const uint32_t nu = ds->nuniq;
-    uint64_t rs[PB_IPT];
-    int32_t lo[PB_IPT], hi[PB_IPT], cc[PB_IPT], L[PB_IPT];
-    uint32_t tt[PB_IPT];
+    uint64_t rs[IPT];
+    int32_t lo[IPT], hi[IPT], cc[IPT], L[IPT];
+    uint32_t tt[IPT];
 #pragma unroll
-    for (int j = 0; j < PB_IPT; ++j) {
-        const int64_t u = u0 + j;
+    for (int j = 0; j < IPT; ++j) {
         lo[j] = hi[j] = L[j] = cc[j] = 0;
         tt[j] = 0;
         rs[j] = 0;
-        if (u < nu) {
-            tt[j] = ut[u];
-            cc[j] = ucol[u];
+        if (u0 + j < nu) {
+            tt[j] = ut[u0 + j];
+            cc[j] = ucol[u0 + j];
         }
     }
 #pragma unroll
-    for (int j = 0; j < PB_IPT; ++j) {
+    for (int j = 0; j < IPT; ++j) {
         if (u0 + j < nu) {
             const uint32_t r = trow[tt[j]];
             rs[j] = row_start[r];
@@ -223,12 +268,12 @@ __global__ __launch_bounds__(TB) void k_upd_probe(int64_t n, const DevState *__r
             hi[j] = L[j];
         }
     }
-    // lower bound of cc[j] in its row, eight searches in lock step
+    // lower bound of cc[j] in its row
     for (;;) {
         bool any = false;
-        int32_t pc[PB_IPT];
+        int32_t pc[IPT];
 #pragma unroll
-        for (int j = 0; j < PB_IPT; ++j) {
+        for (int j = 0; j < IPT; ++j) {
             pc[j] = 0;
             if (lo[j] < hi[j]) {
                 pc[j] = pcol[rs[j] + (uint32_t)((lo[j] + hi[j]) >> 1)];
@@ -237,18 +282,17 @@ __global__ __launch_bounds__(TB) void k_upd_probe(int64_t n, const DevState *__r
         }
         if (!any) break;
 #pragma unroll
-        for (int j = 0; j < PB_IPT; ++j) {
+        for (int j = 0; j < IPT; ++j) {
             if (lo[j] < hi[j]) {
                 const int32_t mid = (lo[j] + hi[j]) >> 1;
                 if (pc[j] < cc[j]) lo[j] = mid + 1; else hi[j] = mid;
             }
         }
     }
-    uint32_t code[PB_IPT];
-    int32_t hc[PB_IPT];
-    float hv[PB_IPT];
+    int32_t hc[IPT];
+    float hv[IPT];
 #pragma unroll
-    for (int j = 0; j < PB_IPT; ++j) {
+    for (int j = 0; j < IPT; ++j) {
         hc[j] = -1;
         hv[j] = 0.0f;
         if (u0 + j < nu && lo[j] < L[j]) {
@@ -258,140 +302,85 @@ __global__ __launch_bounds__(TB) void k_upd_probe(int64_t n, const DevState *__r
     }
     uint32_t ni = 0, nd = 0, na = 0;
 #pragma unroll
-    for (int j = 0; j < PB_IPT; ++j) {
+    for (int j = 0; j < IPT; ++j) {
         const int64_t u = u0 + j;
-        code[j] = C_NOP;
-        if (u < nu) {
-            const bool hit = hc[j] == cc[j];
-            const float s = usum[u];
-            float nv;
-            if (hit) {
-                nv = hv[j] + s;
-                code[j] = (nv != 0.0f) ? C_ADD : C_DEL;
-            } else {
-                nv = s;
-                code[j] = (s != 0.0f) ? C_INS : C_NOP;
-            }
-            uval[u] = nv;
-            upos[u] = (uint32_t)lo[j];
-            ucode[u] = code[j];
-            if (code[j] == C_INS) atomicAdd(&tins[tt[j]], 1u);
-            if (code[j] == C_DEL) atomicAdd(&tdel[tt[j]], 1u);
+        if (u >= nu) continue;
+        uint32_t code;
+        float nv;
+        const float s = usum[u];
+        if (hc[j] == cc[j]) {
+            nv = hv[j] + s;
+            code = (nv != 0.0f) ? C_ADD : C_DEL;
+        } else {
+            nv = s;
+            code = (s != 0.0f) ? C_INS : C_NOP;
         }
-        ni += code[j] == C_INS;
-        nd += code[j] == C_DEL;
-        na += code[j] == C_ADD;
+        uval[u] = nv;
+        upos[u] = (uint32_t)lo[j];
+        ucode[u] = code;
+        if (code == C_INS) atomicAdd(&tins[tt[j]], 1u);
+        if (code == C_DEL) atomicAdd(&tdel[tt[j]], 1u);
+        ni += code == C_INS;
+        nd += code == C_DEL;
+        na += code == C_ADD;
     }
-    uint32_t itot, dtot;
-    uint32_t iex = block_excl_scan_256(ni, lds4, &itot);
-    uint32_t dex = block_excl_scan_256(nd, lds4, &dtot);
+    ni = block_sum_u32(ni, lds4);
+    nd = block_sum_u32(nd, lds4);
+    na = block_sum_u32(na, lds4);
     if (threadIdx.x == 0) {
-        s_agg[0] = itot;
-        s_agg[1] = dtot;
-    }
-    __syncthreads();
-    lb_tile_prefix<2>(s_agg, s_pre, st, tile, tag, ctl);
-    iex += s_pre[0];
-    dex += s_pre[1];
-#pragma unroll
-    for (int j = 0; j < PB_IPT; ++j) {
-        const int64_t u = u0 + j;
-        if (u <= n) {
-            upre[2 * u] = iex;
-            upre[2 * u + 1] = dex;
-        }
-        iex += code[j] == C_INS;
-        dex += code[j] == C_DEL;
-    }
-    const uint32_t atot = block_sum_u32(na, lds4);
-    if (threadIdx.x == 0) {
-        pprobe[tile * 4 + 0] = itot;
-        pprobe[tile * 4 + 1] = atot;
-        pprobe[tile * 4 + 2] = dtot;
-        lb_finish(ctl);
+        tp[tile * TP_WORDS + TP_I] = ni;
+        tp[tile * TP_WORDS + TP_A] = na;
+        tp[tile * TP_WORDS + TP_D] = nd;
     }
 }
 
-// Per touched row: new length, mode and chunk claim, then the (claim, old
-// entries to move) pairs prefix-summed across tiles by look-back into
-// tpre[2t], tpre[2t+1] (t <= n; entry ntouched holds the totals, which the
-// last tile also stores in ds->claim_total / ds->nrewrite_old).  A tile is
-// 2048 rows, 8 per thread (blocked).  Per-tile counters go to pplan.
-__global__ __launch_bounds__(TB) void k_upd_plan(int64_t n, DevState *ds, const uint32_t *__restrict__ trow,
-                                                 const uint32_t *__restrict__ tins, const uint32_t *__restrict__ tdel,
-                                                 const int32_t *__restrict__ row_len,
+// Per touched row: new length, mode (values in place / rewrite) and chunk
+// claim, per-tile sums to the batch tile table and per-tile counters to
+// pplan.  The last workgroup to finish then decides the batch: it makes the
+// claim / moved / insert / delete columns exclusive offsets, totals every
+// counter, rejects the batch (no-op) on a bad index or when the claims do
+// not fit the pool, and otherwise advances the bump pointer.
+__global__ __launch_bounds__(TB) void k_upd_plan(DevState *ds, unsigned long long pool_cap,
+                                                 const uint32_t *__restrict__ trow, const uint32_t *__restrict__ tins,
+                                                 const uint32_t *__restrict__ tdel, const int32_t *__restrict__ row_len,
                                                  const int32_t *__restrict__ row_cap, uint32_t *__restrict__ tnewl,
-                                                 uint32_t *__restrict__ tmode, uint32_t *__restrict__ tpre,
-                                                 long long *__restrict__ pplan, uint32_t *__restrict__ ws,
-                                                 uint32_t epoch)
+                                                 uint32_t *__restrict__ tmode, uint32_t *__restrict__ tt,
+                                                 uint32_t *__restrict__ tp, long long *__restrict__ pplan,
+                                                 unsigned int *fin)
 {
-    __shared__ long long lds[4][NPLAN];
-    __shared__ uint32_t lds4[4];
-    __shared__ uint32_t s_ticket, s_agg[2], s_pre[2];
-    unsigned int *ctl = ws;
-    uint64_t *stw = (uint64_t *)(ws + 4);
-    const uint32_t tag = lb_tag(epoch);
-    const int64_t tile = lb_ticket(ctl, &s_ticket);
-    const int64_t ntiles = (n + 1 + PB_TILE - 1) / PB_TILE;
-    const int64_t t0 = tile * PB_TILE + (int64_t)threadIdx.x * PB_IPT;
+    __shared__ long long lds[4][8];
+    __shared__ uint32_t lds4[4], s_flag;
+    const int64_t tile = blockIdx.x;
+    const int64_t t0 = tile * TILE + (int64_t)threadIdx.x * IPT;
     const uint32_t nt = ds->ntouched;
-    uint32_t claim[PB_IPT], moved[PB_IPT];
     uint32_t csum = 0, msum = 0;
     long long v[NPLAN] = {0, 0, 0, 0, 0};
 #pragma unroll
-    for (int j = 0; j < PB_IPT; ++j) {
+    for (int j = 0; j < IPT; ++j) {
         const int64_t t = t0 + j;
-        claim[j] = 0;
-        moved[j] = 0;
-        if (t < nt) {
-            const uint32_t r = trow[t];
-            const uint32_t L = (uint32_t)row_len[r];
-            const uint32_t cap = (uint32_t)row_cap[r];
-            const uint32_t ni = tins[t], nd = tdel[t];
-            const uint32_t nl = L + ni - nd;
-            const uint32_t mode = (ni | nd) ? M_REWRITE : M_VALUES;
-            if (mode == M_REWRITE) {
-                claim[j] = chunk_round(nl);
-                moved[j] = L;
-                const int hub_now = nl > DCSR_HUB_LEN, hub_was = L > DCSR_HUB_LEN;
-                v[0] += (long long)nl - (long long)L;
-                v[1] += (long long)cap;
-                v[2] += hub_now - hub_was;
-                v[3] += (hub_now ? ((long long)nl + DCSR_HUB_CHUNK - 1) / DCSR_HUB_CHUNK : 0) -
-                        (hub_was ? ((long long)L + DCSR_HUB_CHUNK - 1) / DCSR_HUB_CHUNK : 0);
-                v[4] += hub_now | hub_was;
-            }
-            tnewl[t] = nl;
-            tmode[t] = mode;
+        if (t >= nt) continue;
+        const uint32_t r = trow[t];
+        const uint32_t L = (uint32_t)row_len[r];
+        const uint32_t cap = (uint32_t)row_cap[r];
+        const uint32_t ni = tins[t], nd = tdel[t];
+        const uint32_t nl = L + ni - nd;
+        const uint32_t mode = (ni | nd) ? M_REWRITE : M_VALUES;
+        if (mode == M_REWRITE) {
+            csum += chunk_round(nl);
+            msum += L;
+            const int hub_now = nl > DCSR_HUB_LEN, hub_was = L > DCSR_HUB_LEN;
+            v[0] += (long long)nl - (long long)L;
+            v[1] += (long long)cap;
+            v[2] += hub_now - hub_was;
+            v[3] += (hub_now ? ((long long)nl + DCSR_HUB_CHUNK - 1) / DCSR_HUB_CHUNK : 0) -
+                    (hub_was ? ((long long)L + DCSR_HUB_CHUNK - 1) / DCSR_HUB_CHUNK : 0);
+            v[4] += hub_now | hub_was;
         }
-        csum += claim[j];
-        msum += moved[j];
+        tnewl[t] = nl;
+        tmode[t] = mode;
     }
-    uint32_t ctot, mtot;
-    uint32_t cex = block_excl_scan_256(csum, lds4, &ctot);
-    uint32_t mex = block_excl_scan_256(msum, lds4, &mtot);
-    if (threadIdx.x == 0) {
-        s_agg[0] = ctot;
-        s_agg[1] = mtot;
-    }
-    __syncthreads();
-    lb_tile_prefix<2>(s_agg, s_pre, stw, tile, tag, ctl);
-    cex += s_pre[0];
-    mex += s_pre[1];
-#pragma unroll
-    for (int j = 0; j < PB_IPT; ++j) {
-        const int64_t t = t0 + j;
-        if (t <= n) {
-            tpre[2 * t] = cex;
-            tpre[2 * t + 1] = mex;
-        }
-        cex += claim[j];
-        mex += moved[j];
-    }
-    if (threadIdx.x == 0 && tile == ntiles - 1) {
-        ds->claim_total = s_pre[0] + s_agg[0];
-        ds->nrewrite_old = s_pre[1] + s_agg[1];
-    }
+    csum = block_sum_u32(csum, lds4);
+    msum = block_sum_u32(msum, lds4);
 #pragma unroll
     for (int i = 0; i < NPLAN; ++i) {
         const long long w = wave_sum_i64(v[i]);
@@ -402,45 +391,45 @@ __global__ __launch_bounds__(TB) void k_upd_plan(int64_t n, DevState *ds, const 
         const int i = threadIdx.x;
         pplan[tile * NPLAN + i] = ((lds[0][i] + lds[1][i]) + (lds[2][i] + lds[3][i]));
     }
-    __syncthreads();
-    if (threadIdx.x == 0) lb_finish(ctl);
-}
+    if (threadIdx.x == 0) {
+        tt[tile * TT_WORDS + TT_C] = csum;
+        tt[tile * TT_WORDS + TT_M] = msum;
+    }
+    if (!last_block(&fin[FC_PLAN], &s_flag)) return;
 
-// One workgroup: reduce the per-block counters, decide, advance the bump.
-__global__ __launch_bounds__(1024) void k_upd_check(DevState *ds, unsigned long long pool_cap,
-                                                    const uint32_t *__restrict__ pprobe, int nbprobe,
-                                                    const long long *__restrict__ pplan, int nbplan)
-{
-    __shared__ long long red[16][8];
-    long long acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-    for (int b = threadIdx.x; b < nbprobe; b += 1024) {
-        acc[0] += pprobe[b * 4 + 0];
-        acc[1] += pprobe[b * 4 + 1];
-        acc[2] += pprobe[b * 4 + 2];
-    }
-    for (int b = threadIdx.x; b < nbplan; b += 1024) {
+    const int64_t ntiles = gridDim.x;
+    const uint32_t claim_total = block_scan_column(tt, ntiles, TT_WORDS, TT_C, TT_CX, lds4);
+    const uint32_t moved_total = block_scan_column(tt, ntiles, TT_WORDS, TT_M, TT_MX, lds4);
+    const uint32_t ins_total = block_scan_column(tp, ntiles, TP_WORDS, TP_I, TP_IX, lds4);
+    const uint32_t del_total = block_scan_column(tp, ntiles, TP_WORDS, TP_D, TP_DX, lds4);
+    long long acc[NPLAN + 1] = {0, 0, 0, 0, 0, 0};
+    for (int64_t b = threadIdx.x; b < ntiles; b += TB) {
 #pragma unroll
-        for (int i = 0; i < NPLAN; ++i) acc[3 + i] += pplan[(int64_t)b * NPLAN + i];
+        for (int i = 0; i < NPLAN; ++i) acc[i] += pplan[b * NPLAN + i];
+        acc[NPLAN] += tp[b * TP_WORDS + TP_A];
     }
+    __syncthreads();
 #pragma unroll
-    for (int i = 0; i < 8; ++i) {
-        long long w = wave_sum_i64(acc[i]);
-        if (lane_id() == 0) red[threadIdx.x >> 6][i] = w;
+    for (int i = 0; i < NPLAN + 1; ++i) {
+        const long long w = wave_sum_i64(acc[i]);
+        if (lane_id() == 0) lds[threadIdx.x >> 6][i] = w;
     }
     __syncthreads();
     if (threadIdx.x == 0) {
-        long long tot[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-        for (int wv = 0; wv < 16; ++wv)
-            for (int i = 0; i < 8; ++i) tot[i] += red[wv][i];
-        ds->n_ins = tot[0];
-        ds->n_add = tot[1];
-        ds->n_del = tot[2];
-        ds->d_nnz = tot[3];
-        ds->d_garbage = tot[4];
-        ds->d_hubs = tot[5];
-        ds->d_hub_chunks = tot[6];
-        ds->hub_touch = (unsigned)tot[7];
-        const unsigned long long need = ds->claim_total;
+        long long tot[NPLAN + 1];
+#pragma unroll
+        for (int i = 0; i < NPLAN + 1; ++i) tot[i] = (lds[0][i] + lds[1][i]) + (lds[2][i] + lds[3][i]);
+        ds->claim_total = claim_total;
+        ds->nrewrite_old = moved_total;
+        ds->n_ins = ins_total;
+        ds->n_add = tot[NPLAN];
+        ds->n_del = del_total;
+        ds->d_nnz = tot[0];
+        ds->d_garbage = tot[1];
+        ds->d_hubs = tot[2];
+        ds->d_hub_chunks = tot[3];
+        ds->hub_touch = (unsigned)tot[4];
+        const unsigned long long need = claim_total;
         const unsigned long long base = ds->bump;  // claims start at the current bump
         ds->need = need;
         ds->bump_saved = base;
@@ -457,7 +446,62 @@ __global__ __launch_bounds__(1024) void k_upd_check(DevState *ds, unsigned long 
             ds->go = 1;
             ds->bump = base + need;
         }
+    }
+}
 
+// Exclusive offsets per entry: upre[2u], upre[2u+1] = inserts / deletes
+// before coordinate u (u <= n; entry nuniq holds the totals) and tpre[2t],
+// tpre[2t+1] = chunk slots claimed / old entries moved before touched row t
+// (t <= n), from the in-tile prefixes plus the tile offsets.
+__global__ __launch_bounds__(TB) void k_upd_offsets(int64_t n, const DevState *__restrict__ ds,
+                                                    const uint32_t *__restrict__ ucode,
+                                                    const uint32_t *__restrict__ trow,
+                                                    const uint32_t *__restrict__ tmode,
+                                                    const uint32_t *__restrict__ tnewl,
+                                                    const int32_t *__restrict__ row_len,
+                                                    const uint32_t *__restrict__ tt, const uint32_t *__restrict__ tp,
+                                                    uint32_t *__restrict__ upre, uint32_t *__restrict__ tpre)
+{
+    __shared__ uint32_t lds4[4];
+    const int64_t tile = blockIdx.x;
+    const int64_t e0 = tile * TILE + (int64_t)threadIdx.x * IPT;
+    const uint32_t nu = ds->nuniq, nt = ds->ntouched;
+    uint32_t ins[IPT], del[IPT], cl[IPT], mv[IPT];
+    uint32_t si = 0, sd = 0, sc = 0, sm = 0;
+#pragma unroll
+    for (int j = 0; j < IPT; ++j) {
+        const int64_t e = e0 + j;
+        const uint32_t code = e < nu ? ucode[e] : (uint32_t)C_NOP;
+        ins[j] = code == C_INS;
+        del[j] = code == C_DEL;
+        cl[j] = mv[j] = 0;
+        if (e < nt && tmode[e] == M_REWRITE) {
+            cl[j] = chunk_round(tnewl[e]);
+            mv[j] = (uint32_t)row_len[trow[e]];
+        }
+        si += ins[j];
+        sd += del[j];
+        sc += cl[j];
+        sm += mv[j];
+    }
+    uint32_t tot;
+    si = block_excl_scan_256(si, lds4, &tot) + tp[tile * TP_WORDS + TP_IX];
+    sd = block_excl_scan_256(sd, lds4, &tot) + tp[tile * TP_WORDS + TP_DX];
+    sc = block_excl_scan_256(sc, lds4, &tot) + tt[tile * TT_WORDS + TT_CX];
+    sm = block_excl_scan_256(sm, lds4, &tot) + tt[tile * TT_WORDS + TT_MX];
+#pragma unroll
+    for (int j = 0; j < IPT; ++j) {
+        const int64_t e = e0 + j;
+        if (e <= n) {
+            upre[2 * e] = si;
+            upre[2 * e + 1] = sd;
+            tpre[2 * e] = sc;
+            tpre[2 * e + 1] = sm;
+        }
+        si += ins[j];
+        sd += del[j];
+        sc += cl[j];
+        sm += mv[j];
     }
 }
 
@@ -609,26 +653,24 @@ struct UpdBufs {
     int32_t *ucol;
     float *usum, *uval;
     uint32_t *ut, *upos, *ucode, *upre;
-    uint32_t *trow, *tubeg, *tins, *tdel, *tnewl, *tmode, *tpre, *pprobe;
+    uint32_t *trow, *tubeg, *tins, *tdel, *tnewl, *tmode, *tpre;
+    uint32_t *tt, *tp, *fin;
     long long *pplan;
 };
 
-// Tiles of the probe and plan kernels (entries 0..n).
-static inline unsigned ptiles(int64_t n) { return (unsigned)((n + 1 + PB_TILE - 1) / PB_TILE); }
+// Tiles of the per-entry kernels (entries 0..n: index n carries totals).
+static inline unsigned ptiles(int64_t n) { return (unsigned)((n + 1 + TILE - 1) / TILE); }
 
 dcsr_status upd_bufs(dcsr_handle h, int64_t n, UpdBufs &b)
 {
     const size_t n4 = (size_t)(n + 2) * 4, n8 = (size_t)(n + 2) * 8;
+    const size_t nt = ptiles(n);
     DCSR_TRY(ws_get(h, WS_KEY_A, n8, (void **)&b.k0));
     DCSR_TRY(ws_get(h, WS_KEY_B, n8, (void **)&b.k1));
     DCSR_TRY(ws_get(h, WS_VAL_A, n4, (void **)&b.v0));
     DCSR_TRY(ws_get(h, WS_VAL_B, n4, (void **)&b.v1));
-    // look-back state of the three fused kernels ((n+1)/2048 tiles of 2
-    // components) and scan scratch of the radix passes
-    size_t partw = 4 + 2 * 2 * (size_t)ptiles(n);
-    if (radix_part_words(n) > partw) partw = radix_part_words(n);
     DCSR_TRY(ws_get(h, WS_HIST, radix_hist_words(n) * 4, (void **)&b.hist));
-    DCSR_TRY(ws_get(h, WS_SCAN, partw * 4, (void **)&b.part));
+    DCSR_TRY(ws_get(h, WS_SCAN, radix_part_words(n) * 4, (void **)&b.part));
     DCSR_TRY(ws_get(h, WS_UCOL, n4, (void **)&b.ucol));
     DCSR_TRY(ws_get(h, WS_UPOS, n4, (void **)&b.upos));
     DCSR_TRY(ws_get(h, WS_UVAL, n4, (void **)&b.uval));
@@ -643,22 +685,23 @@ dcsr_status upd_bufs(dcsr_handle h, int64_t n, UpdBufs &b)
     DCSR_TRY(ws_get(h, WS_TMODE, n4, (void **)&b.tmode));
     DCSR_TRY(ws_get(h, WS_TPAIRPRE, n8, (void **)&b.tpre));
     DCSR_TRY(ws_get(h, WS_TMP2, n4, (void **)&b.usum));
-    DCSR_TRY(ws_get(h, WS_PPROBE, (size_t)ptiles(n) * 16, (void **)&b.pprobe));
-    DCSR_TRY(ws_get(h, WS_PPLAN, (size_t)ptiles(n) * NPLAN * 8, (void **)&b.pplan));
+    DCSR_TRY(ws_get(h, WS_TDEST, nt * TT_WORDS * 4, (void **)&b.tt));
+    DCSR_TRY(ws_get(h, WS_PPROBE, nt * TP_WORDS * 4, (void **)&b.tp));
+    DCSR_TRY(ws_get(h, WS_PPLAN, nt * NPLAN * 8, (void **)&b.pplan));
+    // finish counters: zero-filled on allocation, reset by their last user
+    DCSR_TRY(ws_get(h, WS_TOLDS, 64, (void **)&b.fin));
     return DCSR_OK;
 }
 
-dcsr_status plan_and_check(dcsr_handle h, int64_t n, UpdBufs &b, hipStream_t s)
+dcsr_status plan_and_check(dcsr_handle h, UpdBufs &b, int64_t n, hipStream_t s)
 {
-    const unsigned nb = ptiles(n);
-    k_upd_plan<<<nb, TB, 0, s>>>(n, h->ds, b.trow, b.tins, b.tdel, h->row_len, h->row_cap, b.tnewl, b.tmode, b.tpre,
-                                 b.pplan, b.part, lb_next_epoch());
-    k_upd_check<<<1, 1024, 0, s>>>(h->ds, (unsigned long long)h->pool_cap, b.pprobe, (int)nb, b.pplan, (int)nb);
+    k_upd_plan<<<ptiles(n), TB, 0, s>>>(h->ds, (unsigned long long)h->pool_cap, b.trow, b.tins, b.tdel, h->row_len,
+                                        h->row_cap, b.tnewl, b.tmode, b.tt, b.tp, b.pplan, b.fin);
     return hipGetLastError() == hipSuccess ? DCSR_OK : DCSR_ERR_HIP;
 }
 
-// Pack, sort, heads, probe, plan, check: the launches shared by the
-// synchronous and the NOSYNC paths (the matrix is only read).
+// Pack, sort, heads, probe, plan (+ decision), offsets: the launches shared
+// by the synchronous and the NOSYNC paths (the matrix is only read).
 dcsr_status prepare_launch(dcsr_handle h, const int32_t *rows, const int32_t *cols, const float *vals, int64_t n,
                            UpdBufs &b, hipStream_t s)
 {
@@ -668,14 +711,18 @@ dcsr_status prepare_launch(dcsr_handle h, const int32_t *rows, const int32_t *co
     DCSR_HIP(radix_sort_pairs(b.k0, b.v0, b.k1, b.v1, n, nbits, b.hist, b.part, &which, s));
     const uint64_t *key = which ? b.k1 : b.k0;
     const uint32_t *val = which ? b.v1 : b.v0;
-    k_upd_heads<<<(unsigned)((n + HD_TILE - 1) / HD_TILE), TB, 0, s>>>(key, val, n, h->cbits, b.part, lb_next_epoch(),
-                                                                       h->ds, b.ucol, b.usum, b.ut, b.trow, b.tubeg,
-                                                                       b.tins, b.tdel);
-    k_upd_probe<<<ptiles(n), TB, 0, s>>>(n, h->ds, h->row_start, h->row_len, h->pool_col, h->pool_val,
-                                                    b.ucol, b.usum, b.ut, b.trow, b.uval, b.upos, b.ucode, b.upre,
-                                                    b.tins, b.tdel, b.pprobe, b.part, lb_next_epoch());
+    const unsigned ht = (unsigned)((n + TILE - 1) / TILE);
+    k_upd_heads_count<<<ht, TB, 0, s>>>(key, n, h->cbits, b.tt, b.fin, h->ds);
+    k_upd_heads_emit<<<ht, TB, 0, s>>>(key, val, n, h->cbits, b.tt, h->ds, b.ucol, b.usum, b.ut, b.trow, b.tubeg,
+                                       b.tins, b.tdel);
+    k_upd_probe<<<ptiles(n), TB, 0, s>>>(h->ds, h->row_start, h->row_len, h->pool_col, h->pool_val, b.ucol, b.usum,
+                                         b.ut, b.trow, b.uval, b.upos, b.ucode, b.tins, b.tdel, b.tp);
     DCSR_HIP(hipGetLastError());
-    return plan_and_check(h, n, b, s);
+    DCSR_TRY(plan_and_check(h, b, n, s));
+    k_upd_offsets<<<ptiles(n), TB, 0, s>>>(n, h->ds, b.ucode, b.trow, b.tmode, b.tnewl, h->row_len, b.tt, b.tp, b.upre,
+                                           b.tpre);
+    DCSR_HIP(hipGetLastError());
+    return DCSR_OK;
 }
 
 }  // namespace
@@ -725,7 +772,7 @@ dcsr_status update_replan(dcsr_handle h, int64_t n, hipStream_t s, bool *go)
 {
     UpdBufs b;
     DCSR_TRY(upd_bufs(h, n, b));
-    DCSR_TRY(plan_and_check(h, n, b, s));
+    DCSR_TRY(plan_and_check(h, b, n, s));
     DCSR_TRY(pull_state(h, s));
     *go = h->hs->go != 0;
     return DCSR_OK;
