@@ -96,6 +96,20 @@ def roofline(achieved_gbs, kernel, traffic=None):
             "frac": round(achieved_gbs / HBM_PEAK_GBS, 4), "traffic": traffic, "kernel": kernel}
 
 
+def measured_traffic(name, key):
+    """HBM bytes per launch measured with rocprofv3 FETCH_SIZE / WRITE_SIZE
+    (copy-calibrated) and committed under profiles/; None if absent."""
+    path = os.path.join(ROOT, "profiles", name)
+    try:
+        with open(path) as f:
+            for ln in f:
+                if ln.startswith(key + ":"):
+                    return int(ln.split(":", 1)[1].strip())
+    except OSError:
+        pass
+    return None
+
+
 def line(args, value, unit, ms_per_step, config, higher=True, **extra):
     out = {
         "metric": METRIC,
@@ -211,7 +225,12 @@ def bench_powerlaw(args):
     upd_gbs = float(np.mean([b / 1e9 / (t / 1e3) for b, t in zip(upd_algo, upd_ms)]))
 
     if upd_ms.sum() >= spmv_ms.sum():
-        roof = roofline(upd_gbs, "update pipeline (dcsr_update, all kernels of one call)")
+        roof = roofline(upd_gbs, "update pipeline (dcsr_update, all kernels of one call)",
+                        measured_traffic("update_powerlaw_325k_traffic.txt", "traffic_bytes_per_update"))
+        if roof["traffic"] is not None and args.batch == BATCH:
+            roof["traffic_source"] = "profiles/update_powerlaw_325k_traffic.txt"
+        else:
+            roof["traffic"] = None
     else:
         roof = roofline(spmv_gbs, "flat SpMV (dcsr_spmv)")
     roof["algorithmic_bytes_per_launch"] = float(np.mean(upd_algo))
