@@ -529,7 +529,7 @@ def bench_serving(args):
 
     def run_server(rate, warmup=False):
         m = fresh()
-        srv = dcsr_amd.DcsrServer(m, max_cols=64)
+        srv = dcsr_amd.DcsrServer(m, max_cols=128)
         if warmup:
             for i, q in enumerate(warm):
                 if q[0] == "update":
@@ -563,21 +563,22 @@ def bench_serving(args):
     rate_pc = 0.9 * cap_pc
     thr_pc, lat_pc = run_per_call(rate_pc)
     p99_pc = float(np.percentile(lat_pc, 99))
-    # server: raise the offered rate until its p99 exceeds the per-call p99
+    # server at rising offered rates: the best throughput whose p99 stays
+    # within the per-call path's p99 (equal-latency comparison)
     best = None
     sweep = []
-    for mult in (1, 2, 3, 4, 6, 8, 12):
+    for mult in (1, 1.5, 2, 3, 4, 6, 8, 12):
         thr, lat, st = run_server(rate_pc * mult)
         p50, p99 = float(np.percentile(lat, 50)), float(np.percentile(lat, 99))
         sweep.append({"offered_x": mult, "throughput": thr, "p50_ms": p50, "p99_ms": p99,
                       "groups": st["query_groups"], "queries": st["queries"], "compactions": st["compactions"]})
-        if p99 <= p99_pc:
+        if p99 <= p99_pc and (best is None or thr > best["throughput"]):
             best = sweep[-1]
-        else:
+        if p99 > 4 * p99_pc:
             break
+    ratio = best["throughput"] / thr_pc if best else 0.0
     if best is None:
-        best = sweep[0]
-    ratio = best["throughput"] / thr_pc
+        best = min(sweep, key=lambda e: e["p99_ms"])
     line(args, best["throughput"], "requests/s", 1e3 / best["throughput"],
          {"workload": "serving-powerlaw", "rows": R, "nnz0": nnz0, "requests": len(reqs), "update_share": 0.1,
           "update_triples": "log-uniform 1e3..1e6, 30% deletes", "query_k": [1, 4, 16, 32]},

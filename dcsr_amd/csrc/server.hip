@@ -4,10 +4,12 @@
 // src/Fill_Test.inl).  A server owns one resident matrix and two streams:
 //
 //   * queries (SpMV, SpMM with k columns) that sit together in the queue
-//     between two updates see the same matrix version; the scheduler packs
-//     them into ONE multi-column SpMM (their x columns side by side in a
-//     row-major buffer, at most max_cols columns), so A is streamed once for
-//     the whole group, then scatters the result columns back;
+//     between two updates see the same matrix version and form one group,
+//     packed into ONE multi-column SpMM (their x columns side by side in a
+//     row-major buffer, at most max_cols columns): A is streamed once per
+//     32 packed columns and each nonzero gathers one line of the packed X
+//     row instead of one line per query; the result columns are scattered
+//     back;
 //   * an update is prepared (sort, merge plan, pool check -- reads the matrix
 //     only) on the update stream while the previous query group still runs on
 //     the query stream; its commit waits for that group (event), and the next
@@ -208,17 +210,29 @@ static dcsr_status run_update(dcsr_server_s *S, const Req &q)
     return st;
 }
 
+static int req_cols(const dcsr_request &r) { return r.kind == DCSR_REQ_SPMV ? 1 : (int)r.k; }
+
+static dcsr_status run_one(dcsr_server_s *S, const dcsr_request &r)
+{
+    if (r.kind == DCSR_REQ_SPMV) return dcsr_spmv(S->h, r.x, r.y, 1.0f, 0.0f, S->sq);
+    return dcsr_spmm_dense(S->h, r.x, r.k, r.ldx, r.y, r.ldy, 1.0f, 0.0f, S->sq);
+}
+
+// One group (same matrix version).  Two or more queries are packed side by
+// side into one multi-column product: A is streamed once per 32 packed
+// columns instead of once per query, and each nonzero gathers one
+// contiguous run of the packed X row (a 128-byte line holds 32 columns)
+// instead of one line per query.
 static dcsr_status run_queries(dcsr_server_s *S, const std::vector<Req> &g)
 {
     dcsr_handle h = S->h;
     hipStream_t s = S->sq;
     dcsr_status st = DCSR_OK;
-    if (g.size() == 1 && g[0].r.kind == DCSR_REQ_SPMV) {
-        st = dcsr_spmv(h, g[0].r.x, g[0].r.y, 1.0f, 0.0f, s);
-    } else if (g.size() == 1) {
-        st = dcsr_spmm_dense(h, g[0].r.x, g[0].r.k, g[0].r.ldx, g[0].r.y, g[0].r.ldy, 1.0f, 0.0f, s);
-    } else {
-        // pack the group's inputs side by side and run one SpMM
+    std::vector<const dcsr_request *> pack;
+    for (const Req &q : g) pack.push_back(&q.r);
+    if (pack.size() == 1) {
+        if (st == DCSR_OK) st = run_one(S, *pack[0]);
+    } else if (pack.size() > 1 && st == DCSR_OK) {
         const int slot = S->slot;
         S->slot = (S->slot + 1) % S->nslots;
         // the host must not rewrite a slot whose previous copy is pending
@@ -226,19 +240,19 @@ static dcsr_status run_queries(dcsr_server_s *S, const std::vector<Req> &g)
         PackDesc *hd = S->hd + (size_t)slot * MAXQ;
         PackDesc *dd = S->dd + (size_t)slot * MAXQ;
         int K = 0;
-        for (size_t i = 0; i < g.size(); ++i) {
-            const dcsr_request &r = g[i].r;
-            const int k = r.kind == DCSR_REQ_SPMV ? 1 : (int)r.k;
+        for (size_t i = 0; i < pack.size(); ++i) {
+            const dcsr_request &r = *pack[i];
+            const int k = req_cols(r);
             hd[i] = PackDesc{r.x, r.y, r.kind == DCSR_REQ_SPMV ? 1 : r.ldx, r.kind == DCSR_REQ_SPMV ? 1 : r.ldy, k, K};
             K += k;
         }
-        (void)hipMemcpyAsync(dd, hd, g.size() * sizeof(PackDesc), hipMemcpyHostToDevice, s);
+        (void)hipMemcpyAsync(dd, hd, pack.size() * sizeof(PackDesc), hipMemcpyHostToDevice, s);
         const int64_t C = h->cols, R = h->rows;
         const unsigned gx = (unsigned)((C * K + 255) / 256 < 65536 ? (C * K + 255) / 256 : 65536);
         const unsigned gy = (unsigned)((R * K + 255) / 256 < 65536 ? (R * K + 255) / 256 : 65536);
-        if (C > 0) k_pack<<<gx ? gx : 1, 256, 0, s>>>(dd, (int)g.size(), C, K, S->X);
+        if (C > 0) k_pack<<<gx ? gx : 1, 256, 0, s>>>(dd, (int)pack.size(), C, K, S->X);
         st = dcsr_spmm_dense(h, S->X, K, K, S->Y, K, 1.0f, 0.0f, s);
-        if (R > 0) k_unpack<<<gy ? gy : 1, 256, 0, s>>>(dd, (int)g.size(), R, K, S->Y);
+        if (R > 0) k_unpack<<<gy ? gy : 1, 256, 0, s>>>(dd, (int)pack.size(), R, K, S->Y);
         if (hipGetLastError() != hipSuccess) st = DCSR_ERR_HIP;
         (void)hipEventRecord(S->slot_ev[slot], s);
         S->slot_used[slot] = 1;
@@ -264,10 +278,9 @@ static void scheduler_loop(dcsr_server_s *S)
                 S->queue.pop_front();
                 is_update = true;
             } else {
-                int cols = 0;
+                int cols = 0;  // packed columns of the group
                 while (!S->queue.empty() && S->queue.front().r.kind != DCSR_REQ_UPDATE && (int)group.size() < MAXQ) {
-                    const dcsr_request &r = S->queue.front().r;
-                    const int k = r.kind == DCSR_REQ_SPMV ? 1 : (int)r.k;
+                    const int k = req_cols(S->queue.front().r);
                     if (!group.empty() && cols + k > S->max_cols) break;
                     cols += k;
                     group.push_back(S->queue.front());

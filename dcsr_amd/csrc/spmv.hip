@@ -221,35 +221,66 @@ __global__ __launch_bounds__(TB) void k_spmm_groups(uint32_t R, const uint32_t *
     }
 }
 
-// Each carry chain (consecutive units carrying into the same row) is summed
-// by one wave in a fixed order and added to the row's head value.
+// Carry chains: consecutive slots (waves / groups) carrying into the same
+// row, added to the row's head value.  A wave scans 64 slots; for each chain
+// that starts among them: a chain that ends within the 64 is summed by KB
+// lanes (one column each) serially in slot order; a longer chain (a hub row
+// spread over many slots) is summed 64 slots at a time across the lanes and
+// then reduced in a fixed tree.  Either way the order depends only on the
+// chain, so results are reproducible.
 template <int KB>
 __global__ __launch_bounds__(TB) void k_carry(uint32_t nunits, const uint32_t *__restrict__ carry_row,
                                               const float *__restrict__ carry_val, float *__restrict__ Y,
                                               int64_t ldy, float alpha)
 {
-    const uint32_t w = blockIdx.x * (TB / 64) + (threadIdx.x >> 6);
-    if (w >= nunits) return;
-    const uint32_t r = carry_row[w];
-    if (r == NONE || (w > 0 && carry_row[w - 1] == r)) return;
     const int lane = lane_id();
-    float s[KB];
-#pragma unroll
-    for (int q = 0; q < KB; ++q) s[q] = 0.0f;
-    for (uint32_t k = w + lane;; k += 64) {
-        const bool in = k < nunits && carry_row[k] == r;
-        if (in) {
-#pragma unroll
-            for (int q = 0; q < KB; ++q) s[q] = s[q] + carry_val[(uint64_t)k * KB + q];
-        }
-        if (!__ballot(in)) break;
+    const uint64_t base = ((uint64_t)blockIdx.x * (TB / 64) + (threadIdx.x >> 6)) * 64;
+    if (base >= nunits) return;
+    const uint64_t w = base + lane;
+    uint32_t r = NONE;
+    bool head = false;
+    if (w < nunits) {
+        r = carry_row[w];
+        head = r != NONE && (w == 0 || carry_row[w - 1] != r);
     }
+    uint64_t heads = __ballot(head);
+    while (heads) {
+        const int hl = __ffsll((long long)heads) - 1;
+        heads &= heads - 1;
+        const uint64_t hw = base + hl;
+        const uint32_t hr = (uint32_t)__shfl((int)r, hl);
+        // chain length within the first 64 slots from the head
+        const uint64_t k0 = hw + lane;
+        const bool in0 = k0 < nunits && carry_row[k0] == hr;
+        const uint64_t m0 = __ballot(in0);
+        const int len0 = __popcll(m0);
+        if (len0 < 64) {
+            if (lane < KB) {
+                float acc = 0.0f;
+                for (int i = 0; i < len0; ++i) acc = acc + carry_val[(hw + i) * KB + lane];
+                const uint64_t yi = (uint64_t)hr * ldy + lane;
+                Y[yi] = Y[yi] + alpha * acc;
+            }
+            continue;
+        }
+        float sacc[KB];
 #pragma unroll
-    for (int q = 0; q < KB; ++q) {
-        const float t = wave_sum_f32(s[q]);
-        if (lane == 0) {
-            const uint64_t i = (uint64_t)r * ldy + q;
-            Y[i] = Y[i] + alpha * t;
+        for (int q = 0; q < KB; ++q) sacc[q] = 0.0f;
+        for (uint64_t k = k0;; k += 64) {
+            const bool in = k < nunits && carry_row[k] == hr;
+            if (in) {
+#pragma unroll
+                for (int q = 0; q < KB; ++q) sacc[q] = sacc[q] + carry_val[k * KB + q];
+            }
+            if (__ballot(in) != ~0ull) break;
+        }
+#pragma unroll
+        for (int q = 0; q < KB; ++q) {
+            const float t = wave_sum_f32(sacc[q]);
+            if (lane == 0) {
+                const uint64_t yi = (uint64_t)hr * ldy + q;
+                Y[yi] = Y[yi] + alpha * t;
+            }
         }
     }
 }
@@ -336,7 +367,7 @@ static dcsr_status launch_flat(dcsr_handle h, const float *X, int64_t ldx, float
     const uint32_t *rowmap = (const uint32_t *)h->ws.ptr[WS_ROWMAP];
     k_flat<KB><<<blocks, TB, 0, s>>>(R, foff, rowmap, h->row_start, h->pool_col, h->pool_val, X, ldx, Y, ldy, alpha,
                                      beta, pw, crow, cval);
-    k_carry<KB><<<blocks, TB, 0, s>>>((uint32_t)nwaves, crow, cval, Y, ldy, alpha);
+    k_carry<KB><<<grid_for(nwaves, TB), TB, 0, s>>>((uint32_t)nwaves, crow, cval, Y, ldy, alpha);
     DCSR_HIP(hipGetLastError());
     return DCSR_OK;
 }
@@ -358,7 +389,7 @@ static dcsr_status launch_groups(dcsr_handle h, const float *X, int64_t ldx, flo
     const uint32_t *rowmap = (const uint32_t *)h->ws.ptr[WS_ROWMAP];
     k_spmm_groups<KB><<<blocks, TB, 0, s>>>(R, foff, rowmap, h->row_start, h->pool_col, h->pool_val, X, ldx, Y, ldy,
                                             alpha, beta, crow, cval);
-    k_carry<KB><<<grid_for(ngroups, 4), TB, 0, s>>>((uint32_t)ngroups, crow, cval, Y, ldy, alpha);
+    k_carry<KB><<<grid_for(ngroups, TB), TB, 0, s>>>((uint32_t)ngroups, crow, cval, Y, ldy, alpha);
     DCSR_HIP(hipGetLastError());
     return DCSR_OK;
 }

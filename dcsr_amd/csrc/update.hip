@@ -187,8 +187,9 @@ __global__ __launch_bounds__(TB) void k_upd_heads_count(const uint64_t *__restri
 // Pass 2 of the heads: every head writes its compacted record: ucol/usum/ut
 // for coordinate u, trow/tubeg (and zeroed tins/tdel) for touched row t.
 // Sums run left to right from 0 in batch order, as the host path
-// accumulates B_row[j].  Workgroup 0 stores the tubeg[ntouched] = nuniq
-// sentinel.
+// accumulates B_row[j].  A tile's records are contiguous, so they are
+// staged in LDS and written out coalesced.  Workgroup 0 stores the
+// tubeg[ntouched] = nuniq sentinel.
 __global__ __launch_bounds__(TB) void k_upd_heads_emit(const uint64_t *__restrict__ key,
                                                        const uint32_t *__restrict__ val, int64_t n, int cbits,
                                                        const uint32_t *__restrict__ tt, const DevState *ds,
@@ -198,32 +199,47 @@ __global__ __launch_bounds__(TB) void k_upd_heads_emit(const uint64_t *__restric
                                                        uint32_t *__restrict__ tdel)
 {
     __shared__ uint32_t lds4[4];
+    __shared__ int32_t s_ucol[TILE];
+    __shared__ float s_usum[TILE];
+    __shared__ uint32_t s_ut[TILE], s_trow[TILE], s_tub[TILE];
     const int64_t tile = blockIdx.x;
     const int64_t i0 = tile * TILE + (int64_t)threadIdx.x * IPT;
     const HeadFlags f = head_flags(key, n, i0, cbits);
     uint32_t htot, rtot;
-    uint32_t u = block_excl_scan_256(__popc(f.h), lds4, &htot) + tt[tile * TT_WORDS + TT_HX];
-    uint32_t t = block_excl_scan_256(__popc(f.r), lds4, &rtot) + tt[tile * TT_WORDS + TT_RX];
+    uint32_t lu = block_excl_scan_256(__popc(f.h), lds4, &htot);
+    uint32_t lt = block_excl_scan_256(__popc(f.r), lds4, &rtot);
+    const uint32_t ubase = tt[tile * TT_WORDS + TT_HX], tbase = tt[tile * TT_WORDS + TT_RX];
 #pragma unroll
     for (int j = 0; j < IPT; ++j) {
         if (!((f.h >> j) & 1u)) continue;
         const bool rh = (f.r >> j) & 1u;
         const int64_t idx = i0 + j;
         const uint64_t k = key[idx];
-        const uint32_t tr = rh ? t : t - 1u;  // a non-head-of-row item belongs to the last row head
         float s = 0.0f + __uint_as_float(val[idx]);
         for (int64_t m = idx + 1; m < n && key[m] == k; ++m) s = s + __uint_as_float(val[m]);
-        ucol[u] = (int32_t)(k & ((1ull << cbits) - 1ull));
-        usum[u] = s;
-        ut[u] = tr;
+        s_ucol[lu] = (int32_t)(k & ((1ull << cbits) - 1ull));
+        s_usum[lu] = s;
+        // a coordinate that is not a row head belongs to the last row head
+        // before it (possibly in an earlier tile)
+        s_ut[lu] = tbase + lt - (rh ? 0u : 1u);
         if (rh) {
-            trow[tr] = (uint32_t)(k >> cbits);
-            tubeg[tr] = u;
-            tins[tr] = 0;
-            tdel[tr] = 0;
-            ++t;
+            s_trow[lt] = (uint32_t)(k >> cbits);
+            s_tub[lt] = ubase + lu;
+            ++lt;
         }
-        ++u;
+        ++lu;
+    }
+    __syncthreads();
+    for (uint32_t i = threadIdx.x; i < htot; i += TB) {
+        ucol[ubase + i] = s_ucol[i];
+        usum[ubase + i] = s_usum[i];
+        ut[ubase + i] = s_ut[i];
+    }
+    for (uint32_t i = threadIdx.x; i < rtot; i += TB) {
+        trow[tbase + i] = s_trow[i];
+        tubeg[tbase + i] = s_tub[i];
+        tins[tbase + i] = 0;
+        tdel[tbase + i] = 0;
     }
     if (tile == 0 && threadIdx.x == 0) tubeg[ds->ntouched] = ds->nuniq;
 }
