@@ -45,8 +45,10 @@ struct PackDesc {
 
 constexpr int MAXQ = 64;  // requests per group
 
-// Packed X (C x K, row-major) from the group's inputs.
-__global__ void k_pack(const PackDesc *__restrict__ d, int nreq, int64_t C, int K, float *__restrict__ X)
+// Packed X (C x Kp, row-major, leading dimension ld) from the group's
+// inputs; columns [K, Kp) are zero padding.
+__global__ void k_pack(const PackDesc *__restrict__ d, int nreq, int64_t C, int K, int Kp, int64_t ld,
+                       float *__restrict__ X)
 {
     __shared__ PackDesc sd[MAXQ];
     __shared__ int16_t owner[MAXQ * 32];
@@ -58,17 +60,23 @@ __global__ void k_pack(const PackDesc *__restrict__ d, int nreq, int64_t C, int 
         owner[c] = (int16_t)q;
     }
     __syncthreads();
-    const int64_t n = C * K;
+    const int64_t n = C * Kp;
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
-        const int64_t r = i / K;
-        const int c = (int)(i - r * K);
-        const PackDesc &p = sd[owner[c]];
-        X[i] = p.src[r * p.ld_src + (c - p.col0)];
+        const int64_t r = i / Kp;
+        const int c = (int)(i - r * Kp);
+        float v = 0.0f;
+        if (c < K) {
+            const PackDesc &p = sd[owner[c]];
+            v = p.src[r * p.ld_src + (c - p.col0)];
+        }
+        X[r * ld + c] = v;
     }
 }
 
-// Scatter the packed Y (R x K) back to every request's output.
-__global__ void k_unpack(const PackDesc *__restrict__ d, int nreq, int64_t R, int K, const float *__restrict__ Y)
+// Scatter the packed Y (R x K of leading dimension ld) back to every
+// request's output.
+__global__ void k_unpack(const PackDesc *__restrict__ d, int nreq, int64_t R, int K, int64_t ld,
+                         const float *__restrict__ Y)
 {
     __shared__ PackDesc sd[MAXQ];
     __shared__ int16_t owner[MAXQ * 32];
@@ -85,9 +93,22 @@ __global__ void k_unpack(const PackDesc *__restrict__ d, int nreq, int64_t R, in
         const int64_t r = i / K;
         const int c = (int)(i - r * K);
         const PackDesc &p = sd[owner[c]];
-        p.dst[r * p.ld_dst + (c - p.col0)] = Y[i];
+        p.dst[r * p.ld_dst + (c - p.col0)] = Y[r * ld + c];
     }
 }
+
+// Width the packed product runs at: a column-block width (1, 2, 4, 8, 16,
+// 32) up to 32 columns, whole 32-column blocks beyond; rows of the packed
+// buffers are padded to 32 floats (128 bytes, one cache line) so each
+// nonzero's gather of a block never straddles two lines.
+static int packed_width(int K)
+{
+    if (K > 32) return (K + 31) / 32 * 32;
+    int w = 1;
+    while (w < K) w <<= 1;
+    return w;
+}
+static int64_t packed_ld(int Kp) { return (Kp + 31) / 32 * 32; }
 
 using Clock = std::chrono::steady_clock;
 
@@ -248,11 +269,13 @@ static dcsr_status run_queries(dcsr_server_s *S, const std::vector<Req> &g)
         }
         (void)hipMemcpyAsync(dd, hd, pack.size() * sizeof(PackDesc), hipMemcpyHostToDevice, s);
         const int64_t C = h->cols, R = h->rows;
-        const unsigned gx = (unsigned)((C * K + 255) / 256 < 65536 ? (C * K + 255) / 256 : 65536);
+        const int Kp = packed_width(K);
+        const int64_t ld = packed_ld(Kp);
+        const unsigned gx = (unsigned)((C * Kp + 255) / 256 < 65536 ? (C * Kp + 255) / 256 : 65536);
         const unsigned gy = (unsigned)((R * K + 255) / 256 < 65536 ? (R * K + 255) / 256 : 65536);
-        if (C > 0) k_pack<<<gx ? gx : 1, 256, 0, s>>>(dd, (int)pack.size(), C, K, S->X);
-        st = dcsr_spmm_dense(h, S->X, K, K, S->Y, K, 1.0f, 0.0f, s);
-        if (R > 0) k_unpack<<<gy ? gy : 1, 256, 0, s>>>(dd, (int)pack.size(), R, K, S->Y);
+        if (C > 0) k_pack<<<gx ? gx : 1, 256, 0, s>>>(dd, (int)pack.size(), C, K, Kp, ld, S->X);
+        st = dcsr_spmm_dense(h, S->X, Kp, ld, S->Y, ld, 1.0f, 0.0f, s);
+        if (R > 0) k_unpack<<<gy ? gy : 1, 256, 0, s>>>(dd, (int)pack.size(), R, K, ld, S->Y);
         if (hipGetLastError() != hipSuccess) st = DCSR_ERR_HIP;
         (void)hipEventRecord(S->slot_ev[slot], s);
         S->slot_used[slot] = 1;
@@ -319,8 +342,9 @@ dcsr_status dcsr_server_create(dcsr_server *out, dcsr_handle h, int32_t max_cols
     if (hipStreamCreateWithFlags(&S->sq, hipStreamNonBlocking) != hipSuccess ||
         hipStreamCreateWithFlags(&S->su, hipStreamNonBlocking) != hipSuccess)
         st = DCSR_ERR_HIP;
-    if (st == DCSR_OK && (hipMalloc(&S->X, (size_t)(h->cols > 0 ? h->cols : 1) * max_cols * 4) != hipSuccess ||
-                          hipMalloc(&S->Y, (size_t)(h->rows > 0 ? h->rows : 1) * max_cols * 4) != hipSuccess ||
+    const size_t ldmax = (size_t)packed_ld(packed_width(max_cols));
+    if (st == DCSR_OK && (hipMalloc(&S->X, (size_t)(h->cols > 0 ? h->cols : 1) * ldmax * 4) != hipSuccess ||
+                          hipMalloc(&S->Y, (size_t)(h->rows > 0 ? h->rows : 1) * ldmax * 4) != hipSuccess ||
                           hipMalloc(&S->dd, (size_t)S->nslots * MAXQ * sizeof(PackDesc)) != hipSuccess ||
                           hipHostMalloc(&S->hd, (size_t)S->nslots * MAXQ * sizeof(PackDesc), hipHostMallocDefault) !=
                               hipSuccess))

@@ -157,6 +157,183 @@ __global__ __launch_bounds__(TB) void k_flat(uint32_t R, const uint32_t *__restr
     }
 }
 
+// Wide column blocks (KB = 8, 16, 32) on the flat decomposition: a wave owns
+// DCSR_SPMV_PW consecutive nonzeros like the flat kernel and walks them 64
+// rows at a time.  The window's (col, val, row) are first staged in LDS by
+// all 64 lanes (one coalesced pass, no per-row dependent loads); then the
+// wave's 64/KB lane groups take contiguous slices of the staged entries,
+// each of a group's KB lanes accumulating one column (the X-row gathers are
+// contiguous KB*4-byte runs, several entries in flight).  Rows interior to a
+// slice are written directly; a slice's first and last partial rows are
+// combined across groups in slice order by group 0, so every row is summed
+// in entry order.  The row that started before the wave goes to the carry
+// slot, as in the flat kernel.
+constexpr int WIDE_UN = 4;
+template <int KB>
+__global__ __launch_bounds__(TB) void k_spmm_wide(uint32_t R, const uint32_t *__restrict__ foff,
+                                                  const uint32_t *__restrict__ rowmap,
+                                                  const uint32_t *__restrict__ row_start,
+                                                  const int32_t *__restrict__ col, const float *__restrict__ val,
+                                                  const float *__restrict__ X, int64_t ldx, float *__restrict__ Y,
+                                                  int64_t ldy, float alpha, float beta,
+                                                  uint32_t *__restrict__ carry_row, float *__restrict__ carry_val)
+{
+    constexpr int G = 64 / KB;
+    constexpr uint32_t PW = DCSR_SPMV_PW;
+    __shared__ uint32_t s_off[TB / 64][65];
+    __shared__ uint32_t s_st[TB / 64][64];
+    __shared__ int32_t s_col[TB / 64][PW];
+    __shared__ float s_val[TB / 64][PW];
+    __shared__ uint8_t s_row[TB / 64][PW];
+    __shared__ float s_part[TB / 64][2][G][KB];
+    __shared__ int32_t s_prow[TB / 64][2][G];
+    const int w = threadIdx.x >> 6;
+    const int lane = lane_id();
+    const int cl = lane % KB;
+    const int g = lane / KB;
+    const uint32_t wave = blockIdx.x * (TB / 64) + w;
+    const uint32_t total = foff[R];
+    const uint32_t lastw = total ? (total - 1) / PW : 0;
+    if (lane == 0) carry_row[wave] = NONE;
+    if (wave > lastw) return;
+    const uint32_t e0 = wave * PW;
+    const uint32_t e1 = (total - e0) < PW ? total : e0 + PW;
+    const bool last = wave == lastw;
+
+    uint32_t j = rowmap[e0 / GRAN];
+    while (j < R) {
+        const uint32_t r = j + lane;
+        s_off[w][lane] = r <= R ? foff[r] : total;
+        if (lane == 0) s_off[w][64] = (j + 64 <= R) ? foff[j + 64] : total;
+        s_st[w][lane] = r < R ? row_start[r] : 0u;
+        __builtin_amdgcn_wave_barrier();
+        const uint32_t wb = s_off[w][0] > e0 ? s_off[w][0] : e0;
+        const uint32_t wend = s_off[w][64] < e1 ? s_off[w][64] : e1;
+        const uint32_t n = wend > wb ? wend - wb : 0u;
+        // stage the window's entries
+        for (uint32_t b = 0; b < n; b += 64) {
+            const uint32_t i = b + lane;
+            if (i < n) {
+                const uint32_t e = wb + i;
+                const uint32_t jj = (uint32_t)find_seg64(s_off[w], e);
+                const uint64_t pos = (uint64_t)s_st[w][jj] + (e - s_off[w][jj]);
+                s_col[w][i] = col[pos];
+                s_val[w][i] = val[pos];
+                s_row[w][i] = (uint8_t)jj;
+            }
+        }
+        __builtin_amdgcn_wave_barrier();
+        // this group's slice of the staged entries
+        const uint32_t L = (n + G - 1) / G;
+        const uint32_t c0 = (uint32_t)g * L < n ? (uint32_t)g * L : n;
+        const uint32_t c1 = c0 + L < n ? c0 + L : n;
+        int cur = -1, nseg = 0;
+        float acc = 0.0f;
+        for (uint32_t i = c0; i < c1; i += WIDE_UN) {
+            int32_t cc[WIDE_UN];
+            float vv[WIDE_UN], xv[WIDE_UN];
+            int rr[WIDE_UN];
+#pragma unroll
+            for (int u = 0; u < WIDE_UN; ++u) {
+                const uint32_t idx = i + u;
+                const bool ok = idx < c1;
+                cc[u] = ok ? s_col[w][idx] : 0;
+                vv[u] = ok ? s_val[w][idx] : 0.0f;
+                rr[u] = ok ? (int)s_row[w][idx] : -1;
+            }
+#pragma unroll
+            for (int u = 0; u < WIDE_UN; ++u) xv[u] = rr[u] >= 0 ? X[(int64_t)cc[u] * ldx + cl] : 0.0f;
+#pragma unroll
+            for (int u = 0; u < WIDE_UN; ++u) {
+                if (rr[u] < 0) continue;
+                if (rr[u] != cur) {
+                    if (cur >= 0) {
+                        if (nseg == 0) {
+                            // first segment of the slice: may continue the previous slice
+                            s_part[w][0][g][cl] = acc;
+                            if (cl == 0) s_prow[w][0][g] = cur;
+                        } else {
+                            // interior row: starts and ends in this slice, starts in the wave
+                            const uint64_t yi = (uint64_t)(j + cur) * ldy + cl;
+                            Y[yi] = combine(alpha, acc, beta, Y, yi);
+                        }
+                        ++nseg;
+                    }
+                    cur = rr[u];
+                    acc = 0.0f;
+                }
+                acc = acc + vv[u] * xv[u];
+            }
+        }
+        // the slice's last segment (or its only one)
+        if (nseg == 0) {
+            s_part[w][0][g][cl] = acc;
+            if (cl == 0) {
+                s_prow[w][0][g] = cur;  // -1 for an empty slice
+                s_prow[w][1][g] = -1;
+            }
+        } else {
+            s_part[w][1][g][cl] = acc;
+            if (cl == 0) s_prow[w][1][g] = cur;
+        }
+        __builtin_amdgcn_wave_barrier();
+        // combine slice boundary partials in slice order (group 0)
+        if (g == 0) {
+            int crow = -1;
+            float cacc = 0.0f;
+            for (int gg = 0; gg < G; ++gg) {
+#pragma unroll
+                for (int part = 0; part < 2; ++part) {
+                    const int row = s_prow[w][part][gg];
+                    if (row < 0) continue;
+                    const float a = s_part[w][part][gg][cl];
+                    if (row == crow) {
+                        cacc = cacc + a;
+                        continue;
+                    }
+                    if (crow >= 0) {
+                        const uint32_t rs = s_off[w][crow];
+                        if (rs >= e0) {
+                            const uint64_t yi = (uint64_t)(j + crow) * ldy + cl;
+                            Y[yi] = combine(alpha, cacc, beta, Y, yi);
+                        } else {
+                            if (cl == 0) carry_row[wave] = j + crow;
+                            carry_val[(uint64_t)wave * KB + cl] = cacc;
+                        }
+                    }
+                    crow = row;
+                    cacc = a;
+                }
+            }
+            if (crow >= 0) {
+                const uint32_t rs = s_off[w][crow];
+                if (rs >= e0) {
+                    const uint64_t yi = (uint64_t)(j + crow) * ldy + cl;
+                    Y[yi] = combine(alpha, cacc, beta, Y, yi);
+                } else {
+                    if (cl == 0) carry_row[wave] = j + crow;
+                    carry_val[(uint64_t)wave * KB + cl] = cacc;
+                }
+            }
+        }
+        // empty rows that start in the wave's range (and trailing empty rows)
+        if (r < R) {
+            const uint32_t rs = s_off[w][lane], re = s_off[w][lane + 1];
+            if (re == rs && ((rs >= e0 && rs < e1) || (last && rs == total))) {
+#pragma unroll
+                for (int q = 0; q < KB; ++q) {
+                    const uint64_t yi = (uint64_t)r * ldy + q;
+                    Y[yi] = beta == 0.0f ? 0.0f : beta * Y[yi];
+                }
+            }
+        }
+        const uint32_t next_start = s_off[w][64];
+        __builtin_amdgcn_wave_barrier();
+        j += 64;
+        if (!(next_start < e1 || (last && j < R))) break;
+    }
+}
+
 // Wide column blocks (KB = 8, 16, 32): a wave holds 64/KB lane groups; group
 // g owns GRAN consecutive nonzeros of the flat order and walks them row by
 // row, each of its KB lanes accumulating one column of Y (the (col, val)
@@ -373,6 +550,28 @@ static dcsr_status launch_flat(dcsr_handle h, const float *X, int64_t ldx, float
 }
 
 template <int KB>
+static dcsr_status launch_wide(dcsr_handle h, const float *X, int64_t ldx, float *Y, int64_t ldy, float alpha,
+                               float beta, hipStream_t s)
+{
+    const uint32_t R = (uint32_t)h->rows;
+    int64_t nwaves = (h->nnz_h + DCSR_SPMV_PW - 1) / DCSR_SPMV_PW;
+    if (nwaves < 1) nwaves = 1;
+    const unsigned blocks = (unsigned)((nwaves + 3) / 4);
+    nwaves = (int64_t)blocks * 4;
+    uint32_t *crow;
+    float *cval;
+    DCSR_TRY(ws_get(h, WS_SPMM_CROW, (size_t)nwaves * 4, (void **)&crow));
+    DCSR_TRY(ws_get(h, WS_SPMM_CVAL, (size_t)nwaves * 4 * KB, (void **)&cval));
+    const uint32_t *foff = (const uint32_t *)h->ws.ptr[WS_FOFF];
+    const uint32_t *rowmap = (const uint32_t *)h->ws.ptr[WS_ROWMAP];
+    k_spmm_wide<KB><<<blocks, TB, 0, s>>>(R, foff, rowmap, h->row_start, h->pool_col, h->pool_val, X, ldx, Y, ldy,
+                                          alpha, beta, crow, cval);
+    k_carry<KB><<<grid_for(nwaves, TB), TB, 0, s>>>((uint32_t)nwaves, crow, cval, Y, ldy, alpha);
+    DCSR_HIP(hipGetLastError());
+    return DCSR_OK;
+}
+
+template <int KB>
 static dcsr_status launch_groups(dcsr_handle h, const float *X, int64_t ldx, float *Y, int64_t ldy, float alpha,
                                  float beta, hipStream_t s)
 {
@@ -425,9 +624,9 @@ dcsr_status dcsr_spmm_dense(dcsr_handle h, const float *X, int64_t k, int64_t ld
         const int64_t rem = k - cb;
         const float *Xb = X + cb;
         float *Yb = Y + cb;
-        if (rem >= 32) { DCSR_TRY(launch_groups<32>(h, Xb, ldx, Yb, ldy, alpha, beta, s)); cb += 32; }
-        else if (rem >= 16) { DCSR_TRY(launch_groups<16>(h, Xb, ldx, Yb, ldy, alpha, beta, s)); cb += 16; }
-        else if (rem >= 8) { DCSR_TRY(launch_groups<8>(h, Xb, ldx, Yb, ldy, alpha, beta, s)); cb += 8; }
+        if (rem >= 32) { DCSR_TRY(launch_wide<32>(h, Xb, ldx, Yb, ldy, alpha, beta, s)); cb += 32; }
+        else if (rem >= 16) { DCSR_TRY(launch_wide<16>(h, Xb, ldx, Yb, ldy, alpha, beta, s)); cb += 16; }
+        else if (rem >= 8) { DCSR_TRY(launch_wide<8>(h, Xb, ldx, Yb, ldy, alpha, beta, s)); cb += 8; }
         else if (rem >= 4) { DCSR_TRY(launch_flat<4>(h, Xb, ldx, Yb, ldy, alpha, beta, s)); cb += 4; }
         else if (rem >= 2) { DCSR_TRY(launch_flat<2>(h, Xb, ldx, Yb, ldy, alpha, beta, s)); cb += 2; }
         else { DCSR_TRY(launch_flat<1>(h, Xb, ldx, Yb, ldy, alpha, beta, s)); cb += 1; }
