@@ -334,70 +334,6 @@ __global__ __launch_bounds__(TB) void k_spmm_wide(uint32_t R, const uint32_t *__
     }
 }
 
-// Wide column blocks (KB = 8, 16, 32): a wave holds 64/KB lane groups; group
-// g owns GRAN consecutive nonzeros of the flat order and walks them row by
-// row, each of its KB lanes accumulating one column of Y (the (col, val)
-// loads are broadcasts, the X-row gathers are contiguous KB*4 bytes), so no
-// cross-lane reduction is needed.  Rows that started in an earlier group are
-// left as carries, like the flat kernel.
-template <int KB>
-__global__ __launch_bounds__(TB) void k_spmm_groups(uint32_t R, const uint32_t *__restrict__ foff,
-                                                    const uint32_t *__restrict__ rowmap,
-                                                    const uint32_t *__restrict__ row_start,
-                                                    const int32_t *__restrict__ col, const float *__restrict__ val,
-                                                    const float *__restrict__ X, int64_t ldx, float *__restrict__ Y,
-                                                    int64_t ldy, float alpha, float beta,
-                                                    uint32_t *__restrict__ carry_row, float *__restrict__ carry_val)
-{
-    constexpr int G = 64 / KB;
-    constexpr int UN = 4;
-    const int lane = lane_id();
-    const int cl = lane % KB;
-    const uint32_t gid = (blockIdx.x * (TB / 64) + (threadIdx.x >> 6)) * G + lane / KB;
-    const uint32_t total = foff[R];
-    const uint32_t lastg = total ? (total - 1) / GRAN : 0;
-    if (cl == 0) carry_row[gid] = NONE;
-    if (gid > lastg) return;
-    const uint32_t g0 = gid * GRAN;
-    const uint32_t g1 = (total - g0) < GRAN ? total : g0 + GRAN;
-    const bool last = gid == lastg;
-    for (uint32_t r = rowmap[gid]; r < R; ++r) {
-        const uint32_t rs = foff[r], re = foff[r + 1];
-        if (rs >= g1 && !(last && rs == total)) break;
-        const uint64_t yi = (uint64_t)r * ldy + cl;
-        if (re > rs) {
-            const uint32_t a = rs > g0 ? rs : g0;
-            const uint32_t b = re < g1 ? re : g1;
-            if (a >= b) continue;
-            const uint64_t st = (uint64_t)row_start[r] - rs;
-            float acc = 0.0f;
-            uint32_t e = a;
-            for (; e + UN <= b; e += UN) {
-                int32_t c[UN];
-                float v[UN], xv[UN];
-#pragma unroll
-                for (int u = 0; u < UN; ++u) {
-                    c[u] = col[st + e + u];
-                    v[u] = val[st + e + u];
-                }
-#pragma unroll
-                for (int u = 0; u < UN; ++u) xv[u] = X[(int64_t)c[u] * ldx + cl];
-#pragma unroll
-                for (int u = 0; u < UN; ++u) acc = acc + v[u] * xv[u];
-            }
-            for (; e < b; ++e) acc = acc + val[st + e] * X[(int64_t)col[st + e] * ldx + cl];
-            if (rs >= g0) {
-                Y[yi] = combine(alpha, acc, beta, Y, yi);
-            } else {
-                if (cl == 0) carry_row[gid] = r;
-                carry_val[(uint64_t)gid * KB + cl] = acc;
-            }
-        } else if ((rs >= g0 && rs < g1) || (last && rs == total)) {
-            Y[yi] = beta == 0.0f ? 0.0f : beta * Y[yi];
-        }
-    }
-}
-
 // Carry chains: consecutive slots (waves / groups) carrying into the same
 // row, added to the row's head value.  A wave scans 64 slots; for each chain
 // that starts among them: a chain that ends within the 64 is summed by KB
@@ -567,28 +503,6 @@ static dcsr_status launch_wide(dcsr_handle h, const float *X, int64_t ldx, float
     k_spmm_wide<KB><<<blocks, TB, 0, s>>>(R, foff, rowmap, h->row_start, h->pool_col, h->pool_val, X, ldx, Y, ldy,
                                           alpha, beta, crow, cval);
     k_carry<KB><<<grid_for(nwaves, TB), TB, 0, s>>>((uint32_t)nwaves, crow, cval, Y, ldy, alpha);
-    DCSR_HIP(hipGetLastError());
-    return DCSR_OK;
-}
-
-template <int KB>
-static dcsr_status launch_groups(dcsr_handle h, const float *X, int64_t ldx, float *Y, int64_t ldy, float alpha,
-                                 float beta, hipStream_t s)
-{
-    const uint32_t R = (uint32_t)h->rows;
-    constexpr int G = 64 / KB;
-    int64_t ngroups = h->nnz_h / GRAN + 1;
-    const unsigned blocks = (unsigned)((ngroups + G * 4 - 1) / (G * 4));
-    ngroups = (int64_t)blocks * 4 * G;
-    uint32_t *crow;
-    float *cval;
-    DCSR_TRY(ws_get(h, WS_SPMM_CROW, (size_t)ngroups * 4, (void **)&crow));
-    DCSR_TRY(ws_get(h, WS_SPMM_CVAL, (size_t)ngroups * 4 * KB, (void **)&cval));
-    const uint32_t *foff = (const uint32_t *)h->ws.ptr[WS_FOFF];
-    const uint32_t *rowmap = (const uint32_t *)h->ws.ptr[WS_ROWMAP];
-    k_spmm_groups<KB><<<blocks, TB, 0, s>>>(R, foff, rowmap, h->row_start, h->pool_col, h->pool_val, X, ldx, Y, ldy,
-                                            alpha, beta, crow, cval);
-    k_carry<KB><<<grid_for(ngroups, TB), TB, 0, s>>>((uint32_t)ngroups, crow, cval, Y, ldy, alpha);
     DCSR_HIP(hipGetLastError());
     return DCSR_OK;
 }

@@ -8,19 +8,25 @@
 // COO->CSR, cusp::add; cusp/system/detail/sequential/elementwise.h:44-131):
 //
 //   1. pack (row, col) keys, stable LSD radix sort of (key, value) pairs;
-//   2. run heads: one entry per distinct coordinate, summed left-to-right in
-//      batch order starting from 0 (the host path's B_row[j] accumulation);
+//   2. heads (two passes over 2048-entry tiles): one record per distinct
+//      coordinate, summed left-to-right in batch order starting from 0 (the
+//      host path's B_row[j] accumulation), one per touched row;
 //   3. probe every distinct coordinate in its (column-sorted) row: existing
 //      -> new = old + sum, removed when == 0; missing -> inserted when != 0;
 //   4. plan per touched row: rows whose structure is unchanged get their
 //      values written in place; the others claim a new chunk of the pool's
-//      bump region.  Claims are slot offsets from a device-wide prefix sum
-//      (DPP wave scans), so no two rows contend on an atomic;
-//   5. a one-block check reduces the per-block counters, rejects the whole
-//      batch (no-op) on a bad index or pool exhaustion, and advances the bump
-//      pointer: a batch is either fully applied or not at all;
+//      bump region.  Claims are slot offsets from a prefix sum, so no two
+//      rows contend on an atomic.  The last workgroup of the plan launch
+//      turns every per-tile count into offsets and decides the batch: a bad
+//      index or pool exhaustion rejects it whole (no-op), otherwise the bump
+//      pointer advances;
+//   5. offsets: per-entry exclusive counts (inserts / deletes before each
+//      coordinate, claims / moved entries before each row);
 //   6. move surviving old entries and write inserted ones at their merged
 //      positions, then commit the row table.
+//
+// Cross-tile prefixes come from per-tile counts scanned by the last
+// workgroup to finish (an atomic finish counter); nothing polls.
 #include "dcsr_internal.h"
 #include "prims.h"
 #include "segments.h"
