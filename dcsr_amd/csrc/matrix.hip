@@ -289,10 +289,12 @@ dcsr_status ws_get(dcsr_handle h, WsSlot slot, size_t bytes, void **out)
         size_t nb = bytes + bytes / 4;  // headroom for growing batches
         DCSR_HIP(hipMalloc(&h->ws.ptr[slot], nb));
         h->ws.bytes[slot] = nb;
-        // zero-filled once: single-pass scans and sorts keep their control
-        // words in workspace and rely on finding them zero
-        DCSR_HIP(hipMemset(h->ws.ptr[slot], 0, nb));
-        DCSR_HIP(hipDeviceSynchronize());
+        // slots that hold self-resetting control words (scan tickets, finish
+        // counters) start zero-filled; the others are written before use
+        if (slot == WS_SCAN || slot == WS_PART || slot == WS_TOLDS) {
+            DCSR_HIP(hipMemset(h->ws.ptr[slot], 0, nb));
+            DCSR_HIP(hipDeviceSynchronize());
+        }
     }
     *out = h->ws.ptr[slot];
     return DCSR_OK;

@@ -334,52 +334,60 @@ __global__ __launch_bounds__(TB) void k_spmm_wide(uint32_t R, const uint32_t *__
     }
 }
 
-// Carry chains: consecutive slots (waves / groups) carrying into the same
-// row, added to the row's head value.  A wave scans 64 slots; for each chain
-// that starts among them: a chain that ends within the 64 is summed by KB
-// lanes (one column each) serially in slot order; a longer chain (a hub row
-// spread over many slots) is summed 64 slots at a time across the lanes and
-// then reduced in a fixed tree.  Either way the order depends only on the
-// chain, so results are reproducible.
+// Carry chains: consecutive slots (waves) carrying into the same row, added
+// to the row's head value.  A wave scans 64 slots.  Chains shorter than 64
+// slots are summed by lane groups in parallel (64/KB groups, one lane per
+// column, one chain per group, serially in slot order); a chain of 64 or
+// more slots (a hub row spread over many waves) is summed by the whole wave,
+// 64 slots at a time, then reduced in a fixed tree.  Either way the order
+// depends only on the chain, so results are reproducible.
 template <int KB>
 __global__ __launch_bounds__(TB) void k_carry(uint32_t nunits, const uint32_t *__restrict__ carry_row,
                                               const float *__restrict__ carry_val, float *__restrict__ Y,
                                               int64_t ldy, float alpha)
 {
+    constexpr int G = 64 / KB;
     const int lane = lane_id();
+    const int cl = lane % KB;
+    const int g = lane / KB;
     const uint64_t base = ((uint64_t)blockIdx.x * (TB / 64) + (threadIdx.x >> 6)) * 64;
     if (base >= nunits) return;
     const uint64_t w = base + lane;
     uint32_t r = NONE;
-    bool head = false;
+    bool head = false, lng = false;
     if (w < nunits) {
         r = carry_row[w];
         head = r != NONE && (w == 0 || carry_row[w - 1] != r);
+        // chains are contiguous: 64+ slots iff slot w+63 still carries r
+        lng = head && w + 63 < nunits && carry_row[w + 63] == r;
     }
-    uint64_t heads = __ballot(head);
-    while (heads) {
-        const int hl = __ffsll((long long)heads) - 1;
-        heads &= heads - 1;
-        const uint64_t hw = base + hl;
+    const uint64_t longs = __ballot(lng);
+    const uint64_t shorts = __ballot(head) & ~longs;
+    const int nshort = __popcll(shorts);
+    for (int i0 = 0; i0 < nshort; i0 += G) {
+        const int i = i0 + g;
+        // the i-th short head of this wave
+        uint64_t m = shorts;
+        for (int t = 0; t < i && m; ++t) m &= m - 1;
+        const int hl = m ? __ffsll((long long)m) - 1 : 0;
         const uint32_t hr = (uint32_t)__shfl((int)r, hl);
-        // chain length within the first 64 slots from the head
-        const uint64_t k0 = hw + lane;
-        const bool in0 = k0 < nunits && carry_row[k0] == hr;
-        const uint64_t m0 = __ballot(in0);
-        const int len0 = __popcll(m0);
-        if (len0 < 64) {
-            if (lane < KB) {
-                float acc = 0.0f;
-                for (int i = 0; i < len0; ++i) acc = acc + carry_val[(hw + i) * KB + lane];
-                const uint64_t yi = (uint64_t)hr * ldy + lane;
-                Y[yi] = Y[yi] + alpha * acc;
-            }
-            continue;
+        if (i < nshort) {
+            const uint64_t hw = base + hl;
+            float acc = 0.0f;
+            for (uint64_t k = hw; k < nunits && carry_row[k] == hr; ++k) acc = acc + carry_val[k * KB + cl];
+            const uint64_t yi = (uint64_t)hr * ldy + cl;
+            Y[yi] = Y[yi] + alpha * acc;
         }
+    }
+    uint64_t lm = longs;
+    while (lm) {
+        const int hl = __ffsll((long long)lm) - 1;
+        lm &= lm - 1;
+        const uint32_t hr = (uint32_t)__shfl((int)r, hl);
         float sacc[KB];
 #pragma unroll
         for (int q = 0; q < KB; ++q) sacc[q] = 0.0f;
-        for (uint64_t k = k0;; k += 64) {
+        for (uint64_t k = base + hl + lane;; k += 64) {
             const bool in = k < nunits && carry_row[k] == hr;
             if (in) {
 #pragma unroll
@@ -441,6 +449,16 @@ __global__ __launch_bounds__(TB) void k_csr_rows(uint32_t R, const int32_t *__re
 
 }  // namespace
 
+// Carry slots for any nnz up to the pool capacity (one per wave of
+// DCSR_SPMV_PW entries, rounded to whole blocks), so the buffers are not
+// reallocated while the matrix grows within its pool.
+static size_t carry_slots(dcsr_handle h)
+{
+    const int64_t cap = h->pool_cap > h->nnz_h ? h->pool_cap : h->nnz_h;
+    const int64_t waves = (cap + DCSR_SPMV_PW - 1) / DCSR_SPMV_PW + 4;
+    return (size_t)((waves + 3) / 4 * 4);
+}
+
 // Flat offsets of the live row lengths (natural row order), rebuilt lazily
 // after a structural change.  This is the engine's "binning" step.
 dcsr_status ensure_flat(dcsr_handle h, hipStream_t s)
@@ -455,7 +473,10 @@ dcsr_status ensure_flat(dcsr_handle h, hipStream_t s)
     if (!h->state_known) DCSR_TRY(pull_state(h, s));
     const int64_t ng = h->nnz_h / GRAN + 1;
     uint32_t *rowmap;
-    DCSR_TRY(ws_get(h, WS_ROWMAP, (size_t)ng * 4, (void **)&rowmap));
+    // sized by the pool (an upper bound on nnz): no reallocation while the
+    // matrix grows within its pool
+    const int64_t ngcap = (h->pool_cap > h->nnz_h ? h->pool_cap : h->nnz_h) / GRAN + 1;
+    DCSR_TRY(ws_get(h, WS_ROWMAP, (size_t)ngcap * 4, (void **)&rowmap));
     k_rowmap<<<grid_for(ng, TB), TB, 0, s>>>((uint32_t)R, foff, rowmap, (uint32_t)ng);
     DCSR_HIP(hipGetLastError());
     h->flat_dirty = false;
@@ -474,8 +495,9 @@ static dcsr_status launch_flat(dcsr_handle h, const float *X, int64_t ldx, float
     nwaves = (int64_t)blocks * 4;
     uint32_t *crow;
     float *cval;
-    DCSR_TRY(ws_get(h, WS_CARRY_ROW, (size_t)nwaves * 4, (void **)&crow));
-    DCSR_TRY(ws_get(h, WS_CARRY_VAL, (size_t)nwaves * 4 * KB, (void **)&cval));
+    const size_t cap_slots = carry_slots(h);
+    DCSR_TRY(ws_get(h, WS_CARRY_ROW, cap_slots * 4, (void **)&crow));
+    DCSR_TRY(ws_get(h, WS_CARRY_VAL, cap_slots * 4 * KB, (void **)&cval));
     const uint32_t *foff = (const uint32_t *)h->ws.ptr[WS_FOFF];
     const uint32_t *rowmap = (const uint32_t *)h->ws.ptr[WS_ROWMAP];
     k_flat<KB><<<blocks, TB, 0, s>>>(R, foff, rowmap, h->row_start, h->pool_col, h->pool_val, X, ldx, Y, ldy, alpha,
@@ -496,8 +518,9 @@ static dcsr_status launch_wide(dcsr_handle h, const float *X, int64_t ldx, float
     nwaves = (int64_t)blocks * 4;
     uint32_t *crow;
     float *cval;
-    DCSR_TRY(ws_get(h, WS_SPMM_CROW, (size_t)nwaves * 4, (void **)&crow));
-    DCSR_TRY(ws_get(h, WS_SPMM_CVAL, (size_t)nwaves * 4 * KB, (void **)&cval));
+    const size_t cap_slots = carry_slots(h);
+    DCSR_TRY(ws_get(h, WS_SPMM_CROW, cap_slots * 4, (void **)&crow));
+    DCSR_TRY(ws_get(h, WS_SPMM_CVAL, cap_slots * 4 * KB, (void **)&cval));
     const uint32_t *foff = (const uint32_t *)h->ws.ptr[WS_FOFF];
     const uint32_t *rowmap = (const uint32_t *)h->ws.ptr[WS_ROWMAP];
     k_spmm_wide<KB><<<blocks, TB, 0, s>>>(R, foff, rowmap, h->row_start, h->pool_col, h->pool_val, X, ldx, Y, ldy,
