@@ -55,6 +55,7 @@ enum WsSlot {
     WS_SG_P, WS_SG_POFF, WS_SG_CNT, WS_SG_COFF, WS_SG_LIST, WS_SG_STATE, WS_SG_BSUM, WS_SG_TCOL, WS_SG_TVAL,
     WS_SG_HPRE, WS_SG_EA, WS_SG_APOS, WS_SG_KEY0, WS_SG_KEY1, WS_SG_VAL0, WS_SG_VAL1, WS_SG_FLAG, WS_SG_EPOS,
     WS_CANON, WS_TMP0, WS_TMP1, WS_TMP2,
+    WS_PNL_OFF, WS_PNL_BND, WS_PNL_COL, WS_PNL_VAL, WS_PNL_MAP, WS_PNL_PART,
     WS_COUNT
 };
 
@@ -86,6 +87,13 @@ struct dcsr_matrix_s {
     bool hubs_dirty;
     bool flat_dirty;         // flat row offsets (SpMV binning) need a rebuild
     bool state_known;        // host mirrors valid (no NOSYNC update since last sync)
+    // column-paneled copy for SpMV on matrices whose x exceeds the L2 (see
+    // spmv.hip): valid until the next structural change
+    bool pnl_valid;
+    int pnl_hits;            // SpMV calls since the last structural change
+    int pnl_count;           // panels in the current copy
+    int64_t pnl_cols;        // panel width (columns)
+    int64_t pnl_min_nnz;     // smallest nnz worth a paneled copy
     dcsr_config cfg;
     Workspace ws;
     hipStream_t last_stream;

@@ -375,6 +375,13 @@ dcsr_status dcsr_create(dcsr_handle *out, int64_t rows, int64_t cols, const dcsr
     h->hubs_dirty = false;
     h->flat_dirty = true;
     h->state_known = true;
+    // SpMV column panels: 2M columns (8 MB of x; measured best on a 10M x 10M
+    // power-law matrix between 0.5M and 4M) for matrices of at least 16M
+    // entries; both can be overridden for testing
+    h->pnl_cols = 2 << 20;
+    h->pnl_min_nnz = 16ll << 20;
+    if (const char *e = getenv("DCSR_SPMV_PANEL_COLS")) h->pnl_cols = atoll(e) > 0 ? atoll(e) : h->pnl_cols;
+    if (const char *e = getenv("DCSR_SPMV_PANEL_MIN_NNZ")) h->pnl_min_nnz = atoll(e) >= 0 ? atoll(e) : h->pnl_min_nnz;
     dcsr_status st = alloc_rows(h);
     if (st == DCSR_OK) st = alloc_pool(h, h->cfg.min_pool);
     if (st == DCSR_OK && hipMalloc(&h->ds, sizeof(DevState)) != hipSuccess) st = DCSR_ERR_OOM;
