@@ -8,8 +8,10 @@
 // indices), so they share one pool and one allocator.  One persistent kernel
 // does a whole step: every workgroup takes matrix ids from a work counter and,
 // for each, applies that matrix's insert-or-add batch entirely in LDS and
-// per-workgroup scratch (bitonic sort of (row, col, arrival) keys, left-to-
-// right duplicate sums, probe, prefix-sum chunk claims, merge, commit), then
+// per-workgroup scratch (stable LDS radix sort of (row, col) keys, left-to-
+// right duplicate sums, a per-row merge against the stored row for the
+// codes, prefix-sum chunk claims, a per-row merge into the new chunk,
+// commit), then
 // computes that matrix's SpMV with TEAM lanes per row and a whole wave for
 // rows longer than a threshold.  Semantics per matrix are those of
 // dcsr_update followed by dcsr_spmv.  The last workgroup to finish resets the
@@ -115,6 +117,94 @@ __device__ __forceinline__ uint32_t block_sum(uint32_t v, uint32_t *lw)
     return t;
 }
 
+// Block-wide exclusive scan of one value per thread (256 threads); *total
+// receives the sum.
+__device__ __forceinline__ uint32_t block_excl(uint32_t v, uint32_t *lw, uint32_t *total)
+{
+    const uint32_t inc = wave_incl_scan_u32(v);
+    if (lane_id() == 63) lw[threadIdx.x >> 6] = inc;
+    __syncthreads();
+    uint32_t carry = 0, tot = 0;
+#pragma unroll
+    for (int i = 0; i < BT / 64; ++i) {
+        if (i < (int)(threadIdx.x >> 6)) carry += lw[i];
+        tot += lw[i];
+    }
+    __syncthreads();
+    *total = tot;
+    return carry + inc - v;
+}
+
+// One stable pass (8-bit digit at `shift`) of an LDS radix sort of n <= BMAX
+// (key, value) pairs from (ks, vs) to (kd, vd).  Wave w ranks its contiguous
+// quarter in order (ballot digit matching, running per-digit counts in
+// wcnt[w]); digit bases come from a block scan of the digit totals.
+__device__ void lds_radix_pass(const uint32_t *ks, const float *vs, uint32_t *kd, float *vd, uint32_t n, int shift,
+                               uint32_t (*wcnt)[256], uint32_t *lw)
+{
+    constexpr int MAXJ = BMAX / (BT / 64) / 64;  // 64-item chunks per wave
+    const int w = threadIdx.x >> 6;
+    const int lane = lane_id();
+    const uint32_t Q = (((n + 3) / 4) + 63) & ~63u;
+    const uint32_t wb = (uint32_t)w * Q;
+    const uint32_t we = wb + Q < n ? wb + Q : n;
+    for (int i = threadIdx.x; i < (BT / 64) * 256; i += BT) (&wcnt[0][0])[i] = 0;
+    __syncthreads();
+    uint32_t key[MAXJ], rnk[MAXJ];
+    float val[MAXJ];
+    const uint64_t lt = lanemask_lt();
+#pragma unroll
+    for (int j = 0; j < MAXJ; ++j) {
+        const uint32_t idx = wb + (uint32_t)j * 64 + lane;
+        const bool ok = idx < we;
+        key[j] = ok ? ks[idx] : 0u;
+        val[j] = ok ? vs[idx] : 0.0f;
+        rnk[j] = 0;
+        if (wb + (uint32_t)j * 64 < we) {  // wave-uniform
+            const uint32_t d = (key[j] >> shift) & 255u;
+            const uint64_t act = __ballot(ok);
+            const uint64_t peers = wave_match_digit8(d, act);
+            const int leader = __ffsll((long long)peers) - 1;
+            uint32_t b = 0;
+            if (ok && lane == leader) {
+                b = wcnt[w][d];
+                wcnt[w][d] = b + (uint32_t)__popcll(peers);
+            }
+            b = __shfl(b, leader < 0 ? 0 : leader);
+            rnk[j] = b + (uint32_t)__popcll(peers & lt);
+        }
+    }
+    __syncthreads();
+    {
+        const int d = threadIdx.x;
+        uint32_t c[BT / 64];
+        uint32_t tot = 0;
+#pragma unroll
+        for (int i = 0; i < BT / 64; ++i) {
+            c[i] = wcnt[i][d];
+            tot += c[i];
+        }
+        uint32_t all;
+        uint32_t b = block_excl(tot, lw, &all);
+#pragma unroll
+        for (int i = 0; i < BT / 64; ++i) {
+            wcnt[i][d] = b;
+            b += c[i];
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < MAXJ; ++j) {
+        const uint32_t idx = wb + (uint32_t)j * 64 + lane;
+        if (idx < we) {
+            const uint32_t pos = wcnt[w][(key[j] >> shift) & 255u] + rnk[j];
+            kd[pos] = key[j];
+            vd[pos] = val[j];
+        }
+    }
+    __syncthreads();
+}
+
 // Update of matrix m inside one workgroup.  Returns nothing; status bits go
 // to mstat[m].
 __device__ void batch_update(uint32_t m, const uint32_t *__restrict__ row_base, const uint32_t *__restrict__ col_base,
@@ -123,7 +213,8 @@ __device__ void batch_update(uint32_t m, const uint32_t *__restrict__ row_base, 
                              uint32_t *__restrict__ row_start, int32_t *__restrict__ row_len,
                              int32_t *__restrict__ row_cap, int32_t *__restrict__ pcol, float *__restrict__ pval,
                              unsigned long long pool_cap, DevState *ds, uint32_t *__restrict__ mstat, uint32_t *sc,
-                             uint64_t *sk, float *sv, uint32_t *lw, uint32_t *sm)
+                             uint32_t (*skb)[BMAX], float (*svb)[BMAX], uint32_t (*wcnt)[256], uint32_t *lw,
+                             uint32_t *sm)
 {
     const uint32_t t = threadIdx.x;
     const uint32_t gbase = row_base[m];
@@ -142,50 +233,37 @@ __device__ void batch_update(uint32_t m, const uint32_t *__restrict__ row_base, 
         return;
     }
     const uint32_t nb = (uint32_t)nb64;
-    // U1: keys (row, col, arrival) and values into LDS; bitonic sort
-    const uint32_t n2 = next_pow2(nb < 64 ? 64 : nb);
+    // U1: (row, col) keys and values into LDS, stable radix sort (3 passes
+    // of 8 bits over the 24 key bits; stability keeps arrival order among
+    // duplicates)
     if (t == 0) sm[0] = 0;
     __syncthreads();
     uint32_t bad = 0;
-    for (uint32_t i = t; i < n2; i += BT) {
-        uint64_t k = ~0ull;
-        if (i < nb) {
-            const int32_t r = trow[t0 + i], c = tcol[t0 + i];
-            if (r < 0 || (uint32_t)r >= Rm || c < 0 || (uint32_t)c >= Cm) bad = 1;
-            else k = ((uint64_t)r << (2 * KB)) | ((uint64_t)c << KB) | i;
-            sv[i] = tval[t0 + i];
-        }
-        sk[i] = k;
+    for (uint32_t i = t; i < nb; i += BT) {
+        const int32_t r = trow[t0 + i], c = tcol[t0 + i];
+        uint32_t k = 0;
+        if (r < 0 || (uint32_t)r >= Rm || c < 0 || (uint32_t)c >= Cm) bad = 1;
+        else k = ((uint32_t)r << KB) | (uint32_t)c;
+        skb[0][i] = k;
+        svb[0][i] = tval[t0 + i];
     }
     if (block_sum(bad, lw)) {
         if (t == 0) mstat[m] |= BST_BAD_INDEX;
         return;
     }
-    for (uint32_t k = 2; k <= n2; k <<= 1) {
-        for (uint32_t js = k >> 1; js > 0; js >>= 1) {
-            for (uint32_t c = t; c < n2 / 2; c += BT) {
-                const uint32_t i1 = (c / js) * 2 * js + (c % js), i2 = i1 + js;
-                const bool asc = (i1 & k) == 0;
-                const uint64_t ka = sk[i1], kb = sk[i2];
-                if ((ka > kb) == asc) {
-                    sk[i1] = kb;
-                    sk[i2] = ka;
-                    const float va = sv[i1];
-                    sv[i1] = sv[i2];
-                    sv[i2] = va;
-                }
-            }
-            __syncthreads();
-        }
-    }
+    lds_radix_pass(skb[0], svb[0], skb[1], svb[1], nb, 0, wcnt, lw);
+    lds_radix_pass(skb[1], svb[1], skb[0], svb[0], nb, 8, wcnt, lw);
+    lds_radix_pass(skb[0], svb[0], skb[1], svb[1], nb, 16, wcnt, lw);
+    const uint32_t *sk = skb[1];
+    const float *sv = svb[1];
     // U2: distinct coordinates (heads) and touched rows (row heads)
     uint32_t *hf = sc + S_HF, *rf = sc + S_RF;
     for (uint32_t p = t; p <= nb; p += BT) {
         uint32_t h = 0, rh = 0;
         if (p < nb) {
-            const uint64_t k = sk[p] >> KB;
-            h = (p == 0 || (sk[p - 1] >> KB) != k) ? 1u : 0u;
-            rh = (p == 0 || (sk[p - 1] >> (2 * KB)) != (sk[p] >> (2 * KB))) ? 1u : 0u;
+            const uint32_t k = sk[p];
+            h = (p == 0 || sk[p - 1] != k) ? 1u : 0u;
+            rh = (p == 0 || (sk[p - 1] >> KB) != (k >> KB)) ? 1u : 0u;
         }
         hf[p] = h;
         rf[p] = rh;
@@ -197,19 +275,19 @@ __device__ void batch_update(uint32_t m, const uint32_t *__restrict__ row_base, 
     float *uval = (float *)(sc + S_UVAL);
     uint32_t *trw = sc + S_TROW, *tub = sc + S_TUB, *tins = sc + S_TINS, *tdel = sc + S_TDEL;
     for (uint32_t p = t; p < nb; p += BT) {
-        const uint64_t k = sk[p] >> KB;
-        const bool h = p == 0 || (sk[p - 1] >> KB) != k;
+        const uint32_t k = sk[p];
+        const bool h = p == 0 || sk[p - 1] != k;
         if (!h) continue;
-        const bool rh = p == 0 || (sk[p - 1] >> (2 * KB)) != (sk[p] >> (2 * KB));
+        const bool rh = p == 0 || (sk[p - 1] >> KB) != (k >> KB);
         const uint32_t u = hf[p];
         const uint32_t tt = rf[p] + (rh ? 0u : 0xffffffffu);
         float s = 0.0f + sv[p];
-        for (uint32_t q = p + 1; q < nb && (sk[q] >> KB) == k; ++q) s = s + sv[q];
-        urc[u] = (uint32_t)k;
+        for (uint32_t q = p + 1; q < nb && sk[q] == k; ++q) s = s + sv[q];
+        urc[u] = k;
         uval[u] = s;
         ut[u] = tt;
         if (rh) {
-            trw[tt] = (uint32_t)(k >> KB);
+            trw[tt] = k >> KB;
             tub[tt] = u;
             tins[tt] = 0;
             tdel[tt] = 0;
@@ -217,50 +295,52 @@ __device__ void batch_update(uint32_t m, const uint32_t *__restrict__ row_base, 
     }
     if (t == 0) tub[nt] = nu;
     __syncthreads();
-    // U3: probe
-    uint32_t *uip = sc + S_UIP, *udp = sc + S_UDP;
-    uint32_t my_adds = 0;
-    for (uint32_t u = t; u <= nu; u += BT) {
-        uint32_t code = C_NOP;
-        if (u < nu) {
-            const uint32_t r = urc[u] >> KB;
+    // U3: one lane per touched row merges the row's update run (sorted by
+    // column) against its old entries (sorted by column): code and new value
+    // per distinct coordinate, position of a hit, inserts / deletes per row.
+    uint32_t my_adds = 0, my_ins = 0, my_del = 0;
+    for (uint32_t tt = t; tt < nt; tt += BT) {
+        const uint32_t g = gbase + trw[tt];
+        const uint64_t st = row_start[g];
+        const int32_t L = row_len[g];
+        const uint32_t ub = tub[tt], ue = tub[tt + 1];
+        int32_t i = 0;
+        uint32_t ni = 0, nd = 0;
+        for (uint32_t u = ub; u < ue; ++u) {
             const int32_t c = (int32_t)(urc[u] & (LMAX - 1));
-            const uint64_t st = row_start[gbase + r];
-            const int32_t L = row_len[gbase + r];
-            int32_t lo = 0, hi = L;
-            while (lo < hi) {
-                const int32_t mid = (lo + hi) >> 1;
-                if (pcol[st + mid] < c) lo = mid + 1; else hi = mid;
-            }
-            const bool hit = lo < L && pcol[st + lo] == c;
+            while (i < L && pcol[st + i] < c) ++i;
+            const bool hit = i < L && pcol[st + i] == c;
             const float s = uval[u];
+            uint32_t code;
             float nv;
             if (hit) {
-                nv = pval[st + lo] + s;
+                nv = pval[st + i] + s;
                 code = nv != 0.0f ? C_ADD : C_DEL;
             } else {
                 nv = s;
                 code = s != 0.0f ? C_INS : C_NOP;
             }
             uval[u] = nv;
-            upos[u] = (uint32_t)lo;
+            upos[u] = (uint32_t)i;
             ucode[u] = code;
-            if (code == C_INS) atomicAdd(&tins[ut[u]], 1u);
-            if (code == C_DEL) atomicAdd(&tdel[ut[u]], 1u);
-            my_adds += code == C_ADD ? 1u : 0u;
+            ni += code == C_INS;
+            nd += code == C_DEL;
+            my_adds += code == C_ADD;
         }
-        uip[u] = code == C_INS ? 1u : 0u;
-        udp[u] = code == C_DEL ? 1u : 0u;
+        tins[tt] = ni;
+        tdel[tt] = nd;
+        my_ins += ni;
+        my_del += nd;
     }
     __syncthreads();
     const uint32_t nadd = block_sum(my_adds, lw);
-    const uint32_t nins = block_scan_inplace(uip, nu + 1, lw);
-    const uint32_t ndel = block_scan_inplace(udp, nu + 1, lw);
+    const uint32_t nins = block_sum(my_ins, lw);
+    const uint32_t ndel = block_sum(my_del, lw);
     // U4: plan, claim
-    uint32_t *tdst = sc + S_TDST, *tmp = sc + S_TMP, *tmode = sc + S_TMODE, *tnl = sc + S_TNL;
+    uint32_t *tdst = sc + S_TDST, *tmode = sc + S_TMODE, *tnl = sc + S_TNL;
     long long dnnz = 0, dgar = 0;
     for (uint32_t tt = t; tt <= nt; tt += BT) {
-        uint32_t claim = 0, moved = 0;
+        uint32_t claim = 0;
         if (tt < nt) {
             const uint32_t g = gbase + trw[tt];
             const uint32_t L = (uint32_t)row_len[g];
@@ -269,7 +349,6 @@ __device__ void batch_update(uint32_t m, const uint32_t *__restrict__ row_base, 
             const uint32_t mode = (ni | nd) ? 1u : 0u;
             if (mode) {
                 claim = chunk_round(nl);
-                moved = L;
                 dnnz += (long long)nl - (long long)L;
                 dgar += row_cap[g];
             }
@@ -277,11 +356,9 @@ __device__ void batch_update(uint32_t m, const uint32_t *__restrict__ row_base, 
             tnl[tt] = nl;
         }
         tdst[tt] = claim;
-        tmp[tt] = moved;
     }
     __syncthreads();
     const uint32_t claim_tot = block_scan_inplace(tdst, nt + 1, lw);
-    const uint32_t moved_tot = block_scan_inplace(tmp, nt + 1, lw);
     // 64-bit deltas: per-thread values are small; reduce through 32-bit halves
     const uint32_t dn_lo = block_sum((uint32_t)(dnnz & 0xffffffff), lw);
     const uint32_t dg_lo = block_sum((uint32_t)(dgar & 0xffffffff), lw);
@@ -303,43 +380,45 @@ __device__ void batch_update(uint32_t m, const uint32_t *__restrict__ row_base, 
     __syncthreads();
     if (!sm[1]) return;
     const uint32_t base = sm[2];
-    // U5: move surviving old entries, write inserts and in-place values
-    for (uint32_t e = t; e < moved_tot; e += BT) {
-        const uint32_t tt = find_last_le(tmp, nt, e);
-        const uint32_t k = e - tmp[tt];
+    // U5: one lane per touched row: rewritten rows are merged into their new
+    // chunk (old entries and the row's update run, both column-sorted);
+    // unchanged-structure rows get their added values written in place
+    for (uint32_t tt = t; tt < nt; tt += BT) {
         const uint32_t g = gbase + trw[tt];
-        const uint64_t sp = (uint64_t)row_start[g] + k;
-        const int32_t c = pcol[sp];
-        float v = pval[sp];
+        const uint64_t st = row_start[g];
         const uint32_t ub = tub[tt], ue = tub[tt + 1];
-        uint32_t lo = ub, hi = ue;
-        while (lo < hi) {
-            const uint32_t mid = (lo + hi) >> 1;
-            if ((int32_t)(urc[mid] & (LMAX - 1)) < c) lo = mid + 1; else hi = mid;
-        }
-        if (lo < ue && (int32_t)(urc[lo] & (LMAX - 1)) == c) {
-            if (ucode[lo] == C_DEL) continue;
-            v = uval[lo];
-        }
-        const int64_t shift = (int64_t)(uip[lo] - uip[ub]) - (int64_t)(udp[lo] - udp[ub]);
-        const uint64_t d = (uint64_t)base + tdst[tt] + (uint64_t)((int64_t)k + shift);
-        pcol[d] = c;
-        pval[d] = v;
-    }
-    for (uint32_t u = t; u < nu; u += BT) {
-        const uint32_t tt = ut[u];
-        const uint32_t code = ucode[u];
-        const uint32_t g = gbase + trw[tt];
         if (!tmode[tt]) {
-            if (code == C_ADD) pval[(uint64_t)row_start[g] + upos[u]] = uval[u];
+            for (uint32_t u = ub; u < ue; ++u)
+                if (ucode[u] == C_ADD) pval[st + upos[u]] = uval[u];
             continue;
         }
-        if (code == C_INS) {
-            const uint32_t ub = tub[tt];
-            const int64_t shift = (int64_t)(uip[u] - uip[ub]) - (int64_t)(udp[u] - udp[ub]);
-            const uint64_t d = (uint64_t)base + tdst[tt] + (uint64_t)((int64_t)upos[u] + shift);
-            pcol[d] = (int32_t)(urc[u] & (LMAX - 1));
-            pval[d] = uval[u];
+        const int32_t L = row_len[g];
+        const uint64_t d0 = (uint64_t)base + tdst[tt];
+        uint64_t o = d0;
+        int32_t i = 0;
+        for (uint32_t u = ub; u < ue; ++u) {
+            const uint32_t k = upos[u];  // old entries before this coordinate
+            for (; i < (int32_t)k; ++i, ++o) {
+                pcol[o] = pcol[st + i];
+                pval[o] = pval[st + i];
+            }
+            const uint32_t code = ucode[u];
+            if (code == C_INS) {
+                pcol[o] = (int32_t)(urc[u] & (LMAX - 1));
+                pval[o] = uval[u];
+                ++o;
+            } else if (code == C_ADD) {
+                pcol[o] = pcol[st + i];
+                pval[o] = uval[u];
+                ++o;
+                ++i;
+            } else if (code == C_DEL) {
+                ++i;
+            }
+        }
+        for (; i < L; ++i, ++o) {
+            pcol[o] = pcol[st + i];
+            pval[o] = pval[st + i];
         }
     }
     __syncthreads();
@@ -406,8 +485,9 @@ __global__ __launch_bounds__(BT) void k_batch_step(uint32_t M, const uint32_t *_
                                                    const float *__restrict__ x, float *__restrict__ y, uint32_t thresh,
                                                    BatchState *bs, int do_update, int do_spmv)
 {
-    __shared__ uint64_t sk[BMAX];
-    __shared__ float sv[BMAX];
+    __shared__ uint32_t sk[2][BMAX];
+    __shared__ float sv[2][BMAX];
+    __shared__ uint32_t wcnt[BT / 64][256];
     __shared__ uint32_t lw[BT / 64];
     __shared__ uint32_t sm[4];
     uint32_t *sc = scratch + (uint64_t)blockIdx.x * S_WORDS;
@@ -419,7 +499,7 @@ __global__ __launch_bounds__(BT) void k_batch_step(uint32_t M, const uint32_t *_
         if (m >= M) break;
         if (do_update)
             batch_update(m, row_base, col_base, tri_off, ntri, trow, tcol, tval, row_start, row_len, row_cap, pcol, pval,
-                         pool_cap, ds, mstat, sc, sk, sv, lw, sm);
+                         pool_cap, ds, mstat, sc, sk, sv, wcnt, lw, sm);
         if (do_spmv) batch_spmv<TEAM>(m, row_base, col_base, row_start, row_len, pcol, pval, x, y, thresh);
         __syncthreads();
     }
