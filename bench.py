@@ -489,7 +489,7 @@ def bench_serving(args):
         missing coordinate with sum 0.0 is not inserted, an existing one
         keeps old + 0.0) but grows every update buffer, and one packed
         128-column product sizes the query buffers."""
-        m = dcsr_amd.DcsrMatrix(R, R, pool_factor=3.0)
+        m = dcsr_amd.DcsrMatrix(R, R, pool_factor=8.0)
         m.LoadMatrix(d_r, d_c, d_v)
         m.UpdateMatrix(zr, zc, zv)
         for k in (1, 4, 16, 32):
@@ -573,6 +573,8 @@ def bench_serving(args):
     run_server(1e9, warmup=True)  # first-touch workspaces, server threads
     svc = service_times()
     cap_pc = 1.0 / svc.mean()  # per-call capacity (requests/s)
+    kinds = ["update" if q[0] == "update" else f"k{q[1]}" for q in reqs]
+    service_ms = {k: float(np.mean([t for t, kk in zip(svc, kinds) if kk == k]) * 1e3) for k in sorted(set(kinds))}
     # per-call path at 90% of its capacity: its p99 is the latency bar
     rate_pc = 0.9 * cap_pc
     thr_pc, lat_pc = run_per_call(rate_pc)
@@ -603,6 +605,7 @@ def bench_serving(args):
                    "query_p99_ms": float(np.percentile(lat_pc[is_q], 99)),
                    "update_p99_ms": float(np.percentile(lat_pc[~is_q], 99)), "capacity": cap_pc},
          throughput_vs_per_call_at_equal_p99=ratio,
+         per_call_service_ms=service_ms,
          sweep=sweep,
          setup_s=t_gen,
          roofline=None,

@@ -334,6 +334,195 @@ __global__ __launch_bounds__(TB) void k_spmm_wide(uint32_t R, const uint32_t *__
     }
 }
 
+// Y[0..3] = alpha * a + beta * Y[0..3] (beta == 0: Y is not read); Y is
+// 16-byte aligned.
+__device__ __forceinline__ void store4(float *Y, float4 a, float alpha, float beta)
+{
+    float4 o;
+    if (beta == 0.0f) {
+        o = make_float4(alpha * a.x, alpha * a.y, alpha * a.z, alpha * a.w);
+    } else {
+        const float4 y = *reinterpret_cast<const float4 *>(Y);
+        o = make_float4(alpha * a.x + beta * y.x, alpha * a.y + beta * y.y, alpha * a.z + beta * y.z,
+                        alpha * a.w + beta * y.w);
+    }
+    *reinterpret_cast<float4 *>(Y) = o;
+}
+
+// The same kernel with float4 column access (KB/4 lanes per group): used when
+// X and Y rows are 16-byte aligned, it issues a quarter of the loads.
+template <int KB>
+__global__ __launch_bounds__(TB) void k_spmm_wide4(uint32_t R, const uint32_t *__restrict__ foff,
+                                                  const uint32_t *__restrict__ rowmap,
+                                                  const uint32_t *__restrict__ row_start,
+                                                  const int32_t *__restrict__ col, const float *__restrict__ val,
+                                                  const float *__restrict__ X, int64_t ldx, float *__restrict__ Y,
+                                                  int64_t ldy, float alpha, float beta,
+                                                  uint32_t *__restrict__ carry_row, float *__restrict__ carry_val)
+{
+    constexpr int LPG = KB / 4;  // lanes per group, 4 columns (one float4) each
+    constexpr int G = 64 / LPG;
+    constexpr uint32_t PW = DCSR_SPMV_PW;
+    __shared__ uint32_t s_off[TB / 64][65];
+    __shared__ uint32_t s_st[TB / 64][64];
+    __shared__ int32_t s_col[TB / 64][PW];
+    __shared__ float s_val[TB / 64][PW];
+    __shared__ uint8_t s_row[TB / 64][PW];
+    __shared__ float4 s_part[TB / 64][2][G][LPG];
+    __shared__ int32_t s_prow[TB / 64][2][G];
+    const int w = threadIdx.x >> 6;
+    const int lane = lane_id();
+    const int cl = lane % LPG;
+    const int g = lane / LPG;
+    const uint32_t wave = blockIdx.x * (TB / 64) + w;
+    const uint32_t total = foff[R];
+    const uint32_t lastw = total ? (total - 1) / PW : 0;
+    if (lane == 0) carry_row[wave] = NONE;
+    if (wave > lastw) return;
+    const uint32_t e0 = wave * PW;
+    const uint32_t e1 = (total - e0) < PW ? total : e0 + PW;
+    const bool last = wave == lastw;
+
+    uint32_t j = rowmap[e0 / GRAN];
+    while (j < R) {
+        const uint32_t r = j + lane;
+        s_off[w][lane] = r <= R ? foff[r] : total;
+        if (lane == 0) s_off[w][64] = (j + 64 <= R) ? foff[j + 64] : total;
+        s_st[w][lane] = r < R ? row_start[r] : 0u;
+        __builtin_amdgcn_wave_barrier();
+        const uint32_t wb = s_off[w][0] > e0 ? s_off[w][0] : e0;
+        const uint32_t wend = s_off[w][64] < e1 ? s_off[w][64] : e1;
+        const uint32_t n = wend > wb ? wend - wb : 0u;
+        // stage the window's entries
+        for (uint32_t b = 0; b < n; b += 64) {
+            const uint32_t i = b + lane;
+            if (i < n) {
+                const uint32_t e = wb + i;
+                const uint32_t jj = (uint32_t)find_seg64(s_off[w], e);
+                const uint64_t pos = (uint64_t)s_st[w][jj] + (e - s_off[w][jj]);
+                s_col[w][i] = col[pos];
+                s_val[w][i] = val[pos];
+                s_row[w][i] = (uint8_t)jj;
+            }
+        }
+        __builtin_amdgcn_wave_barrier();
+        // this group's slice of the staged entries
+        const uint32_t L = (n + G - 1) / G;
+        const uint32_t c0 = (uint32_t)g * L < n ? (uint32_t)g * L : n;
+        const uint32_t c1 = c0 + L < n ? c0 + L : n;
+        int cur = -1, nseg = 0;
+        float4 acc = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        for (uint32_t i = c0; i < c1; i += WIDE_UN) {
+            int32_t cc[WIDE_UN];
+            float vv[WIDE_UN];
+            float4 xv[WIDE_UN];
+            int rr[WIDE_UN];
+#pragma unroll
+            for (int u = 0; u < WIDE_UN; ++u) {
+                const uint32_t idx = i + u;
+                const bool ok = idx < c1;
+                cc[u] = ok ? s_col[w][idx] : 0;
+                vv[u] = ok ? s_val[w][idx] : 0.0f;
+                rr[u] = ok ? (int)s_row[w][idx] : -1;
+            }
+#pragma unroll
+            for (int u = 0; u < WIDE_UN; ++u)
+                xv[u] = rr[u] >= 0 ? *reinterpret_cast<const float4 *>(X + (int64_t)cc[u] * ldx + 4 * cl)
+                                   : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+#pragma unroll
+            for (int u = 0; u < WIDE_UN; ++u) {
+                if (rr[u] < 0) continue;
+                if (rr[u] != cur) {
+                    if (cur >= 0) {
+                        if (nseg == 0) {
+                            // first segment of the slice: may continue the previous slice
+                            s_part[w][0][g][cl] = acc;
+                            if (cl == 0) s_prow[w][0][g] = cur;
+                        } else {
+                            // interior row: starts and ends in this slice, starts in the wave
+                            store4(Y + (uint64_t)(j + cur) * ldy + 4 * cl, acc, alpha, beta);
+                        }
+                        ++nseg;
+                    }
+                    cur = rr[u];
+                    acc = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+                }
+                acc.x = acc.x + vv[u] * xv[u].x;
+                acc.y = acc.y + vv[u] * xv[u].y;
+                acc.z = acc.z + vv[u] * xv[u].z;
+                acc.w = acc.w + vv[u] * xv[u].w;
+            }
+        }
+        // the slice's last segment (or its only one)
+        if (nseg == 0) {
+            s_part[w][0][g][cl] = acc;
+            if (cl == 0) {
+                s_prow[w][0][g] = cur;  // -1 for an empty slice
+                s_prow[w][1][g] = -1;
+            }
+        } else {
+            s_part[w][1][g][cl] = acc;
+            if (cl == 0) s_prow[w][1][g] = cur;
+        }
+        __builtin_amdgcn_wave_barrier();
+        // combine slice boundary partials in slice order (group 0)
+        if (g == 0) {
+            int crow = -1;
+            float4 cacc = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            for (int gg = 0; gg < G; ++gg) {
+#pragma unroll
+                for (int part = 0; part < 2; ++part) {
+                    const int row = s_prow[w][part][gg];
+                    if (row < 0) continue;
+                    const float4 a = s_part[w][part][gg][cl];
+                    if (row == crow) {
+                        cacc.x = cacc.x + a.x;
+                        cacc.y = cacc.y + a.y;
+                        cacc.z = cacc.z + a.z;
+                        cacc.w = cacc.w + a.w;
+                        continue;
+                    }
+                    if (crow >= 0) {
+                        const uint32_t rs = s_off[w][crow];
+                        if (rs >= e0) {
+                            store4(Y + (uint64_t)(j + crow) * ldy + 4 * cl, cacc, alpha, beta);
+                        } else {
+                            if (cl == 0) carry_row[wave] = j + crow;
+                            *reinterpret_cast<float4 *>(carry_val + (uint64_t)wave * KB + 4 * cl) = cacc;
+                        }
+                    }
+                    crow = row;
+                    cacc = a;
+                }
+            }
+            if (crow >= 0) {
+                const uint32_t rs = s_off[w][crow];
+                if (rs >= e0) {
+                    store4(Y + (uint64_t)(j + crow) * ldy + 4 * cl, cacc, alpha, beta);
+                } else {
+                    if (cl == 0) carry_row[wave] = j + crow;
+                    *reinterpret_cast<float4 *>(carry_val + (uint64_t)wave * KB + 4 * cl) = cacc;
+                }
+            }
+        }
+        // empty rows that start in the wave's range (and trailing empty rows)
+        if (r < R) {
+            const uint32_t rs = s_off[w][lane], re = s_off[w][lane + 1];
+            if (re == rs && ((rs >= e0 && rs < e1) || (last && rs == total))) {
+#pragma unroll
+                for (int q = 0; q < KB; ++q) {
+                    const uint64_t yi = (uint64_t)r * ldy + q;
+                    Y[yi] = beta == 0.0f ? 0.0f : beta * Y[yi];
+                }
+            }
+        }
+        const uint32_t next_start = s_off[w][64];
+        __builtin_amdgcn_wave_barrier();
+        j += 64;
+        if (!(next_start < e1 || (last && j < R))) break;
+    }
+}
+
 // Carry chains: consecutive slots (waves) carrying into the same row, added
 // to the row's head value.  A wave scans 64 slots.  Chains shorter than 64
 // slots are summed by lane groups in parallel (64/KB groups, one lane per
@@ -600,8 +789,13 @@ static dcsr_status launch_wide(dcsr_handle h, const float *X, int64_t ldx, float
     DCSR_TRY(ws_get(h, WS_SPMM_CVAL, cap_slots * 4 * KB, (void **)&cval));
     const uint32_t *foff = (const uint32_t *)h->ws.ptr[WS_FOFF];
     const uint32_t *rowmap = (const uint32_t *)h->ws.ptr[WS_ROWMAP];
-    k_spmm_wide<KB><<<blocks, TB, 0, s>>>(R, foff, rowmap, h->row_start, h->pool_col, h->pool_val, X, ldx, Y, ldy,
-                                          alpha, beta, crow, cval);
+    const bool vec4 = ((uintptr_t)X % 16 == 0) && ((uintptr_t)Y % 16 == 0) && ldx % 4 == 0 && ldy % 4 == 0;
+    if (vec4)
+        k_spmm_wide4<KB><<<blocks, TB, 0, s>>>(R, foff, rowmap, h->row_start, h->pool_col, h->pool_val, X, ldx, Y,
+                                               ldy, alpha, beta, crow, cval);
+    else
+        k_spmm_wide<KB><<<blocks, TB, 0, s>>>(R, foff, rowmap, h->row_start, h->pool_col, h->pool_val, X, ldx, Y,
+                                              ldy, alpha, beta, crow, cval);
     k_carry<KB><<<grid_for(nwaves, TB), TB, 0, s>>>((uint32_t)nwaves, crow, cval, Y, ldy, alpha);
     DCSR_HIP(hipGetLastError());
     return DCSR_OK;
