@@ -190,28 +190,50 @@ def bench_powerlaw(args):
     m = dcsr_amd.DcsrMatrix(R_CFG, R_CFG, pool_factor=8.0)
     m.LoadMatrix(d_r, d_c, d_v)
     for k in range(Wm):
+        m.UpdateMatrix(*d_b[K + k], flags=dcsr_amd.UPDATE_NOSYNC)
+        m.spmv(d_x, d_y)
         m.UpdateMatrix(*d_b[K + k])
         m.spmv(d_x, d_y)
     torch.cuda.synchronize()
-    m.LoadMatrix(d_r, d_c, d_v)
-    m.BinRows()
-    torch.cuda.synchronize()
 
-    ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True),
-           torch.cuda.Event(enable_timing=True)) for _ in range(K)]
-    stats = []
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for k in range(K):
-        e0, e1, e2 = ev[k]
-        e0.record(stream)
-        m.UpdateMatrix(*d_b[k])
-        e1.record(stream)
-        m.spmv(d_x, d_y)
-        e2.record(stream)
-        stats.append(m.info())
-    torch.cuda.synchronize()
-    elapsed = time.perf_counter() - t0
+    def reload():
+        m.LoadMatrix(d_r, d_c, d_v)
+        m.BinRows()
+        torch.cuda.synchronize()
+
+    def timed(flags, with_info):
+        ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True),
+               torch.cuda.Event(enable_timing=True)) for _ in range(K)]
+        stats = []
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for k in range(K):
+            e0, e1, e2 = ev[k]
+            e0.record(stream)
+            m.UpdateMatrix(*d_b[k], flags=flags)
+            e1.record(stream)
+            m.spmv(d_x, d_y)
+            e2.record(stream)
+            if with_info:
+                stats.append(m.info())
+        torch.cuda.synchronize()
+        return time.perf_counter() - t0, ev, stats
+
+    # timed: the asynchronous update (as the reference's UpdateMatrix is a
+    # bare launch): no host read-back per step, pool room from host bounds;
+    # the batch status is read once after the timed steps
+    reload()
+    elapsed, ev, _ = timed(dcsr_amd.UPDATE_NOSYNC, False)
+    if m.status() != 0:
+        raise RuntimeError(f"update status {m.status()} after the timed steps")
+    y_async = d_y.clone()
+    # the same steps through the synchronous call (status per call), also
+    # timed; its per-call counters give the algorithmic bytes, and its last
+    # SpMV must equal the asynchronous run's bit for bit
+    reload()
+    elapsed_sync, _, stats = timed(0, True)
+    if not torch.equal(d_y, y_async):
+        raise RuntimeError("asynchronous and synchronous runs differ")
 
     upd_ms = np.array([e0.elapsed_time(e1) for e0, e1, _ in ev])
     spmv_ms = np.array([e1.elapsed_time(e2) for _, e1, e2 in ev])
@@ -252,6 +274,9 @@ def bench_powerlaw(args):
          p99_step_ms=float(np.percentile(step_ms, 99)),
          mean_update_ms=float(upd_ms.mean()),
          mean_spmv_ms=float(spmv_ms.mean()),
+         update_call="DCSR_UPDATE_NOSYNC, status read once after the timed steps",
+         sync_call_triples_per_s=trip / elapsed_sync,
+         sync_call_ms_per_step=elapsed_sync * 1e3 / K,
          roofline=roof,
          cpu_baseline=cpu)
 

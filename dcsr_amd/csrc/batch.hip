@@ -505,11 +505,11 @@ __global__ __launch_bounds__(BT) void k_batch_step(uint32_t M, const uint32_t *_
     }
     // the last workgroup out resets the counters (no memset: replay-safe)
     if (threadIdx.x == 0) {
-        __threadfence();
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
         if (atomicAdd(&bs->done, 1u) == gridDim.x - 1) {
             bs->work = 0;
             bs->done = 0;
-            __threadfence();
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
         }
     }
 }
@@ -758,9 +758,8 @@ dcsr_status dcsr_batch_step(dcsr_batch b, const int64_t *tri_off, const int32_t 
     }
     DCSR_HIP(hipGetLastError());
     if (do_update) {
-        h->state_known = false;
+        state_unknown(h, n, h->pool_cap);
         h->flat_dirty = true;
-        h->hubs_dirty = true;
     }
     return DCSR_OK;
 }

@@ -49,9 +49,9 @@ struct DevState {
 enum WsSlot {
     WS_KEY_A, WS_KEY_B, WS_VAL_A, WS_VAL_B, WS_FLAGS, WS_SCAN, WS_PART, WS_HIST,
     WS_UCOL, WS_UVAL, WS_UT, WS_UPOS, WS_UCODE, WS_UPRE,
-    WS_TROW, WS_TUBEG, WS_TINS, WS_TDEL, WS_TOLDS, WS_TOLDL, WS_TDEST, WS_TNEWL, WS_TMODE, WS_TOLDPRE,
+    WS_TROW, WS_TUBEG, WS_TINS, WS_TDEL, WS_TOLDS, WS_TMOVED, WS_TDEST, WS_TNEWL, WS_TMODE, WS_TOLDPRE,
     WS_HUB_ROWS, WS_HUB_PRE, WS_HUB_PART, WS_HUB_DONE,
-    WS_PPROBE, WS_PPLAN, WS_TPAIR, WS_TPAIRPRE, WS_FOFF, WS_ROWMAP, WS_CARRY_ROW, WS_CARRY_VAL, WS_SPMM_CROW, WS_SPMM_CVAL,
+    WS_PPROBE, WS_PPLAN, WS_UROW, WS_TPAIRPRE, WS_FOFF, WS_ROWMAP, WS_CARRY_ROW, WS_CARRY_VAL, WS_SPMM_CROW, WS_SPMM_CVAL,
     WS_SG_P, WS_SG_POFF, WS_SG_CNT, WS_SG_COFF, WS_SG_LIST, WS_SG_STATE, WS_SG_BSUM, WS_SG_TCOL, WS_SG_TVAL,
     WS_SG_HPRE, WS_SG_EA, WS_SG_APOS, WS_SG_KEY0, WS_SG_KEY1, WS_SG_VAL0, WS_SG_VAL1, WS_SG_FLAG, WS_SG_EPOS,
     WS_CANON, WS_TMP0, WS_TMP1, WS_TMP2,
@@ -83,6 +83,19 @@ struct dcsr_matrix_s {
     DevState *ds;            // device
     DevState *hs;            // pinned host mirror
     int64_t nnz_h, bump_h, garbage_h, hub_rows_h, hub_chunks_h;
+    // upper bounds of the live entries and of the bump pointer, valid also
+    // while state_known is false (a NOSYNC update of n triples adds at most n
+    // entries and claims at most its claim bound): they size SpMV grids and
+    // decide pool room without reading the device state back
+    int64_t nnz_ub, bump_ub;
+    // asynchronous state snapshot taken after each NOSYNC update: once its
+    // event has completed, the bounds restart from its exact values plus what
+    // was launched after it (nnz_since, bump_since)
+    DevState *snap;          // pinned
+    hipEvent_t snap_ev;
+    bool snap_pending;       // a copy is in flight (at most one at a time)
+    bool snap_stale;         // the state changed outside the counters since
+    int64_t nnz_since, bump_since;
     bool canonical;
     bool hubs_dirty;
     bool flat_dirty;         // flat row offsets (SpMV binning) need a rebuild
@@ -118,9 +131,25 @@ struct dcsr_matrix_s {
 dcsr_status dcsr_hip_fail(hipError_t e, const char *what, const char *file, int line);
 dcsr_status ws_get(dcsr_handle h, WsSlot slot, size_t bytes, void **out);
 dcsr_status pull_state(dcsr_handle h, hipStream_t s);   // copy DevState to host + sync
+// After launches whose device-side results the host does not read back: at
+// most `added` entries were inserted and `claimed` pool slots claimed.
+void state_unknown(dcsr_handle h, int64_t added, int64_t claimed);
+// The host mirrors are exact (after a read-back, or after work whose result
+// the host computed itself).
+void state_exact(dcsr_handle h);
+// Enqueue an asynchronous copy of the device state (NOSYNC paths), and fold
+// a completed one into the bounds (never waits).
+dcsr_status snapshot_state(dcsr_handle h, hipStream_t s);
+void refresh_bounds(dcsr_handle h);
+// Upper bounds of the live entries and of the bump pointer (exact when the
+// state is known).
+static inline int64_t live_bound(dcsr_handle h) { return h->state_known ? h->nnz_h : h->nnz_ub; }
+static inline int64_t bump_bound(dcsr_handle h) { return h->state_known ? h->bump_h : h->bump_ub; }
 dcsr_status ensure_flat(dcsr_handle h, hipStream_t s);  // spmv.hip
 dcsr_status pool_reserve(dcsr_handle h, int64_t min_free, hipStream_t s, const int32_t *reserve_len);
 dcsr_status canonicalize(dcsr_handle h, hipStream_t s);
+// Compaction at the current capacity, in stream order, without a read-back.
+dcsr_status compact_in_order(dcsr_handle h, hipStream_t s);
 dcsr_status alloc_pool(dcsr_handle h, int64_t cap);
 // update.hip: the two phases of dcsr_update (prepare only reads the matrix)
 dcsr_status update_prepare(dcsr_handle h, const int32_t *rows, const int32_t *cols, const float *vals, int64_t n,

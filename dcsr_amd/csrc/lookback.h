@@ -68,11 +68,11 @@ __device__ __forceinline__ uint32_t lb_ticket(unsigned int *ctl, uint32_t *lds_s
 // the counters for the next launch.
 __device__ __forceinline__ void lb_finish(unsigned int *ctl)
 {
-    __threadfence();
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
     if (atomicAdd(&ctl[1], 1u) == gridDim.x - 1) {
         ctl[0] = 0u;
         ctl[1] = 0u;
-        __threadfence();
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
     }
 }
 

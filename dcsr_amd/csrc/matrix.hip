@@ -307,8 +307,56 @@ dcsr_status pull_state(dcsr_handle h, hipStream_t s)
     h->nnz_h = h->hs->nnz;
     h->bump_h = (int64_t)h->hs->bump;
     h->garbage_h = h->hs->garbage;
-    h->state_known = true;
+    state_exact(h);
     return DCSR_OK;
+}
+
+void state_exact(dcsr_handle h)
+{
+    h->state_known = true;
+    h->snap_stale = true;
+}
+
+void state_unknown(dcsr_handle h, int64_t added, int64_t claimed)
+{
+    const int64_t nnz = live_bound(h) + added, bump = bump_bound(h) + claimed;
+    // neither can pass the pool: live entries sit in it and the device
+    // rejects a batch whose claims would
+    h->nnz_ub = nnz < h->pool_cap ? nnz : h->pool_cap;
+    h->bump_ub = bump < h->pool_cap ? bump : h->pool_cap;
+    h->nnz_since += added;
+    h->bump_since += claimed;
+    h->state_known = false;
+    h->hubs_dirty = true;
+}
+
+// At most one snapshot copy is in flight: a new one is enqueued only after
+// the last one's event completed, so the buffer is never written twice at
+// once, whatever streams the calls use.
+dcsr_status snapshot_state(dcsr_handle h, hipStream_t s)
+{
+    if (h->snap_pending) return DCSR_OK;
+    if (!h->snap_ev) DCSR_HIP(hipEventCreateWithFlags(&h->snap_ev, hipEventDisableTiming));
+    DCSR_HIP(hipMemcpyAsync(h->snap, h->ds, sizeof(DevState), hipMemcpyDeviceToHost, s));
+    DCSR_HIP(hipEventRecord(h->snap_ev, s));
+    h->snap_pending = true;
+    h->snap_stale = false;
+    h->nnz_since = 0;
+    h->bump_since = 0;
+    return DCSR_OK;
+}
+
+// The snapshot's values plus the NOSYNC launches counted since it bound the
+// current state, unless the state changed in some other way meanwhile
+// (stale): then the snapshot is only retired.
+void refresh_bounds(dcsr_handle h)
+{
+    if (!h->snap_pending || hipEventQuery(h->snap_ev) != hipSuccess) return;
+    h->snap_pending = false;
+    if (h->snap_stale || h->state_known) return;
+    const int64_t nnz = (int64_t)h->snap->nnz + h->nnz_since, bump = (int64_t)h->snap->bump + h->bump_since;
+    if (nnz < h->nnz_ub) h->nnz_ub = nnz;
+    if (bump < h->bump_ub) h->bump_ub = bump;
 }
 
 static dcsr_status status_from_bits(unsigned bits)
@@ -374,7 +422,7 @@ dcsr_status dcsr_create(dcsr_handle *out, int64_t rows, int64_t cols, const dcsr
     h->canonical = true;
     h->hubs_dirty = false;
     h->flat_dirty = true;
-    h->state_known = true;
+    state_exact(h);
     // SpMV column panels: 2M columns (8 MB of x; measured best on a 10M x 10M
     // power-law matrix between 0.5M and 4M) for matrices of at least 16M
     // entries; both can be overridden for testing
@@ -386,6 +434,7 @@ dcsr_status dcsr_create(dcsr_handle *out, int64_t rows, int64_t cols, const dcsr
     if (st == DCSR_OK) st = alloc_pool(h, h->cfg.min_pool);
     if (st == DCSR_OK && hipMalloc(&h->ds, sizeof(DevState)) != hipSuccess) st = DCSR_ERR_OOM;
     if (st == DCSR_OK && hipHostMalloc(&h->hs, sizeof(DevState), hipHostMallocDefault) != hipSuccess) st = DCSR_ERR_OOM;
+    if (st == DCSR_OK && hipHostMalloc(&h->snap, sizeof(DevState), hipHostMallocDefault) != hipSuccess) st = DCSR_ERR_OOM;
     if (st == DCSR_OK && hipMemset(h->ds, 0, sizeof(DevState)) != hipSuccess) st = DCSR_ERR_HIP;
     if (st == DCSR_OK) {
         // all rows empty
@@ -414,6 +463,8 @@ dcsr_status dcsr_destroy(dcsr_handle h)
     (void)hipFree(h->spare_val);
     (void)hipFree(h->ds);
     if (h->hs) (void)hipHostFree(h->hs);
+    if (h->snap) (void)hipHostFree(h->snap);
+    if (h->snap_ev) (void)hipEventDestroy(h->snap_ev);
     for (int i = 0; i < WS_COUNT; ++i)
         if (h->ws.ptr[i]) (void)hipFree(h->ws.ptr[i]);
     free(h);
@@ -442,7 +493,7 @@ dcsr_status dcsr_initialize(dcsr_handle h, int32_t bin_length, dcsr_stream_t str
     h->canonical = true;
     h->hubs_dirty = true;
     h->flat_dirty = true;
-    h->state_known = true;
+    state_exact(h);
     return DCSR_OK;
 }
 
@@ -637,11 +688,14 @@ dcsr_status dcsr_export_csr(dcsr_handle h, int32_t *row_offsets, int32_t *cols, 
 // The destination is the handle's spare pool (allocated only when it is too
 // small); the old pool becomes the spare, so steady-state compaction makes no
 // allocation and no host synchronization beyond the state read-back.
+// Copy the live rows, in natural order, into the spare pool (grown to
+// new_cap when smaller) and swap pools.  Needs no read-back: the copy is
+// sized from the host's bound of the live entries and stops at the device's
+// count; the device's bump pointer becomes its live count.
 static dcsr_status compact_into(dcsr_handle h, int64_t new_cap, hipStream_t s)
 {
     const int64_t R = h->rows;
-    if (!h->state_known) DCSR_TRY(pull_state(h, s));
-    const int64_t nnz = h->nnz_h;
+    const int64_t nnz = live_bound(h);
     if (new_cap < nnz) new_cap = nnz;
     if (new_cap < h->cfg.min_pool) new_cap = h->cfg.min_pool;
     if (new_cap > (int64_t)0xfffffff0ll) return DCSR_ERR_POOL_EXHAUSTED;
@@ -672,10 +726,19 @@ static dcsr_status compact_into(dcsr_handle h, int64_t new_cap, hipStream_t s)
     h->spare_col = oc;
     h->spare_val = ov;
     h->spare_cap = ocap;
-    h->bump_h = nnz;
-    h->garbage_h = 0;
+    if (h->state_known) {
+        h->bump_h = nnz;
+        h->garbage_h = 0;
+    } else {
+        // the device's bump pointer is now its live count; a snapshot in
+        // flight predates the swap
+        h->bump_ub = h->nnz_ub;
+        h->snap_stale = true;
+    }
     return DCSR_OK;
 }
+
+dcsr_status compact_in_order(dcsr_handle h, hipStream_t s) { return compact_into(h, h->pool_cap, s); }
 
 dcsr_status dcsr_compact(dcsr_handle h, dcsr_stream_t stream)
 {

@@ -7,10 +7,13 @@
 // digit ranking (8 ballots per item give the lanes sharing a digit).
 // Scans are single-pass (one launch; cross-tile prefixes by decoupled
 // look-back, lookback.h).  A radix pass is three launches: per-tile digit
-// counts, one scan of the count table, stable scatter.  (A onesweep pass
-// that resolves each digit's offset by look-back walks hundreds of tiles
-// serially per digit here, where every look-back poll is a cross-XCD
-// coherent load; the table scan is faster.)
+// counts, a scan of the count table, stable scatter.  The table scan runs
+// one workgroup per digit over that digit's tile counts (all loads of a
+// digit issued at once); the scatter adds the exclusive prefix of the 256
+// digit totals, which it scans itself.  (A onesweep pass that resolves each
+// digit's offset by look-back walks hundreds of tiles serially per digit
+// here, where every look-back poll is a cross-XCD coherent load; the table
+// scan is faster.)
 #include <atomic>
 #include "lookback.h"
 #include "prims.h"
@@ -131,11 +134,53 @@ __global__ __launch_bounds__(RS_THREADS) void k_rs_hist(const uint64_t *__restri
     hist[(int64_t)threadIdx.x * nblocks + blockIdx.x] = cnt[threadIdx.x];
 }
 
+// Exclusive scan of each digit's row of the count table (digit-major:
+// hist[d * nb + tile]), in place, one workgroup per digit; dtot[d] gets the
+// digit's total.  Rows of up to RS_SCAN_CHUNKS * 2048 tiles: every load of
+// the row is in flight before the first scan step.
+constexpr int RS_SCAN_CHUNKS = 4;
+__global__ __launch_bounds__(RS_THREADS) void k_rs_scan_digits(uint32_t *__restrict__ hist, int64_t nb,
+                                                               uint32_t *__restrict__ dtot)
+{
+    __shared__ uint32_t lds4[4];
+    uint32_t *row = hist + (int64_t)blockIdx.x * nb;
+    uint32_t v[RS_SCAN_CHUNKS][RS_IPT];
+#pragma unroll
+    for (int c = 0; c < RS_SCAN_CHUNKS; ++c) {
+        const int64_t i0 = (int64_t)c * RS_TILE + (int64_t)threadIdx.x * RS_IPT;
+#pragma unroll
+        for (int j = 0; j < RS_IPT; ++j) v[c][j] = i0 + j < nb ? row[i0 + j] : 0u;
+    }
+    uint32_t carry = 0;
+#pragma unroll
+    for (int c = 0; c < RS_SCAN_CHUNKS; ++c) {
+        if ((int64_t)c * RS_TILE >= nb) break;
+        const int64_t i0 = (int64_t)c * RS_TILE + (int64_t)threadIdx.x * RS_IPT;
+        uint32_t s = 0;
+#pragma unroll
+        for (int j = 0; j < RS_IPT; ++j) s += v[c][j];
+        uint32_t tot;
+        uint32_t p = carry + block_excl_scan_256(s, lds4, &tot);
+#pragma unroll
+        for (int j = 0; j < RS_IPT; ++j) {
+            if (i0 + j < nb) row[i0 + j] = p;
+            p += v[c][j];
+        }
+        carry += tot;
+    }
+    if (threadIdx.x == 0) dtot[blockIdx.x] = carry;
+}
+
+// Stable scatter of one tile.  offs[d * nb + tile] is the tile's offset
+// within digit d (dtot != nullptr: the per-digit scan, whose digit totals
+// dtot the scatter turns into digit bases) or the global offset (dtot ==
+// nullptr: a scan of the whole table).
 __global__ __launch_bounds__(RS_THREADS) void k_rs_scatter(const uint64_t *__restrict__ kin,
                                                            const uint32_t *__restrict__ vin,
                                                            uint64_t *__restrict__ kout, uint32_t *__restrict__ vout,
                                                            int64_t n, int shift,
-                                                           const uint32_t *__restrict__ offs, int64_t nblocks)
+                                                           const uint32_t *__restrict__ offs, int64_t nblocks,
+                                                           const uint32_t *__restrict__ dtot)
 {
     __shared__ uint64_t skey[RS_TILE];
     __shared__ uint32_t sval[RS_TILE];
@@ -146,6 +191,9 @@ __global__ __launch_bounds__(RS_THREADS) void k_rs_scatter(const uint64_t *__res
 
     const int w = threadIdx.x >> 6;
     const int lane = lane_id();
+    // the tile's offsets do not depend on the ranking: load them first
+    const uint32_t go = offs[(int64_t)threadIdx.x * nblocks + blockIdx.x];
+    const uint32_t dt = dtot ? dtot[threadIdx.x] : 0u;
     for (int i = threadIdx.x; i < RS_WAVES * 256; i += RS_THREADS) (&wcnt[0][0])[i] = 0;
     __syncthreads();
 
@@ -185,7 +233,7 @@ __global__ __launch_bounds__(RS_THREADS) void k_rs_scatter(const uint64_t *__res
         }
         uint32_t tot;
         loff[d] = block_excl_scan_256(s, lds4, &tot);
-        goff[d] = offs[(int64_t)d * nblocks + blockIdx.x];
+        goff[d] = go + block_excl_scan_256(dt, lds4, &tot);
     }
     __syncthreads();
 #pragma unroll
@@ -238,30 +286,44 @@ hipError_t scan_exclusive_u32(const uint32_t *in, uint32_t *out, int64_t n, int 
     return hipGetLastError();
 }
 
+// Count table (256 x tiles) plus the 256 digit totals.
 size_t radix_hist_words(int64_t n)
 {
     const int64_t nb = (n + RS_TILE - 1) / RS_TILE;
-    return (size_t)(nb < 1 ? 1 : nb) * 256;
+    return (size_t)(nb < 1 ? 1 : nb) * 256 + 256;
 }
 
-size_t radix_part_words(int64_t n) { return scan_part_words((int64_t)radix_hist_words(n), 1); }
+static bool radix_digit_scan(int64_t nb) { return nb <= (int64_t)RS_SCAN_CHUNKS * RS_TILE; }
 
-// Per pass: per-tile digit counts (column-major: digit-major, tile-minor),
-// one single-pass scan of that table, stable scatter.
+size_t radix_part_words(int64_t n)
+{
+    const int64_t nb = (n + RS_TILE - 1) / RS_TILE;
+    return radix_digit_scan(nb) ? 4 : scan_part_words(nb * 256, 1);
+}
+
+// Per pass: per-tile digit counts (digit-major, tile-minor), a scan of that
+// table (per digit up to 8192 tiles, else one single-pass scan of the whole
+// table), stable scatter.
 hipError_t radix_sort_pairs(uint64_t *k0, uint32_t *v0, uint64_t *k1, uint32_t *v1, int64_t n, int nbits,
                             uint32_t *hist, uint32_t *part, int *which, hipStream_t s)
 {
     *which = 0;
     if (n <= 1) return hipSuccess;
     const int64_t nb = (n + RS_TILE - 1) / RS_TILE;
+    const bool per_digit = radix_digit_scan(nb);
+    uint32_t *dtot = per_digit ? hist + nb * 256 : nullptr;
     uint64_t *ki = k0, *ko = k1;
     uint32_t *vi = v0, *vo = v1;
     int cur = 0;
     for (int shift = 0; shift < nbits; shift += 8) {
         k_rs_hist<<<(unsigned)nb, RS_THREADS, 0, s>>>(ki, n, shift, hist, nb);
-        const hipError_t e = scan_exclusive_u32(hist, hist, nb * 256, 1, part, nullptr, s);
-        if (e != hipSuccess) return e;
-        k_rs_scatter<<<(unsigned)nb, RS_THREADS, 0, s>>>(ki, vi, ko, vo, n, shift, hist, nb);
+        if (per_digit) {
+            k_rs_scan_digits<<<256, RS_THREADS, 0, s>>>(hist, nb, dtot);
+        } else {
+            const hipError_t e = scan_exclusive_u32(hist, hist, nb * 256, 1, part, nullptr, s);
+            if (e != hipSuccess) return e;
+        }
+        k_rs_scatter<<<(unsigned)nb, RS_THREADS, 0, s>>>(ki, vi, ko, vo, n, shift, hist, nb, dtot);
         uint64_t *tk = ki; ki = ko; ko = tk;
         uint32_t *tv = vi; vi = vo; vo = tv;
         cur ^= 1;

@@ -736,12 +736,14 @@ dcsr_status ensure_flat(dcsr_handle h, hipStream_t s)
     DCSR_TRY(ws_get(h, WS_PART, scan_part_words(R + 1, 1) * 4, (void **)&part));
     // row lengths are non-negative int32: scan them as uint32 in place
     DCSR_HIP(scan_exclusive_u32((const uint32_t *)h->row_len, foff, R, 1, part, foff + R, s));
-    if (!h->state_known) DCSR_TRY(pull_state(h, s));
-    const int64_t ng = h->nnz_h / GRAN + 1;
+    // the grid covers an upper bound of the live entries (the kernel stops
+    // at the device's count), so no read-back is needed after NOSYNC updates
+    const int64_t live = live_bound(h);
+    const int64_t ng = live / GRAN + 1;
     uint32_t *rowmap;
     // sized by the pool (an upper bound on nnz): no reallocation while the
     // matrix grows within its pool
-    const int64_t ngcap = (h->pool_cap > h->nnz_h ? h->pool_cap : h->nnz_h) / GRAN + 1;
+    const int64_t ngcap = (h->pool_cap > live ? h->pool_cap : live) / GRAN + 1;
     DCSR_TRY(ws_get(h, WS_ROWMAP, (size_t)ngcap * 4, (void **)&rowmap));
     k_rowmap<<<grid_for(ng, TB), TB, 0, s>>>((uint32_t)R, foff, rowmap, (uint32_t)ng);
     DCSR_HIP(hipGetLastError());
@@ -755,7 +757,7 @@ static dcsr_status launch_flat(dcsr_handle h, const float *X, int64_t ldx, float
 {
     const uint32_t R = (uint32_t)h->rows;
     const uint32_t pw = DCSR_SPMV_PW;
-    int64_t nwaves = (h->nnz_h + pw - 1) / pw;
+    int64_t nwaves = (live_bound(h) + pw - 1) / pw;
     if (nwaves < 1) nwaves = 1;
     const unsigned blocks = (unsigned)((nwaves + 3) / 4);
     nwaves = (int64_t)blocks * 4;
@@ -778,7 +780,7 @@ static dcsr_status launch_wide(dcsr_handle h, const float *X, int64_t ldx, float
                                float beta, hipStream_t s)
 {
     const uint32_t R = (uint32_t)h->rows;
-    int64_t nwaves = (h->nnz_h + DCSR_SPMV_PW - 1) / DCSR_SPMV_PW;
+    int64_t nwaves = (live_bound(h) + DCSR_SPMV_PW - 1) / DCSR_SPMV_PW;
     if (nwaves < 1) nwaves = 1;
     const unsigned blocks = (unsigned)((nwaves + 3) / 4);
     nwaves = (int64_t)blocks * 4;
@@ -804,7 +806,6 @@ static dcsr_status launch_wide(dcsr_handle h, const float *X, int64_t ldx, float
 static dcsr_status prepare_compute(dcsr_handle h, hipStream_t s)
 {
     h->last_stream = s;
-    if (!h->state_known) DCSR_TRY(pull_state(h, s));
     return ensure_flat(h, s);
 }
 
@@ -812,7 +813,7 @@ static dcsr_status prepare_compute(dcsr_handle h, hipStream_t s)
 // large matrix, and the virtual row count within 32 bits.
 static bool pnl_eligible(dcsr_handle h)
 {
-    if (!h->canonical || h->cols < 2 * h->pnl_cols || h->nnz_h < h->pnl_min_nnz) return false;
+    if (!h->state_known || !h->canonical || h->cols < 2 * h->pnl_cols || h->nnz_h < h->pnl_min_nnz) return false;
     const int64_t P = (h->cols + h->pnl_cols - 1) / h->pnl_cols;
     return P * (h->rows + 1) < (1ll << 31);
 }
@@ -855,7 +856,7 @@ static dcsr_status spmv_paneled(dcsr_handle h, const float *x, float *y, float a
     const int P = h->pnl_count;
     const uint32_t Rv = (uint32_t)((int64_t)P * (R + 1));
     const uint32_t pw = DCSR_SPMV_PW;
-    int64_t nwaves = (h->nnz_h + pw - 1) / pw;
+    int64_t nwaves = (live_bound(h) + pw - 1) / pw;
     if (nwaves < 1) nwaves = 1;
     const unsigned blocks = (unsigned)((nwaves + 3) / 4);
     nwaves = (int64_t)blocks * 4;

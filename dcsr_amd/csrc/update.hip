@@ -110,36 +110,71 @@ enum { TP_I = 0, TP_A, TP_D, TP_IX, TP_DX, TP_WORDS = 8 };
 enum { FC_HEADS = 0, FC_PLAN = 1 };
 
 // True in every thread of the last workgroup of the launch to finish; it then
-// sees what every other workgroup wrote before finishing.
+// sees what every other workgroup wrote before finishing.  Each workgroup
+// releases (L2 write-back) before it is counted and only the last one
+// acquires (L2 invalidate): a full fence in every workgroup would also
+// invalidate the L2 under the workgroups still running.
 __device__ __forceinline__ bool last_block(unsigned int *ctr, uint32_t *s_flag)
 {
     __syncthreads();
     if (threadIdx.x == 0) {
-        __threadfence();
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
         const bool last = atomicAdd(ctr, 1u) == gridDim.x - 1;
         if (last) atomicExch(ctr, 0u);
         *s_flag = last ? 1u : 0u;
     }
     __syncthreads();
     const bool last = *s_flag != 0u;
-    if (last) __threadfence();
+    if (last) __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
     return last;
 }
 
-// One workgroup: exclusive scan of column `src` of a table with `stride`
-// words per row (cnt rows) into column `dst`; returns the column total.
-__device__ uint32_t block_scan_column(uint32_t *a, int64_t cnt, int stride, int src, int dst, uint32_t *lds4)
+// One workgroup: exclusive scans of NC columns of tile tables with 8 words
+// per row, tab[c][i * 8 + src[c]] -> tab[c][i * 8 + dst[c]] for i < cnt;
+// tot[c] receives column c's total.  Every load of a 2048-row chunk (all
+// columns) is issued before its first scan step, so a chunk costs one
+// memory round trip.
+static_assert(TT_WORDS == 8 && TP_WORDS == 8, "tile tables have 8 words per tile");
+template <int NC>
+__device__ void block_scan_columns(uint32_t *const (&tab)[NC], const int (&src)[NC], const int (&dst)[NC],
+                                   int64_t cnt, uint32_t *lds4, uint32_t (&tot)[NC])
 {
-    uint32_t carry = 0;
-    for (int64_t base = 0; base < cnt; base += TB) {
-        const int64_t i = base + threadIdx.x;
-        const uint32_t v = i < cnt ? a[i * stride + src] : 0u;
-        uint32_t tot;
-        const uint32_t ex = block_excl_scan_256(v, lds4, &tot);
-        if (i < cnt) a[i * stride + dst] = carry + ex;
-        carry += tot;
+#pragma unroll
+    for (int c = 0; c < NC; ++c) tot[c] = 0;
+    for (int64_t base = 0; base < cnt; base += TILE) {
+        const int64_t i0 = base + (int64_t)threadIdx.x * IPT;
+        uint32_t v[NC][IPT];
+#pragma unroll
+        for (int c = 0; c < NC; ++c) {
+#pragma unroll
+            for (int j = 0; j < IPT; ++j) v[c][j] = i0 + j < cnt ? tab[c][(i0 + j) * 8 + src[c]] : 0u;
+        }
+#pragma unroll
+        for (int c = 0; c < NC; ++c) {
+            uint32_t sum = 0;
+#pragma unroll
+            for (int j = 0; j < IPT; ++j) sum += v[c][j];
+            uint32_t t;
+            uint32_t p = tot[c] + block_excl_scan_256(sum, lds4, &t);
+#pragma unroll
+            for (int j = 0; j < IPT; ++j) {
+                if (i0 + j < cnt) tab[c][(i0 + j) * 8 + dst[c]] = p;
+                p += v[c][j];
+            }
+            tot[c] += t;
+        }
     }
-    return carry;
+}
+
+// The thread's IPT blocked keys (~0 past n) and the key before them (~0
+// before item 0: it differs from every valid key and, shifted, from every
+// valid row).
+__device__ __forceinline__ void load_keys(const uint64_t *__restrict__ key, int64_t n, int64_t i0,
+                                          uint64_t (&k)[IPT], uint64_t &prev)
+{
+    prev = (i0 > 0 && i0 - 1 < n) ? key[i0 - 1] : ~0ull;
+#pragma unroll
+    for (int j = 0; j < IPT; ++j) k[j] = i0 + j < n ? key[i0 + j] : ~0ull;
 }
 
 // Heads of the sorted batch: an item is a coordinate head if it is the
@@ -148,20 +183,15 @@ __device__ uint32_t block_scan_column(uint32_t *a, int64_t cnt, int stride, int 
 struct HeadFlags {
     uint32_t h, r;  // bit j: item j of the thread
 };
-__device__ __forceinline__ HeadFlags head_flags(const uint64_t *__restrict__ key, int64_t n, int64_t i0, int cbits)
+__device__ __forceinline__ HeadFlags head_flags(const uint64_t (&k)[IPT], uint64_t prev, int cbits)
 {
     HeadFlags f{0u, 0u};
-    // ~0 before item 0: differs from every valid key and, shifted, from
-    // every valid row
-    uint64_t prev = (i0 > 0 && i0 - 1 < n) ? key[i0 - 1] : ~0ull;
 #pragma unroll
     for (int j = 0; j < IPT; ++j) {
-        const int64_t i = i0 + j;
-        const uint64_t k = i < n ? key[i] : ~0ull;
-        const bool valid = k != ~0ull;
-        if (valid && prev != k) f.h |= 1u << j;
-        if (valid && (prev >> cbits) != (k >> cbits)) f.r |= 1u << j;
-        prev = k;
+        const bool valid = k[j] != ~0ull;
+        if (valid && prev != k[j]) f.h |= 1u << j;
+        if (valid && (prev >> cbits) != (k[j] >> cbits)) f.r |= 1u << j;
+        prev = k[j];
     }
     return f;
 }
@@ -173,7 +203,9 @@ __global__ __launch_bounds__(TB) void k_upd_heads_count(const uint64_t *__restri
 {
     __shared__ uint32_t lds4[4], s_flag;
     const int64_t tile = blockIdx.x;
-    const HeadFlags f = head_flags(key, n, tile * TILE + (int64_t)threadIdx.x * IPT, cbits);
+    uint64_t k[IPT], prev;
+    load_keys(key, n, tile * TILE + (int64_t)threadIdx.x * IPT, k, prev);
+    const HeadFlags f = head_flags(k, prev, cbits);
     const uint32_t hc = block_sum_u32(__popc(f.h), lds4);
     const uint32_t rc = block_sum_u32(__popc(f.r), lds4);
     if (threadIdx.x == 0) {
@@ -181,55 +213,69 @@ __global__ __launch_bounds__(TB) void k_upd_heads_count(const uint64_t *__restri
         tt[tile * TT_WORDS + TT_R] = rc;
     }
     if (last_block(&fin[FC_HEADS], &s_flag)) {
-        const uint32_t nu = block_scan_column(tt, gridDim.x, TT_WORDS, TT_H, TT_HX, lds4);
-        const uint32_t nt = block_scan_column(tt, gridDim.x, TT_WORDS, TT_R, TT_RX, lds4);
+        uint32_t *const tabs[2] = {tt, tt};
+        uint32_t tot[2];
+        block_scan_columns<2>(tabs, {TT_H, TT_R}, {TT_HX, TT_RX}, gridDim.x, lds4, tot);
         if (threadIdx.x == 0) {
-            ds->nuniq = nu;
-            ds->ntouched = nt;
+            ds->nuniq = tot[0];
+            ds->ntouched = tot[1];
         }
     }
 }
 
-// Pass 2 of the heads: every head writes its compacted record: ucol/usum/ut
-// for coordinate u, trow/tubeg (and zeroed tins/tdel) for touched row t.
-// Sums run left to right from 0 in batch order, as the host path
-// accumulates B_row[j].  A tile's records are contiguous, so they are
-// staged in LDS and written out coalesced.  Workgroup 0 stores the
-// tubeg[ntouched] = nuniq sentinel.
+// Pass 2 of the heads: every head writes its compacted record: ucol/usum/
+// ut/urow for coordinate u, trow/tubeg (and zeroed tins/tdel) for touched
+// row t.  Sums run left to right from 0 in batch order, as the host path
+// accumulates B_row[j]; a run is summed from the thread's registers and
+// only a run that continues past the thread's items reads memory.  A tile's
+// records are contiguous, so they are staged in LDS and written out
+// coalesced.  Workgroup 0 stores the tubeg[ntouched] = nuniq sentinel.
 __global__ __launch_bounds__(TB) void k_upd_heads_emit(const uint64_t *__restrict__ key,
                                                        const uint32_t *__restrict__ val, int64_t n, int cbits,
                                                        const uint32_t *__restrict__ tt, const DevState *ds,
                                                        int32_t *__restrict__ ucol, float *__restrict__ usum,
-                                                       uint32_t *__restrict__ ut, uint32_t *__restrict__ trow,
-                                                       uint32_t *__restrict__ tubeg, uint32_t *__restrict__ tins,
-                                                       uint32_t *__restrict__ tdel)
+                                                       uint32_t *__restrict__ ut, uint32_t *__restrict__ urow,
+                                                       uint32_t *__restrict__ trow, uint32_t *__restrict__ tubeg,
+                                                       uint32_t *__restrict__ tins, uint32_t *__restrict__ tdel)
 {
     __shared__ uint32_t lds4[4];
     __shared__ int32_t s_ucol[TILE];
     __shared__ float s_usum[TILE];
-    __shared__ uint32_t s_ut[TILE], s_trow[TILE], s_tub[TILE];
+    __shared__ uint32_t s_ut[TILE], s_urow[TILE], s_trow[TILE], s_tub[TILE];
     const int64_t tile = blockIdx.x;
     const int64_t i0 = tile * TILE + (int64_t)threadIdx.x * IPT;
-    const HeadFlags f = head_flags(key, n, i0, cbits);
+    const uint32_t ubase = tt[tile * TT_WORDS + TT_HX], tbase = tt[tile * TT_WORDS + TT_RX];
+    uint64_t k[IPT], prev;
+    load_keys(key, n, i0, k, prev);
+    uint32_t vb[IPT];
+#pragma unroll
+    for (int j = 0; j < IPT; ++j) vb[j] = i0 + j < n ? val[i0 + j] : 0u;
+    const HeadFlags f = head_flags(k, prev, cbits);
     uint32_t htot, rtot;
     uint32_t lu = block_excl_scan_256(__popc(f.h), lds4, &htot);
     uint32_t lt = block_excl_scan_256(__popc(f.r), lds4, &rtot);
-    const uint32_t ubase = tt[tile * TT_WORDS + TT_HX], tbase = tt[tile * TT_WORDS + TT_RX];
 #pragma unroll
     for (int j = 0; j < IPT; ++j) {
         if (!((f.h >> j) & 1u)) continue;
         const bool rh = (f.r >> j) & 1u;
-        const int64_t idx = i0 + j;
-        const uint64_t k = key[idx];
-        float s = 0.0f + __uint_as_float(val[idx]);
-        for (int64_t m = idx + 1; m < n && key[m] == k; ++m) s = s + __uint_as_float(val[m]);
-        s_ucol[lu] = (int32_t)(k & ((1ull << cbits) - 1ull));
+        float s = 0.0f + __uint_as_float(vb[j]);
+        bool open = true;
+#pragma unroll
+        for (int m = j + 1; m < IPT; ++m) {
+            if (open && k[m] == k[j]) s = s + __uint_as_float(vb[m]);
+            else open = false;
+        }
+        if (open)
+            for (int64_t m = i0 + IPT; m < n && key[m] == k[j]; ++m) s = s + __uint_as_float(val[m]);
+        const uint32_t r = (uint32_t)(k[j] >> cbits);
+        s_ucol[lu] = (int32_t)(k[j] & ((1ull << cbits) - 1ull));
         s_usum[lu] = s;
+        s_urow[lu] = r;
         // a coordinate that is not a row head belongs to the last row head
         // before it (possibly in an earlier tile)
         s_ut[lu] = tbase + lt - (rh ? 0u : 1u);
         if (rh) {
-            s_trow[lt] = (uint32_t)(k >> cbits);
+            s_trow[lt] = r;
             s_tub[lt] = ubase + lu;
             ++lt;
         }
@@ -240,6 +286,7 @@ __global__ __launch_bounds__(TB) void k_upd_heads_emit(const uint64_t *__restric
         ucol[ubase + i] = s_ucol[i];
         usum[ubase + i] = s_usum[i];
         ut[ubase + i] = s_ut[i];
+        urow[ubase + i] = s_urow[i];
     }
     for (uint32_t i = threadIdx.x; i < rtot; i += TB) {
         trow[tbase + i] = s_trow[i];
@@ -253,13 +300,15 @@ __global__ __launch_bounds__(TB) void k_upd_heads_emit(const uint64_t *__restric
 // Probe every distinct coordinate in its row: existing -> old + sum (delete
 // when it cancels to 0.0), missing -> insert when the sum is not 0.0.  The
 // eight binary searches of a thread advance in lock step so their loads
-// overlap.  Per-tile counts go to the probe table.
+// overlap.  Per-row insert / delete counts are added once per run of equal
+// rows among the thread's coordinates; per-tile counts go to the probe
+// table.
 __global__ __launch_bounds__(TB) void k_upd_probe(const DevState *__restrict__ ds,
                                                   const uint32_t *__restrict__ row_start,
                                                   const int32_t *__restrict__ row_len,
                                                   const int32_t *__restrict__ pcol, const float *__restrict__ pval,
                                                   const int32_t *__restrict__ ucol, const float *__restrict__ usum,
-                                                  const uint32_t *__restrict__ ut, const uint32_t *__restrict__ trow,
+                                                  const uint32_t *__restrict__ ut, const uint32_t *__restrict__ urow,
                                                   float *__restrict__ uval, uint32_t *__restrict__ upos,
                                                   uint32_t *__restrict__ ucode, uint32_t *__restrict__ tins,
                                                   uint32_t *__restrict__ tdel, uint32_t *__restrict__ tp)
@@ -270,21 +319,25 @@ __global__ __launch_bounds__(TB) void k_upd_probe(const DevState *__restrict__ d
     const uint32_t nu = ds->nuniq;
     uint64_t rs[IPT];
     int32_t lo[IPT], hi[IPT], cc[IPT], L[IPT];
-    uint32_t tt[IPT];
+    uint32_t tt[IPT], rr[IPT];
+    float us[IPT];
 #pragma unroll
     for (int j = 0; j < IPT; ++j) {
         lo[j] = hi[j] = L[j] = cc[j] = 0;
-        tt[j] = 0;
+        tt[j] = rr[j] = 0;
         rs[j] = 0;
+        us[j] = 0.0f;
         if (u0 + j < nu) {
             tt[j] = ut[u0 + j];
             cc[j] = ucol[u0 + j];
+            rr[j] = urow[u0 + j];
+            us[j] = usum[u0 + j];
         }
     }
 #pragma unroll
     for (int j = 0; j < IPT; ++j) {
         if (u0 + j < nu) {
-            const uint32_t r = trow[tt[j]];
+            const uint32_t r = rr[j];
             rs[j] = row_start[r];
             L[j] = row_len[r];
             hi[j] = L[j];
@@ -323,13 +376,20 @@ __global__ __launch_bounds__(TB) void k_upd_probe(const DevState *__restrict__ d
         }
     }
     uint32_t ni = 0, nd = 0, na = 0;
+    uint32_t run_t = 0, run_i = 0, run_d = 0;
 #pragma unroll
     for (int j = 0; j < IPT; ++j) {
         const int64_t u = u0 + j;
         if (u >= nu) continue;
+        if (tt[j] != run_t) {
+            if (run_i) atomicAdd(&tins[run_t], run_i);
+            if (run_d) atomicAdd(&tdel[run_t], run_d);
+            run_t = tt[j];
+            run_i = run_d = 0;
+        }
         uint32_t code;
         float nv;
-        const float s = usum[u];
+        const float s = us[j];
         if (hc[j] == cc[j]) {
             nv = hv[j] + s;
             code = (nv != 0.0f) ? C_ADD : C_DEL;
@@ -340,12 +400,14 @@ __global__ __launch_bounds__(TB) void k_upd_probe(const DevState *__restrict__ d
         uval[u] = nv;
         upos[u] = (uint32_t)lo[j];
         ucode[u] = code;
-        if (code == C_INS) atomicAdd(&tins[tt[j]], 1u);
-        if (code == C_DEL) atomicAdd(&tdel[tt[j]], 1u);
+        run_i += code == C_INS;
+        run_d += code == C_DEL;
         ni += code == C_INS;
         nd += code == C_DEL;
         na += code == C_ADD;
     }
+    if (run_i) atomicAdd(&tins[run_t], run_i);
+    if (run_d) atomicAdd(&tdel[run_t], run_d);
     ni = block_sum_u32(ni, lds4);
     nd = block_sum_u32(nd, lds4);
     na = block_sum_u32(na, lds4);
@@ -356,8 +418,8 @@ __global__ __launch_bounds__(TB) void k_upd_probe(const DevState *__restrict__ d
     }
 }
 
-// Per touched row: new length, mode (values in place / rewrite) and chunk
-// claim, per-tile sums to the batch tile table and per-tile counters to
+// Per touched row: new length, mode (values in place / rewrite), chunk
+// claim and old entries to move, per-tile sums to the batch tile table and per-tile counters to
 // pplan.  The last workgroup to finish then decides the batch: it makes the
 // claim / moved / insert / delete columns exclusive offsets, totals every
 // counter, rejects the batch (no-op) on a bad index or when the claims do
@@ -366,7 +428,8 @@ __global__ __launch_bounds__(TB) void k_upd_plan(DevState *ds, unsigned long lon
                                                  const uint32_t *__restrict__ trow, const uint32_t *__restrict__ tins,
                                                  const uint32_t *__restrict__ tdel, const int32_t *__restrict__ row_len,
                                                  const int32_t *__restrict__ row_cap, uint32_t *__restrict__ tnewl,
-                                                 uint32_t *__restrict__ tmode, uint32_t *__restrict__ tt,
+                                                 uint32_t *__restrict__ tmode, uint32_t *__restrict__ tmoved,
+                                                 uint32_t *__restrict__ tt,
                                                  uint32_t *__restrict__ tp, long long *__restrict__ pplan,
                                                  unsigned int *fin)
 {
@@ -377,14 +440,33 @@ __global__ __launch_bounds__(TB) void k_upd_plan(DevState *ds, unsigned long lon
     const uint32_t nt = ds->ntouched;
     uint32_t csum = 0, msum = 0;
     long long v[NPLAN] = {0, 0, 0, 0, 0};
+    // loads in two phases (row ids and counts, then row lengths), so each
+    // phase is one memory round trip for all eight rows
+    uint32_t rr[IPT], ti[IPT], td[IPT], rl[IPT], rc[IPT];
+#pragma unroll
+    for (int j = 0; j < IPT; ++j) {
+        rr[j] = ti[j] = td[j] = 0;
+        if (t0 + j < nt) {
+            rr[j] = trow[t0 + j];
+            ti[j] = tins[t0 + j];
+            td[j] = tdel[t0 + j];
+        }
+    }
+#pragma unroll
+    for (int j = 0; j < IPT; ++j) {
+        rl[j] = rc[j] = 0;
+        if (t0 + j < nt) {
+            rl[j] = (uint32_t)row_len[rr[j]];
+            rc[j] = (uint32_t)row_cap[rr[j]];
+        }
+    }
 #pragma unroll
     for (int j = 0; j < IPT; ++j) {
         const int64_t t = t0 + j;
         if (t >= nt) continue;
-        const uint32_t r = trow[t];
-        const uint32_t L = (uint32_t)row_len[r];
-        const uint32_t cap = (uint32_t)row_cap[r];
-        const uint32_t ni = tins[t], nd = tdel[t];
+        const uint32_t L = rl[j];
+        const uint32_t cap = rc[j];
+        const uint32_t ni = ti[j], nd = td[j];
         const uint32_t nl = L + ni - nd;
         const uint32_t mode = (ni | nd) ? M_REWRITE : M_VALUES;
         if (mode == M_REWRITE) {
@@ -400,6 +482,7 @@ __global__ __launch_bounds__(TB) void k_upd_plan(DevState *ds, unsigned long lon
         }
         tnewl[t] = nl;
         tmode[t] = mode;
+        tmoved[t] = mode == M_REWRITE ? L : 0u;
     }
     csum = block_sum_u32(csum, lds4);
     msum = block_sum_u32(msum, lds4);
@@ -420,11 +503,12 @@ __global__ __launch_bounds__(TB) void k_upd_plan(DevState *ds, unsigned long lon
     if (!last_block(&fin[FC_PLAN], &s_flag)) return;
 
     const int64_t ntiles = gridDim.x;
-    const uint32_t claim_total = block_scan_column(tt, ntiles, TT_WORDS, TT_C, TT_CX, lds4);
-    const uint32_t moved_total = block_scan_column(tt, ntiles, TT_WORDS, TT_M, TT_MX, lds4);
-    const uint32_t ins_total = block_scan_column(tp, ntiles, TP_WORDS, TP_I, TP_IX, lds4);
-    const uint32_t del_total = block_scan_column(tp, ntiles, TP_WORDS, TP_D, TP_DX, lds4);
+    uint32_t *const tabs[4] = {tt, tt, tp, tp};
+    uint32_t ctot[4];
+    block_scan_columns<4>(tabs, {TT_C, TT_M, TP_I, TP_D}, {TT_CX, TT_MX, TP_IX, TP_DX}, ntiles, lds4, ctot);
+    const uint32_t claim_total = ctot[0], moved_total = ctot[1], ins_total = ctot[2], del_total = ctot[3];
     long long acc[NPLAN + 1] = {0, 0, 0, 0, 0, 0};
+#pragma unroll 4
     for (int64_t b = threadIdx.x; b < ntiles; b += TB) {
 #pragma unroll
         for (int i = 0; i < NPLAN; ++i) acc[i] += pplan[b * NPLAN + i];
@@ -458,12 +542,8 @@ __global__ __launch_bounds__(TB) void k_upd_plan(DevState *ds, unsigned long lon
         const unsigned bad = ds->ubad;
         ds->ubad = 0;  // the next batch's pack starts from a clear flag
         ds->ubad_last = bad;
-        if (bad) {
+        if (bad || base + need > pool_cap) {
             ds->go = 0;
-            ds->status |= ST_BAD_INDEX;
-        } else if (base + need > pool_cap) {
-            ds->go = 0;
-            ds->status |= ST_POOL_EXHAUSTED;
         } else {
             ds->go = 1;
             ds->bump = base + need;
@@ -472,15 +552,15 @@ __global__ __launch_bounds__(TB) void k_upd_plan(DevState *ds, unsigned long lon
 }
 
 // Exclusive offsets per entry: upre[2u], upre[2u+1] = inserts / deletes
-// before coordinate u (u <= n; entry nuniq holds the totals) and tpre[2t],
-// tpre[2t+1] = chunk slots claimed / old entries moved before touched row t
-// (t <= n), from the in-tile prefixes plus the tile offsets.
-__global__ __launch_bounds__(TB) void k_upd_offsets(int64_t n, const DevState *__restrict__ ds,
+// before coordinate u (u <= nuniq; entry nuniq holds the totals) and
+// tpre[2t], tpre[2t+1] = chunk slots claimed / old entries moved before
+// touched row t (t <= ntouched), from the in-tile prefixes plus the tile
+// offsets.
+__global__ __launch_bounds__(TB) void k_upd_offsets(const DevState *__restrict__ ds,
                                                     const uint32_t *__restrict__ ucode,
-                                                    const uint32_t *__restrict__ trow,
                                                     const uint32_t *__restrict__ tmode,
                                                     const uint32_t *__restrict__ tnewl,
-                                                    const int32_t *__restrict__ row_len,
+                                                    const uint32_t *__restrict__ tmoved,
                                                     const uint32_t *__restrict__ tt, const uint32_t *__restrict__ tp,
                                                     uint32_t *__restrict__ upre, uint32_t *__restrict__ tpre)
 {
@@ -489,18 +569,27 @@ __global__ __launch_bounds__(TB) void k_upd_offsets(int64_t n, const DevState *_
     const int64_t e0 = tile * TILE + (int64_t)threadIdx.x * IPT;
     const uint32_t nu = ds->nuniq, nt = ds->ntouched;
     uint32_t ins[IPT], del[IPT], cl[IPT], mv[IPT];
-    uint32_t si = 0, sd = 0, sc = 0, sm = 0;
+    uint32_t code[IPT], md[IPT], nl[IPT];
+    // every load first (one memory round trip), then the counts
 #pragma unroll
     for (int j = 0; j < IPT; ++j) {
         const int64_t e = e0 + j;
-        const uint32_t code = e < nu ? ucode[e] : (uint32_t)C_NOP;
-        ins[j] = code == C_INS;
-        del[j] = code == C_DEL;
-        cl[j] = mv[j] = 0;
-        if (e < nt && tmode[e] == M_REWRITE) {
-            cl[j] = chunk_round(tnewl[e]);
-            mv[j] = (uint32_t)row_len[trow[e]];
+        code[j] = C_NOP;
+        md[j] = M_VALUES;
+        nl[j] = mv[j] = 0;
+        if (e < nu) code[j] = ucode[e];
+        if (e < nt) {
+            md[j] = tmode[e];
+            nl[j] = tnewl[e];
+            mv[j] = tmoved[e];
         }
+    }
+    uint32_t si = 0, sd = 0, sc = 0, sm = 0;
+#pragma unroll
+    for (int j = 0; j < IPT; ++j) {
+        ins[j] = code[j] == C_INS;
+        del[j] = code[j] == C_DEL;
+        cl[j] = md[j] == M_REWRITE ? chunk_round(nl[j]) : 0u;
         si += ins[j];
         sd += del[j];
         sc += cl[j];
@@ -514,9 +603,11 @@ __global__ __launch_bounds__(TB) void k_upd_offsets(int64_t n, const DevState *_
 #pragma unroll
     for (int j = 0; j < IPT; ++j) {
         const int64_t e = e0 + j;
-        if (e <= n) {
+        if (e <= nu) {
             upre[2 * e] = si;
             upre[2 * e + 1] = sd;
+        }
+        if (e <= nt) {
             tpre[2 * e] = sc;
             tpre[2 * e + 1] = sm;
         }
@@ -531,8 +622,13 @@ __global__ __launch_bounds__(TB) void k_upd_offsets(int64_t n, const DevState *_
 // tpre[2t] = claim offset of row t (relative to bump_saved), tpre[2t+1] =
 // offset of row t's old entries in the flat move space.  Each wave owns a
 // fixed range of that space; per-row constants of a 64-row window are staged
-// in LDS and two 64-entry chunks are in flight per iteration.
+// in LDS and two 64-entry chunks are in flight per iteration.  An old entry
+// finds its column among its row's update records by binary search; when the
+// records of the rows the wave touches in a window number at most MV_REC,
+// their columns are staged in LDS first, so the search makes no memory round
+// trips.
 constexpr int MV_ILP = 2;
+constexpr uint32_t MV_REC = 1024;
 __global__ __launch_bounds__(TB) void k_upd_move(const DevState *__restrict__ ds, const uint32_t *__restrict__ tpre,
                                                  const uint32_t *__restrict__ trow,
                                                  const uint32_t *__restrict__ row_start,
@@ -547,6 +643,7 @@ __global__ __launch_bounds__(TB) void k_upd_move(const DevState *__restrict__ ds
     __shared__ uint32_t s_dst[TB / 64][64];
     __shared__ uint32_t s_ib[TB / 64][64];
     __shared__ uint32_t s_db[TB / 64][64];
+    __shared__ int32_t s_uc[TB / 64][MV_REC];
     if (!ds->go) return;
     const uint32_t total = ds->nrewrite_old;
     const uint32_t nt = ds->ntouched;
@@ -577,6 +674,17 @@ __global__ __launch_bounds__(TB) void k_upd_move(const DevState *__restrict__ ds
             __builtin_amdgcn_wave_barrier();
             uint32_t wend = s_off[w][64];
             if (wend > e1) wend = e1;
+            // records of the rows holding entries [e, wend): [r0, r1)
+            uint32_t r0 = 0, r1 = 0;
+            if (e < wend) {
+                r0 = s_ub[w][find_seg64(s_off[w], e)];
+                r1 = s_ub[w][find_seg64(s_off[w], wend - 1) + 1];
+            }
+            const bool staged = r1 - r0 <= MV_REC;
+            if (staged) {
+                for (uint32_t i = lane; i < r1 - r0; i += 64) s_uc[w][i] = ucol[r0 + i];
+                __builtin_amdgcn_wave_barrier();
+            }
             for (uint32_t b = e; b < wend; b += 64 * MV_ILP) {
                 uint32_t jj[MV_ILP], k[MV_ILP];
                 int32_t c[MV_ILP];
@@ -600,12 +708,22 @@ __global__ __launch_bounds__(TB) void k_upd_move(const DevState *__restrict__ ds
                     if (jj[q] == 64) continue;
                     const uint32_t ub = s_ub[w][jj[q]], ue = s_ub[w][jj[q] + 1];
                     uint32_t lo = ub, hi = ue;
-                    while (lo < hi) {
-                        uint32_t mid = (lo + hi) >> 1;
-                        if (ucol[mid] < c[q]) lo = mid + 1; else hi = mid;
+                    bool match;
+                    if (staged) {
+                        while (lo < hi) {
+                            uint32_t mid = (lo + hi) >> 1;
+                            if (s_uc[w][mid - r0] < c[q]) lo = mid + 1; else hi = mid;
+                        }
+                        match = lo < ue && s_uc[w][lo - r0] == c[q];
+                    } else {
+                        while (lo < hi) {
+                            uint32_t mid = (lo + hi) >> 1;
+                            if (ucol[mid] < c[q]) lo = mid + 1; else hi = mid;
+                        }
+                        match = lo < ue && ucol[lo] == c[q];
                     }
                     float vv = v[q];
-                    if (lo < ue && ucol[lo] == c[q]) {
+                    if (match) {
                         if (ucode[lo] == C_DEL) continue;
                         vv = uval[lo];
                     }
@@ -623,8 +741,10 @@ __global__ __launch_bounds__(TB) void k_upd_move(const DevState *__restrict__ ds
     }
 }
 
-// Inserted entries, in-place value updates, row-table commit.
-__global__ __launch_bounds__(TB) void k_upd_write(DevState *ds, const uint32_t *__restrict__ ut,
+// Inserted entries, in-place value updates, row-table commit.  A rejected
+// batch writes nothing; with `report` (a caller that does not read the
+// decision back) its cause becomes a sticky status bit.
+__global__ __launch_bounds__(TB) void k_upd_write(DevState *ds, uint32_t report, const uint32_t *__restrict__ ut,
                                                   const uint32_t *__restrict__ trow, const uint32_t *__restrict__ tubeg,
                                                   const uint32_t *__restrict__ tpre,
                                                   const uint32_t *__restrict__ tnewl,
@@ -635,8 +755,11 @@ __global__ __launch_bounds__(TB) void k_upd_write(DevState *ds, const uint32_t *
                                                   float *__restrict__ pval, uint32_t *__restrict__ row_start,
                                                   int32_t *__restrict__ row_len, int32_t *__restrict__ row_cap)
 {
-    if (!ds->go) return;
     int64_t u = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (!ds->go) {
+        if (report && u == 0) ds->status |= ds->ubad_last ? ST_BAD_INDEX : ST_POOL_EXHAUSTED;
+        return;
+    }
     const uint32_t nu = ds->nuniq;
     if (u == 0) {
         ds->nnz += ds->d_nnz;
@@ -674,8 +797,8 @@ struct UpdBufs {
     uint32_t *v0, *v1, *hist, *part;
     int32_t *ucol;
     float *usum, *uval;
-    uint32_t *ut, *upos, *ucode, *upre;
-    uint32_t *trow, *tubeg, *tins, *tdel, *tnewl, *tmode, *tpre;
+    uint32_t *ut, *urow, *upos, *ucode, *upre;
+    uint32_t *trow, *tubeg, *tins, *tdel, *tnewl, *tmode, *tmoved, *tpre;
     uint32_t *tt, *tp, *fin;
     long long *pplan;
 };
@@ -697,6 +820,7 @@ dcsr_status upd_bufs(dcsr_handle h, int64_t n, UpdBufs &b)
     DCSR_TRY(ws_get(h, WS_UPOS, n4, (void **)&b.upos));
     DCSR_TRY(ws_get(h, WS_UVAL, n4, (void **)&b.uval));
     DCSR_TRY(ws_get(h, WS_UT, n4, (void **)&b.ut));
+    DCSR_TRY(ws_get(h, WS_UROW, n4, (void **)&b.urow));
     DCSR_TRY(ws_get(h, WS_UCODE, n4, (void **)&b.ucode));
     DCSR_TRY(ws_get(h, WS_UPRE, n8, (void **)&b.upre));
     DCSR_TRY(ws_get(h, WS_TROW, n4, (void **)&b.trow));
@@ -705,6 +829,7 @@ dcsr_status upd_bufs(dcsr_handle h, int64_t n, UpdBufs &b)
     DCSR_TRY(ws_get(h, WS_TDEL, n4, (void **)&b.tdel));
     DCSR_TRY(ws_get(h, WS_TNEWL, n4, (void **)&b.tnewl));
     DCSR_TRY(ws_get(h, WS_TMODE, n4, (void **)&b.tmode));
+    DCSR_TRY(ws_get(h, WS_TMOVED, n4, (void **)&b.tmoved));
     DCSR_TRY(ws_get(h, WS_TPAIRPRE, n8, (void **)&b.tpre));
     DCSR_TRY(ws_get(h, WS_TMP2, n4, (void **)&b.usum));
     DCSR_TRY(ws_get(h, WS_TDEST, nt * TT_WORDS * 4, (void **)&b.tt));
@@ -718,33 +843,71 @@ dcsr_status upd_bufs(dcsr_handle h, int64_t n, UpdBufs &b)
 dcsr_status plan_and_check(dcsr_handle h, UpdBufs &b, int64_t n, hipStream_t s)
 {
     k_upd_plan<<<ptiles(n), TB, 0, s>>>(h->ds, (unsigned long long)h->pool_cap, b.trow, b.tins, b.tdel, h->row_len,
-                                        h->row_cap, b.tnewl, b.tmode, b.tt, b.tp, b.pplan, b.fin);
+                                        h->row_cap, b.tnewl, b.tmode, b.tmoved, b.tt, b.tp, b.pplan, b.fin);
     return hipGetLastError() == hipSuccess ? DCSR_OK : DCSR_ERR_HIP;
 }
 
-// Pack, sort, heads, probe, plan (+ decision), offsets: the launches shared
-// by the synchronous and the NOSYNC paths (the matrix is only read).
-dcsr_status prepare_launch(dcsr_handle h, const int32_t *rows, const int32_t *cols, const float *vals, int64_t n,
-                           UpdBufs &b, hipStream_t s)
+// Sort, heads, probe, plan (+ decision), offsets of a packed batch: the
+// launches shared by every update path (the matrix is only read).  They
+// read and write workspace only, never the caller's triples.
+dcsr_status sort_plan_launch(dcsr_handle h, int64_t n, UpdBufs &b, hipStream_t s)
 {
     const int nbits = h->rbits + h->cbits + 1;
-    k_upd_pack<<<grid_for(n, TB), TB, 0, s>>>(rows, cols, vals, n, h->rows, h->cols, h->cbits, b.k0, b.v0, h->ds);
     int which = 0;
     DCSR_HIP(radix_sort_pairs(b.k0, b.v0, b.k1, b.v1, n, nbits, b.hist, b.part, &which, s));
     const uint64_t *key = which ? b.k1 : b.k0;
     const uint32_t *val = which ? b.v1 : b.v0;
     const unsigned ht = (unsigned)((n + TILE - 1) / TILE);
     k_upd_heads_count<<<ht, TB, 0, s>>>(key, n, h->cbits, b.tt, b.fin, h->ds);
-    k_upd_heads_emit<<<ht, TB, 0, s>>>(key, val, n, h->cbits, b.tt, h->ds, b.ucol, b.usum, b.ut, b.trow, b.tubeg,
-                                       b.tins, b.tdel);
+    k_upd_heads_emit<<<ht, TB, 0, s>>>(key, val, n, h->cbits, b.tt, h->ds, b.ucol, b.usum, b.ut, b.urow, b.trow,
+                                       b.tubeg, b.tins, b.tdel);
     k_upd_probe<<<ptiles(n), TB, 0, s>>>(h->ds, h->row_start, h->row_len, h->pool_col, h->pool_val, b.ucol, b.usum,
-                                         b.ut, b.trow, b.uval, b.upos, b.ucode, b.tins, b.tdel, b.tp);
+                                         b.ut, b.urow, b.uval, b.upos, b.ucode, b.tins, b.tdel, b.tp);
     DCSR_HIP(hipGetLastError());
     DCSR_TRY(plan_and_check(h, b, n, s));
-    k_upd_offsets<<<ptiles(n), TB, 0, s>>>(n, h->ds, b.ucode, b.trow, b.tmode, b.tnewl, h->row_len, b.tt, b.tp, b.upre,
-                                           b.tpre);
+    k_upd_offsets<<<ptiles(n), TB, 0, s>>>(h->ds, b.ucode, b.tmode, b.tnewl, b.tmoved, b.tt, b.tp, b.upre, b.tpre);
     DCSR_HIP(hipGetLastError());
     return DCSR_OK;
+}
+
+// Pack (validate + key) the caller's triples, then sort .. offsets.
+dcsr_status prepare_launch(dcsr_handle h, const int32_t *rows, const int32_t *cols, const float *vals, int64_t n,
+                           UpdBufs &b, hipStream_t s)
+{
+    k_upd_pack<<<grid_for(n, TB), TB, 0, s>>>(rows, cols, vals, n, h->rows, h->cols, h->cbits, b.k0, b.v0, h->ds);
+    DCSR_HIP(hipGetLastError());
+    return sort_plan_launch(h, n, b, s);
+}
+
+// Move + write launches.  move_bound bounds the old entries to move: the
+// move kernel strides over them with at most MV_BLOCKS workgroups (waves
+// past the device's count exit at once).
+constexpr int64_t MV_BLOCKS = 2048;
+dcsr_status commit_launch(dcsr_handle h, int64_t n, int64_t move_bound, bool report, hipStream_t s)
+{
+    UpdBufs b;
+    DCSR_TRY(upd_bufs(h, n, b));
+    const uint32_t per_wave = 256;
+    const int64_t waves = (move_bound + per_wave - 1) / per_wave;
+    int64_t blocks = (waves + 3) / 4;
+    if (blocks > MV_BLOCKS) blocks = MV_BLOCKS;
+    if (blocks < 1) blocks = 1;
+    k_upd_move<<<(unsigned)blocks, TB, 0, s>>>(h->ds, b.tpre, b.trow, h->row_start, b.tubeg, b.ucol, b.uval, b.ucode,
+                                               b.upre, h->pool_col, h->pool_val, per_wave);
+    k_upd_write<<<grid_for(n, TB), TB, 0, s>>>(h->ds, report ? 1u : 0u, b.ut, b.trow, b.tubeg, b.tpre, b.tnewl, b.tmode, b.ucol, b.uval,
+                                                b.upos, b.ucode, b.upre, h->pool_col, h->pool_val, h->row_start,
+                                                h->row_len, h->row_cap);
+    DCSR_HIP(hipGetLastError());
+    h->flat_dirty = true;
+    return DCSR_OK;
+}
+
+// Sort through commit of a packed batch of n triples (the commit kernels
+// skip themselves when the plan rejected the batch; `report`: see k_upd_write).
+dcsr_status pipeline_launch(dcsr_handle h, int64_t n, UpdBufs &b, bool report, hipStream_t s)
+{
+    DCSR_TRY(sort_plan_launch(h, n, b, s));
+    return commit_launch(h, n, h->pool_cap, report, s);
 }
 
 }  // namespace
@@ -800,60 +963,116 @@ dcsr_status update_replan(dcsr_handle h, int64_t n, hipStream_t s, bool *go)
     return DCSR_OK;
 }
 
+namespace {
+// Per-call counters of a committed batch (host state read after prepare or
+// after commit: both carry them).
+void note_batch(dcsr_handle h, const DevState &st)
+{
+    h->last_ins = st.n_ins;
+    h->last_add = st.n_add;
+    h->last_del = st.n_del;
+    h->last_touched = st.ntouched;
+    if (st.hub_touch) h->hubs_dirty = true;
+}
+
+// The claims of a batch of n triples are at most: every touched row (at
+// most min(n, rows)) rewritten with all of its live entries, every triple
+// inserted, each chunk rounded up.
+int64_t claims_bound(dcsr_handle h, int64_t n)
+{
+    const int64_t touched = n < h->rows ? n : h->rows;
+    return live_bound(h) + n + (int64_t)(DCSR_CHUNK_ALIGN - 1u) * touched;
+}
+
+// When the claim bound fits the free pool (from the host's bound of the
+// bump pointer), the device's pool check cannot fail.
+bool claims_fit(dcsr_handle h, int64_t n) { return bump_bound(h) + claims_bound(h, n) <= h->pool_cap; }
+}  // namespace
+
 // Phase 2 (writes the matrix): moves, inserts, in-place values, row-table
 // commit.  Launches only; the host mirrors are set from the prepare check.
 dcsr_status update_commit(dcsr_handle h, int64_t n, hipStream_t s, bool nosync)
 {
-    UpdBufs b;
-    DCSR_TRY(upd_bufs(h, n, b));
-    {
-        const uint32_t per_wave = 256;
-        int64_t bound = nosync ? h->pool_cap : (int64_t)h->hs->nrewrite_old;
-        int64_t waves = (bound + per_wave - 1) / per_wave;
-        int64_t blocks = (waves + 3) / 4;
-        if (blocks > 65536) blocks = 65536;
-        if (blocks < 1) blocks = 1;
-        k_upd_move<<<(unsigned)blocks, TB, 0, s>>>(h->ds, b.tpre, b.trow, h->row_start, b.tubeg, b.ucol, b.uval,
-                                                   b.ucode, b.upre, h->pool_col, h->pool_val, per_wave);
-    }
-    k_upd_write<<<grid_for(n, TB), TB, 0, s>>>(h->ds, b.ut, b.trow, b.tubeg, b.tpre, b.tnewl, b.tmode, b.ucol, b.uval,
-                                                b.upos, b.ucode, b.upre, h->pool_col, h->pool_val, h->row_start,
-                                                h->row_len, h->row_cap);
-    DCSR_HIP(hipGetLastError());
-    h->flat_dirty = true;
+    DCSR_TRY(commit_launch(h, n, nosync ? h->pool_cap : (int64_t)h->hs->nrewrite_old, nosync, s));
     if (nosync) {
-        h->state_known = false;
-        h->hubs_dirty = true;
+        state_unknown(h, n, claims_bound(h, n));
         return DCSR_OK;
     }
     const DevState &st = *h->hs;
     h->nnz_h = st.nnz + st.d_nnz;
     h->bump_h = (int64_t)st.bump;
     h->garbage_h = st.garbage + st.d_garbage;
-    h->last_ins = st.n_ins;
-    h->last_add = st.n_add;
-    h->last_del = st.n_del;
-    h->last_touched = st.ntouched;
-    if (st.hub_touch) h->hubs_dirty = true;
-    h->state_known = true;
+    note_batch(h, st);
+    state_exact(h);
     return DCSR_OK;
 }
 
-static dcsr_status update_nosync(dcsr_handle h, const int32_t *rows, const int32_t *cols, const float *vals, int64_t n,
-                                 hipStream_t s)
+// NOSYNC: no state read-back.  The host keeps upper bounds of the live
+// entries and the bump pointer (tightened from asynchronous state snapshots
+// once they land).  When they cannot rule out pool exhaustion and growth is
+// allowed, the pool is first compacted in stream order (no read-back) or,
+// when even a compacted pool might not hold the claims, the state is read
+// and the pool grown: the batch is not lost to exhaustion.
+// Synchronous update: the batch's status is known on return.
+static dcsr_status update_sync(dcsr_handle h, const int32_t *rows, const int32_t *cols, const float *vals, int64_t n,
+                               uint32_t flags, hipStream_t s)
 {
-    if (!h->canonical) {
-        bool go;
-        int64_t need;
-        return update_prepare(h, rows, cols, vals, n, s, &go, &need) != DCSR_OK
-                   ? DCSR_ERR_BAD_INDEX
-                   : (go ? update_commit(h, n, s, false) : DCSR_ERR_POOL_EXHAUSTED);
+    bool go = false;
+    int64_t need = 0;
+    if (n > 0 && h->canonical && claims_fit(h, n)) {
+        // prepare and commit back to back, one state read at the end (it
+        // then holds the committed counters: write folds the deltas in)
+        UpdBufs b;
+        DCSR_TRY(upd_bufs(h, n, b));
+        k_upd_pack<<<grid_for(n, TB), TB, 0, s>>>(rows, cols, vals, n, h->rows, h->cols, h->cbits, b.k0, b.v0, h->ds);
+        DCSR_HIP(hipGetLastError());
+        DCSR_TRY(pipeline_launch(h, n, b, false, s));
+        DCSR_TRY(pull_state(h, s));
+        if (h->hs->ubad_last) return DCSR_ERR_BAD_INDEX;
+        if (h->hs->go) {
+            note_batch(h, *h->hs);
+            return DCSR_OK;
+        }
+        // not committed (the bound rules this out): grow and re-plan below
+        need = (int64_t)h->hs->need;
+    } else {
+        DCSR_TRY(update_prepare(h, rows, cols, vals, n, s, &go, &need));
+        if (n == 0) return DCSR_OK;
     }
+    if (!go) {
+        if (!h->cfg.allow_grow || (flags & DCSR_UPDATE_NOGROW)) return DCSR_ERR_POOL_EXHAUSTED;
+        DCSR_TRY(pool_reserve(h, need + need / 8 + 64, s, nullptr));
+        DCSR_TRY(update_replan(h, n, s, &go));
+        if (!go) return DCSR_ERR_POOL_EXHAUSTED;
+    }
+    return update_commit(h, n, s, false);
+}
+
+static dcsr_status update_nosync(dcsr_handle h, const int32_t *rows, const int32_t *cols, const float *vals, int64_t n,
+                                 uint32_t flags, hipStream_t s)
+{
+    // the first update of a matrix loaded with duplicates canonicalizes it:
+    // done synchronously, once
+    if (!h->canonical) return update_sync(h, rows, cols, vals, n, flags, s);
     if (n == 0) return DCSR_OK;
+    refresh_bounds(h);
+    if (!claims_fit(h, n) && h->cfg.allow_grow && !(flags & DCSR_UPDATE_NOGROW)) {
+        const int64_t need = claims_bound(h, n);
+        if (live_bound(h) + need <= h->pool_cap) {
+            DCSR_TRY(compact_in_order(h, s));
+        } else {
+            if (!h->state_known) DCSR_TRY(pull_state(h, s));
+            if (!claims_fit(h, n)) DCSR_TRY(pool_reserve(h, need + need / 8 + 64, s, nullptr));
+        }
+    }
+    const int64_t claimed = claims_bound(h, n);
     UpdBufs b;
     DCSR_TRY(upd_bufs(h, n, b));
-    DCSR_TRY(prepare_launch(h, rows, cols, vals, n, b, s));
-    return update_commit(h, n, s, true);
+    k_upd_pack<<<grid_for(n, TB), TB, 0, s>>>(rows, cols, vals, n, h->rows, h->cols, h->cbits, b.k0, b.v0, h->ds);
+    DCSR_HIP(hipGetLastError());
+    DCSR_TRY(pipeline_launch(h, n, b, true, s));
+    state_unknown(h, n, claimed);
+    return snapshot_state(h, s);
 }
 
 dcsr_status dcsr_update(dcsr_handle h, const int32_t *rows, const int32_t *cols, const float *vals, int64_t n,
@@ -863,16 +1082,6 @@ dcsr_status dcsr_update(dcsr_handle h, const int32_t *rows, const int32_t *cols,
     if (n >= (int64_t)0x7ffffff0ll) return DCSR_ERR_INVALID;
     hipStream_t s = (hipStream_t)stream;
     h->last_stream = s;
-    if (flags & DCSR_UPDATE_NOSYNC) return update_nosync(h, rows, cols, vals, n, s);
-    bool go = false;
-    int64_t need = 0;
-    DCSR_TRY(update_prepare(h, rows, cols, vals, n, s, &go, &need));
-    if (n == 0) return DCSR_OK;
-    if (!go) {
-        if (!h->cfg.allow_grow || (flags & DCSR_UPDATE_NOGROW)) return DCSR_ERR_POOL_EXHAUSTED;
-        DCSR_TRY(pool_reserve(h, need + need / 8 + 64, s, nullptr));
-        DCSR_TRY(update_replan(h, n, s, &go));
-        if (!go) return DCSR_ERR_POOL_EXHAUSTED;
-    }
-    return update_commit(h, n, s, false);
+    if (flags & DCSR_UPDATE_NOSYNC) return update_nosync(h, rows, cols, vals, n, flags, s);
+    return update_sync(h, rows, cols, vals, n, flags, s);
 }

@@ -51,10 +51,15 @@ typedef enum dcsr_status {
 } dcsr_status;
 
 /* Flags for dcsr_update. */
-#define DCSR_UPDATE_NOSYNC   0x1u /* do not read the allocation check back; on
-                                     pool exhaustion the batch becomes a no-op
-                                     and DCSR_ERR_POOL_EXHAUSTED is reported by
-                                     the next dcsr_get_status */
+#define DCSR_UPDATE_NOSYNC   0x1u /* asynchronous: no read-back of the batch's
+                                     outcome.  Pool room comes from host-side
+                                     upper bounds (compaction in stream order,
+                                     or one read-back and growth, when they
+                                     cannot rule out exhaustion).  A batch
+                                     with a bad index, or one that does not
+                                     fit under NOGROW / allow_grow = 0,
+                                     becomes a no-op and its error is
+                                     reported by the next dcsr_get_status */
 #define DCSR_UPDATE_NOGROW   0x2u /* never grow/compact automatically */
 /* Flags for dcsr_export_csr. */
 #define DCSR_EXPORT_CANONICAL 0x1u /* merge duplicate columns and drop exact

@@ -117,6 +117,50 @@ def test_random_update_chain(torch, R, C, n, seed):
     m.close()
 
 
+@pytest.mark.parametrize("pool_factor", [20.0, 1.5])
+def test_nosync_update_chain_and_spmv(torch, pool_factor):
+    """NOSYNC updates (no state read-back) back to back, each followed by an
+    SpMV sized from the host's bound on the live entries: results match the
+    oracle and a handle updated synchronously.  Every batch touches nearly
+    every row, so each rewrites the whole matrix into fresh chunks: the large
+    pool holds all six rewrites, the small one makes the NOSYNC path compact
+    in stream order and grow."""
+    from dcsr_amd import DcsrMatrix, UPDATE_NOSYNC
+    R, C, n = 3000, 2500, 20000
+    rng = np.random.default_rng(21)
+    r, c, v = random_coo(rng, R, C, 40000, 0.0, 0.0, "sym")
+    say(f"nosync chain pool_factor={pool_factor}")
+    sync = DcsrMatrix(R, C, pool_factor=pool_factor)
+    m = DcsrMatrix(R, C, pool_factor=pool_factor)
+    for h in (sync, m):
+        h.LoadMatrix(r, c, v)
+    A = O.coo_to_csr(R, C, r, c, v)
+    x = rng.uniform(-1, 1, C).astype(np.float32)
+    xd = torch.from_numpy(x).cuda()
+    for k in range(6):
+        br, bc, bv = random_coo(rng, R, C, n, 0.2, 0.02, "small_int" if k % 2 else "sym")
+        # cancel some stored entries exactly (deletions)
+        rows_of = np.repeat(np.arange(R), np.diff(A.off))
+        pick = rng.choice(A.nnz, 500, replace=False)
+        br[:500], bc[:500], bv[:500] = rows_of[pick], A.col[pick], -A.val[pick]
+        say(f"  update {k}")
+        sync.UpdateMatrix(br, bc, bv)
+        m.UpdateMatrix(br, bc, bv, flags=UPDATE_NOSYNC)
+        A = O.update(A, br, bc, bv)
+        y = m.spmv(xd)  # no state read-back in between
+        if k % 2 == 0:
+            continue  # two NOSYNC updates back to back before the next check
+        torch.cuda.synchronize()
+        fp64_spmv_check(A, x, y.cpu().numpy())
+        assert m.status() == 0
+        got = to_host_csr(m)
+        assert_same_structure(got, A)
+        assert_same_structure(got, to_host_csr(sync))
+        assert m.info()["nnz"] == A.nnz
+    m.close()
+    sync.close()
+
+
 def test_powerlaw_with_hub_rows(torch):
     """Rows longer than the hub threshold go through the split SpMV path."""
     from dcsr_amd import DcsrMatrix
