@@ -12,6 +12,10 @@ size_t scan_part_words(int64_t n, int nc);
 hipError_t scan_exclusive_u32(const uint32_t *in, uint32_t *out, int64_t n, int nc, uint32_t *part,
                               uint32_t *total, hipStream_t s);
 
+// Entries per tile of the radix sort's count table (digit-major: the count
+// of digit d in tile t sits at hist[d * tiles + t]).
+constexpr int RADIX_TILE = 2048;
+
 // Stable LSD radix sort of (key, value) pairs on key bits [0, nbits).
 // Ping-pongs between (k0, v0) and (k1, v1); returns in *which the buffer
 // index (0 or 1) holding the result.  `hist` is scratch of
@@ -19,5 +23,7 @@ hipError_t scan_exclusive_u32(const uint32_t *in, uint32_t *out, int64_t n, int 
 // words (zero-filled when first allocated, as for scan_exclusive_u32).
 size_t radix_hist_words(int64_t n);
 size_t radix_part_words(int64_t n);
+// counted: the caller already wrote the first pass's (bits 0..7) counts
+// into hist, per RADIX_TILE tile of (k0, v0).
 hipError_t radix_sort_pairs(uint64_t *k0, uint32_t *v0, uint64_t *k1, uint32_t *v1, int64_t n, int nbits,
-                            uint32_t *hist, uint32_t *part, int *which, hipStream_t s);
+                            uint32_t *hist, uint32_t *part, int *which, hipStream_t s, bool counted = false);

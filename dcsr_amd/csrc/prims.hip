@@ -118,6 +118,7 @@ constexpr int RS_WAVES = RS_THREADS / 64;
 constexpr int RS_IPT = 8;                        // items per lane
 constexpr int RS_WAVE_ITEMS = RS_IPT * 64;       // contiguous items per wave
 constexpr int RS_TILE = RS_WAVES * RS_WAVE_ITEMS; // 2048
+static_assert(RS_TILE == RADIX_TILE, "count-table tiles are shared with callers");
 
 __global__ __launch_bounds__(RS_THREADS) void k_rs_hist(const uint64_t *__restrict__ keys, int64_t n, int shift,
                                                         uint32_t *__restrict__ hist, int64_t nblocks)
@@ -305,7 +306,7 @@ size_t radix_part_words(int64_t n)
 // table (per digit up to 8192 tiles, else one single-pass scan of the whole
 // table), stable scatter.
 hipError_t radix_sort_pairs(uint64_t *k0, uint32_t *v0, uint64_t *k1, uint32_t *v1, int64_t n, int nbits,
-                            uint32_t *hist, uint32_t *part, int *which, hipStream_t s)
+                            uint32_t *hist, uint32_t *part, int *which, hipStream_t s, bool counted)
 {
     *which = 0;
     if (n <= 1) return hipSuccess;
@@ -316,7 +317,7 @@ hipError_t radix_sort_pairs(uint64_t *k0, uint32_t *v0, uint64_t *k1, uint32_t *
     uint32_t *vi = v0, *vo = v1;
     int cur = 0;
     for (int shift = 0; shift < nbits; shift += 8) {
-        k_rs_hist<<<(unsigned)nb, RS_THREADS, 0, s>>>(ki, n, shift, hist, nb);
+        if (!(counted && shift == 0)) k_rs_hist<<<(unsigned)nb, RS_THREADS, 0, s>>>(ki, n, shift, hist, nb);
         if (per_digit) {
             k_rs_scan_digits<<<256, RS_THREADS, 0, s>>>(hist, nb, dtot);
         } else {

@@ -58,22 +58,46 @@ __device__ __forceinline__ uint32_t block_excl_scan_256(uint32_t v, uint32_t *ld
     return carry + inc - v;
 }
 
-// Invalid indices become ~0 keys: with one key bit above row|col they sort
-// strictly after every valid key.
-__global__ void k_upd_pack(const int32_t *__restrict__ rows, const int32_t *__restrict__ cols,
-                           const float *__restrict__ vals, int64_t n, int64_t R, int64_t C, int cbits,
-                           uint64_t *__restrict__ key, uint32_t *__restrict__ val, DevState *ds)
+// Pack the caller's triples into sort keys (row << cbits | col) and value
+// bits, one RADIX_TILE tile per workgroup.  Invalid indices become ~0 keys:
+// with one key bit above row|col they sort strictly after every valid key,
+// and they raise the batch's flag.  With `hist`, the tile's counts of the
+// sort's first digit (key bits 0..7) go to the count table on the way, so
+// the sort skips its first count pass.
+__global__ __launch_bounds__(TB) void k_upd_pack(const int32_t *__restrict__ rows, const int32_t *__restrict__ cols,
+                                                 const float *__restrict__ vals, int64_t n, int64_t R, int64_t C,
+                                                 int cbits, uint64_t *__restrict__ key, uint32_t *__restrict__ val,
+                                                 uint32_t *__restrict__ hist, int64_t ntiles, DevState *ds)
 {
-    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    __shared__ uint32_t cnt[256];
+    static_assert(TB == 256, "one digit counter per thread");
+    if (hist) {
+        cnt[threadIdx.x] = 0;
+        __syncthreads();
+    }
+    const int64_t base = (int64_t)blockIdx.x * RADIX_TILE;
     bool bad = false;
-    if (i < n) {
-        int32_t r = rows[i], c = cols[i];
-        bad = (r < 0) | (r >= R) | (c < 0) | (c >= C);
-        key[i] = bad ? ~0ull : (((uint64_t)(uint32_t)r << cbits) | (uint32_t)c);
-        val[i] = __float_as_uint(vals[i]);
+#pragma unroll
+    for (int j = 0; j < RADIX_TILE / TB; ++j) {
+        const int64_t i = base + j * TB + threadIdx.x;
+        if (i < n) {
+            const int32_t r = rows[i], c = cols[i];
+            const bool b = (r < 0) | (r >= R) | (c < 0) | (c >= C);
+            const uint64_t k = b ? ~0ull : (((uint64_t)(uint32_t)r << cbits) | (uint32_t)c);
+            key[i] = k;
+            val[i] = __float_as_uint(vals[i]);
+            if (hist) atomicAdd(&cnt[(uint32_t)k & 255u], 1u);
+            bad |= b;
+        }
     }
     if (__ballot(bad) && lane_id() == 0) atomicOr(&ds->ubad, 1u);
+    if (hist) {
+        __syncthreads();
+        hist[(int64_t)threadIdx.x * ntiles + blockIdx.x] = cnt[threadIdx.x];
+    }
 }
+
+static inline unsigned pack_tiles(int64_t n) { return (unsigned)((n + RADIX_TILE - 1) / RADIX_TILE); }
 
 // Sum of one 32-bit counter over the 256 threads of a block (fixed order).
 __device__ __forceinline__ uint32_t block_sum_u32(uint32_t v, uint32_t *lds4)
@@ -854,7 +878,7 @@ dcsr_status sort_plan_launch(dcsr_handle h, int64_t n, UpdBufs &b, hipStream_t s
 {
     const int nbits = h->rbits + h->cbits + 1;
     int which = 0;
-    DCSR_HIP(radix_sort_pairs(b.k0, b.v0, b.k1, b.v1, n, nbits, b.hist, b.part, &which, s));
+    DCSR_HIP(radix_sort_pairs(b.k0, b.v0, b.k1, b.v1, n, nbits, b.hist, b.part, &which, s, true));
     const uint64_t *key = which ? b.k1 : b.k0;
     const uint32_t *val = which ? b.v1 : b.v0;
     const unsigned ht = (unsigned)((n + TILE - 1) / TILE);
@@ -874,7 +898,8 @@ dcsr_status sort_plan_launch(dcsr_handle h, int64_t n, UpdBufs &b, hipStream_t s
 dcsr_status prepare_launch(dcsr_handle h, const int32_t *rows, const int32_t *cols, const float *vals, int64_t n,
                            UpdBufs &b, hipStream_t s)
 {
-    k_upd_pack<<<grid_for(n, TB), TB, 0, s>>>(rows, cols, vals, n, h->rows, h->cols, h->cbits, b.k0, b.v0, h->ds);
+    k_upd_pack<<<pack_tiles(n), TB, 0, s>>>(rows, cols, vals, n, h->rows, h->cols, h->cbits, b.k0, b.v0, b.hist,
+                                            pack_tiles(n), h->ds);
     DCSR_HIP(hipGetLastError());
     return sort_plan_launch(h, n, b, s);
 }
@@ -928,7 +953,8 @@ dcsr_status update_prepare(dcsr_handle h, const int32_t *rows, const int32_t *co
             DCSR_TRY(ws_get(h, WS_KEY_A, (size_t)n * 8, (void **)&k0));
             DCSR_TRY(ws_get(h, WS_VAL_A, (size_t)n * 4, (void **)&v0));
             k_upd_validate_begin<<<1, 1, 0, s>>>(h->ds);
-            k_upd_pack<<<grid_for(n, TB), TB, 0, s>>>(rows, cols, vals, n, h->rows, h->cols, h->cbits, k0, v0, h->ds);
+            k_upd_pack<<<pack_tiles(n), TB, 0, s>>>(rows, cols, vals, n, h->rows, h->cols, h->cbits, k0, v0, nullptr,
+                                                    pack_tiles(n), h->ds);
             DCSR_HIP(hipGetLastError());
             DCSR_TRY(pull_state(h, s));
             if (h->hs->ubad) return DCSR_ERR_BAD_INDEX;
@@ -1024,7 +1050,8 @@ static dcsr_status update_sync(dcsr_handle h, const int32_t *rows, const int32_t
         // then holds the committed counters: write folds the deltas in)
         UpdBufs b;
         DCSR_TRY(upd_bufs(h, n, b));
-        k_upd_pack<<<grid_for(n, TB), TB, 0, s>>>(rows, cols, vals, n, h->rows, h->cols, h->cbits, b.k0, b.v0, h->ds);
+        k_upd_pack<<<pack_tiles(n), TB, 0, s>>>(rows, cols, vals, n, h->rows, h->cols, h->cbits, b.k0, b.v0, b.hist,
+                                                pack_tiles(n), h->ds);
         DCSR_HIP(hipGetLastError());
         DCSR_TRY(pipeline_launch(h, n, b, false, s));
         DCSR_TRY(pull_state(h, s));
@@ -1068,7 +1095,8 @@ static dcsr_status update_nosync(dcsr_handle h, const int32_t *rows, const int32
     const int64_t claimed = claims_bound(h, n);
     UpdBufs b;
     DCSR_TRY(upd_bufs(h, n, b));
-    k_upd_pack<<<grid_for(n, TB), TB, 0, s>>>(rows, cols, vals, n, h->rows, h->cols, h->cbits, b.k0, b.v0, h->ds);
+    k_upd_pack<<<pack_tiles(n), TB, 0, s>>>(rows, cols, vals, n, h->rows, h->cols, h->cbits, b.k0, b.v0, b.hist,
+                                            pack_tiles(n), h->ds);
     DCSR_HIP(hipGetLastError());
     DCSR_TRY(pipeline_launch(h, n, b, true, s));
     state_unknown(h, n, claimed);
