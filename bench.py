@@ -335,7 +335,7 @@ def bench_batched(args):
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t0
     step_ms = np.array([e0.elapsed_time(e1) for e0, e1 in ev])
-    trip = count * per
+    trip = count * per  # triples per step
     R = b.total_rows
     algo = []
     for inf in infos:
@@ -344,7 +344,7 @@ def bench_batched(args):
                     + spmv_bytes(R, b.total_cols, inf["nnz"]) + 4 * count)
     gbs = float(np.mean([a / 1e9 / (t / 1e3) for a, t in zip(algo, step_ms)]))
     b.close()
-    line(args, trip / elapsed, "triples/s", elapsed * 1e3 / K,
+    line(args, K * trip / elapsed, "triples/s", elapsed * 1e3 / K,
          {"workload": "batched-4096-update-spmv", "matrices": count, "rows_total": int(R), "nnz0": nnz0,
           "triples_per_matrix": per, "team": team, "thresh": thresh},
          step_wall_ms=elapsed * 1e3 / K,
