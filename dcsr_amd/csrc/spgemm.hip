@@ -8,6 +8,9 @@
 // 0 in A-row order then B-row order, is not exactly 0.
 //
 // Rows are bucketed by their partial-product count P_i:
+//   P_i <=  128  a 16-lane team per row, four rows per wave, the same LDS
+//                ESC as below with small per-row buffers (many rows in
+//                flight per CU: a row is a chain of dependent loads);
 //   P_i <= 1024  one wave per row, LDS ESC: the row's partials are expanded
 //                with their generation index (A-entry order, then B-row
 //                order), bitonic-sorted in LDS by (column, generation), and
@@ -24,9 +27,18 @@
 namespace {
 
 constexpr int TB = 256;
+constexpr uint32_t TEAM_MAX = 128;    // partials per row for the team kernel
+constexpr int TEAM = 16;              // lanes per row in the team kernel
+constexpr int TEAM_GB = 7;            // generation bits of the team kernel's 32-bit keys
+// the team kernel's keys are 32-bit: B's columns must stay below this
+constexpr int64_t TEAM_MAX_COLS = (1ll << (32 - TEAM_GB)) - 1;
+constexpr int WAVE_GB = 10;           // generation bits of the wave kernel's keys
+// the wave kernel's keys are 32-bit up to this many columns of B, else 64-bit
+constexpr int64_t WAVE_MAX_COLS32 = (1ll << (32 - WAVE_GB)) - 1;
 constexpr uint32_t WAVE_MAX = 1024;   // partials per row for the wave kernel
 constexpr uint32_t BLOCK_MAX = 4096;  // partials per row for the workgroup kernel
-constexpr int NCLS = 3;
+// row classes (list index): wave, workgroup, hub, team
+enum { CL_WAVE = 0, CL_BLOCK = 1, CL_HUB = 2, CL_TEAM = 3, NCLS = 4 };
 
 struct SgState {
     unsigned long long total;  // sum of P_i (64-bit)
@@ -83,31 +95,44 @@ __global__ __launch_bounds__(TB) void k_sg_count(uint32_t R, const uint32_t *__r
     if (r < R) P[r] = s_acc[w][lane];
 }
 
-// Row classes (wave-aggregated list appends) and per-block 64-bit P sums.
+// Row classes (list appends aggregated per workgroup: one global atomic per
+// class per workgroup) and per-block 64-bit P sums.
 __global__ __launch_bounds__(TB) void k_sg_classify(uint32_t R, const uint32_t *__restrict__ P,
                                                     uint32_t *__restrict__ lists, uint32_t list_stride,
                                                     SgState *st, uint32_t *__restrict__ cnt,
-                                                    unsigned long long *__restrict__ bsum)
+                                                    unsigned long long *__restrict__ bsum, uint32_t team_ok)
 {
     __shared__ unsigned long long ws[TB / 64];
+    __shared__ uint32_t s_cnt[NCLS], s_base[NCLS];
+    if (threadIdx.x < NCLS) s_cnt[threadIdx.x] = 0;
+    __syncthreads();
     const uint32_t r = blockIdx.x * TB + threadIdx.x;
     uint32_t p = 0;
     int cls = -1;
     if (r < R) {
         p = P[r];
-        cls = p == 0 ? -1 : (p <= WAVE_MAX ? 0 : (p <= BLOCK_MAX ? 1 : 2));
+        cls = p == 0 ? -1
+                     : (p <= TEAM_MAX && team_ok ? CL_TEAM
+                                                 : (p <= WAVE_MAX ? CL_WAVE : (p <= BLOCK_MAX ? CL_BLOCK : CL_HUB)));
         if (p == 0) cnt[r] = 0;
     }
+    // slot of this row inside its workgroup's share of its class list
+    uint32_t slot = 0;
 #pragma unroll
     for (int c = 0; c < NCLS; ++c) {
         const uint64_t m = __ballot(cls == c);
         if (!m) continue;
         const int leader = __ffsll((long long)m) - 1;
         uint32_t base = 0;
-        if (lane_id() == leader) base = atomicAdd(&st->count[c], (uint32_t)__popcll(m));
+        if (lane_id() == leader) base = atomicAdd(&s_cnt[c], (uint32_t)__popcll(m));
         base = __shfl(base, leader);
-        if (cls == c) lists[(uint64_t)c * list_stride + base + (uint32_t)__popcll(m & lanemask_lt())] = r;
+        if (cls == c) slot = base + (uint32_t)__popcll(m & lanemask_lt());
     }
+    __syncthreads();
+    if (threadIdx.x < NCLS && s_cnt[threadIdx.x])
+        s_base[threadIdx.x] = atomicAdd(&st->count[threadIdx.x], s_cnt[threadIdx.x]);
+    __syncthreads();
+    if (cls >= 0) lists[(uint64_t)cls * list_stride + s_base[cls] + slot] = r;
     unsigned long long v = p;
 #pragma unroll
     for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o);
@@ -142,17 +167,31 @@ __global__ __launch_bounds__(1024) void k_sg_total(SgState *st, const unsigned l
 // Shared row routine: expand, sort, reduce.  NT threads cooperate (64 for the
 // wave kernel, 256 for the workgroup kernel); GB = generation-index bits.
 // ---------------------------------------------------------------------------
+// A team is NT threads: 16 (a quarter wave), 64 (a wave) or 256 (the
+// workgroup).  Teams inside one wave never exchange data with each other.
 template <int NT>
 __device__ __forceinline__ void team_sync()
 {
-    if (NT == 64) __builtin_amdgcn_wave_barrier();
+    if (NT <= 64) __builtin_amdgcn_wave_barrier();
     else __syncthreads();
 }
 
-// Exclusive scan of one uint32 per team thread (team = NT threads).
+// Exclusive scan of one uint32 per team thread.
 template <int NT>
 __device__ __forceinline__ uint32_t team_excl_scan(uint32_t v, uint32_t *lds_w, uint32_t *total)
 {
+    if (NT < 64) {
+        // within the team's aligned NT-lane slice of the wave
+        const int t = lane_id() % NT;
+        uint32_t inc = v;
+#pragma unroll
+        for (int d = 1; d < NT; d <<= 1) {
+            const uint32_t u = __shfl_up(inc, d, NT);
+            if (t >= d) inc += u;
+        }
+        *total = __shfl(inc, NT - 1, NT);
+        return inc - v;
+    }
     const uint32_t inc = wave_incl_scan_u32(v);
     if (NT == 64) {
         *total = wave_read_lane_u32(inc, 63);
@@ -186,13 +225,16 @@ __device__ __forceinline__ uint32_t find_seg_team(const uint32_t *s, uint32_t e)
     return lo;
 }
 
-template <int NT, int GB>
+// K: the sort key type, (column << GB) | generation.  32-bit keys halve the
+// LDS per row (more rows in flight); they need columns below 2^(32-GB) - 1
+// so that no key reaches the ~0 padding.
+template <int NT, int GB, typename K>
 __device__ void sg_row(uint32_t row, uint32_t P, uint64_t out_base, const uint32_t *__restrict__ a_start,
                        const int32_t *__restrict__ a_len, const int32_t *__restrict__ a_col,
                        const float *__restrict__ a_val, const uint32_t *__restrict__ b_start,
                        const int32_t *__restrict__ b_len, const int32_t *__restrict__ b_col,
                        const float *__restrict__ b_val, int32_t *__restrict__ tcol, float *__restrict__ tval,
-                       uint32_t *__restrict__ cnt, uint64_t *key, float *val, uint32_t *s_ex, uint32_t *s_sb,
+                       uint32_t *__restrict__ cnt, K *key, float *val, uint32_t *s_ex, uint32_t *s_sb,
                        float *s_va, uint32_t *lds_w)
 {
     const uint32_t t = threadIdx.x % NT;
@@ -222,7 +264,7 @@ __device__ void sg_row(uint32_t row, uint32_t P, uint64_t out_base, const uint32
                 const uint32_t q = find_seg_team<NT>(s_ex, pe);
                 const uint64_t bp = (uint64_t)s_sb[q] + (pe - s_ex[q]);
                 const uint32_t g = gbase + pe;
-                key[g] = ((uint64_t)(uint32_t)b_col[bp] << GB) | g;
+                key[g] = (K)(((uint64_t)(uint32_t)b_col[bp] << GB) | g);
                 val[g] = s_va[q] * b_val[bp];
             }
         }
@@ -230,8 +272,8 @@ __device__ void sg_row(uint32_t row, uint32_t P, uint64_t out_base, const uint32
         team_sync<NT>();
     }
     // 2. bitonic sort of n2 keys (padding sorts last)
-    const uint32_t n2 = next_pow2(P < 64 ? 64 : P);
-    for (uint32_t p = P + t; p < n2; p += NT) key[p] = ~0ull;
+    const uint32_t n2 = next_pow2(P < 2 * NT ? 2 * NT : P);
+    for (uint32_t p = P + t; p < n2; p += NT) key[p] = (K)~0ull;
     team_sync<NT>();
     for (uint32_t k = 2; k <= n2; k <<= 1) {
         for (uint32_t js = k >> 1; js > 0; js >>= 1) {
@@ -239,7 +281,7 @@ __device__ void sg_row(uint32_t row, uint32_t P, uint64_t out_base, const uint32
                 const uint32_t i1 = (c / js) * 2 * js + (c % js);
                 const uint32_t i2 = i1 + js;
                 const bool asc = (i1 & k) == 0;
-                const uint64_t ka = key[i1], kb = key[i2];
+                const K ka = key[i1], kb = key[i2];
                 if ((ka > kb) == asc) {
                     key[i1] = kb;
                     key[i2] = ka;
@@ -279,6 +321,9 @@ __device__ void sg_row(uint32_t row, uint32_t P, uint64_t out_base, const uint32
     team_sync<NT>();
 }
 
+// Rows of at most WAVE_MAX partials: one wave per row.  K: 32-bit sort keys
+// (half the LDS per row, more rows in flight) when B's columns allow.
+template <typename K>
 __global__ __launch_bounds__(TB) void k_sg_wave(const uint32_t *__restrict__ list, const SgState *__restrict__ st,
                                                 const uint32_t *__restrict__ P, const uint32_t *__restrict__ Poff,
                                                 const uint32_t *__restrict__ a_start, const int32_t *__restrict__ a_len,
@@ -288,17 +333,42 @@ __global__ __launch_bounds__(TB) void k_sg_wave(const uint32_t *__restrict__ lis
                                                 int32_t *__restrict__ tcol, float *__restrict__ tval,
                                                 uint32_t *__restrict__ cnt)
 {
-    __shared__ uint64_t key[TB / 64][WAVE_MAX];
+    __shared__ K key[TB / 64][WAVE_MAX];
     __shared__ float val[TB / 64][WAVE_MAX];
     __shared__ uint32_t s_ex[TB / 64][64], s_sb[TB / 64][64];
     __shared__ float s_va[TB / 64][64];
     const int w = threadIdx.x >> 6;
-    const uint32_t n = st->count[0];
+    const uint32_t n = st->count[CL_WAVE];
     const uint32_t nw = gridDim.x * (TB / 64);
     for (uint32_t i = blockIdx.x * (TB / 64) + w; i < n; i += nw) {
         const uint32_t row = list[i];
-        sg_row<64, 10>(row, P[row], Poff[row], a_start, a_len, a_col, a_val, b_start, b_len, b_col, b_val, tcol, tval,
-                       cnt, key[w], val[w], s_ex[w], s_sb[w], s_va[w], nullptr);
+        sg_row<64, WAVE_GB, K>(row, P[row], Poff[row], a_start, a_len, a_col, a_val, b_start, b_len, b_col, b_val,
+                               tcol, tval, cnt, key[w], val[w], s_ex[w], s_sb[w], s_va[w], nullptr);
+    }
+}
+
+// Rows of at most TEAM_MAX partials: 16-lane teams, four rows per wave.
+__global__ __launch_bounds__(TB) void k_sg_team(const uint32_t *__restrict__ list, const SgState *__restrict__ st,
+                                                const uint32_t *__restrict__ P, const uint32_t *__restrict__ Poff,
+                                                const uint32_t *__restrict__ a_start, const int32_t *__restrict__ a_len,
+                                                const int32_t *__restrict__ a_col, const float *__restrict__ a_val,
+                                                const uint32_t *__restrict__ b_start, const int32_t *__restrict__ b_len,
+                                                const int32_t *__restrict__ b_col, const float *__restrict__ b_val,
+                                                int32_t *__restrict__ tcol, float *__restrict__ tval,
+                                                uint32_t *__restrict__ cnt)
+{
+    constexpr int NTEAM = TB / TEAM;
+    __shared__ uint32_t key[NTEAM][TEAM_MAX];
+    __shared__ float val[NTEAM][TEAM_MAX];
+    __shared__ uint32_t s_ex[NTEAM][TEAM], s_sb[NTEAM][TEAM];
+    __shared__ float s_va[NTEAM][TEAM];
+    const int tm = threadIdx.x / TEAM;
+    const uint32_t n = st->count[CL_TEAM];
+    const uint32_t nt = gridDim.x * NTEAM;
+    for (uint32_t i = blockIdx.x * NTEAM + tm; i < n; i += nt) {
+        const uint32_t row = list[i];
+        sg_row<TEAM, TEAM_GB, uint32_t>(row, P[row], Poff[row], a_start, a_len, a_col, a_val, b_start, b_len, b_col, b_val, tcol,
+                        tval, cnt, key[tm], val[tm], s_ex[tm], s_sb[tm], s_va[tm], nullptr);
     }
 }
 
@@ -316,10 +386,10 @@ __global__ __launch_bounds__(TB) void k_sg_block(const uint32_t *__restrict__ li
     __shared__ uint32_t s_ex[TB], s_sb[TB];
     __shared__ float s_va[TB];
     __shared__ uint32_t lds_w[TB / 64];
-    const uint32_t n = st->count[1];
+    const uint32_t n = st->count[CL_BLOCK];
     for (uint32_t i = blockIdx.x; i < n; i += gridDim.x) {
         const uint32_t row = list[i];
-        sg_row<TB, 12>(row, P[row], Poff[row], a_start, a_len, a_col, a_val, b_start, b_len, b_col, b_val, tcol, tval,
+        sg_row<TB, 12, uint64_t>(row, P[row], Poff[row], a_start, a_len, a_col, a_val, b_start, b_len, b_col, b_val, tcol, tval,
                        cnt, key, val, s_ex, s_sb, s_va, lds_w);
     }
 }
@@ -485,7 +555,7 @@ dcsr_status dcsr_spgemm(dcsr_handle A, dcsr_handle B, dcsr_handle C, dcsr_stream
     k_sg_state_zero<<<1, 1, 0, s>>>(st);
     if (R > 0) {
         k_sg_count<<<grid_for(R, TB), TB, 0, s>>>(R, A->row_start, A->row_len, A->pool_col, B->row_len, P);
-        k_sg_classify<<<nbR, TB, 0, s>>>(R, P, lists, R + 1, st, cnt, bsum);
+        k_sg_classify<<<nbR, TB, 0, s>>>(R, P, lists, R + 1, st, cnt, bsum, B->cols <= TEAM_MAX_COLS ? 1u : 0u);
         k_sg_total<<<1, 1024, 0, s>>>(st, bsum, (int)nbR);
     }
     DCSR_HIP(scan_exclusive_u32(P, Poff, R, 1, part, Poff + R, s));
@@ -501,19 +571,34 @@ dcsr_status dcsr_spgemm(dcsr_handle A, dcsr_handle B, dcsr_handle C, dcsr_stream
     DCSR_TRY(ws_get(C, WS_SG_TVAL, (size_t)(Ptot + 1) * 4, (void **)&tval));
 
     // 2. numeric phases
-    if (hst.count[0]) {
-        const unsigned g = (unsigned)((hst.count[0] + 3) / 4 < 8192 ? (hst.count[0] + 3) / 4 : 8192);
-        k_sg_wave<<<g, TB, 0, s>>>(lists, st, P, Poff, A->row_start, A->row_len, A->pool_col, A->pool_val,
-                                   B->row_start, B->row_len, B->pool_col, B->pool_val, tcol, tval, cnt);
+    if (hst.count[CL_TEAM]) {
+        const uint32_t per = TB / TEAM;
+        const uint32_t want = (hst.count[CL_TEAM] + per - 1) / per;
+        const unsigned g = want < 8192 ? want : 8192;
+        k_sg_team<<<g, TB, 0, s>>>(lists + CL_TEAM * (uint64_t)(R + 1), st, P, Poff, A->row_start, A->row_len,
+                                   A->pool_col, A->pool_val, B->row_start, B->row_len, B->pool_col, B->pool_val, tcol,
+                                   tval, cnt);
     }
-    if (hst.count[1]) {
-        const unsigned g = hst.count[1] < 4096 ? hst.count[1] : 4096;
-        k_sg_block<<<g, TB, 0, s>>>(lists + (R + 1), st, P, Poff, A->row_start, A->row_len, A->pool_col, A->pool_val,
-                                    B->row_start, B->row_len, B->pool_col, B->pool_val, tcol, tval, cnt);
+    if (hst.count[CL_WAVE]) {
+        const unsigned g = (unsigned)((hst.count[CL_WAVE] + 3) / 4 < 8192 ? (hst.count[CL_WAVE] + 3) / 4 : 8192);
+        if (B->cols <= WAVE_MAX_COLS32)
+            k_sg_wave<uint32_t><<<g, TB, 0, s>>>(lists + CL_WAVE * (uint64_t)(R + 1), st, P, Poff, A->row_start,
+                                                 A->row_len, A->pool_col, A->pool_val, B->row_start, B->row_len,
+                                                 B->pool_col, B->pool_val, tcol, tval, cnt);
+        else
+            k_sg_wave<uint64_t><<<g, TB, 0, s>>>(lists + CL_WAVE * (uint64_t)(R + 1), st, P, Poff, A->row_start,
+                                                 A->row_len, A->pool_col, A->pool_val, B->row_start, B->row_len,
+                                                 B->pool_col, B->pool_val, tcol, tval, cnt);
     }
-    if (hst.count[2]) {
-        const uint32_t hn = hst.count[2];
-        const uint32_t *hlist = lists + 2 * (uint64_t)(R + 1);
+    if (hst.count[CL_BLOCK]) {
+        const unsigned g = hst.count[CL_BLOCK] < 4096 ? hst.count[CL_BLOCK] : 4096;
+        k_sg_block<<<g, TB, 0, s>>>(lists + CL_BLOCK * (uint64_t)(R + 1), st, P, Poff, A->row_start, A->row_len,
+                                    A->pool_col, A->pool_val, B->row_start, B->row_len, B->pool_col, B->pool_val, tcol,
+                                    tval, cnt);
+    }
+    if (hst.count[CL_HUB]) {
+        const uint32_t hn = hst.count[CL_HUB];
+        const uint32_t *hlist = lists + CL_HUB * (uint64_t)(R + 1);
         uint32_t *hpre_in, *hpre, *blen, *apos, *emit, *epos, *v0, *v1;
         uint64_t *k0, *k1;
         EA *ea;

@@ -86,8 +86,15 @@ def test_integer_cancellation_fixture_exact(dev):
     mC.close()
 
 
+# the last three: the widest B the 32-bit sort keys allow, for rows of <= 128
+# partials (columns up to 2^25 - 2, keys just below the padding value) and of
+# 129..1024 partials (up to 2^22 - 2), and one column past the first limit
+# (small rows then take the 64-bit-key path)
 @pytest.mark.parametrize("R,K,N,nnzA,nnzB,seed", [(1, 1, 1, 1, 1, 0), (50, 70, 30, 400, 500, 1),
-                                                  (3000, 2000, 2500, 30000, 25000, 2)])
+                                                  (3000, 2000, 2500, 30000, 25000, 2),
+                                                  (300, 400, (1 << 25) - 1, 3000, 4000, 3),
+                                                  (200, 300, (1 << 22) - 1, 4000, 6000, 5),
+                                                  (300, 400, 1 << 25, 3000, 4000, 4)])
 def test_rectangular_small_int(dev, R, K, N, nnzA, nnzB, seed):
     torch, dcsr_amd = dev
     rng = np.random.default_rng(seed)
