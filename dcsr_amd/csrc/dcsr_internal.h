@@ -52,7 +52,7 @@ enum WsSlot {
     WS_TROW, WS_TUBEG, WS_TINS, WS_TDEL, WS_TOLDS, WS_TMOVED, WS_TDEST, WS_TNEWL, WS_TMODE, WS_TOLDPRE,
     WS_HUB_ROWS, WS_HUB_PRE, WS_HUB_PART, WS_HUB_DONE,
     WS_PPROBE, WS_PPLAN, WS_UROW, WS_TPAIRPRE, WS_FOFF, WS_ROWMAP, WS_CARRY_ROW, WS_CARRY_VAL, WS_SPMM_CROW, WS_SPMM_CVAL,
-    WS_SG_P, WS_SG_POFF, WS_SG_CNT, WS_SG_COFF, WS_SG_LIST, WS_SG_STATE, WS_SG_BSUM, WS_SG_TCOL, WS_SG_TVAL,
+    WS_SG_P, WS_SG_POFF, WS_SG_CNT, WS_SG_COFF, WS_SG_LIST, WS_SG_STATE, WS_SG_BSUM,
     WS_SG_HPRE, WS_SG_EA, WS_SG_APOS, WS_SG_KEY0, WS_SG_KEY1, WS_SG_VAL0, WS_SG_VAL1, WS_SG_FLAG, WS_SG_EPOS,
     WS_CANON, WS_TMP0, WS_TMP1, WS_TMP2,
     WS_PNL_OFF, WS_PNL_BND, WS_PNL_COL, WS_PNL_VAL, WS_PNL_MAP, WS_PNL_PART,

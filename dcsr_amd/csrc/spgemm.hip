@@ -29,6 +29,7 @@ namespace {
 constexpr int TB = 256;
 constexpr uint32_t TEAM_MAX = 128;    // partials per row for the team kernel
 constexpr int TEAM = 16;              // lanes per row in the team kernel
+static_assert(TEAM_MAX <= 8 * TEAM, "team rows: the rank sort holds at most 8 keys per lane");
 constexpr int TEAM_GB = 7;            // generation bits of the team kernel's 32-bit keys
 // the team kernel's keys are 32-bit: B's columns must stay below this
 constexpr int64_t TEAM_MAX_COLS = (1ll << (32 - TEAM_GB)) - 1;
@@ -228,18 +229,97 @@ __device__ __forceinline__ uint32_t find_seg_team(const uint32_t *s, uint32_t e)
 // K: the sort key type, (column << GB) | generation.  32-bit keys halve the
 // LDS per row (more rows in flight); they need columns below 2^(32-GB) - 1
 // so that no key reaches the ~0 padding.
-template <int NT, int GB, typename K>
-__device__ void sg_row(uint32_t row, uint32_t P, uint64_t out_base, const uint32_t *__restrict__ a_start,
-                       const int32_t *__restrict__ a_len, const int32_t *__restrict__ a_col,
-                       const float *__restrict__ a_val, const uint32_t *__restrict__ b_start,
-                       const int32_t *__restrict__ b_len, const int32_t *__restrict__ b_col,
-                       const float *__restrict__ b_val, int32_t *__restrict__ tcol, float *__restrict__ tval,
-                       uint32_t *__restrict__ cnt, K *key, float *val, uint32_t *s_ex, uint32_t *s_sb,
-                       float *s_va, uint32_t *lds_w)
+// What a row needs before its expansion: its partial count, output offset
+// and A row.  Loaded one row ahead by the kernels' row loops (the loads of
+// the next row are in flight while the current row is processed).
+struct SgRowInfo {
+    uint32_t row, P, ast, la;
+    uint64_t out_base;
+};
+
+__device__ __forceinline__ SgRowInfo sg_row_info(uint32_t row, const uint32_t *__restrict__ P,
+                                                 const uint32_t *__restrict__ Poff,
+                                                 const uint32_t *__restrict__ a_start,
+                                                 const int32_t *__restrict__ a_len)
+{
+    SgRowInfo r;
+    r.row = row;
+    r.P = P[row];
+    r.out_base = Poff[row];
+    r.ast = a_start[row];
+    r.la = (uint32_t)a_len[row];
+    return r;
+}
+
+// Rank sort of a team's P staged (key, val) pairs in LDS, for teams narrower
+// than a wave.  The keys are distinct, so a key's sorted position is the
+// number of smaller keys: lane t holds keys t, t + NT, ... in registers and
+// compares them with every key of the row, read back one at a time (the
+// team's lanes all read the same word: one broadcast per key), then every
+// pair is written to its position.  Costs P * KPL compares per lane and no
+// LDS traffic beyond the broadcasts, against a bitonic network's
+// log^2-many LDS passes.
+template <int NT, int KPL, typename K>
+__device__ __forceinline__ void team_rank_sort(K *key, float *val, uint32_t P)
 {
     const uint32_t t = threadIdx.x % NT;
-    const uint32_t ast = a_start[row];
-    const uint32_t la = (uint32_t)a_len[row];
+    K mk[KPL];
+    float mv[KPL];
+    uint32_t rk[KPL];
+#pragma unroll
+    for (int r = 0; r < KPL; ++r) {
+        const uint32_t i = t + (uint32_t)r * NT;
+        mk[r] = i < P ? key[i] : (K)~0ull;
+        mv[r] = i < P ? val[i] : 0.0f;
+        rk[r] = 0;
+    }
+    uint32_t j = 0;
+    for (; j + 4 <= P; j += 4) {
+        K k0, k1, k2, k3;
+        if constexpr (sizeof(K) == 4) {
+            // four keys per read (the team's key row is 16-byte aligned)
+            const uint4 q = *reinterpret_cast<const uint4 *>(key + j);
+            k0 = q.x;
+            k1 = q.y;
+            k2 = q.z;
+            k3 = q.w;
+        } else {
+            k0 = key[j];
+            k1 = key[j + 1];
+            k2 = key[j + 2];
+            k3 = key[j + 3];
+        }
+#pragma unroll
+        for (int r = 0; r < KPL; ++r)
+            rk[r] += (uint32_t)(k0 < mk[r]) + (uint32_t)(k1 < mk[r]) + (uint32_t)(k2 < mk[r]) +
+                     (uint32_t)(k3 < mk[r]);
+    }
+    for (; j < P; ++j) {
+        const K kj = key[j];
+#pragma unroll
+        for (int r = 0; r < KPL; ++r) rk[r] += (uint32_t)(kj < mk[r]);
+    }
+    team_sync<NT>();  // every read of the row is done before the scatter
+#pragma unroll
+    for (int r = 0; r < KPL; ++r) {
+        if (t + (uint32_t)r * NT < P) {
+            key[rk[r]] = mk[r];
+            val[rk[r]] = mv[r];
+        }
+    }
+    team_sync<NT>();
+}
+
+template <int NT, int GB, typename K>
+__device__ void sg_row(const SgRowInfo &ri, const int32_t *__restrict__ a_col, const float *__restrict__ a_val,
+                       const uint32_t *__restrict__ b_start, const int32_t *__restrict__ b_len,
+                       const int32_t *__restrict__ b_col, const float *__restrict__ b_val,
+                       int32_t *__restrict__ tcol, float *__restrict__ tval, uint32_t *__restrict__ cnt, K *key,
+                       float *val, uint32_t *s_ex, uint32_t *s_sb, float *s_va, uint32_t *lds_w)
+{
+    const uint32_t t = threadIdx.x % NT;
+    const uint32_t row = ri.row, P = ri.P, ast = ri.ast, la = ri.la;
+    const uint64_t out_base = ri.out_base;
     // 1. expand in generation order
     uint32_t gbase = 0;
     for (uint32_t a0 = 0; a0 < la; a0 += NT) {
@@ -271,26 +351,32 @@ __device__ void sg_row(uint32_t row, uint32_t P, uint64_t out_base, const uint32
         gbase += tot;
         team_sync<NT>();
     }
-    // 2. bitonic sort of n2 keys (padding sorts last)
-    const uint32_t n2 = next_pow2(P < 2 * NT ? 2 * NT : P);
-    for (uint32_t p = P + t; p < n2; p += NT) key[p] = (K)~0ull;
-    team_sync<NT>();
-    for (uint32_t k = 2; k <= n2; k <<= 1) {
-        for (uint32_t js = k >> 1; js > 0; js >>= 1) {
-            for (uint32_t c = t; c < n2 / 2; c += NT) {
-                const uint32_t i1 = (c / js) * 2 * js + (c % js);
-                const uint32_t i2 = i1 + js;
-                const bool asc = (i1 & k) == 0;
-                const K ka = key[i1], kb = key[i2];
-                if ((ka > kb) == asc) {
-                    key[i1] = kb;
-                    key[i2] = ka;
-                    const float va = val[i1];
-                    val[i1] = val[i2];
-                    val[i2] = va;
+    // 2. sort by (column, generation): rank sort for quarter-wave teams,
+    // bitonic network over LDS for wider ones (padding sorts last)
+    if constexpr (NT < 64) {
+        if (P <= 4 * NT) team_rank_sort<NT, 4, K>(key, val, P);
+        else team_rank_sort<NT, 8, K>(key, val, P);
+    } else {
+        const uint32_t n2 = next_pow2(P < 2 * NT ? 2 * NT : P);
+        for (uint32_t p = P + t; p < n2; p += NT) key[p] = (K)~0ull;
+        team_sync<NT>();
+        for (uint32_t k = 2; k <= n2; k <<= 1) {
+            for (uint32_t js = k >> 1; js > 0; js >>= 1) {
+                for (uint32_t c = t; c < n2 / 2; c += NT) {
+                    const uint32_t i1 = (c / js) * 2 * js + (c % js);
+                    const uint32_t i2 = i1 + js;
+                    const bool asc = (i1 & k) == 0;
+                    const K ka = key[i1], kb = key[i2];
+                    if ((ka > kb) == asc) {
+                        key[i1] = kb;
+                        key[i2] = ka;
+                        const float va = val[i1];
+                        val[i1] = val[i2];
+                        val[i2] = va;
+                    }
                 }
+                team_sync<NT>();
             }
-            team_sync<NT>();
         }
     }
     // 3. runs of equal columns, summed left to right from 0; drop exact zeros
@@ -321,6 +407,27 @@ __device__ void sg_row(uint32_t row, uint32_t P, uint64_t out_base, const uint32
     team_sync<NT>();
 }
 
+// Team loop over a class list: team `first`, stride `step`.  The next row's
+// id and info are loaded one and two iterations ahead.
+template <typename F>
+__device__ __forceinline__ void sg_rows(const uint32_t *__restrict__ list, uint32_t n, uint32_t first, uint32_t step,
+                                        const uint32_t *__restrict__ P, const uint32_t *__restrict__ Poff,
+                                        const uint32_t *__restrict__ a_start, const int32_t *__restrict__ a_len,
+                                        F &&body)
+{
+    if (first >= n) return;
+    SgRowInfo cur = sg_row_info(list[first], P, Poff, a_start, a_len);
+    uint32_t next_id = first + step < n ? list[first + step] : 0u;
+    for (uint32_t i = first; i < n; i += step) {
+        SgRowInfo nxt{};
+        if (i + step < n) nxt = sg_row_info(next_id, P, Poff, a_start, a_len);
+        const uint32_t nn_id = i + 2 * step < n ? list[i + 2 * step] : 0u;
+        body(cur);
+        cur = nxt;
+        next_id = nn_id;
+    }
+}
+
 // Rows of at most WAVE_MAX partials: one wave per row.  K: 32-bit sort keys
 // (half the LDS per row, more rows in flight) when B's columns allow.
 template <typename K>
@@ -339,12 +446,11 @@ __global__ __launch_bounds__(TB) void k_sg_wave(const uint32_t *__restrict__ lis
     __shared__ float s_va[TB / 64][64];
     const int w = threadIdx.x >> 6;
     const uint32_t n = st->count[CL_WAVE];
-    const uint32_t nw = gridDim.x * (TB / 64);
-    for (uint32_t i = blockIdx.x * (TB / 64) + w; i < n; i += nw) {
-        const uint32_t row = list[i];
-        sg_row<64, WAVE_GB, K>(row, P[row], Poff[row], a_start, a_len, a_col, a_val, b_start, b_len, b_col, b_val,
-                               tcol, tval, cnt, key[w], val[w], s_ex[w], s_sb[w], s_va[w], nullptr);
-    }
+    sg_rows(list, n, blockIdx.x * (TB / 64) + w, gridDim.x * (TB / 64), P, Poff, a_start, a_len,
+            [&](const SgRowInfo &ri) {
+                sg_row<64, WAVE_GB, K>(ri, a_col, a_val, b_start, b_len, b_col, b_val, tcol, tval, cnt, key[w],
+                                       val[w], s_ex[w], s_sb[w], s_va[w], nullptr);
+            });
 }
 
 // Rows of at most TEAM_MAX partials: 16-lane teams, four rows per wave.
@@ -358,18 +464,21 @@ __global__ __launch_bounds__(TB) void k_sg_team(const uint32_t *__restrict__ lis
                                                 uint32_t *__restrict__ cnt)
 {
     constexpr int NTEAM = TB / TEAM;
-    __shared__ uint32_t key[NTEAM][TEAM_MAX];
-    __shared__ float val[NTEAM][TEAM_MAX];
+    // per-team rows padded so the four teams of a wave start on different
+    // LDS banks; the pitch keeps every key row 16-byte aligned
+    constexpr uint32_t PITCH = TEAM_MAX + 20;
+    static_assert(PITCH % 4 == 0, "16-byte aligned key rows");
+    __shared__ __align__(16) uint32_t key[NTEAM][PITCH];
+    __shared__ float val[NTEAM][PITCH];
     __shared__ uint32_t s_ex[NTEAM][TEAM], s_sb[NTEAM][TEAM];
     __shared__ float s_va[NTEAM][TEAM];
     const int tm = threadIdx.x / TEAM;
     const uint32_t n = st->count[CL_TEAM];
-    const uint32_t nt = gridDim.x * NTEAM;
-    for (uint32_t i = blockIdx.x * NTEAM + tm; i < n; i += nt) {
-        const uint32_t row = list[i];
-        sg_row<TEAM, TEAM_GB, uint32_t>(row, P[row], Poff[row], a_start, a_len, a_col, a_val, b_start, b_len, b_col, b_val, tcol,
-                        tval, cnt, key[tm], val[tm], s_ex[tm], s_sb[tm], s_va[tm], nullptr);
-    }
+    sg_rows(list, n, blockIdx.x * NTEAM + tm, gridDim.x * NTEAM, P, Poff, a_start, a_len,
+            [&](const SgRowInfo &ri) {
+                sg_row<TEAM, TEAM_GB, uint32_t>(ri, a_col, a_val, b_start, b_len, b_col, b_val, tcol, tval, cnt,
+                                                key[tm], val[tm], s_ex[tm], s_sb[tm], s_va[tm], nullptr);
+            });
 }
 
 __global__ __launch_bounds__(TB) void k_sg_block(const uint32_t *__restrict__ list, const SgState *__restrict__ st,
@@ -387,11 +496,10 @@ __global__ __launch_bounds__(TB) void k_sg_block(const uint32_t *__restrict__ li
     __shared__ float s_va[TB];
     __shared__ uint32_t lds_w[TB / 64];
     const uint32_t n = st->count[CL_BLOCK];
-    for (uint32_t i = blockIdx.x; i < n; i += gridDim.x) {
-        const uint32_t row = list[i];
-        sg_row<TB, 12, uint64_t>(row, P[row], Poff[row], a_start, a_len, a_col, a_val, b_start, b_len, b_col, b_val, tcol, tval,
-                       cnt, key, val, s_ex, s_sb, s_va, lds_w);
-    }
+    sg_rows(list, n, blockIdx.x, gridDim.x, P, Poff, a_start, a_len, [&](const SgRowInfo &ri) {
+        sg_row<TB, 12, uint64_t>(ri, a_col, a_val, b_start, b_len, b_col, b_val, tcol, tval, cnt, key, val, s_ex, s_sb,
+                                 s_va, lds_w);
+    });
 }
 
 // ---------------------------------------------------------------------------
@@ -507,21 +615,23 @@ __global__ void k_sg_hcount(const uint32_t *__restrict__ hlist, uint32_t hn, con
 }
 
 // C row table from the output counts.
-__global__ void k_sg_set_rows(uint32_t R, const uint32_t *__restrict__ coff, const uint32_t *__restrict__ cnt,
+// Row r of C: chunk [Poff[r], Poff[r] + P[r]) holding cnt[r] entries.
+__global__ void k_sg_set_rows(uint32_t R, const uint32_t *__restrict__ Poff, const uint32_t *__restrict__ P,
+                              const uint32_t *__restrict__ cnt, const uint32_t *__restrict__ nnz, uint32_t ptot,
                               uint32_t *__restrict__ start, int32_t *__restrict__ len, int32_t *__restrict__ cap,
                               DevState *ds)
 {
     const uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
     if (r < R) {
-        start[r] = coff[r];
+        start[r] = Poff[r];
         len[r] = (int32_t)cnt[r];
-        cap[r] = (int32_t)cnt[r];
+        cap[r] = (int32_t)P[r];
     }
     if (r == 0) {
-        const uint32_t tot = coff[R];
-        ds->bump = tot;
+        const uint32_t tot = nnz[0];
+        ds->bump = ptot;
         ds->nnz = tot;
-        ds->garbage = 0;
+        ds->garbage = ptot - tot;
         ds->status = 0;
     }
 }
@@ -565,10 +675,15 @@ dcsr_status dcsr_spgemm(dcsr_handle A, dcsr_handle B, dcsr_handle C, dcsr_stream
     if (hst.total >= 0xfffffff0ull) return DCSR_ERR_UNSUPPORTED;
     const uint32_t Ptot = (uint32_t)hst.total;
 
-    int32_t *tcol;
-    float *tval;
-    DCSR_TRY(ws_get(C, WS_SG_TCOL, (size_t)(Ptot + 1) * 4, (void **)&tcol));
-    DCSR_TRY(ws_get(C, WS_SG_TVAL, (size_t)(Ptot + 1) * 4, (void **)&tval));
+    // C's rows are written in place in its pool at their partial offsets:
+    // row r owns the P_r slots from Poff[r] (its chunk; the slots its merged
+    // columns do not fill are garbage), so no copy assembles C afterwards
+    if ((int64_t)Ptot > C->pool_cap) {
+        const int64_t want = (int64_t)(Ptot * C->cfg.pool_factor);
+        DCSR_TRY(alloc_pool(C, want > (int64_t)Ptot ? want : (int64_t)Ptot));
+    }
+    int32_t *tcol = C->pool_col;
+    float *tval = C->pool_val;
 
     // 2. numeric phases
     if (hst.count[CL_TEAM]) {
@@ -647,18 +762,12 @@ dcsr_status dcsr_spgemm(dcsr_handle A, dcsr_handle B, dcsr_handle C, dcsr_stream
     }
     DCSR_HIP(hipGetLastError());
 
-    // 3. assemble C in natural row order
+    // 3. C's row table: chunks at the partial offsets; nnz from a scan of
+    // the row counts
     DCSR_TRY(ws_get(C, WS_PART, scan_part_words(R + 1, 1) * 4, (void **)&part));
     DCSR_HIP(scan_exclusive_u32(cnt, coff, R, 1, part, coff + R, s));
-    uint32_t nnzC = 0;
-    DCSR_HIP(hipMemcpyAsync(&nnzC, coff + R, 4, hipMemcpyDeviceToHost, s));
-    DCSR_HIP(hipStreamSynchronize(s));
-    if ((int64_t)nnzC > C->pool_cap) {
-        int64_t want = (int64_t)(nnzC * C->cfg.pool_factor);
-        DCSR_TRY(alloc_pool(C, want > (int64_t)nnzC ? want : (int64_t)nnzC));
-    }
-    launch_copy_rows(coff, R, nnzC, Poff, tcol, tval, C->pool_col, C->pool_val, s);
-    k_sg_set_rows<<<grid_for(R + 1, TB), TB, 0, s>>>(R, coff, cnt, C->row_start, C->row_len, C->row_cap, C->ds);
+    k_sg_set_rows<<<grid_for(R + 1, TB), TB, 0, s>>>(R, Poff, P, cnt, coff + R, Ptot, C->row_start, C->row_len,
+                                                     C->row_cap, C->ds);
     DCSR_HIP(hipGetLastError());
     DCSR_TRY(pull_state(C, s));
     C->canonical = true;
