@@ -608,7 +608,7 @@ def bench_serving(args):
     # within the per-call path's p99 (equal-latency comparison)
     best = None
     sweep = []
-    for mult in (1, 1.5, 2, 3, 4, 6, 8, 12):
+    for mult in (1, 1.15, 1.3, 1.45, 1.6, 1.8, 2, 3, 4, 6, 8, 12):
         thr, lat, st = run_server(rate_pc * mult)
         p50, p99 = float(np.percentile(lat, 50)), float(np.percentile(lat, 99))
         sweep.append({"offered_x": mult, "throughput": thr, "p50_ms": p50, "p99_ms": p99,

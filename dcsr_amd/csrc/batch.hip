@@ -454,7 +454,21 @@ __device__ void batch_spmv(uint32_t m, const uint32_t *__restrict__ row_base, co
             mine = L <= thresh;
             if (mine) {
                 const uint64_t st = row_start[gbase + r];
-                for (uint32_t k = tl; k < L; k += TEAM) acc = acc + pval[st + k] * xm[pcol[st + k]];
+                uint32_t k = tl;
+                // four entries of the lane in flight; the lane still sums
+                // its entries in ascending order
+                for (; k + 3 * TEAM < L; k += 4 * TEAM) {
+                    const int32_t c0 = pcol[st + k], c1 = pcol[st + k + TEAM], c2 = pcol[st + k + 2 * TEAM],
+                                  c3 = pcol[st + k + 3 * TEAM];
+                    const float v0 = pval[st + k], v1 = pval[st + k + TEAM], v2 = pval[st + k + 2 * TEAM],
+                                v3 = pval[st + k + 3 * TEAM];
+                    const float x0 = xm[c0], x1 = xm[c1], x2 = xm[c2], x3 = xm[c3];
+                    acc = acc + v0 * x0;
+                    acc = acc + v1 * x1;
+                    acc = acc + v2 * x2;
+                    acc = acc + v3 * x3;
+                }
+                for (; k < L; k += TEAM) acc = acc + pval[st + k] * xm[pcol[st + k]];
             }
         }
 #pragma unroll
@@ -467,7 +481,17 @@ __device__ void batch_spmv(uint32_t m, const uint32_t *__restrict__ row_base, co
         if (L <= thresh) continue;
         const uint64_t st = row_start[gbase + r];
         float acc = 0.0f;
-        for (uint32_t k = lane_id(); k < L; k += 64) acc = acc + pval[st + k] * xm[pcol[st + k]];
+        uint32_t k = lane_id();
+        for (; k + 3 * 64 < L; k += 4 * 64) {
+            const int32_t c0 = pcol[st + k], c1 = pcol[st + k + 64], c2 = pcol[st + k + 128], c3 = pcol[st + k + 192];
+            const float v0 = pval[st + k], v1 = pval[st + k + 64], v2 = pval[st + k + 128], v3 = pval[st + k + 192];
+            const float x0 = xm[c0], x1 = xm[c1], x2 = xm[c2], x3 = xm[c3];
+            acc = acc + v0 * x0;
+            acc = acc + v1 * x1;
+            acc = acc + v2 * x2;
+            acc = acc + v3 * x3;
+        }
+        for (; k < L; k += 64) acc = acc + pval[st + k] * xm[pcol[st + k]];
         acc = wave_sum_f32(acc);
         if (lane_id() == 0) y[gbase + r] = acc;
     }
