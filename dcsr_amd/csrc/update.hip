@@ -767,8 +767,11 @@ __global__ __launch_bounds__(TB) void k_upd_move(const DevState *__restrict__ ds
 
 // Inserted entries, in-place value updates, row-table commit.  A rejected
 // batch writes nothing; with `report` (a caller that does not read the
-// decision back) its cause becomes a sticky status bit.
-__global__ __launch_bounds__(TB) void k_upd_write(DevState *ds, uint32_t report, const uint32_t *__restrict__ ut,
+// decision back) its cause becomes a sticky status bit.  The per-coordinate
+// loads (in bounds for every u < n) are issued before the decision is read,
+// then the per-row ones: two round trips before the writes.
+__global__ __launch_bounds__(TB) void k_upd_write(DevState *ds, uint32_t report, int64_t n,
+                                                  const uint32_t *__restrict__ ut,
                                                   const uint32_t *__restrict__ trow, const uint32_t *__restrict__ tubeg,
                                                   const uint32_t *__restrict__ tpre,
                                                   const uint32_t *__restrict__ tnewl,
@@ -779,7 +782,19 @@ __global__ __launch_bounds__(TB) void k_upd_write(DevState *ds, uint32_t report,
                                                   float *__restrict__ pval, uint32_t *__restrict__ row_start,
                                                   int32_t *__restrict__ row_len, int32_t *__restrict__ row_cap)
 {
-    int64_t u = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t u = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    uint32_t t = 0, code = C_NOP, pos = 0, ui = 0, ud = 0;
+    int32_t col = 0;
+    float v = 0.0f;
+    if (u < n) {
+        t = ut[u];
+        code = ucode[u];
+        pos = upos[u];
+        v = uval[u];
+        col = ucol[u];
+        ui = upre[2 * u];
+        ud = upre[2 * u + 1];
+    }
     if (!ds->go) {
         if (report && u == 0) ds->status |= ds->ubad_last ? ST_BAD_INDEX : ST_POOL_EXHAUSTED;
         return;
@@ -790,24 +805,23 @@ __global__ __launch_bounds__(TB) void k_upd_write(DevState *ds, uint32_t report,
         ds->garbage += ds->d_garbage;
     }
     if (u >= nu) return;
-    const uint32_t t = ut[u];
-    const uint32_t code = ucode[u];
     const uint32_t mode = tmode[t];
     const uint32_t r = trow[t];
+    const uint32_t ub = tubeg[t];
+    const uint32_t tp = tpre[2 * t];
+    const uint32_t nl = tnewl[t];
     if (mode == M_VALUES) {
-        if (code == C_ADD) pval[(uint64_t)row_start[r] + upos[u]] = uval[u];
+        if (code == C_ADD) pval[(uint64_t)row_start[r] + pos] = v;
         return;
     }
-    const uint32_t ub = tubeg[t];
-    const uint64_t dest = ds->bump_saved + tpre[2 * t];
+    const uint64_t dest = ds->bump_saved + tp;
     if (code == C_INS) {
-        int64_t shift = (int64_t)(upre[2 * u] - upre[2 * ub]) - (int64_t)(upre[2 * u + 1] - upre[2 * ub + 1]);
-        uint64_t d = dest + (uint64_t)((int64_t)upos[u] + shift);
-        pcol[d] = ucol[u];
-        pval[d] = uval[u];
+        const int64_t shift = (int64_t)(ui - upre[2 * ub]) - (int64_t)(ud - upre[2 * ub + 1]);
+        const uint64_t d = dest + (uint64_t)((int64_t)pos + shift);
+        pcol[d] = col;
+        pval[d] = v;
     }
     if (u == ub) {
-        uint32_t nl = tnewl[t];
         row_start[r] = (uint32_t)dest;
         row_len[r] = (int32_t)nl;
         row_cap[r] = (int32_t)chunk_round(nl);
@@ -919,7 +933,7 @@ dcsr_status commit_launch(dcsr_handle h, int64_t n, int64_t move_bound, bool rep
     if (blocks < 1) blocks = 1;
     k_upd_move<<<(unsigned)blocks, TB, 0, s>>>(h->ds, b.tpre, b.trow, h->row_start, b.tubeg, b.ucol, b.uval, b.ucode,
                                                b.upre, h->pool_col, h->pool_val, per_wave);
-    k_upd_write<<<grid_for(n, TB), TB, 0, s>>>(h->ds, report ? 1u : 0u, b.ut, b.trow, b.tubeg, b.tpre, b.tnewl, b.tmode, b.ucol, b.uval,
+    k_upd_write<<<grid_for(n, TB), TB, 0, s>>>(h->ds, report ? 1u : 0u, n, b.ut, b.trow, b.tubeg, b.tpre, b.tnewl, b.tmode, b.ucol, b.uval,
                                                 b.upos, b.ucode, b.upre, h->pool_col, h->pool_val, h->row_start,
                                                 h->row_len, h->row_cap);
     DCSR_HIP(hipGetLastError());
