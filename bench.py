@@ -73,6 +73,7 @@ def parse():
                     help="only run a plain 256 MiB device copy three times (calibrates HBM traffic counters)")
     ap.add_argument("--serving-rows", type=int, default=10_000_000)
     ap.add_argument("--serving-nnz", type=int, default=160_000_000)
+    ap.add_argument("--serving-max-cols", type=int, default=64, help="server group width (packed columns)")
     args = ap.parse_args()
     if args.steps is None:
         args.steps = DEFAULT_STEPS[args.workload]
@@ -566,7 +567,7 @@ def bench_serving(args):
 
     def run_server(rate, warmup=False):
         m = fresh()
-        srv = dcsr_amd.DcsrServer(m, max_cols=64)
+        srv = dcsr_amd.DcsrServer(m, max_cols=args.serving_max_cols)
         if warmup:
             for i, q in enumerate(warm):
                 if q[0] == "update":
@@ -624,6 +625,7 @@ def bench_serving(args):
         best = min(sweep, key=lambda e: e["p99_ms"])
     line(args, best["throughput"], "requests/s", 1e3 / best["throughput"],
          {"workload": "serving-powerlaw", "rows": R, "nnz0": nnz0, "requests": len(reqs), "update_share": 0.1,
+          "max_cols": args.serving_max_cols,
           "update_triples": "log-uniform 1e3..1e6, 30% deletes", "query_k": [1, 4, 16, 32]},
          p50_ms=best["p50_ms"], p99_ms=best["p99_ms"],
          per_call={"throughput": thr_pc, "p50_ms": float(np.percentile(lat_pc, 50)), "p99_ms": p99_pc,

@@ -12,6 +12,8 @@
 
 // Rows longer than this are counted as power-law hubs in dcsr_info.
 #define DCSR_HUB_LEN 4096
+// SpMV calls on one structure that pay for building its paneled copy
+#define DCSR_PNL_REUSE 4
 #define DCSR_HUB_CHUNK 8192
 // Nonzeros owned by one wave in the flat SpMV / SpMM kernels.
 #define DCSR_SPMV_PW 256u
@@ -103,7 +105,8 @@ struct dcsr_matrix_s {
     // column-paneled copy for SpMV on matrices whose x exceeds the L2 (see
     // spmv.hip): valid until the next structural change
     bool pnl_valid;
-    int pnl_hits;            // SpMV calls since the last structural change
+    int pnl_calls;           // SpMV calls since the last structural change
+    int pnl_prev_calls;      // the same for the last structure that had any
     int pnl_count;           // panels in the current copy
     int64_t pnl_cols;        // panel width (columns)
     int64_t pnl_min_nnz;     // smallest nnz worth a paneled copy

@@ -427,6 +427,7 @@ dcsr_status dcsr_create(dcsr_handle *out, int64_t rows, int64_t cols, const dcsr
     // power-law matrix between 0.5M and 4M) for matrices of at least 16M
     // entries; both can be overridden for testing
     h->pnl_cols = 2 << 20;
+    h->pnl_prev_calls = DCSR_PNL_REUSE;  // no evidence yet: a fresh matrix is assumed to be reused
     h->pnl_min_nnz = 16ll << 20;
     if (const char *e = getenv("DCSR_SPMV_PANEL_COLS")) h->pnl_cols = atoll(e) > 0 ? atoll(e) : h->pnl_cols;
     if (const char *e = getenv("DCSR_SPMV_PANEL_MIN_NNZ")) h->pnl_min_nnz = atoll(e) >= 0 ? atoll(e) : h->pnl_min_nnz;

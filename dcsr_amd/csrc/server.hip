@@ -44,6 +44,11 @@ struct PackDesc {
 };
 
 constexpr int MAXQ = 64;  // requests per group
+// Query groups launched and not yet finished on the GPU: one running, one
+// queued behind it.  The scheduler forms the next group only when fewer are
+// in flight, so requests that arrive meanwhile join it (a group costs about
+// one pass over A whatever its width up to 32 packed columns).
+constexpr size_t MAX_INFLIGHT_GROUPS = 2;
 
 // Packed X (C x Kp, row-major, leading dimension ld) from the group's
 // inputs; columns [K, Kp) are zero padding.
@@ -288,14 +293,35 @@ static dcsr_status run_queries(dcsr_server_s *S, const std::vector<Req> &g)
 static void scheduler_loop(dcsr_server_s *S)
 {
     (void)hipSetDevice(S->device);
+    std::deque<hipEvent_t> groups_in_flight;  // oldest first
     for (;;) {
         std::vector<Req> group;
         Req upd;
         bool is_update = false;
+        bool query_next;
         {
             std::unique_lock<std::mutex> lk(S->mu);
             S->cv_sched.wait(lk, [&] { return S->stop || !S->queue.empty(); });
-            if (S->queue.empty() && S->stop) return;
+            if (S->queue.empty() && S->stop) break;
+            query_next = S->queue.front().r.kind != DCSR_REQ_UPDATE;
+        }
+        if (query_next) {
+            // retire finished groups; with MAX_INFLIGHT_GROUPS still on the
+            // GPU, wait for the oldest (outside the lock: arrivals keep
+            // queueing and join the next group)
+            while (!groups_in_flight.empty() && hipEventQuery(groups_in_flight.front()) == hipSuccess) {
+                (void)hipEventDestroy(groups_in_flight.front());
+                groups_in_flight.pop_front();
+            }
+            if (groups_in_flight.size() >= MAX_INFLIGHT_GROUPS) {
+                (void)hipEventSynchronize(groups_in_flight.front());
+                (void)hipEventDestroy(groups_in_flight.front());
+                groups_in_flight.pop_front();
+            }
+        }
+        {
+            std::unique_lock<std::mutex> lk(S->mu);
+            if (S->queue.empty()) continue;
             if (S->queue.front().r.kind == DCSR_REQ_UPDATE) {
                 upd = S->queue.front();
                 S->queue.pop_front();
@@ -317,16 +343,24 @@ static void scheduler_loop(dcsr_server_s *S)
             push_inflight(S, S->su, std::move(t), st);
         } else {
             dcsr_status st = run_queries(S, group);
-            hipEvent_t ev;
+            hipEvent_t ev, gev;
             (void)hipEventCreateWithFlags(&ev, hipEventDisableTiming);
             (void)hipEventRecord(ev, S->sq);
             if (S->have_query_ev) (void)hipEventDestroy(S->last_query_ev);
             S->last_query_ev = ev;
             S->have_query_ev = true;
+            if (hipEventCreateWithFlags(&gev, hipEventDisableTiming) == hipSuccess) {
+                (void)hipEventRecord(gev, S->sq);
+                groups_in_flight.push_back(gev);
+            }
             std::vector<uint64_t> t;
             for (auto &q : group) t.push_back(q.ticket);
             push_inflight(S, S->sq, std::move(t), st);
         }
+    }
+    for (hipEvent_t e : groups_in_flight) {
+        (void)hipEventSynchronize(e);
+        (void)hipEventDestroy(e);
     }
 }
 

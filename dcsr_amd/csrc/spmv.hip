@@ -729,7 +729,8 @@ dcsr_status ensure_flat(dcsr_handle h, hipStream_t s)
 {
     if (!h->flat_dirty) return DCSR_OK;
     h->pnl_valid = false;
-    h->pnl_hits = 0;
+    if (h->pnl_calls > 0) h->pnl_prev_calls = h->pnl_calls;
+    h->pnl_calls = 0;
     const int64_t R = h->rows;
     uint32_t *foff, *part;
     DCSR_TRY(ws_get(h, WS_FOFF, (size_t)(R + 1) * 4, (void **)&foff));
@@ -890,7 +891,15 @@ dcsr_status dcsr_spmv(dcsr_handle h, const float *x, float *y, float alpha, floa
     if (h->rows == 0) return DCSR_OK;
     DCSR_TRY(prepare_compute(h, s));
     if (pnl_eligible(h)) {
-        if (!h->pnl_valid && ++h->pnl_hits >= 2) DCSR_TRY(build_panels(h, s));
+        // the copy costs about two thirds of a direct SpMV and saves about a
+        // third of one per later call: build it on the second call of a
+        // structure when the
+        // last structure that saw SpMVs saw DCSR_PNL_REUSE or more (a stream
+        // that updates every few calls never pays for it), else on the
+        // DCSR_PNL_REUSE-th call
+        const int c = ++h->pnl_calls;
+        if (!h->pnl_valid && c >= 2 && (h->pnl_prev_calls >= DCSR_PNL_REUSE || c >= DCSR_PNL_REUSE))
+            DCSR_TRY(build_panels(h, s));
         if (h->pnl_valid) return spmv_paneled(h, x, y, alpha, beta, s);
     }
     return launch_flat<1>(h, x, 1, y, 1, alpha, beta, s);
