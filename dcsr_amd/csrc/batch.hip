@@ -501,7 +501,8 @@ template <int TEAM>
 __global__ __launch_bounds__(BT) void k_batch_step(uint32_t M, const uint32_t *__restrict__ row_base,
                                                    const uint32_t *__restrict__ col_base,
                                                    const int64_t *__restrict__ tri_off, int64_t ntri,
-                                                   const int32_t *__restrict__ trow, const int32_t *__restrict__ tcol, const float *__restrict__ tval,
+                                                   const int32_t *__restrict__ trow, const int32_t *__restrict__ tcol,
+                                                   const float *__restrict__ tval,
                                                    uint32_t *__restrict__ row_start, int32_t *__restrict__ row_len,
                                                    int32_t *__restrict__ row_cap, int32_t *__restrict__ pcol,
                                                    float *__restrict__ pval, unsigned long long pool_cap, DevState *ds,

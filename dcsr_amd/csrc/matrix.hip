@@ -435,7 +435,8 @@ dcsr_status dcsr_create(dcsr_handle *out, int64_t rows, int64_t cols, const dcsr
     if (st == DCSR_OK) st = alloc_pool(h, h->cfg.min_pool);
     if (st == DCSR_OK && hipMalloc(&h->ds, sizeof(DevState)) != hipSuccess) st = DCSR_ERR_OOM;
     if (st == DCSR_OK && hipHostMalloc(&h->hs, sizeof(DevState), hipHostMallocDefault) != hipSuccess) st = DCSR_ERR_OOM;
-    if (st == DCSR_OK && hipHostMalloc(&h->snap, sizeof(DevState), hipHostMallocDefault) != hipSuccess) st = DCSR_ERR_OOM;
+    if (st == DCSR_OK && hipHostMalloc(&h->snap, sizeof(DevState), hipHostMallocDefault) != hipSuccess)
+        st = DCSR_ERR_OOM;
     if (st == DCSR_OK && hipMemset(h->ds, 0, sizeof(DevState)) != hipSuccess) st = DCSR_ERR_HIP;
     if (st == DCSR_OK) {
         // all rows empty

@@ -7,7 +7,8 @@
 // engine produces exactly the host path's result (sort batch by row,
 // COO->CSR, cusp::add; cusp/system/detail/sequential/elementwise.h:44-131):
 //
-//   1. pack (row, col) keys, stable LSD radix sort of (key, value) pairs;
+//   1. pack (row, col) keys (counting the sort's first digit per tile on
+//      the way), stable LSD radix sort of (key, value) pairs;
 //   2. heads (two passes over 2048-entry tiles): one record per distinct
 //      coordinate, summed left-to-right in batch order starting from 0 (the
 //      host path's B_row[j] accumulation), one per touched row;
@@ -27,6 +28,14 @@
 //
 // Cross-tile prefixes come from per-tile counts scanned by the last
 // workgroup to finish (an atomic finish counter); nothing polls.
+//
+// dcsr_update is synchronous by default: the batch's status comes back with
+// the call, from one read of the device state (at the end when the host can
+// bound the claims below the free pool, else between the plan and the
+// writes).  With DCSR_UPDATE_NOSYNC nothing is read back: the host keeps
+// upper bounds of the live entries and of the bump pointer, tightened from
+// asynchronous state snapshots, and compacts in stream order when they
+// cannot rule out exhaustion.
 #include "dcsr_internal.h"
 #include "prims.h"
 #include "segments.h"
