@@ -245,6 +245,45 @@ def test_errors_leave_matrix_unchanged(torch):
     m.close()
 
 
+def test_nosync_errors_are_sticky_and_leave_matrix_unchanged(torch):
+    """NOSYNC calls return at once; a rejected batch is a no-op whose cause is
+    reported by the next status read, which then clears it.  Later batches
+    still apply."""
+    from dcsr_amd import DcsrMatrix, UPDATE_NOGROW, UPDATE_NOSYNC
+    rng = np.random.default_rng(13)
+    R = 300
+    keys = rng.choice(R * R, 3000, replace=False)
+    r, c = (keys // R).astype(np.int32), (keys % R).astype(np.int32)
+    v = rng.uniform(-1, 1, keys.size).astype(np.float32)
+    m = DcsrMatrix(R, R, pool_factor=1.0, min_pool=16, allow_grow=False)
+    m.LoadMatrix(r, c, v)
+    # an empty synchronous update canonicalizes the loaded matrix, so the
+    # NOSYNC calls below take the asynchronous path
+    m.UpdateMatrix(np.zeros(0, np.int32), np.zeros(0, np.int32), np.zeros(0, np.float32))
+    A = O.coo_to_csr(R, R, r, c, v)
+    say("nosync bad index")
+    m.UpdateMatrix(np.array([3, R + 5], np.int32), np.array([1, 1], np.int32), np.array([1.0, 2.0], np.float32),
+                   flags=UPDATE_NOSYNC)
+    assert m.status() == 2  # DCSR_ERR_BAD_INDEX
+    assert m.status() == 0  # read clears it
+    assert_same_structure(to_host_csr(m), A)
+    say("nosync pool exhaustion without growth")
+    br, bc, bv = random_coo(rng, R, R, 6000, 0.0, 0.0, "sym")
+    m.UpdateMatrix(br, bc, bv, flags=UPDATE_NOSYNC | UPDATE_NOGROW)
+    assert m.status() == 3  # DCSR_ERR_POOL_EXHAUSTED
+    assert_same_structure(to_host_csr(m), A)
+    say("nosync batch that fits, after the rejected ones")
+    sr, sc, sv = A.off, A.col, A.val
+    rows_of = np.repeat(np.arange(R), np.diff(sr))
+    pick = rng.choice(A.nnz, 100, replace=False)
+    ur, uc, uv = rows_of[pick].astype(np.int32), sc[pick].astype(np.int32), (0.5 * sv[pick]).astype(np.float32)
+    m.UpdateMatrix(ur, uc, uv, flags=UPDATE_NOSYNC)  # adds only: values in place, no claims
+    A = O.update(A, ur, uc, uv)
+    assert m.status() == 0
+    assert_same_structure(to_host_csr(m), A)
+    m.close()
+
+
 def test_csr_load_and_initialize(torch):
     from dcsr_amd import DcsrMatrix
     rng = np.random.default_rng(13)
