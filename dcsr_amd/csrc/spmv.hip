@@ -880,9 +880,9 @@ static dcsr_status spmv_paneled(dcsr_handle h, const float *x, float *y, float a
 }
 
 // SpMV.  A matrix whose x spans several L2-sized column panels is multiplied
-// through its paneled copy once it has been queried twice since its last
-// structural change (the copy costs about one extra pass over the matrix,
-// so update-then-multiply patterns keep the direct flat kernel).
+// through its paneled copy once its structure has seen enough SpMV calls to
+// pay for the copy (about one extra pass over the matrix), so
+// update-then-multiply patterns keep the direct flat kernel.
 dcsr_status dcsr_spmv(dcsr_handle h, const float *x, float *y, float alpha, float beta, dcsr_stream_t stream)
 {
     if (!h || (h->rows > 0 && (!x || !y))) return DCSR_ERR_INVALID;
@@ -893,10 +893,9 @@ dcsr_status dcsr_spmv(dcsr_handle h, const float *x, float *y, float alpha, floa
     if (pnl_eligible(h)) {
         // the copy costs about two thirds of a direct SpMV and saves about a
         // third of one per later call: build it on the second call of a
-        // structure when the
-        // last structure that saw SpMVs saw DCSR_PNL_REUSE or more (a stream
-        // that updates every few calls never pays for it), else on the
-        // DCSR_PNL_REUSE-th call
+        // structure when the last structure that saw SpMVs saw
+        // DCSR_PNL_REUSE or more (a stream that updates every few calls
+        // never pays for it), else on the DCSR_PNL_REUSE-th call
         const int c = ++h->pnl_calls;
         if (!h->pnl_valid && c >= 2 && (h->pnl_prev_calls >= DCSR_PNL_REUSE || c >= DCSR_PNL_REUSE))
             DCSR_TRY(build_panels(h, s));
