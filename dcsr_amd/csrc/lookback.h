@@ -7,18 +7,21 @@
 // inclusive prefix, then publishes its own inclusive prefix.  A status word
 // packs (epoch, flag) in the high half and the 32-bit value in the low half,
 // so one 64-bit agent-scope atomic store publishes both and readers need no
-// fence.  Epochs are unique per launch (process-wide counter), so status
-// arrays are never cleared; words from older launches read as "not ready".
+// fence.  The epoch is a per-launch count kept on the device (ctl[3], bumped
+// by the last workgroup of each launch, read by every workgroup at its
+// start), so status arrays are never cleared, words from older launches read
+// as "not ready", and replays of a captured graph get fresh epochs too.
 //
 // Every wait is bounded: after LB_SPIN_LIMIT polls a tile gives up, counts
 // the fault in ctl[2] of its launch (never reset; 0 in a correct run) and
 // proceeds, so no launch can hang the device.
 //
 // Control words ctl[0..3] sit at the head of the launch's workspace:
-// ctl[0] ticket, ctl[1] finished workgroups, ctl[2] fault count.  The ticket
-// and finish counters are self-resetting (the last workgroup to finish a
-// launch zeroes them); they start at zero because workspace buffers are
-// zero-filled on allocation.
+// ctl[0] ticket, ctl[1] finished workgroups, ctl[2] fault count, ctl[3]
+// epoch.  The ticket and finish counters are self-resetting (the last
+// workgroup to finish a launch zeroes them); all start at zero because
+// workspace buffers are zero-filled on allocation.  Launches that share a
+// control block must be stream-ordered.
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -53,12 +56,19 @@ __device__ __forceinline__ bool lb_inclusive(uint64_t w, uint32_t tag)
     return (hi & ~3u) == tag && (hi & 3u) == LB_INC;
 }
 
-// Ticket of this workgroup (thread 0 draws; broadcast through LDS).
-__device__ __forceinline__ uint32_t lb_ticket(unsigned int *ctl, uint32_t *lds_slot)
+// Ticket of this workgroup and the launch's status tag (thread 0 draws and
+// reads; broadcast through LDS, two words).  The epoch word was last written
+// by the previous launch's last workgroup, which finished before this launch
+// started, so every workgroup reads the same value.
+__device__ __forceinline__ uint32_t lb_ticket(unsigned int *ctl, uint32_t *lds2, uint32_t *tag)
 {
-    if (threadIdx.x == 0) *lds_slot = atomicAdd(&ctl[0], 1u);
+    if (threadIdx.x == 0) {
+        lds2[0] = atomicAdd(&ctl[0], 1u);
+        lds2[1] = __hip_atomic_load(&ctl[3], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
     __syncthreads();
-    const uint32_t t = *lds_slot;
+    const uint32_t t = lds2[0];
+    *tag = lb_tag(lds2[1]);
     __syncthreads();
     return t;
 }
@@ -70,6 +80,7 @@ __device__ __forceinline__ void lb_finish(unsigned int *ctl)
 {
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
     if (atomicAdd(&ctl[1], 1u) == gridDim.x - 1) {
+        ctl[3] = ctl[3] + 1u;
         ctl[0] = 0u;
         ctl[1] = 0u;
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
@@ -163,5 +174,3 @@ __device__ __forceinline__ void lb_tile_prefix(const uint32_t *s_agg, uint32_t *
     __syncthreads();
 }
 
-// Host: a fresh epoch for one launch (process-wide, thread-safe).
-uint32_t lb_next_epoch();

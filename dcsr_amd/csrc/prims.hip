@@ -14,7 +14,6 @@
 // digit's offset by look-back walks hundreds of tiles serially per digit
 // here, where every look-back poll is a cross-XCD coherent load; the table
 // scan is faster.)
-#include <atomic>
 #include "lookback.h"
 #include "prims.h"
 #include "wave.h"
@@ -31,53 +30,41 @@ constexpr int SCAN_THREADS = 256;
 constexpr int SCAN_IPT = 8;
 constexpr int SCAN_TILE = SCAN_THREADS * SCAN_IPT;
 
-// Block-wide exclusive scan of one uint32 per thread (256 threads).
-// Returns the exclusive prefix; *block_total receives the block sum.
-__device__ __forceinline__ uint32_t block_excl_scan_256(uint32_t v, uint32_t *lds4, uint32_t *block_total)
-{
-    const int w = threadIdx.x >> 6;
-    uint32_t inc = wave_incl_scan_u32(v);
-    if (lane_id() == 63) lds4[w] = inc;
-    __syncthreads();
-    uint32_t carry = 0, tot = 0;
-#pragma unroll
-    for (int i = 0; i < 4; ++i) {
-        uint32_t t = lds4[i];
-        if (i < w) carry += t;
-        tot += t;
-    }
-    __syncthreads();
-    *block_total = tot;
-    return carry + inc - v;
-}
-
 // Single-pass exclusive scan of NC interleaved components: tile = ticket,
 // local blocked scan, decoupled look-back per component, write.
 // part: [ctl 4 words][status: tiles x NC words of 64 bits].
 template <int NC>
 __global__ __launch_bounds__(SCAN_THREADS) void k_scan1(const uint32_t *in, uint32_t *out, int64_t n,
-                                                        uint32_t *__restrict__ part, uint32_t epoch,
-                                                        uint32_t *__restrict__ total)
+                                                        uint32_t *__restrict__ part, uint32_t *__restrict__ total)
 {
     __shared__ uint32_t tile_s[NC][SCAN_TILE];
     __shared__ uint32_t lds4[4];
-    __shared__ uint32_t s_ticket;
+    __shared__ uint32_t s_ticket[2];
     __shared__ uint32_t s_agg[NC];
     __shared__ uint32_t s_pre[NC];
     unsigned int *ctl = part;
     uint64_t *st = (uint64_t *)(part + 4);
-    const uint32_t tag = lb_tag(epoch);
-    const int64_t tile = lb_ticket(ctl, &s_ticket);
+    uint32_t tag;
+    const int64_t tile = lb_ticket(ctl, s_ticket, &tag);
     const int64_t ntiles = (n + SCAN_TILE - 1) / SCAN_TILE;
     const int64_t base = tile * SCAN_TILE;
+    // all loads in flight first (a tail item reads the tile's first element
+    // and counts as 0)
+    uint32_t ld[NC][SCAN_IPT];
 #pragma unroll
-    for (int c = 0; c < NC; ++c) {
+    for (int c = 0; c < NC; ++c)
 #pragma unroll
         for (int i = 0; i < SCAN_IPT; ++i) {
             const int64_t idx = base + i * SCAN_THREADS + threadIdx.x;
-            tile_s[c][i * SCAN_THREADS + threadIdx.x] = idx < n ? in[idx * NC + c] : 0u;
+            ld[c][i] = in[(idx < n ? idx : base) * NC + c];
         }
-    }
+#pragma unroll
+    for (int c = 0; c < NC; ++c)
+#pragma unroll
+        for (int i = 0; i < SCAN_IPT; ++i) {
+            const int64_t idx = base + i * SCAN_THREADS + threadIdx.x;
+            tile_s[c][i * SCAN_THREADS + threadIdx.x] = idx < n ? ld[c][i] : 0u;
+        }
     __syncthreads();
 #pragma unroll
     for (int c = 0; c < NC; ++c) {
@@ -127,10 +114,17 @@ __global__ __launch_bounds__(RS_THREADS) void k_rs_hist(const uint64_t *__restri
     cnt[threadIdx.x] = 0;
     __syncthreads();
     const int64_t base = (int64_t)blockIdx.x * RS_TILE;
-    for (int i = threadIdx.x; i < RS_TILE; i += RS_THREADS) {
-        int64_t idx = base + i;
-        if (idx < n) atomicAdd(&cnt[(uint32_t)(keys[idx] >> shift) & 255u], 1u);
+    // all loads in flight before the first count (a tail item reads the
+    // tile's first key and is not counted)
+    uint64_t k[RS_TILE / RS_THREADS];
+#pragma unroll
+    for (int j = 0; j < RS_TILE / RS_THREADS; ++j) {
+        const int64_t idx = base + j * RS_THREADS + threadIdx.x;
+        k[j] = keys[idx < n ? idx : base];
     }
+#pragma unroll
+    for (int j = 0; j < RS_TILE / RS_THREADS; ++j)
+        if (base + j * RS_THREADS + threadIdx.x < n) atomicAdd(&cnt[(uint32_t)(k[j] >> shift) & 255u], 1u);
     __syncthreads();
     hist[(int64_t)threadIdx.x * nblocks + blockIdx.x] = cnt[threadIdx.x];
 }
@@ -204,12 +198,17 @@ __global__ __launch_bounds__(RS_THREADS) void k_rs_scatter(const uint64_t *__res
     uint32_t val[RS_IPT];
     uint32_t rnk[RS_IPT];
     const uint64_t lt = lanemask_lt();
+    // every load issued before the ranking starts: out-of-range items read
+    // the tile's first pair, then become ~0 keys (last digit, ranked last)
 #pragma unroll
     for (int j = 0; j < RS_IPT; ++j) {
-        int64_t idx = wbase + j * 64 + lane;
-        bool ok = idx < n;
-        key[j] = ok ? kin[idx] : ~0ull;
-        val[j] = ok ? vin[idx] : 0u;
+        const int64_t idx = wbase + j * 64 + lane;
+        key[j] = kin[idx < n ? idx : base];
+        val[j] = vin[idx < n ? idx : base];
+    }
+#pragma unroll
+    for (int j = 0; j < RS_IPT; ++j) {
+        if (wbase + j * 64 + lane >= n) key[j] = ~0ull;
         uint32_t d = (uint32_t)(key[j] >> shift) & 255u;
         uint64_t peers = wave_match_digit8(d, ~0ull);
         int leader = __ffsll((long long)peers) - 1;
@@ -257,12 +256,6 @@ __global__ __launch_bounds__(RS_THREADS) void k_rs_scatter(const uint64_t *__res
 
 }  // namespace
 
-uint32_t lb_next_epoch()
-{
-    static std::atomic<uint32_t> epoch{1};
-    return epoch.fetch_add(1u, std::memory_order_relaxed);
-}
-
 size_t scan_part_words(int64_t n, int nc)
 {
     const int64_t tiles = (n + SCAN_TILE - 1) / SCAN_TILE;
@@ -277,11 +270,10 @@ hipError_t scan_exclusive_u32(const uint32_t *in, uint32_t *out, int64_t n, int 
         return hipGetLastError();
     }
     const int64_t nb = (n + SCAN_TILE - 1) / SCAN_TILE;
-    const uint32_t epoch = lb_next_epoch();
     if (nc == 1)
-        k_scan1<1><<<(unsigned)nb, SCAN_THREADS, 0, s>>>(in, out, n, part, epoch, total);
+        k_scan1<1><<<(unsigned)nb, SCAN_THREADS, 0, s>>>(in, out, n, part, total);
     else if (nc == 2)
-        k_scan1<2><<<(unsigned)nb, SCAN_THREADS, 0, s>>>(in, out, n, part, epoch, total);
+        k_scan1<2><<<(unsigned)nb, SCAN_THREADS, 0, s>>>(in, out, n, part, total);
     else
         return hipErrorInvalidValue;
     return hipGetLastError();

@@ -18,6 +18,7 @@
 // slot, and a second small kernel adds each row's carry chain in wave order.
 // All reductions have a fixed association order: y is bitwise reproducible.
 #include "dcsr_internal.h"
+#include "lookback.h"
 #include "prims.h"
 #include "segments.h"
 
@@ -44,6 +45,81 @@ __global__ void k_rowmap(uint32_t R, const uint32_t *__restrict__ foff, uint32_t
         if (foff[mid + 1] >= e) hi = mid; else lo = mid + 1;
     }
     rowmap[i] = lo;
+}
+
+// Flat offsets and row map of the row lengths in one single-pass scan:
+// foff[r] = len[0] + ... + len[r-1] (foff[R] = total) and rowmap[g] as
+// k_rowmap defines it, for g <= total / GRAN: row r owns the granules g with
+// foff[r] < g*GRAN <= foff[r+1], and granule 0 belongs to row 0.  Tiles of
+// 8192 rows (32 contiguous rows per thread, vector loads and stores), so a
+// matrix of a few hundred thousand rows has a few dozen tiles and the
+// cross-tile look-back (lookback.h) resolves in about one round.
+constexpr int SR_IPT = 32;
+constexpr int SR_TILE = TB * SR_IPT;
+__global__ __launch_bounds__(TB) void k_scan_rows(uint32_t R, const int32_t *__restrict__ len,
+                                                  uint32_t *__restrict__ foff, uint32_t *__restrict__ rowmap,
+                                                  uint32_t *__restrict__ part)
+{
+    __shared__ uint32_t lds4[4], s_tk[2], s_pre;
+    unsigned int *ctl = part;
+    uint64_t *st = (uint64_t *)(part + 4);
+    uint32_t tag;
+    const int64_t tile = lb_ticket(ctl, s_tk, &tag);
+    const int64_t r0 = tile * SR_TILE + (int64_t)threadIdx.x * SR_IPT;
+    const bool full = r0 + SR_IPT <= (int64_t)R;
+    uint32_t v[SR_IPT];
+    if (full) {
+        const int4 *p = (const int4 *)(len + r0);
+#pragma unroll
+        for (int q = 0; q < SR_IPT / 4; ++q) {
+            const int4 t = p[q];
+            v[4 * q] = (uint32_t)t.x;
+            v[4 * q + 1] = (uint32_t)t.y;
+            v[4 * q + 2] = (uint32_t)t.z;
+            v[4 * q + 3] = (uint32_t)t.w;
+        }
+    } else {
+#pragma unroll
+        for (int j = 0; j < SR_IPT; ++j) v[j] = r0 + j < (int64_t)R ? (uint32_t)len[r0 + j] : 0u;
+    }
+    uint32_t sum = 0;
+#pragma unroll
+    for (int j = 0; j < SR_IPT; ++j) sum += v[j];
+    uint32_t tot;
+    const uint32_t pre = block_excl_scan_256(sum, lds4, &tot);
+    if (threadIdx.x == 0) lb_publish(&st[tile], tag, tile == 0 ? LB_INC : LB_AGG, tot);
+    if (threadIdx.x < 64) {
+        uint32_t p[1] = {0u};
+        if (tile > 0) lb_wave_lookback<1>(st, tile, tag, ctl, p);
+        if (threadIdx.x == 0) {
+            s_pre = p[0];
+            if (tile > 0) lb_publish(&st[tile], tag, LB_INC, p[0] + tot);
+        }
+    }
+    __syncthreads();
+    uint32_t a = s_pre + pre;
+    uint32_t o[SR_IPT];
+#pragma unroll
+    for (int j = 0; j < SR_IPT; ++j) {
+        o[j] = a;
+        const uint32_t b = a + v[j];
+        for (uint32_t g = a / GRAN + 1; g <= b / GRAN; ++g) rowmap[g] = (uint32_t)(r0 + j);
+        a = b;
+    }
+    if (full) {
+        uint4 *q = (uint4 *)(foff + r0);
+#pragma unroll
+        for (int k = 0; k < SR_IPT / 4; ++k) q[k] = make_uint4(o[4 * k], o[4 * k + 1], o[4 * k + 2], o[4 * k + 3]);
+    } else {
+#pragma unroll
+        for (int j = 0; j < SR_IPT; ++j)
+            if (r0 + j < (int64_t)R) foff[r0 + j] = o[j];
+    }
+    // the thread holding row R-1 stores the total
+    if (r0 < (int64_t)R && r0 + SR_IPT >= (int64_t)R) foff[R] = a;
+    if (tile == 0 && threadIdx.x == 0) rowmap[0] = 0u;
+    __syncthreads();
+    if (threadIdx.x == 0) lb_finish(ctl);
 }
 
 __device__ __forceinline__ float combine(float alpha, float acc, float beta, const float *y, uint64_t i)
@@ -525,11 +601,14 @@ __global__ __launch_bounds__(TB) void k_spmm_wide4(uint32_t R, const uint32_t *_
 
 // Carry chains: consecutive slots (waves) carrying into the same row, added
 // to the row's head value.  A wave scans 64 slots.  Chains shorter than 64
-// slots are summed by lane groups in parallel (64/KB groups, one lane per
-// column, one chain per group, serially in slot order); a chain of 64 or
-// more slots (a hub row spread over many waves) is summed by the whole wave,
-// 64 slots at a time, then reduced in a fixed tree.  Either way the order
-// depends only on the chain, so results are reproducible.
+// slots: with one column (SpMV) a segmented wave scan sums every chain of
+// the 128 slots from the wave's base at once (a chain whose head is in the
+// wave ends within them); with KB columns, lane groups sum them (64/KB
+// groups, one lane per column, one chain per group, serially in slot
+// order).  A chain of 64 or more slots (a hub row spread over many waves) is
+// summed by the whole wave, 64 slots at a time, then reduced in a fixed
+// tree.  Either way the order depends only on the chain, so results are
+// reproducible.
 template <int KB>
 __global__ __launch_bounds__(TB) void k_carry(uint32_t nunits, const uint32_t *__restrict__ carry_row,
                                               const float *__restrict__ carry_val, float *__restrict__ Y,
@@ -544,31 +623,72 @@ __global__ __launch_bounds__(TB) void k_carry(uint32_t nunits, const uint32_t *_
     const uint64_t w = base + lane;
     uint32_t r = NONE;
     bool head = false, lng = false;
-    if (w < nunits) {
-        r = carry_row[w];
-        head = r != NONE && (w == 0 || carry_row[w - 1] != r);
-        // chains are contiguous: 64+ slots iff slot w+63 still carries r
-        lng = head && w + 63 < nunits && carry_row[w + 63] == r;
-    }
-    const uint64_t longs = __ballot(lng);
-    const uint64_t shorts = __ballot(head) & ~longs;
-    const int nshort = __popcll(shorts);
-    for (int i0 = 0; i0 < nshort; i0 += G) {
-        const int i = i0 + g;
-        // the i-th short head of this wave
-        uint64_t m = shorts;
-        for (int t = 0; t < i && m; ++t) m &= m - 1;
-        const int hl = m ? __ffsll((long long)m) - 1 : 0;
-        const uint32_t hr = (uint32_t)__shfl((int)r, hl);
-        if (i < nshort) {
-            const uint64_t hw = base + hl;
-            float acc = 0.0f;
-            for (uint64_t k = hw; k < nunits && carry_row[k] == hr; ++k) acc = acc + carry_val[k * KB + cl];
-            const uint64_t yi = (uint64_t)hr * ldy + cl;
-            Y[yi] = Y[yi] + alpha * acc;
+    if constexpr (KB == 1) {
+        // every load first: this wave's slots, the next 64, the slot before
+        const uint64_t w2 = w + 64;
+        r = w < nunits ? carry_row[w] : NONE;
+        const uint32_t r2 = w2 < nunits ? carry_row[w2] : NONE;
+        const uint32_t rb = base > 0 ? carry_row[base - 1] : NONE;
+        float v = w < nunits ? carry_val[w] : 0.0f;
+        float v2 = w2 < nunits ? carry_val[w2] : 0.0f;
+        if (r == NONE) v = 0.0f;
+        if (r2 == NONE) v2 = 0.0f;
+        // every lane takes part in the shuffle (a ternary would skip lane 0,
+        // and a shuffle from an inactive lane reads 0)
+        const uint32_t ru = (uint32_t)__shfl_up((int)r, 1);
+        const uint32_t rl = lane == 0 ? rb : ru;
+        head = r != NONE && rl != r;
+        // slot w + 63: lane 63 of this wave for lane 0, else lane - 1 of the next
+        const uint32_t r63 = (uint32_t)__shfl((int)r, 63);
+        const uint32_t r2p = (uint32_t)__shfl((int)r2, (lane + 63) & 63);
+        lng = head && (lane == 0 ? r63 : r2p) == r;
+        const uint64_t longs = __ballot(lng);
+        const uint64_t heads = __ballot(head);
+        const uint64_t starts = __ballot(r != NONE && (lane == 0 || rl != r));
+        const float s1 = wave_seg_incl_scan_f32(v, r);
+        const float s2 = wave_seg_incl_scan_f32(v2, r2);
+        // the sum of the next 64 slots' first run (a chain continuing past this wave)
+        const uint32_t r20 = (uint32_t)__shfl((int)r2, 0);
+        const int e2 = __popcll(__ballot(r2 == r20)) - 1;
+        const float t2 = __shfl(s2, e2 < 0 ? 0 : e2);
+        const uint32_t rn = (uint32_t)__shfl_down((int)r, 1);
+        if (r != NONE && (lane == 63 || rn != r)) {
+            // this lane ends a run; the run started at the highest start at or below it
+            const uint64_t upto = lane == 63 ? ~0ull : ((2ull << lane) - 1ull);
+            const int f = 63 - __clzll(starts & upto);
+            if (((heads >> f) & 1ull) && !((longs >> f) & 1ull)) {
+                const float acc = lane == 63 && r20 == r ? s1 + t2 : s1;
+                const uint64_t yi = (uint64_t)r * ldy;
+                Y[yi] = Y[yi] + alpha * acc;
+            }
+        }
+    } else {
+        if (w < nunits) {
+            r = carry_row[w];
+            head = r != NONE && (w == 0 || carry_row[w - 1] != r);
+            // chains are contiguous: 64+ slots iff slot w+63 still carries r
+            lng = head && w + 63 < nunits && carry_row[w + 63] == r;
+        }
+        const uint64_t longs = __ballot(lng);
+        const uint64_t shorts = __ballot(head) & ~longs;
+        const int nshort = __popcll(shorts);
+        for (int i0 = 0; i0 < nshort; i0 += G) {
+            const int i = i0 + g;
+            // the i-th short head of this wave
+            uint64_t m = shorts;
+            for (int t = 0; t < i && m; ++t) m &= m - 1;
+            const int hl = m ? __ffsll((long long)m) - 1 : 0;
+            const uint32_t hr = (uint32_t)__shfl((int)r, hl);
+            if (i < nshort) {
+                const uint64_t hw = base + hl;
+                float acc = 0.0f;
+                for (uint64_t k = hw; k < nunits && carry_row[k] == hr; ++k) acc = acc + carry_val[k * KB + cl];
+                const uint64_t yi = (uint64_t)hr * ldy + cl;
+                Y[yi] = Y[yi] + alpha * acc;
+            }
         }
     }
-    uint64_t lm = longs;
+    uint64_t lm = __ballot(lng);
     while (lm) {
         const int hl = __ffsll((long long)lm) - 1;
         lm &= lm - 1;
@@ -734,19 +854,17 @@ dcsr_status ensure_flat(dcsr_handle h, hipStream_t s)
     const int64_t R = h->rows;
     uint32_t *foff, *part;
     DCSR_TRY(ws_get(h, WS_FOFF, (size_t)(R + 1) * 4, (void **)&foff));
-    DCSR_TRY(ws_get(h, WS_PART, scan_part_words(R + 1, 1) * 4, (void **)&part));
-    // row lengths are non-negative int32: scan them as uint32 in place
-    DCSR_HIP(scan_exclusive_u32((const uint32_t *)h->row_len, foff, R, 1, part, foff + R, s));
-    // the grid covers an upper bound of the live entries (the kernel stops
-    // at the device's count), so no read-back is needed after NOSYNC updates
+    // look-back block: 4 control words and one 64-bit status word per tile
+    const int64_t tiles = (R + SR_TILE - 1) / SR_TILE;
+    DCSR_TRY(ws_get(h, WS_PART, (size_t)(4 + 2 * tiles) * 4, (void **)&part));
+    // the row map covers the live entries, which the pool bounds: sized by
+    // the pool, it needs no reallocation while the matrix grows within it
+    // (no read-back, also after NOSYNC updates)
     const int64_t live = live_bound(h);
-    const int64_t ng = live / GRAN + 1;
     uint32_t *rowmap;
-    // sized by the pool (an upper bound on nnz): no reallocation while the
-    // matrix grows within its pool
     const int64_t ngcap = (h->pool_cap > live ? h->pool_cap : live) / GRAN + 1;
     DCSR_TRY(ws_get(h, WS_ROWMAP, (size_t)ngcap * 4, (void **)&rowmap));
-    k_rowmap<<<grid_for(ng, TB), TB, 0, s>>>((uint32_t)R, foff, rowmap, (uint32_t)ng);
+    if (R > 0) k_scan_rows<<<(unsigned)tiles, TB, 0, s>>>((uint32_t)R, h->row_len, foff, rowmap, part);
     DCSR_HIP(hipGetLastError());
     h->flat_dirty = false;
     return DCSR_OK;

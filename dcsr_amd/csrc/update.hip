@@ -47,26 +47,6 @@ enum { C_NOP = 0, C_ADD = 1, C_DEL = 2, C_INS = 3 };
 enum { M_VALUES = 0, M_REWRITE = 1 };
 constexpr int NPLAN = 5;  // per-block plan counters
 
-// Block-wide exclusive scan of one uint32 per thread (256 threads);
-// *block_total receives the block sum.
-__device__ __forceinline__ uint32_t block_excl_scan_256(uint32_t v, uint32_t *lds4, uint32_t *block_total)
-{
-    const int w = threadIdx.x >> 6;
-    const uint32_t inc = wave_incl_scan_u32(v);
-    if (lane_id() == 63) lds4[w] = inc;
-    __syncthreads();
-    uint32_t carry = 0, tot = 0;
-#pragma unroll
-    for (int i = 0; i < 4; ++i) {
-        const uint32_t t = lds4[i];
-        if (i < w) carry += t;
-        tot += t;
-    }
-    __syncthreads();
-    *block_total = tot;
-    return carry + inc - v;
-}
-
 // Pack the caller's triples into sort keys (row << cbits | col) and value
 // bits, one RADIX_TILE tile per workgroup.  Invalid indices become ~0 keys:
 // with one key bit above row|col they sort strictly after every valid key,
@@ -86,15 +66,27 @@ __global__ __launch_bounds__(TB) void k_upd_pack(const int32_t *__restrict__ row
     }
     const int64_t base = (int64_t)blockIdx.x * RADIX_TILE;
     bool bad = false;
+    // all loads in flight first (a tail item reads the tile's first triple
+    // and is dropped)
+    int32_t rr[RADIX_TILE / TB], cc[RADIX_TILE / TB];
+    uint32_t vv[RADIX_TILE / TB];
+#pragma unroll
+    for (int j = 0; j < RADIX_TILE / TB; ++j) {
+        const int64_t i = base + j * TB + threadIdx.x;
+        const int64_t ci = i < n ? i : base;
+        rr[j] = rows[ci];
+        cc[j] = cols[ci];
+        vv[j] = __float_as_uint(vals[ci]);
+    }
 #pragma unroll
     for (int j = 0; j < RADIX_TILE / TB; ++j) {
         const int64_t i = base + j * TB + threadIdx.x;
         if (i < n) {
-            const int32_t r = rows[i], c = cols[i];
+            const int32_t r = rr[j], c = cc[j];
             const bool b = (r < 0) | (r >= R) | (c < 0) | (c >= C);
             const uint64_t k = b ? ~0ull : (((uint64_t)(uint32_t)r << cbits) | (uint32_t)c);
             key[i] = k;
-            val[i] = __float_as_uint(vals[i]);
+            val[i] = vv[j];
             if (hist) atomicAdd(&cnt[(uint32_t)k & 255u], 1u);
             bad |= b;
         }
@@ -205,9 +197,15 @@ __device__ void block_scan_columns(uint32_t *const (&tab)[NC], const int (&src)[
 __device__ __forceinline__ void load_keys(const uint64_t *__restrict__ key, int64_t n, int64_t i0,
                                           uint64_t (&k)[IPT], uint64_t &prev)
 {
-    prev = (i0 > 0 && i0 - 1 < n) ? key[i0 - 1] : ~0ull;
+    // unconditional loads (a tail item reads the last key and becomes ~0),
+    // all in flight together
+    prev = key[i0 > 0 ? (i0 - 1 < n ? i0 - 1 : n - 1) : 0];
 #pragma unroll
-    for (int j = 0; j < IPT; ++j) k[j] = i0 + j < n ? key[i0 + j] : ~0ull;
+    for (int j = 0; j < IPT; ++j) k[j] = key[i0 + j < n ? i0 + j : n - 1];
+    if (!(i0 > 0 && i0 - 1 < n)) prev = ~0ull;
+#pragma unroll
+    for (int j = 0; j < IPT; ++j)
+        if (i0 + j >= n) k[j] = ~0ull;
 }
 
 // Heads of the sorted batch: an item is a coordinate head if it is the
@@ -282,7 +280,7 @@ __global__ __launch_bounds__(TB) void k_upd_heads_emit(const uint64_t *__restric
     load_keys(key, n, i0, k, prev);
     uint32_t vb[IPT];
 #pragma unroll
-    for (int j = 0; j < IPT; ++j) vb[j] = i0 + j < n ? val[i0 + j] : 0u;
+    for (int j = 0; j < IPT; ++j) vb[j] = val[i0 + j < n ? i0 + j : n - 1];
     const HeadFlags f = head_flags(k, prev, cbits);
     uint32_t htot, rtot;
     uint32_t lu = block_excl_scan_256(__popc(f.h), lds4, &htot);

@@ -98,6 +98,26 @@ __device__ __forceinline__ uint64_t wave_match_digit8(uint32_t d, uint64_t activ
     return peers;
 }
 
+// Block-wide exclusive scan of one uint32 per thread (256 threads, four
+// waves); *block_total receives the block sum.  lds4: four words of LDS.
+__device__ __forceinline__ uint32_t block_excl_scan_256(uint32_t v, uint32_t *lds4, uint32_t *block_total)
+{
+    const int w = threadIdx.x >> 6;
+    const uint32_t inc = wave_incl_scan_u32(v);
+    if (lane_id() == 63) lds4[w] = inc;
+    __syncthreads();
+    uint32_t carry = 0, tot = 0;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        const uint32_t t = lds4[i];
+        if (i < w) carry += t;
+        tot += t;
+    }
+    __syncthreads();
+    *block_total = tot;
+    return carry + inc - v;
+}
+
 // Fixed-order float sum over the wave (tree over lane index).
 __device__ __forceinline__ float wave_sum_f32(float v)
 {
