@@ -51,10 +51,12 @@ __global__ void k_rowmap(uint32_t R, const uint32_t *__restrict__ foff, uint32_t
 // foff[r] = len[0] + ... + len[r-1] (foff[R] = total) and rowmap[g] as
 // k_rowmap defines it, for g <= total / GRAN: row r owns the granules g with
 // foff[r] < g*GRAN <= foff[r+1], and granule 0 belongs to row 0.  Tiles of
-// 8192 rows (32 contiguous rows per thread, vector loads and stores), so a
-// matrix of a few hundred thousand rows has a few dozen tiles and the
-// cross-tile look-back (lookback.h) resolves in about one round.
-constexpr int SR_IPT = 32;
+// 4096 rows (16 contiguous rows per thread, vector loads and stores): a
+// matrix of a few hundred thousand rows has under a hundred tiles, so the
+// cross-tile look-back (lookback.h) resolves in one or two rounds.  Measured
+// on 325K rows against 8 and 32 rows per thread: 55 vs 57 and 59 us per
+// SpMV after an update.
+constexpr int SR_IPT = 16;
 constexpr int SR_TILE = TB * SR_IPT;
 __global__ __launch_bounds__(TB) void k_scan_rows(uint32_t R, const int32_t *__restrict__ len,
                                                   uint32_t *__restrict__ foff, uint32_t *__restrict__ rowmap,
