@@ -86,3 +86,39 @@ def test_spmm_after_updates(setup):
     Y = m.spmm(torch.from_numpy(X).cuda())
     torch.cuda.synchronize()
     check_spmm(A2, X, np.zeros((R, 16), np.float32), Y.cpu().numpy(), 1.0, 0.0)
+
+
+@pytest.fixture(scope="module")
+def wide_setup():
+    """A matrix whose dense operand exceeds one XCD's 4 MB L2 even for one
+    column (1.2M columns): SpMV and SpMM stream its entries with
+    non-temporal loads.  Power-law rows, so carries and hub rows occur."""
+    import torch
+    import dcsr_amd
+    dcsr_amd.lib()
+    R, C = 3000, 1_200_000
+    rng = np.random.default_rng(17)
+    d = np.minimum(rng.zipf(1.7, R), 20000)
+    rows = np.repeat(np.arange(R, dtype=np.int64), d)
+    cols = rng.integers(0, C, rows.size)
+    key = np.unique(rows * C + cols)
+    r, c = (key // C).astype(np.int32), (key % C).astype(np.int32)
+    v = rng.uniform(-1, 1, r.size).astype(np.float32)
+    m = dcsr_amd.DcsrMatrix(R, C)
+    m.LoadMatrix(r, c, v)
+    yield torch, m, O.coo_to_csr(R, C, r, c, v)
+    m.close()
+
+
+@pytest.mark.parametrize("k", [1, 3, 4, 16, 32])
+def test_spmm_streamed_entries(wide_setup, k):
+    torch, m, A = wide_setup
+    rng = np.random.default_rng(200 + k)
+    ldx = k + (1 if k == 16 else 0)  # k = 16 with an unaligned row: the scalar wide kernel
+    Xfull = rng.uniform(-1, 1, (A.cols, ldx)).astype(np.float32)
+    X = Xfull[:, :k]
+    say(f"streamed spmm k={k} ldx={ldx} cols={A.cols}")
+    Xd = torch.from_numpy(Xfull).cuda()[:, :k]
+    Y = m.spmv(Xd[:, 0]).unsqueeze(1) if k == 1 else m.spmm(Xd)
+    torch.cuda.synchronize()
+    check_spmm(A, X, np.zeros((A.rows, k), np.float32), Y.cpu().numpy(), 1.0, 0.0)
