@@ -38,6 +38,7 @@ constexpr int WAVE_GB = 10;           // generation bits of the wave kernel's ke
 constexpr int64_t WAVE_MAX_COLS32 = (1ll << (32 - WAVE_GB)) - 1;
 constexpr uint32_t WAVE_MAX = 1024;   // partials per row for the wave kernel
 constexpr uint32_t BLOCK_MAX = 4096;  // partials per row for the workgroup kernel
+constexpr int SG_EX = 8;              // partials per lane whose loads are in flight together
 // row classes (list index): wave, workgroup, hub, team
 enum { CL_WAVE = 0, CL_BLOCK = 1, CL_HUB = 2, CL_TEAM = 3, NCLS = 4 };
 
@@ -338,14 +339,29 @@ __device__ void sg_row(const SgRowInfo &ri, const int32_t *__restrict__ a_col, c
         s_sb[t] = sb;
         s_va[t] = va;
         team_sync<NT>();
-        for (uint32_t e0 = 0; e0 < tot; e0 += NT) {
-            const uint32_t pe = e0 + t;
-            if (pe < tot) {
-                const uint32_t q = find_seg_team<NT>(s_ex, pe);
-                const uint64_t bp = (uint64_t)s_sb[q] + (pe - s_ex[q]);
-                const uint32_t g = gbase + pe;
-                key[g] = (K)(((uint64_t)(uint32_t)b_col[bp] << GB) | g);
-                val[g] = s_va[q] * b_val[bp];
+        // up to SG_EX partials per lane per round, every load of a round
+        // issued before the first wait (a lane past tot reads entry 0 of B
+        // and writes nothing)
+        for (uint32_t e0 = 0; e0 < tot; e0 += SG_EX * NT) {
+            int32_t bc[SG_EX];
+            float bv[SG_EX], av[SG_EX];
+#pragma unroll
+            for (int i = 0; i < SG_EX; ++i) {
+                const uint32_t pe = e0 + (uint32_t)i * NT + t;
+                const uint32_t q = find_seg_team<NT>(s_ex, pe < tot ? pe : 0u);
+                const uint64_t bp = pe < tot ? (uint64_t)s_sb[q] + (pe - s_ex[q]) : 0u;
+                bc[i] = b_col[bp];
+                bv[i] = b_val[bp];
+                av[i] = s_va[q];
+            }
+#pragma unroll
+            for (int i = 0; i < SG_EX; ++i) {
+                const uint32_t pe = e0 + (uint32_t)i * NT + t;
+                if (pe < tot) {
+                    const uint32_t g = gbase + pe;
+                    key[g] = (K)(((uint64_t)(uint32_t)bc[i] << GB) | g);
+                    val[g] = av[i] * bv[i];
+                }
             }
         }
         gbase += tot;
