@@ -40,7 +40,7 @@ constexpr uint32_t WAVE_MAX = 1024;   // partials per row for the wave kernel
 constexpr uint32_t BLOCK_MAX = 4096;  // partials per row for the workgroup kernel
 constexpr int SG_EX = 8;              // partials per lane whose loads are in flight together
 // row classes (list index): wave, workgroup, hub, team
-enum { CL_WAVE = 0, CL_BLOCK = 1, CL_HUB = 2, CL_TEAM = 3, NCLS = 4 };
+enum { CL_WAVE = 0, CL_BLOCK = 1, CL_HUB = 2, CL_TEAM = 3, CL_TEAM8 = 4, NCLS = 5 };
 
 struct SgState {
     unsigned long long total;  // sum of P_i (64-bit)
@@ -113,9 +113,14 @@ __global__ __launch_bounds__(TB) void k_sg_classify(uint32_t R, const uint32_t *
     int cls = -1;
     if (r < R) {
         p = P[r];
-        cls = p == 0 ? -1
-                     : (p <= TEAM_MAX && team_ok ? CL_TEAM
-                                                 : (p <= WAVE_MAX ? CL_WAVE : (p <= BLOCK_MAX ? CL_BLOCK : CL_HUB)));
+        // team rows in two classes by rank-sort width (4 or 8 keys per lane),
+        // so the four teams of a wave run the same sort
+        cls = p == 0                               ? -1
+              : (p <= 4 * TEAM && team_ok)         ? CL_TEAM
+              : (p <= TEAM_MAX && team_ok)         ? CL_TEAM8
+              : p <= WAVE_MAX                      ? CL_WAVE
+              : p <= BLOCK_MAX                     ? CL_BLOCK
+                                                   : CL_HUB;
         if (p == 0) cnt[r] = 0;
     }
     // slot of this row inside its workgroup's share of its class list
@@ -311,7 +316,9 @@ __device__ __forceinline__ void team_rank_sort(K *key, float *val, uint32_t P)
     team_sync<NT>();
 }
 
-template <int NT, int GB, typename K>
+// KPL (teams only): keys per lane of the rank sort, fixed by the row's class
+// (P <= KPL * NT); 0 picks it per row.
+template <int NT, int GB, typename K, int KPL = 0>
 __device__ void sg_row(const SgRowInfo &ri, const int32_t *__restrict__ a_col, const float *__restrict__ a_val,
                        const uint32_t *__restrict__ b_start, const int32_t *__restrict__ b_len,
                        const int32_t *__restrict__ b_col, const float *__restrict__ b_val,
@@ -342,11 +349,12 @@ __device__ void sg_row(const SgRowInfo &ri, const int32_t *__restrict__ a_col, c
         // up to SG_EX partials per lane per round, every load of a round
         // issued before the first wait (a lane past tot reads entry 0 of B
         // and writes nothing)
-        for (uint32_t e0 = 0; e0 < tot; e0 += SG_EX * NT) {
-            int32_t bc[SG_EX];
-            float bv[SG_EX], av[SG_EX];
+        constexpr int EX = KPL ? KPL : SG_EX;
+        for (uint32_t e0 = 0; e0 < tot; e0 += EX * NT) {
+            int32_t bc[EX];
+            float bv[EX], av[EX];
 #pragma unroll
-            for (int i = 0; i < SG_EX; ++i) {
+            for (int i = 0; i < EX; ++i) {
                 const uint32_t pe = e0 + (uint32_t)i * NT + t;
                 const uint32_t q = find_seg_team<NT>(s_ex, pe < tot ? pe : 0u);
                 const uint64_t bp = pe < tot ? (uint64_t)s_sb[q] + (pe - s_ex[q]) : 0u;
@@ -355,7 +363,7 @@ __device__ void sg_row(const SgRowInfo &ri, const int32_t *__restrict__ a_col, c
                 av[i] = s_va[q];
             }
 #pragma unroll
-            for (int i = 0; i < SG_EX; ++i) {
+            for (int i = 0; i < EX; ++i) {
                 const uint32_t pe = e0 + (uint32_t)i * NT + t;
                 if (pe < tot) {
                     const uint32_t g = gbase + pe;
@@ -369,7 +377,9 @@ __device__ void sg_row(const SgRowInfo &ri, const int32_t *__restrict__ a_col, c
     }
     // 2. sort by (column, generation): rank sort for quarter-wave teams,
     // bitonic network over LDS for wider ones (padding sorts last)
-    if constexpr (NT < 64) {
+    if constexpr (NT < 64 && KPL != 0) {
+        team_rank_sort<NT, KPL, K>(key, val, P);
+    } else if constexpr (NT < 64) {
         if (P <= 4 * NT) team_rank_sort<NT, 4, K>(key, val, P);
         else team_rank_sort<NT, 8, K>(key, val, P);
     } else {
@@ -469,7 +479,9 @@ __global__ __launch_bounds__(TB) void k_sg_wave(const uint32_t *__restrict__ lis
             });
 }
 
-// Rows of at most TEAM_MAX partials: 16-lane teams, four rows per wave.
+// Rows of at most KPL * TEAM partials (class CL_TEAM: 4, CL_TEAM8: 8 keys
+// per lane): 16-lane teams, four rows per wave.
+template <int KPL>
 __global__ __launch_bounds__(TB) void k_sg_team(const uint32_t *__restrict__ list, const SgState *__restrict__ st,
                                                 const uint32_t *__restrict__ P, const uint32_t *__restrict__ Poff,
                                                 const uint32_t *__restrict__ a_start, const int32_t *__restrict__ a_len,
@@ -489,11 +501,12 @@ __global__ __launch_bounds__(TB) void k_sg_team(const uint32_t *__restrict__ lis
     __shared__ uint32_t s_ex[NTEAM][TEAM], s_sb[NTEAM][TEAM];
     __shared__ float s_va[NTEAM][TEAM];
     const int tm = threadIdx.x / TEAM;
-    const uint32_t n = st->count[CL_TEAM];
+    static_assert(KPL * TEAM <= TEAM_MAX, "a team row fits its LDS row");
+    const uint32_t n = st->count[KPL == 4 ? CL_TEAM : CL_TEAM8];
     sg_rows(list, n, blockIdx.x * NTEAM + tm, gridDim.x * NTEAM, P, Poff, a_start, a_len,
             [&](const SgRowInfo &ri) {
-                sg_row<TEAM, TEAM_GB, uint32_t>(ri, a_col, a_val, b_start, b_len, b_col, b_val, tcol, tval, cnt,
-                                                key[tm], val[tm], s_ex[tm], s_sb[tm], s_va[tm], nullptr);
+                sg_row<TEAM, TEAM_GB, uint32_t, KPL>(ri, a_col, a_val, b_start, b_len, b_col, b_val, tcol, tval,
+                                                     cnt, key[tm], val[tm], s_ex[tm], s_sb[tm], s_va[tm], nullptr);
             });
 }
 
@@ -702,13 +715,18 @@ dcsr_status dcsr_spgemm(dcsr_handle A, dcsr_handle B, dcsr_handle C, dcsr_stream
     float *tval = C->pool_val;
 
     // 2. numeric phases
-    if (hst.count[CL_TEAM]) {
+    for (int c : {CL_TEAM, CL_TEAM8}) {
+        if (!hst.count[c]) continue;
         const uint32_t per = TB / TEAM;
-        const uint32_t want = (hst.count[CL_TEAM] + per - 1) / per;
+        const uint32_t want = (hst.count[c] + per - 1) / per;
         const unsigned g = want < 8192 ? want : 8192;
-        k_sg_team<<<g, TB, 0, s>>>(lists + CL_TEAM * (uint64_t)(R + 1), st, P, Poff, A->row_start, A->row_len,
-                                   A->pool_col, A->pool_val, B->row_start, B->row_len, B->pool_col, B->pool_val, tcol,
-                                   tval, cnt);
+        const uint32_t *list = lists + c * (uint64_t)(R + 1);
+        if (c == CL_TEAM)
+            k_sg_team<4><<<g, TB, 0, s>>>(list, st, P, Poff, A->row_start, A->row_len, A->pool_col, A->pool_val,
+                                          B->row_start, B->row_len, B->pool_col, B->pool_val, tcol, tval, cnt);
+        else
+            k_sg_team<8><<<g, TB, 0, s>>>(list, st, P, Poff, A->row_start, A->row_len, A->pool_col, A->pool_val,
+                                          B->row_start, B->row_len, B->pool_col, B->pool_val, tcol, tval, cnt);
     }
     if (hst.count[CL_WAVE]) {
         const unsigned g = (unsigned)((hst.count[CL_WAVE] + 3) / 4 < 8192 ? (hst.count[CL_WAVE] + 3) / 4 : 8192);
