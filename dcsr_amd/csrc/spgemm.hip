@@ -40,7 +40,9 @@ constexpr uint32_t WAVE_MAX = 1024;   // partials per row for the wave kernel
 constexpr uint32_t BLOCK_MAX = 4096;  // partials per row for the workgroup kernel
 constexpr int SG_EX = 8;              // partials per lane whose loads are in flight together
 // row classes (list index): wave, workgroup, hub, team
-enum { CL_WAVE = 0, CL_BLOCK = 1, CL_HUB = 2, CL_TEAM = 3, CL_TEAM8 = 4, NCLS = 5 };
+// team classes CL_TEAM + i: rows of at most 2 (i + 1) * TEAM partials, sorted
+// with 2 (i + 1) keys per lane
+enum { CL_WAVE = 0, CL_BLOCK = 1, CL_HUB = 2, CL_TEAM = 3, NTEAMCL = 4, NCLS = CL_TEAM + NTEAMCL };
 
 struct SgState {
     unsigned long long total;  // sum of P_i (64-bit)
@@ -113,14 +115,14 @@ __global__ __launch_bounds__(TB) void k_sg_classify(uint32_t R, const uint32_t *
     int cls = -1;
     if (r < R) {
         p = P[r];
-        // team rows in two classes by rank-sort width (4 or 8 keys per lane),
-        // so the four teams of a wave run the same sort
-        cls = p == 0                               ? -1
-              : (p <= 4 * TEAM && team_ok)         ? CL_TEAM
-              : (p <= TEAM_MAX && team_ok)         ? CL_TEAM8
-              : p <= WAVE_MAX                      ? CL_WAVE
-              : p <= BLOCK_MAX                     ? CL_BLOCK
-                                                   : CL_HUB;
+        // team rows in classes by rank-sort width (2, 4, 6 or 8 keys per
+        // lane), so the four teams of a wave run the same sort over similar
+        // row sizes
+        cls = p == 0                         ? -1
+              : (p <= TEAM_MAX && team_ok)   ? CL_TEAM + (int)((p - 1) / (2 * TEAM))
+              : p <= WAVE_MAX                ? CL_WAVE
+              : p <= BLOCK_MAX               ? CL_BLOCK
+                                             : CL_HUB;
         if (p == 0) cnt[r] = 0;
     }
     // slot of this row inside its workgroup's share of its class list
@@ -479,8 +481,8 @@ __global__ __launch_bounds__(TB) void k_sg_wave(const uint32_t *__restrict__ lis
             });
 }
 
-// Rows of at most KPL * TEAM partials (class CL_TEAM: 4, CL_TEAM8: 8 keys
-// per lane): 16-lane teams, four rows per wave.
+// Rows of at most KPL * TEAM partials (class CL_TEAM + KPL / 2 - 1): 16-lane
+// teams, four rows per wave.
 template <int KPL>
 __global__ __launch_bounds__(TB) void k_sg_team(const uint32_t *__restrict__ list, const SgState *__restrict__ st,
                                                 const uint32_t *__restrict__ P, const uint32_t *__restrict__ Poff,
@@ -502,7 +504,7 @@ __global__ __launch_bounds__(TB) void k_sg_team(const uint32_t *__restrict__ lis
     __shared__ float s_va[NTEAM][TEAM];
     const int tm = threadIdx.x / TEAM;
     static_assert(KPL * TEAM <= TEAM_MAX, "a team row fits its LDS row");
-    const uint32_t n = st->count[KPL == 4 ? CL_TEAM : CL_TEAM8];
+    const uint32_t n = st->count[CL_TEAM + KPL / 2 - 1];
     sg_rows(list, n, blockIdx.x * NTEAM + tm, gridDim.x * NTEAM, P, Poff, a_start, a_len,
             [&](const SgRowInfo &ri) {
                 sg_row<TEAM, TEAM_GB, uint32_t, KPL>(ri, a_col, a_val, b_start, b_len, b_col, b_val, tcol, tval,
@@ -715,18 +717,22 @@ dcsr_status dcsr_spgemm(dcsr_handle A, dcsr_handle B, dcsr_handle C, dcsr_stream
     float *tval = C->pool_val;
 
     // 2. numeric phases
-    for (int c : {CL_TEAM, CL_TEAM8}) {
+    for (int c = CL_TEAM; c < CL_TEAM + NTEAMCL; ++c) {
         if (!hst.count[c]) continue;
         const uint32_t per = TB / TEAM;
         const uint32_t want = (hst.count[c] + per - 1) / per;
         const unsigned g = want < 8192 ? want : 8192;
         const uint32_t *list = lists + c * (uint64_t)(R + 1);
-        if (c == CL_TEAM)
-            k_sg_team<4><<<g, TB, 0, s>>>(list, st, P, Poff, A->row_start, A->row_len, A->pool_col, A->pool_val,
-                                          B->row_start, B->row_len, B->pool_col, B->pool_val, tcol, tval, cnt);
-        else
-            k_sg_team<8><<<g, TB, 0, s>>>(list, st, P, Poff, A->row_start, A->row_len, A->pool_col, A->pool_val,
-                                          B->row_start, B->row_len, B->pool_col, B->pool_val, tcol, tval, cnt);
+#define DCSR_SG_TEAM(K)                                                                                            \
+    k_sg_team<K><<<g, TB, 0, s>>>(list, st, P, Poff, A->row_start, A->row_len, A->pool_col, A->pool_val, B->row_start, \
+                                  B->row_len, B->pool_col, B->pool_val, tcol, tval, cnt)
+        switch (c - CL_TEAM) {
+        case 0: DCSR_SG_TEAM(2); break;
+        case 1: DCSR_SG_TEAM(4); break;
+        case 2: DCSR_SG_TEAM(6); break;
+        default: DCSR_SG_TEAM(8); break;
+        }
+#undef DCSR_SG_TEAM
     }
     if (hst.count[CL_WAVE]) {
         const unsigned g = (unsigned)((hst.count[CL_WAVE] + 3) / 4 < 8192 ? (hst.count[CL_WAVE] + 3) / 4 : 8192);
