@@ -153,12 +153,23 @@ __device__ void lds_radix_pass(const uint32_t *ks, const float *vs, uint32_t *kd
     uint32_t key[MAXJ], rnk[MAXJ];
     float val[MAXJ];
     const uint64_t lt = lanemask_lt();
+    // every item read before the ranking writes the counters (the buffers
+    // arrive as generic pointers, so the compiler may not move a read past
+    // a counter write itself); indices stay below 4 * Q <= BMAX
+#pragma unroll
+    for (int j = 0; j < MAXJ; ++j) {
+        const uint32_t idx = wb + (uint32_t)j * 64 + lane;
+        key[j] = ks[idx];
+        val[j] = vs[idx];
+    }
 #pragma unroll
     for (int j = 0; j < MAXJ; ++j) {
         const uint32_t idx = wb + (uint32_t)j * 64 + lane;
         const bool ok = idx < we;
-        key[j] = ok ? ks[idx] : 0u;
-        val[j] = ok ? vs[idx] : 0.0f;
+        if (!ok) {
+            key[j] = 0u;
+            val[j] = 0.0f;
+        }
         rnk[j] = 0;
         if (wb + (uint32_t)j * 64 < we) {  // wave-uniform
             const uint32_t d = (key[j] >> shift) & 255u;
@@ -193,13 +204,16 @@ __device__ void lds_radix_pass(const uint32_t *ks, const float *vs, uint32_t *kd
         }
     }
     __syncthreads();
+    // every digit base read before the first write (generic pointers again)
+    uint32_t pos[MAXJ];
+#pragma unroll
+    for (int j = 0; j < MAXJ; ++j) pos[j] = wcnt[w][(key[j] >> shift) & 255u] + rnk[j];
 #pragma unroll
     for (int j = 0; j < MAXJ; ++j) {
         const uint32_t idx = wb + (uint32_t)j * 64 + lane;
         if (idx < we) {
-            const uint32_t pos = wcnt[w][(key[j] >> shift) & 255u] + rnk[j];
-            kd[pos] = key[j];
-            vd[pos] = val[j];
+            kd[pos[j]] = key[j];
+            vd[pos[j]] = val[j];
         }
     }
     __syncthreads();
@@ -308,8 +322,27 @@ __device__ void batch_update(uint32_t m, const uint32_t *__restrict__ row_base, 
         uint32_t ni = 0, nd = 0;
         for (uint32_t u = ub; u < ue; ++u) {
             const int32_t c = (int32_t)(urc[u] & (LMAX - 1));
-            while (i < L && pcol[st + i] < c) ++i;
-            const bool hit = i < L && pcol[st + i] == c;
+            // advance past the old columns below c, eight per step: the
+            // step's loads are issued together (indices clamp into the row)
+            // and counted with unrolled compares; the first column not below
+            // c is picked the same way
+            int32_t at = -1;  // old column at position i (-1: i == L)
+            while (i < L) {
+                int32_t pc[8];
+#pragma unroll
+                for (int j = 0; j < 8; ++j) pc[j] = pcol[st + (uint64_t)(i + j < L ? i + j : L - 1)];
+                int32_t below = 0;
+#pragma unroll
+                for (int j = 0; j < 8; ++j) below += (i + j < L && pc[j] < c) ? 1 : 0;
+                at = -1;
+#pragma unroll
+                for (int j = 0; j < 8; ++j)
+                    if (j == below && i + j < L) at = pc[j];
+                i += below;
+                if (below < 8) break;
+                at = -1;
+            }
+            const bool hit = i < L && at == c;
             const float s = uval[u];
             uint32_t code;
             float nv;
@@ -396,12 +429,34 @@ __device__ void batch_update(uint32_t m, const uint32_t *__restrict__ row_base, 
         const uint64_t d0 = (uint64_t)base + tdst[tt];
         uint64_t o = d0;
         int32_t i = 0;
+        // old entries [i, to) to o.., eight at a time with all eight loads
+        // issued before their stores (source and destination share the
+        // pool arrays, so the compiler keeps each load after the stores
+        // before it); to <= L, so a clamped index stays in the row
+        auto copy_run = [&](int32_t to) {
+            for (; i < to; i += 8, o += 8) {
+                int32_t cc[8];
+                float vv[8];
+#pragma unroll
+                for (int j = 0; j < 8; ++j) {
+                    const uint64_t si = st + (uint64_t)(i + j < to ? i + j : to - 1);
+                    cc[j] = pcol[si];
+                    vv[j] = pval[si];
+                }
+#pragma unroll
+                for (int j = 0; j < 8; ++j) {
+                    if (i + j < to) {
+                        pcol[o + j] = cc[j];
+                        pval[o + j] = vv[j];
+                    }
+                }
+            }
+            o -= (uint64_t)(i - to);  // the last chunk stopped at to
+            i = to;
+        };
         for (uint32_t u = ub; u < ue; ++u) {
             const uint32_t k = upos[u];  // old entries before this coordinate
-            for (; i < (int32_t)k; ++i, ++o) {
-                pcol[o] = pcol[st + i];
-                pval[o] = pval[st + i];
-            }
+            if (i < (int32_t)k) copy_run((int32_t)k);
             const uint32_t code = ucode[u];
             if (code == C_INS) {
                 pcol[o] = (int32_t)(urc[u] & (LMAX - 1));
@@ -416,10 +471,7 @@ __device__ void batch_update(uint32_t m, const uint32_t *__restrict__ row_base, 
                 ++i;
             }
         }
-        for (; i < L; ++i, ++o) {
-            pcol[o] = pcol[st + i];
-            pval[o] = pval[st + i];
-        }
+        if (i < L) copy_run(L);
     }
     __syncthreads();
     for (uint32_t tt = t; tt < nt; tt += BT) {
