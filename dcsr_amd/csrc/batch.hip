@@ -253,13 +253,32 @@ __device__ void batch_update(uint32_t m, const uint32_t *__restrict__ row_base, 
     if (t == 0) sm[0] = 0;
     __syncthreads();
     uint32_t bad = 0;
-    for (uint32_t i = t; i < nb; i += BT) {
-        const int32_t r = trow[t0 + i], c = tcol[t0 + i];
-        uint32_t k = 0;
-        if (r < 0 || (uint32_t)r >= Rm || c < 0 || (uint32_t)c >= Cm) bad = 1;
-        else k = ((uint32_t)r << KB) | (uint32_t)c;
-        skb[0][i] = k;
-        svb[0][i] = tval[t0 + i];
+    {
+        // a thread's triples (at most BMAX / BT) are all loaded before the
+        // first LDS store; indices clamp to the matrix's last triple
+        constexpr int TPT = BMAX / BT;
+        int32_t rr[TPT], cc[TPT];
+        float vv[TPT];
+#pragma unroll
+        for (int j = 0; j < TPT; ++j) {
+            const uint32_t i = t + (uint32_t)j * BT;
+            const int64_t gi = t0 + (i < nb ? i : nb - 1);
+            rr[j] = trow[gi];
+            cc[j] = tcol[gi];
+            vv[j] = tval[gi];
+        }
+#pragma unroll
+        for (int j = 0; j < TPT; ++j) {
+            const uint32_t i = t + (uint32_t)j * BT;
+            if (i < nb) {
+                const int32_t r = rr[j], c = cc[j];
+                uint32_t k = 0;
+                if (r < 0 || (uint32_t)r >= Rm || c < 0 || (uint32_t)c >= Cm) bad = 1;
+                else k = ((uint32_t)r << KB) | (uint32_t)c;
+                skb[0][i] = k;
+                svb[0][i] = vv[j];
+            }
+        }
     }
     if (block_sum(bad, lw)) {
         if (t == 0) mstat[m] |= BST_BAD_INDEX;
