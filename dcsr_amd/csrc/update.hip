@@ -688,19 +688,33 @@ __global__ __launch_bounds__(TB) void k_upd_move(const DevState *__restrict__ ds
         uint32_t j = wave_find_last_le<2>(tpre + 1, nt, e0);
         uint32_t e = e0;
         while (e < e1) {
+            // the window's per-row constants in two load phases (independent
+            // loads, then the row starts and record prefixes they select);
+            // indices clamp into the tables (nt >= 1 here: the range is
+            // non-empty), lanes past the window's rows keep the sentinels
             const uint32_t t = j + lane;
-            s_off[w][lane] = t <= nt ? tpre[2 * (uint64_t)t + 1] : 0xffffffffu;
-            if (lane == 0) s_off[w][64] = (j + 64 <= nt) ? tpre[2 * (uint64_t)(j + 64) + 1] : 0xffffffffu;
+            const uint32_t tq = t <= nt ? t : nt;          // tables of nt + 1 entries
+            const uint32_t tc = t < nt ? t : nt - 1;       // tables of nt entries
+            const uint32_t t64 = j + 64 <= nt ? j + 64 : nt;
+            const uint32_t off_t = tpre[2 * (uint64_t)tq + 1];
+            const uint32_t off_64 = tpre[2 * (uint64_t)t64 + 1];
+            const uint32_t ub = tubeg[tq];
+            const uint32_t ub1 = tubeg[t + 1 <= nt ? t + 1 : nt];
+            const uint32_t tr = trow[tc];
+            const uint32_t dst = tpre[2 * (uint64_t)tc];
+            const uint32_t src = row_start[tr];
+            const uint32_t ib = upre[2 * (uint64_t)ub], db = upre[2 * (uint64_t)ub + 1];
+            s_off[w][lane] = t <= nt ? off_t : 0xffffffffu;
+            if (lane == 0) s_off[w][64] = (j + 64 <= nt) ? off_64 : 0xffffffffu;
             if (t < nt) {
-                const uint32_t ub = tubeg[t];
-                s_src[w][lane] = row_start[trow[t]];
+                s_src[w][lane] = src;
                 s_ub[w][lane] = ub;
-                s_dst[w][lane] = tpre[2 * t];
-                s_ib[w][lane] = upre[2 * ub];
-                s_db[w][lane] = upre[2 * ub + 1];
-                if (lane == 63) s_ub[w][64] = tubeg[t + 1];
+                s_dst[w][lane] = dst;
+                s_ib[w][lane] = ib;
+                s_db[w][lane] = db;
+                if (lane == 63) s_ub[w][64] = ub1;
             } else if (t == nt) {
-                s_ub[w][lane] = tubeg[t];
+                s_ub[w][lane] = ub;
             }
             __builtin_amdgcn_wave_barrier();
             uint32_t wend = s_off[w][64];
