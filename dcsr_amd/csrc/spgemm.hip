@@ -667,6 +667,14 @@ __global__ void k_sg_set_rows(uint32_t R, const uint32_t *__restrict__ Poff, con
     }
 }
 
+template <int KPL>
+void launch_team(unsigned g, hipStream_t s, const uint32_t *list, const SgState *st, const uint32_t *P,
+                 const uint32_t *Poff, dcsr_handle A, dcsr_handle B, int32_t *tcol, float *tval, uint32_t *cnt)
+{
+    k_sg_team<KPL><<<g, TB, 0, s>>>(list, st, P, Poff, A->row_start, A->row_len, A->pool_col, A->pool_val,
+                                    B->row_start, B->row_len, B->pool_col, B->pool_val, tcol, tval, cnt);
+}
+
 }  // namespace
 
 dcsr_status dcsr_spgemm(dcsr_handle A, dcsr_handle B, dcsr_handle C, dcsr_stream_t stream)
@@ -723,16 +731,12 @@ dcsr_status dcsr_spgemm(dcsr_handle A, dcsr_handle B, dcsr_handle C, dcsr_stream
         const uint32_t want = (hst.count[c] + per - 1) / per;
         const unsigned g = want < 8192 ? want : 8192;
         const uint32_t *list = lists + c * (uint64_t)(R + 1);
-#define DCSR_SG_TEAM(K)                                                                                            \
-    k_sg_team<K><<<g, TB, 0, s>>>(list, st, P, Poff, A->row_start, A->row_len, A->pool_col, A->pool_val, B->row_start, \
-                                  B->row_len, B->pool_col, B->pool_val, tcol, tval, cnt)
         switch (c - CL_TEAM) {
-        case 0: DCSR_SG_TEAM(2); break;
-        case 1: DCSR_SG_TEAM(4); break;
-        case 2: DCSR_SG_TEAM(6); break;
-        default: DCSR_SG_TEAM(8); break;
+        case 0: launch_team<2>(g, s, list, st, P, Poff, A, B, tcol, tval, cnt); break;
+        case 1: launch_team<4>(g, s, list, st, P, Poff, A, B, tcol, tval, cnt); break;
+        case 2: launch_team<6>(g, s, list, st, P, Poff, A, B, tcol, tval, cnt); break;
+        default: launch_team<8>(g, s, list, st, P, Poff, A, B, tcol, tval, cnt); break;
         }
-#undef DCSR_SG_TEAM
     }
     if (hst.count[CL_WAVE]) {
         const unsigned g = (unsigned)((hst.count[CL_WAVE] + 3) / 4 < 8192 ? (hst.count[CL_WAVE] + 3) / 4 : 8192);
