@@ -227,10 +227,14 @@ __device__ __forceinline__ HeadFlags head_flags(const uint64_t (&k)[IPT], uint64
     return f;
 }
 
-// Pass 1 of the heads: per-tile counts; the last workgroup makes them
-// offsets and stores nuniq / ntouched.
+// Pass 1 of the heads: per-tile counts.  With `table` the last workgroup
+// makes them offsets and stores nuniq / ntouched; without it (batches of at
+// most HEADS_DIRECT_TILES tiles) pass 2 sums the counts before each tile
+// itself, and this kernel ends at the counts.
+constexpr int64_t HEADS_DIRECT_TILES = 4 * TB;
 __global__ __launch_bounds__(TB) void k_upd_heads_count(const uint64_t *__restrict__ key, int64_t n, int cbits,
-                                                        uint32_t *__restrict__ tt, unsigned int *fin, DevState *ds)
+                                                        uint32_t *__restrict__ tt, unsigned int *fin, DevState *ds,
+                                                        bool table)
 {
     __shared__ uint32_t lds4[4], s_flag;
     const int64_t tile = blockIdx.x;
@@ -243,6 +247,7 @@ __global__ __launch_bounds__(TB) void k_upd_heads_count(const uint64_t *__restri
         tt[tile * TT_WORDS + TT_H] = hc;
         tt[tile * TT_WORDS + TT_R] = rc;
     }
+    if (!table) return;
     if (last_block(&fin[FC_HEADS], &s_flag)) {
         uint32_t *const tabs[2] = {tt, tt};
         uint32_t tot[2];
@@ -260,10 +265,11 @@ __global__ __launch_bounds__(TB) void k_upd_heads_count(const uint64_t *__restri
 // accumulates B_row[j]; a run is summed from the thread's registers and
 // only a run that continues past the thread's items reads memory.  A tile's
 // records are contiguous, so they are staged in LDS and written out
-// coalesced.  Workgroup 0 stores the tubeg[ntouched] = nuniq sentinel.
+// coalesced.  The tubeg[ntouched] = nuniq sentinel is stored by workgroup 0
+// (table) or by the last tile, which then also stores nuniq / ntouched.
 __global__ __launch_bounds__(TB) void k_upd_heads_emit(const uint64_t *__restrict__ key,
                                                        const uint32_t *__restrict__ val, int64_t n, int cbits,
-                                                       const uint32_t *__restrict__ tt, const DevState *ds,
+                                                       const uint32_t *__restrict__ tt, DevState *ds, bool table,
                                                        int32_t *__restrict__ ucol, float *__restrict__ usum,
                                                        uint32_t *__restrict__ ut, uint32_t *__restrict__ urow,
                                                        uint32_t *__restrict__ trow, uint32_t *__restrict__ tubeg,
@@ -275,13 +281,38 @@ __global__ __launch_bounds__(TB) void k_upd_heads_emit(const uint64_t *__restric
     __shared__ uint32_t s_ut[TILE], s_urow[TILE], s_trow[TILE], s_tub[TILE];
     const int64_t tile = blockIdx.x;
     const int64_t i0 = tile * TILE + (int64_t)threadIdx.x * IPT;
-    const uint32_t ubase = tt[tile * TT_WORDS + TT_HX], tbase = tt[tile * TT_WORDS + TT_RX];
+    // counts of the tiles before this one (loaded with the keys): each
+    // thread sums up to four of them (index clamped to tile 0, masked)
+    uint32_t ch[HEADS_DIRECT_TILES / TB], cr[HEADS_DIRECT_TILES / TB];
+    uint32_t ubase = 0, tbase = 0;
+    if (table) {
+        ubase = tt[tile * TT_WORDS + TT_HX];
+        tbase = tt[tile * TT_WORDS + TT_RX];
+    } else {
+#pragma unroll
+        for (int q = 0; q < HEADS_DIRECT_TILES / TB; ++q) {
+            const int64_t pt = (int64_t)threadIdx.x + q * TB;
+            ch[q] = tt[(pt < tile ? pt : 0) * TT_WORDS + TT_H];
+            cr[q] = tt[(pt < tile ? pt : 0) * TT_WORDS + TT_R];
+        }
+    }
     uint64_t k[IPT], prev;
     load_keys(key, n, i0, k, prev);
     uint32_t vb[IPT];
 #pragma unroll
     for (int j = 0; j < IPT; ++j) vb[j] = val[i0 + j < n ? i0 + j : n - 1];
     const HeadFlags f = head_flags(k, prev, cbits);
+    if (!table) {
+        uint32_t sh = 0, sr = 0;
+#pragma unroll
+        for (int q = 0; q < HEADS_DIRECT_TILES / TB; ++q) {
+            const bool in = (int64_t)threadIdx.x + q * TB < tile;
+            sh += in ? ch[q] : 0u;
+            sr += in ? cr[q] : 0u;
+        }
+        ubase = block_sum_u32(sh, lds4);
+        tbase = block_sum_u32(sr, lds4);
+    }
     uint32_t htot, rtot;
     uint32_t lu = block_excl_scan_256(__popc(f.h), lds4, &htot);
     uint32_t lt = block_excl_scan_256(__popc(f.r), lds4, &rtot);
@@ -325,7 +356,15 @@ __global__ __launch_bounds__(TB) void k_upd_heads_emit(const uint64_t *__restric
         tins[tbase + i] = 0;
         tdel[tbase + i] = 0;
     }
-    if (tile == 0 && threadIdx.x == 0) tubeg[ds->ntouched] = ds->nuniq;
+    if (threadIdx.x == 0) {
+        if (table) {
+            if (tile == 0) tubeg[ds->ntouched] = ds->nuniq;
+        } else if (tile == gridDim.x - 1) {
+            ds->nuniq = ubase + htot;
+            ds->ntouched = tbase + rtot;
+            tubeg[tbase + rtot] = ubase + htot;
+        }
+    }
 }
 
 // Probe every distinct coordinate in its row: existing -> old + sum (delete
@@ -917,9 +956,10 @@ dcsr_status sort_plan_launch(dcsr_handle h, int64_t n, UpdBufs &b, hipStream_t s
     const uint64_t *key = which ? b.k1 : b.k0;
     const uint32_t *val = which ? b.v1 : b.v0;
     const unsigned ht = (unsigned)((n + TILE - 1) / TILE);
-    k_upd_heads_count<<<ht, TB, 0, s>>>(key, n, h->cbits, b.tt, b.fin, h->ds);
-    k_upd_heads_emit<<<ht, TB, 0, s>>>(key, val, n, h->cbits, b.tt, h->ds, b.ucol, b.usum, b.ut, b.urow, b.trow,
-                                       b.tubeg, b.tins, b.tdel);
+    const bool table = ht > HEADS_DIRECT_TILES;
+    k_upd_heads_count<<<ht, TB, 0, s>>>(key, n, h->cbits, b.tt, b.fin, h->ds, table);
+    k_upd_heads_emit<<<ht, TB, 0, s>>>(key, val, n, h->cbits, b.tt, h->ds, table, b.ucol, b.usum, b.ut, b.urow,
+                                       b.trow, b.tubeg, b.tins, b.tdel);
     k_upd_probe<<<ptiles(n), TB, 0, s>>>(h->ds, h->row_start, h->row_len, h->pool_col, h->pool_val, b.ucol, b.usum,
                                          b.ut, b.urow, b.uval, b.upos, b.ucode, b.tins, b.tdel, b.tp);
     DCSR_HIP(hipGetLastError());
