@@ -488,6 +488,60 @@ __global__ __launch_bounds__(TB) void k_upd_probe(const DevState *__restrict__ d
     }
 }
 
+// The batch's totals and decision, by one workgroup once every plan and
+// probe tile table is final: per-tile plan counters (pplan) and probe adds
+// are summed over all ntiles tiles, the claims / moved / inserts / deletes
+// totals come in.  The batch goes ahead only if every index was valid and
+// bump + claims fits the pool; only then does the bump pointer advance.
+__device__ void plan_totals_decide(DevState *ds, unsigned long long pool_cap, const long long *__restrict__ pplan,
+                                   const uint32_t *__restrict__ tp, int64_t ntiles, uint32_t claim_total,
+                                   uint32_t moved_total, uint32_t ins_total, uint32_t del_total,
+                                   long long (*lds)[8])
+{
+    long long acc[NPLAN + 1] = {0, 0, 0, 0, 0, 0};
+#pragma unroll 4
+    for (int64_t b = threadIdx.x; b < ntiles; b += TB) {
+#pragma unroll
+        for (int i = 0; i < NPLAN; ++i) acc[i] += pplan[b * NPLAN + i];
+        acc[NPLAN] += tp[b * TP_WORDS + TP_A];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < NPLAN + 1; ++i) {
+        const long long w = wave_sum_i64(acc[i]);
+        if (lane_id() == 0) lds[threadIdx.x >> 6][i] = w;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        long long tot[NPLAN + 1];
+#pragma unroll
+        for (int i = 0; i < NPLAN + 1; ++i) tot[i] = (lds[0][i] + lds[1][i]) + (lds[2][i] + lds[3][i]);
+        ds->claim_total = claim_total;
+        ds->nrewrite_old = moved_total;
+        ds->n_ins = ins_total;
+        ds->n_add = tot[NPLAN];
+        ds->n_del = del_total;
+        ds->d_nnz = tot[0];
+        ds->d_garbage = tot[1];
+        ds->d_hubs = tot[2];
+        ds->d_hub_chunks = tot[3];
+        ds->hub_touch = (unsigned)tot[4];
+        const unsigned long long need = claim_total;
+        const unsigned long long base = ds->bump;  // claims start at the current bump
+        ds->need = need;
+        ds->bump_saved = base;
+        const unsigned bad = ds->ubad;
+        ds->ubad = 0;  // the next batch's pack starts from a clear flag
+        ds->ubad_last = bad;
+        if (bad || base + need > pool_cap) {
+            ds->go = 0;
+        } else {
+            ds->go = 1;
+            ds->bump = base + need;
+        }
+    }
+}
+
 // Per touched row: new length, mode (values in place / rewrite), chunk
 // claim and old entries to move, per-tile sums to the batch tile table and per-tile counters to
 // pplan.  The last workgroup to finish then decides the batch: it makes the
@@ -501,7 +555,7 @@ __global__ __launch_bounds__(TB) void k_upd_plan(DevState *ds, unsigned long lon
                                                  uint32_t *__restrict__ tmode, uint32_t *__restrict__ tmoved,
                                                  uint32_t *__restrict__ tt,
                                                  uint32_t *__restrict__ tp, long long *__restrict__ pplan,
-                                                 unsigned int *fin)
+                                                 unsigned int *fin, bool table)
 {
     __shared__ long long lds[4][8];
     __shared__ uint32_t lds4[4], s_flag;
@@ -570,55 +624,16 @@ __global__ __launch_bounds__(TB) void k_upd_plan(DevState *ds, unsigned long lon
         tt[tile * TT_WORDS + TT_C] = csum;
         tt[tile * TT_WORDS + TT_M] = msum;
     }
-    if (!last_block(&fin[FC_PLAN], &s_flag)) return;
+    // without a table scan (at most HEADS_DIRECT_TILES tiles) the offsets
+    // pass sums the counts before each tile and decides the batch
+    if (!table || !last_block(&fin[FC_PLAN], &s_flag)) return;
 
     const int64_t ntiles = gridDim.x;
     uint32_t *const tabs[4] = {tt, tt, tp, tp};
     uint32_t ctot[4];
     block_scan_columns<4>(tabs, {TT_C, TT_M, TP_I, TP_D}, {TT_CX, TT_MX, TP_IX, TP_DX}, ntiles, lds4, ctot);
     const uint32_t claim_total = ctot[0], moved_total = ctot[1], ins_total = ctot[2], del_total = ctot[3];
-    long long acc[NPLAN + 1] = {0, 0, 0, 0, 0, 0};
-#pragma unroll 4
-    for (int64_t b = threadIdx.x; b < ntiles; b += TB) {
-#pragma unroll
-        for (int i = 0; i < NPLAN; ++i) acc[i] += pplan[b * NPLAN + i];
-        acc[NPLAN] += tp[b * TP_WORDS + TP_A];
-    }
-    __syncthreads();
-#pragma unroll
-    for (int i = 0; i < NPLAN + 1; ++i) {
-        const long long w = wave_sum_i64(acc[i]);
-        if (lane_id() == 0) lds[threadIdx.x >> 6][i] = w;
-    }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        long long tot[NPLAN + 1];
-#pragma unroll
-        for (int i = 0; i < NPLAN + 1; ++i) tot[i] = (lds[0][i] + lds[1][i]) + (lds[2][i] + lds[3][i]);
-        ds->claim_total = claim_total;
-        ds->nrewrite_old = moved_total;
-        ds->n_ins = ins_total;
-        ds->n_add = tot[NPLAN];
-        ds->n_del = del_total;
-        ds->d_nnz = tot[0];
-        ds->d_garbage = tot[1];
-        ds->d_hubs = tot[2];
-        ds->d_hub_chunks = tot[3];
-        ds->hub_touch = (unsigned)tot[4];
-        const unsigned long long need = claim_total;
-        const unsigned long long base = ds->bump;  // claims start at the current bump
-        ds->need = need;
-        ds->bump_saved = base;
-        const unsigned bad = ds->ubad;
-        ds->ubad = 0;  // the next batch's pack starts from a clear flag
-        ds->ubad_last = bad;
-        if (bad || base + need > pool_cap) {
-            ds->go = 0;
-        } else {
-            ds->go = 1;
-            ds->bump = base + need;
-        }
-    }
+    plan_totals_decide(ds, pool_cap, pplan, tp, ntiles, claim_total, moved_total, ins_total, del_total, lds);
 }
 
 // Exclusive offsets per entry: upre[2u], upre[2u+1] = inserts / deletes
@@ -626,17 +641,36 @@ __global__ __launch_bounds__(TB) void k_upd_plan(DevState *ds, unsigned long lon
 // tpre[2t], tpre[2t+1] = chunk slots claimed / old entries moved before
 // touched row t (t <= ntouched), from the in-tile prefixes plus the tile
 // offsets.
-__global__ __launch_bounds__(TB) void k_upd_offsets(const DevState *__restrict__ ds,
+__global__ __launch_bounds__(TB) void k_upd_offsets(DevState *__restrict__ ds,
                                                     const uint32_t *__restrict__ ucode,
                                                     const uint32_t *__restrict__ tmode,
                                                     const uint32_t *__restrict__ tnewl,
                                                     const uint32_t *__restrict__ tmoved,
                                                     const uint32_t *__restrict__ tt, const uint32_t *__restrict__ tp,
-                                                    uint32_t *__restrict__ upre, uint32_t *__restrict__ tpre)
+                                                    uint32_t *__restrict__ upre, uint32_t *__restrict__ tpre,
+                                                    const long long *__restrict__ pplan, unsigned long long pool_cap,
+                                                    bool table)
 {
     __shared__ uint32_t lds4[4];
+    __shared__ long long lds[4][8];
     const int64_t tile = blockIdx.x;
     const int64_t e0 = tile * TILE + (int64_t)threadIdx.x * IPT;
+    // tile bases: scanned by the plan's last workgroup (table), or summed
+    // here from the counts of the tiles before this one (loaded with the
+    // entries: up to four tiles per thread, index clamped to tile 0, masked)
+    constexpr int QT = HEADS_DIRECT_TILES / TB;
+    uint32_t qc[QT], qm[QT], qi[QT], qd[QT];
+    if (!table) {
+#pragma unroll
+        for (int q = 0; q < QT; ++q) {
+            const int64_t pt = (int64_t)threadIdx.x + q * TB;
+            const int64_t ix = pt < tile ? pt : 0;
+            qc[q] = tt[ix * TT_WORDS + TT_C];
+            qm[q] = tt[ix * TT_WORDS + TT_M];
+            qi[q] = tp[ix * TP_WORDS + TP_I];
+            qd[q] = tp[ix * TP_WORDS + TP_D];
+        }
+    }
     const uint32_t nu = ds->nuniq, nt = ds->ntouched;
     uint32_t ins[IPT], del[IPT], cl[IPT], mv[IPT];
     uint32_t code[IPT], md[IPT], nl[IPT];
@@ -665,11 +699,32 @@ __global__ __launch_bounds__(TB) void k_upd_offsets(const DevState *__restrict__
         sc += cl[j];
         sm += mv[j];
     }
-    uint32_t tot;
-    si = block_excl_scan_256(si, lds4, &tot) + tp[tile * TP_WORDS + TP_IX];
-    sd = block_excl_scan_256(sd, lds4, &tot) + tp[tile * TP_WORDS + TP_DX];
-    sc = block_excl_scan_256(sc, lds4, &tot) + tt[tile * TT_WORDS + TT_CX];
-    sm = block_excl_scan_256(sm, lds4, &tot) + tt[tile * TT_WORDS + TT_MX];
+    uint32_t bi, bd, bc, bm;
+    if (table) {
+        bi = tp[tile * TP_WORDS + TP_IX];
+        bd = tp[tile * TP_WORDS + TP_DX];
+        bc = tt[tile * TT_WORDS + TT_CX];
+        bm = tt[tile * TT_WORDS + TT_MX];
+    } else {
+        uint32_t a0 = 0, a1 = 0, a2 = 0, a3 = 0;
+#pragma unroll
+        for (int q = 0; q < QT; ++q) {
+            const bool in = (int64_t)threadIdx.x + q * TB < tile;
+            a0 += in ? qi[q] : 0u;
+            a1 += in ? qd[q] : 0u;
+            a2 += in ? qc[q] : 0u;
+            a3 += in ? qm[q] : 0u;
+        }
+        bi = block_sum_u32(a0, lds4);
+        bd = block_sum_u32(a1, lds4);
+        bc = block_sum_u32(a2, lds4);
+        bm = block_sum_u32(a3, lds4);
+    }
+    uint32_t ti, td, tc, tm;
+    si = block_excl_scan_256(si, lds4, &ti) + bi;
+    sd = block_excl_scan_256(sd, lds4, &td) + bd;
+    sc = block_excl_scan_256(sc, lds4, &tc) + bc;
+    sm = block_excl_scan_256(sm, lds4, &tm) + bm;
 #pragma unroll
     for (int j = 0; j < IPT; ++j) {
         const int64_t e = e0 + j;
@@ -686,6 +741,9 @@ __global__ __launch_bounds__(TB) void k_upd_offsets(const DevState *__restrict__
         sc += cl[j];
         sm += mv[j];
     }
+    // the last tile knows every total: its bases plus its own counts
+    if (!table && tile == gridDim.x - 1)
+        plan_totals_decide(ds, pool_cap, pplan, tp, gridDim.x, bc + tc, bm + tm, bi + ti, bd + td, lds);
 }
 
 // Move surviving old entries of rewritten rows to their merged positions.
@@ -938,10 +996,24 @@ dcsr_status upd_bufs(dcsr_handle h, int64_t n, UpdBufs &b)
     return DCSR_OK;
 }
 
+// Tile tables scanned by a last workgroup (table) only for batches of more
+// than HEADS_DIRECT_TILES tiles; smaller ones sum them where they are used.
+static inline bool use_tables(int64_t n) { return ptiles(n) > HEADS_DIRECT_TILES; }
+
+static void launch_offsets(dcsr_handle h, UpdBufs &b, int64_t n, hipStream_t s)
+{
+    k_upd_offsets<<<ptiles(n), TB, 0, s>>>(h->ds, b.ucode, b.tmode, b.tnewl, b.tmoved, b.tt, b.tp, b.upre, b.tpre,
+                                           b.pplan, (unsigned long long)h->pool_cap, use_tables(n));
+}
+
+// Plan and decision.  With tile tables the plan's last workgroup decides;
+// without them the offsets pass does (its last tile), so it runs here.
 dcsr_status plan_and_check(dcsr_handle h, UpdBufs &b, int64_t n, hipStream_t s)
 {
+    const bool table = use_tables(n);
     k_upd_plan<<<ptiles(n), TB, 0, s>>>(h->ds, (unsigned long long)h->pool_cap, b.trow, b.tins, b.tdel, h->row_len,
-                                        h->row_cap, b.tnewl, b.tmode, b.tmoved, b.tt, b.tp, b.pplan, b.fin);
+                                        h->row_cap, b.tnewl, b.tmode, b.tmoved, b.tt, b.tp, b.pplan, b.fin, table);
+    if (!table) launch_offsets(h, b, n, s);
     return hipGetLastError() == hipSuccess ? DCSR_OK : DCSR_ERR_HIP;
 }
 
@@ -964,7 +1036,7 @@ dcsr_status sort_plan_launch(dcsr_handle h, int64_t n, UpdBufs &b, hipStream_t s
                                          b.ut, b.urow, b.uval, b.upos, b.ucode, b.tins, b.tdel, b.tp);
     DCSR_HIP(hipGetLastError());
     DCSR_TRY(plan_and_check(h, b, n, s));
-    k_upd_offsets<<<ptiles(n), TB, 0, s>>>(h->ds, b.ucode, b.tmode, b.tnewl, b.tmoved, b.tt, b.tp, b.upre, b.tpre);
+    if (use_tables(n)) launch_offsets(h, b, n, s);
     DCSR_HIP(hipGetLastError());
     return DCSR_OK;
 }

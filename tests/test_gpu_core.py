@@ -185,6 +185,28 @@ def test_powerlaw_with_hub_rows(torch):
     m.close()
 
 
+@pytest.mark.parametrize("n", [2_090_000, 2_300_000])
+def test_large_batches(torch, n):
+    """Batches around 1,024 tiles of 2,048 triples: up to that size each pass
+    sums the tile counts before its tile itself (2,090,000: 1,021 tiles, every
+    summing lane used); above it a last workgroup scans them (2,300,000)."""
+    from dcsr_amd import DcsrMatrix
+    R = 200_000
+    r, c, v = W.powerlaw(R, 1_000_000, max_degree=3000, seed=21, signed=True)
+    m = DcsrMatrix(R, R)
+    m.LoadMatrix(r, c, v)
+    A = O.coo_to_csr(R, R, r, c, v)
+    rng = np.random.default_rng(23)
+    br, bc, bv = W.batch_mixed(rng, r, c, R, R, n, signed=True)
+    say(f"large batch n={n}")
+    m.UpdateMatrix(br, bc, bv)
+    A = O.update(A, br, bc, bv)
+    assert_same_structure(to_host_csr(m), A)
+    x = rng.uniform(-1, 1, R).astype(np.float32)
+    fp64_spmv_check(A, x, spmv_host(m, x))
+    m.close()
+
+
 def test_alpha_beta_and_csr_comparator(torch):
     from dcsr_amd import DcsrMatrix, csr_spmv
     rng = np.random.default_rng(11)
