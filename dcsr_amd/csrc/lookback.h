@@ -75,15 +75,18 @@ __device__ __forceinline__ uint32_t lb_ticket(unsigned int *ctl, uint32_t *lds2,
 
 // Called by thread 0 of every workgroup once it has finished all its
 // reads and writes of the launch's shared state: the last one out resets
-// the counters for the next launch.
+// the counters and bumps the epoch for the next launch.  No fence: the
+// ticket and finish counters are atomics, the status words are atomic
+// stores read only within this launch (all before the last finish), and
+// the kernel boundary publishes the reset to the next launch.  (A release
+// fence here writes back the XCD's whole L2, dirty lines of earlier kernels
+// included, in every workgroup.)
 __device__ __forceinline__ void lb_finish(unsigned int *ctl)
 {
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
     if (atomicAdd(&ctl[1], 1u) == gridDim.x - 1) {
         ctl[3] = ctl[3] + 1u;
         ctl[0] = 0u;
         ctl[1] = 0u;
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
     }
 }
 
