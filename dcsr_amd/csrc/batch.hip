@@ -599,13 +599,14 @@ __global__ __launch_bounds__(BT) void k_batch_step(uint32_t M, const uint32_t *_
         if (do_spmv) batch_spmv<TEAM>(m, row_base, col_base, row_start, row_len, pcol, pval, x, y, thresh);
         __syncthreads();
     }
-    // the last workgroup out resets the counters (no memset: replay-safe)
+    // the last workgroup out resets the counters (no memset: replay-safe).
+    // No fence: both counters are atomics, every workgroup's last work claim
+    // returned before its finish count, and the kernel boundary publishes the
+    // reset (a release fence would write back the XCD's whole L2 here)
     if (threadIdx.x == 0) {
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
         if (atomicAdd(&bs->done, 1u) == gridDim.x - 1) {
             bs->work = 0;
             bs->done = 0;
-            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
         }
     }
 }
