@@ -20,7 +20,7 @@
  *     half-updated: a rejected update is a no-op.
  *   - Threading: a handle may be used by one host thread at a time.
  *
- * Storage model (see DESIGN.md): every row is one column-sorted segment
+ * Storage model: every row is one column-sorted segment
  * [start, start+len) inside a pooled (col, val) store with capacity cap >= len;
  * structural updates write the new row to a chunk claimed from the pool's
  * bump region and retire the old chunk; compaction rewrites all rows
