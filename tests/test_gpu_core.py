@@ -185,11 +185,13 @@ def test_powerlaw_with_hub_rows(torch):
     m.close()
 
 
-@pytest.mark.parametrize("n", [2_090_000, 2_300_000])
+@pytest.mark.parametrize("n", [2_090_000, 1024 * 2048, 2_300_000])
 def test_large_batches(torch, n):
     """Batches around 1,024 tiles of 2,048 triples: up to that size each pass
     sums the tile counts before its tile itself (2,090,000: 1,021 tiles, every
-    summing lane used); above it a last workgroup scans them (2,300,000)."""
+    summing lane used); above it a last workgroup scans them (2,300,000).  At
+    exactly 1024 * 2048 the heads (1,024 tiles of n) sum directly while the
+    per-entry passes (tiles of n + 1 entries: 1,025) use the tables."""
     from dcsr_amd import DcsrMatrix
     R = 200_000
     r, c, v = W.powerlaw(R, 1_000_000, max_degree=3000, seed=21, signed=True)
