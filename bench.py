@@ -675,7 +675,8 @@ def bench_spmv(args):
          {"workload": "spmv-powerlaw-10M", "rows": R, "nnz": nnz, "zipf": 2.1},
          spmv_hbm_frac=res[1]["hbm_frac"],
          spmm={str(k): res[k] for k in (4, 16, 32)},
-         roofline=roofline(gbs, "flat SpMV (dcsr_spmv: flat kernel + carry)"),
+         roofline=roofline(gbs, "SpMV over the paneled copy (dcsr_spmv: flat kernel, carry, panel reduce)",
+                           measured_traffic("spmv_powerlaw_10m_160m_traffic.txt", "traffic_bytes_per_spmv")),
          cpu_baseline=None)
 
 
