@@ -423,10 +423,11 @@ dcsr_status dcsr_create(dcsr_handle *out, int64_t rows, int64_t cols, const dcsr
     h->hubs_dirty = false;
     h->flat_dirty = true;
     state_exact(h);
-    // SpMV column panels: 2M columns (8 MB of x; measured best on a 10M x 10M
-    // power-law matrix between 0.5M and 4M) for matrices of at least 16M
-    // entries; both can be overridden for testing
-    h->pnl_cols = 2 << 20;
+    // SpMV column panels: 1.5M columns (6 MB of x; measured best on a 10M x
+    // 10M power-law matrix among 1M, 1.25M, 1.5M, 2M and 3M with non-temporal
+    // entry loads: 1.91, 1.76, 1.74, 1.80, 2.13 ms) for matrices of at least
+    // 16M entries; both can be overridden for testing
+    h->pnl_cols = 1536 << 10;
     h->pnl_prev_calls = DCSR_PNL_REUSE;  // no evidence yet: a fresh matrix is assumed to be reused
     h->pnl_min_nnz = 16ll << 20;
     if (const char *e = getenv("DCSR_SPMV_PANEL_COLS")) h->pnl_cols = atoll(e) > 0 ? atoll(e) : h->pnl_cols;
