@@ -429,58 +429,6 @@ def bench_spgemm(args):
 # the plain per-call path
 # --------------------------------------------------------------------------
 
-def serving_matrix(R, N, seed=5):
-    """Zipf(2.1) out-degrees scaled to N (capped at R/10), uniform columns;
-    coordinates made unique on the GPU.  Returns host (rows, cols, vals)."""
-    import torch
-    from dcsr_amd import workloads as W
-    rng = np.random.default_rng(seed)
-    d, _ = W.zipf_degrees(rng, R, N, 2.1, max(R // 10, 1))
-    g = torch.Generator(device="cuda")
-    g.manual_seed(seed)
-    rows = torch.repeat_interleave(torch.arange(R, device="cuda", dtype=torch.int64),
-                                   torch.from_numpy(d).to("cuda"))
-    cols = torch.randint(0, R, (rows.numel(),), device="cuda", generator=g, dtype=torch.int64)
-    key = torch.unique(rows * R + cols, sorted=True)
-    del rows, cols
-    r = (key // R).to(torch.int32).cpu().numpy()
-    c = (key % R).to(torch.int32).cpu().numpy()
-    del key
-    v = (1.0 - rng.random(r.size, dtype=np.float32)).astype(np.float32)
-    return r, c, v
-
-
-def serving_stream(rng, r, c, v, R, nreq, upd_frac=0.1):
-    """Request list: (kind, k or n, arrays).  Updates: log-uniform 1e3..1e6
-    triples, 30% deletions (-current of a coordinate no earlier request
-    touched), 35% adds to stored coordinates, 35% new coordinates on
-    degree-drawn rows.  Queries: k = 1, 4, 16, 32 with weights .55/.2/.15/.1."""
-    dirty = np.zeros(r.size, bool)
-    reqs = []
-    ks = np.array([1, 4, 16, 32])
-    for _ in range(nreq):
-        if rng.random() < upd_frac:
-            n = int(np.exp(rng.uniform(np.log(1e3), np.log(1e6))))
-            nd = int(0.3 * n)
-            na = int(0.35 * n)
-            nn = n - nd - na
-            pick = rng.integers(0, r.size, 2 * nd + 16)
-            pick = np.unique(pick[~dirty[pick]])[:nd]
-            nd = pick.size
-            add = rng.integers(0, r.size, na)
-            dirty[pick] = True
-            dirty[add] = True
-            newrow = r[rng.integers(0, r.size, nn)]
-            rows = np.concatenate([r[pick], r[add], newrow]).astype(np.int32)
-            cols = np.concatenate([c[pick], c[add], rng.integers(0, R, nn)]).astype(np.int32)
-            vals = np.concatenate([-v[pick], (1.0 - rng.random(na + nn, dtype=np.float32))]).astype(np.float32)
-            perm = rng.permutation(rows.size)
-            reqs.append(("update", rows.size, (rows[perm], cols[perm], vals[perm])))
-        else:
-            reqs.append(("query", int(rng.choice(ks, p=[0.55, 0.2, 0.15, 0.1])), None))
-    return reqs
-
-
 def bench_serving(args):
     import torch
     import dcsr_amd
@@ -488,9 +436,9 @@ def bench_serving(args):
     R, N = args.serving_rows, args.serving_nnz
     nreq = args.steps
     t_gen = time.perf_counter()
-    r, c, v = serving_matrix(R, N)
+    r, c, v = W.serving_matrix(R, N)
     rng = np.random.default_rng(55)
-    reqs = serving_stream(rng, r, c, v, R, nreq + args.warmup * 4)
+    reqs = W.serving_stream(rng, r, c, v, R, nreq + args.warmup * 4)
     warm, reqs = reqs[:args.warmup * 4], reqs[args.warmup * 4:]
     dev = torch.device("cuda:0")
     d_r, d_c, d_v = (torch.from_numpy(a).to(dev) for a in (r, c, v))
@@ -644,7 +592,7 @@ def bench_spmv(args):
     import dcsr_amd
 
     R, N = args.serving_rows, args.serving_nnz
-    r, c, v = serving_matrix(R, N)
+    r, c, v = W.serving_matrix(R, N)
     dev = torch.device("cuda:0")
     m = dcsr_amd.DcsrMatrix(R, R, pool_factor=1.0)
     m.LoadMatrix(*(torch.from_numpy(a).to(dev) for a in (r, c, v)))

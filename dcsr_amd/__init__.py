@@ -53,11 +53,19 @@ class DcsrServerStats(ctypes.Structure):
     _fields_ = [(n, ctypes.c_int64) for n in ("updates", "update_triples", "queries", "query_groups", "compactions")]
 
 
+class DcsrHybView(ctypes.Structure):
+    _fields_ = [("rows", ctypes.c_int64), ("cols", ctypes.c_int64), ("width", ctypes.c_int64),
+                ("pitch", ctypes.c_int64), ("ell_col", ctypes.c_void_p), ("ell_val", ctypes.c_void_p),
+                ("row_sizes", ctypes.c_void_p), ("coo_row", ctypes.c_void_p), ("coo_col", ctypes.c_void_p),
+                ("coo_val", ctypes.c_void_p), ("num_overflow", ctypes.c_int64), ("coo_cap", ctypes.c_int64)]
+
+
 class DcsrInfo(ctypes.Structure):
     _fields_ = [(n, ctypes.c_int64) for n in (
         "rows", "cols", "nnz", "pool_cap", "pool_used", "garbage", "hub_rows",
         "last_inserted", "last_added", "last_deleted", "last_touched_rows")] + \
-        [("canonical", ctypes.c_int32), ("max_segments", ctypes.c_int32)]
+        [("canonical", ctypes.c_int32), ("max_segments", ctypes.c_int32),
+         ("scan_fallback", ctypes.c_int32), ("spmv_paneled", ctypes.c_int32)]
 
     def as_dict(self):
         return {n: getattr(self, n) for n, _ in self._fields_}
@@ -126,6 +134,18 @@ def lib():
         "dcsr_server_destroy": (ctypes.c_int, [_H]),
         "dcsr_query": (ctypes.c_int, [_H, ctypes.POINTER(DcsrInfo)]),
         "dcsr_csr_spmv": (ctypes.c_int, [_I64, _P, _P, _P, _P, _P, _P]),
+        "dcsr_export_layout": (ctypes.c_int, [_H, _P, _P, _P, _P]),
+        "dcsr_hyb_create": (ctypes.c_int, [ctypes.POINTER(_H), _I64, _I64, ctypes.c_int32, _I64]),
+        "dcsr_hyb_destroy": (ctypes.c_int, [_H]),
+        "dcsr_hyb_view_get": (ctypes.c_int, [_H, ctypes.POINTER(DcsrHybView)]),
+        "dcsr_hyb_load_coo": (ctypes.c_int, [_H, _P, _P, _P, _I64, _P]),
+        "dcsr_hyb_load_csr": (ctypes.c_int, [_H, _P, _P, _P, _P]),
+        "dcsr_hyb_update": (ctypes.c_int, [_H, _P, _P, _P, _I64, _P]),
+        "dcsr_hyb_spmv": (ctypes.c_int, [_H, _P, _P, _F, _F, _P]),
+        "dcsr_hyb_export": (ctypes.c_int, [_H, _P, _P, _P, _P, _P, _P, _I64, ctypes.POINTER(_I64), _P]),
+        "dcsr_hyb_to_dcsr": (ctypes.c_int, [_H, _H, _P]),
+        "dcsr_hyb_from_dcsr": (ctypes.c_int, [_H, _H, _P]),
+        "dcsr_set_scan_wait_limit": (None, [ctypes.c_int32]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -315,6 +335,17 @@ class DcsrMatrix:
         check(lib().dcsr_query(self._h, ctypes.byref(inf)), "dcsr_query")
         return inf.as_dict()
 
+    def layout(self):
+        """The allocator's row table (start, len, cap) as CUDA tensors."""
+        torch = _torch()
+        R = max(self.num_rows, 1)
+        st = torch.empty(R, dtype=torch.int32, device="cuda")
+        ln = torch.empty(R, dtype=torch.int32, device="cuda")
+        cp = torch.empty(R, dtype=torch.int32, device="cuda")
+        check(lib().dcsr_export_layout(self._h, _ptr(st), _ptr(ln), _ptr(cp), _stream()), "dcsr_export_layout")
+        n = self.num_rows
+        return st[:n], ln[:n], cp[:n]
+
     def status(self):
         return lib().dcsr_get_status(self._h)
 
@@ -340,7 +371,7 @@ class DcsrBatch:
     """Many small DCSR matrices stepped (update + SpMV) in one persistent
     launch.  Matrix m is rows[m] x cols[m] (each <= 4096), local indices."""
 
-    def __init__(self, rows, cols, pool_factor=3.0):
+    def __init__(self, rows, cols, pool_factor=3.0, allow_grow=True, min_pool=1024):
         import numpy as np
         torch = _torch()
         torch.cuda.init()
@@ -348,7 +379,7 @@ class DcsrBatch:
         c = np.ascontiguousarray(cols, np.int32)
         assert r.size == c.size
         self.count = r.size
-        cfg = DcsrConfig(pool_factor, 1024, 1, 0)
+        cfg = DcsrConfig(pool_factor, min_pool, 1 if allow_grow else 0, 0)
         h = _H()
         check(lib().dcsr_batch_create(ctypes.byref(h), r.size, r.ctypes.data_as(_P), c.ctypes.data_as(_P),
                                       ctypes.byref(cfg)), "dcsr_batch_create")
@@ -426,6 +457,15 @@ class DcsrBatch:
         check(lib().dcsr_query(lib().dcsr_batch_handle(self._b), ctypes.byref(inf)), "dcsr_query")
         return inf.as_dict()
 
+    def layout(self):
+        """The allocator's row table of the block-diagonal storage."""
+        torch = _torch()
+        n = max(self.total_rows, 1)
+        st, ln, cp = (torch.empty(n, dtype=torch.int32, device="cuda") for _ in range(3))
+        check(lib().dcsr_export_layout(lib().dcsr_batch_handle(self._b), _ptr(st), _ptr(ln), _ptr(cp), _stream()),
+              "dcsr_export_layout")
+        return st[: self.total_rows], ln[: self.total_rows], cp[: self.total_rows]
+
     def to_csr(self):
         """The whole block-diagonal batch as CSR (local column indices)."""
         torch = _torch()
@@ -440,6 +480,93 @@ class DcsrBatch:
         check(lib().dcsr_export_csr(h, _ptr(off), _ptr(col), _ptr(val), cap, ctypes.byref(nnz), 0, _stream()),
               "dcsr_export_csr")
         return off, col[: nnz.value], val[: nnz.value]
+
+
+class DcsrHyb:
+    """Dynamic HYB matrix (ELL + COO overflow): the reference's hyb_matrix
+    (include/dcsr_matrix.h:148-172) with LoadMatrix (COO / CSR -> HYB),
+    UpdateMatrix (append) and SpMV (include/load.h:61-106, 174-205,
+    327-373)."""
+
+    def __init__(self, num_rows, num_cols, width, coo_cap=1024):
+        torch = _torch()
+        torch.cuda.init()
+        self.num_rows, self.num_cols, self.width = int(num_rows), int(num_cols), int(width)
+        h = _H()
+        check(lib().dcsr_hyb_create(ctypes.byref(h), self.num_rows, self.num_cols, self.width, int(coo_cap)),
+              "dcsr_hyb_create")
+        self._h = h
+        self.pitch = self.view()["pitch"]
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().dcsr_hyb_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def handle(self):
+        return self._h
+
+    def view(self):
+        v = DcsrHybView()
+        check(lib().dcsr_hyb_view_get(self._h, ctypes.byref(v)), "dcsr_hyb_view_get")
+        return {n: getattr(v, n) for n, _ in v._fields_}
+
+    def LoadMatrix(self, rows, cols, vals):
+        """LoadMatrix(hyb&, rows, cols, vals, N) (reference include/load.h:61-106)."""
+        torch = _torch()
+        r, c, v = _dev(rows, torch.int32), _dev(cols, torch.int32), _dev(vals, torch.float32)
+        check(lib().dcsr_hyb_load_coo(self._h, _ptr(r), _ptr(c), _ptr(v), r.numel(), _stream()), "dcsr_hyb_load_coo")
+
+    def load_csr(self, row_offsets, cols, vals):
+        """LoadMatrix(hyb&, const csr&) (reference include/load.h:174-205)."""
+        torch = _torch()
+        o, c, v = _dev(row_offsets, torch.int32), _dev(cols, torch.int32), _dev(vals, torch.float32)
+        check(lib().dcsr_hyb_load_csr(self._h, _ptr(o), _ptr(c), _ptr(v), _stream()), "dcsr_hyb_load_csr")
+
+    def UpdateMatrix(self, rows, cols, vals):
+        """UpdateMatrix(hyb&, rows, cols, vals, N) (reference include/load.h:327-373): append."""
+        torch = _torch()
+        r, c, v = _dev(rows, torch.int32), _dev(cols, torch.int32), _dev(vals, torch.float32)
+        check(lib().dcsr_hyb_update(self._h, _ptr(r), _ptr(c), _ptr(v), r.numel(), _stream()), "dcsr_hyb_update")
+
+    def spmv(self, x, y=None, alpha=1.0, beta=0.0):
+        torch = _torch()
+        x = _dev(x, torch.float32)
+        if y is None:
+            y = torch.empty(self.num_rows, dtype=torch.float32, device="cuda")
+            beta = 0.0
+        check(lib().dcsr_hyb_spmv(self._h, _ptr(x), _ptr(y), float(alpha), float(beta), _stream()), "dcsr_hyb_spmv")
+        return y
+
+    def export(self):
+        """(ell_col, ell_val, row_sizes, coo_row, coo_col, coo_val) as CUDA tensors."""
+        torch = _torch()
+        n = self.view()["num_overflow"]
+        ncell = max(self.width * self.pitch, 1)
+        ec = torch.empty(ncell, dtype=torch.int32, device="cuda")
+        ev = torch.empty(ncell, dtype=torch.float32, device="cuda")
+        rs = torch.empty(self.num_rows + 1, dtype=torch.int32, device="cuda")
+        cr, cc = (torch.empty(max(n, 1), dtype=torch.int32, device="cuda") for _ in range(2))
+        cv = torch.empty(max(n, 1), dtype=torch.float32, device="cuda")
+        got = _I64(0)
+        check(lib().dcsr_hyb_export(self._h, _ptr(ec), _ptr(ev), _ptr(rs), _ptr(cr), _ptr(cc), _ptr(cv), max(n, 1),
+                                    ctypes.byref(got), _stream()), "dcsr_hyb_export")
+        k = got.value
+        m = self.width * self.pitch
+        return ec[:m], ev[:m], rs, cr[:k], cc[:k], cv[:k]
+
+    def to_dcsr(self, m):
+        check(lib().dcsr_hyb_to_dcsr(self._h, m.handle, _stream()), "dcsr_hyb_to_dcsr")
+
+    def from_dcsr(self, m):
+        check(lib().dcsr_hyb_from_dcsr(self._h, m.handle, _stream()), "dcsr_hyb_from_dcsr")
 
 
 REQ_UPDATE, REQ_SPMV, REQ_SPMM = 0, 1, 2
@@ -514,6 +641,12 @@ class DcsrServer:
             self.close()
         except Exception:
             pass
+
+
+def set_scan_wait_limit(polls):
+    """Poll limit of the scans' bounded look-back waits (negative: the
+    default); 0 sends every tile down its exact fallback without waiting."""
+    lib().dcsr_set_scan_wait_limit(int(polls))
 
 
 def csr_spmv(row_offsets, cols, vals, x, y=None):

@@ -16,7 +16,8 @@ CSRC = os.path.join(HERE, "csrc")
 OBJ = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libdcsr.so")
 
-HIP_SOURCES = ["prims.hip", "matrix.hip", "update.hip", "spmv.hip", "spgemm.hip", "hyb.hip", "batch.hip", "server.hip"]
+HIP_SOURCES = ["prims.hip", "matrix.hip", "update.hip", "spmv.hip", "spgemm.hip", "hyb.hip", "hybstore.hip", "batch.hip",
+               "server.hip"]
 CXX_SOURCES: list[str] = []
 
 ARCH = os.environ.get("DCSR_OFFLOAD_ARCH", "gfx950")
@@ -57,6 +58,28 @@ def build_lib(verbose: bool = False) -> str:
     return LIB
 
 
+DRIVER_SRC = os.path.join(ROOT, "tests", "shim", "ref_style_driver.cpp")
+DRIVER_BIN = os.path.join(ROOT, "tests", "shim", "ref_style_driver")
+
+
+def driver_cmd(out: str) -> list[str]:
+    """hipcc line that builds the reference-style driver (test infrastructure:
+    a caller of the C++ drop-in shim include/dcsr_matrix.hpp) against
+    libdcsr.so, found next to the tree at run time."""
+    return [HIPCC, "-std=c++17", "-O2", "-Wall", "-I", os.path.join(ROOT, "include"),
+            "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops", "-Wno-unused-command-line-argument",
+            DRIVER_SRC, "-o", out, "-L", HERE, "-ldcsr", "-Wl,-rpath,$ORIGIN/../../dcsr_amd"]
+
+
+def build_driver(verbose: bool = False) -> None:
+    r = subprocess.run(driver_cmd(DRIVER_BIN + ".tmp"), capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"driver build failed:\n{r.stdout}\n{r.stderr}")
+    os.replace(DRIVER_BIN + ".tmp", DRIVER_BIN)
+    if verbose:
+        print("built", DRIVER_BIN)
+
+
 def build_oracle(verbose: bool = False) -> None:
     r = subprocess.run(["make", "-C", os.path.join(ROOT, "oracle")], capture_output=True, text=True)
     if r.returncode != 0:
@@ -67,6 +90,7 @@ def build_oracle(verbose: bool = False) -> None:
 
 def build(verbose: bool = False) -> None:
     build_lib(verbose)
+    build_driver(verbose)
     build_oracle(verbose)
 
 

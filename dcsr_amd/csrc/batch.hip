@@ -415,8 +415,26 @@ __device__ void batch_update(uint32_t m, const uint32_t *__restrict__ row_base, 
     const uint32_t dn_lo = block_sum((uint32_t)(dnnz & 0xffffffff), lw);
     const uint32_t dg_lo = block_sum((uint32_t)(dgar & 0xffffffff), lw);
     if (t == 0) {
-        const unsigned long long base = claim_tot ? atomicAdd(&ds->bump, (unsigned long long)claim_tot) : 0ull;
-        const bool ok = base + claim_tot <= pool_cap;
+        // claim by compare-and-swap: the bump pointer advances only when the
+        // claim fits, so a matrix that does not fit leaves room for later
+        // (smaller) matrices of the same step
+        unsigned long long base = 0ull;
+        bool ok = true;
+        if (claim_tot) {
+            unsigned long long cur = __hip_atomic_load(&ds->bump, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            for (;;) {
+                if (cur + claim_tot > pool_cap) {
+                    ok = false;
+                    break;
+                }
+                const unsigned long long seen = atomicCAS(&ds->bump, cur, cur + claim_tot);
+                if (seen == cur) {
+                    base = cur;
+                    break;
+                }
+                cur = seen;  // another matrix claimed first: retry from its end
+            }
+        }
         sm[1] = ok ? 1u : 0u;
         sm[2] = (uint32_t)base;
         if (!ok) mstat[m] |= BST_EXHAUSTED;
@@ -806,12 +824,16 @@ static void launch_any(dcsr_batch b, const int64_t *tri_off, int64_t n, const in
 
 // Pool headroom for one step: every matrix may rewrite all its rows plus its
 // batch.  Compacts/grows ahead of the launch so the kernel never runs out.
+// With allow_grow = 0 the pool keeps its capacity (retired chunks are
+// reclaimed first): a matrix whose claim then does not fit fails alone with
+// its exhausted bit and leaves room for the others.
 static dcsr_status batch_reserve(dcsr_batch b, int64_t n, hipStream_t s)
 {
     dcsr_handle h = b->h;
     if (!h->state_known) DCSR_TRY(pull_state(h, s));
     const int64_t need = h->nnz_h + n + 4 * (int64_t)h->rows + 64;
     if (h->pool_cap - h->bump_h >= need) return DCSR_OK;
+    if (!h->cfg.allow_grow) return h->bump_h > h->nnz_h ? compact_in_order(h, s) : DCSR_OK;
     return pool_reserve(h, need, s, nullptr);
 }
 

@@ -9,6 +9,11 @@
 #define ST_BAD_INDEX 0x1u
 #define ST_POOL_EXHAUSTED 0x2u
 #define ST_SEGMENT_OVERFLOW 0x4u
+// a single-pass scan's bounded look-back wait gave up and the tile computed
+// its prefix by the exact fallback (lookback.h LB_FALLBACK_BIT): results are
+// correct, the bit is a diagnostic (dcsr_info.scan_fallback) and survives
+// status reads
+#define ST_SCAN_FALLBACK 0x8u
 
 // Rows longer than this are counted as power-law hubs in dcsr_info.
 #define DCSR_HUB_LEN 4096
@@ -162,6 +167,9 @@ dcsr_status update_commit(dcsr_handle h, int64_t n, hipStream_t s, bool nosync);
 // Flat copy of R rows (src_start[r], length dst_off[r+1]-dst_off[r]) to dst_off.
 void launch_copy_rows(const uint32_t *dst_off, int64_t R, int64_t total, const uint32_t *src_start,
                       const int32_t *scol, const float *sval, int32_t *dcol, float *dval, hipStream_t s);
+
+// The handle's sticky status word (device): scans set ST_SCAN_FALLBACK in it.
+static inline unsigned int *fault_word(dcsr_handle h) { return &h->ds->status; }
 
 static inline int bits_for(int64_t n)
 {

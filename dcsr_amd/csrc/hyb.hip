@@ -116,7 +116,7 @@ dcsr_status dcsr_export_hyb(dcsr_handle h, int32_t width, int64_t pitch, int32_t
     DCSR_TRY(ws_get(h, WS_TMP0, (size_t)(R + 1) * 4, (void **)&coff));
     DCSR_TRY(ws_get(h, WS_PART, scan_part_words(R + 1, 1) * 4, (void **)&part));
     if (R > 0) k_hyb_overflow<<<grid_for(R, TB), TB, 0, s>>>(R, h->row_len, (uint32_t)width, ov);
-    DCSR_HIP(scan_exclusive_u32(ov, coff, R, 1, part, coff + R, s));
+    DCSR_HIP(scan_exclusive_u32(ov, coff, R, 1, part, coff + R, s, fault_word(h)));
     uint32_t ncoo = 0;
     DCSR_HIP(hipMemcpyAsync(&ncoo, coff + R, 4, hipMemcpyDeviceToHost, s));
     DCSR_HIP(hipStreamSynchronize(s));
@@ -155,7 +155,7 @@ dcsr_status dcsr_load_hyb(dcsr_handle h, int32_t width, int64_t pitch, const int
     DCSR_TRY(ws_get(h, WS_TMP0, (size_t)(ncell + 2) * 4, (void **)&pos));
     DCSR_TRY(ws_get(h, WS_PART, scan_part_words(ncell + 2, 1) * 4, (void **)&part));
     k_hyb_ell_flags<<<grid_for(ncell + 1, TB), TB, 0, s>>>(ell_col, ncell, (uint64_t)pitch, (uint32_t)h->rows, flag);
-    DCSR_HIP(scan_exclusive_u32(flag, pos, ncell + 1, 1, part, nullptr, s));
+    DCSR_HIP(scan_exclusive_u32(flag, pos, ncell + 1, 1, part, nullptr, s, fault_word(h)));
     uint32_t nell = 0;
     DCSR_HIP(hipMemcpyAsync(&nell, pos + ncell, 4, hipMemcpyDeviceToHost, s));
     DCSR_HIP(hipStreamSynchronize(s));

@@ -12,9 +12,16 @@
 // start), so status arrays are never cleared, words from older launches read
 // as "not ready", and replays of a captured graph get fresh epochs too.
 //
-// Every wait is bounded: after LB_SPIN_LIMIT polls a tile gives up, counts
-// the fault in ctl[2] of its launch (never reset; 0 in a correct run) and
-// proceeds, so no launch can hang the device.
+// Every wait is bounded: after `limit` polls (LB_SPIN_LIMIT unless the host
+// sets another) a tile gives up waiting and computes its exclusive prefix
+// exactly from the raw inputs of every earlier tile instead (the caller's
+// fallback functor: one strided pass of the wave over them, slow but never
+// wrong and waiting for nothing), so no launch can hang and no consumer
+// ever sees a short prefix.  The event is counted in ctl[2] of the launch
+// (never reset) and raises LB_FALLBACK_BIT in the owner's sticky status word
+// (`fault`, may be null), which the handle reports as a diagnostic.  A limit
+// of 0 sends every tile past the first down the fallback without polling (a
+// test of that path).
 //
 // Control words ctl[0..3] sit at the head of the launch's workspace:
 // ctl[0] ticket, ctl[1] finished workgroups, ctl[2] fault count, ctl[3]
@@ -30,6 +37,9 @@
 #define LB_AGG 1u
 #define LB_INC 2u
 #define LB_SPIN_LIMIT (1 << 20)
+// Sticky status bit raised by a tile that took the exact fallback (the
+// handle's ST_SCAN_FALLBACK).
+#define LB_FALLBACK_BIT 0x8u
 
 __device__ __forceinline__ uint32_t lb_tag(uint32_t epoch) { return (epoch & 0x3fffffffu) << 2; }
 
@@ -94,10 +104,12 @@ __device__ __forceinline__ void lb_finish(unsigned int *ctl)
 // words sit at st[j * NC + c] (j < tile), computed by one whole wave: 64
 // predecessors are examined per round, nearest first, all components in the
 // same round (each stops at its own nearest inclusive word).  pre[c] gets
-// the same value in every lane.
-template <int NC>
+// the same value in every lane.  After `limit` polls without progress the
+// wave calls fb(c) instead, which must return (in every lane, called by all
+// 64 lanes) component c's exact sum over every earlier tile's inputs.
+template <int NC, typename Fallback>
 __device__ __forceinline__ void lb_wave_lookback(const uint64_t *st, int64_t tile, uint32_t tag, unsigned int *ctl,
-                                                 uint32_t *pre)
+                                                 unsigned int *fault, int limit, Fallback &&fb, uint32_t *pre)
 {
     const int lane = lane_id();
     bool done[NC];
@@ -108,7 +120,8 @@ __device__ __forceinline__ void lb_wave_lookback(const uint64_t *st, int64_t til
     }
     int64_t j = tile - 1;
     int spins = 0;
-    while (j >= 0) {
+    bool gave_up = limit <= 0;  // 0: every tile takes the fallback (a test of it)
+    while (j >= 0 && !gave_up) {
         const int64_t idx = j - lane;
         uint64_t w[NC];
 #pragma unroll
@@ -121,12 +134,13 @@ __device__ __forceinline__ void lb_wave_lookback(const uint64_t *st, int64_t til
                 ok = ok && (idx < 0 || done[c] || lb_ready(w[c], tag));
             }
             if (__all(ok)) break;
-            if (++spins > LB_SPIN_LIMIT) {
-                if (lane == 0) atomicAdd(&ctl[2], 1u);
+            if (++spins > limit) {  // wave-uniform: every lane counts the same polls
+                gave_up = true;
                 break;
             }
             __builtin_amdgcn_s_sleep(1);
         }
+        if (gave_up) break;
         bool all_done = true;
 #pragma unroll
         for (int c = 0; c < NC; ++c) {
@@ -144,15 +158,24 @@ __device__ __forceinline__ void lb_wave_lookback(const uint64_t *st, int64_t til
         if (all_done) break;
         j -= 64;
     }
+    if (gave_up) {
+        if (lane == 0) {
+            atomicAdd(&ctl[2], 1u);
+            if (fault) atomicOr(fault, LB_FALLBACK_BIT);
+        }
+#pragma unroll
+        for (int c = 0; c < NC; ++c) pre[c] = fb(c);
+    }
 }
 
 // Tile prefix of NC components in one call: thread 0 publishes the tile's
 // aggregates s_agg[0..NC) (LDS, written and synchronised by the caller),
 // wave 0 resolves the exclusive prefixes into s_pre[0..NC) (LDS) and
 // publishes the inclusive ones.  Ends with a __syncthreads.
-template <int NC>
+template <int NC, typename Fallback>
 __device__ __forceinline__ void lb_tile_prefix(const uint32_t *s_agg, uint32_t *s_pre, uint64_t *st, int64_t tile,
-                                               uint32_t tag, unsigned int *ctl)
+                                               uint32_t tag, unsigned int *ctl, unsigned int *fault, int limit,
+                                               Fallback &&fb)
 {
     if (threadIdx.x == 0) {
 #pragma unroll
@@ -161,7 +184,7 @@ __device__ __forceinline__ void lb_tile_prefix(const uint32_t *s_agg, uint32_t *
     if (threadIdx.x < 64) {
         uint32_t p[NC];
         if (tile > 0) {
-            lb_wave_lookback<NC>(st, tile, tag, ctl, p);
+            lb_wave_lookback<NC>(st, tile, tag, ctl, fault, limit, fb, p);
         } else {
 #pragma unroll
             for (int c = 0; c < NC; ++c) p[c] = 0u;
@@ -177,3 +200,16 @@ __device__ __forceinline__ void lb_tile_prefix(const uint32_t *s_agg, uint32_t *
     __syncthreads();
 }
 
+// Exact sum, in every lane, of the uint32 words src[i * stride] for i < n,
+// by one whole wave (the look-back fallback: wrapping arithmetic, as the
+// scans themselves).
+__device__ __forceinline__ uint32_t lb_wave_sum_prefix(const uint32_t *src, int64_t n, int stride)
+{
+    uint32_t s = 0;
+    for (int64_t i = lane_id(); i < n; i += 64) s += src[i * stride];
+    return wave_sum_u32(s);
+}
+
+// Poll limit of the look-back waits: LB_SPIN_LIMIT, or the value set by
+// dcsr_set_scan_wait_limit (host side, process-wide).
+int lb_spin_limit();

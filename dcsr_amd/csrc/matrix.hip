@@ -39,7 +39,7 @@ __global__ void k_init_rows(int64_t R, uint32_t L, uint32_t *start, int32_t *len
         ds->bump = (unsigned long long)R * L;
         ds->nnz = 0;
         ds->garbage = 0;
-        ds->status = 0;
+        ds->status &= ST_SCAN_FALLBACK;
         ds->hub_count = 0;
         ds->hub_chunks = 0;
     }
@@ -367,7 +367,8 @@ static dcsr_status status_from_bits(unsigned bits)
     return DCSR_OK;
 }
 
-__global__ void k_clear_status(DevState *ds) { ds->status = 0; }
+// Clears the error bits (the scan-fallback diagnostic stays for the handle's life).
+__global__ void k_clear_status(DevState *ds) { ds->status &= ST_SCAN_FALLBACK; }
 
 dcsr_status dcsr_get_status(dcsr_handle h)
 {
@@ -515,7 +516,7 @@ static dcsr_status load_sorted(dcsr_handle h, const uint64_t *key, const uint32_
     uint32_t *lens;
     DCSR_TRY(ws_get(h, WS_TMP1, (size_t)(R + 1) * sizeof(uint32_t), (void **)&lens));
     k_len_u32<<<grid_for(R, TB), TB, 0, s>>>(h->row_len, R, lens);
-    DCSR_HIP(scan_exclusive_u32(lens, h->row_start, R, 1, scan_part, flag + 1, s));
+    DCSR_HIP(scan_exclusive_u32(lens, h->row_start, R, 1, scan_part, flag + 1, s, fault_word(h)));
     int64_t want = (int64_t)(n * h->cfg.pool_factor);
     if (want < n) want = n;
     if (want > h->pool_cap || n > h->pool_cap) {
@@ -617,7 +618,7 @@ static dcsr_status row_offsets_u32(dcsr_handle h, uint32_t *off, hipStream_t s)
     const int64_t R = h->rows;
     DCSR_TRY(ws_get(h, WS_PART, scan_part_words(R + 1, 1) * 4, (void **)&part));
     // lengths are non-negative int32: scan them as uint32
-    DCSR_HIP(scan_exclusive_u32((const uint32_t *)h->row_len, off, R, 1, part, off + R, s));
+    DCSR_HIP(scan_exclusive_u32((const uint32_t *)h->row_len, off, R, 1, part, off + R, s, fault_word(h)));
     return DCSR_OK;
 }
 
@@ -671,9 +672,9 @@ dcsr_status dcsr_export_csr(dcsr_handle h, int32_t *row_offsets, int32_t *cols, 
     launch_copy_rows(off, R, nnz, h->row_start, h->pool_col, h->pool_val, tcol, tval, s);
     if (nnz > 0) {
         k_canon_flags<<<grid_for(nnz, TB), TB, 0, s>>>(toff, R, tcol, tval, nnz, emit, tsum);
-        DCSR_HIP(scan_exclusive_u32(emit, pos, nnz, 1, part, tot, s));
+        DCSR_HIP(scan_exclusive_u32(emit, pos, nnz, 1, part, tot, s, fault_word(h)));
     } else {
-        DCSR_HIP(scan_exclusive_u32(emit, pos, 0, 1, part, tot, s));
+        DCSR_HIP(scan_exclusive_u32(emit, pos, 0, 1, part, tot, s, fault_word(h)));
     }
     uint32_t total = 0;
     DCSR_HIP(hipMemcpyAsync(&total, tot, 4, hipMemcpyDeviceToHost, s));
@@ -827,5 +828,22 @@ dcsr_status dcsr_query(dcsr_handle h, dcsr_info *info)
     info->last_touched_rows = h->last_touched;
     info->canonical = h->canonical ? 1 : 0;
     info->max_segments = 1;
+    info->scan_fallback = (h->hs->status & ST_SCAN_FALLBACK) ? 1 : 0;
+    info->spmv_paneled = h->pnl_valid ? 1 : 0;
     return DCSR_OK;
+}
+
+dcsr_status dcsr_export_layout(dcsr_handle h, uint32_t *row_start, int32_t *row_len, int32_t *row_cap,
+                               dcsr_stream_t stream)
+{
+    if (!h || (h->rows > 0 && (!row_start || !row_len || !row_cap))) return DCSR_ERR_INVALID;
+    hipStream_t s = (hipStream_t)stream;
+    h->last_stream = s;
+    const size_t R = (size_t)h->rows;
+    if (R > 0) {
+        DCSR_HIP(hipMemcpyAsync(row_start, h->row_start, R * 4, hipMemcpyDeviceToDevice, s));
+        DCSR_HIP(hipMemcpyAsync(row_len, h->row_len, R * 4, hipMemcpyDeviceToDevice, s));
+        DCSR_HIP(hipMemcpyAsync(row_cap, h->row_cap, R * 4, hipMemcpyDeviceToDevice, s));
+    }
+    return pull_state(h, s);
 }

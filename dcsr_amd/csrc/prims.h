@@ -5,12 +5,16 @@
 #include <stdint.h>
 
 // Exclusive prefix sum of NC interleaved uint32 components (in[i*NC + c]),
-// n elements, in place allowed.  `total` (device, NC words) receives the sums.
+// n elements, out of place (in != out: a tile whose bounded look-back wait
+// gives up re-reads the inputs of the tiles before it).  `total` (device, NC
+// words) receives the sums.
 // `part` is scratch of scan_part_words(n, NC) words, zero-filled when first
 // allocated; calls that share a `part` buffer must be stream-ordered.
+// `fault` (device, may be null) is the owner's sticky status word: a tile
+// that took that exact fallback sets LB_FALLBACK_BIT (0x8) in it.
 size_t scan_part_words(int64_t n, int nc);
 hipError_t scan_exclusive_u32(const uint32_t *in, uint32_t *out, int64_t n, int nc, uint32_t *part,
-                              uint32_t *total, hipStream_t s);
+                              uint32_t *total, hipStream_t s, unsigned int *fault);
 
 // Entries per tile of the radix sort's count table (digit-major: the count
 // of digit d in tile t sits at hist[d * tiles + t]).

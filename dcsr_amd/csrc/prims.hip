@@ -34,8 +34,9 @@ constexpr int SCAN_TILE = SCAN_THREADS * SCAN_IPT;
 // local blocked scan, decoupled look-back per component, write.
 // part: [ctl 4 words][status: tiles x NC words of 64 bits].
 template <int NC>
-__global__ __launch_bounds__(SCAN_THREADS) void k_scan1(const uint32_t *in, uint32_t *out, int64_t n,
-                                                        uint32_t *__restrict__ part, uint32_t *__restrict__ total)
+__global__ __launch_bounds__(SCAN_THREADS) void k_scan1(const uint32_t *__restrict__ in, uint32_t *__restrict__ out,
+                                                        int64_t n, uint32_t *__restrict__ part,
+                                                        uint32_t *__restrict__ total, unsigned int *fault, int limit)
 {
     __shared__ uint32_t tile_s[NC][SCAN_TILE];
     __shared__ uint32_t lds4[4];
@@ -82,7 +83,10 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_scan1(const uint32_t *in, uint
         if (threadIdx.x == 0) s_agg[c] = tot;
     }
     __syncthreads();
-    lb_tile_prefix<NC>(s_agg, s_pre, st, tile, tag, ctl);
+    // fallback: the inputs of every earlier tile (never overwritten: the
+    // scan is out of place)
+    lb_tile_prefix<NC>(s_agg, s_pre, st, tile, tag, ctl, fault, limit,
+                       [&](int c) { return lb_wave_sum_prefix(in + c, base, NC); });
     if (threadIdx.x < NC && tile == ntiles - 1 && total) total[threadIdx.x] = s_pre[threadIdx.x] + s_agg[threadIdx.x];
 #pragma unroll
     for (int c = 0; c < NC; ++c) {
@@ -131,45 +135,46 @@ __global__ __launch_bounds__(RS_THREADS) void k_rs_hist(const uint64_t *__restri
 
 // Exclusive scan of each digit's row of the count table (digit-major:
 // hist[d * nb + tile]), in place, one workgroup per digit; dtot[d] gets the
-// digit's total.  Rows of up to RS_SCAN_CHUNKS * 2048 tiles: every load of
-// the row is in flight before the first scan step.
+// digit's total.  The row is taken RS_SCAN_CHUNKS * 2048 tiles at a time,
+// every load of such a group in flight before its first scan step (one
+// group covers batches of up to 16M keys).
 constexpr int RS_SCAN_CHUNKS = 4;
 __global__ __launch_bounds__(RS_THREADS) void k_rs_scan_digits(uint32_t *__restrict__ hist, int64_t nb,
                                                                uint32_t *__restrict__ dtot)
 {
     __shared__ uint32_t lds4[4];
     uint32_t *row = hist + (int64_t)blockIdx.x * nb;
-    uint32_t v[RS_SCAN_CHUNKS][RS_IPT];
-#pragma unroll
-    for (int c = 0; c < RS_SCAN_CHUNKS; ++c) {
-        const int64_t i0 = (int64_t)c * RS_TILE + (int64_t)threadIdx.x * RS_IPT;
-#pragma unroll
-        for (int j = 0; j < RS_IPT; ++j) v[c][j] = i0 + j < nb ? row[i0 + j] : 0u;
-    }
     uint32_t carry = 0;
+    for (int64_t g0 = 0; g0 < nb; g0 += (int64_t)RS_SCAN_CHUNKS * RS_TILE) {
+        uint32_t v[RS_SCAN_CHUNKS][RS_IPT];
 #pragma unroll
-    for (int c = 0; c < RS_SCAN_CHUNKS; ++c) {
-        if ((int64_t)c * RS_TILE >= nb) break;
-        const int64_t i0 = (int64_t)c * RS_TILE + (int64_t)threadIdx.x * RS_IPT;
-        uint32_t s = 0;
+        for (int c = 0; c < RS_SCAN_CHUNKS; ++c) {
+            const int64_t i0 = g0 + (int64_t)c * RS_TILE + (int64_t)threadIdx.x * RS_IPT;
 #pragma unroll
-        for (int j = 0; j < RS_IPT; ++j) s += v[c][j];
-        uint32_t tot;
-        uint32_t p = carry + block_excl_scan_256(s, lds4, &tot);
-#pragma unroll
-        for (int j = 0; j < RS_IPT; ++j) {
-            if (i0 + j < nb) row[i0 + j] = p;
-            p += v[c][j];
+            for (int j = 0; j < RS_IPT; ++j) v[c][j] = i0 + j < nb ? row[i0 + j] : 0u;
         }
-        carry += tot;
+#pragma unroll
+        for (int c = 0; c < RS_SCAN_CHUNKS; ++c) {
+            if (g0 + (int64_t)c * RS_TILE >= nb) break;
+            const int64_t i0 = g0 + (int64_t)c * RS_TILE + (int64_t)threadIdx.x * RS_IPT;
+            uint32_t s = 0;
+#pragma unroll
+            for (int j = 0; j < RS_IPT; ++j) s += v[c][j];
+            uint32_t tot;
+            uint32_t p = carry + block_excl_scan_256(s, lds4, &tot);
+#pragma unroll
+            for (int j = 0; j < RS_IPT; ++j) {
+                if (i0 + j < nb) row[i0 + j] = p;
+                p += v[c][j];
+            }
+            carry += tot;
+        }
     }
     if (threadIdx.x == 0) dtot[blockIdx.x] = carry;
 }
 
 // Stable scatter of one tile.  offs[d * nb + tile] is the tile's offset
-// within digit d (dtot != nullptr: the per-digit scan, whose digit totals
-// dtot the scatter turns into digit bases) or the global offset (dtot ==
-// nullptr: a scan of the whole table).
+// within digit d; the scatter turns the digit totals dtot into digit bases.
 __global__ __launch_bounds__(RS_THREADS) void k_rs_scatter(const uint64_t *__restrict__ kin,
                                                            const uint32_t *__restrict__ vin,
                                                            uint64_t *__restrict__ kout, uint32_t *__restrict__ vout,
@@ -188,7 +193,7 @@ __global__ __launch_bounds__(RS_THREADS) void k_rs_scatter(const uint64_t *__res
     const int lane = lane_id();
     // the tile's offsets do not depend on the ranking: load them first
     const uint32_t go = offs[(int64_t)threadIdx.x * nblocks + blockIdx.x];
-    const uint32_t dt = dtot ? dtot[threadIdx.x] : 0u;
+    const uint32_t dt = dtot[threadIdx.x];
     for (int i = threadIdx.x; i < RS_WAVES * 256; i += RS_THREADS) (&wcnt[0][0])[i] = 0;
     __syncthreads();
 
@@ -263,17 +268,18 @@ size_t scan_part_words(int64_t n, int nc)
 }
 
 hipError_t scan_exclusive_u32(const uint32_t *in, uint32_t *out, int64_t n, int nc, uint32_t *part,
-                              uint32_t *total, hipStream_t s)
+                              uint32_t *total, hipStream_t s, unsigned int *fault)
 {
+    if (in == out) return hipErrorInvalidValue;  // the look-back fallback re-reads inputs
     if (n <= 0) {
         if (total) k_fill_u32<<<1, 64, 0, s>>>(total, nc, 0u);
         return hipGetLastError();
     }
     const int64_t nb = (n + SCAN_TILE - 1) / SCAN_TILE;
     if (nc == 1)
-        k_scan1<1><<<(unsigned)nb, SCAN_THREADS, 0, s>>>(in, out, n, part, total);
+        k_scan1<1><<<(unsigned)nb, SCAN_THREADS, 0, s>>>(in, out, n, part, total, fault, lb_spin_limit());
     else if (nc == 2)
-        k_scan1<2><<<(unsigned)nb, SCAN_THREADS, 0, s>>>(in, out, n, part, total);
+        k_scan1<2><<<(unsigned)nb, SCAN_THREADS, 0, s>>>(in, out, n, part, total, fault, lb_spin_limit());
     else
         return hipErrorInvalidValue;
     return hipGetLastError();
@@ -286,36 +292,23 @@ size_t radix_hist_words(int64_t n)
     return (size_t)(nb < 1 ? 1 : nb) * 256 + 256;
 }
 
-static bool radix_digit_scan(int64_t nb) { return nb <= (int64_t)RS_SCAN_CHUNKS * RS_TILE; }
+size_t radix_part_words(int64_t) { return 4; }
 
-size_t radix_part_words(int64_t n)
-{
-    const int64_t nb = (n + RS_TILE - 1) / RS_TILE;
-    return radix_digit_scan(nb) ? 4 : scan_part_words(nb * 256, 1);
-}
-
-// Per pass: per-tile digit counts (digit-major, tile-minor), a scan of that
-// table (per digit up to 8192 tiles, else one single-pass scan of the whole
-// table), stable scatter.
+// Per pass: per-tile digit counts (digit-major, tile-minor), a per-digit
+// scan of that table, stable scatter.
 hipError_t radix_sort_pairs(uint64_t *k0, uint32_t *v0, uint64_t *k1, uint32_t *v1, int64_t n, int nbits,
-                            uint32_t *hist, uint32_t *part, int *which, hipStream_t s, bool counted)
+                            uint32_t *hist, uint32_t *, int *which, hipStream_t s, bool counted)
 {
     *which = 0;
     if (n <= 1) return hipSuccess;
     const int64_t nb = (n + RS_TILE - 1) / RS_TILE;
-    const bool per_digit = radix_digit_scan(nb);
-    uint32_t *dtot = per_digit ? hist + nb * 256 : nullptr;
+    uint32_t *dtot = hist + nb * 256;
     uint64_t *ki = k0, *ko = k1;
     uint32_t *vi = v0, *vo = v1;
     int cur = 0;
     for (int shift = 0; shift < nbits; shift += 8) {
         if (!(counted && shift == 0)) k_rs_hist<<<(unsigned)nb, RS_THREADS, 0, s>>>(ki, n, shift, hist, nb);
-        if (per_digit) {
-            k_rs_scan_digits<<<256, RS_THREADS, 0, s>>>(hist, nb, dtot);
-        } else {
-            const hipError_t e = scan_exclusive_u32(hist, hist, nb * 256, 1, part, nullptr, s);
-            if (e != hipSuccess) return e;
-        }
+        k_rs_scan_digits<<<256, RS_THREADS, 0, s>>>(hist, nb, dtot);
         k_rs_scatter<<<(unsigned)nb, RS_THREADS, 0, s>>>(ki, vi, ko, vo, n, shift, hist, nb, dtot);
         uint64_t *tk = ki; ki = ko; ko = tk;
         uint32_t *tv = vi; vi = vo; vo = tv;
@@ -324,3 +317,8 @@ hipError_t radix_sort_pairs(uint64_t *k0, uint32_t *v0, uint64_t *k1, uint32_t *
     *which = cur;
     return hipGetLastError();
 }
+
+static int g_lb_limit = LB_SPIN_LIMIT;
+int lb_spin_limit() { return g_lb_limit; }
+
+extern "C" void dcsr_set_scan_wait_limit(int32_t polls) { g_lb_limit = polls < 0 ? LB_SPIN_LIMIT : polls; }

@@ -653,17 +653,20 @@ __global__ void k_sg_set_rows(uint32_t R, const uint32_t *__restrict__ Poff, con
                               DevState *ds)
 {
     const uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
+    // a row's chunk keeps its P_r slots at Poff[r]; the P_r - cnt_r it does
+    // not fill are counted as garbage, so its capacity is its length (bump
+    // = sum of capacities + garbage, as for every other producer)
     if (r < R) {
         start[r] = Poff[r];
         len[r] = (int32_t)cnt[r];
-        cap[r] = (int32_t)P[r];
+        cap[r] = (int32_t)cnt[r];
     }
     if (r == 0) {
         const uint32_t tot = nnz[0];
         ds->bump = ptot;
         ds->nnz = tot;
         ds->garbage = ptot - tot;
-        ds->status = 0;
+        ds->status &= ST_SCAN_FALLBACK;
     }
 }
 
@@ -707,7 +710,7 @@ dcsr_status dcsr_spgemm(dcsr_handle A, dcsr_handle B, dcsr_handle C, dcsr_stream
         k_sg_classify<<<nbR, TB, 0, s>>>(R, P, lists, R + 1, st, cnt, bsum, B->cols <= TEAM_MAX_COLS ? 1u : 0u);
         k_sg_total<<<1, 1024, 0, s>>>(st, bsum, (int)nbR);
     }
-    DCSR_HIP(scan_exclusive_u32(P, Poff, R, 1, part, Poff + R, s));
+    DCSR_HIP(scan_exclusive_u32(P, Poff, R, 1, part, Poff + R, s, fault_word(C)));
     SgState hst;
     DCSR_HIP(hipMemcpyAsync(&hst, st, sizeof(SgState), hipMemcpyDeviceToHost, s));
     DCSR_HIP(hipStreamSynchronize(s));
@@ -767,7 +770,7 @@ dcsr_status dcsr_spgemm(dcsr_handle A, dcsr_handle B, dcsr_handle C, dcsr_stream
         uint32_t htot[2];
         uint32_t *htot_d;
         DCSR_TRY(ws_get(C, WS_TMP2, 64, (void **)&htot_d));
-        DCSR_HIP(scan_exclusive_u32(hpre_in, hpre, hn + 1, 2, part, htot_d, s));
+        DCSR_HIP(scan_exclusive_u32(hpre_in, hpre, hn + 1, 2, part, htot_d, s, fault_word(C)));
         DCSR_HIP(hipMemcpyAsync(htot, htot_d, 8, hipMemcpyDeviceToHost, s));
         DCSR_HIP(hipStreamSynchronize(s));
         const uint32_t nent = htot[0], np = htot[1];
@@ -782,7 +785,7 @@ dcsr_status dcsr_spgemm(dcsr_handle A, dcsr_handle B, dcsr_handle C, dcsr_stream
         if (scan_part_words((int64_t)np + 1, 1) > pw) pw = scan_part_words((int64_t)np + 1, 1);
         if (radix_part_words(np) > pw) pw = radix_part_words(np);
         DCSR_TRY(ws_get(C, WS_SCAN, pw * 4, (void **)&part));
-        DCSR_HIP(scan_exclusive_u32(blen, apos, nent, 1, part, apos + nent, s));
+        DCSR_HIP(scan_exclusive_u32(blen, apos, nent, 1, part, apos + nent, s, fault_word(C)));
         DCSR_TRY(ws_get(C, WS_SG_KEY0, (size_t)(np + 1) * 8, (void **)&k0));
         DCSR_TRY(ws_get(C, WS_SG_KEY1, (size_t)(np + 1) * 8, (void **)&k1));
         DCSR_TRY(ws_get(C, WS_SG_VAL0, (size_t)(np + 1) * 4, (void **)&v0));
@@ -800,7 +803,7 @@ dcsr_status dcsr_spgemm(dcsr_handle A, dcsr_handle B, dcsr_handle C, dcsr_stream
         sum = (float *)(emit + (np + 2));
         DCSR_TRY(ws_get(C, WS_SG_EPOS, (size_t)(np + 2) * 4, (void **)&epos));
         k_sg_hreduce<<<grid_for(np + 1, TB), TB, 0, s>>>(key, val, np, emit, sum);
-        DCSR_HIP(scan_exclusive_u32(emit, epos, np + 1, 1, part, nullptr, s));
+        DCSR_HIP(scan_exclusive_u32(emit, epos, np + 1, 1, part, nullptr, s, fault_word(C)));
         k_sg_hwrite<<<grid_for(np, TB), TB, 0, s>>>(key, np, C->cbits, emit, epos, sum, hlist, hpre, Poff, tcol, tval);
         k_sg_hcount<<<grid_for(hn, TB), TB, 0, s>>>(hlist, hn, hpre, epos, cnt);
     }
@@ -809,7 +812,7 @@ dcsr_status dcsr_spgemm(dcsr_handle A, dcsr_handle B, dcsr_handle C, dcsr_stream
     // 3. C's row table: chunks at the partial offsets; nnz from a scan of
     // the row counts
     DCSR_TRY(ws_get(C, WS_PART, scan_part_words(R + 1, 1) * 4, (void **)&part));
-    DCSR_HIP(scan_exclusive_u32(cnt, coff, R, 1, part, coff + R, s));
+    DCSR_HIP(scan_exclusive_u32(cnt, coff, R, 1, part, coff + R, s, fault_word(C)));
     k_sg_set_rows<<<grid_for(R + 1, TB), TB, 0, s>>>(R, Poff, P, cnt, coff + R, Ptot, C->row_start, C->row_len,
                                                      C->row_cap, C->ds);
     DCSR_HIP(hipGetLastError());

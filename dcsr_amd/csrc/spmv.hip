@@ -60,7 +60,7 @@ constexpr int SR_IPT = 16;
 constexpr int SR_TILE = TB * SR_IPT;
 __global__ __launch_bounds__(TB) void k_scan_rows(uint32_t R, const int32_t *__restrict__ len,
                                                   uint32_t *__restrict__ foff, uint32_t *__restrict__ rowmap,
-                                                  uint32_t *__restrict__ part)
+                                                  uint32_t *__restrict__ part, unsigned int *fault, int limit)
 {
     __shared__ uint32_t lds4[4], s_tk[2], s_pre;
     unsigned int *ctl = part;
@@ -92,7 +92,10 @@ __global__ __launch_bounds__(TB) void k_scan_rows(uint32_t R, const int32_t *__r
     if (threadIdx.x == 0) lb_publish(&st[tile], tag, tile == 0 ? LB_INC : LB_AGG, tot);
     if (threadIdx.x < 64) {
         uint32_t p[1] = {0u};
-        if (tile > 0) lb_wave_lookback<1>(st, tile, tag, ctl, p);
+        // fallback: the lengths of every earlier tile's rows
+        if (tile > 0)
+            lb_wave_lookback<1>(st, tile, tag, ctl, fault, limit,
+                                [&](int) { return lb_wave_sum_prefix((const uint32_t *)len, tile * SR_TILE, 1); }, p);
         if (threadIdx.x == 0) {
             s_pre = p[0];
             if (tile > 0) lb_publish(&st[tile], tag, LB_INC, p[0] + tot);
@@ -814,7 +817,8 @@ __global__ void k_pnl_bounds(uint32_t R, int P, int64_t panel, const uint32_t *_
     }
 }
 
-// Copy every live entry to its panel position: poff = exclusive scan of cnt.
+// Copy every live entry to its panel position: poff = exclusive scan of the
+// per-(panel, row) counts.
 __global__ void k_pnl_fill(uint32_t nnz, uint32_t R, int64_t panel, const uint32_t *__restrict__ foff,
                            const uint32_t *__restrict__ rowmap, const uint32_t *__restrict__ row_start,
                            const int32_t *__restrict__ col, const float *__restrict__ val,
@@ -878,7 +882,9 @@ dcsr_status ensure_flat(dcsr_handle h, hipStream_t s)
     uint32_t *rowmap;
     const int64_t ngcap = (h->pool_cap > live ? h->pool_cap : live) / GRAN + 1;
     DCSR_TRY(ws_get(h, WS_ROWMAP, (size_t)ngcap * 4, (void **)&rowmap));
-    if (R > 0) k_scan_rows<<<(unsigned)tiles, TB, 0, s>>>((uint32_t)R, h->row_len, foff, rowmap, part);
+    if (R > 0)
+        k_scan_rows<<<(unsigned)tiles, TB, 0, s>>>((uint32_t)R, h->row_len, foff, rowmap, part, fault_word(h),
+                                                   lb_spin_limit());
     DCSR_HIP(hipGetLastError());
     h->flat_dirty = false;
     return DCSR_OK;
@@ -984,9 +990,12 @@ static dcsr_status build_panels(dcsr_handle h, hipStream_t s)
     DCSR_TRY(ws_get(h, WS_PART, scan_part_words(Rv + 1, 1) * 4, (void **)&part));
     const uint32_t *foff = (const uint32_t *)h->ws.ptr[WS_FOFF];
     const uint32_t *rowmap = (const uint32_t *)h->ws.ptr[WS_ROWMAP];
+    // per-(panel, row) counts go to the partial buffer (free until the first
+    // paneled SpMV), then are scanned into poff
+    uint32_t *pcnt = (uint32_t *)ppart;
     k_pnl_bounds<<<grid_for(R, TB), TB, 0, s>>>((uint32_t)R, P, h->pnl_cols, h->row_start, h->row_len, h->pool_col,
-                                                poff, bnd);
-    DCSR_HIP(scan_exclusive_u32(poff, poff, Rv, 1, part, poff + Rv, s));
+                                                pcnt, bnd);
+    DCSR_HIP(scan_exclusive_u32(pcnt, poff, Rv, 1, part, poff + Rv, s, fault_word(h)));
     if (nnz > 0)
         k_pnl_fill<<<grid_for(nnz, TB), TB, 0, s>>>((uint32_t)nnz, (uint32_t)R, h->pnl_cols, foff, rowmap,
                                                     h->row_start, h->pool_col, h->pool_val, poff, bnd, pcol, pval);

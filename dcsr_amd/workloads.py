@@ -19,6 +19,9 @@ Workloads (sizes as in the reference's driver configurations):
   existing coordinates (sampled uniformly among stored entries, hence rows in
   proportion to degree), the rest are new columns on degree-drawn rows.
 * ``many_small``  -- many independent small matrices (batched launch).
+* ``serving_matrix`` / ``serving_stream`` -- the resident 10M-row power-law
+  matrix of the serving loop (coordinates made unique on the GPU) and its
+  Poisson request mix of update batches and SpMV / SpMM queries.
 
 Values are drawn from (0, 1] or (-1, 1] so that no generated value is an exact
 zero (an exact zero is a structural event in the insert-or-add semantics).
@@ -200,3 +203,54 @@ def many_small(count=4096, min_rows=64, max_rows=4096, lam=16.0, seed=4):
         rr, cc, vv = _sorted_coo(r, c, v, n)
         mats.append((n, rr, cc, vv))
     return mats
+
+
+def serving_matrix(R, N, seed=5):
+    """Zipf(2.1) out-degrees scaled to N (capped at R/10), uniform columns;
+    coordinates made unique on the GPU.  Returns host (rows, cols, vals)."""
+    import torch
+    rng = np.random.default_rng(seed)
+    d, _ = zipf_degrees(rng, R, N, 2.1, max(R // 10, 1))
+    g = torch.Generator(device="cuda")
+    g.manual_seed(seed)
+    rows = torch.repeat_interleave(torch.arange(R, device="cuda", dtype=torch.int64),
+                                   torch.from_numpy(d).to("cuda"))
+    cols = torch.randint(0, R, (rows.numel(),), device="cuda", generator=g, dtype=torch.int64)
+    key = torch.unique(rows * R + cols, sorted=True)
+    del rows, cols
+    r = (key // R).to(torch.int32).cpu().numpy()
+    c = (key % R).to(torch.int32).cpu().numpy()
+    del key
+    v = (1.0 - rng.random(r.size, dtype=np.float32)).astype(np.float32)
+    return r, c, v
+
+
+def serving_stream(rng, r, c, v, R, nreq, upd_frac=0.1):
+    """Request list: (kind, k or n, arrays).  Updates: log-uniform 1e3..1e6
+    triples, 30% deletions (-current of a coordinate no earlier request
+    touched), 35% adds to stored coordinates, 35% new coordinates on
+    degree-drawn rows.  Queries: k = 1, 4, 16, 32 with weights .55/.2/.15/.1."""
+    dirty = np.zeros(r.size, bool)
+    reqs = []
+    ks = np.array([1, 4, 16, 32])
+    for _ in range(nreq):
+        if rng.random() < upd_frac:
+            n = int(np.exp(rng.uniform(np.log(1e3), np.log(1e6))))
+            nd = int(0.3 * n)
+            na = int(0.35 * n)
+            nn = n - nd - na
+            pick = rng.integers(0, r.size, 2 * nd + 16)
+            pick = np.unique(pick[~dirty[pick]])[:nd]
+            nd = pick.size
+            add = rng.integers(0, r.size, na)
+            dirty[pick] = True
+            dirty[add] = True
+            newrow = r[rng.integers(0, r.size, nn)]
+            rows = np.concatenate([r[pick], r[add], newrow]).astype(np.int32)
+            cols = np.concatenate([c[pick], c[add], rng.integers(0, R, nn)]).astype(np.int32)
+            vals = np.concatenate([-v[pick], (1.0 - rng.random(na + nn, dtype=np.float32))]).astype(np.float32)
+            perm = rng.permutation(rows.size)
+            reqs.append(("update", rows.size, (rows[perm], cols[perm], vals[perm])))
+        else:
+            reqs.append(("query", int(rng.choice(ks, p=[0.55, 0.2, 0.15, 0.1])), None))
+    return reqs

@@ -87,6 +87,11 @@ typedef struct dcsr_info {
     int64_t last_touched_rows;
     int32_t canonical;    /* 1 if rows hold no duplicate columns and no exact zeros */
     int32_t max_segments; /* segments per row: always 1 in this layout */
+    int32_t scan_fallback; /* 1 if a single-pass scan on this handle ever gave up
+                              its bounded look-back wait and took its exact
+                              (slow) fallback: results stay correct; 0 in a
+                              healthy run */
+    int32_t spmv_paneled; /* 1 while SpMV runs over the column-paneled copy */
 } dcsr_info;
 
 const char *dcsr_status_string(dcsr_status s);
@@ -269,8 +274,88 @@ dcsr_status dcsr_server_drain(dcsr_server s);
 dcsr_status dcsr_server_get_stats(dcsr_server s, dcsr_server_stats *out);
 dcsr_status dcsr_server_destroy(dcsr_server s);
 
+/* ---------------------------------------------------------------------------
+ * Dynamic HYB matrix (ELL + COO overflow), the reference's hyb_matrix
+ * wrapper (include/dcsr_matrix.h:148-172) with its load, append-update and
+ * SpMV.  Layout: `width` ELL slots per row, column-major with pitch = rows
+ * rounded up to 32 (slot k of row r at k * pitch + r; padding column -1,
+ * value 0); COO overflow sorted by row; row_sizes[r] = entries of row r
+ * (ELL + COO), row_sizes[rows] = COO count.  The view's device pointers stay
+ * valid until the next load / update of the store (an update may move the
+ * COO part to its other buffer).
+ * ------------------------------------------------------------------------- */
+typedef struct dcsr_hyb_s *dcsr_hyb;
+typedef struct dcsr_hyb_view {
+    int64_t rows, cols, width, pitch;
+    int32_t *ell_col;     /* width * pitch */
+    float *ell_val;
+    int32_t *row_sizes;   /* rows + 1 */
+    int32_t *coo_row, *coo_col;
+    float *coo_val;
+    int64_t num_overflow; /* COO entries */
+    int64_t coo_cap;
+} dcsr_hyb_view;
+
+/* An empty store; coo_cap is the initial COO capacity (grown on demand).
+ * Replaces hyb_matrix::resize(rows, cols, num_coo_entries, num_cols_per_row)
+ * (include/dcsr_matrix.h:160-171). */
+dcsr_status dcsr_hyb_create(dcsr_hyb *out, int64_t rows, int64_t cols, int32_t width, int64_t coo_cap);
+dcsr_status dcsr_hyb_destroy(dcsr_hyb h);
+/* Device pointers and sizes of the store (no sync). */
+dcsr_status dcsr_hyb_view_get(dcsr_hyb h, dcsr_hyb_view *v);
+/* COO -> HYB: stable by row, the first `width` entries of each row (input
+ * order) in its ELL slots, the rest in COO in row order; an out-of-range
+ * index rejects the load (no-op, DCSR_ERR_BAD_INDEX).  Syncs.  Replaces
+ * LoadMatrix(hyb&, rows, cols, vals, N) (include/load.h:61-106). */
+dcsr_status dcsr_hyb_load_coo(dcsr_hyb h, const int32_t *rows, const int32_t *cols, const float *vals, int64_t n,
+                              dcsr_stream_t stream);
+/* CSR -> HYB, same placement.  Syncs.  Replaces LoadMatrix(hyb&, const csr&)
+ * (include/load.h:174-205). */
+dcsr_status dcsr_hyb_load_csr(dcsr_hyb h, const int32_t *row_offsets, const int32_t *cols, const float *vals,
+                              dcsr_stream_t stream);
+/* Append update: in stable row order each triple of row r fills ELL slot
+ * row_sizes[r] while that is below the width, else it goes to r's COO
+ * overflow after r's earlier entries; no merging (a coordinate may be stored
+ * several times; the matrix is the sum of its entries).  Syncs once (COO
+ * capacity).  Replaces UpdateMatrix(hyb&, rows, cols, vals, N)
+ * (include/load.h:327-373 -> UpdateMatrix_hyb, include/sparse_update.inl:318-367). */
+dcsr_status dcsr_hyb_update(dcsr_hyb h, const int32_t *rows, const int32_t *cols, const float *vals, int64_t n,
+                            dcsr_stream_t stream);
+/* y = alpha * A x + beta * y; per row the ELL slots in slot order then the
+ * COO entries in order are added left to right from 0, the host hyb
+ * multiply's order (cusp/system/detail/sequential/multiply/hyb_spmv.h:42-58).
+ * Replaces cusp::multiply(hyb, x, y) as the reference's driver calls it
+ * (src/Fill_Test.inl:167-170). */
+dcsr_status dcsr_hyb_spmv(dcsr_hyb h, const float *x, float *y, float alpha, float beta, dcsr_stream_t stream);
+/* Copy the store into caller buffers (ELL width * pitch, row_sizes rows + 1,
+ * COO up to coo_cap; *ncoo receives the COO count).  Syncs. */
+dcsr_status dcsr_hyb_export(dcsr_hyb h, int32_t *ell_col, float *ell_val, int32_t *row_sizes, int32_t *coo_row,
+                            int32_t *coo_col, float *coo_val, int64_t coo_cap, int64_t *ncoo, dcsr_stream_t stream);
+/* HYB -> DCSR (every stored entry; duplicates kept, as dcsr_load_hyb) and
+ * DCSR -> HYB (the rows with columns ascending).  Sync. */
+dcsr_status dcsr_hyb_to_dcsr(dcsr_hyb h, dcsr_handle dst, dcsr_stream_t stream);
+dcsr_status dcsr_hyb_from_dcsr(dcsr_hyb h, dcsr_handle src, dcsr_stream_t stream);
+
 /* Counters and sizes of the handle (syncs). */
 dcsr_status dcsr_query(dcsr_handle h, dcsr_info *info);
+
+/* The row table of the pool allocator: for each row its chunk start, live
+ * length and chunk capacity (device arrays of `rows` entries).  With
+ * dcsr_query's pool_used (bump pointer), garbage and pool_cap it exposes the
+ * allocator's invariants: chunks [start, start + cap) are disjoint and lie
+ * below the bump pointer, len <= cap, bump <= pool_cap, and bump equals the
+ * sum of the capacities plus the garbage.  Syncs.  Diagnostic; the
+ * reference's row_offsets / row_sizes arrays (include/dcsr_matrix.h:54-57)
+ * were public members. */
+dcsr_status dcsr_export_layout(dcsr_handle h, uint32_t *row_start, int32_t *row_len, int32_t *row_cap,
+                               dcsr_stream_t stream);
+
+/* Poll limit of the bounded look-back waits of the single-pass scans
+ * (process-wide; default 2^20, negative restores it).  A tile that exceeds
+ * it computes its prefix exactly from the earlier tiles' inputs instead, so
+ * results never depend on it; 0 sends every tile down that fallback without
+ * waiting (used to test it).  No reference counterpart. */
+void dcsr_set_scan_wait_limit(int32_t polls);
 
 /* Plain CSR SpMV comparator, y = A x.  Replaces device::spmv(const csr&, x, y)
  * -> spmv_csr_vector_kernel (include/spmv.h:310-332, include/spmv.inl:708-781). */

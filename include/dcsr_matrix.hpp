@@ -3,12 +3,13 @@
 // and argument order:
 //   dcsr_matrix<I,V,MEM,BINS>           include/dcsr_matrix.h:8-109
 //   device::Initialize_Matrix           include/load.h:8-22
-//   device::LoadMatrix (COO/CSR->DCSR, DCSR->CSR, COO/CSR->HYB)
+//   device::LoadMatrix (COO/CSR->DCSR, DCSR->CSR, COO/CSR->HYB, DCSR<->HYB)
 //                                        include/load.h:27-59, 61-106, 174-253
-//   device::UpdateMatrix                include/load.h:258-325
+//   device::UpdateMatrix (DCSR, HYB)    include/load.h:258-325, 327-373
 //   device::SortMatrixRow               include/load.h:404-581
 //   device::BinRows                     include/load.h:583-610
-//   device::spmv (DCSR, CSR)            include/spmv.h:121-248, 310-332
+//   device::spmv (DCSR, CSR, HYB)      include/spmv.h:121-248, 310-332;
+//                                        src/Fill_Test.inl:167-170
 //   device::spmm (SpGEMM)               include/spmm.h:30-391
 // cusp containers are replaced by the repo-owned dcsr::device_array<T> and the
 // small csr_matrix / hyb_matrix holders below; errors throw std::runtime_error
@@ -114,25 +115,51 @@ struct csr_matrix {
     }
 };
 
-// HYB holder: column-major ELL block (width x pitch) plus COO overflow, the
-// layout of the reference's hyb_matrix wrapper (include/dcsr_matrix.h:148-172).
+// Dynamic HYB matrix (the reference's hyb_matrix wrapper,
+// include/dcsr_matrix.h:148-172): an owning handle of the library's HYB store
+// (ELL block of num_cols_per_row slots per row, column-major with pitch, COO
+// overflow sorted by row, row sizes).  The device arrays are reached through
+// view(); num_entries / num_overflow are refreshed by every operation.
 template <typename I = int, typename V = float, typename MEM = device_memory>
 struct hyb_matrix {
-    size_t num_rows = 0, num_cols = 0, num_cols_per_row = 0, pitch = 0, num_overflow = 0;
-    dcsr::device_array<I> ell_column_indices, coo_row_indices, coo_column_indices;
-    dcsr::device_array<V> ell_values, coo_values;
+    static_assert(std::is_same<I, int>::value && std::is_same<V, float>::value, "the engine is built for <int, float>");
+    dcsr_hyb h = nullptr;
+    size_t num_rows = 0, num_cols = 0, num_cols_per_row = 0, pitch = 0, num_entries = 0, num_overflow = 0;
+
+    hyb_matrix() = default;
+    ~hyb_matrix()
+    {
+        if (h) (void)dcsr_hyb_destroy(h);
+    }
+    hyb_matrix(const hyb_matrix &) = delete;
+    hyb_matrix &operator=(const hyb_matrix &) = delete;
+
+    // resize(rows, cols, num_coo_entries, num_cols_per_row): an empty store;
+    // the COO part starts with room for 2 * num_coo_entries, as the reference
+    // reserved, and grows on demand
     void resize(size_t rows, size_t cols, size_t num_coo_entries, size_t cols_per_row)
     {
+        if (h) (void)dcsr_hyb_destroy(h);
+        h = nullptr;
+        dcsr::throw_if(dcsr_hyb_create(&h, (int64_t)rows, (int64_t)cols, (int32_t)cols_per_row,
+                                       (int64_t)(2 * num_coo_entries)),
+                       "hyb_matrix::resize");
         num_rows = rows;
         num_cols = cols;
         num_cols_per_row = cols_per_row;
-        pitch = (rows + 31) / 32 * 32;
-        ell_column_indices.resize(cols_per_row * pitch);
-        ell_values.resize(cols_per_row * pitch);
-        coo_row_indices.resize(num_coo_entries);
-        coo_column_indices.resize(num_coo_entries);
-        coo_values.resize(num_coo_entries);
-        num_overflow = 0;
+        refresh();
+    }
+    dcsr_hyb_view view() const
+    {
+        dcsr_hyb_view v;
+        dcsr::throw_if(dcsr_hyb_view_get(h, &v), "dcsr_hyb_view_get");
+        return v;
+    }
+    void refresh()
+    {
+        const dcsr_hyb_view v = view();
+        pitch = (size_t)v.pitch;
+        num_overflow = (size_t)v.num_overflow;
     }
 };
 
@@ -223,34 +250,53 @@ void LoadMatrix(csr_matrix<I, V> &dst, const dcsr_matrix<I, V, device_memory, BI
     dst.num_entries = (size_t)got;
 }
 
-// DCSR -> HYB with the holder's num_cols_per_row
+// DCSR -> HYB with the holder's num_cols_per_row (rows with columns ascending)
 template <typename I, typename V, size_t BINS>
 void LoadMatrix(hyb_matrix<I, V> &dst, const dcsr_matrix<I, V, device_memory, BINS> &src, hipStream_t s = nullptr)
 {
-    const size_t nnz = src.num_entries();
-    if (dst.num_rows != src.num_rows) dst.resize(src.num_rows, src.num_cols, nnz, dst.num_cols_per_row);
-    if (dst.coo_values.size() < nnz) {
-        dst.coo_row_indices.resize(nnz);
-        dst.coo_column_indices.resize(nnz);
-        dst.coo_values.resize(nnz);
-    }
-    int64_t n = 0;
-    dcsr::throw_if(dcsr_export_hyb(src.h, (int32_t)dst.num_cols_per_row, (int64_t)dst.pitch,
-                                   dst.ell_column_indices.data(), dst.ell_values.data(), dst.coo_row_indices.data(),
-                                   dst.coo_column_indices.data(), dst.coo_values.data(),
-                                   (int64_t)dst.coo_values.size(), &n, s),
-                   "LoadMatrix(DCSR->HYB)");
-    dst.num_overflow = (size_t)n;
+    if (!dst.h || dst.num_rows != src.num_rows || dst.num_cols != src.num_cols)
+        dst.resize(src.num_rows, src.num_cols, 0, dst.num_cols_per_row);
+    dcsr::throw_if(dcsr_hyb_from_dcsr(dst.h, src.h, s), "LoadMatrix(DCSR->HYB)");
+    dst.refresh();
 }
 
-// HYB -> DCSR
+// HYB -> DCSR (every stored entry)
 template <typename I, typename V, size_t BINS>
 void LoadMatrix(dcsr_matrix<I, V, device_memory, BINS> &dst, const hyb_matrix<I, V> &src, hipStream_t s = nullptr)
 {
-    dcsr::throw_if(dcsr_load_hyb(dst.h, (int32_t)src.num_cols_per_row, (int64_t)src.pitch,
-                                 src.ell_column_indices.data(), src.ell_values.data(), src.coo_row_indices.data(),
-                                 src.coo_column_indices.data(), src.coo_values.data(), (int64_t)src.num_overflow, s),
-                   "LoadMatrix(HYB->DCSR)");
+    dcsr::throw_if(dcsr_hyb_to_dcsr(src.h, dst.h, s), "LoadMatrix(HYB->DCSR)");
+}
+
+// COO -> HYB: the first num_cols_per_row entries of each row fill its ELL
+// slots, the rest go to the COO overflow (rows need not be sorted)
+template <typename I, typename V>
+void LoadMatrix(hyb_matrix<I, V> &mat, const device_array<I> &rows, const device_array<I> &cols,
+                const device_array<V> &vals, const int N, hipStream_t s = nullptr)
+{
+    dcsr::throw_if(dcsr_hyb_load_coo(mat.h, rows.data(), cols.data(), vals.data(), N, s), "LoadMatrix(COO->HYB)");
+    mat.num_entries = (size_t)N;
+    mat.refresh();
+}
+
+// CSR -> HYB
+template <typename I, typename V>
+void LoadMatrix(hyb_matrix<I, V> &dst, const csr_matrix<I, V> &src, hipStream_t s = nullptr)
+{
+    dcsr::throw_if(dcsr_hyb_load_csr(dst.h, src.row_offsets.data(), src.column_indices.data(), src.values.data(), s),
+                   "LoadMatrix(CSR->HYB)");
+    dst.num_entries = src.num_entries;
+    dst.refresh();
+}
+
+// Append update of the HYB matrix (entries past a row's ELL slots overflow
+// into the COO part; duplicates are kept and add up in SpMV)
+template <typename I, typename V>
+void UpdateMatrix(hyb_matrix<I, V> &mat, const device_array<I> &rows, const device_array<I> &cols,
+                  const device_array<V> &vals, const int N, hipStream_t s = nullptr)
+{
+    dcsr::throw_if(dcsr_hyb_update(mat.h, rows.data(), cols.data(), vals.data(), N, s), "UpdateMatrix(HYB)");
+    mat.num_entries += (size_t)N;
+    mat.refresh();
 }
 
 // insert-or-add; `permuted` selected a kernel variant in the reference and
@@ -282,6 +328,13 @@ void spmv(const dcsr_matrix<I, V, device_memory, BINS> &A, const device_array<V>
           const hipStream_t *streams = nullptr)
 {
     dcsr::throw_if(dcsr_spmv(A.h, x.data(), y.data(), 1.0f, 0.0f, streams ? streams[0] : nullptr), "spmv");
+}
+
+// y = A x on the HYB matrix (the reference's driver called cusp::multiply)
+template <typename I, typename V>
+void spmv(const hyb_matrix<I, V> &A, const device_array<V> &x, device_array<V> &y, hipStream_t s = nullptr)
+{
+    dcsr::throw_if(dcsr_hyb_spmv(A.h, x.data(), y.data(), 1.0f, 0.0f, s), "spmv(HYB)");
 }
 
 template <typename I, typename V>

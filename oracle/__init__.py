@@ -64,6 +64,14 @@ def lib():
         _orc.orc_coo_to_csr.argtypes = [_I64, _I64, _P, _P, _P, _P]
         _orc.orc_update.restype = _I64
         _orc.orc_update.argtypes = [_I64, _I64, _P, _P, _P, _I64, _P, _P, _P, _P, _P, _P]
+        _orc.orc_sort_rows_by_col.restype = ctypes.c_int
+        _orc.orc_sort_rows_by_col.argtypes = [_I64, _P, _P, _P]
+        _orc.orc_hyb_spmv.restype = None
+        _orc.orc_hyb_spmv.argtypes = [_I64, _I64, _I64, _P, _P, _I64, _P, _P, _P, _P, _P]
+        _orc.orc_hyb_update.restype = _I64
+        _orc.orc_hyb_update.argtypes = [_I64, _I64, _I64, _P, _P, _P, _I64, _P, _P, _P, _I64, _P, _P, _P, _P, _P, _P]
+        _orc.orc_spgemm_check_f64.restype = _I64
+        _orc.orc_spgemm_check_f64.argtypes = [_I64, _I64] + [_P] * 9 + [ctypes.c_double, ctypes.POINTER(_I64)]
     return _orc
 
 
@@ -92,6 +100,8 @@ def ref():
         _ref.ref_csr_spmv.argtypes = [_I64, _I64, _P, _P, _P, _P, _P]
         _ref.ref_spgemm.restype = _I64
         _ref.ref_spgemm.argtypes = [_I64, _I64, _I64] + [_P] * 9
+        _ref.ref_hyb_spmv.restype = None
+        _ref.ref_hyb_spmv.argtypes = [_I64, _I64, _I64, _P, _P, _I64, _P, _P, _P, _P, _P]
         _ref.ref_csr_to_hyb.restype = _I64
         _ref.ref_csr_to_hyb.argtypes = [_I64, _I64, _P, _P, _P, _I64, _P, _P, _P, _P, _P, _P]
     return _ref
@@ -131,13 +141,20 @@ class Csr:
 
     def canonical(self):
         """Rows stably sorted by column (the comparison form; the stored order
-        inside a row is not part of the structure)."""
-        off = self.off
+        inside a row is not part of the structure).  Sorted per row in C
+        (orc_sort_rows_by_col); canonical_np is the numpy statement of the
+        same order, which pins it."""
         col = self.col.copy()
         val = self.val.copy()
+        if col.size and lib().orc_sort_rows_by_col(self.rows, _p(self.off), _p(col), _p(val)) != 0:
+            raise MemoryError("orc_sort_rows_by_col failed")
+        return Csr(self.rows, self.cols, self.off, col, val)
+
+    def canonical_np(self):
+        off = self.off
         rowid = np.repeat(np.arange(self.rows, dtype=np.int64), np.diff(off))
-        order = np.lexsort((np.arange(col.size), col.astype(np.int64), rowid))
-        return Csr(self.rows, self.cols, off, col[order], val[order])
+        order = np.lexsort((np.arange(self.col.size), self.col.astype(np.int64), rowid))
+        return Csr(self.rows, self.cols, off, self.col[order], self.val[order])
 
     def to_scipy(self, dtype=np.float64):
         import scipy.sparse as sp
@@ -240,6 +257,18 @@ def spgemm(A: Csr, B: Csr, use_ref=False) -> Csr:
     return Csr(A.rows, B.cols, off, col[:nnz], val[:nnz])
 
 
+def spgemm_check_f64(A: Csr, B: Csr, C: Csr, rtol=1e-5):
+    """Number of stored entries of C outside rtol * (|A||B|) of the FP64
+    product A*B (entries without any partial product count as failures),
+    and the flat index of the first one (-1 if none)."""
+    first = ctypes.c_int64(-1)
+    bad = lib().orc_spgemm_check_f64(A.rows, B.cols, _p(A.off), _p(A.col), _p(A.val), _p(B.off), _p(B.col),
+                                     _p(B.val), _p(C.off), _p(C.col), _p(C.val), float(rtol), ctypes.byref(first))
+    if bad < 0:
+        raise MemoryError("orc_spgemm_check_f64 failed")
+    return int(bad), int(first.value)
+
+
 def spgemm_count(A: Csr, B: Csr) -> int:
     return int(lib().orc_spgemm_count(A.rows, B.cols, _p(A.off), _p(A.col), _p(B.off), _p(B.col)))
 
@@ -267,3 +296,78 @@ def csr_to_hyb(A: Csr, width, use_ref=False):
                                   _p(ell_c), _p(ell_v), _p(cr), _p(cc), _p(cv))
     n = width * pitch
     return ell_c[:n], ell_v[:n], cr[:nc], cc[:nc], cv[:nc], pitch
+
+
+class Hyb:
+    """Host HYB matrix in the reference's dynamic-HYB layout
+    (include/dcsr_matrix.h:148-172): ELL block of `width` slots per row,
+    column-major with pitch (padding column -1, value 0), COO overflow sorted
+    by row, and row_sizes[r] = entries of row r (ELL + COO), row_sizes[R] =
+    COO count."""
+
+    __slots__ = ("rows", "cols", "width", "pitch", "row_sizes", "ell_col", "ell_val", "coo_row", "coo_col",
+                 "coo_val")
+
+    def __init__(self, rows, cols, width, pitch, row_sizes, ell_col, ell_val, coo_row, coo_col, coo_val):
+        self.rows, self.cols, self.width, self.pitch = int(rows), int(cols), int(width), int(pitch)
+        self.row_sizes = _i32(row_sizes)
+        self.ell_col, self.ell_val = _i32(ell_col), _f32(ell_val)
+        self.coo_row, self.coo_col, self.coo_val = _i32(coo_row), _i32(coo_col), _f32(coo_val)
+
+    @property
+    def ncoo(self):
+        return int(self.coo_row.size)
+
+    def to_coo(self):
+        """Every stored entry (ELL slots in slot order per row, then COO)."""
+        R, w, p = self.rows, self.width, self.pitch
+        ec = self.ell_col[: w * p].reshape(w, p)[:, :R]
+        ev = self.ell_val[: w * p].reshape(w, p)[:, :R]
+        k, rr = np.nonzero(ec != -1)
+        order = np.lexsort((k, rr))
+        r = np.concatenate([rr[order], self.coo_row])
+        c = np.concatenate([ec[k, rr][order], self.coo_col])
+        v = np.concatenate([ev[k, rr][order], self.coo_val])
+        return r.astype(np.int32), c.astype(np.int32), v.astype(np.float32)
+
+
+def csr_to_hyb_full(A: Csr, width, use_ref=False) -> Hyb:
+    """CSR -> HYB (cusp's generic conversion) with the row sizes."""
+    ec, ev, cr, cc, cv, pitch = csr_to_hyb(A, width, use_ref=use_ref)
+    rs = np.zeros(A.rows + 1, np.int32)
+    rs[: A.rows] = np.diff(A.off)
+    rs[A.rows] = cr.size
+    return Hyb(A.rows, A.cols, width, pitch, rs, ec, ev, cr, cc, cv)
+
+
+def coo_to_hyb(R, C, rows, cols, vals, width, use_ref=False) -> Hyb:
+    """COO -> HYB: sort_by_row + COO->CSR, then the CSR -> HYB conversion
+    (the first `width` entries of each row, in input order, fill its ELL
+    slots; the rest go to COO in row order)."""
+    return csr_to_hyb_full(coo_to_csr(R, C, rows, cols, vals, use_ref=use_ref), width, use_ref=use_ref)
+
+
+def hyb_update(H: Hyb, rows, cols, vals) -> Hyb:
+    """The reference's append-only HYB update (orc_hyb_update)."""
+    r, c, v = _i32(rows), _i32(cols), _f32(vals)
+    rs = H.row_sizes.copy()
+    ec, ev = H.ell_col.copy(), H.ell_val.copy()
+    cap = max(H.ncoo + r.size, 1)
+    orow, ocol, oval = np.zeros(cap, np.int32), np.zeros(cap, np.int32), np.zeros(cap, np.float32)
+    k = lib().orc_hyb_update(H.rows, H.width, H.pitch, _p(rs), _p(ec), _p(ev), H.ncoo, _p(H.coo_row), _p(H.coo_col),
+                             _p(H.coo_val), r.size, _p(r), _p(c), _p(v), _p(orow), _p(ocol), _p(oval))
+    if k < 0:
+        raise MemoryError("orc_hyb_update failed")
+    return Hyb(H.rows, H.cols, H.width, H.pitch, rs, ec, ev, orow[:k], ocol[:k], oval[:k])
+
+
+def hyb_spmv(H: Hyb, x, use_ref=False):
+    x = _f32(x)
+    y = np.zeros(H.rows, np.float32)
+    if use_ref:
+        ref().ref_hyb_spmv(H.rows, H.cols, H.width, _p(H.ell_col), _p(H.ell_val), H.ncoo, _p(H.coo_row),
+                           _p(H.coo_col), _p(H.coo_val), _p(x), _p(y))
+    else:
+        lib().orc_hyb_spmv(H.rows, H.width, H.pitch, _p(H.ell_col), _p(H.ell_val), H.ncoo, _p(H.coo_row),
+                           _p(H.coo_col), _p(H.coo_val), _p(x), _p(y))
+    return y

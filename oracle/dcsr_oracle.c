@@ -26,6 +26,21 @@
  *   orc_spmm_dense         no reference counterpart: defined as k column-wise
  *                          orc_csr_spmv calls; pinned against orc_csr_spmv and
  *                          scipy FP64.
+ *
+ *   orc_hyb_spmv           vs oracle/_ref (cusp::multiply on a host
+ *                          hyb_matrix) bit-exact, vs scipy FP64 within tol
+ *   orc_hyb_update         restates the reference's device-only append
+ *                          (no cusp host counterpart); pinned by a per-row
+ *                          list brute force and by the linear-operator
+ *                          identity HYB(A) + B == A + B (duplicates summed)
+ *
+ * Checker utilities (no reference counterpart; they compare, they do not
+ * restate):
+ *   orc_sort_rows_by_col   stable per-row sort by column (the comparison
+ *                          form); pinned against numpy's stable lexsort
+ *   orc_spgemm_check_f64   |C - A*B| <= rtol (|A||B|) per stored entry of C,
+ *                          with A*B and |A||B| accumulated in FP64; pinned
+ *                          against scipy FP64 products and planted errors
  */
 #include <stdint.h>
 #include <stdlib.h>
@@ -294,4 +309,171 @@ int64_t orc_update(int64_t R, int64_t C,
     else nnz = orc_csr_add(R, C, Aoff, Acol, Aval, boff, c, v, Coff, Ccol, Cval);
     free(r); free(c); free(v); free(boff);
     return nnz;
+}
+
+/* Stable sort of each row's (col, val) by column, in place (insertion sort
+ * for short rows, a stable merge sort through scratch for long ones). */
+static void merge_by_col(idx_t *c, float *v, idx_t *tc, float *tv, int64_t n)
+{
+    if (n <= 24) {
+        for (int64_t i = 1; i < n; ++i) {
+            idx_t kc = c[i];
+            float kv = v[i];
+            int64_t j = i - 1;
+            while (j >= 0 && c[j] > kc) { c[j + 1] = c[j]; v[j + 1] = v[j]; --j; }
+            c[j + 1] = kc;
+            v[j + 1] = kv;
+        }
+        return;
+    }
+    int64_t h = n / 2;
+    merge_by_col(c, v, tc, tv, h);
+    merge_by_col(c + h, v + h, tc, tv, n - h);
+    int64_t i = 0, j = h, k = 0;
+    while (i < h && j < n) {
+        if (c[j] < c[i]) { tc[k] = c[j]; tv[k] = v[j]; ++j; }
+        else { tc[k] = c[i]; tv[k] = v[i]; ++i; }
+        ++k;
+    }
+    while (i < h) { tc[k] = c[i]; tv[k] = v[i]; ++i; ++k; }
+    while (j < n) { tc[k] = c[j]; tv[k] = v[j]; ++j; ++k; }
+    memcpy(c, tc, n * sizeof(idx_t));
+    memcpy(v, tv, n * sizeof(float));
+}
+
+int orc_sort_rows_by_col(int64_t R, const idx_t *off, idx_t *col, float *val)
+{
+    int64_t maxlen = 1;
+    for (int64_t i = 0; i < R; ++i)
+        if (off[i + 1] - off[i] > maxlen) maxlen = off[i + 1] - off[i];
+    idx_t *tc = (idx_t *)malloc(maxlen * sizeof(idx_t));
+    float *tv = (float *)malloc(maxlen * sizeof(float));
+    if (!tc || !tv) { free(tc); free(tv); return -1; }
+    for (int64_t i = 0; i < R; ++i) merge_by_col(col + off[i], val + off[i], tc, tv, off[i + 1] - off[i]);
+    free(tc); free(tv);
+    return 0;
+}
+
+/* For every stored entry (i, k) of C: with s = sum_j a_ij b_jk and
+ * t = sum_j |a_ij b_jk| accumulated in FP64, count |c_ik - s| > rtol * t.
+ * Also counts stored entries with no partial product at all.  Returns the
+ * number of failing entries (-1 on allocation failure); *first_bad receives
+ * the flat index of the first one (or -1). */
+int64_t orc_spgemm_check_f64(int64_t R, int64_t Cb,
+                             const idx_t *Aoff, const idx_t *Acol, const float *Aval,
+                             const idx_t *Boff, const idx_t *Bcol, const float *Bval,
+                             const idx_t *Coff, const idx_t *Ccol, const float *Cval,
+                             double rtol, int64_t *first_bad)
+{
+    double *s = (double *)calloc(Cb > 0 ? Cb : 1, sizeof(double));
+    double *t = (double *)calloc(Cb > 0 ? Cb : 1, sizeof(double));
+    unsigned char *hit = (unsigned char *)calloc(Cb > 0 ? Cb : 1, 1);
+    if (!s || !t || !hit) { free(s); free(t); free(hit); return -1; }
+    int64_t bad = 0;
+    *first_bad = -1;
+    for (int64_t i = 0; i < R; ++i) {
+        for (idx_t jj = Aoff[i]; jj < Aoff[i + 1]; ++jj) {
+            const idx_t j = Acol[jj];
+            const double a = (double)Aval[jj];
+            for (idx_t kk = Boff[j]; kk < Boff[j + 1]; ++kk) {
+                const idx_t k = Bcol[kk];
+                const double p = a * (double)Bval[kk];
+                s[k] += p;
+                t[k] += p < 0 ? -p : p;
+                hit[k] = 1;
+            }
+        }
+        for (idx_t e = Coff[i]; e < Coff[i + 1]; ++e) {
+            const idx_t k = Ccol[e];
+            const double d = (double)Cval[e] - s[k];
+            if (!hit[k] || (d < 0 ? -d : d) > rtol * t[k]) {
+                if (*first_bad < 0) *first_bad = e;
+                bad++;
+            }
+        }
+        for (idx_t jj = Aoff[i]; jj < Aoff[i + 1]; ++jj) {
+            const idx_t j = Acol[jj];
+            for (idx_t kk = Boff[j]; kk < Boff[j + 1]; ++kk) {
+                const idx_t k = Bcol[kk];
+                s[k] = 0.0;
+                t[k] = 0.0;
+                hit[k] = 0;
+            }
+        }
+    }
+    free(s); free(t); free(hit);
+    return bad;
+}
+
+/* y = A x for HYB = ELL (width slots per row, column-major with pitch,
+ * padding column -1) + COO (sorted by row): the host path initializes y to 0,
+ * adds the ELL slots slot by slot (n outer, rows inner), then the COO entries
+ * in stored order.  cusp/system/detail/sequential/multiply/hyb_spmv.h:42-58,
+ * ell_spmv.h:54-73, coo_spmv.h:57-73. */
+void orc_hyb_spmv(int64_t R, int64_t width, int64_t pitch, const idx_t *ell_col, const float *ell_val,
+                  int64_t ncoo, const idx_t *coo_row, const idx_t *coo_col, const float *coo_val,
+                  const float *x, float *y)
+{
+    for (int64_t i = 0; i < R; ++i) y[i] = 0.0f;
+    for (int64_t n = 0; n < width; ++n)
+        for (int64_t i = 0; i < R; ++i) {
+            const idx_t j = ell_col[n * pitch + i];
+            if (j != -1) y[i] = y[i] + ell_val[n * pitch + i] * x[j];
+        }
+    for (int64_t e = 0; e < ncoo; ++e) y[coo_row[e]] = y[coo_row[e]] + coo_val[e] * x[coo_col[e]];
+}
+
+/* Append-only HYB update, the reference's UpdateMatrix(hyb&, rows, cols,
+ * vals, N) (include/load.h:327-373 -> UpdateMatrix_hyb,
+ * include/sparse_update.inl:318-367): the batch is taken in row order
+ * (stable: the reference requires it row-sorted, indices_to_offsets); each
+ * entry of row r fills ELL slot row_sizes[r] while that is below the width,
+ * else it is appended to the COO part; row_sizes[r] counts every entry of
+ * the row.  The COO part is then stably sorted by row (thrust::sort_by_key),
+ * which leaves a row's earlier overflow entries before the new ones in batch
+ * order.  No merging, no zero drop: a coordinate may occur several times and
+ * the represented matrix is the sum of its entries.  row_sizes has R + 1
+ * entries ([R] = COO count); the COO output (capacity >= ncoo + n) must not
+ * alias the input.  Returns the new COO count, or -1. */
+int64_t orc_hyb_update(int64_t R, int64_t width, int64_t pitch, idx_t *row_sizes, idx_t *ell_col, float *ell_val,
+                       int64_t ncoo, const idx_t *coo_row, const idx_t *coo_col, const float *coo_val,
+                       int64_t n, const idx_t *brow, const idx_t *bcol, const float *bval,
+                       idx_t *out_row, idx_t *out_col, float *out_val)
+{
+    idx_t *r = (idx_t *)malloc((n > 0 ? n : 1) * sizeof(idx_t));
+    idx_t *c = (idx_t *)malloc((n > 0 ? n : 1) * sizeof(idx_t));
+    float *v = (float *)malloc((n > 0 ? n : 1) * sizeof(float));
+    idx_t *nr = (idx_t *)malloc((n > 0 ? n : 1) * sizeof(idx_t));
+    idx_t *nc = (idx_t *)malloc((n > 0 ? n : 1) * sizeof(idx_t));
+    float *nv = (float *)malloc((n > 0 ? n : 1) * sizeof(float));
+    if (!r || !c || !v || !nr || !nc || !nv) { free(r); free(c); free(v); free(nr); free(nc); free(nv); return -1; }
+    if (n > 0) {
+        memcpy(r, brow, n * sizeof(idx_t));
+        memcpy(c, bcol, n * sizeof(idx_t));
+        memcpy(v, bval, n * sizeof(float));
+    }
+    orc_sort_by_row(n, r, c, v);
+    int64_t m = 0;  /* new overflow entries, in row order */
+    for (int64_t e = 0; e < n; ++e) {
+        const idx_t row = r[e];
+        const idx_t rl = row_sizes[row];
+        if (rl < width) {
+            ell_col[(int64_t)rl * pitch + row] = c[e];
+            ell_val[(int64_t)rl * pitch + row] = v[e];
+        } else {
+            nr[m] = row; nc[m] = c[e]; nv[m] = v[e]; ++m;
+        }
+        row_sizes[row] = rl + 1;
+    }
+    /* stable merge by row: old COO entries of a row before the new ones */
+    int64_t i = 0, j = 0, k = 0;
+    while (i < ncoo || j < m) {
+        const int take_old = j >= m || (i < ncoo && coo_row[i] <= nr[j]);
+        if (take_old) { out_row[k] = coo_row[i]; out_col[k] = coo_col[i]; out_val[k] = coo_val[i]; ++i; }
+        else { out_row[k] = nr[j]; out_col[k] = nc[j]; out_val[k] = nv[j]; ++j; }
+        ++k;
+    }
+    row_sizes[R] = (idx_t)k;
+    free(r); free(c); free(v); free(nr); free(nc); free(nv);
+    return k;
 }

@@ -135,4 +135,27 @@ int64_t ref_csr_to_hyb(int64_t R, int64_t C, const int *off, const int *col, con
     return nc;
 }
 
+// y = A x for a host HYB matrix given as arrays (ELL column-major with
+// pitch = rows rounded up to 32, padding column -1; COO sorted by row):
+// cusp::multiply on cusp::hyb_matrix<int, float, host_memory>.
+void ref_hyb_spmv(int64_t R, int64_t C, int64_t width, const int *ell_col, const float *ell_val, int64_t ncoo,
+                  const int *coo_row, const int *coo_col, const float *coo_val, const float *x, float *y)
+{
+    cusp::hyb_matrix<int, float, cusp::host_memory> H;
+    int64_t ne = 0;
+    const int64_t pitch = (R + 31) / 32 * 32;
+    for (int64_t t = 0; t < width * pitch; ++t) ne += ell_col[t] != -1;
+    H.resize(R, C, ne, ncoo, width, 32);
+    std::memcpy(&H.ell.column_indices.values[0], ell_col, width * pitch * sizeof(int));
+    std::memcpy(&H.ell.values.values[0], ell_val, width * pitch * sizeof(float));
+    if (ncoo) {
+        std::memcpy(&H.coo.row_indices[0], coo_row, ncoo * sizeof(int));
+        std::memcpy(&H.coo.column_indices[0], coo_col, ncoo * sizeof(int));
+        std::memcpy(&H.coo.values[0], coo_val, ncoo * sizeof(float));
+    }
+    HostF xv(x, x + C), yv(R, 0.0f);
+    cusp::multiply(H, xv, yv);
+    std::memcpy(y, &yv[0], R * sizeof(float));
+}
+
 }  // extern "C"

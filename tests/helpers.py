@@ -59,3 +59,30 @@ def random_coo(rng, R, C, n, dup_frac=0.0, zero_frac=0.0, values="sym"):
         z = rng.random(n) < zero_frac
         v[z] = 0.0
     return r, c, v
+
+
+def check_layout(mat):
+    """Allocator invariants of a DcsrMatrix (SURVEY-style pins of the pool):
+    every row's chunk [start, start + cap) lies below the bump pointer and
+    chunks are pairwise disjoint, len <= cap, bump <= pool capacity, the
+    live lengths sum to nnz, and bump = sum of capacities + garbage."""
+    st, ln, cp = (t.cpu().numpy().astype(np.int64) for t in mat.layout())
+    info = mat.info()
+    bump, pool_cap, garbage = info["pool_used"], info["pool_cap"], info["garbage"]
+    assert 0 <= bump <= pool_cap, (bump, pool_cap)
+    assert (ln >= 0).all() and (ln <= cp).all(), "a row is longer than its chunk"
+    assert int(ln.sum()) == info["nnz"], (int(ln.sum()), info["nnz"])
+    assert int(cp.sum()) + garbage == bump, (int(cp.sum()), garbage, bump)
+    live = cp > 0
+    s, c = st[live], cp[live]
+    order = np.argsort(s, kind="stable")
+    s, c = s[order], c[order]
+    assert (s + c <= bump).all(), "a chunk lies past the bump pointer"
+    assert (s[1:] >= s[:-1] + c[:-1]).all(), "two chunks overlap"
+    return info
+
+
+def check_healthy(mat):
+    """No sticky error and no scan ever took its look-back fallback."""
+    assert mat.status() == 0
+    assert mat.info()["scan_fallback"] == 0

@@ -123,3 +123,42 @@ def test_batch_autotune_and_errors(dev):
         want = A[m] if m == 3 else O.update(A[m], *batches[m])
         assert_same_structure(got[m], want)
     b.close()
+
+
+def test_exhausted_matrix_leaves_room_for_the_others(dev):
+    """A pool that may not grow: matrix 0's claim (its whole 2,000-row matrix
+    rewritten plus 4,000 inserts) cannot fit, the small matrices' claims can.
+    Matrix 0 fails alone with its exhausted bit and stays unchanged; every
+    other matrix commits (a claim that does not fit never advances the bump
+    pointer); the allocator's invariants hold afterwards."""
+    torch, dcsr_amd = dev
+    from tests.helpers import check_layout
+    rng = np.random.default_rng(77)
+    big = (2000, *W.powerlaw(2000, 32000, 2.1, 200, seed=78, signed=True)[:3])
+    small = [(100, *W.powerlaw(100, 400, 2.1, 20, seed=80 + i, signed=True)[:3]) for i in range(15)]
+    mats = [big] + small
+    sizes = np.array([m[0] for m in mats], np.int32)
+    nnz = sum(m[1].size for m in mats)
+    free = 20000
+    b = dcsr_amd.DcsrBatch(sizes, sizes, pool_factor=(nnz + free) / nnz, allow_grow=False, min_pool=16)
+    b.load_coo(*concat([(m[1], m[2], m[3]) for m in mats]))
+    A = [O.coo_to_csr(n, n, r, c, v) for n, r, c, v in mats]
+    # matrix 0: 4,000 new coordinates over all its rows; the others: 50 triples each
+    br0 = np.arange(4000) % 2000
+    bc0 = (np.arange(4000) * 7 + 1) % 2000
+    key = set(zip(mats[0][1].tolist(), mats[0][2].tolist()))
+    sel = np.array([(r, c) not in key for r, c in zip(br0, bc0)])
+    batches = [(br0[sel].astype(np.int32), bc0[sel].astype(np.int32), np.ones(int(sel.sum()), np.float32))]
+    batches += [W.batch_mixed(rng, r, c, n, n, 50, signed=True) for n, r, c, v in small]
+    say("batch step that exhausts the pool for matrix 0 only")
+    b.step(*concat(batches))
+    code, per = b.status()
+    assert code == 3  # DCSR_ERR_POOL_EXHAUSTED
+    assert per[0] != 0 and not per[1:].any()
+    off, col, val = (t.cpu().numpy() for t in b.to_csr())
+    got = split_csr(b, off, col, val)
+    assert_same_structure(got[0], A[0])
+    for i in range(1, len(mats)):
+        assert_same_structure(got[i], O.update(A[i], *batches[i]))
+    check_layout(b)
+    b.close()

@@ -366,3 +366,83 @@ def test_spmv_empty_rows_and_long_rows(torch):
     assert (np.abs(y.cpu().numpy() - want) <= 1e-5 * scale + 1e-30).all()
     np.testing.assert_array_equal(spmv_host(m, x), yv)  # reproducible
     m.close()
+
+
+def test_scan_fallback_path_gives_identical_results(torch):
+    """With the look-back wait limit at 0 every scan tile past the first takes
+    the exact fallback (its prefix summed from the earlier tiles' inputs):
+    load (row-start scan), SpMV (row-length scan + row map), export and
+    compaction give bit-identical results, and the handle reports that the
+    fallback ran.  With the default limit a fresh handle reports none."""
+    import dcsr_amd
+    from tests.helpers import check_layout
+    R = 600_000
+    r, c, v = W.powerlaw(R, 3_000_000, max_degree=3000, seed=61, signed=True)
+    rng = np.random.default_rng(62)
+    br, bc, bv = W.batch_mixed(rng, r, c, R, R, 500_000, signed=True)
+    x = rng.uniform(-1, 1, R).astype(np.float32)
+
+    def run():
+        m = dcsr_amd.DcsrMatrix(R, R)
+        m.LoadMatrix(r, c, v)
+        m.UpdateMatrix(br, bc, bv)
+        y = spmv_host(m, x)
+        m.SortMatrixRow()
+        y2 = spmv_host(m, x)
+        return m, to_host_csr(m), y, y2
+
+    say("default scan wait limit")
+    m0, C0, y0, y0b = run()
+    assert m0.info()["scan_fallback"] == 0
+    say("scan wait limit 0: every tile takes the fallback")
+    dcsr_amd.set_scan_wait_limit(0)
+    try:
+        m1, C1, y1, y1b = run()
+    finally:
+        dcsr_amd.set_scan_wait_limit(-1)
+    assert m1.info()["scan_fallback"] == 1
+    assert m1.status() == 0  # a diagnostic, not an error
+    assert_same_structure(C1, C0)
+    np.testing.assert_array_equal(y1, y0)
+    np.testing.assert_array_equal(y1b, y0b)
+    check_layout(m1)
+    m0.close()
+    m1.close()
+
+
+@pytest.mark.parametrize("seed", [0, 1])
+def test_allocator_invariants_through_updates_and_compaction(torch, seed):
+    """Chunks stay disjoint and below the bump pointer, and bump = sum of
+    capacities + garbage, across synchronous and asynchronous updates (with
+    deletions), in-stream compaction, growth, SpGEMM output and HYB load."""
+    import dcsr_amd
+    from tests.helpers import check_layout
+    rng = np.random.default_rng(70 + seed)
+    R, C = 20000, 15000
+    r, c, v = random_coo(rng, R, C, 200000, 0.0, 0.0, "sym")
+    m = dcsr_amd.DcsrMatrix(R, C, pool_factor=1.3)
+    m.LoadMatrix(r, c, v)
+    A = O.coo_to_csr(R, C, r, c, v)
+    check_layout(m)
+    for k in range(5):
+        br, bc, bv = random_coo(rng, R, C, 60000, 0.1, 0.0, "sym")
+        rows_of = np.repeat(np.arange(R), np.diff(A.off))
+        pick = rng.choice(A.nnz, 3000, replace=False)
+        br[:3000], bc[:3000], bv[:3000] = rows_of[pick], A.col[pick], -A.val[pick]
+        m.UpdateMatrix(br, bc, bv, flags=dcsr_amd.UPDATE_NOSYNC if k % 2 else 0)
+        A = O.update(A, br, bc, bv)
+        info = check_layout(m)
+        assert info["nnz"] == A.nnz
+    m.SortMatrixRow()
+    assert check_layout(m)["garbage"] == 0
+    assert_same_structure(to_host_csr(m), A)
+    sq = dcsr_amd.DcsrMatrix(R, R)
+    sq.LoadMatrix(r[r < R], np.minimum(c, R - 1)[r < R], v[r < R])
+    Cm = dcsr_amd.spgemm(sq, sq)
+    check_layout(Cm)
+    ec, ev, cr, cc, cv, pitch = m.to_hyb(8)
+    h2 = dcsr_amd.DcsrMatrix(R, C)
+    h2.load_hyb(8, pitch, ec, ev, cr, cc, cv)
+    check_layout(h2)
+    for x in (m, sq, Cm, h2):
+        x.close()

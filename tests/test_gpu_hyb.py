@@ -8,7 +8,7 @@ import pytest
 
 import oracle as O
 from dcsr_amd import workloads as W
-from tests.helpers import assert_same_structure, random_coo, to_host_csr
+from tests.helpers import assert_same_structure, fp64_spmv_check, random_coo, to_host_csr
 
 pytestmark = pytest.mark.gpu
 GOLD = os.path.join(os.path.dirname(__file__), "golden")
@@ -108,3 +108,92 @@ def test_hyb_powerlaw_roundtrip(dev):
     assert_same_structure(to_host_csr(m2), A)
     m.close()
     m2.close()
+
+
+def hyb_host(H):
+    ec, ev, rs, cr, cc, cv = (t.cpu().numpy() for t in H.export())
+    return O.Hyb(H.num_rows, H.num_cols, H.width, H.pitch, rs, ec, ev, cr, cc, cv)
+
+
+def assert_same_hyb(got, want):
+    """Bit-exact: ELL slots (columns and values), row sizes, COO in order."""
+    assert got.pitch == want.pitch and got.width == want.width
+    np.testing.assert_array_equal(got.row_sizes, want.row_sizes)
+    np.testing.assert_array_equal(got.ell_col, want.ell_col)
+    np.testing.assert_array_equal(got.ell_val.view(np.uint32), want.ell_val.view(np.uint32))
+    np.testing.assert_array_equal(got.coo_row, want.coo_row)
+    np.testing.assert_array_equal(got.coo_col, want.coo_col)
+    np.testing.assert_array_equal(got.coo_val.view(np.uint32), want.coo_val.view(np.uint32))
+
+
+@pytest.mark.parametrize("R,C,n,width,seed", [(1, 1, 3, 1, 0), (300, 200, 4000, 4, 1), (5000, 4000, 80000, 16, 2),
+                                              (20000, 20000, 300000, 8, 3)])
+def test_hyb_store_load_update_spmv_match_oracle(R, C, n, width, seed):
+    """The standalone HYB store: COO load, three append updates (duplicates,
+    explicit zeros, rows pushed past the ELL width), CSR load; every state
+    equals the oracle bit for bit and SpMV equals the host hyb multiply bit
+    for bit (same left-to-right order), within 1e-5 of FP64 too."""
+    import torch
+    import dcsr_amd
+    rng = np.random.default_rng(seed)
+    r, c, v = random_coo(rng, R, C, n, 0.1, 0.02, "sym")
+    say(f"hyb store R={R} n={n} width={width}")
+    H = dcsr_amd.DcsrHyb(R, C, width, coo_cap=16)
+    H.LoadMatrix(r, c, v)
+    want = O.coo_to_hyb(R, C, r, c, v, width)
+    assert_same_hyb(hyb_host(H), want)
+    x = rng.uniform(-1, 1, C).astype(np.float32)
+    for k in range(3):
+        br, bc, bv = random_coo(rng, R, C, max(1, n // 3), 0.2, 0.05, "sym")
+        if k == 1:  # many entries on a few rows: deep overflow
+            br[: br.size // 2] = rng.integers(0, min(R, 7), br.size // 2)
+        H.UpdateMatrix(br, bc, bv)
+        want = O.hyb_update(want, br, bc, bv)
+        assert_same_hyb(hyb_host(H), want)
+        y = H.spmv(torch.from_numpy(x).cuda())
+        torch.cuda.synchronize()
+        yw = O.hyb_spmv(want, x)
+        np.testing.assert_array_equal(y.cpu().numpy().view(np.uint32), yw.view(np.uint32))
+        hr, hc, hv = want.to_coo()
+        fp64_spmv_check(O.coo_to_csr(R, C, hr, hc, hv), x, y.cpu().numpy())
+    say("hyb store CSR load")
+    A = O.coo_to_csr(R, C, r, c, v)
+    H.load_csr(A.off, A.col, A.val)
+    assert_same_hyb(hyb_host(H), O.csr_to_hyb_full(A, width))
+    H.close()
+
+
+def test_hyb_store_dcsr_round_trip_and_bad_index():
+    """HYB store -> DCSR keeps every stored entry (duplicates summed on the
+    first update, as any loaded matrix); DCSR -> HYB store gives the
+    column-sorted conversion; an out-of-range index rejects the update and
+    leaves the store unchanged."""
+    import torch
+    import dcsr_amd
+    from dcsr_amd import DcsrError
+    rng = np.random.default_rng(7)
+    R, C, width = 3000, 2500, 6
+    r, c, v = random_coo(rng, R, C, 40000, 0.1, 0.0, "sym")
+    H = dcsr_amd.DcsrHyb(R, C, width)
+    H.LoadMatrix(r, c, v)
+    br, bc, bv = random_coo(rng, R, C, 10000, 0.3, 0.0, "sym")
+    H.UpdateMatrix(br, bc, bv)
+    want = O.hyb_update(O.coo_to_hyb(R, C, r, c, v, width), br, bc, bv)
+    say("hyb store -> dcsr")
+    m = dcsr_amd.DcsrMatrix(R, C)
+    H.to_dcsr(m)
+    hr, hc, hv = want.to_coo()
+    assert_same_structure(to_host_csr(m), O.coo_to_csr(R, C, hr, hc, hv))
+    say("dcsr -> hyb store")
+    m.UpdateMatrix(np.zeros(0, np.int32), np.zeros(0, np.int32), np.zeros(0, np.float32))  # canonicalize
+    H2 = dcsr_amd.DcsrHyb(R, C, width)
+    H2.from_dcsr(m)
+    assert_same_hyb(hyb_host(H2), O.csr_to_hyb_full(to_host_csr(m), width))
+    say("bad index update")
+    before = hyb_host(H)
+    with pytest.raises(DcsrError) as e:
+        H.UpdateMatrix(np.array([1, R], np.int32), np.array([0, 0], np.int32), np.ones(2, np.float32))
+    assert e.value.code == 2
+    assert_same_hyb(hyb_host(H), before)
+    for x in (H, H2, m):
+        x.close()

@@ -50,3 +50,14 @@ def test_header_compiles_as_c_and_cpp(tmp_path):
                    check=True)
     subprocess.run(["g++", "-std=c++17", "-Wall", "-Werror", "-x", "c++", "-I", inc, "-c", str(src), "-o",
                     str(tmp_path / "t2.o")], check=True)
+
+
+def test_reference_style_driver_compiles_against_the_shim(tmp_path):
+    """A caller written like the reference's drivers (every device:: entry
+    point of include/dcsr_matrix.hpp: DCSR, CSR and HYB overloads) compiles
+    and links against libdcsr.so."""
+    from dcsr_amd import build as B
+    out = str(tmp_path / "driver")
+    r = subprocess.run(B.driver_cmd(out), capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
+    assert os.path.getsize(out) > 0

@@ -163,3 +163,116 @@ def test_hyb_brute_force():
         for k in range(len(cols), w):
             assert ec[k * pitch + i] == -1
     assert [(int(a), int(b), float(c)) for a, b, c in zip(cr, cc, cv)] == [(a, int(b), float(c)) for a, b, c in coo]
+
+
+@pytest.mark.parametrize("seed", [0, 1])
+def test_canonical_row_sort_matches_stable_lexsort(seed):
+    """The C per-row sort (short rows by insertion, long ones by merge) gives
+    the same stable order as numpy's lexsort, duplicates in stored order."""
+    rng = np.random.default_rng(seed)
+    R, C = 300, 60
+    r, c, v = random_coo(rng, R, C, 6000, 0.4, 0.0, "sym")
+    r = np.concatenate([r, np.full(3000, 7, np.int32)])  # one long row with duplicates
+    c = np.concatenate([c, rng.integers(0, C, 3000).astype(np.int32)])
+    v = np.concatenate([v, rng.uniform(-1, 1, 3000).astype(np.float32)])
+    A = O.coo_to_csr(R, C, r, c, v)  # rows keep the (random) input order inside
+    got, want = A.canonical(), A.canonical_np()
+    np.testing.assert_array_equal(got.col, want.col)
+    np.testing.assert_array_equal(got.val.view(np.uint32), want.val.view(np.uint32))
+    assert (np.diff(got.col.astype(np.int64))[np.diff(np.repeat(np.arange(R), np.diff(A.off))) == 0] >= 0).all()
+
+
+def test_spgemm_fp64_checker_against_scipy_and_planted_errors():
+    """orc_spgemm_check_f64 accepts the FP32 oracle product (whose error is
+    far below 1e-5 of |A||B|), agrees with scipy's FP64 product, and counts
+    a planted relative error of 1e-4 and a stored entry without partials."""
+    rng = np.random.default_rng(5)
+    R = 400
+    r, c, v = random_coo(rng, R, R, 8000, 0.0, 0.0, "sym")
+    A = O.coo_to_csr(R, R, r, c, v)
+    Cm = O.spgemm(A, A).canonical()
+    assert O.spgemm_check_f64(A, A, Cm) == (0, -1)
+    S = A.to_scipy(np.float64)
+    P = (S @ S).tocsr()
+    P.sort_indices()
+    dense_want = P.toarray()
+    rows_of = np.repeat(np.arange(R), np.diff(Cm.off))
+    np.testing.assert_allclose(Cm.val, dense_want[rows_of, Cm.col], rtol=1e-4, atol=1e-6)
+    bad = O.Csr(R, R, Cm.off, Cm.col, Cm.val.copy())
+    k = Cm.nnz // 2
+    bad.val[k] = bad.val[k] + np.float32(abs(bad.val[k]) * 1e-3 + 1e-3)
+    assert O.spgemm_check_f64(A, A, bad) == (1, k)
+    # a stored coordinate with no partial product in its row
+    absent = np.setdiff1d(np.arange(R), Cm.col[Cm.off[3]:Cm.off[4]])[0]
+    col = Cm.col.copy()
+    col[Cm.off[3]] = absent
+    assert O.spgemm_check_f64(A, A, O.Csr(R, R, Cm.off, col, Cm.val))[0] >= 1
+
+
+def brute_hyb_update(H, rows, cols, vals):
+    """Per-row lists: the batch in stable row order, each entry into the next
+    free ELL slot of its row, else onto the row's overflow list; the COO part
+    is the rows' overflow lists in row order."""
+    R, w = H.rows, H.width
+    slots = {i: [] for i in range(R)}
+    for k in range(w):
+        for i in range(R):
+            if H.ell_col[k * H.pitch + i] != -1:
+                slots[i].append((int(H.ell_col[k * H.pitch + i]), np.float32(H.ell_val[k * H.pitch + i])))
+    ovf = {i: [] for i in range(R)}
+    for r_, c_, v_ in zip(H.coo_row, H.coo_col, H.coo_val):
+        ovf[int(r_)].append((int(c_), np.float32(v_)))
+    size = {i: int(H.row_sizes[i]) for i in range(R)}
+    order = sorted(range(len(rows)), key=lambda e: (int(rows[e]), e))
+    for e in order:
+        i = int(rows[e])
+        if size[i] < w:
+            slots[i].append((int(cols[e]), np.float32(vals[e])))
+        else:
+            ovf[i].append((int(cols[e]), np.float32(vals[e])))
+        size[i] += 1
+    return slots, ovf, size
+
+
+@pytest.mark.parametrize("width,seed", [(1, 0), (3, 1), (6, 2)])
+def test_hyb_update_brute_force_and_operator_identity(width, seed):
+    """orc_hyb_update equals the per-row list model, and the represented
+    matrix (duplicates summed) of HYB(A) updated with B is A + B."""
+    rng = np.random.default_rng(seed)
+    R, C = 40, 30
+    r, c, v = random_coo(rng, R, C, 200, 0.2, 0.0, "sym")
+    H = O.coo_to_hyb(R, C, r, c, v, width)
+    br, bc, bv = random_coo(rng, R, C, 150, 0.3, 0.1, "sym")
+    H2 = O.hyb_update(H, br, bc, bv)
+    slots, ovf, size = brute_hyb_update(H, br, bc, bv)
+    for i in range(R):
+        got_slots = [(int(H2.ell_col[k * H2.pitch + i]), H2.ell_val[k * H2.pitch + i]) for k in range(width)
+                     if H2.ell_col[k * H2.pitch + i] != -1]
+        assert [(a, np.float32(b).view(np.uint32)) for a, b in got_slots] == \
+            [(a, np.float32(b).view(np.uint32)) for a, b in slots[i]]
+        assert H2.row_sizes[i] == size[i]
+    want_coo = [(i, cc, vv) for i in range(R) for cc, vv in ovf[i]]
+    assert H2.row_sizes[R] == len(want_coo) == H2.ncoo
+    assert [(int(a), int(b), np.float32(cv).view(np.uint32)) for a, b, cv in zip(H2.coo_row, H2.coo_col, H2.coo_val)] \
+        == [(a, b, np.float32(cv).view(np.uint32)) for a, b, cv in want_coo]
+    # operator identity in FP64: sum of stored entries per coordinate
+    import scipy.sparse as sp
+    hr, hc, hv = H2.to_coo()
+    got = sp.coo_matrix((hv.astype(np.float64), (hr, hc)), shape=(R, C)).toarray()
+    want = sp.coo_matrix((np.concatenate([v, bv]).astype(np.float64), (np.concatenate([r, br]),
+                          np.concatenate([c, bc]))), shape=(R, C)).toarray()
+    np.testing.assert_allclose(got, want, rtol=0, atol=1e-12)
+
+
+def test_hyb_spmv_against_fp64():
+    rng = np.random.default_rng(9)
+    R, C = 500, 400
+    r, c, v = random_coo(rng, R, C, 6000, 0.1, 0.0, "sym")
+    H = O.hyb_update(O.coo_to_hyb(R, C, r, c, v, 8), *random_coo(rng, R, C, 3000, 0.2, 0.0, "sym"))
+    x = rng.uniform(-1, 1, C).astype(np.float32)
+    hr, hc, hv = H.to_coo()
+    import scipy.sparse as sp
+    S = sp.coo_matrix((hv.astype(np.float64), (hr, hc)), shape=(R, C)).tocsr()
+    y64 = S @ x.astype(np.float64)
+    scale = abs(S) @ np.abs(x.astype(np.float64))
+    assert (np.abs(O.hyb_spmv(H, x) - y64) <= 1e-5 * scale + 1e-30).all()

@@ -70,3 +70,20 @@ def test_spmv_spgemm_hyb(R, C, n, dup, zero, vals):
         assert got[5] == ref[5]
         for g, w in zip(got[:5], ref[:5]):
             np.testing.assert_array_equal(np.asarray(g).view(np.uint32), np.asarray(w).view(np.uint32))
+
+
+@pytest.mark.parametrize("R,C,n,dup,zero,vals", CASES)
+def test_hyb_spmv_after_appends(R, C, n, dup, zero, vals):
+    """The oracle's HYB SpMV equals cusp::multiply on a host hyb_matrix bit
+    for bit, also after append updates that leave duplicates in both parts."""
+    rng = np.random.default_rng(5 * R + n)
+    r, c, v = random_coo(rng, R, C, n, dup, zero, vals)
+    for width in (1, 4, 9):
+        H = O.coo_to_hyb(R, C, r, c, v, width)
+        Hr = O.coo_to_hyb(R, C, r, c, v, width, use_ref=True)
+        for a, b in zip(H.to_coo(), Hr.to_coo()):
+            np.testing.assert_array_equal(np.asarray(a).view(np.uint32), np.asarray(b).view(np.uint32))
+        for _ in range(2):
+            H = O.hyb_update(H, *random_coo(rng, R, C, max(1, n // 3), dup, zero, vals))
+        x = rng.uniform(-1, 1, C).astype(np.float32)
+        np.testing.assert_array_equal(O.hyb_spmv(H, x).view(np.uint32), O.hyb_spmv(H, x, use_ref=True).view(np.uint32))
