@@ -146,6 +146,16 @@ def lib():
         "dcsr_hyb_to_dcsr": (ctypes.c_int, [_H, _H, _P]),
         "dcsr_hyb_from_dcsr": (ctypes.c_int, [_H, _H, _P]),
         "dcsr_set_scan_wait_limit": (None, [ctypes.c_int32]),
+        "dcsr_set_update_fast_path": (None, [ctypes.c_int32]),
+        "dcsr_csr_add": (ctypes.c_int, [_I64, _I64, _P, _P, _P, _P, _P, _P, _I64, _P, _P, _P, _I64,
+                                        ctypes.POINTER(_I64), _P]),
+        "dcsr_csr_spgemm": (ctypes.c_int, [_I64, _I64, _I64, _P, _P, _P, _P, _P, _P, _P, _P, _P, _I64,
+                                           ctypes.POINTER(_I64), _P]),
+        "dcsr_mtx_read": (ctypes.c_int, [ctypes.c_char_p, ctypes.POINTER(_H), ctypes.POINTER(_I64),
+                                         ctypes.POINTER(_I64), ctypes.POINTER(_I64)]),
+        "dcsr_mtx_copy": (ctypes.c_int, [_H, _P, _P, _P]),
+        "dcsr_mtx_free": (ctypes.c_int, [_H]),
+        "dcsr_mtx_error": (ctypes.c_char_p, []),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -649,6 +659,12 @@ def set_scan_wait_limit(polls):
     lib().dcsr_set_scan_wait_limit(int(polls))
 
 
+def set_update_fast_path(enable):
+    """Process-wide: 0 sends every dcsr_update batch down the multi-launch
+    pipeline (include/dcsr.h dcsr_set_update_fast_path)."""
+    lib().dcsr_set_update_fast_path(1 if enable else 0)
+
+
 def csr_spmv(row_offsets, cols, vals, x, y=None):
     """Plain CSR SpMV comparator (reference include/spmv.h:310-332)."""
     torch = _torch()
@@ -661,3 +677,58 @@ def csr_spmv(row_offsets, cols, vals, x, y=None):
         y = torch.empty(R, dtype=torch.float32, device="cuda")
     check(lib().dcsr_csr_spmv(R, _ptr(o), _ptr(c), _ptr(v), _ptr(x), _ptr(y), _stream()), "dcsr_csr_spmv")
     return y
+
+
+def read_mtx(path):
+    """MatrixMarket file -> (rows, cols, row, col, val): 0-based int32 / float32
+    numpy arrays sorted by (row, column), symmetric storage expanded, the
+    file's values kept (include/dcsr_mtx.hpp; host only, no GPU needed)."""
+    import numpy as np
+    L = lib()
+    h = _H()
+    R, C, N = _I64(), _I64(), _I64()
+    code = L.dcsr_mtx_read(os.fsencode(path), ctypes.byref(h), ctypes.byref(R), ctypes.byref(C), ctypes.byref(N))
+    if code != DCSR_OK:
+        raise DcsrError(code, f"dcsr_mtx_read({path}): {L.dcsr_mtx_error().decode()}")
+    try:
+        n = N.value
+        r = np.empty(n, np.int32)
+        c = np.empty(n, np.int32)
+        v = np.empty(n, np.float32)
+        check(L.dcsr_mtx_copy(h, r.ctypes.data, c.ctypes.data, v.ctypes.data), "dcsr_mtx_copy")
+    finally:
+        L.dcsr_mtx_free(h)
+    return R.value, C.value, r, c, v
+
+
+def csr_add(rows, cols, row_offsets, col_idx, vals, b_rows, b_cols, b_vals):
+    """CSR-rebuild comparator C = A + B (cusp's generic add; include/dcsr.h
+    dcsr_csr_add).  Returns device tensors (row_offsets, cols, vals)."""
+    torch = _torch()
+    o, c, v = _dev(row_offsets, torch.int32), _dev(col_idx, torch.int32), _dev(vals, torch.float32)
+    br, bc, bv = _dev(b_rows, torch.int32), _dev(b_cols, torch.int32), _dev(b_vals, torch.float32)
+    cap = c.numel() + br.numel()
+    co = torch.empty(rows + 1, dtype=torch.int32, device="cuda")
+    cc = torch.empty(max(cap, 1), dtype=torch.int32, device="cuda")
+    cv = torch.empty(max(cap, 1), dtype=torch.float32, device="cuda")
+    nnz = _I64()
+    check(lib().dcsr_csr_add(rows, cols, _ptr(o), _ptr(c), _ptr(v), _ptr(br), _ptr(bc), _ptr(bv), br.numel(),
+                             _ptr(co), _ptr(cc), _ptr(cv), cap, ctypes.byref(nnz), _stream()), "dcsr_csr_add")
+    return co, cc[:nnz.value], cv[:nnz.value]
+
+
+def csr_spgemm(a_rows, a_cols, b_cols, a_off, a_col, a_val, b_off, b_col, b_val):
+    """CSR global-ESC SpGEMM comparator C = A * B (include/dcsr.h
+    dcsr_csr_spgemm).  Returns device tensors (row_offsets, cols, vals)."""
+    torch = _torch()
+    ao, ac, av = _dev(a_off, torch.int32), _dev(a_col, torch.int32), _dev(a_val, torch.float32)
+    bo, bc, bv = _dev(b_off, torch.int32), _dev(b_col, torch.int32), _dev(b_val, torch.float32)
+    blen = (bo[1:] - bo[:-1]).to(torch.int64)
+    cap = int(blen[ac.to(torch.int64)].sum().item()) if ac.numel() else 0
+    co = torch.empty(a_rows + 1, dtype=torch.int32, device="cuda")
+    cc = torch.empty(max(cap, 1), dtype=torch.int32, device="cuda")
+    cv = torch.empty(max(cap, 1), dtype=torch.float32, device="cuda")
+    nnz = _I64()
+    check(lib().dcsr_csr_spgemm(a_rows, a_cols, b_cols, _ptr(ao), _ptr(ac), _ptr(av), _ptr(bo), _ptr(bc), _ptr(bv),
+                                _ptr(co), _ptr(cc), _ptr(cv), cap, ctypes.byref(nnz), _stream()), "dcsr_csr_spgemm")
+    return co, cc[:nnz.value], cv[:nnz.value]

@@ -16,7 +16,7 @@ CSRC = os.path.join(HERE, "csrc")
 OBJ = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libdcsr.so")
 
-HIP_SOURCES = ["prims.hip", "matrix.hip", "update.hip", "spmv.hip", "spgemm.hip", "hyb.hip", "hybstore.hip", "batch.hip",
+HIP_SOURCES = ["prims.hip", "gsort.hip", "mtx.hip", "csrcmp.hip", "matrix.hip", "update.hip", "spmv.hip", "spgemm.hip", "hyb.hip", "hybstore.hip", "batch.hip",
                "server.hip"]
 CXX_SOURCES: list[str] = []
 

@@ -86,6 +86,11 @@ struct dcsr_matrix_s {
     int32_t *spare_col;
     float *spare_val;
     int64_t spare_cap;
+    // spare buffers replaced by larger ones while work may still be queued
+    // on them: freed at the next point where the handle's stream is known
+    // idle (a state read-back) or at destroy, never with a sync of their own
+    void *retired[32];
+    int nretired;
     // device/host state
     DevState *ds;            // device
     DevState *hs;            // pinned host mirror
@@ -139,6 +144,9 @@ struct dcsr_matrix_s {
 dcsr_status dcsr_hip_fail(hipError_t e, const char *what, const char *file, int line);
 dcsr_status ws_get(dcsr_handle h, WsSlot slot, size_t bytes, void **out);
 dcsr_status pull_state(dcsr_handle h, hipStream_t s);   // copy DevState to host + sync
+// Grow the pool in stream order, without reading the device state back:
+// the live rows are compacted into a new pool of at least new_cap entries.
+dcsr_status grow_in_order(dcsr_handle h, int64_t new_cap, hipStream_t s);
 // After launches whose device-side results the host does not read back: at
 // most `added` entries were inserted and `claimed` pool slots claimed.
 void state_unknown(dcsr_handle h, int64_t added, int64_t claimed);
@@ -164,6 +172,12 @@ dcsr_status update_prepare(dcsr_handle h, const int32_t *rows, const int32_t *co
                            hipStream_t s, bool *go, int64_t *need);
 dcsr_status update_replan(dcsr_handle h, int64_t n, hipStream_t s, bool *go);
 dcsr_status update_commit(dcsr_handle h, int64_t n, hipStream_t s, bool nosync);
+// update.hip: an update with its scratch carved from a fixed device region
+// (no allocation, no pool growth or compaction; canonical matrices only),
+// and the bytes such an update of n triples needs.
+dcsr_status update_fixed_scratch(dcsr_handle h, const int32_t *rows, const int32_t *cols, const float *vals,
+                                 int64_t n, uint32_t flags, hipStream_t s, uint8_t *base, size_t cap);
+size_t update_scratch_bytes(dcsr_handle h, int64_t n);
 // Flat copy of R rows (src_start[r], length dst_off[r+1]-dst_off[r]) to dst_off.
 void launch_copy_rows(const uint32_t *dst_off, int64_t R, int64_t total, const uint32_t *src_start,
                       const int32_t *scol, const float *sval, int32_t *dcol, float *dval, hipStream_t s);

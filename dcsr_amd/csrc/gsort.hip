@@ -1,0 +1,488 @@
+// gsort.hip -- row-group sort of an update batch's (key, value) pairs.
+//
+// Keys are (row << cbits) | col; invalid ones are ~0.  The rows are split
+// into G groups of 2^gshift consecutive rows, and a key's digit is its group
+// (key >> lb with lb = gshift + cbits; ~0 goes to digit G, after every
+// group).  The caller's pack pass counts the digits per 2048-pair tile into
+// a blocked table; then
+//
+//   1. k_gs_scan: each digit's tile counts become exclusive offsets (one
+//      workgroup per 64 digits, one lane per digit, every load in flight
+//      before the first add);
+//   2. k_gs_scatter: one stable scatter places each group's pairs
+//      contiguously, in arrival order (digits ranked by wave ballots, staged
+//      in LDS so runs are written together);
+//   3. k_gs_sort: one workgroup per group sorts the group's pairs by their
+//      low lb key bits (all of its keys share the bits above), a stable LSD
+//      sort with 8-bit digits; groups of up to GS_CAP pairs sort in LDS,
+//      larger ones out of global memory with the same ranking.
+//
+// The result is exactly the stable sort of the batch by key, which the LSD
+// radix sort (prims.hip) gives in one count/scan/scatter round per 8 key
+// bits: here the pairs cross HBM in one scatter and one group pass.  Stable
+// ranks keep equal keys in arrival order, which the insert-or-add sums rely
+// on.
+#include "prims.h"
+#include "wave.h"
+
+namespace {
+
+constexpr int GT = 256;              // threads of the scan-free kernels
+constexpr int GIPT = 8;              // pairs per lane in a scatter tile
+constexpr int GTILE = GT * GIPT;     // 2048 == RADIX_TILE
+static_assert(GTILE == RADIX_TILE, "pack counts tiles of the scatter's size");
+constexpr int GS_SCAN_T = 1024;      // k_gs_scan: 16 waves per 64 digits
+constexpr int GS_SCAN_Q = 32;        // tiles per wave per round
+
+// Lanes holding the same digit of `bits` bits (every lane active).
+__device__ __forceinline__ uint64_t wave_match_bits(uint32_t d, int bits)
+{
+    uint64_t peers = ~0ull;
+    for (int b = 0; b < bits; ++b) {
+        const uint64_t m = __ballot((d >> b) & 1u);
+        peers &= ((d >> b) & 1u) ? m : ~m;
+    }
+    return peers;
+}
+
+__device__ __forceinline__ uint32_t group_digit(uint64_t k, int lb, int ndig)
+{
+    return k == ~0ull ? (uint32_t)(ndig - 1) : (uint32_t)(k >> lb);
+}
+
+// Exclusive offsets of each digit's tile counts, in place; dtot[d] receives
+// the digit's total.  Block b covers digits [64b, 64b + 64): lane = digit.
+// A round covers NW * GS_SCAN_Q consecutive tiles, GS_SCAN_Q per wave in
+// wave order; the wave's counts stay in registers between the sum and the
+// write of the prefixes.
+__global__ __launch_bounds__(GS_SCAN_T) void k_gs_scan(uint32_t *__restrict__ hist, int64_t nb,
+                                                       uint32_t *__restrict__ dtot)
+{
+    constexpr int NW = GS_SCAN_T / 64;
+    __shared__ uint32_t wsum[NW][64];
+    const int w = threadIdx.x >> 6, lane = lane_id();
+    uint32_t *col = hist + (int64_t)blockIdx.x * nb * 64 + lane;
+    uint32_t carry_in = 0;  // the digit's count in earlier rounds
+    for (int64_t r0 = 0; r0 < nb; r0 += (int64_t)NW * GS_SCAN_Q) {
+        const int64_t t0 = r0 + (int64_t)w * GS_SCAN_Q;
+        const int64_t t1 = t0 + GS_SCAN_Q < nb ? t0 + GS_SCAN_Q : nb;
+        uint32_t v[GS_SCAN_Q];
+        uint32_t s = 0;
+#pragma unroll
+        for (int q = 0; q < GS_SCAN_Q; ++q) {
+            const int64_t t = t0 + q;
+            v[q] = t < t1 ? col[t * 64] : 0u;
+        }
+#pragma unroll
+        for (int q = 0; q < GS_SCAN_Q; ++q) s += v[q];
+        wsum[w][lane] = s;
+        __syncthreads();
+        uint32_t before = 0, tot = 0;
+#pragma unroll
+        for (int i = 0; i < NW; ++i) {
+            const uint32_t x = wsum[i][lane];
+            before += i < w ? x : 0u;
+            tot += x;
+        }
+        __syncthreads();
+        uint32_t p = carry_in + before;
+#pragma unroll
+        for (int q = 0; q < GS_SCAN_Q; ++q) {
+            const int64_t t = t0 + q;
+            if (t < t1) col[t * 64] = p;
+            p += v[q];
+        }
+        carry_in += tot;
+    }
+    if (w == 0) dtot[(int64_t)blockIdx.x * 64 + lane] = carry_in;
+}
+
+// Stable scatter of one 2048-pair tile into the group order.  gbase_out
+// (block 0) receives the exclusive offsets of the ndig digits plus the total.
+__global__ __launch_bounds__(GT) void k_gs_scatter(const uint64_t *__restrict__ kin, const uint32_t *__restrict__ vin,
+                                                   uint64_t *__restrict__ kout, uint32_t *__restrict__ vout,
+                                                   int64_t n, int lb, int ndig, int dbits,
+                                                   const uint32_t *__restrict__ hist, int64_t nb,
+                                                   const uint32_t *__restrict__ dtot, uint32_t *__restrict__ gbase_out)
+{
+    __shared__ uint32_t wcnt[GT / 64][GS_MAX_DIGITS];
+    __shared__ uint32_t lofs[GS_MAX_DIGITS];
+    __shared__ uint32_t gofs[GS_MAX_DIGITS];
+    __shared__ uint64_t skey[GTILE];
+    __shared__ uint32_t sval[GTILE];
+    __shared__ uint32_t lds4[4];
+    constexpr int DPT = GS_MAX_DIGITS / GT;  // digits per thread in the offset phase
+    const int w = threadIdx.x >> 6, lane = lane_id();
+    const int64_t tile = blockIdx.x;
+    const int64_t base = tile * GTILE;
+    const int64_t wbase = base + (int64_t)w * (GIPT * 64);
+    // every load first: the tile's offsets and the digit totals of this
+    // thread's digits, then the pairs
+    uint32_t ho[DPT], dt[DPT];
+    const int d0 = (int)threadIdx.x * DPT;
+#pragma unroll
+    for (int q = 0; q < DPT; ++q) {
+        const int d = d0 + q;
+        const bool in = d < ndig;
+        ho[q] = in ? hist[gs_hist_index(d, tile, nb)] : 0u;
+        dt[q] = in ? dtot[d] : 0u;
+    }
+    uint64_t key[GIPT];
+    uint32_t val[GIPT];
+#pragma unroll
+    for (int j = 0; j < GIPT; ++j) {
+        const int64_t idx = wbase + j * 64 + lane;
+        key[j] = kin[idx < n ? idx : base];
+        val[j] = vin[idx < n ? idx : base];
+    }
+    for (int d = threadIdx.x; d < ndig; d += GT)
+#pragma unroll
+        for (int i = 0; i < GT / 64; ++i) wcnt[i][d] = 0;
+    __syncthreads();
+    // rank: items past n take the last digit and rank after every real one
+    const uint64_t lt = lanemask_lt();
+    uint32_t rnk[GIPT], dg[GIPT];
+#pragma unroll
+    for (int j = 0; j < GIPT; ++j) {
+        const bool real = wbase + j * 64 + lane < n;
+        const uint32_t d = real ? group_digit(key[j], lb, ndig) : (uint32_t)(ndig - 1);
+        const uint64_t peers = wave_match_bits(d, dbits);
+        const int leader = __ffsll((long long)peers) - 1;
+        uint32_t b = 0;
+        if (lane == leader) {
+            b = wcnt[w][d];
+            wcnt[w][d] = b + (uint32_t)__popcll(peers);
+        }
+        b = __shfl(b, leader);
+        rnk[j] = b + (uint32_t)__popcll(peers & lt);
+        dg[j] = d;
+    }
+    __syncthreads();
+    {
+        uint32_t cs[DPT], gs[DPT];
+        uint32_t csum = 0, gsum = 0;
+#pragma unroll
+        for (int q = 0; q < DPT; ++q) {
+            const int d = d0 + q;
+            uint32_t c = 0;
+            if (d < ndig) {
+#pragma unroll
+                for (int i = 0; i < GT / 64; ++i) {
+                    const uint32_t x = wcnt[i][d];
+                    wcnt[i][d] = c;
+                    c += x;
+                }
+            }
+            cs[q] = csum;
+            csum += c;
+            gs[q] = gsum;
+            gsum += dt[q];
+        }
+        uint32_t tot;
+        const uint32_t lp = block_excl_scan_256(csum, lds4, &tot);
+        const uint32_t gp = block_excl_scan_256(gsum, lds4, &tot);
+#pragma unroll
+        for (int q = 0; q < DPT; ++q) {
+            const int d = d0 + q;
+            if (d < ndig) {
+                lofs[d] = lp + cs[q];
+                gofs[d] = gp + gs[q] + ho[q];
+                if (tile == 0) gbase_out[d] = gp + gs[q];
+            }
+        }
+        if (tile == 0 && threadIdx.x == 0) gbase_out[ndig] = tot;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < GIPT; ++j) {
+        const uint32_t p = lofs[dg[j]] + wcnt[w][dg[j]] + rnk[j];
+        skey[p] = key[j];
+        sval[p] = val[j];
+    }
+    __syncthreads();
+    const int64_t valid = (n - base) < GTILE ? (n - base) : GTILE;
+    for (int p = threadIdx.x; p < valid; p += GT) {
+        const uint64_t k = skey[p];
+        const uint32_t d = group_digit(k, lb, ndig);
+        const int64_t o = (int64_t)gofs[d] + (p - (int64_t)lofs[d]);
+        kout[o] = k;
+        vout[o] = sval[p];
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Per-group sort.  Ranking of one chunk of up to 4 * 64 * J items in their
+// current order (wave w holds positions [w * 64J, (w + 1) * 64J), item j of
+// a lane at w * 64J + 64 j + lane): rnk[j] = stable rank of item j among the
+// chunk's items of its digit in earlier waves excluded; afterwards
+// wcnt[w][d] = items of digit d in waves before w, loff[d] = items of the
+// chunk with a smaller digit, cnt_out[d] (if not null) = items of digit d.
+// ---------------------------------------------------------------------------
+constexpr int SJ = GS_CAP / GT;  // items per lane in a full LDS group
+
+template <int J>
+__device__ __forceinline__ void rank_chunk(const uint32_t (&dg)[J], const bool (&real)[J], int jmax,
+                                           uint32_t (&rnk)[J], uint32_t (*wcnt)[256], uint32_t *loff,
+                                           uint32_t *lds4)
+{
+    const int w = threadIdx.x >> 6, lane = lane_id();
+    const uint64_t lt = lanemask_lt();
+    wcnt[w][lane] = 0;
+    wcnt[w][lane + 64] = 0;
+    wcnt[w][lane + 128] = 0;
+    wcnt[w][lane + 192] = 0;
+    __builtin_amdgcn_wave_barrier();
+#pragma unroll
+    for (int j = 0; j < J; ++j) {
+        rnk[j] = 0;
+        if (j >= jmax) continue;  // uniform over the workgroup
+        const uint64_t act = __ballot(real[j]);
+        if (!real[j]) continue;
+        uint64_t peers = act;
+#pragma unroll
+        for (int b = 0; b < 8; ++b) {
+            const uint64_t m = __ballot((dg[j] >> b) & 1u);
+            peers &= ((dg[j] >> b) & 1u) ? m : ~m;
+        }
+        const int leader = __ffsll((long long)peers) - 1;
+        uint32_t c = 0;
+        if (lane == leader) {
+            c = wcnt[w][dg[j]];
+            wcnt[w][dg[j]] = c + (uint32_t)__popcll(peers);
+        }
+        c = __shfl(c, leader);
+        rnk[j] = c + (uint32_t)__popcll(peers & lt);
+    }
+    __syncthreads();
+    {
+        const int d = threadIdx.x;  // one digit per thread
+        uint32_t s = 0;
+#pragma unroll
+        for (int i = 0; i < GT / 64; ++i) {
+            const uint32_t x = wcnt[i][d];
+            wcnt[i][d] = s;
+            s += x;
+        }
+        uint32_t tot;
+        loff[d] = block_excl_scan_256(s, lds4, &tot);
+    }
+    __syncthreads();
+}
+
+// LDS sort of cnt <= GS_CAP pairs of group g: keys (local bits) and the
+// pairs' indices ping-pong between two LDS arrays.
+template <typename K>
+struct GsLds {
+    K key[2][GS_CAP];
+    uint16_t idx[2][GS_CAP];
+    uint32_t wcnt[GT / 64][256];
+    uint32_t loff[256];
+    uint32_t lds4[4];
+};
+
+template <typename K>
+__device__ void group_sort_lds(GsLds<K> &S, const uint64_t *__restrict__ kin, const uint32_t *__restrict__ vin,
+                               uint64_t *__restrict__ kout, uint32_t *__restrict__ vout, int64_t s0, int cnt, int lb,
+                               uint64_t high)
+{
+    const int w = threadIdx.x >> 6, lane = lane_id();
+    const uint64_t lmask = lb >= 64 ? ~0ull : ((1ull << lb) - 1ull);
+    // every key load in flight before the first store (indices clamped into
+    // the group, extra items dropped)
+    {
+        uint64_t kl[SJ];
+#pragma unroll
+        for (int j = 0; j < SJ; ++j) {
+            const int i = j * GT + threadIdx.x;
+            kl[j] = kin[s0 + (i < cnt ? i : cnt - 1)];
+        }
+#pragma unroll
+        for (int j = 0; j < SJ; ++j) {
+            const int i = j * GT + threadIdx.x;
+            if (i < cnt) {
+                S.key[0][i] = (K)(kl[j] & lmask);
+                S.idx[0][i] = (uint16_t)i;
+            }
+        }
+    }
+    __syncthreads();
+    // positions per wave: a multiple of 64 covering cnt / 4
+    const int jmax = (cnt + GT - 1) / GT;
+    const int pw = jmax * 64;
+    int cur = 0;
+    for (int sh = 0; sh < lb; sh += 8) {
+        // only the ranks stay in registers; keys are read again from LDS for
+        // the scatter
+        uint32_t dg[SJ], rnk[SJ];
+        bool real[SJ];
+#pragma unroll
+        for (int j = 0; j < SJ; ++j) {
+            const int p = w * pw + j * 64 + lane;
+            real[j] = j < jmax && p < cnt;
+            dg[j] = real[j] ? (uint32_t)(S.key[cur][p] >> sh) & 255u : 0u;
+        }
+        rank_chunk<SJ>(dg, real, jmax, rnk, S.wcnt, S.loff, S.lds4);
+#pragma unroll
+        for (int j = 0; j < SJ; ++j) {
+            if (!real[j]) continue;
+            const int p = w * pw + j * 64 + lane;
+            const uint32_t q = S.loff[dg[j]] + S.wcnt[w][dg[j]] + rnk[j];
+            S.key[cur ^ 1][q] = S.key[cur][p];
+            S.idx[cur ^ 1][q] = S.idx[cur][p];
+        }
+        __syncthreads();
+        cur ^= 1;
+    }
+    // the value gather: every load in flight before the first store
+    uint32_t vv[SJ];
+#pragma unroll
+    for (int j = 0; j < SJ; ++j) {
+        const int i = j * GT + threadIdx.x;
+        vv[j] = vin[s0 + (i < cnt ? S.idx[cur][i] : 0)];
+    }
+#pragma unroll
+    for (int j = 0; j < SJ; ++j) {
+        const int i = j * GT + threadIdx.x;
+        if (i < cnt) {
+            kout[s0 + i] = high | (uint64_t)S.key[cur][i];
+            vout[s0 + i] = vv[j];
+        }
+    }
+}
+
+// A group larger than GS_CAP: stable LSD passes out of global memory by this
+// workgroup alone, chunk after chunk (each chunk ranked as above, each
+// digit's running offset carried in LDS).  Ping-pongs between the group's
+// ranges of (a_k, a_v) (input) and (b_k, b_v) and ends in (b_k, b_v).
+struct GsBig {
+    uint32_t wcnt[GT / 64][256];
+    uint32_t loff[256];
+    uint32_t run[256];
+    uint32_t lds4[4];
+};
+
+__device__ void group_sort_big(GsBig &S, uint64_t *__restrict__ a_k, uint32_t *__restrict__ a_v,
+                               uint64_t *__restrict__ b_k, uint32_t *__restrict__ b_v, int64_t s0, int64_t cnt,
+                               int lb)
+{
+    constexpr int BJ = 4;                 // items per lane per chunk
+    constexpr int CH = GT * BJ;           // 1024 pairs per chunk
+    const int w = threadIdx.x >> 6, lane = lane_id();
+    uint64_t *sk = a_k, *dk = b_k;
+    uint32_t *sv = a_v, *dv = b_v;
+    bool in_b = false;
+    for (int sh = 0; sh < lb; sh += 8) {
+        // digit counts of the whole group
+        S.run[threadIdx.x] = 0;
+        __syncthreads();
+        for (int64_t i = threadIdx.x; i < cnt; i += GT) atomicAdd(&S.run[(uint32_t)(sk[s0 + i] >> sh) & 255u], 1u);
+        __syncthreads();
+        {
+            uint32_t tot;
+            const uint32_t c = S.run[threadIdx.x];
+            __syncthreads();
+            S.run[threadIdx.x] = block_excl_scan_256(c, S.lds4, &tot);
+        }
+        __syncthreads();
+        for (int64_t c0 = 0; c0 < cnt; c0 += CH) {
+            const int64_t left = cnt - c0;
+            const int jmax = left >= CH ? BJ : (int)((left + GT - 1) / GT);
+            const int pw = jmax * 64;
+            // only digits and ranks stay in registers; the pairs are read
+            // again for the scatter (L2 hits)
+            uint32_t dg[BJ], rnk[BJ];
+            bool real[BJ];
+#pragma unroll
+            for (int j = 0; j < BJ; ++j) {
+                const int64_t p = (int64_t)w * pw + j * 64 + lane;
+                real[j] = j < jmax && p < left;
+                dg[j] = real[j] ? (uint32_t)(sk[s0 + c0 + p] >> sh) & 255u : 0u;
+            }
+            rank_chunk<BJ>(dg, real, jmax, rnk, S.wcnt, S.loff, S.lds4);
+#pragma unroll
+            for (int j = 0; j < BJ; ++j) {
+                if (!real[j]) continue;
+                const int64_t p = (int64_t)w * pw + j * 64 + lane;
+                const int64_t o = (int64_t)S.run[dg[j]] + S.wcnt[w][dg[j]] + rnk[j];
+                dk[s0 + o] = sk[s0 + c0 + p];
+                dv[s0 + o] = sv[s0 + c0 + p];
+            }
+            __syncthreads();
+            // the chunk's count of digit d: loff[d + 1] - loff[d]
+            {
+                const int d = threadIdx.x;
+                const uint32_t c = (d == 255 ? (uint32_t)(left < CH ? left : CH) : S.loff[d + 1]) - S.loff[d];
+                S.run[d] += c;
+            }
+            __syncthreads();
+        }
+        uint64_t *tk = sk; sk = dk; dk = tk;
+        uint32_t *tv = sv; sv = dv; dv = tv;
+        in_b = !in_b;
+    }
+    if (!in_b) {
+        for (int64_t i = threadIdx.x; i < cnt; i += GT) {
+            b_k[s0 + i] = a_k[s0 + i];
+            b_v[s0 + i] = a_v[s0 + i];
+        }
+    }
+}
+
+// One workgroup per digit (group); the last digit holds the invalid keys,
+// copied as they are.  In: (kin, vin) in group order; out: (kout, vout)
+// sorted.  gbase: ndig + 1 exclusive group offsets.
+template <typename K>
+__global__ __launch_bounds__(GT) void k_gs_sort(uint64_t *__restrict__ kin, uint32_t *__restrict__ vin,
+                                                uint64_t *__restrict__ kout, uint32_t *__restrict__ vout, int lb,
+                                                int ndig, const uint32_t *__restrict__ gbase)
+{
+    __shared__ union {
+        GsLds<K> lds;
+        GsBig big;
+    } S;
+    const int g = blockIdx.x;
+    const int64_t s0 = gbase[g];
+    const int64_t cnt = (int64_t)gbase[g + 1] - s0;
+    if (cnt <= 0) return;
+    if (g == ndig - 1) {
+        for (int64_t i = threadIdx.x; i < cnt; i += GT) {
+            kout[s0 + i] = kin[s0 + i];
+            vout[s0 + i] = vin[s0 + i];
+        }
+        return;
+    }
+    if (cnt <= GS_CAP) {
+        group_sort_lds<K>(S.lds, kin, vin, kout, vout, s0, (int)cnt, lb, (uint64_t)g << lb);
+    } else {
+        group_sort_big(S.big, kin, vin, kout, vout, s0, cnt, lb);
+    }
+}
+
+}  // namespace
+
+size_t gs_hist_words(int64_t n, int ndig)
+{
+    const int64_t nb = (n + GTILE - 1) / GTILE;
+    const int64_t blocks = (ndig + 63) / 64;
+    return (size_t)((nb < 1 ? 1 : nb) * 64 * blocks) + 2 * (GS_MAX_DIGITS + 64);
+}
+
+hipError_t group_sort_pairs(uint64_t *k0, uint32_t *v0, uint64_t *k1, uint32_t *v1, int64_t n, int lb, int ndig,
+                            uint32_t *hist, hipStream_t s)
+{
+    if (n <= 0) return hipSuccess;
+    if (ndig < 1 || ndig > GS_MAX_DIGITS || lb < 1 || lb > 62) return hipErrorInvalidValue;
+    const int64_t nb = (n + GTILE - 1) / GTILE;
+    const int blocks = (ndig + 63) / 64;
+    uint32_t *dtot = hist + nb * 64 * blocks;
+    uint32_t *gbase = dtot + GS_MAX_DIGITS + 64;
+    int dbits = 1;
+    while ((1 << dbits) < ndig) ++dbits;
+    k_gs_scan<<<blocks, GS_SCAN_T, 0, s>>>(hist, nb, dtot);
+    k_gs_scatter<<<(unsigned)nb, GT, 0, s>>>(k0, v0, k1, v1, n, lb, ndig, dbits, hist, nb, dtot, gbase);
+    if (lb <= 32)
+        k_gs_sort<uint32_t><<<ndig, GT, 0, s>>>(k1, v1, k0, v0, lb, ndig, gbase);
+    else
+        k_gs_sort<uint64_t><<<ndig, GT, 0, s>>>(k1, v1, k0, v0, lb, ndig, gbase);
+    return hipGetLastError();
+}

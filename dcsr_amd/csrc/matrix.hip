@@ -300,10 +300,13 @@ dcsr_status ws_get(dcsr_handle h, WsSlot slot, size_t bytes, void **out)
     return DCSR_OK;
 }
 
+static void release_retired(dcsr_handle h);
+
 dcsr_status pull_state(dcsr_handle h, hipStream_t s)
 {
     DCSR_HIP(hipMemcpyAsync(h->hs, h->ds, sizeof(DevState), hipMemcpyDeviceToHost, s));
     DCSR_HIP(hipStreamSynchronize(s));
+    if (h->nretired) release_retired(h);
     h->nnz_h = h->hs->nnz;
     h->bump_h = (int64_t)h->hs->bump;
     h->garbage_h = h->hs->garbage;
@@ -440,6 +443,8 @@ dcsr_status dcsr_create(dcsr_handle *out, int64_t rows, int64_t cols, const dcsr
     if (st == DCSR_OK && hipHostMalloc(&h->snap, sizeof(DevState), hipHostMallocDefault) != hipSuccess)
         st = DCSR_ERR_OOM;
     if (st == DCSR_OK && hipMemset(h->ds, 0, sizeof(DevState)) != hipSuccess) st = DCSR_ERR_HIP;
+    // the NOSYNC snapshot's event, made here so that no update call creates one
+    if (st == DCSR_OK && hipEventCreateWithFlags(&h->snap_ev, hipEventDisableTiming) != hipSuccess) st = DCSR_ERR_HIP;
     if (st == DCSR_OK) {
         // all rows empty
         k_init_rows<<<grid_for(rows, TB), TB>>>(rows, 0, h->row_start, h->row_len, h->row_cap, h->ds);
@@ -465,6 +470,7 @@ dcsr_status dcsr_destroy(dcsr_handle h)
     (void)hipFree(h->pool_val);
     (void)hipFree(h->spare_col);
     (void)hipFree(h->spare_val);
+    release_retired(h);
     (void)hipFree(h->ds);
     if (h->hs) (void)hipHostFree(h->hs);
     if (h->snap) (void)hipHostFree(h->snap);
@@ -696,6 +702,13 @@ dcsr_status dcsr_export_csr(dcsr_handle h, int32_t *row_offsets, int32_t *cols, 
 // new_cap when smaller) and swap pools.  Needs no read-back: the copy is
 // sized from the host's bound of the live entries and stops at the device's
 // count; the device's bump pointer becomes its live count.
+// Free the retired spare buffers (the caller knows the stream is idle).
+static void release_retired(dcsr_handle h)
+{
+    for (int i = 0; i < h->nretired; ++i) (void)hipFree(h->retired[i]);
+    h->nretired = 0;
+}
+
 static dcsr_status compact_into(dcsr_handle h, int64_t new_cap, hipStream_t s)
 {
     const int64_t R = h->rows;
@@ -707,9 +720,15 @@ static dcsr_status compact_into(dcsr_handle h, int64_t new_cap, hipStream_t s)
     DCSR_TRY(ws_get(h, WS_TMP0, (size_t)(R + 1) * 4, (void **)&off));
     DCSR_TRY(row_offsets_u32(h, off, s));
     if (h->spare_cap < new_cap) {
-        DCSR_HIP(hipStreamSynchronize(s));
-        if (h->spare_col) (void)hipFree(h->spare_col);
-        if (h->spare_val) (void)hipFree(h->spare_val);
+        // kernels queued earlier may still read the old spare (the pool of
+        // the compaction before): it is retired, not freed, so that growing
+        // never waits for the stream
+        if (h->nretired + 2 > (int)(sizeof(h->retired) / sizeof(h->retired[0]))) {
+            DCSR_HIP(hipStreamSynchronize(s));
+            release_retired(h);
+        }
+        if (h->spare_col) h->retired[h->nretired++] = h->spare_col;
+        if (h->spare_val) h->retired[h->nretired++] = h->spare_val;
         h->spare_col = nullptr;
         h->spare_val = nullptr;
         h->spare_cap = 0;
@@ -743,6 +762,12 @@ static dcsr_status compact_into(dcsr_handle h, int64_t new_cap, hipStream_t s)
 }
 
 dcsr_status compact_in_order(dcsr_handle h, hipStream_t s) { return compact_into(h, h->pool_cap, s); }
+
+dcsr_status grow_in_order(dcsr_handle h, int64_t new_cap, hipStream_t s)
+{
+    if (new_cap < h->pool_cap) new_cap = h->pool_cap;
+    return compact_into(h, new_cap, s);
+}
 
 dcsr_status dcsr_compact(dcsr_handle h, dcsr_stream_t stream)
 {

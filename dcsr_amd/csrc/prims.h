@@ -31,3 +31,27 @@ size_t radix_part_words(int64_t n);
 // into hist, per RADIX_TILE tile of (k0, v0).
 hipError_t radix_sort_pairs(uint64_t *k0, uint32_t *v0, uint64_t *k1, uint32_t *v1, int64_t n, int nbits,
                             uint32_t *hist, uint32_t *part, int *which, hipStream_t s, bool counted = false);
+
+// Row-group sort (gsort.hip) of pairs keyed (row << cbits) | col, ~0 for an
+// invalid key: digit d = key >> lb (lb = gshift + cbits: the row's group of
+// 2^gshift rows), ~0 -> digit ndig - 1.  The caller counts the digits of each
+// RADIX_TILE tile of (k0, v0) into hist (blocked: see gs_hist_index); the
+// stable result ends in (k0, v0), (k1, v1) is scratch.
+constexpr int GS_MAX_DIGITS = 2048;  // groups + 1
+constexpr int GS_CAP = 3072;         // pairs of a group sorted in LDS
+// Count of digit d in tile t: 64-digit blocks, each tile-major.
+__host__ __device__ inline int64_t gs_hist_index(int d, int64_t t, int64_t ntiles)
+{
+    return (int64_t)(d >> 6) * ntiles * 64 + t * 64 + (d & 63);
+}
+size_t gs_hist_words(int64_t n, int ndig);
+hipError_t group_sort_pairs(uint64_t *k0, uint32_t *v0, uint64_t *k1, uint32_t *v1, int64_t n, int lb, int ndig,
+                            uint32_t *hist, hipStream_t s);
+// Exclusive group offsets (ndig + 1 entries) that group_sort_pairs leaves in
+// its hist buffer: pairs of group g sit at [gbase[g], gbase[g + 1]).
+inline uint32_t *gs_group_base(uint32_t *hist, int64_t n, int ndig)
+{
+    const int64_t nb = (n + RADIX_TILE - 1) / RADIX_TILE;
+    const int64_t blocks = (ndig + 63) / 64;
+    return hist + nb * 64 * blocks + GS_MAX_DIGITS + 64;
+}

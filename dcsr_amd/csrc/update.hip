@@ -44,24 +44,29 @@ namespace {
 
 constexpr int TB = 256;
 enum { C_NOP = 0, C_ADD = 1, C_DEL = 2, C_INS = 3 };
+// process-wide: send every batch down the multi-launch pipeline (tests)
+bool g_no_fast = false;
 enum { M_VALUES = 0, M_REWRITE = 1 };
 constexpr int NPLAN = 5;  // per-block plan counters
 
 // Pack the caller's triples into sort keys (row << cbits | col) and value
 // bits, one RADIX_TILE tile per workgroup.  Invalid indices become ~0 keys:
 // with one key bit above row|col they sort strictly after every valid key,
-// and they raise the batch's flag.  With `hist`, the tile's counts of the
-// sort's first digit (key bits 0..7) go to the count table on the way, so
-// the sort skips its first count pass.
+// and they raise the batch's flag.  With `hist`, the tile's digit counts go
+// to the sort's count table on the way: for the radix sort the first digit
+// (key bits 0..7, digit-major table), for the row-group sort (lb > 0) the
+// row group (key >> lb, ~0 -> ndig - 1; blocked table of gs_hist_index).
 __global__ __launch_bounds__(TB) void k_upd_pack(const int32_t *__restrict__ rows, const int32_t *__restrict__ cols,
                                                  const float *__restrict__ vals, int64_t n, int64_t R, int64_t C,
                                                  int cbits, uint64_t *__restrict__ key, uint32_t *__restrict__ val,
-                                                 uint32_t *__restrict__ hist, int64_t ntiles, DevState *ds)
+                                                 uint32_t *__restrict__ hist, int64_t ntiles, DevState *ds, int lb,
+                                                 int ndig)
 {
-    __shared__ uint32_t cnt[256];
+    __shared__ uint32_t cnt[GS_MAX_DIGITS];
     static_assert(TB == 256, "one digit counter per thread");
+    const int ncnt = lb > 0 ? ((ndig + 63) & ~63) : 256;
     if (hist) {
-        cnt[threadIdx.x] = 0;
+        for (int d = threadIdx.x; d < ncnt; d += TB) cnt[d] = 0;
         __syncthreads();
     }
     const int64_t base = (int64_t)blockIdx.x * RADIX_TILE;
@@ -87,14 +92,21 @@ __global__ __launch_bounds__(TB) void k_upd_pack(const int32_t *__restrict__ row
             const uint64_t k = b ? ~0ull : (((uint64_t)(uint32_t)r << cbits) | (uint32_t)c);
             key[i] = k;
             val[i] = vv[j];
-            if (hist) atomicAdd(&cnt[(uint32_t)k & 255u], 1u);
+            if (hist) {
+                const uint32_t d = lb > 0 ? (b ? (uint32_t)(ndig - 1) : (uint32_t)(k >> lb)) : ((uint32_t)k & 255u);
+                atomicAdd(&cnt[d], 1u);
+            }
             bad |= b;
         }
     }
     if (__ballot(bad) && lane_id() == 0) atomicOr(&ds->ubad, 1u);
     if (hist) {
         __syncthreads();
-        hist[(int64_t)threadIdx.x * ntiles + blockIdx.x] = cnt[threadIdx.x];
+        if (lb > 0) {
+            for (int d = threadIdx.x; d < ncnt; d += TB) hist[gs_hist_index(d, blockIdx.x, ntiles)] = cnt[d];
+        } else {
+            hist[(int64_t)threadIdx.x * ntiles + blockIdx.x] = cnt[threadIdx.x];
+        }
     }
 }
 
@@ -946,8 +958,6 @@ __global__ __launch_bounds__(TB) void k_upd_write(DevState *ds, uint32_t report,
     }
 }
 
-__global__ void k_upd_validate_begin(DevState *ds) { ds->ubad = 0; }
-
 struct UpdBufs {
     uint64_t *k0, *k1;
     uint32_t *v0, *v1, *hist, *part;
@@ -957,43 +967,512 @@ struct UpdBufs {
     uint32_t *trow, *tubeg, *tins, *tdel, *tnewl, *tmode, *tmoved, *tpre;
     uint32_t *tt, *tp, *fin;
     long long *pplan;
+    // sort: row groups (lb = gshift + cbits > 0, ndig = groups + 1) or, for
+    // batches past GS_MAX_N triples, the LSD radix sort (lb = 0)
+    int lb, ndig;
 };
+
+// ---------------------------------------------------------------------------
+// One-pass apply of a group-sorted batch (the fast path).  When the host's
+// claim bound guarantees room in the pool's bump region (every touched row
+// rewritten whole, every triple inserted), no batch-wide decision is needed
+// before writing, and one workgroup per row group does the heads, probe,
+// plan, move and write of its rows alone: the group's pairs are contiguous
+// and sorted, its rows belong to no other workgroup, and its chunk claims
+// take one atomic on the bump pointer.  The results equal the multi-launch
+// pipeline's (same sums, same codes, same chunk sizes; only the chunks'
+// places in the bump region differ).  Records and the touched-row table live
+// in LDS when the group fits (AP_REC pairs, AP_ROW touched rows), else in the
+// group's slice of the workspace arrays (the same code through generic
+// pointers).
+// ---------------------------------------------------------------------------
+constexpr int AP_REC = 1536;
+constexpr int AP_ROW = 256;
+constexpr int AP_IPR = 4;  // probe searches per thread in lock step
+constexpr int AP_MV = 4;   // moved entries per thread per round
+
+struct ApplyLds {
+    int32_t rcol[AP_REC];
+    float rval[AP_REC];
+    uint32_t rt[AP_REC], rlo[AP_REC], rcode[AP_REC], rpre[2 * AP_REC + 2];
+    uint32_t tr[AP_ROW], tbeg[AP_ROW + 1], tL[AP_ROW], tst[AP_ROW], tcap[AP_ROW], tins[AP_ROW], tdel[AP_ROW];
+    uint32_t tpre[2 * AP_ROW + 2];
+    uint32_t lds4[4];
+    long long acc[4][8];
+    uint32_t s_u[2];
+    unsigned long long base;
+};
+
+// The group's record and row arrays: LDS or its workspace slice.
+struct ApplyView {
+    int32_t *rcol;
+    float *rval;
+    uint32_t *rt, *rlo, *rcode, *rpre;
+    uint32_t *tr, *tbeg, *tL, *tst, *tcap, *tins, *tdel, *tpre;
+};
+
+__device__ __forceinline__ long long block_sum_i64(long long v, long long (*acc)[8], int slot)
+{
+    const long long w = wave_sum_i64(v);
+    if (lane_id() == 0) acc[threadIdx.x >> 6][slot] = w;
+    __syncthreads();
+    const long long t = (acc[0][slot] + acc[1][slot]) + (acc[2][slot] + acc[3][slot]);
+    __syncthreads();
+    return t;
+}
+
+__global__ __launch_bounds__(TB) void k_upd_apply(DevState *ds, const uint64_t *__restrict__ key,
+                                                  const uint32_t *__restrict__ val, const uint32_t *__restrict__ gbase,
+                                                  int ngroups, int cbits, UpdBufs b, uint32_t *__restrict__ row_start,
+                                                  int32_t *__restrict__ row_len, int32_t *__restrict__ row_cap,
+                                                  int32_t *__restrict__ pcol, float *__restrict__ pval)
+{
+    __shared__ ApplyLds S;
+    if (!ds->go) return;
+    const int g = blockIdx.x;
+    if (g >= ngroups) return;
+    const int64_t s0 = gbase[g];
+    const int64_t ng = (int64_t)gbase[g + 1] - s0;
+    if (ng <= 0) return;
+    // touched rows <= min(pairs, rows of the group)
+    ApplyView V;
+    const int64_t go = s0 + g;  // slice start of the arrays with one extra entry per group
+    if (ng <= AP_REC) {
+        V.rcol = S.rcol; V.rval = S.rval; V.rt = S.rt; V.rlo = S.rlo; V.rcode = S.rcode; V.rpre = S.rpre;
+    } else {
+        V.rcol = b.ucol + s0; V.rval = b.uval + s0; V.rt = b.ut + s0; V.rlo = b.upos + s0; V.rcode = b.ucode + s0;
+        V.rpre = b.upre + 2 * go;
+    }
+    if (ng <= AP_ROW) {
+        V.tr = S.tr; V.tbeg = S.tbeg; V.tL = S.tL; V.tst = S.tst; V.tcap = S.tcap; V.tins = S.tins;
+        V.tdel = S.tdel; V.tpre = S.tpre;
+    } else {
+        V.tr = b.trow + s0; V.tbeg = b.tubeg + go; V.tL = b.tmoved + s0; V.tst = b.urow + s0;
+        V.tcap = b.tmode + s0; V.tins = b.tins + s0; V.tdel = b.tdel + s0; V.tpre = b.tpre + 2 * go;
+    }
+    const uint64_t cmask = (1ull << cbits) - 1ull;
+
+    // heads: records (distinct coordinates, summed left to right from 0 in
+    // batch order) and touched rows, 2048 pairs per round
+    uint32_t U = 0, T = 0;
+    for (int64_t c0 = 0; c0 < ng; c0 += TILE) {
+        const int64_t i0 = s0 + c0 + (int64_t)threadIdx.x * IPT;
+        const int64_t iend = s0 + ng;
+        uint64_t k[IPT];
+        uint32_t vb[IPT];
+        const uint64_t prev = i0 > s0 && i0 - 1 < iend ? key[i0 - 1] : ~0ull;
+#pragma unroll
+        for (int j = 0; j < IPT; ++j) {
+            const int64_t i = i0 + j;
+            k[j] = i < iend ? key[i] : ~0ull;
+            vb[j] = i < iend ? val[i] : 0u;
+        }
+        uint32_t hm = 0, rm = 0;
+        {
+            uint64_t p = prev;
+#pragma unroll
+            for (int j = 0; j < IPT; ++j) {
+                if (k[j] != ~0ull) {
+                    if (k[j] != p) hm |= 1u << j;
+                    if ((k[j] >> cbits) != (p >> cbits)) rm |= 1u << j;
+                }
+                p = k[j];
+            }
+        }
+        uint32_t htot, rtot;
+        uint32_t lu = U + block_excl_scan_256(__popc(hm), S.lds4, &htot);
+        uint32_t lt = T + block_excl_scan_256(__popc(rm), S.lds4, &rtot);
+#pragma unroll
+        for (int j = 0; j < IPT; ++j) {
+            if (!((hm >> j) & 1u)) continue;
+            float sum = 0.0f + __uint_as_float(vb[j]);
+            bool open = true;
+#pragma unroll
+            for (int m = j + 1; m < IPT; ++m) {
+                if (open && k[m] == k[j]) sum = sum + __uint_as_float(vb[m]);
+                else open = false;
+            }
+            if (open)
+                for (int64_t m = i0 + IPT; m < iend && key[m] == k[j]; ++m) sum = sum + __uint_as_float(val[m]);
+            const bool rh = (rm >> j) & 1u;
+            if (rh) {
+                V.tr[lt] = (uint32_t)(k[j] >> cbits);
+                V.tbeg[lt] = lu;
+                ++lt;
+            }
+            V.rcol[lu] = (int32_t)(k[j] & cmask);
+            V.rval[lu] = sum;
+            V.rt[lu] = lt - 1u;
+            ++lu;
+        }
+        U += htot;
+        T += rtot;
+    }
+    if (threadIdx.x == 0) V.tbeg[T] = U;
+    __syncthreads();
+    // the rows' current chunks
+    for (uint32_t t = threadIdx.x; t < T; t += TB) {
+        const uint32_t r = V.tr[t];
+        V.tL[t] = (uint32_t)row_len[r];
+        V.tst[t] = row_start[r];
+        V.tcap[t] = (uint32_t)row_cap[r];
+        V.tins[t] = 0;
+        V.tdel[t] = 0;
+    }
+    __syncthreads();
+
+    // probe: lower bound of every record's column in its row, AP_IPR
+    // searches per thread in lock step
+    long long n_ins = 0, n_add = 0, n_del = 0;
+    for (uint32_t u0 = 0; u0 < U; u0 += TB * AP_IPR) {
+        int32_t lo[AP_IPR], hi[AP_IPR], cc[AP_IPR];
+        uint32_t st[AP_IPR], tt[AP_IPR];
+#pragma unroll
+        for (int q = 0; q < AP_IPR; ++q) {
+            const uint32_t u = u0 + q * TB + threadIdx.x;
+            lo[q] = hi[q] = cc[q] = 0;
+            st[q] = tt[q] = 0;
+            if (u < U) {
+                tt[q] = V.rt[u];
+                cc[q] = V.rcol[u];
+                st[q] = V.tst[tt[q]];
+                hi[q] = (int32_t)V.tL[tt[q]];
+            }
+        }
+        for (;;) {
+            bool any = false;
+            int32_t pc[AP_IPR];
+#pragma unroll
+            for (int q = 0; q < AP_IPR; ++q) {
+                pc[q] = 0;
+                if (lo[q] < hi[q]) {
+                    pc[q] = pcol[(uint64_t)st[q] + (uint32_t)((lo[q] + hi[q]) >> 1)];
+                    any = true;
+                }
+            }
+            if (!any) break;
+#pragma unroll
+            for (int q = 0; q < AP_IPR; ++q) {
+                if (lo[q] < hi[q]) {
+                    const int32_t mid = (lo[q] + hi[q]) >> 1;
+                    if (pc[q] < cc[q]) lo[q] = mid + 1; else hi[q] = mid;
+                }
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < AP_IPR; ++q) {
+            const uint32_t u = u0 + q * TB + threadIdx.x;
+            if (u >= U) continue;
+            const uint32_t L = V.tL[tt[q]];
+            const float s = V.rval[u];
+            uint32_t code;
+            float nv;
+            if ((uint32_t)lo[q] < L && pcol[(uint64_t)st[q] + lo[q]] == cc[q]) {
+                nv = pval[(uint64_t)st[q] + lo[q]] + s;
+                code = nv != 0.0f ? C_ADD : C_DEL;
+            } else {
+                nv = s;
+                code = s != 0.0f ? C_INS : C_NOP;
+            }
+            V.rval[u] = nv;
+            V.rlo[u] = (uint32_t)lo[q];
+            V.rcode[u] = code;
+            if (code == C_INS) atomicAdd(&V.tins[tt[q]], 1u);
+            if (code == C_DEL) atomicAdd(&V.tdel[tt[q]], 1u);
+            n_ins += code == C_INS;
+            n_add += code == C_ADD;
+            n_del += code == C_DEL;
+        }
+    }
+    __syncthreads();
+    // inserts / deletes before each record (rpre[2u], rpre[2u + 1]; u <= U)
+    {
+        uint32_t ci = 0, cd = 0;
+        for (uint32_t c0 = 0; c0 <= U; c0 += TILE) {
+            const uint32_t e0 = c0 + threadIdx.x * IPT;
+            uint32_t fi[IPT], fd[IPT], si = 0, sd = 0;
+#pragma unroll
+            for (int j = 0; j < IPT; ++j) {
+                const uint32_t code = e0 + j < U ? V.rcode[e0 + j] : (uint32_t)C_NOP;
+                fi[j] = code == C_INS;
+                fd[j] = code == C_DEL;
+                si += fi[j];
+                sd += fd[j];
+            }
+            uint32_t ti, td;
+            uint32_t pi = ci + block_excl_scan_256(si, S.lds4, &ti);
+            uint32_t pd = cd + block_excl_scan_256(sd, S.lds4, &td);
+#pragma unroll
+            for (int j = 0; j < IPT; ++j) {
+                if (e0 + j <= U) {
+                    V.rpre[2 * (e0 + j)] = pi;
+                    V.rpre[2 * (e0 + j) + 1] = pd;
+                }
+                pi += fi[j];
+                pd += fd[j];
+            }
+            ci += ti;
+            cd += td;
+        }
+    }
+    // plan: new lengths, chunk claims and old entries to move per row
+    // (tpre[2t], tpre[2t + 1]; t <= T)
+    uint32_t claim_tot = 0, moved_tot = 0;
+    long long d_nnz = 0, d_garb = 0;
+    uint32_t hub = 0;
+    for (uint32_t c0 = 0; c0 <= T; c0 += TILE) {
+        const uint32_t t0 = c0 + threadIdx.x * IPT;
+        uint32_t cl[IPT], mv[IPT], sc = 0, sm = 0;
+#pragma unroll
+        for (int j = 0; j < IPT; ++j) {
+            cl[j] = mv[j] = 0;
+            const uint32_t t = t0 + j;
+            if (t < T) {
+                const uint32_t L = V.tL[t], ni = V.tins[t], nd = V.tdel[t];
+                if (ni | nd) {
+                    const uint32_t nl = L + ni - nd;
+                    cl[j] = chunk_round(nl);
+                    mv[j] = L;
+                    d_nnz += (long long)nl - (long long)L;
+                    d_garb += V.tcap[t];
+                    hub |= (nl > DCSR_HUB_LEN) | (L > DCSR_HUB_LEN);
+                }
+            }
+            sc += cl[j];
+            sm += mv[j];
+        }
+        uint32_t tc, tm;
+        uint32_t pc = claim_tot + block_excl_scan_256(sc, S.lds4, &tc);
+        uint32_t pm = moved_tot + block_excl_scan_256(sm, S.lds4, &tm);
+#pragma unroll
+        for (int j = 0; j < IPT; ++j) {
+            if (t0 + j <= T) {
+                V.tpre[2 * (t0 + j)] = pc;
+                V.tpre[2 * (t0 + j) + 1] = pm;
+            }
+            pc += cl[j];
+            pm += mv[j];
+        }
+        claim_tot += tc;
+        moved_tot += tm;
+    }
+    // totals of the group; claims from the bump pointer (the host's bound
+    // guarantees they fit)
+    n_ins = block_sum_i64(n_ins, S.acc, 0);
+    n_add = block_sum_i64(n_add, S.acc, 1);
+    n_del = block_sum_i64(n_del, S.acc, 2);
+    d_nnz = block_sum_i64(d_nnz, S.acc, 3);
+    d_garb = block_sum_i64(d_garb, S.acc, 4);
+    const long long hubs = block_sum_i64(hub, S.acc, 5);
+    if (threadIdx.x == 0) {
+        S.base = claim_tot ? atomicAdd(&ds->bump, (unsigned long long)claim_tot) : 0ull;
+        if (n_ins) atomicAdd((unsigned long long *)&ds->n_ins, (unsigned long long)n_ins);
+        if (n_add) atomicAdd((unsigned long long *)&ds->n_add, (unsigned long long)n_add);
+        if (n_del) atomicAdd((unsigned long long *)&ds->n_del, (unsigned long long)n_del);
+        if (d_nnz) atomicAdd((unsigned long long *)&ds->nnz, (unsigned long long)d_nnz);
+        if (d_garb) atomicAdd((unsigned long long *)&ds->garbage, (unsigned long long)d_garb);
+        atomicAdd(&ds->ntouched, T);
+        atomicAdd(&ds->nuniq, U);
+        atomicAdd(&ds->claim_total, claim_tot);
+        atomicAdd(&ds->nrewrite_old, moved_tot);
+        if (hubs) atomicOr(&ds->hub_touch, 1u);
+    }
+    __syncthreads();
+    const unsigned long long base = S.base;
+
+    // move the old entries of rewritten rows to their merged places, AP_MV
+    // entries per thread with their loads in flight together
+    for (uint32_t e0 = 0; e0 < moved_tot; e0 += TB * AP_MV) {
+        uint32_t tq[AP_MV], kq[AP_MV];
+        int32_t cq[AP_MV];
+        float vq[AP_MV];
+#pragma unroll
+        for (int q = 0; q < AP_MV; ++q) {
+            const uint32_t e = e0 + q * TB + threadIdx.x;
+            tq[q] = kq[q] = 0;
+            cq[q] = 0;
+            vq[q] = 0.0f;
+            if (e < moved_tot) {
+                // last row t with tpre[2t + 1] <= e
+                uint32_t lo = 0, hi = T;
+                while (hi - lo > 1) {
+                    const uint32_t mid = (lo + hi) >> 1;
+                    if (V.tpre[2 * mid + 1] <= e) lo = mid; else hi = mid;
+                }
+                tq[q] = lo;
+                kq[q] = e - V.tpre[2 * lo + 1];
+                const uint64_t sp = (uint64_t)V.tst[lo] + kq[q];
+                cq[q] = pcol[sp];
+                vq[q] = pval[sp];
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < AP_MV; ++q) {
+            if (e0 + q * TB + threadIdx.x >= moved_tot) continue;
+            const uint32_t t = tq[q], k = kq[q];
+            const int32_t c = cq[q];
+            float v = vq[q];
+            const uint32_t ub = V.tbeg[t], ue = V.tbeg[t + 1];
+            uint32_t a = ub, z = ue;
+            while (a < z) {
+                const uint32_t mid = (a + z) >> 1;
+                if (V.rcol[mid] < c) a = mid + 1; else z = mid;
+            }
+            if (a < ue && V.rcol[a] == c) {
+                if (V.rcode[a] == C_DEL) continue;
+                v = V.rval[a];
+            }
+            const int64_t shift = (int64_t)(V.rpre[2 * a] - V.rpre[2 * ub]) -
+                                  (int64_t)(V.rpre[2 * a + 1] - V.rpre[2 * ub + 1]);
+            const uint64_t d = base + V.tpre[2 * t] + (uint64_t)((int64_t)k + shift);
+            pcol[d] = c;
+            pval[d] = v;
+        }
+    }
+    // inserted entries, and values in place for rows without inserts or
+    // deletes
+    for (uint32_t u = threadIdx.x; u < U; u += TB) {
+        const uint32_t code = V.rcode[u];
+        if (code != C_INS && code != C_ADD) continue;
+        const uint32_t t = V.rt[u];
+        const bool rewrite = (V.tins[t] | V.tdel[t]) != 0u;
+        if (!rewrite) {
+            pval[(uint64_t)V.tst[t] + V.rlo[u]] = V.rval[u];
+            continue;
+        }
+        if (code != C_INS) continue;  // moved with the old entries
+        const uint32_t ub = V.tbeg[t];
+        const int64_t shift = (int64_t)(V.rpre[2 * u] - V.rpre[2 * ub]) - (int64_t)(V.rpre[2 * u + 1] - V.rpre[2 * ub + 1]);
+        const uint64_t d = base + V.tpre[2 * t] + (uint64_t)((int64_t)V.rlo[u] + shift);
+        pcol[d] = V.rcol[u];
+        pval[d] = V.rval[u];
+    }
+    // row table
+    for (uint32_t t = threadIdx.x; t < T; t += TB) {
+        if (!(V.tins[t] | V.tdel[t])) continue;
+        const uint32_t r = V.tr[t];
+        const uint32_t nl = V.tL[t] + V.tins[t] - V.tdel[t];
+        row_start[r] = (uint32_t)(base + V.tpre[2 * t]);
+        row_len[r] = (int32_t)nl;
+        row_cap[r] = (int32_t)chunk_round(nl);
+    }
+}
+
+// Decision of a fast-path batch (one thread, before the apply): a bad index
+// rejects it (with `report`, as a sticky status bit); otherwise the apply's
+// counters start from zero.
+__global__ void k_upd_fast_begin(DevState *ds, uint32_t report)
+{
+    const unsigned bad = ds->ubad;
+    ds->ubad = 0;
+    ds->ubad_last = bad;
+    ds->go = bad ? 0u : 1u;
+    ds->bump_saved = ds->bump;
+    ds->n_ins = ds->n_add = ds->n_del = 0;
+    ds->d_nnz = ds->d_garbage = 0;
+    ds->d_hubs = ds->d_hub_chunks = 0;
+    ds->need = 0;
+    ds->ntouched = ds->nuniq = ds->claim_total = ds->nrewrite_old = 0;
+    ds->hub_touch = 0;
+    if (bad && report) ds->status |= ST_BAD_INDEX;
+}
+
+__global__ void k_upd_validate_begin(DevState *ds) { ds->ubad = 0; }
+
 
 // Tiles of the per-entry kernels (entries 0..n: index n carries totals).
 static inline unsigned ptiles(int64_t n) { return (unsigned)((n + 1 + TILE - 1) / TILE); }
 
+// Row-group sort plan of a batch of n triples: groups of 2^gshift rows, as
+// few rows per group as keeps the group count within GS_MAX_DIGITS - 1 and
+// the mean group near GS_TARGET pairs (groups are sorted in LDS up to
+// GS_CAP pairs; power-law batches drawn by degree peak near 3x the mean).
+constexpr int64_t GS_MAX_N = 1 << 21;
+constexpr int64_t GS_TARGET = 512;
+void sort_plan(dcsr_handle h, int64_t n, int &lb, int &ndig)
+{
+    lb = 0;
+    ndig = 0;
+    if (n > GS_MAX_N || n < 1) return;
+    int64_t want = n / GS_TARGET;
+    if (want < 1) want = 1;
+    if (want > GS_MAX_DIGITS - 1) want = GS_MAX_DIGITS - 1;
+    int gshift = 0;
+    while (((h->rows + (int64_t(1) << gshift) - 1) >> gshift) > want) ++gshift;
+    const int64_t groups = (h->rows + (int64_t(1) << gshift) - 1) >> gshift;
+    lb = gshift + h->cbits;
+    ndig = (int)groups + 1;
+}
+
+// Where an update's scratch comes from: the handle's grow-only workspace
+// (base == nullptr), or a fixed device region carved in order at 256-byte
+// alignment (the allocation-free entry point).  With `measure`, nothing is
+// handed out and `used` only counts the bytes the region must hold.
+struct BufSrc {
+    dcsr_handle h;
+    uint8_t *base;
+    size_t cap, used;
+    bool measure;
+};
+
+dcsr_status take(BufSrc &src, WsSlot slot, size_t bytes, void **out)
+{
+    if (!src.base && !src.measure) return ws_get(src.h, slot, bytes, out);
+    const size_t at = (src.used + 255) & ~size_t(255);
+    src.used = at + bytes;
+    if (src.measure) {
+        *out = nullptr;
+        return DCSR_OK;
+    }
+    if (src.used > src.cap) return DCSR_ERR_CAPACITY;
+    *out = src.base + at;
+    return DCSR_OK;
+}
+
+dcsr_status upd_bufs_src(BufSrc &src, int64_t n, UpdBufs &b)
+{
+    dcsr_handle h = src.h;
+    // one extra entry per row group in the arrays the fast path slices by
+    // group (a sentinel after each group's records and rows)
+    const size_t n4 = (size_t)(n + 2 + GS_MAX_DIGITS) * 4, n8 = (size_t)(n + 2 + GS_MAX_DIGITS) * 8;
+    const size_t nt = ptiles(n);
+    sort_plan(h, n, b.lb, b.ndig);
+    size_t hw = radix_hist_words(n);
+    if (b.lb > 0 && gs_hist_words(n, b.ndig) > hw) hw = gs_hist_words(n, b.ndig);
+    DCSR_TRY(take(src, WS_KEY_A, n8, (void **)&b.k0));
+    DCSR_TRY(take(src, WS_KEY_B, n8, (void **)&b.k1));
+    DCSR_TRY(take(src, WS_VAL_A, n4, (void **)&b.v0));
+    DCSR_TRY(take(src, WS_VAL_B, n4, (void **)&b.v1));
+    DCSR_TRY(take(src, WS_HIST, hw * 4, (void **)&b.hist));
+    DCSR_TRY(take(src, WS_SCAN, radix_part_words(n) * 4, (void **)&b.part));
+    DCSR_TRY(take(src, WS_UCOL, n4, (void **)&b.ucol));
+    DCSR_TRY(take(src, WS_UPOS, n4, (void **)&b.upos));
+    DCSR_TRY(take(src, WS_UVAL, n4, (void **)&b.uval));
+    DCSR_TRY(take(src, WS_UT, n4, (void **)&b.ut));
+    DCSR_TRY(take(src, WS_UROW, n4, (void **)&b.urow));
+    DCSR_TRY(take(src, WS_UCODE, n4, (void **)&b.ucode));
+    DCSR_TRY(take(src, WS_UPRE, n8, (void **)&b.upre));
+    DCSR_TRY(take(src, WS_TROW, n4, (void **)&b.trow));
+    DCSR_TRY(take(src, WS_TUBEG, n4 + 4, (void **)&b.tubeg));
+    DCSR_TRY(take(src, WS_TINS, n4, (void **)&b.tins));
+    DCSR_TRY(take(src, WS_TDEL, n4, (void **)&b.tdel));
+    DCSR_TRY(take(src, WS_TNEWL, n4, (void **)&b.tnewl));
+    DCSR_TRY(take(src, WS_TMODE, n4, (void **)&b.tmode));
+    DCSR_TRY(take(src, WS_TMOVED, n4, (void **)&b.tmoved));
+    DCSR_TRY(take(src, WS_TPAIRPRE, n8, (void **)&b.tpre));
+    DCSR_TRY(take(src, WS_TMP2, n4, (void **)&b.usum));
+    DCSR_TRY(take(src, WS_TDEST, nt * TT_WORDS * 4, (void **)&b.tt));
+    DCSR_TRY(take(src, WS_PPROBE, nt * TP_WORDS * 4, (void **)&b.tp));
+    DCSR_TRY(take(src, WS_PPLAN, nt * NPLAN * 8, (void **)&b.pplan));
+    // finish counters: zero-filled on allocation, reset by their last user
+    DCSR_TRY(take(src, WS_TOLDS, 64, (void **)&b.fin));
+    return DCSR_OK;
+}
+
 dcsr_status upd_bufs(dcsr_handle h, int64_t n, UpdBufs &b)
 {
-    const size_t n4 = (size_t)(n + 2) * 4, n8 = (size_t)(n + 2) * 8;
-    const size_t nt = ptiles(n);
-    DCSR_TRY(ws_get(h, WS_KEY_A, n8, (void **)&b.k0));
-    DCSR_TRY(ws_get(h, WS_KEY_B, n8, (void **)&b.k1));
-    DCSR_TRY(ws_get(h, WS_VAL_A, n4, (void **)&b.v0));
-    DCSR_TRY(ws_get(h, WS_VAL_B, n4, (void **)&b.v1));
-    DCSR_TRY(ws_get(h, WS_HIST, radix_hist_words(n) * 4, (void **)&b.hist));
-    DCSR_TRY(ws_get(h, WS_SCAN, radix_part_words(n) * 4, (void **)&b.part));
-    DCSR_TRY(ws_get(h, WS_UCOL, n4, (void **)&b.ucol));
-    DCSR_TRY(ws_get(h, WS_UPOS, n4, (void **)&b.upos));
-    DCSR_TRY(ws_get(h, WS_UVAL, n4, (void **)&b.uval));
-    DCSR_TRY(ws_get(h, WS_UT, n4, (void **)&b.ut));
-    DCSR_TRY(ws_get(h, WS_UROW, n4, (void **)&b.urow));
-    DCSR_TRY(ws_get(h, WS_UCODE, n4, (void **)&b.ucode));
-    DCSR_TRY(ws_get(h, WS_UPRE, n8, (void **)&b.upre));
-    DCSR_TRY(ws_get(h, WS_TROW, n4, (void **)&b.trow));
-    DCSR_TRY(ws_get(h, WS_TUBEG, n4 + 4, (void **)&b.tubeg));
-    DCSR_TRY(ws_get(h, WS_TINS, n4, (void **)&b.tins));
-    DCSR_TRY(ws_get(h, WS_TDEL, n4, (void **)&b.tdel));
-    DCSR_TRY(ws_get(h, WS_TNEWL, n4, (void **)&b.tnewl));
-    DCSR_TRY(ws_get(h, WS_TMODE, n4, (void **)&b.tmode));
-    DCSR_TRY(ws_get(h, WS_TMOVED, n4, (void **)&b.tmoved));
-    DCSR_TRY(ws_get(h, WS_TPAIRPRE, n8, (void **)&b.tpre));
-    DCSR_TRY(ws_get(h, WS_TMP2, n4, (void **)&b.usum));
-    DCSR_TRY(ws_get(h, WS_TDEST, nt * TT_WORDS * 4, (void **)&b.tt));
-    DCSR_TRY(ws_get(h, WS_PPROBE, nt * TP_WORDS * 4, (void **)&b.tp));
-    DCSR_TRY(ws_get(h, WS_PPLAN, nt * NPLAN * 8, (void **)&b.pplan));
-    // finish counters: zero-filled on allocation, reset by their last user
-    DCSR_TRY(ws_get(h, WS_TOLDS, 64, (void **)&b.fin));
-    return DCSR_OK;
+    BufSrc src{h, nullptr, 0, 0, false};
+    return upd_bufs_src(src, n, b);
 }
 
 // Tile tables scanned by a last workgroup (table) only for batches of more
@@ -1024,7 +1503,10 @@ dcsr_status sort_plan_launch(dcsr_handle h, int64_t n, UpdBufs &b, hipStream_t s
 {
     const int nbits = h->rbits + h->cbits + 1;
     int which = 0;
-    DCSR_HIP(radix_sort_pairs(b.k0, b.v0, b.k1, b.v1, n, nbits, b.hist, b.part, &which, s, true));
+    if (b.lb > 0)
+        DCSR_HIP(group_sort_pairs(b.k0, b.v0, b.k1, b.v1, n, b.lb, b.ndig, b.hist, s));
+    else
+        DCSR_HIP(radix_sort_pairs(b.k0, b.v0, b.k1, b.v1, n, nbits, b.hist, b.part, &which, s, true));
     const uint64_t *key = which ? b.k1 : b.k0;
     const uint32_t *val = which ? b.v1 : b.v0;
     const unsigned ht = (unsigned)((n + TILE - 1) / TILE);
@@ -1041,12 +1523,20 @@ dcsr_status sort_plan_launch(dcsr_handle h, int64_t n, UpdBufs &b, hipStream_t s
     return DCSR_OK;
 }
 
+// Pack (validate + key) the caller's triples into (k0, v0), counting the
+// first digit of the batch's sort on the way.
+void launch_pack(dcsr_handle h, const int32_t *rows, const int32_t *cols, const float *vals, int64_t n,
+                 const UpdBufs &b, hipStream_t s)
+{
+    k_upd_pack<<<pack_tiles(n), TB, 0, s>>>(rows, cols, vals, n, h->rows, h->cols, h->cbits, b.k0, b.v0, b.hist,
+                                            pack_tiles(n), h->ds, b.lb, b.ndig);
+}
+
 // Pack (validate + key) the caller's triples, then sort .. offsets.
 dcsr_status prepare_launch(dcsr_handle h, const int32_t *rows, const int32_t *cols, const float *vals, int64_t n,
                            UpdBufs &b, hipStream_t s)
 {
-    k_upd_pack<<<pack_tiles(n), TB, 0, s>>>(rows, cols, vals, n, h->rows, h->cols, h->cbits, b.k0, b.v0, b.hist,
-                                            pack_tiles(n), h->ds);
+    launch_pack(h, rows, cols, vals, n, b, s);
     DCSR_HIP(hipGetLastError());
     return sort_plan_launch(h, n, b, s);
 }
@@ -1055,10 +1545,9 @@ dcsr_status prepare_launch(dcsr_handle h, const int32_t *rows, const int32_t *co
 // move kernel strides over them with at most MV_BLOCKS workgroups (waves
 // past the device's count exit at once).
 constexpr int64_t MV_BLOCKS = 2048;
-dcsr_status commit_launch(dcsr_handle h, int64_t n, int64_t move_bound, bool report, hipStream_t s)
+dcsr_status commit_launch(dcsr_handle h, const UpdBufs &b, int64_t n, int64_t move_bound, bool report,
+                          hipStream_t s)
 {
-    UpdBufs b;
-    DCSR_TRY(upd_bufs(h, n, b));
     const uint32_t per_wave = 256;
     const int64_t waves = (move_bound + per_wave - 1) / per_wave;
     int64_t blocks = (waves + 3) / 4;
@@ -1079,7 +1568,22 @@ dcsr_status commit_launch(dcsr_handle h, int64_t n, int64_t move_bound, bool rep
 dcsr_status pipeline_launch(dcsr_handle h, int64_t n, UpdBufs &b, bool report, hipStream_t s)
 {
     DCSR_TRY(sort_plan_launch(h, n, b, s));
-    return commit_launch(h, n, h->pool_cap, report, s);
+    return commit_launch(h, b, n, h->pool_cap, report, s);
+}
+
+// The fast path of a packed batch: row-group sort, then one apply launch.
+// Only for batches sorted by row groups whose claims the host has bounded
+// below the free pool (claims_fit): nothing can be rejected but a bad index.
+dcsr_status fast_launch(dcsr_handle h, int64_t n, UpdBufs &b, bool report, hipStream_t s)
+{
+    DCSR_HIP(group_sort_pairs(b.k0, b.v0, b.k1, b.v1, n, b.lb, b.ndig, b.hist, s));
+    k_upd_fast_begin<<<1, 1, 0, s>>>(h->ds, report ? 1u : 0u);
+    const int groups = b.ndig - 1;
+    k_upd_apply<<<(unsigned)groups, TB, 0, s>>>(h->ds, b.k0, b.v0, gs_group_base(b.hist, n, b.ndig), groups, h->cbits,
+                                                b, h->row_start, h->row_len, h->row_cap, h->pool_col, h->pool_val);
+    DCSR_HIP(hipGetLastError());
+    h->flat_dirty = true;
+    return DCSR_OK;
 }
 
 }  // namespace
@@ -1101,7 +1605,7 @@ dcsr_status update_prepare(dcsr_handle h, const int32_t *rows, const int32_t *co
             DCSR_TRY(ws_get(h, WS_VAL_A, (size_t)n * 4, (void **)&v0));
             k_upd_validate_begin<<<1, 1, 0, s>>>(h->ds);
             k_upd_pack<<<pack_tiles(n), TB, 0, s>>>(rows, cols, vals, n, h->rows, h->cols, h->cbits, k0, v0, nullptr,
-                                                    pack_tiles(n), h->ds);
+                                                    pack_tiles(n), h->ds, 0, 0);
             DCSR_HIP(hipGetLastError());
             DCSR_TRY(pull_state(h, s));
             if (h->hs->ubad) return DCSR_ERR_BAD_INDEX;
@@ -1166,7 +1670,9 @@ bool claims_fit(dcsr_handle h, int64_t n) { return bump_bound(h) + claims_bound(
 // commit.  Launches only; the host mirrors are set from the prepare check.
 dcsr_status update_commit(dcsr_handle h, int64_t n, hipStream_t s, bool nosync)
 {
-    DCSR_TRY(commit_launch(h, n, nosync ? h->pool_cap : (int64_t)h->hs->nrewrite_old, nosync, s));
+    UpdBufs b;
+    DCSR_TRY(upd_bufs(h, n, b));
+    DCSR_TRY(commit_launch(h, b, n, nosync ? h->pool_cap : (int64_t)h->hs->nrewrite_old, nosync, s));
     if (nosync) {
         state_unknown(h, n, claims_bound(h, n));
         return DCSR_OK;
@@ -1197,10 +1703,12 @@ static dcsr_status update_sync(dcsr_handle h, const int32_t *rows, const int32_t
         // then holds the committed counters: write folds the deltas in)
         UpdBufs b;
         DCSR_TRY(upd_bufs(h, n, b));
-        k_upd_pack<<<pack_tiles(n), TB, 0, s>>>(rows, cols, vals, n, h->rows, h->cols, h->cbits, b.k0, b.v0, b.hist,
-                                                pack_tiles(n), h->ds);
+        launch_pack(h, rows, cols, vals, n, b, s);
         DCSR_HIP(hipGetLastError());
-        DCSR_TRY(pipeline_launch(h, n, b, false, s));
+        if (b.lb > 0 && !g_no_fast)
+            DCSR_TRY(fast_launch(h, n, b, false, s));
+        else
+            DCSR_TRY(pipeline_launch(h, n, b, false, s));
         DCSR_TRY(pull_state(h, s));
         if (h->hs->ubad_last) return DCSR_ERR_BAD_INDEX;
         if (h->hs->go) {
@@ -1231,24 +1739,85 @@ static dcsr_status update_nosync(dcsr_handle h, const int32_t *rows, const int32
     if (n == 0) return DCSR_OK;
     refresh_bounds(h);
     if (!claims_fit(h, n) && h->cfg.allow_grow && !(flags & DCSR_UPDATE_NOGROW)) {
+        // compaction (or growth, when even a compacted pool might not hold
+        // the claims) in stream order, from the host's bounds alone: no
+        // read-back, so the host never waits for the GPU here
         const int64_t need = claims_bound(h, n);
-        if (live_bound(h) + need <= h->pool_cap) {
+        const int64_t live = live_bound(h);
+        if (live + need <= h->pool_cap) {
             DCSR_TRY(compact_in_order(h, s));
         } else {
-            if (!h->state_known) DCSR_TRY(pull_state(h, s));
-            if (!claims_fit(h, n)) DCSR_TRY(pool_reserve(h, need + need / 8 + 64, s, nullptr));
+            const int64_t want = live + 2 * need;
+            DCSR_TRY(grow_in_order(h, want > 2 * h->pool_cap ? want : 2 * h->pool_cap, s));
         }
     }
     const int64_t claimed = claims_bound(h, n);
+    const bool fits = claims_fit(h, n);
     UpdBufs b;
     DCSR_TRY(upd_bufs(h, n, b));
-    k_upd_pack<<<pack_tiles(n), TB, 0, s>>>(rows, cols, vals, n, h->rows, h->cols, h->cbits, b.k0, b.v0, b.hist,
-                                            pack_tiles(n), h->ds);
+    launch_pack(h, rows, cols, vals, n, b, s);
     DCSR_HIP(hipGetLastError());
-    DCSR_TRY(pipeline_launch(h, n, b, true, s));
+    if (fits && b.lb > 0 && !g_no_fast)
+        DCSR_TRY(fast_launch(h, n, b, true, s));
+    else
+        DCSR_TRY(pipeline_launch(h, n, b, true, s));
     state_unknown(h, n, claimed);
     return snapshot_state(h, s);
 }
+
+namespace {
+__global__ void k_zero_u32(uint32_t *p, int n)
+{
+    for (int i = threadIdx.x; i < n; i += blockDim.x) p[i] = 0;
+}
+}  // namespace
+
+// Bytes of scratch an update of n triples on h carves from a fixed region.
+size_t update_scratch_bytes(dcsr_handle h, int64_t n)
+{
+    BufSrc src{h, nullptr, 0, 0, true};
+    UpdBufs b;
+    if (upd_bufs_src(src, n, b) != DCSR_OK) return ~size_t(0);
+    return src.used;
+}
+
+// An update whose scratch is carved from the fixed device region [base,
+// base + cap) instead of the handle's workspace: nothing is allocated, the
+// pool is never grown or compacted (a batch whose claims do not fit is
+// rejected whole: DCSR_ERR_POOL_EXHAUSTED, or with NOSYNC the sticky bit),
+// and the matrix must be canonical (canonicalizing allocates).  The region's
+// finish counters are cleared first (a fill kernel: safe under capture).
+dcsr_status update_fixed_scratch(dcsr_handle h, const int32_t *rows, const int32_t *cols, const float *vals,
+                                 int64_t n, uint32_t flags, hipStream_t s, uint8_t *base, size_t cap)
+{
+    if (!h->canonical) return DCSR_ERR_INVALID;
+    h->last_stream = s;
+    if (n == 0) return DCSR_OK;
+    BufSrc src{h, base, cap, 0, false};
+    UpdBufs b;
+    DCSR_TRY(upd_bufs_src(src, n, b));
+    k_zero_u32<<<1, 64, 0, s>>>(b.fin, 16);
+    const bool nosync = (flags & DCSR_UPDATE_NOSYNC) != 0;
+    if (nosync) refresh_bounds(h);
+    const bool fits = claims_fit(h, n);
+    launch_pack(h, rows, cols, vals, n, b, s);
+    DCSR_HIP(hipGetLastError());
+    if (fits && b.lb > 0 && !g_no_fast)
+        DCSR_TRY(fast_launch(h, n, b, nosync, s));
+    else
+        DCSR_TRY(pipeline_launch(h, n, b, nosync, s));
+    if (nosync) {
+        state_unknown(h, n, claims_bound(h, n));
+        return snapshot_state(h, s);
+    }
+    DCSR_TRY(pull_state(h, s));
+    if (h->hs->ubad_last) return DCSR_ERR_BAD_INDEX;
+    if (!h->hs->go) return DCSR_ERR_POOL_EXHAUSTED;
+    note_batch(h, *h->hs);
+    return DCSR_OK;
+}
+
+extern "C" void dcsr_set_update_fast_path(int32_t enable) { g_no_fast = enable == 0; }
 
 dcsr_status dcsr_update(dcsr_handle h, const int32_t *rows, const int32_t *cols, const float *vals, int64_t n,
                         uint32_t flags, dcsr_stream_t stream)

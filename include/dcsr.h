@@ -357,10 +357,68 @@ dcsr_status dcsr_export_layout(dcsr_handle h, uint32_t *row_start, int32_t *row_
  * waiting (used to test it).  No reference counterpart. */
 void dcsr_set_scan_wait_limit(int32_t polls);
 
+/* Process-wide switch of dcsr_update's one-pass path (default on): batches
+ * of up to 2^21 triples whose chunk claims the host can bound below the free
+ * pool are sorted by row groups and applied by one workgroup per group;
+ * 0 sends every batch down the multi-launch pipeline instead (results are
+ * identical; used to test both).  No reference counterpart. */
+void dcsr_set_update_fast_path(int32_t enable);
+
 /* Plain CSR SpMV comparator, y = A x.  Replaces device::spmv(const csr&, x, y)
  * -> spmv_csr_vector_kernel (include/spmv.h:310-332, include/spmv.inl:708-781). */
 dcsr_status dcsr_csr_spmv(int64_t rows, const int32_t *row_offsets, const int32_t *cols,
                           const float *vals, const float *x, float *y, dcsr_stream_t stream);
+
+/* CSR-rebuild comparators (the reference's CSR baselines of its DCSR-vs-CSR
+ * experiment).  Device CSR inputs (any order within rows; duplicates are
+ * summed in stored order); C comes out with columns ascending, unique;
+ * C goes to caller buffers of c_cap entries (c_off: rows + 1); *c_nnz (host)
+ * receives C's entry count, and when it exceeds c_cap nothing is written
+ * and DCSR_ERR_CAPACITY is returned.  Each call allocates its temporaries
+ * and syncs.
+ *
+ * C = A + B, B a batch of nb triples in any order (duplicates allowed), by
+ * a whole rebuild: cusp's generic add (A and B concatenated as COO, sorted
+ * by (row, column), reduced by key, zeros removed; cusp/system/detail/
+ * generic/elementwise.inl:177-225), as src/Fill_Test.inl:240-258 times it.
+ * Sums follow the host cusp::add order, so C equals dcsr_update's result
+ * bit for bit.  An out-of-range triple: DCSR_ERR_BAD_INDEX, nothing written. */
+dcsr_status dcsr_csr_add(int64_t rows, int64_t cols, const int32_t *a_off, const int32_t *a_col,
+                         const float *a_val, const int32_t *b_row, const int32_t *b_col, const float *b_val,
+                         int64_t nb, int32_t *c_off, int32_t *c_col, float *c_val, int64_t c_cap, int64_t *c_nnz,
+                         dcsr_stream_t stream);
+/* C = A * B (A: a_rows x a_cols, B: a_cols x b_cols) by global ESC: every
+ * partial product expanded, sorted by (row, column), reduced by key
+ * (cusp/system/detail/generic/multiply/spgemm.h:43-312, the comparator of
+ * src/SPMM_Test.inl:142).  Partials are summed from 0 in A-row then B-row
+ * order and exact zeros dropped (the host path's csr_spgemm order), so C
+ * equals dcsr_spgemm's result on the same column-sorted rows bit for bit.
+ * DCSR_ERR_UNSUPPORTED past 2^31 - 16 partial products. */
+dcsr_status dcsr_csr_spgemm(int64_t a_rows, int64_t a_cols, int64_t b_cols, const int32_t *a_off,
+                            const int32_t *a_col, const float *a_val, const int32_t *b_off, const int32_t *b_col,
+                            const float *b_val, int32_t *c_off, int32_t *c_col, float *c_val, int64_t c_cap,
+                            int64_t *c_nnz, dcsr_stream_t stream);
+
+/* ---------------------------------------------------------------------------
+ * MatrixMarket ingest (host; include/dcsr_mtx.hpp is the same reader for C++
+ * callers).  Coordinate files of type real / integer / pattern (pattern
+ * entries are 1), general / symmetric (off-diagonal entries mirrored) /
+ * skew-symmetric (mirrored with the value negated) storage; the triples come
+ * back 0-based, sorted by (row, column), with the file's values.  Replaces
+ * cusp::io::read_matrix_market_file (cusp/io/detail/matrix_market.inl:
+ * 117-269) and the drivers' ReadMatrixFile (include/load_matrix.h:54-116),
+ * which forced every value to 1.0 and kept symmetric storage unexpanded.
+ * ------------------------------------------------------------------------- */
+typedef struct dcsr_mtx_s *dcsr_mtx;
+/* Parse the file; *rows, *cols, *nnz (host, may be NULL) receive the shape
+ * and the expanded entry count.  DCSR_ERR_INVALID on a format error (the
+ * message: dcsr_mtx_error()). */
+dcsr_status dcsr_mtx_read(const char *path, dcsr_mtx *out, int64_t *rows, int64_t *cols, int64_t *nnz);
+/* Copy the triples into host arrays of nnz entries. */
+dcsr_status dcsr_mtx_copy(dcsr_mtx m, int32_t *row, int32_t *col, float *val);
+dcsr_status dcsr_mtx_free(dcsr_mtx m);
+/* Message of this thread's last failed dcsr_mtx_read ("" after a success). */
+const char *dcsr_mtx_error(void);
 
 #ifdef __cplusplus
 }
