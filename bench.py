@@ -67,6 +67,8 @@ def parse():
     ap.add_argument("--workload", choices=WORKLOADS, default="powerlaw")
     ap.add_argument("--batch", type=int, default=BATCH)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-hbm-spmv", action="store_true",
+                    help="skip the 10M-row / 160M-entry SpMV reported beside the power-law headline")
     ap.add_argument("--cpu-sample-batches", type=int, default=8)
     ap.add_argument("--cpu-min-seconds", type=float, default=10.0)
     ap.add_argument("--copy-calibration", action="store_true",
@@ -157,7 +159,7 @@ def cpu_baseline_powerlaw(r, c, v, batches, x, nsample, min_seconds):
             t_spmv += t2 - t1
             trip += br.size
         reps += 1
-    return {
+    out = {
         "value": trip / (t_upd + t_spmv),
         "unit": "triples/s",
         "cores": 1,
@@ -165,7 +167,46 @@ def cpu_baseline_powerlaw(r, c, v, batches, x, nsample, min_seconds):
         "sample": f"{reps} x {nsample} steps (1M-triple batch: sort_by_row + COO->CSR + cusp::add, then CSR SpMV) on the "
                   f"325,557-row power-law matrix, single thread; update {t_upd:.2f}s, SpMV {t_spmv:.2f}s",
         "update_triples_per_s": trip / t_upd,
+        "host_cores": host_cores(),
     }
+    # the same steps on every host core: the C restatement with its per-row
+    # work split over OpenMP threads (bit-identical results)
+    th = host_cores()
+    tm_upd = tm_spmv = 0.0
+    trip_m = 0
+    reps_m = 0
+    while reps_m == 0 or tm_upd + tm_spmv < min_seconds / 2:
+        A = O.coo_to_csr(R_CFG, R_CFG, r, c, v)
+        for k in range(nsample):
+            br, bc, bv = batches[k]
+            t0 = time.perf_counter()
+            A = O.update_mt(A, br, bc, bv, th)
+            t1 = time.perf_counter()
+            O.spmv_mt(A, x, th)
+            t2 = time.perf_counter()
+            tm_upd += t1 - t0
+            tm_spmv += t2 - t1
+            trip_m += br.size
+        reps_m += 1
+    out["multicore"] = {
+        "value": trip_m / (tm_upd + tm_spmv), "unit": "triples/s", "cores": th, "kind": "port",
+        "sample": f"{reps_m} x {nsample} steps, the same flow on {th} OpenMP threads (rows split per thread, same "
+                  f"results bit for bit); update {tm_upd:.2f}s, SpMV {tm_spmv:.2f}s",
+        "update_triples_per_s": trip_m / tm_upd,
+    }
+    return out
+
+
+def host_cores():
+    """Host cores this process may use: the OpenMP thread count the box
+    grants (OMP_NUM_THREADS), else the CPU affinity set."""
+    env = os.environ.get("OMP_NUM_THREADS", "")
+    if env.isdigit() and int(env) > 0:
+        return int(env)
+    try:
+        return len(os.sched_getaffinity(0))
+    except AttributeError:
+        return os.cpu_count() or 1
 
 
 def bench_powerlaw(args):
@@ -263,6 +304,12 @@ def bench_powerlaw(args):
     if not args.no_cpu:
         cpu = cpu_baseline_powerlaw(r, c, v, batches, x, min(args.cpu_sample_batches, len(batches)),
                                     args.cpu_min_seconds)
+    hbm = None
+    if not args.no_hbm_spmv:
+        m.close()
+        del d_b
+        torch.cuda.empty_cache()
+        hbm = spmv_hbm_leg(args)
     line(args, trip / elapsed, "triples/s", elapsed * 1e3 / K,
          {"workload": "powerlaw-update-spmv", "rows": R_CFG, "nnz0": NNZ_CFG, "zipf": 2.1, "max_degree": MAXDEG,
           "batch_triples": args.batch, "nnz_final": nnz_after[-1]},
@@ -279,7 +326,44 @@ def bench_powerlaw(args):
          sync_call_triples_per_s=trip / elapsed_sync,
          sync_call_ms_per_step=elapsed_sync * 1e3 / K,
          roofline=roof,
-         cpu_baseline=cpu)
+         cpu_baseline=cpu,
+         spmv_hbm=hbm)
+
+
+def spmv_hbm_leg(args, reps=10):
+    """SpMV at HBM scale beside the headline: the 10M x 10M power-law matrix
+    with 160M entries (A is 1.3 GB, x 40 MB: far past the 256 MB last-level
+    cache), median of `reps` SpMVs after two warm-up calls (the second builds
+    the column-paneled copy), HIP events on the launch stream."""
+    import torch
+    import dcsr_amd
+    from dcsr_amd import workloads as W
+    R, N = args.serving_rows, args.serving_nnz
+    r, c, v = W.serving_matrix(R, N)
+    dev = torch.device("cuda:0")
+    m = dcsr_amd.DcsrMatrix(R, R, pool_factor=1.0)
+    m.LoadMatrix(*(torch.from_numpy(a).to(dev) for a in (r, c, v)))
+    nnz = int(r.size)
+    del r, c, v
+    m.BinRows()
+    g = torch.Generator(device="cuda").manual_seed(9)
+    x = torch.rand(R, generator=g, device=dev)
+    y = torch.empty(R, device=dev)
+    for _ in range(2):
+        m.spmv(x, y)
+    stream = torch.cuda.current_stream()
+    ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(reps)]
+    for e0, e1 in ev:
+        e0.record(stream)
+        m.spmv(x, y)
+        e1.record(stream)
+    torch.cuda.synchronize()
+    ms = float(np.median([e0.elapsed_time(e1) for e0, e1 in ev]))
+    gbs = spmv_bytes(R, R, nnz) / 1e9 / (ms / 1e3)
+    m.close()
+    return {"rows": R, "nnz": nnz, "ms": ms, "gbs": gbs, "hbm_frac": gbs / HBM_PEAK_GBS,
+            "traffic_bytes": measured_traffic("spmv_powerlaw_10m_160m_traffic.txt", "traffic_bytes_per_spmv"),
+            "traffic_source": "profiles/spmv_powerlaw_10m_160m_traffic.txt"}
 
 
 # --------------------------------------------------------------------------
@@ -590,6 +674,7 @@ def bench_serving(args):
 def bench_spmv(args):
     import torch
     import dcsr_amd
+    from dcsr_amd import workloads as W
 
     R, N = args.serving_rows, args.serving_nnz
     r, c, v = W.serving_matrix(R, N)

@@ -1023,7 +1023,8 @@ __device__ __forceinline__ long long block_sum_i64(long long v, long long (*acc)
 
 __global__ __launch_bounds__(TB) void k_upd_apply(DevState *ds, const uint64_t *__restrict__ key,
                                                   const uint32_t *__restrict__ val, const uint32_t *__restrict__ gbase,
-                                                  int ngroups, int cbits, UpdBufs b, uint32_t *__restrict__ row_start,
+                                                  int ngroups, int cbits, int gshift, UpdBufs b,
+                                                  uint32_t *__restrict__ row_start,
                                                   int32_t *__restrict__ row_len, int32_t *__restrict__ row_cap,
                                                   int32_t *__restrict__ pcol, float *__restrict__ pval)
 {
@@ -1035,6 +1036,7 @@ __global__ __launch_bounds__(TB) void k_upd_apply(DevState *ds, const uint64_t *
     const int64_t ng = (int64_t)gbase[g + 1] - s0;
     if (ng <= 0) return;
     // touched rows <= min(pairs, rows of the group)
+    const int64_t tmax = ng < (int64_t(1) << gshift) ? ng : (int64_t(1) << gshift);
     ApplyView V;
     const int64_t go = s0 + g;  // slice start of the arrays with one extra entry per group
     if (ng <= AP_REC) {
@@ -1043,7 +1045,7 @@ __global__ __launch_bounds__(TB) void k_upd_apply(DevState *ds, const uint64_t *
         V.rcol = b.ucol + s0; V.rval = b.uval + s0; V.rt = b.ut + s0; V.rlo = b.upos + s0; V.rcode = b.ucode + s0;
         V.rpre = b.upre + 2 * go;
     }
-    if (ng <= AP_ROW) {
+    if (tmax <= AP_ROW) {
         V.tr = S.tr; V.tbeg = S.tbeg; V.tL = S.tL; V.tst = S.tst; V.tcap = S.tcap; V.tins = S.tins;
         V.tdel = S.tdel; V.tpre = S.tpre;
     } else {
@@ -1580,7 +1582,8 @@ dcsr_status fast_launch(dcsr_handle h, int64_t n, UpdBufs &b, bool report, hipSt
     k_upd_fast_begin<<<1, 1, 0, s>>>(h->ds, report ? 1u : 0u);
     const int groups = b.ndig - 1;
     k_upd_apply<<<(unsigned)groups, TB, 0, s>>>(h->ds, b.k0, b.v0, gs_group_base(b.hist, n, b.ndig), groups, h->cbits,
-                                                b, h->row_start, h->row_len, h->row_cap, h->pool_col, h->pool_val);
+                                                b.lb - h->cbits, b, h->row_start, h->row_len, h->row_cap, h->pool_col,
+                                                h->pool_val);
     DCSR_HIP(hipGetLastError());
     h->flat_dirty = true;
     return DCSR_OK;

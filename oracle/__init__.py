@@ -64,6 +64,10 @@ def lib():
         _orc.orc_coo_to_csr.argtypes = [_I64, _I64, _P, _P, _P, _P]
         _orc.orc_update.restype = _I64
         _orc.orc_update.argtypes = [_I64, _I64, _P, _P, _P, _I64, _P, _P, _P, _P, _P, _P]
+        _orc.orc_csr_spmv_mt.restype = None
+        _orc.orc_csr_spmv_mt.argtypes = [_I64, _P, _P, _P, _P, _P, ctypes.c_int]
+        _orc.orc_update_mt.restype = _I64
+        _orc.orc_update_mt.argtypes = [_I64, _I64, _P, _P, _P, _I64, _P, _P, _P, _P, _P, _P, ctypes.c_int]
         _orc.orc_sort_rows_by_col.restype = ctypes.c_int
         _orc.orc_sort_rows_by_col.argtypes = [_I64, _P, _P, _P]
         _orc.orc_hyb_spmv.restype = None
@@ -224,6 +228,21 @@ def update(A: Csr, rows, cols, vals, use_ref=False) -> Csr:
     return Csr(A.rows, A.cols, off, col[:nnz], val[:nnz])
 
 
+def update_mt(A: Csr, rows, cols, vals, threads) -> Csr:
+    """update() on `threads` OpenMP threads (orc_update_mt): the same result,
+    bit for bit; the multi-core CPU baseline."""
+    r, c, v = _i32(rows), _i32(cols), _f32(vals)
+    cap = max(A.nnz + r.size, 1)
+    off = np.zeros(A.rows + 1, np.int32)
+    col = np.zeros(cap, np.int32)
+    val = np.zeros(cap, np.float32)
+    nnz = lib().orc_update_mt(A.rows, A.cols, _p(A.off), _p(A.col), _p(A.val), r.size, _p(r), _p(c), _p(v),
+                              _p(off), _p(col), _p(val), int(threads))
+    if nnz < 0:
+        raise ValueError("update_mt failed (%d)" % nnz)
+    return Csr(A.rows, A.cols, off, col[:nnz], val[:nnz])
+
+
 def spmv(A: Csr, x, use_ref=False):
     x = _f32(x)
     y = np.zeros(A.rows, np.float32)
@@ -231,6 +250,14 @@ def spmv(A: Csr, x, use_ref=False):
         ref().ref_csr_spmv(A.rows, A.cols, _p(A.off), _p(A.col), _p(A.val), _p(x), _p(y))
     else:
         lib().orc_csr_spmv(A.rows, _p(A.off), _p(A.col), _p(A.val), _p(x), _p(y))
+    return y
+
+
+def spmv_mt(A: Csr, x, threads):
+    """spmv() with rows split over `threads` OpenMP threads (same result)."""
+    x = _f32(x)
+    y = np.zeros(A.rows, np.float32)
+    lib().orc_csr_spmv_mt(A.rows, _p(A.off), _p(A.col), _p(A.val), _p(x), _p(y), int(threads))
     return y
 
 

@@ -311,6 +311,180 @@ int64_t orc_update(int64_t R, int64_t C,
     return nnz;
 }
 
+/* orc_update on `threads` OpenMP threads, for the multi-core CPU baseline:
+ * the same batch sort (a stable counting sort by row, done as per-thread
+ * counts over contiguous chunks, an exclusive scan in (row, chunk) order and
+ * a scatter: the order of equal rows is the input order, as the sequential
+ * counting sort gives) and the same per-row cusp::add computation, with the
+ * rows split into contiguous ranges, one per thread, each written to a
+ * private buffer and then copied into place.  Every row's entries and their
+ * order equal orc_update's (pinned bit-exactly against it).  Returns nnz(C),
+ * or a negative value on a bad row index or an allocation failure. */
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+int64_t orc_update_mt(int64_t R, int64_t C,
+                      const idx_t *Aoff, const idx_t *Acol, const float *Aval,
+                      int64_t n, const idx_t *brow, const idx_t *bcol, const float *bval,
+                      idx_t *Coff, idx_t *Ccol, float *Cval, int threads)
+{
+    if (threads < 1) threads = 1;
+    const int T = threads;
+    idx_t *r = (idx_t *)malloc((n > 0 ? n : 1) * sizeof(idx_t));
+    idx_t *c = (idx_t *)malloc((n > 0 ? n : 1) * sizeof(idx_t));
+    float *v = (float *)malloc((n > 0 ? n : 1) * sizeof(float));
+    idx_t *boff = (idx_t *)calloc(R + 1, sizeof(idx_t));
+    int64_t *cnt = (int64_t *)calloc((size_t)T * (R + 1), sizeof(int64_t));
+    int64_t *rown = (int64_t *)calloc(R + 1, sizeof(int64_t));
+    int bad = 0;
+    if (!r || !c || !v || !boff || !cnt || !rown) {
+        free(r); free(c); free(v); free(boff); free(cnt); free(rown);
+        return -1;
+    }
+    /* counting sort by row: chunk t counts its rows, then positions */
+    #pragma omp parallel num_threads(T) reduction(|:bad)
+    {
+#ifdef _OPENMP
+        const int t = omp_get_thread_num();
+#else
+        const int t = 0;
+#endif
+        const int64_t i0 = n * t / T, i1 = n * (t + 1) / T;
+        int64_t *ct = cnt + (size_t)t * (R + 1);
+        for (int64_t i = i0; i < i1; ++i) {
+            if (brow[i] < 0 || brow[i] >= R) { bad = 1; continue; }
+            ct[brow[i]]++;
+        }
+    }
+    if (bad) { free(r); free(c); free(v); free(boff); free(cnt); free(rown); return -2; }
+    {
+        int64_t run = 0;
+        for (int64_t row = 0; row < R; ++row) {
+            boff[row] = (idx_t)run;
+            for (int t = 0; t < T; ++t) {
+                int64_t *ct = cnt + (size_t)t * (R + 1);
+                const int64_t k = ct[row];
+                ct[row] = run;
+                run += k;
+            }
+        }
+        boff[R] = (idx_t)run;
+    }
+    #pragma omp parallel num_threads(T)
+    {
+#ifdef _OPENMP
+        const int t = omp_get_thread_num();
+#else
+        const int t = 0;
+#endif
+        const int64_t i0 = n * t / T, i1 = n * (t + 1) / T;
+        int64_t *ct = cnt + (size_t)t * (R + 1);
+        for (int64_t i = i0; i < i1; ++i) {
+            const int64_t p = ct[brow[i]]++;
+            r[p] = brow[i];
+            c[p] = bcol[i];
+            v[p] = bval[i];
+        }
+    }
+    /* per-row add, rows split into contiguous ranges */
+    idx_t **bufc = (idx_t **)calloc(T, sizeof(idx_t *));
+    float **bufv = (float **)calloc(T, sizeof(float *));
+    int fail = 0;
+    #pragma omp parallel num_threads(T) reduction(|:fail)
+    {
+#ifdef _OPENMP
+        const int t = omp_get_thread_num();
+#else
+        const int t = 0;
+#endif
+        const int64_t r0 = R * t / T, r1 = R * (t + 1) / T;
+        const int64_t cap = (int64_t)(Aoff[r1] - Aoff[r0]) + (int64_t)(boff[r1] - boff[r0]);
+        idx_t *next = (idx_t *)malloc((C > 0 ? C : 1) * sizeof(idx_t));
+        float *arow = (float *)calloc(C > 0 ? C : 1, sizeof(float));
+        float *brw = (float *)calloc(C > 0 ? C : 1, sizeof(float));
+        idx_t *oc = (idx_t *)malloc((cap > 0 ? cap : 1) * sizeof(idx_t));
+        float *ov = (float *)malloc((cap > 0 ? cap : 1) * sizeof(float));
+        if (!next || !arow || !brw || !oc || !ov) {
+            fail = 1;
+        } else {
+            for (int64_t j = 0; j < C; ++j) next[j] = -1;
+            int64_t k = 0;
+            for (int64_t i = r0; i < r1; ++i) {
+                idx_t head = -2, length = 0;
+                for (idx_t jj = Aoff[i]; jj < Aoff[i + 1]; ++jj) {
+                    idx_t j = Acol[jj];
+                    arow[j] += Aval[jj];
+                    if (next[j] == -1) { next[j] = head; head = j; length++; }
+                }
+                for (idx_t jj = boff[i]; jj < boff[i + 1]; ++jj) {
+                    idx_t j = c[jj];
+                    brw[j] += v[jj];
+                    if (next[j] == -1) { next[j] = head; head = j; length++; }
+                }
+                const int64_t k0 = k;
+                for (idx_t jj = 0; jj < length; ++jj) {
+                    float result = arow[head] + brw[head];
+                    if (result != 0.0f) {
+                        oc[k] = head;
+                        ov[k] = result;
+                        k++;
+                    }
+                    idx_t prev = head;
+                    head = next[head];
+                    next[prev] = -1;
+                    arow[prev] = 0.0f;
+                    brw[prev] = 0.0f;
+                }
+                rown[i] = k - k0;
+            }
+        }
+        free(next); free(arow); free(brw);
+        bufc[t] = oc;
+        bufv[t] = ov;
+    }
+    int64_t nnz = -1;
+    if (!fail) {
+        int64_t run = 0;
+        Coff[0] = 0;
+        for (int64_t i = 0; i < R; ++i) {
+            run += rown[i];
+            Coff[i + 1] = (idx_t)run;
+        }
+        nnz = run;
+        #pragma omp parallel num_threads(T)
+        {
+#ifdef _OPENMP
+            const int t = omp_get_thread_num();
+#else
+            const int t = 0;
+#endif
+            const int64_t r0 = R * t / T, r1 = R * (t + 1) / T;
+            const int64_t base = Coff[r0], len = (int64_t)Coff[r1] - base;
+            if (len > 0) {
+                memcpy(Ccol + base, bufc[t], (size_t)len * sizeof(idx_t));
+                memcpy(Cval + base, bufv[t], (size_t)len * sizeof(float));
+            }
+        }
+    }
+    for (int t = 0; t < T; ++t) { free(bufc[t]); free(bufv[t]); }
+    free(bufc); free(bufv);
+    free(r); free(c); free(v); free(boff); free(cnt); free(rown);
+    return nnz;
+}
+
+/* orc_csr_spmv with rows split over `threads` OpenMP threads (each row is
+ * summed in stored order as there: the same result bit for bit). */
+void orc_csr_spmv_mt(int64_t R, const idx_t *off, const idx_t *col, const float *val, const float *x, float *y,
+                     int threads)
+{
+    #pragma omp parallel for num_threads(threads < 1 ? 1 : threads) schedule(static)
+    for (int64_t i = 0; i < R; ++i) {
+        float s = 0.0f;
+        for (idx_t jj = off[i]; jj < off[i + 1]; ++jj) s = s + val[jj] * x[col[jj]];
+        y[i] = s;
+    }
+}
+
 /* Stable sort of each row's (col, val) by column, in place (insertion sort
  * for short rows, a stable merge sort through scratch for long ones). */
 static void merge_by_col(idx_t *c, float *v, idx_t *tc, float *tv, int64_t n)

@@ -276,3 +276,41 @@ def test_hyb_spmv_against_fp64():
     y64 = S @ x.astype(np.float64)
     scale = abs(S) @ np.abs(x.astype(np.float64))
     assert (np.abs(O.hyb_spmv(H, x) - y64) <= 1e-5 * scale + 1e-30).all()
+
+
+@pytest.mark.parametrize("threads", [1, 3, 8])
+def test_update_mt_equals_sequential(threads):
+    """The multi-threaded update (multi-core CPU baseline) gives the
+    sequential update's CSR exactly: offsets, column order within rows and
+    value bits, with duplicates, cancellations and empty rows."""
+    rng = np.random.default_rng(90 + threads)
+    R, C = 500, 300
+    r = rng.integers(0, R, 4000).astype(np.int32)
+    c = rng.integers(0, C, 4000).astype(np.int32)
+    v = rng.uniform(-1, 1, 4000).astype(np.float32)
+    A = O.coo_to_csr(R, C, r, c, v)
+    for k in range(3):
+        br = rng.integers(0, R // 2, 3000).astype(np.int32)  # half the rows untouched
+        bc = rng.integers(0, C, 3000).astype(np.int32)
+        bv = rng.integers(-2, 3, 3000).astype(np.float32)
+        want = O.update(A, br, bc, bv)
+        got = O.update_mt(A, br, bc, bv, threads)
+        np.testing.assert_array_equal(got.off, want.off)
+        np.testing.assert_array_equal(got.col, want.col)
+        np.testing.assert_array_equal(got.val.view(np.uint32), want.val.view(np.uint32))
+        A = want
+
+
+def test_update_mt_rejects_bad_row():
+    A = O.coo_to_csr(4, 4, np.array([0], np.int32), np.array([0], np.int32), np.array([1.0], np.float32))
+    with pytest.raises(ValueError):
+        O.update_mt(A, np.array([4], np.int32), np.array([0], np.int32), np.array([1.0], np.float32), 2)
+
+
+def test_spmv_mt_equals_sequential():
+    rng = np.random.default_rng(95)
+    R, C = 700, 400
+    A = O.coo_to_csr(R, C, rng.integers(0, R, 9000).astype(np.int32), rng.integers(0, C, 9000).astype(np.int32),
+                     rng.uniform(-1, 1, 9000).astype(np.float32))
+    x = rng.uniform(-1, 1, C).astype(np.float32)
+    np.testing.assert_array_equal(O.spmv_mt(A, x, 5).view(np.uint32), O.spmv(A, x).view(np.uint32))
