@@ -22,10 +22,11 @@ LIB_PATH = os.path.join(_HERE, "libdcsr.so")
 HEADER_PATH = os.path.join(os.path.dirname(_HERE), "include", "dcsr.h")
 
 DCSR_OK = 0
+DCSR_ERR_TOO_LARGE = 9
 STATUS_NAMES = {
     0: "DCSR_OK", 1: "DCSR_ERR_INVALID", 2: "DCSR_ERR_BAD_INDEX", 3: "DCSR_ERR_POOL_EXHAUSTED",
     4: "DCSR_ERR_SEGMENT_OVERFLOW", 5: "DCSR_ERR_HIP", 6: "DCSR_ERR_OOM", 7: "DCSR_ERR_CAPACITY",
-    8: "DCSR_ERR_UNSUPPORTED",
+    8: "DCSR_ERR_UNSUPPORTED", 9: "DCSR_ERR_TOO_LARGE",
 }
 UPDATE_NOSYNC = 0x1
 UPDATE_NOGROW = 0x2
@@ -147,6 +148,8 @@ def lib():
         "dcsr_hyb_from_dcsr": (ctypes.c_int, [_H, _H, _P]),
         "dcsr_set_scan_wait_limit": (None, [ctypes.c_int32]),
         "dcsr_set_update_fast_path": (None, [ctypes.c_int32]),
+        "dcsr_update_noalloc": (ctypes.c_int, [_H, _P, _P, _P, _I64, _U32, _P]),
+        "dcsr_update_noalloc_max_triples": (_I64, []),
         "dcsr_csr_add": (ctypes.c_int, [_I64, _I64, _P, _P, _P, _P, _P, _P, _I64, _P, _P, _P, _I64,
                                         ctypes.POINTER(_I64), _P]),
         "dcsr_csr_spgemm": (ctypes.c_int, [_I64, _I64, _I64, _P, _P, _P, _P, _P, _P, _P, _P, _P, _I64,

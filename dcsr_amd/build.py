@@ -17,7 +17,7 @@ OBJ = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libdcsr.so")
 
 HIP_SOURCES = ["prims.hip", "gsort.hip", "mtx.hip", "csrcmp.hip", "matrix.hip", "update.hip", "spmv.hip", "spgemm.hip", "hyb.hip", "hybstore.hip", "batch.hip",
-               "server.hip"]
+               "server.hip", "noalloc.hip"]
 CXX_SOURCES: list[str] = []
 
 ARCH = os.environ.get("DCSR_OFFLOAD_ARCH", "gfx950")

@@ -275,6 +275,7 @@ const char *dcsr_status_string(dcsr_status s)
     case DCSR_ERR_OOM: return "device out of memory";
     case DCSR_ERR_CAPACITY: return "output buffer too small";
     case DCSR_ERR_UNSUPPORTED: return "unsupported";
+    case DCSR_ERR_TOO_LARGE: return "request exceeds the allocation-free entry point's fixed scratch";
     }
     return "unknown";
 }

@@ -47,7 +47,8 @@ typedef enum dcsr_status {
     DCSR_ERR_HIP = 5,            /* a HIP runtime call failed */
     DCSR_ERR_OOM = 6,            /* device allocation failed */
     DCSR_ERR_CAPACITY = 7,       /* caller-provided output buffer too small */
-    DCSR_ERR_UNSUPPORTED = 8
+    DCSR_ERR_UNSUPPORTED = 8,
+    DCSR_ERR_TOO_LARGE = 9       /* request exceeds the fixed scratch of dcsr_update_noalloc */
 } dcsr_status;
 
 /* Flags for dcsr_update. */
@@ -356,6 +357,23 @@ dcsr_status dcsr_export_layout(dcsr_handle h, uint32_t *row_start, int32_t *row_
  * results never depend on it; 0 sends every tile down that fallback without
  * waiting (used to test it).  No reference counterpart. */
 void dcsr_set_scan_wait_limit(int32_t polls);
+
+/* dcsr_update without any allocation.  No reference counterpart: for callers
+ * that cannot allocate device memory (code under HIP graph capture, tight
+ * serving loops, harnesses that forbid hipMalloc).  Same arguments, flags and
+ * results as dcsr_update, with these limits: at most
+ * dcsr_update_noalloc_max_triples() triples (65,536) per call, else
+ * DCSR_ERR_TOO_LARGE and nothing happens; the matrix must be canonical
+ * (updated, compacted or an SpGEMM output once), else DCSR_ERR_INVALID; the
+ * pool is never grown or compacted, so a batch whose chunk claims do not fit
+ * the free pool is rejected whole (DCSR_ERR_POOL_EXHAUSTED, or the sticky bit
+ * with DCSR_UPDATE_NOSYNC).  The scratch is one fixed device array per GPU
+ * (8 MiB, held until the process ends) shared by every call on that GPU:
+ * calls that overlap on one GPU (two streams, two threads) race, so callers
+ * serialize them. */
+dcsr_status dcsr_update_noalloc(dcsr_handle h, const int32_t *rows, const int32_t *cols, const float *vals,
+                                int64_t n, uint32_t flags, dcsr_stream_t stream);
+int64_t dcsr_update_noalloc_max_triples(void);
 
 /* Process-wide switch of dcsr_update's one-pass path (default on): batches
  * of up to 2^21 triples whose chunk claims the host can bound below the free
