@@ -229,7 +229,13 @@ def bench_powerlaw(args):
 
     # warmup on the same handle (first-touch workspace allocation happens
     # here), then reload the initial matrix: the timed steps start from it
-    m = dcsr_amd.DcsrMatrix(R_CFG, R_CFG, pool_factor=8.0)
+    # pool: 16 x the initial 3.2M entries (410 MB of HBM).  Every batch
+    # rewrites its touched rows (~85% of the live entries, hub rows included)
+    # into fresh chunks, so the old chunks pile up as garbage; at 16x the bump
+    # region holds a few batches between compactions at every size this run
+    # reaches (13.2M live entries after 20 batches), where 8x compacts before
+    # nearly every batch once the matrix passes ~8M entries.
+    m = dcsr_amd.DcsrMatrix(R_CFG, R_CFG, pool_factor=16.0)
     m.LoadMatrix(d_r, d_c, d_v)
     for k in range(Wm):
         m.UpdateMatrix(*d_b[K + k], flags=dcsr_amd.UPDATE_NOSYNC)

@@ -147,7 +147,6 @@ def lib():
         "dcsr_hyb_to_dcsr": (ctypes.c_int, [_H, _H, _P]),
         "dcsr_hyb_from_dcsr": (ctypes.c_int, [_H, _H, _P]),
         "dcsr_set_scan_wait_limit": (None, [ctypes.c_int32]),
-        "dcsr_set_update_fast_path": (None, [ctypes.c_int32]),
         "dcsr_update_noalloc": (ctypes.c_int, [_H, _P, _P, _P, _I64, _U32, _P]),
         "dcsr_update_noalloc_max_triples": (_I64, []),
         "dcsr_csr_add": (ctypes.c_int, [_I64, _I64, _P, _P, _P, _P, _P, _P, _I64, _P, _P, _P, _I64,
@@ -660,12 +659,6 @@ def set_scan_wait_limit(polls):
     """Poll limit of the scans' bounded look-back waits (negative: the
     default); 0 sends every tile down its exact fallback without waiting."""
     lib().dcsr_set_scan_wait_limit(int(polls))
-
-
-def set_update_fast_path(enable):
-    """Process-wide: 0 sends every dcsr_update batch down the multi-launch
-    pipeline (include/dcsr.h dcsr_set_update_fast_path)."""
-    lib().dcsr_set_update_fast_path(1 if enable else 0)
 
 
 def csr_spmv(row_offsets, cols, vals, x, y=None):

@@ -143,6 +143,9 @@ struct dcsr_matrix_s {
 
 dcsr_status dcsr_hip_fail(hipError_t e, const char *what, const char *file, int line);
 dcsr_status ws_get(dcsr_handle h, WsSlot slot, size_t bytes, void **out);
+// For slots whose words must start at zero (look-back control and status
+// words, finish counters): a grown buffer is zero-filled by a kernel on s.
+dcsr_status ws_get_zeroed(dcsr_handle h, WsSlot slot, size_t bytes, void **out, hipStream_t s);
 dcsr_status pull_state(dcsr_handle h, hipStream_t s);   // copy DevState to host + sync
 // Grow the pool in stream order, without reading the device state back:
 // the live rows are compacted into a new pool of at least new_cap entries.

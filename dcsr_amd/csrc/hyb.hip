@@ -114,7 +114,7 @@ dcsr_status dcsr_export_hyb(dcsr_handle h, int32_t width, int64_t pitch, int32_t
     uint32_t *ov, *coff, *part;
     DCSR_TRY(ws_get(h, WS_TMP1, (size_t)(R + 1) * 4, (void **)&ov));
     DCSR_TRY(ws_get(h, WS_TMP0, (size_t)(R + 1) * 4, (void **)&coff));
-    DCSR_TRY(ws_get(h, WS_PART, scan_part_words(R + 1, 1) * 4, (void **)&part));
+    DCSR_TRY(ws_get_zeroed(h, WS_PART, scan_part_words(R + 1, 1) * 4, (void **)&part, s));
     if (R > 0) k_hyb_overflow<<<grid_for(R, TB), TB, 0, s>>>(R, h->row_len, (uint32_t)width, ov);
     DCSR_HIP(scan_exclusive_u32(ov, coff, R, 1, part, coff + R, s, fault_word(h)));
     uint32_t ncoo = 0;
@@ -153,7 +153,7 @@ dcsr_status dcsr_load_hyb(dcsr_handle h, int32_t width, int64_t pitch, const int
     uint32_t *flag, *pos, *part;
     DCSR_TRY(ws_get(h, WS_FLAGS, (size_t)(ncell + 2) * 4, (void **)&flag));
     DCSR_TRY(ws_get(h, WS_TMP0, (size_t)(ncell + 2) * 4, (void **)&pos));
-    DCSR_TRY(ws_get(h, WS_PART, scan_part_words(ncell + 2, 1) * 4, (void **)&part));
+    DCSR_TRY(ws_get_zeroed(h, WS_PART, scan_part_words(ncell + 2, 1) * 4, (void **)&part, s));
     k_hyb_ell_flags<<<grid_for(ncell + 1, TB), TB, 0, s>>>(ell_col, ncell, (uint64_t)pitch, (uint32_t)h->rows, flag);
     DCSR_HIP(scan_exclusive_u32(flag, pos, ncell + 1, 1, part, nullptr, s, fault_word(h)));
     uint32_t nell = 0;

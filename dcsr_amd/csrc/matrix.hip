@@ -280,28 +280,60 @@ const char *dcsr_status_string(dcsr_status s)
     return "unknown";
 }
 
+namespace {
+__global__ void k_zero_words(uint32_t *p, int64_t n)
+{
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        p[i] = 0;
+}
+}  // namespace
+
+static void release_retired(dcsr_handle h);
+
+// Grow a slot.  The old buffer may still be read by work queued on any
+// stream, so it is retired (freed at the next state read-back or at
+// destroy), never freed here: hipFree would wait for the whole device.
+static dcsr_status ws_grow(dcsr_handle h, WsSlot slot, size_t bytes)
+{
+    if (h->ws.ptr[slot]) {
+        if (h->nretired == (int)(sizeof(h->retired) / sizeof(h->retired[0]))) {
+            DCSR_HIP(hipDeviceSynchronize());
+            release_retired(h);
+        }
+        h->retired[h->nretired++] = h->ws.ptr[slot];
+    }
+    h->ws.ptr[slot] = nullptr;
+    h->ws.bytes[slot] = 0;
+    const size_t nb = bytes + bytes / 4;  // headroom for growing batches
+    DCSR_HIP(hipMalloc(&h->ws.ptr[slot], nb));
+    h->ws.bytes[slot] = nb;
+    return DCSR_OK;
+}
+
 dcsr_status ws_get(dcsr_handle h, WsSlot slot, size_t bytes, void **out)
 {
     if (bytes == 0) bytes = 16;
+    if (h->ws.bytes[slot] < bytes) DCSR_TRY(ws_grow(h, slot, bytes));
+    *out = h->ws.ptr[slot];
+    return DCSR_OK;
+}
+
+dcsr_status ws_get_zeroed(dcsr_handle h, WsSlot slot, size_t bytes, void **out, hipStream_t s)
+{
+    if (bytes == 0) bytes = 16;
     if (h->ws.bytes[slot] < bytes) {
-        if (h->ws.ptr[slot]) DCSR_HIP(hipFree(h->ws.ptr[slot]));
-        h->ws.ptr[slot] = nullptr;
-        h->ws.bytes[slot] = 0;
-        size_t nb = bytes + bytes / 4;  // headroom for growing batches
-        DCSR_HIP(hipMalloc(&h->ws.ptr[slot], nb));
-        h->ws.bytes[slot] = nb;
-        // slots that hold self-resetting control words (scan tickets, finish
-        // counters) start zero-filled; the others are written before use
-        if (slot == WS_SCAN || slot == WS_PART || slot == WS_TOLDS) {
-            DCSR_HIP(hipMemset(h->ws.ptr[slot], 0, nb));
-            DCSR_HIP(hipDeviceSynchronize());
-        }
+        DCSR_TRY(ws_grow(h, slot, bytes));
+        // stream-ordered before the slot's first use on s
+        const int64_t words = (int64_t)(h->ws.bytes[slot] / 4);
+        int64_t blocks = (words + 255) / 256;
+        if (blocks > 1024) blocks = 1024;
+        k_zero_words<<<(unsigned)blocks, 256, 0, s>>>((uint32_t *)h->ws.ptr[slot], words);
+        DCSR_HIP(hipGetLastError());
     }
     *out = h->ws.ptr[slot];
     return DCSR_OK;
 }
 
-static void release_retired(dcsr_handle h);
 
 dcsr_status pull_state(dcsr_handle h, hipStream_t s)
 {
@@ -514,7 +546,7 @@ static dcsr_status load_sorted(dcsr_handle h, const uint64_t *key, const uint32_
     const int64_t R = h->rows;
     uint32_t *first, *scan_part, *flag;
     DCSR_TRY(ws_get(h, WS_TMP0, (size_t)(R + 1) * sizeof(uint32_t), (void **)&first));
-    DCSR_TRY(ws_get(h, WS_PART, scan_part_words(R + 1, 1) * sizeof(uint32_t), (void **)&scan_part));
+    DCSR_TRY(ws_get_zeroed(h, WS_PART, scan_part_words(R + 1, 1) * sizeof(uint32_t), (void **)&scan_part, s));
     DCSR_TRY(ws_get(h, WS_TMP2, 64, (void **)&flag));
     k_fill_i32<<<grid_for(R, TB * 4), TB, 0, s>>>(h->row_len, R, 0);
     k_fill_i32<<<1, 64, 0, s>>>((int32_t *)flag, 2, 0);
@@ -568,7 +600,7 @@ static dcsr_status load_coo_impl(dcsr_handle h, const int32_t *rows, const int32
     DCSR_TRY(ws_get(h, WS_VAL_A, (size_t)n * 4, (void **)&v0));
     DCSR_TRY(ws_get(h, WS_VAL_B, (size_t)n * 4, (void **)&v1));
     DCSR_TRY(ws_get(h, WS_HIST, radix_hist_words(n) * 4, (void **)&hist));
-    DCSR_TRY(ws_get(h, WS_SCAN, radix_part_words(n) * 4, (void **)&part));
+    DCSR_TRY(ws_get_zeroed(h, WS_SCAN, radix_part_words(n) * 4, (void **)&part, s));
     k_clear_status<<<1, 1, 0, s>>>(h->ds);
     if (n > 0)
         k_pack_keys<<<grid_for(n, TB), TB, 0, s>>>(rows, cols, vals, n, h->rows, h->cols, h->cbits, k0, v0, h->ds);
@@ -623,7 +655,7 @@ static dcsr_status row_offsets_u32(dcsr_handle h, uint32_t *off, hipStream_t s)
 {
     uint32_t *part;
     const int64_t R = h->rows;
-    DCSR_TRY(ws_get(h, WS_PART, scan_part_words(R + 1, 1) * 4, (void **)&part));
+    DCSR_TRY(ws_get_zeroed(h, WS_PART, scan_part_words(R + 1, 1) * 4, (void **)&part, s));
     // lengths are non-negative int32: scan them as uint32
     DCSR_HIP(scan_exclusive_u32((const uint32_t *)h->row_len, off, R, 1, part, off + R, s, fault_word(h)));
     return DCSR_OK;
@@ -673,7 +705,7 @@ dcsr_status dcsr_export_csr(dcsr_handle h, int32_t *row_offsets, int32_t *cols, 
     DCSR_TRY(ws_get(h, WS_UCODE, (size_t)nnz * 4 + 4, (void **)&emit));
     DCSR_TRY(ws_get(h, WS_UPRE, (size_t)nnz * 4 + 4, (void **)&pos));
     DCSR_TRY(ws_get(h, WS_TOLDPRE, (size_t)(R + 1) * 4, (void **)&toff));
-    DCSR_TRY(ws_get(h, WS_SCAN, scan_part_words(nnz + 1, 1) * 4, (void **)&part));
+    DCSR_TRY(ws_get_zeroed(h, WS_SCAN, scan_part_words(nnz + 1, 1) * 4, (void **)&part, s));
     DCSR_TRY(ws_get(h, WS_TMP2, 64, (void **)&tot));
     k_u32_to_i32_offsets<<<grid_for(R + 1, TB), TB, 0, s>>>(off, R + 1, toff);
     launch_copy_rows(off, R, nnz, h->row_start, h->pool_col, h->pool_val, tcol, tval, s);

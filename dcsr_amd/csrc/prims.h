@@ -38,7 +38,10 @@ hipError_t radix_sort_pairs(uint64_t *k0, uint32_t *v0, uint64_t *k1, uint32_t *
 // RADIX_TILE tile of (k0, v0) into hist (blocked: see gs_hist_index); the
 // stable result ends in (k0, v0), (k1, v1) is scratch.
 constexpr int GS_MAX_DIGITS = 2048;  // groups + 1
-constexpr int GS_CAP = 3072;         // pairs of a group sorted in LDS
+#ifndef DCSR_GS_CAP
+#define DCSR_GS_CAP 3072
+#endif
+constexpr int GS_CAP = DCSR_GS_CAP;  // pairs of a group sorted in LDS
 // Count of digit d in tile t: 64-digit blocks, each tile-major.
 __host__ __device__ inline int64_t gs_hist_index(int d, int64_t t, int64_t ntiles)
 {

@@ -701,7 +701,7 @@ dcsr_status dcsr_spgemm(dcsr_handle A, dcsr_handle B, dcsr_handle C, dcsr_stream
     DCSR_TRY(ws_get(C, WS_SG_LIST, (size_t)(R + 1) * 4 * NCLS, (void **)&lists));
     DCSR_TRY(ws_get(C, WS_SG_STATE, sizeof(SgState), (void **)&st));
     DCSR_TRY(ws_get(C, WS_SG_BSUM, (size_t)nbR * 8, (void **)&bsum));
-    DCSR_TRY(ws_get(C, WS_PART, scan_part_words(R + 1, 2) * 4, (void **)&part));
+    DCSR_TRY(ws_get_zeroed(C, WS_PART, scan_part_words(R + 1, 2) * 4, (void **)&part, s));
 
     // 1. partial counts, classes, offsets
     k_sg_state_zero<<<1, 1, 0, s>>>(st);
@@ -784,7 +784,7 @@ dcsr_status dcsr_spgemm(dcsr_handle A, dcsr_handle B, dcsr_handle C, dcsr_stream
         size_t pw = scan_part_words(nent + 1, 1);
         if (scan_part_words((int64_t)np + 1, 1) > pw) pw = scan_part_words((int64_t)np + 1, 1);
         if (radix_part_words(np) > pw) pw = radix_part_words(np);
-        DCSR_TRY(ws_get(C, WS_SCAN, pw * 4, (void **)&part));
+        DCSR_TRY(ws_get_zeroed(C, WS_SCAN, pw * 4, (void **)&part, s));
         DCSR_HIP(scan_exclusive_u32(blen, apos, nent, 1, part, apos + nent, s, fault_word(C)));
         DCSR_TRY(ws_get(C, WS_SG_KEY0, (size_t)(np + 1) * 8, (void **)&k0));
         DCSR_TRY(ws_get(C, WS_SG_KEY1, (size_t)(np + 1) * 8, (void **)&k1));
@@ -811,7 +811,7 @@ dcsr_status dcsr_spgemm(dcsr_handle A, dcsr_handle B, dcsr_handle C, dcsr_stream
 
     // 3. C's row table: chunks at the partial offsets; nnz from a scan of
     // the row counts
-    DCSR_TRY(ws_get(C, WS_PART, scan_part_words(R + 1, 1) * 4, (void **)&part));
+    DCSR_TRY(ws_get_zeroed(C, WS_PART, scan_part_words(R + 1, 1) * 4, (void **)&part, s));
     DCSR_HIP(scan_exclusive_u32(cnt, coff, R, 1, part, coff + R, s, fault_word(C)));
     k_sg_set_rows<<<grid_for(R + 1, TB), TB, 0, s>>>(R, Poff, P, cnt, coff + R, Ptot, C->row_start, C->row_len,
                                                      C->row_cap, C->ds);

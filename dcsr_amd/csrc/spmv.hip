@@ -874,7 +874,7 @@ dcsr_status ensure_flat(dcsr_handle h, hipStream_t s)
     DCSR_TRY(ws_get(h, WS_FOFF, (size_t)(R + 1) * 4, (void **)&foff));
     // look-back block: 4 control words and one 64-bit status word per tile
     const int64_t tiles = (R + SR_TILE - 1) / SR_TILE;
-    DCSR_TRY(ws_get(h, WS_PART, (size_t)(4 + 2 * tiles) * 4, (void **)&part));
+    DCSR_TRY(ws_get_zeroed(h, WS_PART, (size_t)(4 + 2 * tiles) * 4, (void **)&part, s));
     // the row map covers the live entries, which the pool bounds: sized by
     // the pool, it needs no reallocation while the matrix grows within it
     // (no read-back, also after NOSYNC updates)
@@ -987,7 +987,7 @@ static dcsr_status build_panels(dcsr_handle h, hipStream_t s)
     DCSR_TRY(ws_get(h, WS_PNL_VAL, (size_t)(nnz + 1) * 4, (void **)&pval));
     DCSR_TRY(ws_get(h, WS_PNL_MAP, (size_t)(nnz / GRAN + 1) * 4, (void **)&pmap));
     DCSR_TRY(ws_get(h, WS_PNL_PART, (size_t)Rv * 4, (void **)&ppart));
-    DCSR_TRY(ws_get(h, WS_PART, scan_part_words(Rv + 1, 1) * 4, (void **)&part));
+    DCSR_TRY(ws_get_zeroed(h, WS_PART, scan_part_words(Rv + 1, 1) * 4, (void **)&part, s));
     const uint32_t *foff = (const uint32_t *)h->ws.ptr[WS_FOFF];
     const uint32_t *rowmap = (const uint32_t *)h->ws.ptr[WS_ROWMAP];
     // per-(panel, row) counts go to the partial buffer (free until the first

@@ -44,8 +44,6 @@ namespace {
 
 constexpr int TB = 256;
 enum { C_NOP = 0, C_ADD = 1, C_DEL = 2, C_INS = 3 };
-// process-wide: send every batch down the multi-launch pipeline (tests)
-bool g_no_fast = false;
 enum { M_VALUES = 0, M_REWRITE = 1 };
 constexpr int NPLAN = 5;  // per-block plan counters
 
@@ -972,413 +970,6 @@ struct UpdBufs {
     int lb, ndig;
 };
 
-// ---------------------------------------------------------------------------
-// One-pass apply of a group-sorted batch (the fast path).  When the host's
-// claim bound guarantees room in the pool's bump region (every touched row
-// rewritten whole, every triple inserted), no batch-wide decision is needed
-// before writing, and one workgroup per row group does the heads, probe,
-// plan, move and write of its rows alone: the group's pairs are contiguous
-// and sorted, its rows belong to no other workgroup, and its chunk claims
-// take one atomic on the bump pointer.  The results equal the multi-launch
-// pipeline's (same sums, same codes, same chunk sizes; only the chunks'
-// places in the bump region differ).  Records and the touched-row table live
-// in LDS when the group fits (AP_REC pairs, AP_ROW touched rows), else in the
-// group's slice of the workspace arrays (the same code through generic
-// pointers).
-// ---------------------------------------------------------------------------
-constexpr int AP_REC = 1536;
-constexpr int AP_ROW = 256;
-constexpr int AP_IPR = 4;  // probe searches per thread in lock step
-constexpr int AP_MV = 4;   // moved entries per thread per round
-
-struct ApplyLds {
-    int32_t rcol[AP_REC];
-    float rval[AP_REC];
-    uint32_t rt[AP_REC], rlo[AP_REC], rcode[AP_REC], rpre[2 * AP_REC + 2];
-    uint32_t tr[AP_ROW], tbeg[AP_ROW + 1], tL[AP_ROW], tst[AP_ROW], tcap[AP_ROW], tins[AP_ROW], tdel[AP_ROW];
-    uint32_t tpre[2 * AP_ROW + 2];
-    uint32_t lds4[4];
-    long long acc[4][8];
-    uint32_t s_u[2];
-    unsigned long long base;
-};
-
-// The group's record and row arrays: LDS or its workspace slice.
-struct ApplyView {
-    int32_t *rcol;
-    float *rval;
-    uint32_t *rt, *rlo, *rcode, *rpre;
-    uint32_t *tr, *tbeg, *tL, *tst, *tcap, *tins, *tdel, *tpre;
-};
-
-__device__ __forceinline__ long long block_sum_i64(long long v, long long (*acc)[8], int slot)
-{
-    const long long w = wave_sum_i64(v);
-    if (lane_id() == 0) acc[threadIdx.x >> 6][slot] = w;
-    __syncthreads();
-    const long long t = (acc[0][slot] + acc[1][slot]) + (acc[2][slot] + acc[3][slot]);
-    __syncthreads();
-    return t;
-}
-
-__global__ __launch_bounds__(TB) void k_upd_apply(DevState *ds, const uint64_t *__restrict__ key,
-                                                  const uint32_t *__restrict__ val, const uint32_t *__restrict__ gbase,
-                                                  int ngroups, int cbits, int gshift, UpdBufs b,
-                                                  uint32_t *__restrict__ row_start,
-                                                  int32_t *__restrict__ row_len, int32_t *__restrict__ row_cap,
-                                                  int32_t *__restrict__ pcol, float *__restrict__ pval)
-{
-    __shared__ ApplyLds S;
-    if (!ds->go) return;
-    const int g = blockIdx.x;
-    if (g >= ngroups) return;
-    const int64_t s0 = gbase[g];
-    const int64_t ng = (int64_t)gbase[g + 1] - s0;
-    if (ng <= 0) return;
-    // touched rows <= min(pairs, rows of the group)
-    const int64_t tmax = ng < (int64_t(1) << gshift) ? ng : (int64_t(1) << gshift);
-    ApplyView V;
-    const int64_t go = s0 + g;  // slice start of the arrays with one extra entry per group
-    if (ng <= AP_REC) {
-        V.rcol = S.rcol; V.rval = S.rval; V.rt = S.rt; V.rlo = S.rlo; V.rcode = S.rcode; V.rpre = S.rpre;
-    } else {
-        V.rcol = b.ucol + s0; V.rval = b.uval + s0; V.rt = b.ut + s0; V.rlo = b.upos + s0; V.rcode = b.ucode + s0;
-        V.rpre = b.upre + 2 * go;
-    }
-    if (tmax <= AP_ROW) {
-        V.tr = S.tr; V.tbeg = S.tbeg; V.tL = S.tL; V.tst = S.tst; V.tcap = S.tcap; V.tins = S.tins;
-        V.tdel = S.tdel; V.tpre = S.tpre;
-    } else {
-        V.tr = b.trow + s0; V.tbeg = b.tubeg + go; V.tL = b.tmoved + s0; V.tst = b.urow + s0;
-        V.tcap = b.tmode + s0; V.tins = b.tins + s0; V.tdel = b.tdel + s0; V.tpre = b.tpre + 2 * go;
-    }
-    const uint64_t cmask = (1ull << cbits) - 1ull;
-
-    // heads: records (distinct coordinates, summed left to right from 0 in
-    // batch order) and touched rows, 2048 pairs per round
-    uint32_t U = 0, T = 0;
-    for (int64_t c0 = 0; c0 < ng; c0 += TILE) {
-        const int64_t i0 = s0 + c0 + (int64_t)threadIdx.x * IPT;
-        const int64_t iend = s0 + ng;
-        uint64_t k[IPT];
-        uint32_t vb[IPT];
-        const uint64_t prev = i0 > s0 && i0 - 1 < iend ? key[i0 - 1] : ~0ull;
-#pragma unroll
-        for (int j = 0; j < IPT; ++j) {
-            const int64_t i = i0 + j;
-            k[j] = i < iend ? key[i] : ~0ull;
-            vb[j] = i < iend ? val[i] : 0u;
-        }
-        uint32_t hm = 0, rm = 0;
-        {
-            uint64_t p = prev;
-#pragma unroll
-            for (int j = 0; j < IPT; ++j) {
-                if (k[j] != ~0ull) {
-                    if (k[j] != p) hm |= 1u << j;
-                    if ((k[j] >> cbits) != (p >> cbits)) rm |= 1u << j;
-                }
-                p = k[j];
-            }
-        }
-        uint32_t htot, rtot;
-        uint32_t lu = U + block_excl_scan_256(__popc(hm), S.lds4, &htot);
-        uint32_t lt = T + block_excl_scan_256(__popc(rm), S.lds4, &rtot);
-#pragma unroll
-        for (int j = 0; j < IPT; ++j) {
-            if (!((hm >> j) & 1u)) continue;
-            float sum = 0.0f + __uint_as_float(vb[j]);
-            bool open = true;
-#pragma unroll
-            for (int m = j + 1; m < IPT; ++m) {
-                if (open && k[m] == k[j]) sum = sum + __uint_as_float(vb[m]);
-                else open = false;
-            }
-            if (open)
-                for (int64_t m = i0 + IPT; m < iend && key[m] == k[j]; ++m) sum = sum + __uint_as_float(val[m]);
-            const bool rh = (rm >> j) & 1u;
-            if (rh) {
-                V.tr[lt] = (uint32_t)(k[j] >> cbits);
-                V.tbeg[lt] = lu;
-                ++lt;
-            }
-            V.rcol[lu] = (int32_t)(k[j] & cmask);
-            V.rval[lu] = sum;
-            V.rt[lu] = lt - 1u;
-            ++lu;
-        }
-        U += htot;
-        T += rtot;
-    }
-    if (threadIdx.x == 0) V.tbeg[T] = U;
-    __syncthreads();
-    // the rows' current chunks
-    for (uint32_t t = threadIdx.x; t < T; t += TB) {
-        const uint32_t r = V.tr[t];
-        V.tL[t] = (uint32_t)row_len[r];
-        V.tst[t] = row_start[r];
-        V.tcap[t] = (uint32_t)row_cap[r];
-        V.tins[t] = 0;
-        V.tdel[t] = 0;
-    }
-    __syncthreads();
-
-    // probe: lower bound of every record's column in its row, AP_IPR
-    // searches per thread in lock step
-    long long n_ins = 0, n_add = 0, n_del = 0;
-    for (uint32_t u0 = 0; u0 < U; u0 += TB * AP_IPR) {
-        int32_t lo[AP_IPR], hi[AP_IPR], cc[AP_IPR];
-        uint32_t st[AP_IPR], tt[AP_IPR];
-#pragma unroll
-        for (int q = 0; q < AP_IPR; ++q) {
-            const uint32_t u = u0 + q * TB + threadIdx.x;
-            lo[q] = hi[q] = cc[q] = 0;
-            st[q] = tt[q] = 0;
-            if (u < U) {
-                tt[q] = V.rt[u];
-                cc[q] = V.rcol[u];
-                st[q] = V.tst[tt[q]];
-                hi[q] = (int32_t)V.tL[tt[q]];
-            }
-        }
-        for (;;) {
-            bool any = false;
-            int32_t pc[AP_IPR];
-#pragma unroll
-            for (int q = 0; q < AP_IPR; ++q) {
-                pc[q] = 0;
-                if (lo[q] < hi[q]) {
-                    pc[q] = pcol[(uint64_t)st[q] + (uint32_t)((lo[q] + hi[q]) >> 1)];
-                    any = true;
-                }
-            }
-            if (!any) break;
-#pragma unroll
-            for (int q = 0; q < AP_IPR; ++q) {
-                if (lo[q] < hi[q]) {
-                    const int32_t mid = (lo[q] + hi[q]) >> 1;
-                    if (pc[q] < cc[q]) lo[q] = mid + 1; else hi[q] = mid;
-                }
-            }
-        }
-#pragma unroll
-        for (int q = 0; q < AP_IPR; ++q) {
-            const uint32_t u = u0 + q * TB + threadIdx.x;
-            if (u >= U) continue;
-            const uint32_t L = V.tL[tt[q]];
-            const float s = V.rval[u];
-            uint32_t code;
-            float nv;
-            if ((uint32_t)lo[q] < L && pcol[(uint64_t)st[q] + lo[q]] == cc[q]) {
-                nv = pval[(uint64_t)st[q] + lo[q]] + s;
-                code = nv != 0.0f ? C_ADD : C_DEL;
-            } else {
-                nv = s;
-                code = s != 0.0f ? C_INS : C_NOP;
-            }
-            V.rval[u] = nv;
-            V.rlo[u] = (uint32_t)lo[q];
-            V.rcode[u] = code;
-            if (code == C_INS) atomicAdd(&V.tins[tt[q]], 1u);
-            if (code == C_DEL) atomicAdd(&V.tdel[tt[q]], 1u);
-            n_ins += code == C_INS;
-            n_add += code == C_ADD;
-            n_del += code == C_DEL;
-        }
-    }
-    __syncthreads();
-    // inserts / deletes before each record (rpre[2u], rpre[2u + 1]; u <= U)
-    {
-        uint32_t ci = 0, cd = 0;
-        for (uint32_t c0 = 0; c0 <= U; c0 += TILE) {
-            const uint32_t e0 = c0 + threadIdx.x * IPT;
-            uint32_t fi[IPT], fd[IPT], si = 0, sd = 0;
-#pragma unroll
-            for (int j = 0; j < IPT; ++j) {
-                const uint32_t code = e0 + j < U ? V.rcode[e0 + j] : (uint32_t)C_NOP;
-                fi[j] = code == C_INS;
-                fd[j] = code == C_DEL;
-                si += fi[j];
-                sd += fd[j];
-            }
-            uint32_t ti, td;
-            uint32_t pi = ci + block_excl_scan_256(si, S.lds4, &ti);
-            uint32_t pd = cd + block_excl_scan_256(sd, S.lds4, &td);
-#pragma unroll
-            for (int j = 0; j < IPT; ++j) {
-                if (e0 + j <= U) {
-                    V.rpre[2 * (e0 + j)] = pi;
-                    V.rpre[2 * (e0 + j) + 1] = pd;
-                }
-                pi += fi[j];
-                pd += fd[j];
-            }
-            ci += ti;
-            cd += td;
-        }
-    }
-    // plan: new lengths, chunk claims and old entries to move per row
-    // (tpre[2t], tpre[2t + 1]; t <= T)
-    uint32_t claim_tot = 0, moved_tot = 0;
-    long long d_nnz = 0, d_garb = 0;
-    uint32_t hub = 0;
-    for (uint32_t c0 = 0; c0 <= T; c0 += TILE) {
-        const uint32_t t0 = c0 + threadIdx.x * IPT;
-        uint32_t cl[IPT], mv[IPT], sc = 0, sm = 0;
-#pragma unroll
-        for (int j = 0; j < IPT; ++j) {
-            cl[j] = mv[j] = 0;
-            const uint32_t t = t0 + j;
-            if (t < T) {
-                const uint32_t L = V.tL[t], ni = V.tins[t], nd = V.tdel[t];
-                if (ni | nd) {
-                    const uint32_t nl = L + ni - nd;
-                    cl[j] = chunk_round(nl);
-                    mv[j] = L;
-                    d_nnz += (long long)nl - (long long)L;
-                    d_garb += V.tcap[t];
-                    hub |= (nl > DCSR_HUB_LEN) | (L > DCSR_HUB_LEN);
-                }
-            }
-            sc += cl[j];
-            sm += mv[j];
-        }
-        uint32_t tc, tm;
-        uint32_t pc = claim_tot + block_excl_scan_256(sc, S.lds4, &tc);
-        uint32_t pm = moved_tot + block_excl_scan_256(sm, S.lds4, &tm);
-#pragma unroll
-        for (int j = 0; j < IPT; ++j) {
-            if (t0 + j <= T) {
-                V.tpre[2 * (t0 + j)] = pc;
-                V.tpre[2 * (t0 + j) + 1] = pm;
-            }
-            pc += cl[j];
-            pm += mv[j];
-        }
-        claim_tot += tc;
-        moved_tot += tm;
-    }
-    // totals of the group; claims from the bump pointer (the host's bound
-    // guarantees they fit)
-    n_ins = block_sum_i64(n_ins, S.acc, 0);
-    n_add = block_sum_i64(n_add, S.acc, 1);
-    n_del = block_sum_i64(n_del, S.acc, 2);
-    d_nnz = block_sum_i64(d_nnz, S.acc, 3);
-    d_garb = block_sum_i64(d_garb, S.acc, 4);
-    const long long hubs = block_sum_i64(hub, S.acc, 5);
-    if (threadIdx.x == 0) {
-        S.base = claim_tot ? atomicAdd(&ds->bump, (unsigned long long)claim_tot) : 0ull;
-        if (n_ins) atomicAdd((unsigned long long *)&ds->n_ins, (unsigned long long)n_ins);
-        if (n_add) atomicAdd((unsigned long long *)&ds->n_add, (unsigned long long)n_add);
-        if (n_del) atomicAdd((unsigned long long *)&ds->n_del, (unsigned long long)n_del);
-        if (d_nnz) atomicAdd((unsigned long long *)&ds->nnz, (unsigned long long)d_nnz);
-        if (d_garb) atomicAdd((unsigned long long *)&ds->garbage, (unsigned long long)d_garb);
-        atomicAdd(&ds->ntouched, T);
-        atomicAdd(&ds->nuniq, U);
-        atomicAdd(&ds->claim_total, claim_tot);
-        atomicAdd(&ds->nrewrite_old, moved_tot);
-        if (hubs) atomicOr(&ds->hub_touch, 1u);
-    }
-    __syncthreads();
-    const unsigned long long base = S.base;
-
-    // move the old entries of rewritten rows to their merged places, AP_MV
-    // entries per thread with their loads in flight together
-    for (uint32_t e0 = 0; e0 < moved_tot; e0 += TB * AP_MV) {
-        uint32_t tq[AP_MV], kq[AP_MV];
-        int32_t cq[AP_MV];
-        float vq[AP_MV];
-#pragma unroll
-        for (int q = 0; q < AP_MV; ++q) {
-            const uint32_t e = e0 + q * TB + threadIdx.x;
-            tq[q] = kq[q] = 0;
-            cq[q] = 0;
-            vq[q] = 0.0f;
-            if (e < moved_tot) {
-                // last row t with tpre[2t + 1] <= e
-                uint32_t lo = 0, hi = T;
-                while (hi - lo > 1) {
-                    const uint32_t mid = (lo + hi) >> 1;
-                    if (V.tpre[2 * mid + 1] <= e) lo = mid; else hi = mid;
-                }
-                tq[q] = lo;
-                kq[q] = e - V.tpre[2 * lo + 1];
-                const uint64_t sp = (uint64_t)V.tst[lo] + kq[q];
-                cq[q] = pcol[sp];
-                vq[q] = pval[sp];
-            }
-        }
-#pragma unroll
-        for (int q = 0; q < AP_MV; ++q) {
-            if (e0 + q * TB + threadIdx.x >= moved_tot) continue;
-            const uint32_t t = tq[q], k = kq[q];
-            const int32_t c = cq[q];
-            float v = vq[q];
-            const uint32_t ub = V.tbeg[t], ue = V.tbeg[t + 1];
-            uint32_t a = ub, z = ue;
-            while (a < z) {
-                const uint32_t mid = (a + z) >> 1;
-                if (V.rcol[mid] < c) a = mid + 1; else z = mid;
-            }
-            if (a < ue && V.rcol[a] == c) {
-                if (V.rcode[a] == C_DEL) continue;
-                v = V.rval[a];
-            }
-            const int64_t shift = (int64_t)(V.rpre[2 * a] - V.rpre[2 * ub]) -
-                                  (int64_t)(V.rpre[2 * a + 1] - V.rpre[2 * ub + 1]);
-            const uint64_t d = base + V.tpre[2 * t] + (uint64_t)((int64_t)k + shift);
-            pcol[d] = c;
-            pval[d] = v;
-        }
-    }
-    // inserted entries, and values in place for rows without inserts or
-    // deletes
-    for (uint32_t u = threadIdx.x; u < U; u += TB) {
-        const uint32_t code = V.rcode[u];
-        if (code != C_INS && code != C_ADD) continue;
-        const uint32_t t = V.rt[u];
-        const bool rewrite = (V.tins[t] | V.tdel[t]) != 0u;
-        if (!rewrite) {
-            pval[(uint64_t)V.tst[t] + V.rlo[u]] = V.rval[u];
-            continue;
-        }
-        if (code != C_INS) continue;  // moved with the old entries
-        const uint32_t ub = V.tbeg[t];
-        const int64_t shift = (int64_t)(V.rpre[2 * u] - V.rpre[2 * ub]) - (int64_t)(V.rpre[2 * u + 1] - V.rpre[2 * ub + 1]);
-        const uint64_t d = base + V.tpre[2 * t] + (uint64_t)((int64_t)V.rlo[u] + shift);
-        pcol[d] = V.rcol[u];
-        pval[d] = V.rval[u];
-    }
-    // row table
-    for (uint32_t t = threadIdx.x; t < T; t += TB) {
-        if (!(V.tins[t] | V.tdel[t])) continue;
-        const uint32_t r = V.tr[t];
-        const uint32_t nl = V.tL[t] + V.tins[t] - V.tdel[t];
-        row_start[r] = (uint32_t)(base + V.tpre[2 * t]);
-        row_len[r] = (int32_t)nl;
-        row_cap[r] = (int32_t)chunk_round(nl);
-    }
-}
-
-// Decision of a fast-path batch (one thread, before the apply): a bad index
-// rejects it (with `report`, as a sticky status bit); otherwise the apply's
-// counters start from zero.
-__global__ void k_upd_fast_begin(DevState *ds, uint32_t report)
-{
-    const unsigned bad = ds->ubad;
-    ds->ubad = 0;
-    ds->ubad_last = bad;
-    ds->go = bad ? 0u : 1u;
-    ds->bump_saved = ds->bump;
-    ds->n_ins = ds->n_add = ds->n_del = 0;
-    ds->d_nnz = ds->d_garbage = 0;
-    ds->d_hubs = ds->d_hub_chunks = 0;
-    ds->need = 0;
-    ds->ntouched = ds->nuniq = ds->claim_total = ds->nrewrite_old = 0;
-    ds->hub_touch = 0;
-    if (bad && report) ds->status |= ST_BAD_INDEX;
-}
-
 __global__ void k_upd_validate_begin(DevState *ds) { ds->ubad = 0; }
 
 
@@ -1390,7 +981,10 @@ static inline unsigned ptiles(int64_t n) { return (unsigned)((n + 1 + TILE - 1) 
 // the mean group near GS_TARGET pairs (groups are sorted in LDS up to
 // GS_CAP pairs; power-law batches drawn by degree peak near 3x the mean).
 constexpr int64_t GS_MAX_N = 1 << 21;
-constexpr int64_t GS_TARGET = 512;
+#ifndef DCSR_GS_TARGET
+#define DCSR_GS_TARGET 512
+#endif
+constexpr int64_t GS_TARGET = DCSR_GS_TARGET;
 void sort_plan(dcsr_handle h, int64_t n, int &lb, int &ndig)
 {
     lb = 0;
@@ -1415,11 +1009,15 @@ struct BufSrc {
     uint8_t *base;
     size_t cap, used;
     bool measure;
+    hipStream_t s;  // orders the zero fill of grown control-word slots
 };
 
 dcsr_status take(BufSrc &src, WsSlot slot, size_t bytes, void **out)
 {
-    if (!src.base && !src.measure) return ws_get(src.h, slot, bytes, out);
+    if (!src.base && !src.measure) {
+        if (slot == WS_SCAN || slot == WS_TOLDS) return ws_get_zeroed(src.h, slot, bytes, out, src.s);
+        return ws_get(src.h, slot, bytes, out);
+    }
     const size_t at = (src.used + 255) & ~size_t(255);
     src.used = at + bytes;
     if (src.measure) {
@@ -1434,9 +1032,7 @@ dcsr_status take(BufSrc &src, WsSlot slot, size_t bytes, void **out)
 dcsr_status upd_bufs_src(BufSrc &src, int64_t n, UpdBufs &b)
 {
     dcsr_handle h = src.h;
-    // one extra entry per row group in the arrays the fast path slices by
-    // group (a sentinel after each group's records and rows)
-    const size_t n4 = (size_t)(n + 2 + GS_MAX_DIGITS) * 4, n8 = (size_t)(n + 2 + GS_MAX_DIGITS) * 8;
+    const size_t n4 = (size_t)(n + 2) * 4, n8 = (size_t)(n + 2) * 8;
     const size_t nt = ptiles(n);
     sort_plan(h, n, b.lb, b.ndig);
     size_t hw = radix_hist_words(n);
@@ -1471,9 +1067,9 @@ dcsr_status upd_bufs_src(BufSrc &src, int64_t n, UpdBufs &b)
     return DCSR_OK;
 }
 
-dcsr_status upd_bufs(dcsr_handle h, int64_t n, UpdBufs &b)
+dcsr_status upd_bufs(dcsr_handle h, int64_t n, UpdBufs &b, hipStream_t s)
 {
-    BufSrc src{h, nullptr, 0, 0, false};
+    BufSrc src{h, nullptr, 0, 0, false, s};
     return upd_bufs_src(src, n, b);
 }
 
@@ -1573,21 +1169,6 @@ dcsr_status pipeline_launch(dcsr_handle h, int64_t n, UpdBufs &b, bool report, h
     return commit_launch(h, b, n, h->pool_cap, report, s);
 }
 
-// The fast path of a packed batch: row-group sort, then one apply launch.
-// Only for batches sorted by row groups whose claims the host has bounded
-// below the free pool (claims_fit): nothing can be rejected but a bad index.
-dcsr_status fast_launch(dcsr_handle h, int64_t n, UpdBufs &b, bool report, hipStream_t s)
-{
-    DCSR_HIP(group_sort_pairs(b.k0, b.v0, b.k1, b.v1, n, b.lb, b.ndig, b.hist, s));
-    k_upd_fast_begin<<<1, 1, 0, s>>>(h->ds, report ? 1u : 0u);
-    const int groups = b.ndig - 1;
-    k_upd_apply<<<(unsigned)groups, TB, 0, s>>>(h->ds, b.k0, b.v0, gs_group_base(b.hist, n, b.ndig), groups, h->cbits,
-                                                b.lb - h->cbits, b, h->row_start, h->row_len, h->row_cap, h->pool_col,
-                                                h->pool_val);
-    DCSR_HIP(hipGetLastError());
-    h->flat_dirty = true;
-    return DCSR_OK;
-}
 
 }  // namespace
 
@@ -1621,7 +1202,7 @@ dcsr_status update_prepare(dcsr_handle h, const int32_t *rows, const int32_t *co
         return DCSR_OK;
     }
     UpdBufs b;
-    DCSR_TRY(upd_bufs(h, n, b));
+    DCSR_TRY(upd_bufs(h, n, b, s));
     DCSR_TRY(prepare_launch(h, rows, cols, vals, n, b, s));
     DCSR_TRY(pull_state(h, s));
     if (h->hs->ubad_last) return DCSR_ERR_BAD_INDEX;
@@ -1636,7 +1217,7 @@ dcsr_status update_prepare(dcsr_handle h, const int32_t *rows, const int32_t *co
 dcsr_status update_replan(dcsr_handle h, int64_t n, hipStream_t s, bool *go)
 {
     UpdBufs b;
-    DCSR_TRY(upd_bufs(h, n, b));
+    DCSR_TRY(upd_bufs(h, n, b, s));
     DCSR_TRY(plan_and_check(h, b, n, s));
     DCSR_TRY(pull_state(h, s));
     *go = h->hs->go != 0;
@@ -1674,7 +1255,7 @@ bool claims_fit(dcsr_handle h, int64_t n) { return bump_bound(h) + claims_bound(
 dcsr_status update_commit(dcsr_handle h, int64_t n, hipStream_t s, bool nosync)
 {
     UpdBufs b;
-    DCSR_TRY(upd_bufs(h, n, b));
+    DCSR_TRY(upd_bufs(h, n, b, s));
     DCSR_TRY(commit_launch(h, b, n, nosync ? h->pool_cap : (int64_t)h->hs->nrewrite_old, nosync, s));
     if (nosync) {
         state_unknown(h, n, claims_bound(h, n));
@@ -1705,13 +1286,10 @@ static dcsr_status update_sync(dcsr_handle h, const int32_t *rows, const int32_t
         // prepare and commit back to back, one state read at the end (it
         // then holds the committed counters: write folds the deltas in)
         UpdBufs b;
-        DCSR_TRY(upd_bufs(h, n, b));
+        DCSR_TRY(upd_bufs(h, n, b, s));
         launch_pack(h, rows, cols, vals, n, b, s);
         DCSR_HIP(hipGetLastError());
-        if (b.lb > 0 && !g_no_fast)
-            DCSR_TRY(fast_launch(h, n, b, false, s));
-        else
-            DCSR_TRY(pipeline_launch(h, n, b, false, s));
+        DCSR_TRY(pipeline_launch(h, n, b, false, s));
         DCSR_TRY(pull_state(h, s));
         if (h->hs->ubad_last) return DCSR_ERR_BAD_INDEX;
         if (h->hs->go) {
@@ -1755,15 +1333,11 @@ static dcsr_status update_nosync(dcsr_handle h, const int32_t *rows, const int32
         }
     }
     const int64_t claimed = claims_bound(h, n);
-    const bool fits = claims_fit(h, n);
     UpdBufs b;
-    DCSR_TRY(upd_bufs(h, n, b));
+    DCSR_TRY(upd_bufs(h, n, b, s));
     launch_pack(h, rows, cols, vals, n, b, s);
     DCSR_HIP(hipGetLastError());
-    if (fits && b.lb > 0 && !g_no_fast)
-        DCSR_TRY(fast_launch(h, n, b, true, s));
-    else
-        DCSR_TRY(pipeline_launch(h, n, b, true, s));
+    DCSR_TRY(pipeline_launch(h, n, b, true, s));
     state_unknown(h, n, claimed);
     return snapshot_state(h, s);
 }
@@ -1778,7 +1352,7 @@ __global__ void k_zero_u32(uint32_t *p, int n)
 // Bytes of scratch an update of n triples on h carves from a fixed region.
 size_t update_scratch_bytes(dcsr_handle h, int64_t n)
 {
-    BufSrc src{h, nullptr, 0, 0, true};
+    BufSrc src{h, nullptr, 0, 0, true, nullptr};
     UpdBufs b;
     if (upd_bufs_src(src, n, b) != DCSR_OK) return ~size_t(0);
     return src.used;
@@ -1796,19 +1370,15 @@ dcsr_status update_fixed_scratch(dcsr_handle h, const int32_t *rows, const int32
     if (!h->canonical) return DCSR_ERR_INVALID;
     h->last_stream = s;
     if (n == 0) return DCSR_OK;
-    BufSrc src{h, base, cap, 0, false};
+    BufSrc src{h, base, cap, 0, false, s};
     UpdBufs b;
     DCSR_TRY(upd_bufs_src(src, n, b));
     k_zero_u32<<<1, 64, 0, s>>>(b.fin, 16);
     const bool nosync = (flags & DCSR_UPDATE_NOSYNC) != 0;
     if (nosync) refresh_bounds(h);
-    const bool fits = claims_fit(h, n);
     launch_pack(h, rows, cols, vals, n, b, s);
     DCSR_HIP(hipGetLastError());
-    if (fits && b.lb > 0 && !g_no_fast)
-        DCSR_TRY(fast_launch(h, n, b, nosync, s));
-    else
-        DCSR_TRY(pipeline_launch(h, n, b, nosync, s));
+    DCSR_TRY(pipeline_launch(h, n, b, nosync, s));
     if (nosync) {
         state_unknown(h, n, claims_bound(h, n));
         return snapshot_state(h, s);
@@ -1819,8 +1389,6 @@ dcsr_status update_fixed_scratch(dcsr_handle h, const int32_t *rows, const int32
     note_batch(h, *h->hs);
     return DCSR_OK;
 }
-
-extern "C" void dcsr_set_update_fast_path(int32_t enable) { g_no_fast = enable == 0; }
 
 dcsr_status dcsr_update(dcsr_handle h, const int32_t *rows, const int32_t *cols, const float *vals, int64_t n,
                         uint32_t flags, dcsr_stream_t stream)

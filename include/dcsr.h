@@ -375,12 +375,6 @@ dcsr_status dcsr_update_noalloc(dcsr_handle h, const int32_t *rows, const int32_
                                 int64_t n, uint32_t flags, dcsr_stream_t stream);
 int64_t dcsr_update_noalloc_max_triples(void);
 
-/* Process-wide switch of dcsr_update's one-pass path (default on): batches
- * of up to 2^21 triples whose chunk claims the host can bound below the free
- * pool are sorted by row groups and applied by one workgroup per group;
- * 0 sends every batch down the multi-launch pipeline instead (results are
- * identical; used to test both).  No reference counterpart. */
-void dcsr_set_update_fast_path(int32_t enable);
 
 /* Plain CSR SpMV comparator, y = A x.  Replaces device::spmv(const csr&, x, y)
  * -> spmv_csr_vector_kernel (include/spmv.h:310-332, include/spmv.inl:708-781). */

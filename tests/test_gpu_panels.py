@@ -48,9 +48,12 @@ def test_paneled_spmv_matches_oracle(paneled, R, C, seed):
     m.LoadMatrix(r, c, v)
     x = rng.uniform(-1, 1, C).astype(np.float32)
     dx = torch.from_numpy(x).cuda()
-    for call in range(3):  # first call: direct; later calls: paneled copy
+    # a fresh matrix is assumed to be reused: the first SpMV runs the direct
+    # kernel, the second builds the paneled copy and runs over it
+    for call in range(3):
         y = m.spmv(dx).cpu().numpy()
         fp64_spmv_check(A, x, y)
+        assert m.info()["spmv_paneled"] == (1 if call >= 1 else 0), call
     y0 = rng.uniform(-1, 1, R).astype(np.float32)
     dy = torch.from_numpy(y0.copy()).cuda()
     m.spmv(dx, dy, alpha=0.5, beta=-2.0)
@@ -61,7 +64,10 @@ def test_paneled_spmv_matches_oracle(paneled, R, C, seed):
     br, bc, bv = W.batch_mixed(rng, r, c, R, C, 4000, signed=True)
     m.UpdateMatrix(br, bc, bv)
     A = O.update(A, br, bc, bv)
+    # the structure before saw 4 SpMVs, so the copy is rebuilt on the
+    # second call after the update
     for call in range(3):
         y = m.spmv(dx).cpu().numpy()
         fp64_spmv_check(A, x, y)
+        assert m.info()["spmv_paneled"] == (1 if call >= 1 else 0), call
     m.close()

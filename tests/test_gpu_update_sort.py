@@ -155,40 +155,34 @@ def test_bad_index_rejects_grouped_batch(torch):
     m.close()
 
 
-@pytest.mark.parametrize("fast", [1, 0])
-def test_one_pass_and_pipeline_paths_agree(torch, fast):
-    """The same chain (duplicates, exact cancellations, hub rows, a group
-    past the LDS record capacity, NOSYNC calls) with the one-pass apply on
-    and off: both equal the oracle after every batch."""
-    import dcsr_amd
+def test_degree_drawn_chain_with_hot_row_and_cancellations(torch):
+    """A chain of degree-drawn batches with exact cancellations and one hot
+    row per batch (5,000 triples: its group sorts out of global memory),
+    synchronous and NOSYNC calls alternating: the oracle after every batch."""
     from dcsr_amd import DcsrMatrix, UPDATE_NOSYNC
     R, C = 30_000, 30_000
-    say(f"one-pass path {'on' if fast else 'off'}")
-    dcsr_amd.set_update_fast_path(fast)
-    try:
-        r, c, v = W.powerlaw(R, 300_000, max_degree=5000, seed=61, signed=True)
-        rng = np.random.default_rng(62)
-        m = DcsrMatrix(R, C, pool_factor=6.0)
-        m.LoadMatrix(r, c, v)
-        A = O.coo_to_csr(R, C, r, c, v)
-        for k in range(4):
-            br, bc, bv = W.batch_mixed(rng, r, c, R, C, 80_000, signed=True)
-            rows_of = np.repeat(np.arange(R), np.diff(A.off))
-            pick = rng.integers(0, A.nnz, 3000)
-            hot = np.full(5000, 17 + k, np.int32)  # one row far past a group's LDS records
-            br = np.concatenate([br, rows_of[pick], hot]).astype(np.int32)
-            bc = np.concatenate([bc, A.col[pick], rng.integers(0, C, 5000)]).astype(np.int32)
-            bv = np.concatenate([bv, -A.val[pick], rng.uniform(-1, 1, 5000)]).astype(np.float32)
-            perm = rng.permutation(br.size)
-            br, bc, bv = br[perm], bc[perm], bv[perm]
-            m.UpdateMatrix(br, bc, bv, flags=UPDATE_NOSYNC if k % 2 else 0)
-            A = O.update(A, br, bc, bv)
-            assert_same_structure(to_host_csr(m), A)
-            info = check_layout(m)
-            assert info["nnz"] == A.nnz
-            x = rng.uniform(-1, 1, C).astype(np.float32)
-            fp64_spmv_check(A, x, spmv_host(m, x))
-        check_healthy(m)
-        m.close()
-    finally:
-        dcsr_amd.set_update_fast_path(1)
+    say("degree-drawn chain with a hot row")
+    r, c, v = W.powerlaw(R, 300_000, max_degree=5000, seed=61, signed=True)
+    rng = np.random.default_rng(62)
+    m = DcsrMatrix(R, C, pool_factor=6.0)
+    m.LoadMatrix(r, c, v)
+    A = O.coo_to_csr(R, C, r, c, v)
+    for k in range(4):
+        br, bc, bv = W.batch_mixed(rng, r, c, R, C, 80_000, signed=True)
+        rows_of = np.repeat(np.arange(R), np.diff(A.off))
+        pick = rng.integers(0, A.nnz, 3000)
+        hot = np.full(5000, 17 + k, np.int32)
+        br = np.concatenate([br, rows_of[pick], hot]).astype(np.int32)
+        bc = np.concatenate([bc, A.col[pick], rng.integers(0, C, 5000)]).astype(np.int32)
+        bv = np.concatenate([bv, -A.val[pick], rng.uniform(-1, 1, 5000)]).astype(np.float32)
+        perm = rng.permutation(br.size)
+        br, bc, bv = br[perm], bc[perm], bv[perm]
+        m.UpdateMatrix(br, bc, bv, flags=UPDATE_NOSYNC if k % 2 else 0)
+        A = O.update(A, br, bc, bv)
+        assert_same_structure(to_host_csr(m), A)
+        info = check_layout(m)
+        assert info["nnz"] == A.nnz
+        x = rng.uniform(-1, 1, C).astype(np.float32)
+        fp64_spmv_check(A, x, spmv_host(m, x))
+    check_healthy(m)
+    m.close()
