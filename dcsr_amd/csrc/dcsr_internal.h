@@ -50,6 +50,7 @@ struct DevState {
     unsigned int ubad;              // current batch holds an out-of-range index
     unsigned int hub_touch;         // current update: rewritten rows that were/are hubs
     unsigned int ubad_last;         // current update's batch held an out-of-range index
+    unsigned int ptotal;            // current update: probe lengths of the touched rows, summed
 };
 
 // Grow-only device scratch buffers owned by a handle.
@@ -63,6 +64,7 @@ enum WsSlot {
     WS_SG_HPRE, WS_SG_EA, WS_SG_APOS, WS_SG_KEY0, WS_SG_KEY1, WS_SG_VAL0, WS_SG_VAL1, WS_SG_FLAG, WS_SG_EPOS,
     WS_CANON, WS_TMP0, WS_TMP1, WS_TMP2,
     WS_PNL_OFF, WS_PNL_BND, WS_PNL_COL, WS_PNL_VAL, WS_PNL_MAP, WS_PNL_PART,
+    WS_TRS, WS_TLEN,
     WS_COUNT
 };
 

@@ -139,8 +139,9 @@ constexpr int IPT = 8;
 constexpr int TILE = TB * IPT;
 // batch tile table: heads / row heads (count, offset), claims / moved (sum, offset)
 enum { TT_H = 0, TT_R, TT_HX, TT_RX, TT_C, TT_M, TT_CX, TT_MX, TT_WORDS };
-// probe tile table: inserts, adds, deletes; insert / delete offsets
-enum { TP_I = 0, TP_A, TP_D, TP_IX, TP_DX, TP_WORDS = 8 };
+// probe tile table: inserts, adds, deletes; insert / delete offsets; probe
+// lengths of the tile's touched rows (sum, offset)
+enum { TP_I = 0, TP_A, TP_D, TP_IX, TP_DX, TP_L, TP_LX, TP_WORDS = 8 };
 // finish counters (self-resetting)
 enum { FC_HEADS = 0, FC_PLAN = 1 };
 
@@ -237,41 +238,68 @@ __device__ __forceinline__ HeadFlags head_flags(const uint64_t (&k)[IPT], uint64
     return f;
 }
 
-// Pass 1 of the heads: per-tile counts.  With `table` the last workgroup
-// makes them offsets and stores nuniq / ntouched; without it (batches of at
-// most HEADS_DIRECT_TILES tiles) pass 2 sums the counts before each tile
-// itself, and this kernel ends at the counts.
+// Probe length of a touched row: its entries, at least 1 (a row without
+// entries still owns one position, where its records are probed).
+__device__ __forceinline__ uint32_t probe_len(int32_t len) { return len > 1 ? (uint32_t)len : 1u; }
+
+// Sum of the probe lengths of the row heads among the thread's items, and
+// their row lengths (one round trip: every row length is loaded at once).
+__device__ __forceinline__ uint32_t head_rows_len(const uint64_t (&k)[IPT], uint32_t rflags, int cbits,
+                                                  const int32_t *__restrict__ row_len, int32_t (&lj)[IPT])
+{
+    uint32_t sum = 0;
+#pragma unroll
+    for (int j = 0; j < IPT; ++j) {
+        lj[j] = 0;
+        if ((rflags >> j) & 1u) lj[j] = row_len[(uint32_t)(k[j] >> cbits)];
+    }
+#pragma unroll
+    for (int j = 0; j < IPT; ++j) sum += (rflags >> j) & 1u ? probe_len(lj[j]) : 0u;
+    return sum;
+}
+
+// Pass 1 of the heads: per-tile counts (coordinates, touched rows, and the
+// rows' probe lengths).  With `table` the last workgroup makes them offsets
+// and stores nuniq / ntouched / ptotal; without it (batches of at most
+// HEADS_DIRECT_TILES tiles) pass 2 sums the counts before each tile itself,
+// and this kernel ends at the counts.
 constexpr int64_t HEADS_DIRECT_TILES = 4 * TB;
 __global__ __launch_bounds__(TB) void k_upd_heads_count(const uint64_t *__restrict__ key, int64_t n, int cbits,
-                                                        uint32_t *__restrict__ tt, unsigned int *fin, DevState *ds,
-                                                        bool table)
+                                                        const int32_t *__restrict__ row_len,
+                                                        uint32_t *__restrict__ tt, uint32_t *__restrict__ tp,
+                                                        unsigned int *fin, DevState *ds, bool table)
 {
     __shared__ uint32_t lds4[4], s_flag;
     const int64_t tile = blockIdx.x;
     uint64_t k[IPT], prev;
     load_keys(key, n, tile * TILE + (int64_t)threadIdx.x * IPT, k, prev);
     const HeadFlags f = head_flags(k, prev, cbits);
+    int32_t lj[IPT];
+    const uint32_t ls = head_rows_len(k, f.r, cbits, row_len, lj);
     const uint32_t hc = block_sum_u32(__popc(f.h), lds4);
     const uint32_t rc = block_sum_u32(__popc(f.r), lds4);
+    const uint32_t lc = block_sum_u32(ls, lds4);
     if (threadIdx.x == 0) {
         tt[tile * TT_WORDS + TT_H] = hc;
         tt[tile * TT_WORDS + TT_R] = rc;
+        tp[tile * TP_WORDS + TP_L] = lc;
     }
     if (!table) return;
     if (last_block(&fin[FC_HEADS], &s_flag)) {
-        uint32_t *const tabs[2] = {tt, tt};
-        uint32_t tot[2];
-        block_scan_columns<2>(tabs, {TT_H, TT_R}, {TT_HX, TT_RX}, gridDim.x, lds4, tot);
+        uint32_t *const tabs[3] = {tt, tt, tp};
+        uint32_t tot[3];
+        block_scan_columns<3>(tabs, {TT_H, TT_R, TP_L}, {TT_HX, TT_RX, TP_LX}, gridDim.x, lds4, tot);
         if (threadIdx.x == 0) {
             ds->nuniq = tot[0];
             ds->ntouched = tot[1];
+            ds->ptotal = tot[2];
         }
     }
 }
 
 // Pass 2 of the heads: every head writes its compacted record: ucol/usum/
-// ut/urow for coordinate u, trow/tubeg (and zeroed tins/tdel) for touched
-// row t.  Sums run left to right from 0 in batch order, as the host path
+// ut for coordinate u, trow/tubeg/tlpre (probe-length offset), the row's
+// pool start and length (trs, tlen) and zeroed tins/tdel for touched row t.  Sums run left to right from 0 in batch order, as the host path
 // accumulates B_row[j]; a run is summed from the thread's registers and
 // only a run that continues past the thread's items reads memory.  A tile's
 // records are contiguous, so they are staged in LDS and written out
@@ -281,29 +309,34 @@ __global__ __launch_bounds__(TB) void k_upd_heads_emit(const uint64_t *__restric
                                                        const uint32_t *__restrict__ val, int64_t n, int cbits,
                                                        const uint32_t *__restrict__ tt, DevState *ds, bool table,
                                                        int32_t *__restrict__ ucol, float *__restrict__ usum,
-                                                       uint32_t *__restrict__ ut, uint32_t *__restrict__ urow,
-                                                       uint32_t *__restrict__ trow, uint32_t *__restrict__ tubeg,
+                                                       uint32_t *__restrict__ ut, uint32_t *__restrict__ trow,
+                                                       uint32_t *__restrict__ tubeg, const int32_t *__restrict__ row_len,
+                                                       const uint32_t *__restrict__ row_start,
+                                                       const uint32_t *__restrict__ tp, uint32_t *__restrict__ tlpre,
+                                                       uint32_t *__restrict__ trs, uint32_t *__restrict__ tlen,
                                                        uint32_t *__restrict__ tins, uint32_t *__restrict__ tdel)
 {
     __shared__ uint32_t lds4[4];
     __shared__ int32_t s_ucol[TILE];
     __shared__ float s_usum[TILE];
-    __shared__ uint32_t s_ut[TILE], s_urow[TILE], s_trow[TILE], s_tub[TILE];
+    __shared__ uint32_t s_ut[TILE], s_trow[TILE], s_tub[TILE], s_tl[TILE];
     const int64_t tile = blockIdx.x;
     const int64_t i0 = tile * TILE + (int64_t)threadIdx.x * IPT;
     // counts of the tiles before this one (loaded with the keys): each
     // thread sums up to four of them (index clamped to tile 0, masked)
-    uint32_t ch[HEADS_DIRECT_TILES / TB], cr[HEADS_DIRECT_TILES / TB];
-    uint32_t ubase = 0, tbase = 0;
+    uint32_t ch[HEADS_DIRECT_TILES / TB], cr[HEADS_DIRECT_TILES / TB], cl[HEADS_DIRECT_TILES / TB];
+    uint32_t ubase = 0, tbase = 0, lbase = 0;
     if (table) {
         ubase = tt[tile * TT_WORDS + TT_HX];
         tbase = tt[tile * TT_WORDS + TT_RX];
+        lbase = tp[tile * TP_WORDS + TP_LX];
     } else {
 #pragma unroll
         for (int q = 0; q < HEADS_DIRECT_TILES / TB; ++q) {
             const int64_t pt = (int64_t)threadIdx.x + q * TB;
             ch[q] = tt[(pt < tile ? pt : 0) * TT_WORDS + TT_H];
             cr[q] = tt[(pt < tile ? pt : 0) * TT_WORDS + TT_R];
+            cl[q] = tp[(pt < tile ? pt : 0) * TP_WORDS + TP_L];
         }
     }
     uint64_t k[IPT], prev;
@@ -312,20 +345,28 @@ __global__ __launch_bounds__(TB) void k_upd_heads_emit(const uint64_t *__restric
 #pragma unroll
     for (int j = 0; j < IPT; ++j) vb[j] = val[i0 + j < n ? i0 + j : n - 1];
     const HeadFlags f = head_flags(k, prev, cbits);
+    int32_t lj[IPT];
+    uint32_t rsj[IPT];
+#pragma unroll
+    for (int j = 0; j < IPT; ++j) rsj[j] = (f.r >> j) & 1u ? row_start[(uint32_t)(k[j] >> cbits)] : 0u;
+    const uint32_t ls = head_rows_len(k, f.r, cbits, row_len, lj);
     if (!table) {
-        uint32_t sh = 0, sr = 0;
+        uint32_t sh = 0, sr = 0, sl = 0;
 #pragma unroll
         for (int q = 0; q < HEADS_DIRECT_TILES / TB; ++q) {
             const bool in = (int64_t)threadIdx.x + q * TB < tile;
             sh += in ? ch[q] : 0u;
             sr += in ? cr[q] : 0u;
+            sl += in ? cl[q] : 0u;
         }
         ubase = block_sum_u32(sh, lds4);
         tbase = block_sum_u32(sr, lds4);
+        lbase = block_sum_u32(sl, lds4);
     }
-    uint32_t htot, rtot;
+    uint32_t htot, rtot, ltot;
     uint32_t lu = block_excl_scan_256(__popc(f.h), lds4, &htot);
     uint32_t lt = block_excl_scan_256(__popc(f.r), lds4, &rtot);
+    uint32_t ll = block_excl_scan_256(ls, lds4, &ltot) + lbase;
 #pragma unroll
     for (int j = 0; j < IPT; ++j) {
         if (!((f.h >> j) & 1u)) continue;
@@ -342,13 +383,16 @@ __global__ __launch_bounds__(TB) void k_upd_heads_emit(const uint64_t *__restric
         const uint32_t r = (uint32_t)(k[j] >> cbits);
         s_ucol[lu] = (int32_t)(k[j] & ((1ull << cbits) - 1ull));
         s_usum[lu] = s;
-        s_urow[lu] = r;
         // a coordinate that is not a row head belongs to the last row head
         // before it (possibly in an earlier tile)
         s_ut[lu] = tbase + lt - (rh ? 0u : 1u);
         if (rh) {
             s_trow[lt] = r;
             s_tub[lt] = ubase + lu;
+            s_tl[lt] = ll;
+            ll += probe_len(lj[j]);
+            trs[tbase + lt] = rsj[j];
+            tlen[tbase + lt] = (uint32_t)lj[j];
             ++lt;
         }
         ++lu;
@@ -358,143 +402,234 @@ __global__ __launch_bounds__(TB) void k_upd_heads_emit(const uint64_t *__restric
         ucol[ubase + i] = s_ucol[i];
         usum[ubase + i] = s_usum[i];
         ut[ubase + i] = s_ut[i];
-        urow[ubase + i] = s_urow[i];
     }
     for (uint32_t i = threadIdx.x; i < rtot; i += TB) {
         trow[tbase + i] = s_trow[i];
         tubeg[tbase + i] = s_tub[i];
+        tlpre[tbase + i] = s_tl[i];
         tins[tbase + i] = 0;
         tdel[tbase + i] = 0;
     }
     if (threadIdx.x == 0) {
         if (table) {
-            if (tile == 0) tubeg[ds->ntouched] = ds->nuniq;
+            if (tile == 0) {
+                tubeg[ds->ntouched] = ds->nuniq;
+                tlpre[ds->ntouched] = ds->ptotal;
+            }
         } else if (tile == gridDim.x - 1) {
             ds->nuniq = ubase + htot;
             ds->ntouched = tbase + rtot;
+            ds->ptotal = lbase + ltot;
             tubeg[tbase + rtot] = ubase + htot;
+            tlpre[tbase + rtot] = lbase + ltot;
         }
     }
 }
 
-// Probe every distinct coordinate in its row: existing -> old + sum (delete
-// when it cancels to 0.0), missing -> insert when the sum is not 0.0.  The
-// eight binary searches of a thread advance in lock step so their loads
-// overlap.  Per-row insert / delete counts are added once per run of equal
-// rows among the thread's coordinates; per-tile counts go to the probe
-// table.
+// Probe every distinct coordinate in its (column-sorted) row: existing ->
+// old + sum (delete when it cancels to 0.0), missing -> insert when the sum
+// is not 0.0.  The touched rows, each of probe length max(entries, 1), lie
+// end to end in a flat probe space (offsets tlpre, from the heads); each
+// wave owns PR_SPAN positions of it.  It stages their columns in LDS with
+// every load in flight at once, then each coordinate of the rows it covers
+// finds its lower bound there.  A row cut by a span's edge gives each span
+// the coordinates whose lower bound lies in it, told apart by comparing
+// with the columns at the cut.  So a row is read once, coalesced, and the
+// work per wave does not depend on the row-length distribution.  Per-row
+// insert / delete counts are summed in LDS and added once per row and span;
+// the per-tile code counts are made by the plan.
+constexpr uint32_t PR_SPAN = 2048;
+constexpr uint32_t PR_PER = PR_SPAN / 64;  // staged columns per lane
+constexpr int PR_RU = 4;                   // record rounds of 64 in flight
+struct ProbeLds {
+    int32_t col[PR_SPAN];
+    uint32_t a[65];    // probe offset of window row j (~0 past the last row)
+    uint32_t ub[65];   // first record of window row j (~0 past the last row)
+    uint32_t rs[64];   // pool start of row j
+    uint32_t len[64];  // entries of row j
+    uint32_t ni[64], nd[64];
+};
+
+// Code and value of record u, whose lower bound in its row (pool start rs)
+// is pos; `match`: the entry there holds the record's column.
+__device__ __forceinline__ void probe_finish(uint32_t u, uint32_t pos, bool match, uint32_t rs, float s,
+                                             const float *__restrict__ pval, float *__restrict__ uval,
+                                             uint32_t *__restrict__ upos, uint32_t *__restrict__ ucode,
+                                             uint32_t *ni, uint32_t *nd)
+{
+    uint32_t code;
+    float nv;
+    if (match) {
+        nv = pval[(uint64_t)rs + pos] + s;
+        code = (nv != 0.0f) ? C_ADD : C_DEL;
+    } else {
+        nv = s;
+        code = (s != 0.0f) ? C_INS : C_NOP;
+    }
+    uval[u] = nv;
+    upos[u] = pos;
+    ucode[u] = code;
+    if (code == C_INS) atomicAdd(ni, 1u);
+    else if (code == C_DEL) atomicAdd(nd, 1u);
+}
+
+// First index in [lo, hi) whose a[] is > key (UPPER) or >= key (!UPPER),
+// for a[] non-decreasing on [lo, hi) (hi when there is none).  By the whole
+// wave, 16 loads per lane: one memory round trip when hi - lo <= 1024, one
+// more per 1024-fold narrowing.  All lanes must call it; every lane gets the
+// result.
+template <bool UPPER, class T>
+__device__ __forceinline__ uint32_t wave_bound(const T *__restrict__ a, uint32_t lo, uint32_t hi, T key)
+{
+    const uint32_t lane = (uint32_t)lane_id();
+    while (hi - lo > 1024u) {
+        const uint32_t step = (hi - lo + 1023u) / 1024u;
+        uint32_t c = 0;
+#pragma unroll
+        for (uint32_t i = 0; i < 16u; ++i) {
+            const uint32_t idx = lo + (lane * 16u + i) * step;
+            c += (idx < hi && (UPPER ? a[idx] <= key : a[idx] < key)) ? 1u : 0u;
+        }
+        const uint32_t k = wave_sum_u32(c);  // probes below the bound: a prefix
+        if (k == 0) return lo;
+        const uint32_t nh = lo + k * step;
+        lo = lo + (k - 1u) * step + 1u;
+        hi = nh < hi ? nh : hi;
+    }
+    uint32_t c = 0;
+#pragma unroll
+    for (uint32_t i = 0; i < 16u; ++i) {
+        const uint32_t idx = lo + lane * 16u + i;
+        c += (idx < hi && (UPPER ? a[idx] <= key : a[idx] < key)) ? 1u : 0u;
+    }
+    return lo + wave_sum_u32(c);
+}
+
 __global__ __launch_bounds__(TB) void k_upd_probe(const DevState *__restrict__ ds,
                                                   const uint32_t *__restrict__ row_start,
                                                   const int32_t *__restrict__ row_len,
                                                   const int32_t *__restrict__ pcol, const float *__restrict__ pval,
+                                                  const uint32_t *__restrict__ tubeg, const uint32_t *__restrict__ tlpre,
+                                                  const uint32_t *__restrict__ trs, const uint32_t *__restrict__ tlen,
                                                   const int32_t *__restrict__ ucol, const float *__restrict__ usum,
-                                                  const uint32_t *__restrict__ ut, const uint32_t *__restrict__ urow,
                                                   float *__restrict__ uval, uint32_t *__restrict__ upos,
                                                   uint32_t *__restrict__ ucode, uint32_t *__restrict__ tins,
-                                                  uint32_t *__restrict__ tdel, uint32_t *__restrict__ tp)
+                                                  uint32_t *__restrict__ tdel)
 {
-    __shared__ uint32_t lds4[4];
-    const int64_t tile = blockIdx.x;
-    const int64_t u0 = tile * TILE + (int64_t)threadIdx.x * IPT;
-    const uint32_t nu = ds->nuniq;
-    uint64_t rs[IPT];
-    int32_t lo[IPT], hi[IPT], cc[IPT], L[IPT];
-    uint32_t tt[IPT], rr[IPT];
-    float us[IPT];
-#pragma unroll
-    for (int j = 0; j < IPT; ++j) {
-        lo[j] = hi[j] = L[j] = cc[j] = 0;
-        tt[j] = rr[j] = 0;
-        rs[j] = 0;
-        us[j] = 0.0f;
-        if (u0 + j < nu) {
-            tt[j] = ut[u0 + j];
-            cc[j] = ucol[u0 + j];
-            rr[j] = urow[u0 + j];
-            us[j] = usum[u0 + j];
-        }
-    }
-#pragma unroll
-    for (int j = 0; j < IPT; ++j) {
-        if (u0 + j < nu) {
-            const uint32_t r = rr[j];
-            rs[j] = row_start[r];
-            L[j] = row_len[r];
-            hi[j] = L[j];
-        }
-    }
-    // lower bound of cc[j] in its row
-    for (;;) {
-        bool any = false;
-        int32_t pc[IPT];
-#pragma unroll
-        for (int j = 0; j < IPT; ++j) {
-            pc[j] = 0;
-            if (lo[j] < hi[j]) {
-                pc[j] = pcol[rs[j] + (uint32_t)((lo[j] + hi[j]) >> 1)];
-                any = true;
+    __shared__ ProbeLds sh[TB / 64];
+    ProbeLds &S = sh[threadIdx.x >> 6];
+    const uint32_t lane = (uint32_t)lane_id();
+    const uint32_t nt = ds->ntouched;
+    const uint32_t total = ds->ptotal;
+    const uint64_t nwaves = (uint64_t)gridDim.x * (TB / 64);
+    for (uint64_t wave = (uint64_t)blockIdx.x * (TB / 64) + (threadIdx.x >> 6); wave * PR_SPAN < total;
+         wave += nwaves) {
+        const uint32_t e0 = (uint32_t)(wave * PR_SPAN);
+        const uint32_t e1 = total - e0 < PR_SPAN ? total : e0 + PR_SPAN;
+        // every probe length is >= 1, so exactly one row holds e0
+        uint32_t j = wave_find_last_le(tlpre, nt, e0);
+        uint32_t e = e0;
+        while (e < e1) {
+            // rows j .. j+63 (nt >= 1: the span is non-empty)
+            const uint32_t t = j + lane;
+            const uint32_t tq = t <= nt ? t : nt;
+            const uint32_t tc = t < nt ? t : nt - 1;
+            const uint32_t t64 = j + 64u <= nt ? j + 64u : nt;
+            const uint32_t a_t = tlpre[tq], a_64 = tlpre[t64];
+            const uint32_t ub = tubeg[tq], ub_64 = tubeg[t64];
+            const uint32_t rs = trs[tc];
+            const uint32_t L = tlen[tc];
+            S.a[lane] = t <= nt ? a_t : 0xffffffffu;
+            S.ub[lane] = t <= nt ? ub : 0xffffffffu;
+            if (lane == 0) {
+                S.a[64] = j + 64u <= nt ? a_64 : 0xffffffffu;
+                S.ub[64] = j + 64u <= nt ? ub_64 : 0xffffffffu;
             }
-        }
-        if (!any) break;
+            S.rs[lane] = rs;
+            S.len[lane] = t < nt ? L : 0u;
+            S.ni[lane] = 0;
+            S.nd[lane] = 0;
+            __builtin_amdgcn_wave_barrier();
+            // positions [e, wend): at least one (row j holds e)
+            const uint32_t wend = S.a[64] < e1 ? S.a[64] : e1;
+            const uint32_t cnt = wend - e;
+            const int jA = find_seg64(S.a, e), jB = find_seg64(S.a, wend - 1u);
+            const uint32_t kA = e - S.a[jA];
+            int32_t before = 0;  // the column before the cut of row jA (a cut row has >= 2 entries)
+            {
+                if (kA > 0u) before = pcol[(uint64_t)S.rs[jA] + kA - 1u];
+                int32_t cv[PR_PER];
 #pragma unroll
-        for (int j = 0; j < IPT; ++j) {
-            if (lo[j] < hi[j]) {
-                const int32_t mid = (lo[j] + hi[j]) >> 1;
-                if (pc[j] < cc[j]) lo[j] = mid + 1; else hi[j] = mid;
+                for (uint32_t q = 0; q < PR_PER; ++q) {
+                    const uint32_t p = q * 64u + lane;
+                    cv[q] = 0x7fffffff;
+                    if (p < cnt) {
+                        const int jr = find_seg64(S.a, e + p);
+                        const uint32_t k = e + p - S.a[jr];
+                        if (k < S.len[jr]) cv[q] = pcol[(uint64_t)S.rs[jr] + k];
+                    }
+                }
+#pragma unroll
+                for (uint32_t q = 0; q < PR_PER; ++q) {
+                    const uint32_t p = q * 64u + lane;
+                    if (p < cnt) S.col[p] = cv[q];
+                }
             }
-        }
-    }
-    int32_t hc[IPT];
-    float hv[IPT];
+            // coordinates [rA, rB): of the row cut by e, those above its
+            // column before the cut; of the row cut by wend, those up to its
+            // last staged column (a row cut at both ends: both)
+            uint32_t rA = S.ub[jA];
+            if (kA > 0u) rA = wave_bound<true, int32_t>(ucol, rA, S.ub[jA + 1], before);
+            uint32_t rB = S.ub[jB + 1];
+            __builtin_amdgcn_wave_barrier();
+            if (S.a[jB + 1] > wend) {
+                const int32_t lastc = S.col[cnt - 1u];
+                rB = wave_bound<true, int32_t>(ucol, S.ub[jB] > rA ? S.ub[jB] : rA, rB, lastc);
+            }
+            for (uint32_t ur = rA; ur < rB; ur += 64u * PR_RU) {
+                int32_t c[PR_RU];
+                float sv[PR_RU];
 #pragma unroll
-    for (int j = 0; j < IPT; ++j) {
-        hc[j] = -1;
-        hv[j] = 0.0f;
-        if (u0 + j < nu && lo[j] < L[j]) {
-            hc[j] = pcol[rs[j] + lo[j]];
-            hv[j] = pval[rs[j] + lo[j]];
-        }
-    }
-    uint32_t ni = 0, nd = 0, na = 0;
-    uint32_t run_t = 0, run_i = 0, run_d = 0;
+                for (int q = 0; q < PR_RU; ++q) {
+                    const uint32_t u = ur + (uint32_t)q * 64u + lane;
+                    c[q] = 0;
+                    sv[q] = 0.0f;
+                    if (u < rB) {
+                        c[q] = ucol[u];
+                        sv[q] = usum[u];
+                    }
+                }
 #pragma unroll
-    for (int j = 0; j < IPT; ++j) {
-        const int64_t u = u0 + j;
-        if (u >= nu) continue;
-        if (tt[j] != run_t) {
-            if (run_i) atomicAdd(&tins[run_t], run_i);
-            if (run_d) atomicAdd(&tdel[run_t], run_d);
-            run_t = tt[j];
-            run_i = run_d = 0;
+                for (int q = 0; q < PR_RU; ++q) {
+                    const uint32_t u = ur + (uint32_t)q * 64u + lane;
+                    if (u >= rB) continue;
+                    const int jr = find_seg64(S.ub, u);
+                    // the row's staged part: its entries [k0, k1) at p0
+                    const uint32_t ar = S.a[jr];
+                    const uint32_t k0 = ar < e ? e - ar : 0u;
+                    const uint32_t lend = ar + S.len[jr];
+                    const uint32_t k1 = (lend < wend ? lend : wend) - ar;
+                    const uint32_t p0 = ar + k0 - e;
+                    uint32_t lo = 0, hi = k1 > k0 ? k1 - k0 : 0u;
+                    const uint32_t n = hi;
+                    while (lo < hi) {
+                        const uint32_t m = (lo + hi) >> 1;
+                        if (S.col[p0 + m] < c[q]) lo = m + 1; else hi = m;
+                    }
+                    const bool match = lo < n && S.col[p0 + lo] == c[q];
+                    probe_finish(u, k0 + lo, match, S.rs[jr], sv[q], pval, uval, upos, ucode, &S.ni[jr],
+                                 &S.nd[jr]);
+                }
+            }
+            __builtin_amdgcn_wave_barrier();
+            if (t < nt) {
+                if (S.ni[lane]) atomicAdd(&tins[t], S.ni[lane]);
+                if (S.nd[lane]) atomicAdd(&tdel[t], S.nd[lane]);
+            }
+            __builtin_amdgcn_wave_barrier();
+            e = wend;
+            j += 64;
         }
-        uint32_t code;
-        float nv;
-        const float s = us[j];
-        if (hc[j] == cc[j]) {
-            nv = hv[j] + s;
-            code = (nv != 0.0f) ? C_ADD : C_DEL;
-        } else {
-            nv = s;
-            code = (s != 0.0f) ? C_INS : C_NOP;
-        }
-        uval[u] = nv;
-        upos[u] = (uint32_t)lo[j];
-        ucode[u] = code;
-        run_i += code == C_INS;
-        run_d += code == C_DEL;
-        ni += code == C_INS;
-        nd += code == C_DEL;
-        na += code == C_ADD;
-    }
-    if (run_i) atomicAdd(&tins[run_t], run_i);
-    if (run_d) atomicAdd(&tdel[run_t], run_d);
-    ni = block_sum_u32(ni, lds4);
-    nd = block_sum_u32(nd, lds4);
-    na = block_sum_u32(na, lds4);
-    if (threadIdx.x == 0) {
-        tp[tile * TP_WORDS + TP_I] = ni;
-        tp[tile * TP_WORDS + TP_A] = na;
-        tp[tile * TP_WORDS + TP_D] = nd;
     }
 }
 
@@ -554,13 +689,15 @@ __device__ void plan_totals_decide(DevState *ds, unsigned long long pool_cap, co
 
 // Per touched row: new length, mode (values in place / rewrite), chunk
 // claim and old entries to move, per-tile sums to the batch tile table and per-tile counters to
-// pplan.  The last workgroup to finish then decides the batch: it makes the
+// pplan; per tile of coordinates, the probe's insert / add / delete counts
+// to the probe table.  The last workgroup to finish then decides the batch: it makes the
 // claim / moved / insert / delete columns exclusive offsets, totals every
 // counter, rejects the batch (no-op) on a bad index or when the claims do
 // not fit the pool, and otherwise advances the bump pointer.
 __global__ __launch_bounds__(TB) void k_upd_plan(DevState *ds, unsigned long long pool_cap,
                                                  const uint32_t *__restrict__ trow, const uint32_t *__restrict__ tins,
-                                                 const uint32_t *__restrict__ tdel, const int32_t *__restrict__ row_len,
+                                                 const uint32_t *__restrict__ tdel, const uint32_t *__restrict__ ucode,
+                                                 const int32_t *__restrict__ row_len,
                                                  const int32_t *__restrict__ row_cap, uint32_t *__restrict__ tnewl,
                                                  uint32_t *__restrict__ tmode, uint32_t *__restrict__ tmoved,
                                                  uint32_t *__restrict__ tt,
@@ -571,15 +708,16 @@ __global__ __launch_bounds__(TB) void k_upd_plan(DevState *ds, unsigned long lon
     __shared__ uint32_t lds4[4], s_flag;
     const int64_t tile = blockIdx.x;
     const int64_t t0 = tile * TILE + (int64_t)threadIdx.x * IPT;
-    const uint32_t nt = ds->ntouched;
+    const uint32_t nt = ds->ntouched, nu = ds->nuniq;
     uint32_t csum = 0, msum = 0;
     long long v[NPLAN] = {0, 0, 0, 0, 0};
-    // loads in two phases (row ids and counts, then row lengths), so each
-    // phase is one memory round trip for all eight rows
-    uint32_t rr[IPT], ti[IPT], td[IPT], rl[IPT], rc[IPT];
+    // loads in two phases (row ids, counts and the tile's codes, then row
+    // lengths), so each phase is one memory round trip for all eight rows
+    uint32_t rr[IPT], ti[IPT], td[IPT], rl[IPT], rc[IPT], cd[IPT];
 #pragma unroll
     for (int j = 0; j < IPT; ++j) {
         rr[j] = ti[j] = td[j] = 0;
+        cd[j] = t0 + j < nu ? ucode[t0 + j] : (uint32_t)C_NOP;
         if (t0 + j < nt) {
             rr[j] = trow[t0 + j];
             ti[j] = tins[t0 + j];
@@ -620,6 +758,15 @@ __global__ __launch_bounds__(TB) void k_upd_plan(DevState *ds, unsigned long lon
     }
     csum = block_sum_u32(csum, lds4);
     msum = block_sum_u32(msum, lds4);
+    // inserts | adds << 16 (each at most TILE), deletes
+    uint32_t cia = 0, cdl = 0;
+#pragma unroll
+    for (int j = 0; j < IPT; ++j) {
+        cia += (cd[j] == C_INS ? 1u : 0u) + (cd[j] == C_ADD ? 0x10000u : 0u);
+        cdl += cd[j] == C_DEL ? 1u : 0u;
+    }
+    cia = block_sum_u32(cia, lds4);
+    cdl = block_sum_u32(cdl, lds4);
 #pragma unroll
     for (int i = 0; i < NPLAN; ++i) {
         const long long w = wave_sum_i64(v[i]);
@@ -633,6 +780,9 @@ __global__ __launch_bounds__(TB) void k_upd_plan(DevState *ds, unsigned long lon
     if (threadIdx.x == 0) {
         tt[tile * TT_WORDS + TT_C] = csum;
         tt[tile * TT_WORDS + TT_M] = msum;
+        tp[tile * TP_WORDS + TP_I] = cia & 0xffffu;
+        tp[tile * TP_WORDS + TP_A] = cia >> 16;
+        tp[tile * TP_WORDS + TP_D] = cdl;
     }
     // without a table scan (at most HEADS_DIRECT_TILES tiles) the offsets
     // pass sums the counts before each tile and decides the batch
@@ -758,133 +908,180 @@ __global__ __launch_bounds__(TB) void k_upd_offsets(DevState *__restrict__ ds,
 
 // Move surviving old entries of rewritten rows to their merged positions.
 // tpre[2t] = claim offset of row t (relative to bump_saved), tpre[2t+1] =
-// offset of row t's old entries in the flat move space.  Each wave owns a
-// fixed range of that space; per-row constants of a 64-row window are staged
-// in LDS and two 64-entry chunks are in flight per iteration.  An old entry
-// finds its column among its row's update records by binary search; when the
-// records of the rows the wave touches in a window number at most MV_REC,
-// their columns are staged in LDS first, so the search makes no memory round
-// trips.
-constexpr int MV_ILP = 2;
-constexpr uint32_t MV_REC = 1024;
+// offset of row t's old entries in the flat move space (rows written in
+// place hold none).  Each wave owns MV_SPAN entries of that space, taken in
+// chunks of at most 64 rows.  The loads of a chunk's entries are issued
+// first; while they are in flight, the update records of the chunk's rows
+// mark the entries they shift (an insert +1 on the entry it goes before, a
+// delete a tombstone on its entry and -1 on the row's next one, an add its
+// record index) and a per-row scan of the marks, in 64-wide rounds, gives
+// every entry its shift.  So the copy is a coalesced stream with no
+// per-entry search.  A row cut by the chunk's start begins from the shift
+// of its records before the cut, located by a search on their positions.
+constexpr uint32_t MV_SPAN = 1024;
+constexpr uint32_t MV_PER = MV_SPAN / 64;  // entries per lane
+constexpr uint32_t MV_TOMB = 0xffffffffu;
+constexpr int MV_RU = 4;                   // record rounds of 64 in flight
+struct MoveLds {
+    int32_t mk[MV_SPAN];   // marks
+    uint32_t ov[MV_SPAN];  // 0, MV_TOMB or 1 + the index of the add record
+    uint32_t off[65];      // move-space start of window row j (~0 past the last row)
+    uint32_t ub[65];       // first record of window row j (~0 past the last row)
+    uint32_t src[64];      // old pool start of row j
+    uint32_t dst[64];      // claim offset of row j
+};
+
 __global__ __launch_bounds__(TB) void k_upd_move(const DevState *__restrict__ ds, const uint32_t *__restrict__ tpre,
                                                  const uint32_t *__restrict__ trow,
                                                  const uint32_t *__restrict__ row_start,
-                                                 const uint32_t *__restrict__ tubeg, const int32_t *__restrict__ ucol,
-                                                 const float *__restrict__ uval, const uint32_t *__restrict__ ucode,
+                                                 const uint32_t *__restrict__ tubeg, const float *__restrict__ uval,
+                                                 const uint32_t *__restrict__ ucode, const uint32_t *__restrict__ upos,
                                                  const uint32_t *__restrict__ upre, int32_t *__restrict__ pcol,
-                                                 float *__restrict__ pval, uint32_t per_wave)
+                                                 float *__restrict__ pval)
 {
-    __shared__ uint32_t s_off[TB / 64][65];
-    __shared__ uint32_t s_src[TB / 64][64];
-    __shared__ uint32_t s_ub[TB / 64][65];
-    __shared__ uint32_t s_dst[TB / 64][64];
-    __shared__ uint32_t s_ib[TB / 64][64];
-    __shared__ uint32_t s_db[TB / 64][64];
-    __shared__ int32_t s_uc[TB / 64][MV_REC];
+    __shared__ MoveLds sh[TB / 64];
     if (!ds->go) return;
+    MoveLds &S = sh[threadIdx.x >> 6];
     const uint32_t total = ds->nrewrite_old;
     const uint32_t nt = ds->ntouched;
     const uint64_t base = ds->bump_saved;
-    const int w = threadIdx.x >> 6;
-    const int lane = lane_id();
+    const uint32_t lane = (uint32_t)lane_id();
     const uint64_t nwaves = (uint64_t)gridDim.x * (TB / 64);
-    for (uint64_t wave = (uint64_t)blockIdx.x * (TB / 64) + w; wave * per_wave < total; wave += nwaves) {
-        const uint32_t e0 = (uint32_t)(wave * per_wave);
-        const uint32_t e1 = total - e0 < per_wave ? total : e0 + per_wave;
+    for (uint64_t wave = (uint64_t)blockIdx.x * (TB / 64) + (threadIdx.x >> 6); wave * MV_SPAN < total;
+         wave += nwaves) {
+        const uint32_t e0 = (uint32_t)(wave * MV_SPAN);
+        const uint32_t e1 = total - e0 < MV_SPAN ? total : e0 + MV_SPAN;
         uint32_t j = wave_find_last_le<2>(tpre + 1, nt, e0);
         uint32_t e = e0;
         while (e < e1) {
-            // the window's per-row constants in two load phases (independent
-            // loads, then the row starts and record prefixes they select);
-            // indices clamp into the tables (nt >= 1 here: the range is
-            // non-empty), lanes past the window's rows keep the sentinels
+            // rows j .. j+63: constants in two load phases (nt >= 1 here:
+            // the range is non-empty); lanes past the last row keep ~0
             const uint32_t t = j + lane;
-            const uint32_t tq = t <= nt ? t : nt;          // tables of nt + 1 entries
-            const uint32_t tc = t < nt ? t : nt - 1;       // tables of nt entries
-            const uint32_t t64 = j + 64 <= nt ? j + 64 : nt;
+            const uint32_t tq = t <= nt ? t : nt;     // tables of nt + 1 entries
+            const uint32_t tc = t < nt ? t : nt - 1;  // tables of nt entries
+            const uint32_t t64 = j + 64u <= nt ? j + 64u : nt;
             const uint32_t off_t = tpre[2 * (uint64_t)tq + 1];
             const uint32_t off_64 = tpre[2 * (uint64_t)t64 + 1];
             const uint32_t ub = tubeg[tq];
-            const uint32_t ub1 = tubeg[t + 1 <= nt ? t + 1 : nt];
-            const uint32_t tr = trow[tc];
+            const uint32_t ub_64 = tubeg[t64];
             const uint32_t dst = tpre[2 * (uint64_t)tc];
-            const uint32_t src = row_start[tr];
-            const uint32_t ib = upre[2 * (uint64_t)ub], db = upre[2 * (uint64_t)ub + 1];
-            s_off[w][lane] = t <= nt ? off_t : 0xffffffffu;
-            if (lane == 0) s_off[w][64] = (j + 64 <= nt) ? off_64 : 0xffffffffu;
-            if (t < nt) {
-                s_src[w][lane] = src;
-                s_ub[w][lane] = ub;
-                s_dst[w][lane] = dst;
-                s_ib[w][lane] = ib;
-                s_db[w][lane] = db;
-                if (lane == 63) s_ub[w][64] = ub1;
-            } else if (t == nt) {
-                s_ub[w][lane] = ub;
+            const uint32_t src = row_start[trow[tc]];
+            S.off[lane] = t <= nt ? off_t : 0xffffffffu;
+            S.ub[lane] = t <= nt ? ub : 0xffffffffu;
+            if (lane == 0) {
+                S.off[64] = j + 64u <= nt ? off_64 : 0xffffffffu;
+                S.ub[64] = j + 64u <= nt ? ub_64 : 0xffffffffu;
+            }
+            S.src[lane] = src;
+            S.dst[lane] = dst;
+            __builtin_amdgcn_wave_barrier();
+            const uint32_t wend = S.off[64] < e1 ? S.off[64] : e1;
+            const uint32_t cnt = wend - e;
+            if (cnt == 0u) {  // 64 rows without old entries to move
+                __builtin_amdgcn_wave_barrier();
+                j += 64;
+                continue;
+            }
+            // the chunk's entries: row, destination before the shift, and
+            // the loads
+            int32_t c[MV_PER];
+            float v[MV_PER];
+            uint32_t jq[MV_PER], dq[MV_PER];
+#pragma unroll
+            for (uint32_t q = 0; q < MV_PER; ++q) {
+                const uint32_t p = q * 64u + lane;
+                c[q] = 0;
+                v[q] = 0.0f;
+                jq[q] = 64u;  // past the chunk: above every row
+                dq[q] = 0;
+                if (p < cnt) {
+                    jq[q] = (uint32_t)find_seg64(S.off, e + p);
+                    const uint32_t k = e + p - S.off[jq[q]];
+                    dq[q] = S.dst[jq[q]] + k;
+                    const uint64_t sp = (uint64_t)S.src[jq[q]] + k;
+                    c[q] = pcol[sp];
+                    v[q] = pval[sp];
+                }
+            }
+            for (uint32_t p = lane; p < cnt; p += 64u) {
+                S.mk[p] = 0;
+                S.ov[p] = 0u;
+            }
+            // records whose marks fall in [e, wend): from the first one at or
+            // past the cut of row jA to the last one before the cut of row jB
+            const int jA = find_seg64(S.off, e), jB = find_seg64(S.off, wend - 1u);
+            const uint32_t kA = e - S.off[jA];
+            uint32_t rA = S.ub[jA];
+            int32_t bshift = 0;
+            if (kA > 0u) {
+                const uint32_t ubA = rA;
+                rA = wave_bound<false, uint32_t>(upos, ubA, S.ub[jA + 1], kA);
+                bshift = (int32_t)((upre[2 * (uint64_t)rA] - upre[2 * (uint64_t)ubA]) -
+                                   (upre[2 * (uint64_t)rA + 1] - upre[2 * (uint64_t)ubA + 1]));
+            }
+            uint32_t rB = S.ub[jB + 1];
+            if (S.off[jB + 1] > wend)
+                rB = wave_bound<false, uint32_t>(upos, S.ub[jB] > rA ? S.ub[jB] : rA, rB, wend - S.off[jB]);
+            __builtin_amdgcn_wave_barrier();
+            for (uint32_t ur = rA; ur < rB; ur += 64u * MV_RU) {
+                uint32_t kr[MV_RU], cr[MV_RU];
+#pragma unroll
+                for (int q = 0; q < MV_RU; ++q) {
+                    const uint32_t u = ur + (uint32_t)q * 64u + lane;
+                    kr[q] = cr[q] = 0;
+                    if (u < rB) {
+                        kr[q] = upos[u];
+                        cr[q] = ucode[u];
+                    }
+                }
+#pragma unroll
+                for (int q = 0; q < MV_RU; ++q) {
+                    const uint32_t u = ur + (uint32_t)q * 64u + lane;
+                    if (u >= rB) continue;
+                    const uint32_t k = kr[q];
+                    const uint32_t code = cr[q];
+                    const int jr = find_seg64(S.ub, u);
+                    const uint32_t rlen = S.off[jr + 1] - S.off[jr];
+                    const uint32_t pa = S.off[jr] + k;
+                    // rows without old entries to move (written in place, or
+                    // new) have rlen 0; inserts past a row's end shift nothing
+                    if (k < rlen && pa >= e && pa < wend) {
+                        const uint32_t p = pa - e;
+                        if (code == C_INS) {
+                            atomicAdd(&S.mk[p], 1);
+                        } else if (code == C_DEL) {
+                            S.ov[p] = MV_TOMB;
+                            if (k + 1u < rlen && p + 1u < cnt) atomicAdd(&S.mk[p + 1u], -1);
+                        } else if (code == C_ADD) {
+                            S.ov[p] = u + 1u;
+                        }
+                    }
+                }
             }
             __builtin_amdgcn_wave_barrier();
-            uint32_t wend = s_off[w][64];
-            if (wend > e1) wend = e1;
-            // records of the rows holding entries [e, wend): [r0, r1)
-            uint32_t r0 = 0, r1 = 0;
-            if (e < wend) {
-                r0 = s_ub[w][find_seg64(s_off[w], e)];
-                r1 = s_ub[w][find_seg64(s_off[w], wend - 1) + 1];
-            }
-            const bool staged = r1 - r0 <= MV_REC;
-            if (staged) {
-                for (uint32_t i = lane; i < r1 - r0; i += 64) s_uc[w][i] = ucol[r0 + i];
-                __builtin_amdgcn_wave_barrier();
-            }
-            for (uint32_t b = e; b < wend; b += 64 * MV_ILP) {
-                uint32_t jj[MV_ILP], k[MV_ILP];
-                int32_t c[MV_ILP];
-                float v[MV_ILP];
+            // per-row inclusive scan of the marks in rounds of 64 entries,
+            // carrying the last lane's sum into the next round while the row
+            // continues; the cut row jA starts from bshift
+            int32_t carry = bshift;
+            uint32_t carry_row = (uint32_t)jA;
 #pragma unroll
-                for (int q = 0; q < MV_ILP; ++q) {
-                    const uint32_t ee = b + q * 64 + lane;
-                    jj[q] = 64;
-                    c[q] = 0;
-                    v[q] = 0.0f;
-                    if (ee < wend) {
-                        jj[q] = (uint32_t)find_seg64(s_off[w], ee);
-                        k[q] = ee - s_off[w][jj[q]];
-                        const uint64_t sp = (uint64_t)s_src[w][jj[q]] + k[q];
-                        c[q] = pcol[sp];
-                        v[q] = pval[sp];
-                    }
-                }
+            for (uint32_t q = 0; q < MV_PER; ++q) {
+                const uint32_t p = q * 64u + lane;
+                int32_t m = p < cnt ? S.mk[p] : 0;
+                if (lane == 0 && jq[q] == carry_row) m += carry;
+                const int32_t inc = (int32_t)wave_seg_incl_scan_u32((uint32_t)m, jq[q]);
+                dq[q] += (uint32_t)inc;
+                carry = (int32_t)wave_read_lane_u32((uint32_t)inc, 63);
+                carry_row = wave_read_lane_u32(jq[q], 63);
+            }
 #pragma unroll
-                for (int q = 0; q < MV_ILP; ++q) {
-                    if (jj[q] == 64) continue;
-                    const uint32_t ub = s_ub[w][jj[q]], ue = s_ub[w][jj[q] + 1];
-                    uint32_t lo = ub, hi = ue;
-                    bool match;
-                    if (staged) {
-                        while (lo < hi) {
-                            uint32_t mid = (lo + hi) >> 1;
-                            if (s_uc[w][mid - r0] < c[q]) lo = mid + 1; else hi = mid;
-                        }
-                        match = lo < ue && s_uc[w][lo - r0] == c[q];
-                    } else {
-                        while (lo < hi) {
-                            uint32_t mid = (lo + hi) >> 1;
-                            if (ucol[mid] < c[q]) lo = mid + 1; else hi = mid;
-                        }
-                        match = lo < ue && ucol[lo] == c[q];
-                    }
-                    float vv = v[q];
-                    if (match) {
-                        if (ucode[lo] == C_DEL) continue;
-                        vv = uval[lo];
-                    }
-                    const int64_t shift = (int64_t)(upre[2 * lo] - s_ib[w][jj[q]]) -
-                                          (int64_t)(upre[2 * lo + 1] - s_db[w][jj[q]]);
-                    const uint64_t d = base + s_dst[w][jj[q]] + (uint64_t)((int64_t)k[q] + shift);
-                    pcol[d] = c[q];
-                    pval[d] = vv;
-                }
+            for (uint32_t q = 0; q < MV_PER; ++q) {
+                const uint32_t p = q * 64u + lane;
+                const uint32_t o = p < cnt ? S.ov[p] : MV_TOMB;
+                if (o == MV_TOMB) continue;
+                const float vv = o ? uval[o - 1u] : v[q];
+                pcol[base + dq[q]] = c[q];
+                pval[base + dq[q]] = vv;
             }
             __builtin_amdgcn_wave_barrier();
             e = wend;
@@ -961,8 +1158,8 @@ struct UpdBufs {
     uint32_t *v0, *v1, *hist, *part;
     int32_t *ucol;
     float *usum, *uval;
-    uint32_t *ut, *urow, *upos, *ucode, *upre;
-    uint32_t *trow, *tubeg, *tins, *tdel, *tnewl, *tmode, *tmoved, *tpre;
+    uint32_t *ut, *upos, *ucode, *upre;
+    uint32_t *trow, *tubeg, *tlpre, *trs, *tlen, *tins, *tdel, *tnewl, *tmode, *tmoved, *tpre;
     uint32_t *tt, *tp, *fin;
     long long *pplan;
     // sort: row groups (lb = gshift + cbits > 0, ndig = groups + 1) or, for
@@ -1047,11 +1244,13 @@ dcsr_status upd_bufs_src(BufSrc &src, int64_t n, UpdBufs &b)
     DCSR_TRY(take(src, WS_UPOS, n4, (void **)&b.upos));
     DCSR_TRY(take(src, WS_UVAL, n4, (void **)&b.uval));
     DCSR_TRY(take(src, WS_UT, n4, (void **)&b.ut));
-    DCSR_TRY(take(src, WS_UROW, n4, (void **)&b.urow));
     DCSR_TRY(take(src, WS_UCODE, n4, (void **)&b.ucode));
     DCSR_TRY(take(src, WS_UPRE, n8, (void **)&b.upre));
     DCSR_TRY(take(src, WS_TROW, n4, (void **)&b.trow));
     DCSR_TRY(take(src, WS_TUBEG, n4 + 4, (void **)&b.tubeg));
+    DCSR_TRY(take(src, WS_UROW, n4 + 4, (void **)&b.tlpre));
+    DCSR_TRY(take(src, WS_TRS, n4, (void **)&b.trs));
+    DCSR_TRY(take(src, WS_TLEN, n4, (void **)&b.tlen));
     DCSR_TRY(take(src, WS_TINS, n4, (void **)&b.tins));
     DCSR_TRY(take(src, WS_TDEL, n4, (void **)&b.tdel));
     DCSR_TRY(take(src, WS_TNEWL, n4, (void **)&b.tnewl));
@@ -1088,10 +1287,21 @@ static void launch_offsets(dcsr_handle h, UpdBufs &b, int64_t n, hipStream_t s)
 dcsr_status plan_and_check(dcsr_handle h, UpdBufs &b, int64_t n, hipStream_t s)
 {
     const bool table = use_tables(n);
-    k_upd_plan<<<ptiles(n), TB, 0, s>>>(h->ds, (unsigned long long)h->pool_cap, b.trow, b.tins, b.tdel, h->row_len,
+    k_upd_plan<<<ptiles(n), TB, 0, s>>>(h->ds, (unsigned long long)h->pool_cap, b.trow, b.tins, b.tdel, b.ucode, h->row_len,
                                         h->row_cap, b.tnewl, b.tmode, b.tmoved, b.tt, b.tp, b.pplan, b.fin, table);
     if (!table) launch_offsets(h, b, n, s);
     return hipGetLastError() == hipSuccess ? DCSR_OK : DCSR_ERR_HIP;
+}
+
+// Probe launch: one wave per PR_SPAN positions of the probe space (at most
+// the live entries plus one per touched row), the grid capped at what the
+// device holds at once (waves stride over the spans).
+constexpr int64_t PR_BLOCKS = 1024;
+static inline unsigned probe_blocks(dcsr_handle h, int64_t n)
+{
+    const int64_t spans = (live_bound(h) + n + PR_SPAN - 1) / PR_SPAN;
+    const int64_t b = (spans + 3) / 4;
+    return (unsigned)(b < 1 ? 1 : (b > PR_BLOCKS ? PR_BLOCKS : b));
 }
 
 // Sort, heads, probe, plan (+ decision), offsets of a packed batch: the
@@ -1109,11 +1319,13 @@ dcsr_status sort_plan_launch(dcsr_handle h, int64_t n, UpdBufs &b, hipStream_t s
     const uint32_t *val = which ? b.v1 : b.v0;
     const unsigned ht = (unsigned)((n + TILE - 1) / TILE);
     const bool table = ht > HEADS_DIRECT_TILES;
-    k_upd_heads_count<<<ht, TB, 0, s>>>(key, n, h->cbits, b.tt, b.fin, h->ds, table);
-    k_upd_heads_emit<<<ht, TB, 0, s>>>(key, val, n, h->cbits, b.tt, h->ds, table, b.ucol, b.usum, b.ut, b.urow,
-                                       b.trow, b.tubeg, b.tins, b.tdel);
-    k_upd_probe<<<ptiles(n), TB, 0, s>>>(h->ds, h->row_start, h->row_len, h->pool_col, h->pool_val, b.ucol, b.usum,
-                                         b.ut, b.urow, b.uval, b.upos, b.ucode, b.tins, b.tdel, b.tp);
+    k_upd_heads_count<<<ht, TB, 0, s>>>(key, n, h->cbits, h->row_len, b.tt, b.tp, b.fin, h->ds, table);
+    k_upd_heads_emit<<<ht, TB, 0, s>>>(key, val, n, h->cbits, b.tt, h->ds, table, b.ucol, b.usum, b.ut, b.trow,
+                                       b.tubeg, h->row_len, h->row_start, b.tp, b.tlpre, b.trs, b.tlen, b.tins,
+                                       b.tdel);
+    k_upd_probe<<<probe_blocks(h, n), TB, 0, s>>>(h->ds, h->row_start, h->row_len, h->pool_col, h->pool_val,
+                                                  b.tubeg, b.tlpre, b.trs, b.tlen, b.ucol, b.usum, b.uval, b.upos,
+                                                  b.ucode, b.tins, b.tdel);
     DCSR_HIP(hipGetLastError());
     DCSR_TRY(plan_and_check(h, b, n, s));
     if (use_tables(n)) launch_offsets(h, b, n, s);
@@ -1142,17 +1354,16 @@ dcsr_status prepare_launch(dcsr_handle h, const int32_t *rows, const int32_t *co
 // Move + write launches.  move_bound bounds the old entries to move: the
 // move kernel strides over them with at most MV_BLOCKS workgroups (waves
 // past the device's count exit at once).
-constexpr int64_t MV_BLOCKS = 2048;
+constexpr int64_t MV_BLOCKS = 1024;
 dcsr_status commit_launch(dcsr_handle h, const UpdBufs &b, int64_t n, int64_t move_bound, bool report,
                           hipStream_t s)
 {
-    const uint32_t per_wave = 256;
-    const int64_t waves = (move_bound + per_wave - 1) / per_wave;
+    const int64_t waves = (move_bound + MV_SPAN - 1) / MV_SPAN;
     int64_t blocks = (waves + 3) / 4;
     if (blocks > MV_BLOCKS) blocks = MV_BLOCKS;
     if (blocks < 1) blocks = 1;
-    k_upd_move<<<(unsigned)blocks, TB, 0, s>>>(h->ds, b.tpre, b.trow, h->row_start, b.tubeg, b.ucol, b.uval, b.ucode,
-                                               b.upre, h->pool_col, h->pool_val, per_wave);
+    k_upd_move<<<(unsigned)blocks, TB, 0, s>>>(h->ds, b.tpre, b.trow, h->row_start, b.tubeg, b.uval, b.ucode, b.upos,
+                                               b.upre, h->pool_col, h->pool_val);
     k_upd_write<<<grid_for(n, TB), TB, 0, s>>>(h->ds, report ? 1u : 0u, n, b.ut, b.trow, b.tubeg, b.tpre, b.tnewl, b.tmode, b.ucol, b.uval,
                                                 b.upos, b.ucode, b.upre, h->pool_col, h->pool_val, h->row_start,
                                                 h->row_len, h->row_cap);
