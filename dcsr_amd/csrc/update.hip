@@ -555,9 +555,26 @@ __global__ __launch_bounds__(TB) void k_upd_probe(const DevState *__restrict__ d
             const uint32_t cnt = wend - e;
             const int jA = find_seg64(S.a, e), jB = find_seg64(S.a, wend - 1u);
             const uint32_t kA = e - S.a[jA];
+            // the coordinates of rows jA .. jB; when they are few, they are
+            // loaded with the columns and the cut rows' ones are told apart
+            // by column, else located by two searches
+            const uint32_t R0 = S.ub[jA], R1 = S.ub[jB + 1];
+            const bool few = R1 - R0 <= 64u * PR_RU;
+            int32_t rc[PR_RU];
+            float rsv[PR_RU];
             int32_t before = 0;  // the column before the cut of row jA (a cut row has >= 2 entries)
             {
                 if (kA > 0u) before = pcol[(uint64_t)S.rs[jA] + kA - 1u];
+#pragma unroll
+                for (int q = 0; q < PR_RU; ++q) {
+                    const uint32_t u = R0 + (uint32_t)q * 64u + lane;
+                    rc[q] = 0;
+                    rsv[q] = 0.0f;
+                    if (few && u < R1) {
+                        rc[q] = ucol[u];
+                        rsv[q] = usum[u];
+                    }
+                }
                 int32_t cv[PR_PER];
 #pragma unroll
                 for (uint32_t q = 0; q < PR_PER; ++q) {
@@ -575,50 +592,61 @@ __global__ __launch_bounds__(TB) void k_upd_probe(const DevState *__restrict__ d
                     if (p < cnt) S.col[p] = cv[q];
                 }
             }
-            // coordinates [rA, rB): of the row cut by e, those above its
-            // column before the cut; of the row cut by wend, those up to its
-            // last staged column (a row cut at both ends: both)
-            uint32_t rA = S.ub[jA];
-            if (kA > 0u) rA = wave_bound<true, int32_t>(ucol, rA, S.ub[jA + 1], before);
-            uint32_t rB = S.ub[jB + 1];
             __builtin_amdgcn_wave_barrier();
-            if (S.a[jB + 1] > wend) {
-                const int32_t lastc = S.col[cnt - 1u];
-                rB = wave_bound<true, int32_t>(ucol, S.ub[jB] > rA ? S.ub[jB] : rA, rB, lastc);
-            }
-            for (uint32_t ur = rA; ur < rB; ur += 64u * PR_RU) {
-                int32_t c[PR_RU];
-                float sv[PR_RU];
-#pragma unroll
-                for (int q = 0; q < PR_RU; ++q) {
-                    const uint32_t u = ur + (uint32_t)q * 64u + lane;
-                    c[q] = 0;
-                    sv[q] = 0.0f;
-                    if (u < rB) {
-                        c[q] = ucol[u];
-                        sv[q] = usum[u];
-                    }
+            // a coordinate of the span: its lower bound among the row's
+            // staged entries [k0, k1), at p0
+            auto resolve = [&](uint32_t u, int jr, int32_t c, float sv) {
+                const uint32_t ar = S.a[jr];
+                const uint32_t k0 = ar < e ? e - ar : 0u;
+                const uint32_t lend = ar + S.len[jr];
+                const uint32_t k1 = (lend < wend ? lend : wend) - ar;
+                const uint32_t p0 = ar + k0 - e;
+                uint32_t lo = 0, hi = k1 > k0 ? k1 - k0 : 0u;
+                const uint32_t n = hi;
+                while (lo < hi) {
+                    const uint32_t m = (lo + hi) >> 1;
+                    if (S.col[p0 + m] < c) lo = m + 1; else hi = m;
                 }
+                const bool match = lo < n && S.col[p0 + lo] == c;
+                probe_finish(u, k0 + lo, match, S.rs[jr], sv, pval, uval, upos, ucode, &S.ni[jr], &S.nd[jr]);
+            };
+            // of the row cut by e, the coordinates above its column before the
+            // cut; of the row cut by wend, those up to its last staged column
+            // (a row cut at both ends: both)
+            const bool cutB = S.a[jB + 1] > wend;
+            const int32_t lastc = cutB ? S.col[cnt - 1u] : 0;
+            if (few) {
 #pragma unroll
                 for (int q = 0; q < PR_RU; ++q) {
-                    const uint32_t u = ur + (uint32_t)q * 64u + lane;
-                    if (u >= rB) continue;
+                    const uint32_t u = R0 + (uint32_t)q * 64u + lane;
+                    if (u >= R1) continue;
                     const int jr = find_seg64(S.ub, u);
-                    // the row's staged part: its entries [k0, k1) at p0
-                    const uint32_t ar = S.a[jr];
-                    const uint32_t k0 = ar < e ? e - ar : 0u;
-                    const uint32_t lend = ar + S.len[jr];
-                    const uint32_t k1 = (lend < wend ? lend : wend) - ar;
-                    const uint32_t p0 = ar + k0 - e;
-                    uint32_t lo = 0, hi = k1 > k0 ? k1 - k0 : 0u;
-                    const uint32_t n = hi;
-                    while (lo < hi) {
-                        const uint32_t m = (lo + hi) >> 1;
-                        if (S.col[p0 + m] < c[q]) lo = m + 1; else hi = m;
+                    if (jr == jA && kA > 0u && rc[q] <= before) continue;
+                    if (jr == jB && cutB && rc[q] > lastc) continue;
+                    resolve(u, jr, rc[q], rsv[q]);
+                }
+            } else {
+                uint32_t rA = R0, rB = R1;
+                if (kA > 0u) rA = wave_bound<true, int32_t>(ucol, rA, S.ub[jA + 1], before);
+                if (cutB) rB = wave_bound<true, int32_t>(ucol, S.ub[jB] > rA ? S.ub[jB] : rA, rB, lastc);
+                for (uint32_t ur = rA; ur < rB; ur += 64u * PR_RU) {
+                    int32_t c[PR_RU];
+                    float sv[PR_RU];
+#pragma unroll
+                    for (int q = 0; q < PR_RU; ++q) {
+                        const uint32_t u = ur + (uint32_t)q * 64u + lane;
+                        c[q] = 0;
+                        sv[q] = 0.0f;
+                        if (u < rB) {
+                            c[q] = ucol[u];
+                            sv[q] = usum[u];
+                        }
                     }
-                    const bool match = lo < n && S.col[p0 + lo] == c[q];
-                    probe_finish(u, k0 + lo, match, S.rs[jr], sv[q], pval, uval, upos, ucode, &S.ni[jr],
-                                 &S.nd[jr]);
+#pragma unroll
+                    for (int q = 0; q < PR_RU; ++q) {
+                        const uint32_t u = ur + (uint32_t)q * 64u + lane;
+                        if (u < rB) resolve(u, find_seg64(S.ub, u), c[q], sv[q]);
+                    }
                 }
             }
             __builtin_amdgcn_wave_barrier();
@@ -698,7 +726,9 @@ __global__ __launch_bounds__(TB) void k_upd_plan(DevState *ds, unsigned long lon
                                                  const uint32_t *__restrict__ trow, const uint32_t *__restrict__ tins,
                                                  const uint32_t *__restrict__ tdel, const uint32_t *__restrict__ ucode,
                                                  const int32_t *__restrict__ row_len,
-                                                 const int32_t *__restrict__ row_cap, uint32_t *__restrict__ tnewl,
+                                                 const int32_t *__restrict__ row_cap,
+                                                 const uint32_t *__restrict__ row_start, uint32_t *__restrict__ trs,
+                                                 uint32_t *__restrict__ tnewl,
                                                  uint32_t *__restrict__ tmode, uint32_t *__restrict__ tmoved,
                                                  uint32_t *__restrict__ tt,
                                                  uint32_t *__restrict__ tp, long long *__restrict__ pplan,
@@ -724,12 +754,14 @@ __global__ __launch_bounds__(TB) void k_upd_plan(DevState *ds, unsigned long lon
             td[j] = tdel[t0 + j];
         }
     }
+    uint32_t rsv[IPT];
 #pragma unroll
     for (int j = 0; j < IPT; ++j) {
-        rl[j] = rc[j] = 0;
+        rl[j] = rc[j] = rsv[j] = 0;
         if (t0 + j < nt) {
             rl[j] = (uint32_t)row_len[rr[j]];
             rc[j] = (uint32_t)row_cap[rr[j]];
+            rsv[j] = row_start[rr[j]];
         }
     }
 #pragma unroll
@@ -754,6 +786,7 @@ __global__ __launch_bounds__(TB) void k_upd_plan(DevState *ds, unsigned long lon
         }
         tnewl[t] = nl;
         tmode[t] = mode;
+        trs[t] = rsv[j];
         tmoved[t] = mode == M_REWRITE ? L : 0u;
     }
     csum = block_sum_u32(csum, lds4);
@@ -932,8 +965,7 @@ struct MoveLds {
 };
 
 __global__ __launch_bounds__(TB) void k_upd_move(const DevState *__restrict__ ds, const uint32_t *__restrict__ tpre,
-                                                 const uint32_t *__restrict__ trow,
-                                                 const uint32_t *__restrict__ row_start,
+                                                 const uint32_t *__restrict__ trs,
                                                  const uint32_t *__restrict__ tubeg, const float *__restrict__ uval,
                                                  const uint32_t *__restrict__ ucode, const uint32_t *__restrict__ upos,
                                                  const uint32_t *__restrict__ upre, int32_t *__restrict__ pcol,
@@ -954,8 +986,8 @@ __global__ __launch_bounds__(TB) void k_upd_move(const DevState *__restrict__ ds
         uint32_t j = wave_find_last_le<2>(tpre + 1, nt, e0);
         uint32_t e = e0;
         while (e < e1) {
-            // rows j .. j+63: constants in two load phases (nt >= 1 here:
-            // the range is non-empty); lanes past the last row keep ~0
+            // rows j .. j+63 (nt >= 1 here: the range is non-empty); lanes
+            // past the last row keep ~0
             const uint32_t t = j + lane;
             const uint32_t tq = t <= nt ? t : nt;     // tables of nt + 1 entries
             const uint32_t tc = t < nt ? t : nt - 1;  // tables of nt entries
@@ -965,7 +997,7 @@ __global__ __launch_bounds__(TB) void k_upd_move(const DevState *__restrict__ ds
             const uint32_t ub = tubeg[tq];
             const uint32_t ub_64 = tubeg[t64];
             const uint32_t dst = tpre[2 * (uint64_t)tc];
-            const uint32_t src = row_start[trow[tc]];
+            const uint32_t src = trs[tc];
             S.off[lane] = t <= nt ? off_t : 0xffffffffu;
             S.ub[lane] = t <= nt ? ub : 0xffffffffu;
             if (lane == 0) {
@@ -1003,58 +1035,88 @@ __global__ __launch_bounds__(TB) void k_upd_move(const DevState *__restrict__ ds
                     v[q] = pval[sp];
                 }
             }
+            // the records of rows jA .. jB; when they are few, they are
+            // loaded with the entries and filtered by position, else the
+            // cut rows' ones are located by searches
+            const int jA = find_seg64(S.off, e), jB = find_seg64(S.off, wend - 1u);
+            const uint32_t kA = e - S.off[jA];
+            const uint32_t R0 = S.ub[jA], R1 = S.ub[jB + 1];
+            const bool few = R1 - R0 <= 64u * MV_RU;
+            uint32_t kr[MV_RU], cr[MV_RU];
+#pragma unroll
+            for (int q = 0; q < MV_RU; ++q) {
+                const uint32_t u = R0 + (uint32_t)q * 64u + lane;
+                kr[q] = cr[q] = 0;
+                if (few && u < R1) {
+                    kr[q] = upos[u];
+                    cr[q] = ucode[u];
+                }
+            }
             for (uint32_t p = lane; p < cnt; p += 64u) {
                 S.mk[p] = 0;
                 S.ov[p] = 0u;
             }
-            // records whose marks fall in [e, wend): from the first one at or
-            // past the cut of row jA to the last one before the cut of row jB
-            const int jA = find_seg64(S.off, e), jB = find_seg64(S.off, wend - 1u);
-            const uint32_t kA = e - S.off[jA];
-            uint32_t rA = S.ub[jA];
-            int32_t bshift = 0;
-            if (kA > 0u) {
-                const uint32_t ubA = rA;
-                rA = wave_bound<false, uint32_t>(upos, ubA, S.ub[jA + 1], kA);
-                bshift = (int32_t)((upre[2 * (uint64_t)rA] - upre[2 * (uint64_t)ubA]) -
-                                   (upre[2 * (uint64_t)rA + 1] - upre[2 * (uint64_t)ubA + 1]));
-            }
-            uint32_t rB = S.ub[jB + 1];
-            if (S.off[jB + 1] > wend)
-                rB = wave_bound<false, uint32_t>(upos, S.ub[jB] > rA ? S.ub[jB] : rA, rB, wend - S.off[jB]);
             __builtin_amdgcn_wave_barrier();
-            for (uint32_t ur = rA; ur < rB; ur += 64u * MV_RU) {
-                uint32_t kr[MV_RU], cr[MV_RU];
-#pragma unroll
-                for (int q = 0; q < MV_RU; ++q) {
-                    const uint32_t u = ur + (uint32_t)q * 64u + lane;
-                    kr[q] = cr[q] = 0;
-                    if (u < rB) {
-                        kr[q] = upos[u];
-                        cr[q] = ucode[u];
+            // a record whose mark falls in [e, wend)
+            auto mark = [&](uint32_t u, uint32_t k, uint32_t code) {
+                const int jr = find_seg64(S.ub, u);
+                const uint32_t rlen = S.off[jr + 1] - S.off[jr];
+                const uint32_t pa = S.off[jr] + k;
+                // rows without old entries to move (written in place, or
+                // new) have rlen 0; inserts past a row's end shift nothing
+                if (k < rlen && pa >= e && pa < wend) {
+                    const uint32_t p = pa - e;
+                    if (code == C_INS) {
+                        atomicAdd(&S.mk[p], 1);
+                    } else if (code == C_DEL) {
+                        S.ov[p] = MV_TOMB;
+                        if (k + 1u < rlen && p + 1u < cnt) atomicAdd(&S.mk[p + 1u], -1);
+                    } else if (code == C_ADD) {
+                        S.ov[p] = u + 1u;
                     }
                 }
+            };
+            // the cut row jA starts from the shift of its records before the cut
+            int32_t bshift = 0;
+            if (few) {
+                int32_t bs = 0;
 #pragma unroll
                 for (int q = 0; q < MV_RU; ++q) {
-                    const uint32_t u = ur + (uint32_t)q * 64u + lane;
-                    if (u >= rB) continue;
-                    const uint32_t k = kr[q];
-                    const uint32_t code = cr[q];
-                    const int jr = find_seg64(S.ub, u);
-                    const uint32_t rlen = S.off[jr + 1] - S.off[jr];
-                    const uint32_t pa = S.off[jr] + k;
-                    // rows without old entries to move (written in place, or
-                    // new) have rlen 0; inserts past a row's end shift nothing
-                    if (k < rlen && pa >= e && pa < wend) {
-                        const uint32_t p = pa - e;
-                        if (code == C_INS) {
-                            atomicAdd(&S.mk[p], 1);
-                        } else if (code == C_DEL) {
-                            S.ov[p] = MV_TOMB;
-                            if (k + 1u < rlen && p + 1u < cnt) atomicAdd(&S.mk[p + 1u], -1);
-                        } else if (code == C_ADD) {
-                            S.ov[p] = u + 1u;
+                    const uint32_t u = R0 + (uint32_t)q * 64u + lane;
+                    if (u >= R1 || kr[q] >= kA || find_seg64(S.ub, u) != jA) continue;
+                    bs += (cr[q] == C_INS ? 1 : 0) - (cr[q] == C_DEL ? 1 : 0);
+                }
+                bshift = (int32_t)wave_sum_u32((uint32_t)bs);
+#pragma unroll
+                for (int q = 0; q < MV_RU; ++q) {
+                    const uint32_t u = R0 + (uint32_t)q * 64u + lane;
+                    if (u < R1) mark(u, kr[q], cr[q]);
+                }
+            } else {
+                uint32_t rA = R0;
+                if (kA > 0u) {
+                    rA = wave_bound<false, uint32_t>(upos, R0, S.ub[jA + 1], kA);
+                    bshift = (int32_t)((upre[2 * (uint64_t)rA] - upre[2 * (uint64_t)R0]) -
+                                       (upre[2 * (uint64_t)rA + 1] - upre[2 * (uint64_t)R0 + 1]));
+                }
+                uint32_t rB = R1;
+                if (S.off[jB + 1] > wend)
+                    rB = wave_bound<false, uint32_t>(upos, S.ub[jB] > rA ? S.ub[jB] : rA, rB, wend - S.off[jB]);
+                for (uint32_t ur = rA; ur < rB; ur += 64u * MV_RU) {
+                    uint32_t kk[MV_RU], cc[MV_RU];
+#pragma unroll
+                    for (int q = 0; q < MV_RU; ++q) {
+                        const uint32_t u = ur + (uint32_t)q * 64u + lane;
+                        kk[q] = cc[q] = 0;
+                        if (u < rB) {
+                            kk[q] = upos[u];
+                            cc[q] = ucode[u];
                         }
+                    }
+#pragma unroll
+                    for (int q = 0; q < MV_RU; ++q) {
+                        const uint32_t u = ur + (uint32_t)q * 64u + lane;
+                        if (u < rB) mark(u, kk[q], cc[q]);
                     }
                 }
             }
@@ -1288,7 +1350,8 @@ dcsr_status plan_and_check(dcsr_handle h, UpdBufs &b, int64_t n, hipStream_t s)
 {
     const bool table = use_tables(n);
     k_upd_plan<<<ptiles(n), TB, 0, s>>>(h->ds, (unsigned long long)h->pool_cap, b.trow, b.tins, b.tdel, b.ucode, h->row_len,
-                                        h->row_cap, b.tnewl, b.tmode, b.tmoved, b.tt, b.tp, b.pplan, b.fin, table);
+                                        h->row_cap, h->row_start, b.trs, b.tnewl, b.tmode, b.tmoved, b.tt, b.tp,
+                                        b.pplan, b.fin, table);
     if (!table) launch_offsets(h, b, n, s);
     return hipGetLastError() == hipSuccess ? DCSR_OK : DCSR_ERR_HIP;
 }
@@ -1362,8 +1425,8 @@ dcsr_status commit_launch(dcsr_handle h, const UpdBufs &b, int64_t n, int64_t mo
     int64_t blocks = (waves + 3) / 4;
     if (blocks > MV_BLOCKS) blocks = MV_BLOCKS;
     if (blocks < 1) blocks = 1;
-    k_upd_move<<<(unsigned)blocks, TB, 0, s>>>(h->ds, b.tpre, b.trow, h->row_start, b.tubeg, b.uval, b.ucode, b.upos,
-                                               b.upre, h->pool_col, h->pool_val);
+    k_upd_move<<<(unsigned)blocks, TB, 0, s>>>(h->ds, b.tpre, b.trs, b.tubeg, b.uval, b.ucode, b.upos, b.upre,
+                                               h->pool_col, h->pool_val);
     k_upd_write<<<grid_for(n, TB), TB, 0, s>>>(h->ds, report ? 1u : 0u, n, b.ut, b.trow, b.tubeg, b.tpre, b.tnewl, b.tmode, b.ucol, b.uval,
                                                 b.upos, b.ucode, b.upre, h->pool_col, h->pool_val, h->row_start,
                                                 h->row_len, h->row_cap);
