@@ -76,3 +76,39 @@ __device__ __forceinline__ void wave_visit(const uint32_t *__restrict__ off, uin
         __builtin_amdgcn_wave_barrier();
     }
 }
+
+// Start map of up to 64 * 64 items: bit i set when item i starts a segment
+// (segments in order, each starting at a distinct item).  Loaded from LDS
+// words (two per 64-bit word) into lane w = word w, with the count of set
+// bits in the words before it; all lanes must call it.
+struct StartMap {
+    uint64_t word;
+    uint32_t before;
+};
+__device__ __forceinline__ StartMap start_map_load(const uint32_t *m, int nwords64)
+{
+    const int lane = lane_id();
+    uint64_t w = 0;
+    if (lane < nwords64) w = (uint64_t)m[2 * lane] | ((uint64_t)m[2 * lane + 1] << 32);
+    const uint32_t c = (uint32_t)__popcll(w);
+    return {w, wave_incl_scan_u32(c) - c};
+}
+
+// Set bits of the map at items <= q * 64 + lane (the calling lane); q must be
+// the same in every lane (a compile-time index in unrolled loops).
+__device__ __forceinline__ uint32_t start_map_count(const StartMap &sm, int q)
+{
+    const uint64_t w = ((uint64_t)wave_read_lane_u32((uint32_t)(sm.word >> 32), q) << 32) |
+                       wave_read_lane_u32((uint32_t)sm.word, q);
+    const uint32_t lane = (uint32_t)lane_id();
+    return wave_read_lane_u32(sm.before, q) + (uint32_t)__popcll(w & ((2ull << lane) - 1ull));
+}
+
+// Set bits of the map strictly before item i (any i per lane).
+__device__ __forceinline__ uint32_t start_map_rank(const StartMap &sm, uint32_t i)
+{
+    const int wi = (int)(i >> 6);
+    const uint64_t w = ((uint64_t)(uint32_t)__shfl((int)(uint32_t)(sm.word >> 32), wi) << 32) |
+                       (uint32_t)__shfl((int)(uint32_t)sm.word, wi);
+    return (uint32_t)__shfl((int)sm.before, wi) + (uint32_t)__popcll(w & ((1ull << (i & 63)) - 1ull));
+}

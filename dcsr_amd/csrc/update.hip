@@ -137,6 +137,10 @@ __device__ __forceinline__ long long wave_sum_i64(long long v)
 // ---------------------------------------------------------------------------
 constexpr int IPT = 8;
 constexpr int TILE = TB * IPT;
+// Positions of the probe space (touched rows end to end) and of the move
+// space (old entries of rewritten rows end to end) owned by one wave.
+constexpr uint32_t PR_SPAN = 2048;
+constexpr uint32_t MV_SPAN = 1024;
 // batch tile table: heads / row heads (count, offset), claims / moved (sum, offset)
 enum { TT_H = 0, TT_R, TT_HX, TT_RX, TT_C, TT_M, TT_CX, TT_MX, TT_WORDS };
 // probe tile table: inserts, adds, deletes; insert / delete offsets; probe
@@ -299,7 +303,8 @@ __global__ __launch_bounds__(TB) void k_upd_heads_count(const uint64_t *__restri
 
 // Pass 2 of the heads: every head writes its compacted record: ucol/usum/
 // ut for coordinate u, trow/tubeg/tlpre (probe-length offset), the row's
-// pool start and length (trs, tlen) and zeroed tins/tdel for touched row t.  Sums run left to right from 0 in batch order, as the host path
+// pool start and length (trs, tlen) and zeroed tins/tdel for touched row t,
+// and (pspan) touched row t for every probe span that starts in it.  Sums run left to right from 0 in batch order, as the host path
 // accumulates B_row[j]; a run is summed from the thread's registers and
 // only a run that continues past the thread's items reads memory.  A tile's
 // records are contiguous, so they are staged in LDS and written out
@@ -314,7 +319,8 @@ __global__ __launch_bounds__(TB) void k_upd_heads_emit(const uint64_t *__restric
                                                        const uint32_t *__restrict__ row_start,
                                                        const uint32_t *__restrict__ tp, uint32_t *__restrict__ tlpre,
                                                        uint32_t *__restrict__ trs, uint32_t *__restrict__ tlen,
-                                                       uint32_t *__restrict__ tins, uint32_t *__restrict__ tdel)
+                                                       uint32_t *__restrict__ tins, uint32_t *__restrict__ tdel,
+                                                       uint32_t *__restrict__ pspan)
 {
     __shared__ uint32_t lds4[4];
     __shared__ int32_t s_ucol[TILE];
@@ -390,6 +396,9 @@ __global__ __launch_bounds__(TB) void k_upd_heads_emit(const uint64_t *__restric
             s_trow[lt] = r;
             s_tub[lt] = ubase + lu;
             s_tl[lt] = ll;
+            if (pspan)
+                for (uint32_t k = (ll + PR_SPAN - 1u) / PR_SPAN; k * PR_SPAN < ll + probe_len(lj[j]); ++k)
+                    pspan[k] = tbase + lt;
             ll += probe_len(lj[j]);
             trs[tbase + lt] = rsj[j];
             tlen[tbase + lt] = (uint32_t)lj[j];
@@ -438,11 +447,12 @@ __global__ __launch_bounds__(TB) void k_upd_heads_emit(const uint64_t *__restric
 // work per wave does not depend on the row-length distribution.  Per-row
 // insert / delete counts are summed in LDS and added once per row and span;
 // the per-tile code counts are made by the plan.
-constexpr uint32_t PR_SPAN = 2048;
 constexpr uint32_t PR_PER = PR_SPAN / 64;  // staged columns per lane
 constexpr int PR_RU = 4;                   // record rounds of 64 in flight
 struct ProbeLds {
     int32_t col[PR_SPAN];
+    uint32_t pmap[PR_SPAN / 32];        // start map of the span's positions
+    uint32_t rmap[64 * PR_RU / 32];     // start map of the rows' coordinates (few)
     uint32_t a[65];    // probe offset of window row j (~0 past the last row)
     uint32_t ub[65];   // first record of window row j (~0 past the last row)
     uint32_t rs[64];   // pool start of row j
@@ -514,7 +524,7 @@ __global__ __launch_bounds__(TB) void k_upd_probe(const DevState *__restrict__ d
                                                   const int32_t *__restrict__ ucol, const float *__restrict__ usum,
                                                   float *__restrict__ uval, uint32_t *__restrict__ upos,
                                                   uint32_t *__restrict__ ucode, uint32_t *__restrict__ tins,
-                                                  uint32_t *__restrict__ tdel)
+                                                  uint32_t *__restrict__ tdel, const uint32_t *__restrict__ pspan)
 {
     __shared__ ProbeLds sh[TB / 64];
     ProbeLds &S = sh[threadIdx.x >> 6];
@@ -527,7 +537,7 @@ __global__ __launch_bounds__(TB) void k_upd_probe(const DevState *__restrict__ d
         const uint32_t e0 = (uint32_t)(wave * PR_SPAN);
         const uint32_t e1 = total - e0 < PR_SPAN ? total : e0 + PR_SPAN;
         // every probe length is >= 1, so exactly one row holds e0
-        uint32_t j = wave_find_last_le(tlpre, nt, e0);
+        uint32_t j = pspan ? pspan[wave] : wave_find_last_le(tlpre, nt, e0);
         uint32_t e = e0;
         while (e < e1) {
             // rows j .. j+63 (nt >= 1: the span is non-empty)
@@ -560,9 +570,23 @@ __global__ __launch_bounds__(TB) void k_upd_probe(const DevState *__restrict__ d
             // by column, else located by two searches
             const uint32_t R0 = S.ub[jA], R1 = S.ub[jB + 1];
             const bool few = R1 - R0 <= 64u * PR_RU;
+            // the rows' first positions (and first coordinates) as start
+            // maps: a position's row is a popcount away, no search
+            if (lane < PR_SPAN / 32) S.pmap[lane] = 0u;
+            if (lane < 2 * PR_RU) S.rmap[lane] = 0u;
+            __builtin_amdgcn_wave_barrier();
+            if (t < nt) {
+                if (a_t >= e && a_t < wend) atomicOr(&S.pmap[(a_t - e) >> 5], 1u << ((a_t - e) & 31u));
+                if (few && ub >= R0 && ub < R1) atomicOr(&S.rmap[(ub - R0) >> 5], 1u << ((ub - R0) & 31u));
+            }
+            __builtin_amdgcn_wave_barrier();
+            const StartMap pm = start_map_load(S.pmap, PR_SPAN / 64);
+            const StartMap rm = start_map_load(S.rmap, PR_RU);
+            const int pbase = jA - (kA == 0u ? 1 : 0);
             int32_t rc[PR_RU];
             float rsv[PR_RU];
-            int32_t before = 0;  // the column before the cut of row jA (a cut row has >= 2 entries)
+            // the column before the cut of row jA (a cut row has >= 2 entries)
+            int32_t before = 0;
             {
                 if (kA > 0u) before = pcol[(uint64_t)S.rs[jA] + kA - 1u];
 #pragma unroll
@@ -580,8 +604,8 @@ __global__ __launch_bounds__(TB) void k_upd_probe(const DevState *__restrict__ d
                 for (uint32_t q = 0; q < PR_PER; ++q) {
                     const uint32_t p = q * 64u + lane;
                     cv[q] = 0x7fffffff;
+                    const int jr = pbase + (int)start_map_count(pm, (int)q);
                     if (p < cnt) {
-                        const int jr = find_seg64(S.a, e + p);
                         const uint32_t k = e + p - S.a[jr];
                         if (k < S.len[jr]) cv[q] = pcol[(uint64_t)S.rs[jr] + k];
                     }
@@ -619,8 +643,8 @@ __global__ __launch_bounds__(TB) void k_upd_probe(const DevState *__restrict__ d
 #pragma unroll
                 for (int q = 0; q < PR_RU; ++q) {
                     const uint32_t u = R0 + (uint32_t)q * 64u + lane;
+                    const int jr = jA - 1 + (int)start_map_count(rm, q);
                     if (u >= R1) continue;
-                    const int jr = find_seg64(S.ub, u);
                     if (jr == jA && kA > 0u && rc[q] <= before) continue;
                     if (jr == jB && cutB && rc[q] > lastc) continue;
                     resolve(u, jr, rc[q], rsv[q]);
@@ -833,7 +857,7 @@ __global__ __launch_bounds__(TB) void k_upd_plan(DevState *ds, unsigned long lon
 // before coordinate u (u <= nuniq; entry nuniq holds the totals) and
 // tpre[2t], tpre[2t+1] = chunk slots claimed / old entries moved before
 // touched row t (t <= ntouched), from the in-tile prefixes plus the tile
-// offsets.
+// offsets; (mspan) touched row t for every move span that starts in it.
 __global__ __launch_bounds__(TB) void k_upd_offsets(DevState *__restrict__ ds,
                                                     const uint32_t *__restrict__ ucode,
                                                     const uint32_t *__restrict__ tmode,
@@ -842,7 +866,7 @@ __global__ __launch_bounds__(TB) void k_upd_offsets(DevState *__restrict__ ds,
                                                     const uint32_t *__restrict__ tt, const uint32_t *__restrict__ tp,
                                                     uint32_t *__restrict__ upre, uint32_t *__restrict__ tpre,
                                                     const long long *__restrict__ pplan, unsigned long long pool_cap,
-                                                    bool table)
+                                                    bool table, uint32_t *__restrict__ mspan)
 {
     __shared__ uint32_t lds4[4];
     __shared__ long long lds[4][8];
@@ -928,6 +952,8 @@ __global__ __launch_bounds__(TB) void k_upd_offsets(DevState *__restrict__ ds,
         if (e <= nt) {
             tpre[2 * e] = sc;
             tpre[2 * e + 1] = sm;
+            if (mspan && e < nt)
+                for (uint32_t k = (sm + MV_SPAN - 1u) / MV_SPAN; k * MV_SPAN < sm + mv[j]; ++k) mspan[k] = (uint32_t)e;
         }
         si += ins[j];
         sd += del[j];
@@ -951,12 +977,14 @@ __global__ __launch_bounds__(TB) void k_upd_offsets(DevState *__restrict__ ds,
 // every entry its shift.  So the copy is a coalesced stream with no
 // per-entry search.  A row cut by the chunk's start begins from the shift
 // of its records before the cut, located by a search on their positions.
-constexpr uint32_t MV_SPAN = 1024;
 constexpr uint32_t MV_PER = MV_SPAN / 64;  // entries per lane
 constexpr uint32_t MV_TOMB = 0xffffffffu;
 constexpr int MV_RU = 4;                   // record rounds of 64 in flight
 struct MoveLds {
     int32_t mk[MV_SPAN];   // marks
+    uint32_t pmap[MV_SPAN / 32];     // start map of the chunk's rows with entries
+    uint32_t rmap[64 * MV_RU / 32];  // start map of the rows' records (few)
+    uint32_t tab[64];                // window row of the k-th start in pmap
     uint32_t ov[MV_SPAN];  // 0, MV_TOMB or 1 + the index of the add record
     uint32_t off[65];      // move-space start of window row j (~0 past the last row)
     uint32_t ub[65];       // first record of window row j (~0 past the last row)
@@ -964,12 +992,12 @@ struct MoveLds {
     uint32_t dst[64];      // claim offset of row j
 };
 
-__global__ __launch_bounds__(TB) void k_upd_move(const DevState *__restrict__ ds, const uint32_t *__restrict__ tpre,
+__global__ __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(4))) void k_upd_move(const DevState *__restrict__ ds, const uint32_t *__restrict__ tpre,
                                                  const uint32_t *__restrict__ trs,
                                                  const uint32_t *__restrict__ tubeg, const float *__restrict__ uval,
                                                  const uint32_t *__restrict__ ucode, const uint32_t *__restrict__ upos,
                                                  const uint32_t *__restrict__ upre, int32_t *__restrict__ pcol,
-                                                 float *__restrict__ pval)
+                                                 float *__restrict__ pval, const uint32_t *__restrict__ mspan)
 {
     __shared__ MoveLds sh[TB / 64];
     if (!ds->go) return;
@@ -983,7 +1011,7 @@ __global__ __launch_bounds__(TB) void k_upd_move(const DevState *__restrict__ ds
          wave += nwaves) {
         const uint32_t e0 = (uint32_t)(wave * MV_SPAN);
         const uint32_t e1 = total - e0 < MV_SPAN ? total : e0 + MV_SPAN;
-        uint32_t j = wave_find_last_le<2>(tpre + 1, nt, e0);
+        uint32_t j = mspan ? mspan[wave] : wave_find_last_le<2>(tpre + 1, nt, e0);
         uint32_t e = e0;
         while (e < e1) {
             // rows j .. j+63 (nt >= 1 here: the range is non-empty); lanes
@@ -1014,34 +1042,57 @@ __global__ __launch_bounds__(TB) void k_upd_move(const DevState *__restrict__ ds
                 j += 64;
                 continue;
             }
-            // the chunk's entries: row, destination before the shift, and
-            // the loads
+            // old entries of window row j
+            auto old_len = [&](int jr) { return S.off[jr + 1] - S.off[jr]; };
+            const int jA = find_seg64(S.off, e), jB = find_seg64(S.off, wend - 1u);
+            const uint32_t kA = e - S.off[jA];
+            const uint32_t R0 = S.ub[jA], R1 = S.ub[jB + 1];
+            const bool few = R1 - R0 <= 64u * MV_RU;
+            // start maps: of the chunk's rows with entries (their ranks to
+            // rows in tab; rows without entries share a start) and of the
+            // rows' records (every touched row has one), so that an entry's
+            // or a record's row is a popcount away, no search
+            if (lane < MV_SPAN / 32) S.pmap[lane] = 0u;
+            if (lane < 2 * MV_RU) S.rmap[lane] = 0u;
+            __builtin_amdgcn_wave_barrier();
+            const bool starts_here = t < nt && off_t >= e && off_t < wend && old_len((int)lane) > 0u;
+            if (starts_here) atomicOr(&S.pmap[(off_t - e) >> 5], 1u << ((off_t - e) & 31u));
+            if (few && t < nt && ub >= R0 && ub < R1) atomicOr(&S.rmap[(ub - R0) >> 5], 1u << ((ub - R0) & 31u));
+            __builtin_amdgcn_wave_barrier();
+            const StartMap pm = start_map_load(S.pmap, MV_SPAN / 64);
+            const StartMap rm = start_map_load(S.rmap, MV_RU);
+            const uint32_t rank = start_map_rank(pm, starts_here ? off_t - e : 0u);  // every lane takes part
+            if (starts_here) S.tab[rank] = lane;
+            __builtin_amdgcn_wave_barrier();
+            // the chunk's entries: row (8 bits each, four to a word),
+            // destination before the shift, and the loads (bit q of `real`:
+            // position q holds an entry)
             int32_t c[MV_PER];
             float v[MV_PER];
-            uint32_t jq[MV_PER], dq[MV_PER];
+            uint32_t jpk[MV_PER / 4], dq[MV_PER];
+            uint32_t real = 0;
 #pragma unroll
             for (uint32_t q = 0; q < MV_PER; ++q) {
                 const uint32_t p = q * 64u + lane;
                 c[q] = 0;
                 v[q] = 0.0f;
-                jq[q] = 64u;  // past the chunk: above every row
                 dq[q] = 0;
+                uint32_t jr = 64u;  // past the chunk: above every row
+                const uint32_t nst = start_map_count(pm, (int)q);
                 if (p < cnt) {
-                    jq[q] = (uint32_t)find_seg64(S.off, e + p);
-                    const uint32_t k = e + p - S.off[jq[q]];
-                    dq[q] = S.dst[jq[q]] + k;
-                    const uint64_t sp = (uint64_t)S.src[jq[q]] + k;
-                    c[q] = pcol[sp];
-                    v[q] = pval[sp];
+                    jr = nst == 0u ? (uint32_t)jA : S.tab[nst - 1u];
+                    const uint32_t k = e + p - S.off[jr];
+                    dq[q] = S.dst[jr] + k;
+                    if (k < old_len((int)jr)) {
+                        real |= 1u << q;
+                        const uint64_t sp = (uint64_t)S.src[jr] + k;
+                        c[q] = pcol[sp];
+                        v[q] = pval[sp];
+                    }
                 }
+                if ((q & 3u) == 0u) jpk[q >> 2] = 0u;
+                jpk[q >> 2] |= jr << (8u * (q & 3u));
             }
-            // the records of rows jA .. jB; when they are few, they are
-            // loaded with the entries and filtered by position, else the
-            // cut rows' ones are located by searches
-            const int jA = find_seg64(S.off, e), jB = find_seg64(S.off, wend - 1u);
-            const uint32_t kA = e - S.off[jA];
-            const uint32_t R0 = S.ub[jA], R1 = S.ub[jB + 1];
-            const bool few = R1 - R0 <= 64u * MV_RU;
             uint32_t kr[MV_RU], cr[MV_RU];
 #pragma unroll
             for (int q = 0; q < MV_RU; ++q) {
@@ -1057,10 +1108,9 @@ __global__ __launch_bounds__(TB) void k_upd_move(const DevState *__restrict__ ds
                 S.ov[p] = 0u;
             }
             __builtin_amdgcn_wave_barrier();
-            // a record whose mark falls in [e, wend)
-            auto mark = [&](uint32_t u, uint32_t k, uint32_t code) {
-                const int jr = find_seg64(S.ub, u);
-                const uint32_t rlen = S.off[jr + 1] - S.off[jr];
+            // a record of window row jr whose mark falls in [e, wend)
+            auto mark = [&](uint32_t u, int jr, uint32_t k, uint32_t code) {
+                const uint32_t rlen = old_len(jr);
                 const uint32_t pa = S.off[jr] + k;
                 // rows without old entries to move (written in place, or
                 // new) have rlen 0; inserts past a row's end shift nothing
@@ -1083,15 +1133,12 @@ __global__ __launch_bounds__(TB) void k_upd_move(const DevState *__restrict__ ds
 #pragma unroll
                 for (int q = 0; q < MV_RU; ++q) {
                     const uint32_t u = R0 + (uint32_t)q * 64u + lane;
-                    if (u >= R1 || kr[q] >= kA || find_seg64(S.ub, u) != jA) continue;
-                    bs += (cr[q] == C_INS ? 1 : 0) - (cr[q] == C_DEL ? 1 : 0);
+                    const int jr = jA - 1 + (int)start_map_count(rm, q);
+                    if (u >= R1) continue;
+                    if (jr == jA && kr[q] < kA) bs += (cr[q] == C_INS ? 1 : 0) - (cr[q] == C_DEL ? 1 : 0);
+                    mark(u, jr, kr[q], cr[q]);
                 }
                 bshift = (int32_t)wave_sum_u32((uint32_t)bs);
-#pragma unroll
-                for (int q = 0; q < MV_RU; ++q) {
-                    const uint32_t u = R0 + (uint32_t)q * 64u + lane;
-                    if (u < R1) mark(u, kr[q], cr[q]);
-                }
             } else {
                 uint32_t rA = R0;
                 if (kA > 0u) {
@@ -1116,7 +1163,7 @@ __global__ __launch_bounds__(TB) void k_upd_move(const DevState *__restrict__ ds
 #pragma unroll
                     for (int q = 0; q < MV_RU; ++q) {
                         const uint32_t u = ur + (uint32_t)q * 64u + lane;
-                        if (u < rB) mark(u, kk[q], cc[q]);
+                        if (u < rB) mark(u, find_seg64(S.ub, u), kk[q], cc[q]);
                     }
                 }
             }
@@ -1129,17 +1176,18 @@ __global__ __launch_bounds__(TB) void k_upd_move(const DevState *__restrict__ ds
 #pragma unroll
             for (uint32_t q = 0; q < MV_PER; ++q) {
                 const uint32_t p = q * 64u + lane;
+                const uint32_t jr = (jpk[q >> 2] >> (8u * (q & 3u))) & 0xffu;
                 int32_t m = p < cnt ? S.mk[p] : 0;
-                if (lane == 0 && jq[q] == carry_row) m += carry;
-                const int32_t inc = (int32_t)wave_seg_incl_scan_u32((uint32_t)m, jq[q]);
+                if (lane == 0 && jr == carry_row) m += carry;
+                const int32_t inc = (int32_t)wave_seg_incl_scan_u32((uint32_t)m, jr);
                 dq[q] += (uint32_t)inc;
                 carry = (int32_t)wave_read_lane_u32((uint32_t)inc, 63);
-                carry_row = wave_read_lane_u32(jq[q], 63);
+                carry_row = wave_read_lane_u32(jr, 63);
             }
 #pragma unroll
             for (uint32_t q = 0; q < MV_PER; ++q) {
                 const uint32_t p = q * 64u + lane;
-                const uint32_t o = p < cnt ? S.ov[p] : MV_TOMB;
+                const uint32_t o = (real >> q) & 1u ? S.ov[p] : MV_TOMB;
                 if (o == MV_TOMB) continue;
                 const float vv = o ? uval[o - 1u] : v[q];
                 pcol[base + dq[q]] = c[q];
@@ -1223,6 +1271,8 @@ struct UpdBufs {
     uint32_t *ut, *upos, *ucode, *upre;
     uint32_t *trow, *tubeg, *tlpre, *trs, *tlen, *tins, *tdel, *tnewl, *tmode, *tmoved, *tpre;
     uint32_t *tt, *tp, *fin;
+    // first touched row of every probe / move span (null: searched for)
+    uint32_t *pspan, *mspan;
     long long *pplan;
     // sort: row groups (lb = gshift + cbits > 0, ndig = groups + 1) or, for
     // batches past GS_MAX_N triples, the LSD radix sort (lb = 0)
@@ -1325,6 +1375,13 @@ dcsr_status upd_bufs_src(BufSrc &src, int64_t n, UpdBufs &b)
     DCSR_TRY(take(src, WS_PPLAN, nt * NPLAN * 8, (void **)&b.pplan));
     // finish counters: zero-filled on allocation, reset by their last user
     DCSR_TRY(take(src, WS_TOLDS, 64, (void **)&b.fin));
+    // span tables grow with the matrix, so a fixed region goes without them
+    b.pspan = b.mspan = nullptr;
+    if (!src.base && !src.measure) {
+        const int64_t live = live_bound(h);
+        DCSR_TRY(take(src, WS_PSPAN, (size_t)((live + n) / PR_SPAN + 2) * 4, (void **)&b.pspan));
+        DCSR_TRY(take(src, WS_MSPAN, (size_t)(live / MV_SPAN + 2) * 4, (void **)&b.mspan));
+    }
     return DCSR_OK;
 }
 
@@ -1341,7 +1398,7 @@ static inline bool use_tables(int64_t n) { return ptiles(n) > HEADS_DIRECT_TILES
 static void launch_offsets(dcsr_handle h, UpdBufs &b, int64_t n, hipStream_t s)
 {
     k_upd_offsets<<<ptiles(n), TB, 0, s>>>(h->ds, b.ucode, b.tmode, b.tnewl, b.tmoved, b.tt, b.tp, b.upre, b.tpre,
-                                           b.pplan, (unsigned long long)h->pool_cap, use_tables(n));
+                                           b.pplan, (unsigned long long)h->pool_cap, use_tables(n), b.mspan);
 }
 
 // Plan and decision.  With tile tables the plan's last workgroup decides;
@@ -1385,10 +1442,10 @@ dcsr_status sort_plan_launch(dcsr_handle h, int64_t n, UpdBufs &b, hipStream_t s
     k_upd_heads_count<<<ht, TB, 0, s>>>(key, n, h->cbits, h->row_len, b.tt, b.tp, b.fin, h->ds, table);
     k_upd_heads_emit<<<ht, TB, 0, s>>>(key, val, n, h->cbits, b.tt, h->ds, table, b.ucol, b.usum, b.ut, b.trow,
                                        b.tubeg, h->row_len, h->row_start, b.tp, b.tlpre, b.trs, b.tlen, b.tins,
-                                       b.tdel);
+                                       b.tdel, b.pspan);
     k_upd_probe<<<probe_blocks(h, n), TB, 0, s>>>(h->ds, h->row_start, h->row_len, h->pool_col, h->pool_val,
                                                   b.tubeg, b.tlpre, b.trs, b.tlen, b.ucol, b.usum, b.uval, b.upos,
-                                                  b.ucode, b.tins, b.tdel);
+                                                  b.ucode, b.tins, b.tdel, b.pspan);
     DCSR_HIP(hipGetLastError());
     DCSR_TRY(plan_and_check(h, b, n, s));
     if (use_tables(n)) launch_offsets(h, b, n, s);
@@ -1426,7 +1483,7 @@ dcsr_status commit_launch(dcsr_handle h, const UpdBufs &b, int64_t n, int64_t mo
     if (blocks > MV_BLOCKS) blocks = MV_BLOCKS;
     if (blocks < 1) blocks = 1;
     k_upd_move<<<(unsigned)blocks, TB, 0, s>>>(h->ds, b.tpre, b.trs, b.tubeg, b.uval, b.ucode, b.upos, b.upre,
-                                               h->pool_col, h->pool_val);
+                                               h->pool_col, h->pool_val, b.mspan);
     k_upd_write<<<grid_for(n, TB), TB, 0, s>>>(h->ds, report ? 1u : 0u, n, b.ut, b.trow, b.tubeg, b.tpre, b.tnewl, b.tmode, b.ucol, b.uval,
                                                 b.upos, b.ucode, b.upre, h->pool_col, h->pool_val, h->row_start,
                                                 h->row_len, h->row_cap);
