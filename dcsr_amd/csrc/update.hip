@@ -640,14 +640,58 @@ __global__ __launch_bounds__(TB) void k_upd_probe(const DevState *__restrict__ d
             const bool cutB = S.a[jB + 1] > wend;
             const int32_t lastc = cutB ? S.col[cnt - 1u] : 0;
             if (few) {
+                // in three stages over all rounds, so that the searches and
+                // then the value loads of every round overlap: lower bounds
+                // (a fixed number of halving steps), old values, results
+                uint32_t pos[PR_RU], at[PR_RU];
+                int jq[PR_RU];
+                bool act[PR_RU], mt[PR_RU];
 #pragma unroll
                 for (int q = 0; q < PR_RU; ++q) {
                     const uint32_t u = R0 + (uint32_t)q * 64u + lane;
                     const int jr = jA - 1 + (int)start_map_count(rm, q);
-                    if (u >= R1) continue;
-                    if (jr == jA && kA > 0u && rc[q] <= before) continue;
-                    if (jr == jB && cutB && rc[q] > lastc) continue;
-                    resolve(u, jr, rc[q], rsv[q]);
+                    act[q] = u < R1 && !(jr == jA && kA > 0u && rc[q] <= before) &&
+                             !(jr == jB && cutB && rc[q] > lastc);
+                    jq[q] = jr;
+                    const uint32_t ar = S.a[jr];
+                    const uint32_t k0 = ar < e ? e - ar : 0u;
+                    const uint32_t lend = ar + S.len[jr];
+                    const uint32_t k1 = (lend < wend ? lend : wend) - ar;
+                    const uint32_t p0 = ar + k0 - e;
+                    uint32_t b = p0, len = act[q] && k1 > k0 ? k1 - k0 : 0u;
+#pragma unroll
+                    for (int it = 0; it < 12; ++it) {  // 2^12 > PR_SPAN
+                        const uint32_t half = len >> 1;
+                        const uint32_t ix = b + half < PR_SPAN ? b + half : PR_SPAN - 1u;
+                        const bool go = len > 0u && S.col[ix] < rc[q];
+                        b = go ? b + half + 1u : b;
+                        len = go ? len - half - 1u : half;
+                    }
+                    pos[q] = k0 + (b - p0);
+                    mt[q] = act[q] && b - p0 < (k1 > k0 ? k1 - k0 : 0u) && S.col[b < PR_SPAN ? b : PR_SPAN - 1u] == rc[q];
+                    at[q] = S.rs[jr] + pos[q];
+                }
+                float hv[PR_RU];
+#pragma unroll
+                for (int q = 0; q < PR_RU; ++q) hv[q] = mt[q] ? pval[at[q]] : 0.0f;
+#pragma unroll
+                for (int q = 0; q < PR_RU; ++q) {
+                    if (!act[q]) continue;
+                    const uint32_t u = R0 + (uint32_t)q * 64u + lane;
+                    uint32_t code;
+                    float nv;
+                    if (mt[q]) {
+                        nv = hv[q] + rsv[q];
+                        code = (nv != 0.0f) ? C_ADD : C_DEL;
+                    } else {
+                        nv = rsv[q];
+                        code = (rsv[q] != 0.0f) ? C_INS : C_NOP;
+                    }
+                    uval[u] = nv;
+                    upos[u] = pos[q];
+                    ucode[u] = code;
+                    if (code == C_INS) atomicAdd(&S.ni[jq[q]], 1u);
+                    else if (code == C_DEL) atomicAdd(&S.nd[jq[q]], 1u);
                 }
             } else {
                 uint32_t rA = R0, rB = R1;
@@ -965,27 +1009,28 @@ __global__ __launch_bounds__(TB) void k_upd_offsets(DevState *__restrict__ ds,
         plan_totals_decide(ds, pool_cap, pplan, tp, gridDim.x, bc + tc, bm + tm, bi + ti, bd + td, lds);
 }
 
-// Move surviving old entries of rewritten rows to their merged positions.
+// Move surviving old entries of rewritten rows to their merged positions
+// (with their old values: k_upd_write stores the new ones of added-to
+// entries).
 // tpre[2t] = claim offset of row t (relative to bump_saved), tpre[2t+1] =
 // offset of row t's old entries in the flat move space (rows written in
 // place hold none).  Each wave owns MV_SPAN entries of that space, taken in
 // chunks of at most 64 rows.  The loads of a chunk's entries are issued
 // first; while they are in flight, the update records of the chunk's rows
 // mark the entries they shift (an insert +1 on the entry it goes before, a
-// delete a tombstone on its entry and -1 on the row's next one, an add its
-// record index) and a per-row scan of the marks, in 64-wide rounds, gives
+// delete a tombstone on its entry and -1 on the row's next one) and a
+// per-row scan of the marks, in 64-wide rounds, gives
 // every entry its shift.  So the copy is a coalesced stream with no
 // per-entry search.  A row cut by the chunk's start begins from the shift
 // of its records before the cut, located by a search on their positions.
 constexpr uint32_t MV_PER = MV_SPAN / 64;  // entries per lane
-constexpr uint32_t MV_TOMB = 0xffffffffu;
 constexpr int MV_RU = 4;                   // record rounds of 64 in flight
 struct MoveLds {
-    int32_t mk[MV_SPAN];   // marks
+    int32_t mk[MV_SPAN];             // marks
+    uint32_t tomb[MV_SPAN / 32];     // deleted entries
     uint32_t pmap[MV_SPAN / 32];     // start map of the chunk's rows with entries
     uint32_t rmap[64 * MV_RU / 32];  // start map of the rows' records (few)
     uint32_t tab[64];                // window row of the k-th start in pmap
-    uint32_t ov[MV_SPAN];  // 0, MV_TOMB or 1 + the index of the add record
     uint32_t off[65];      // move-space start of window row j (~0 past the last row)
     uint32_t ub[65];       // first record of window row j (~0 past the last row)
     uint32_t src[64];      // old pool start of row j
@@ -994,7 +1039,7 @@ struct MoveLds {
 
 __global__ __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(4))) void k_upd_move(const DevState *__restrict__ ds, const uint32_t *__restrict__ tpre,
                                                  const uint32_t *__restrict__ trs,
-                                                 const uint32_t *__restrict__ tubeg, const float *__restrict__ uval,
+                                                 const uint32_t *__restrict__ tubeg,
                                                  const uint32_t *__restrict__ ucode, const uint32_t *__restrict__ upos,
                                                  const uint32_t *__restrict__ upre, int32_t *__restrict__ pcol,
                                                  float *__restrict__ pval, const uint32_t *__restrict__ mspan)
@@ -1052,7 +1097,10 @@ __global__ __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(4))) void k_
             // rows in tab; rows without entries share a start) and of the
             // rows' records (every touched row has one), so that an entry's
             // or a record's row is a popcount away, no search
-            if (lane < MV_SPAN / 32) S.pmap[lane] = 0u;
+            if (lane < MV_SPAN / 32) {
+                S.pmap[lane] = 0u;
+                S.tomb[lane] = 0u;
+            }
             if (lane < 2 * MV_RU) S.rmap[lane] = 0u;
             __builtin_amdgcn_wave_barrier();
             const bool starts_here = t < nt && off_t >= e && off_t < wend && old_len((int)lane) > 0u;
@@ -1064,25 +1112,22 @@ __global__ __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(4))) void k_
             const uint32_t rank = start_map_rank(pm, starts_here ? off_t - e : 0u);  // every lane takes part
             if (starts_here) S.tab[rank] = lane;
             __builtin_amdgcn_wave_barrier();
-            // the chunk's entries: row (8 bits each, four to a word),
-            // destination before the shift, and the loads (bit q of `real`:
-            // position q holds an entry)
+            // the chunk's entries: row (8 bits each, four to a word) and
+            // the loads (bit q of `real`: position q holds an entry)
             int32_t c[MV_PER];
             float v[MV_PER];
-            uint32_t jpk[MV_PER / 4], dq[MV_PER];
+            uint32_t jpk[MV_PER / 4];
             uint32_t real = 0;
 #pragma unroll
             for (uint32_t q = 0; q < MV_PER; ++q) {
                 const uint32_t p = q * 64u + lane;
                 c[q] = 0;
                 v[q] = 0.0f;
-                dq[q] = 0;
                 uint32_t jr = 64u;  // past the chunk: above every row
                 const uint32_t nst = start_map_count(pm, (int)q);
                 if (p < cnt) {
                     jr = nst == 0u ? (uint32_t)jA : S.tab[nst - 1u];
                     const uint32_t k = e + p - S.off[jr];
-                    dq[q] = S.dst[jr] + k;
                     if (k < old_len((int)jr)) {
                         real |= 1u << q;
                         const uint64_t sp = (uint64_t)S.src[jr] + k;
@@ -1103,10 +1148,7 @@ __global__ __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(4))) void k_
                     cr[q] = ucode[u];
                 }
             }
-            for (uint32_t p = lane; p < cnt; p += 64u) {
-                S.mk[p] = 0;
-                S.ov[p] = 0u;
-            }
+            for (uint32_t p = lane; p < cnt; p += 64u) S.mk[p] = 0;
             __builtin_amdgcn_wave_barrier();
             // a record of window row jr whose mark falls in [e, wend)
             auto mark = [&](uint32_t u, int jr, uint32_t k, uint32_t code) {
@@ -1119,10 +1161,8 @@ __global__ __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(4))) void k_
                     if (code == C_INS) {
                         atomicAdd(&S.mk[p], 1);
                     } else if (code == C_DEL) {
-                        S.ov[p] = MV_TOMB;
+                        atomicOr(&S.tomb[p >> 5], 1u << (p & 31u));
                         if (k + 1u < rlen && p + 1u < cnt) atomicAdd(&S.mk[p + 1u], -1);
-                    } else if (code == C_ADD) {
-                        S.ov[p] = u + 1u;
                     }
                 }
             };
@@ -1173,25 +1213,25 @@ __global__ __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(4))) void k_
             // continues; the cut row jA starts from bshift
             int32_t carry = bshift;
             uint32_t carry_row = (uint32_t)jA;
+            uint32_t dq[MV_PER];  // destination: the row's claim + offset in the row + shift
 #pragma unroll
             for (uint32_t q = 0; q < MV_PER; ++q) {
                 const uint32_t p = q * 64u + lane;
                 const uint32_t jr = (jpk[q >> 2] >> (8u * (q & 3u))) & 0xffu;
-                int32_t m = p < cnt ? S.mk[p] : 0;
+                const bool in = p < cnt;
+                int32_t m = in ? S.mk[p] : 0;
                 if (lane == 0 && jr == carry_row) m += carry;
                 const int32_t inc = (int32_t)wave_seg_incl_scan_u32((uint32_t)m, jr);
-                dq[q] += (uint32_t)inc;
+                dq[q] = in ? S.dst[jr] + (e + p - S.off[jr]) + (uint32_t)inc : 0u;
                 carry = (int32_t)wave_read_lane_u32((uint32_t)inc, 63);
                 carry_row = wave_read_lane_u32(jr, 63);
             }
 #pragma unroll
             for (uint32_t q = 0; q < MV_PER; ++q) {
                 const uint32_t p = q * 64u + lane;
-                const uint32_t o = (real >> q) & 1u ? S.ov[p] : MV_TOMB;
-                if (o == MV_TOMB) continue;
-                const float vv = o ? uval[o - 1u] : v[q];
+                if (!((real >> q) & 1u) || ((S.tomb[p >> 5] >> (p & 31u)) & 1u)) continue;
                 pcol[base + dq[q]] = c[q];
-                pval[base + dq[q]] = vv;
+                pval[base + dq[q]] = v[q];
             }
             __builtin_amdgcn_wave_barrier();
             e = wend;
@@ -1250,10 +1290,12 @@ __global__ __launch_bounds__(TB) void k_upd_write(DevState *ds, uint32_t report,
         return;
     }
     const uint64_t dest = ds->bump_saved + tp;
-    if (code == C_INS) {
+    if (code == C_INS || code == C_ADD) {
+        // the inserts / deletes before it in the row shift it (an added-to
+        // entry was moved there with its old value)
         const int64_t shift = (int64_t)(ui - upre[2 * ub]) - (int64_t)(ud - upre[2 * ub + 1]);
         const uint64_t d = dest + (uint64_t)((int64_t)pos + shift);
-        pcol[d] = col;
+        if (code == C_INS) pcol[d] = col;
         pval[d] = v;
     }
     if (u == ub) {
@@ -1482,8 +1524,8 @@ dcsr_status commit_launch(dcsr_handle h, const UpdBufs &b, int64_t n, int64_t mo
     int64_t blocks = (waves + 3) / 4;
     if (blocks > MV_BLOCKS) blocks = MV_BLOCKS;
     if (blocks < 1) blocks = 1;
-    k_upd_move<<<(unsigned)blocks, TB, 0, s>>>(h->ds, b.tpre, b.trs, b.tubeg, b.uval, b.ucode, b.upos, b.upre,
-                                               h->pool_col, h->pool_val, b.mspan);
+    k_upd_move<<<(unsigned)blocks, TB, 0, s>>>(h->ds, b.tpre, b.trs, b.tubeg, b.ucode, b.upos, b.upre, h->pool_col,
+                                               h->pool_val, b.mspan);
     k_upd_write<<<grid_for(n, TB), TB, 0, s>>>(h->ds, report ? 1u : 0u, n, b.ut, b.trow, b.tubeg, b.tpre, b.tnewl, b.tmode, b.ucol, b.uval,
                                                 b.upos, b.ucode, b.upre, h->pool_col, h->pool_val, h->row_start,
                                                 h->row_len, h->row_cap);
