@@ -14,7 +14,8 @@
 //      in LDS so runs are written together);
 //   3. k_gs_sort: one workgroup per group sorts the group's pairs by their
 //      low lb key bits (all of its keys share the bits above), a stable LSD
-//      sort with 8-bit digits; groups of up to GS_CAP pairs sort in LDS,
+//      sort with 8- or 9-bit digits (whichever takes fewer passes); groups of
+//      up to GS_CAP pairs sort in LDS,
 //      larger ones out of global memory with the same ranking.
 //
 // The result is exactly the stable sort of the batch by key, which the LSD
@@ -220,17 +221,18 @@ __global__ __launch_bounds__(GT) void k_gs_scatter(const uint64_t *__restrict__ 
 // ---------------------------------------------------------------------------
 constexpr int SJ = GS_CAP / GT;  // items per lane in a full LDS group
 
-template <int J>
+template <int J, int DB = 8>
 __device__ __forceinline__ void rank_chunk(const uint32_t (&dg)[J], const bool (&real)[J], int jmax,
-                                           uint32_t (&rnk)[J], uint32_t (*wcnt)[256], uint32_t *loff,
+                                           uint32_t (&rnk)[J], uint32_t (*wcnt)[1 << DB], uint32_t *loff,
                                            uint32_t *lds4)
 {
+    constexpr int ND = 1 << DB;     // digits
+    constexpr int DPT = ND / GT;    // digits per thread in the offset scan
+    static_assert(DPT >= 1 && ND % GT == 0, "at least one digit per thread");
     const int w = threadIdx.x >> 6, lane = lane_id();
     const uint64_t lt = lanemask_lt();
-    wcnt[w][lane] = 0;
-    wcnt[w][lane + 64] = 0;
-    wcnt[w][lane + 128] = 0;
-    wcnt[w][lane + 192] = 0;
+#pragma unroll
+    for (int i = 0; i < ND / 64; ++i) wcnt[w][lane + 64 * i] = 0;
     __builtin_amdgcn_wave_barrier();
 #pragma unroll
     for (int j = 0; j < J; ++j) {
@@ -240,7 +242,7 @@ __device__ __forceinline__ void rank_chunk(const uint32_t (&dg)[J], const bool (
         if (!real[j]) continue;
         uint64_t peers = act;
 #pragma unroll
-        for (int b = 0; b < 8; ++b) {
+        for (int b = 0; b < DB; ++b) {
             const uint64_t m = __ballot((dg[j] >> b) & 1u);
             peers &= ((dg[j] >> b) & 1u) ? m : ~m;
         }
@@ -255,33 +257,47 @@ __device__ __forceinline__ void rank_chunk(const uint32_t (&dg)[J], const bool (
     }
     __syncthreads();
     {
-        const int d = threadIdx.x;  // one digit per thread
-        uint32_t s = 0;
+        // digits threadIdx.x * DPT .. + DPT - 1 (contiguous: the scan stays
+        // in digit order)
+        uint32_t sd[DPT], st = 0;
 #pragma unroll
-        for (int i = 0; i < GT / 64; ++i) {
-            const uint32_t x = wcnt[i][d];
-            wcnt[i][d] = s;
-            s += x;
+        for (int k = 0; k < DPT; ++k) {
+            const int d = threadIdx.x * DPT + k;
+            uint32_t s = 0;
+#pragma unroll
+            for (int i = 0; i < GT / 64; ++i) {
+                const uint32_t x = wcnt[i][d];
+                wcnt[i][d] = s;
+                s += x;
+            }
+            sd[k] = s;
+            st += s;
         }
         uint32_t tot;
-        loff[d] = block_excl_scan_256(s, lds4, &tot);
+        uint32_t pre = block_excl_scan_256(st, lds4, &tot);
+#pragma unroll
+        for (int k = 0; k < DPT; ++k) {
+            loff[threadIdx.x * DPT + k] = pre;
+            pre += sd[k];
+        }
     }
     __syncthreads();
 }
 
 // LDS sort of cnt <= GS_CAP pairs of group g: keys (local bits) and the
-// pairs' indices ping-pong between two LDS arrays.
-template <typename K>
+// pairs' indices ping-pong between two LDS arrays; digits of DB bits (9 when
+// that takes fewer passes over the lb local bits than 8).
+template <typename K, int DB>
 struct GsLds {
     K key[2][GS_CAP];
     uint16_t idx[2][GS_CAP];
-    uint32_t wcnt[GT / 64][256];
-    uint32_t loff[256];
+    uint32_t wcnt[GT / 64][1 << DB];
+    uint32_t loff[1 << DB];
     uint32_t lds4[4];
 };
 
-template <typename K>
-__device__ void group_sort_lds(GsLds<K> &S, const uint64_t *__restrict__ kin, const uint32_t *__restrict__ vin,
+template <typename K, int DB>
+__device__ void group_sort_lds(GsLds<K, DB> &S, const uint64_t *__restrict__ kin, const uint32_t *__restrict__ vin,
                                uint64_t *__restrict__ kout, uint32_t *__restrict__ vout, int64_t s0, int cnt, int lb,
                                uint64_t high)
 {
@@ -310,7 +326,7 @@ __device__ void group_sort_lds(GsLds<K> &S, const uint64_t *__restrict__ kin, co
     const int jmax = (cnt + GT - 1) / GT;
     const int pw = jmax * 64;
     int cur = 0;
-    for (int sh = 0; sh < lb; sh += 8) {
+    for (int sh = 0; sh < lb; sh += DB) {
         // only the ranks stay in registers; keys are read again from LDS for
         // the scatter
         uint32_t dg[SJ], rnk[SJ];
@@ -319,9 +335,9 @@ __device__ void group_sort_lds(GsLds<K> &S, const uint64_t *__restrict__ kin, co
         for (int j = 0; j < SJ; ++j) {
             const int p = w * pw + j * 64 + lane;
             real[j] = j < jmax && p < cnt;
-            dg[j] = real[j] ? (uint32_t)(S.key[cur][p] >> sh) & 255u : 0u;
+            dg[j] = real[j] ? (uint32_t)(S.key[cur][p] >> sh) & ((1u << DB) - 1u) : 0u;
         }
-        rank_chunk<SJ>(dg, real, jmax, rnk, S.wcnt, S.loff, S.lds4);
+        rank_chunk<SJ, DB>(dg, real, jmax, rnk, S.wcnt, S.loff, S.lds4);
 #pragma unroll
         for (int j = 0; j < SJ; ++j) {
             if (!real[j]) continue;
@@ -431,13 +447,13 @@ __device__ void group_sort_big(GsBig &S, uint64_t *__restrict__ a_k, uint32_t *_
 // One workgroup per digit (group); the last digit holds the invalid keys,
 // copied as they are.  In: (kin, vin) in group order; out: (kout, vout)
 // sorted.  gbase: ndig + 1 exclusive group offsets.
-template <typename K>
+template <typename K, int DB>
 __global__ __launch_bounds__(GT) void k_gs_sort(uint64_t *__restrict__ kin, uint32_t *__restrict__ vin,
                                                 uint64_t *__restrict__ kout, uint32_t *__restrict__ vout, int lb,
                                                 int ndig, const uint32_t *__restrict__ gbase)
 {
     __shared__ union {
-        GsLds<K> lds;
+        GsLds<K, DB> lds;
         GsBig big;
     } S;
     const int g = blockIdx.x;
@@ -452,7 +468,7 @@ __global__ __launch_bounds__(GT) void k_gs_sort(uint64_t *__restrict__ kin, uint
         return;
     }
     if (cnt <= GS_CAP) {
-        group_sort_lds<K>(S.lds, kin, vin, kout, vout, s0, (int)cnt, lb, (uint64_t)g << lb);
+        group_sort_lds<K, DB>(S.lds, kin, vin, kout, vout, s0, (int)cnt, lb, (uint64_t)g << lb);
     } else {
         group_sort_big(S.big, kin, vin, kout, vout, s0, cnt, lb);
     }
@@ -480,9 +496,14 @@ hipError_t group_sort_pairs(uint64_t *k0, uint32_t *v0, uint64_t *k1, uint32_t *
     while ((1 << dbits) < ndig) ++dbits;
     k_gs_scan<<<blocks, GS_SCAN_T, 0, s>>>(hist, nb, dtot);
     k_gs_scatter<<<(unsigned)nb, GT, 0, s>>>(k0, v0, k1, v1, n, lb, ndig, dbits, hist, nb, dtot, gbase);
-    if (lb <= 32)
-        k_gs_sort<uint32_t><<<ndig, GT, 0, s>>>(k1, v1, k0, v0, lb, ndig, gbase);
-    else
-        k_gs_sort<uint64_t><<<ndig, GT, 0, s>>>(k1, v1, k0, v0, lb, ndig, gbase);
+    // 9-bit digits where they save a pass over the lb local bits
+    const bool nine = (lb + 8) / 9 < (lb + 7) / 8;
+    if (lb <= 32) {
+        if (nine) k_gs_sort<uint32_t, 9><<<ndig, GT, 0, s>>>(k1, v1, k0, v0, lb, ndig, gbase);
+        else k_gs_sort<uint32_t, 8><<<ndig, GT, 0, s>>>(k1, v1, k0, v0, lb, ndig, gbase);
+    } else {
+        if (nine) k_gs_sort<uint64_t, 9><<<ndig, GT, 0, s>>>(k1, v1, k0, v0, lb, ndig, gbase);
+        else k_gs_sort<uint64_t, 8><<<ndig, GT, 0, s>>>(k1, v1, k0, v0, lb, ndig, gbase);
+    }
     return hipGetLastError();
 }
