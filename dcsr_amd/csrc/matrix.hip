@@ -139,27 +139,85 @@ __global__ void k_set_loaded(int64_t R, const uint32_t *__restrict__ start, cons
 }
 
 // Flat copy of every row into destination offsets (compaction / export).
-// dst_off[r] is the row's new start; src rows are (start, len).
+// dst_off[r] is the row's new start; src rows are (start, len).  Each wave
+// owns CP_SPAN destination entries, in chunks of at most 64 rows: a start
+// map of the chunk's rows (ranked to rows through `tab`: empty rows share a
+// start) gives every entry its row without a search, and all of a lane's
+// loads are in flight before its first store.
+constexpr uint32_t CP_SPAN = 1024;
+constexpr uint32_t CP_PER = CP_SPAN / 64;
 __global__ __launch_bounds__(TB) void k_copy_rows(const uint32_t *__restrict__ dst_off, uint32_t R, uint32_t total,
-                                                  uint32_t per_wave, const uint32_t *__restrict__ src_start,
+                                                  const uint32_t *__restrict__ src_start,
                                                   const int32_t *__restrict__ src_col, const float *__restrict__ src_val,
                                                   int32_t *__restrict__ dst_col, float *__restrict__ dst_val)
 {
-    __shared__ uint32_t s_off[TB / 64][65];
-    const int w = threadIdx.x >> 6;
-    const uint64_t wave = (uint64_t)blockIdx.x * (TB / 64) + w;
+    struct Lds {
+        uint32_t off[65];
+        uint32_t src[64];
+        uint32_t pmap[CP_SPAN / 32];
+        uint32_t tab[64];
+    };
+    __shared__ Lds sh[TB / 64];
+    Lds &S = sh[threadIdx.x >> 6];
+    const uint32_t lane = (uint32_t)lane_id();
     const uint32_t dev_total = dst_off[R];  // the offsets decide, not the caller's count
     if (total > dev_total) total = dev_total;
-    uint64_t e0 = wave * per_wave;
-    if (e0 >= total) return;
-    uint64_t e1 = e0 + per_wave < total ? e0 + per_wave : total;
-    wave_visit(dst_off, R, (uint32_t)e0, (uint32_t)e1, s_off[w], [&](bool valid, uint32_t r, uint32_t k, uint32_t e) {
-        if (valid) {
-            uint64_t sp = (uint64_t)src_start[r] + k;
-            dst_col[e] = src_col[sp];
-            dst_val[e] = src_val[sp];
+    const uint64_t nwaves = (uint64_t)gridDim.x * (TB / 64);
+    for (uint64_t wave = (uint64_t)blockIdx.x * (TB / 64) + (threadIdx.x >> 6); wave * CP_SPAN < total;
+         wave += nwaves) {
+        const uint32_t e0 = (uint32_t)(wave * CP_SPAN);
+        const uint32_t e1 = total - e0 < CP_SPAN ? total : e0 + CP_SPAN;
+        uint32_t j = wave_find_last_le(dst_off, R, e0);
+        uint32_t e = e0;
+        while (e < e1) {
+            const uint32_t t = j + lane;
+            const uint32_t off_t = t <= R ? dst_off[t] : 0xffffffffu;
+            const uint32_t off_64 = j + 64u <= R ? dst_off[j + 64u] : 0xffffffffu;
+            const uint32_t src = t < R ? src_start[t] : 0u;
+            S.off[lane] = off_t;
+            if (lane == 0) S.off[64] = off_64;
+            S.src[lane] = src;
+            if (lane < CP_SPAN / 32) S.pmap[lane] = 0u;
+            __builtin_amdgcn_wave_barrier();
+            const uint32_t wend = S.off[64] < e1 ? S.off[64] : e1;
+            const uint32_t cnt = wend - e;
+            const int j0 = find_seg64(S.off, e);  // the row holding e
+            // rows with entries starting in the chunk
+            const bool starts_here = t < R && off_t >= e && off_t < wend && S.off[lane + 1] > off_t;
+            if (starts_here) atomicOr(&S.pmap[(off_t - e) >> 5], 1u << ((off_t - e) & 31u));
+            __builtin_amdgcn_wave_barrier();
+            const StartMap pm = start_map_load(S.pmap, CP_SPAN / 64);
+            const uint32_t rank = start_map_rank(pm, starts_here ? off_t - e : 0u);  // every lane takes part
+            if (starts_here) S.tab[rank] = lane;
+            __builtin_amdgcn_wave_barrier();
+            int32_t c[CP_PER];
+            float v[CP_PER];
+#pragma unroll
+            for (uint32_t q = 0; q < CP_PER; ++q) {
+                const uint32_t p = q * 64u + lane;
+                const uint32_t nst = start_map_count(pm, (int)q);
+                c[q] = 0;
+                v[q] = 0.0f;
+                if (p < cnt) {
+                    const uint32_t jr = nst == 0u ? (uint32_t)j0 : S.tab[nst - 1u];
+                    const uint64_t sp = (uint64_t)S.src[jr] + (e + p - S.off[jr]);
+                    c[q] = src_col[sp];
+                    v[q] = src_val[sp];
+                }
+            }
+#pragma unroll
+            for (uint32_t q = 0; q < CP_PER; ++q) {
+                const uint32_t p = q * 64u + lane;
+                if (p < cnt) {
+                    dst_col[e + p] = c[q];
+                    dst_val[e + p] = v[q];
+                }
+            }
+            __builtin_amdgcn_wave_barrier();
+            e = wend;
+            j += 64;
         }
-    });
+    }
 }
 
 __global__ void k_len_u32(const int32_t *__restrict__ len, int64_t R, uint32_t *__restrict__ out)
@@ -665,11 +723,11 @@ void launch_copy_rows(const uint32_t *dst_off, int64_t R, int64_t total, const u
                              const int32_t *scol, const float *sval, int32_t *dcol, float *dval, hipStream_t s)
 {
     if (total <= 0 || R <= 0) return;
-    const uint32_t per_wave = 1024;
-    int64_t waves = (total + per_wave - 1) / per_wave;
-    unsigned blocks = (unsigned)((waves + 3) / 4);
-    k_copy_rows<<<blocks, TB, 0, s>>>(dst_off, (uint32_t)R, (uint32_t)total, per_wave, src_start, scol, sval, dcol,
-                                      dval);
+    const int64_t waves = (total + CP_SPAN - 1) / CP_SPAN;
+    int64_t blocks = (waves + 3) / 4;
+    if (blocks > 2048) blocks = 2048;  // waves stride over the rest
+    k_copy_rows<<<(unsigned)blocks, TB, 0, s>>>(dst_off, (uint32_t)R, (uint32_t)total, src_start, scol, sval, dcol,
+                                                dval);
 }
 
 dcsr_status dcsr_export_csr(dcsr_handle h, int32_t *row_offsets, int32_t *cols, float *vals, int64_t cap,
