@@ -229,13 +229,14 @@ def bench_powerlaw(args):
 
     # warmup on the same handle (first-touch workspace allocation happens
     # here), then reload the initial matrix: the timed steps start from it
-    # pool: 16 x the initial 3.2M entries (410 MB of HBM).  Every batch
-    # rewrites its touched rows (~85% of the live entries, hub rows included)
-    # into fresh chunks, so the old chunks pile up as garbage; at 16x the bump
-    # region holds a few batches between compactions at every size this run
-    # reaches (13.2M live entries after 20 batches), where 8x compacts before
-    # nearly every batch once the matrix passes ~8M entries.
-    m = dcsr_amd.DcsrMatrix(R_CFG, R_CFG, pool_factor=16.0)
+    # pool: 64 x the initial 3.2M entries (1.6 GB of the 288 GB of HBM).
+    # Every batch rewrites its touched rows (~85% of the live entries, hub
+    # rows included) into fresh chunks, so the old chunks pile up as garbage
+    # until an in-order compaction (about 40 us at 13M entries) reclaims
+    # them.  At 64x the 20-step run (13.2M live entries at its end) compacts
+    # about once; at 16x it compacted every two to three batches past ~10M
+    # entries (0.333 vs 0.315 ms per step).
+    m = dcsr_amd.DcsrMatrix(R_CFG, R_CFG, pool_factor=64.0)
     m.LoadMatrix(d_r, d_c, d_v)
     for k in range(Wm):
         m.UpdateMatrix(*d_b[K + k], flags=dcsr_amd.UPDATE_NOSYNC)
@@ -249,43 +250,58 @@ def bench_powerlaw(args):
         m.BinRows()
         torch.cuda.synchronize()
 
-    def timed(flags, with_info):
-        ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True),
-               torch.cuda.Event(enable_timing=True)) for _ in range(K)]
+    def timed(flags, split, with_info):
+        # split: events before the update, between update and SpMV and after
+        # the SpMV of every step; otherwise one event per step boundary (each
+        # event costs the stream a few microseconds: 3 per step measured
+        # 0.333 ms per step where 1 per step measured 0.325)
+        nev = 3 * K if split else K + 1
+        ev = [torch.cuda.Event(enable_timing=True) for _ in range(nev)]
         stats = []
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         for k in range(K):
-            e0, e1, e2 = ev[k]
-            e0.record(stream)
+            ev[3 * k if split else k].record(stream)
             m.UpdateMatrix(*d_b[k], flags=flags)
-            e1.record(stream)
+            if split:
+                ev[3 * k + 1].record(stream)
             m.spmv(d_x, d_y)
-            e2.record(stream)
+            if split:
+                ev[3 * k + 2].record(stream)
             if with_info:
                 stats.append(m.info())
+        if not split:
+            ev[K].record(stream)
         torch.cuda.synchronize()
         return time.perf_counter() - t0, ev, stats
 
     # timed: the asynchronous update (as the reference's UpdateMatrix is a
     # bare launch): no host read-back per step, pool room from host bounds;
-    # the batch status is read once after the timed steps
+    # the batch status is read once after the timed steps.  The throughput
+    # and step-latency pass records one event per step boundary; a second
+    # pass of the same steps records the update / SpMV split.
     reload()
-    elapsed, ev, _ = timed(dcsr_amd.UPDATE_NOSYNC, False)
+    compactions0 = m.info()["compactions"]
+    elapsed, ev_b, _ = timed(dcsr_amd.UPDATE_NOSYNC, False, False)
     if m.status() != 0:
         raise RuntimeError(f"update status {m.status()} after the timed steps")
+    compactions = m.info()["compactions"] - compactions0
     y_async = d_y.clone()
+    reload()
+    elapsed_split, ev, _ = timed(dcsr_amd.UPDATE_NOSYNC, True, False)
+    if m.status() != 0 or not torch.equal(d_y, y_async):
+        raise RuntimeError("the split-timed pass differs from the throughput pass")
     # the same steps through the synchronous call (status per call), also
     # timed; its per-call counters give the algorithmic bytes, and its last
     # SpMV must equal the asynchronous run's bit for bit
     reload()
-    elapsed_sync, _, stats = timed(0, True)
+    elapsed_sync, _, stats = timed(0, False, True)
     if not torch.equal(d_y, y_async):
         raise RuntimeError("asynchronous and synchronous runs differ")
 
-    upd_ms = np.array([e0.elapsed_time(e1) for e0, e1, _ in ev])
-    spmv_ms = np.array([e1.elapsed_time(e2) for _, e1, e2 in ev])
-    step_ms = np.array([e0.elapsed_time(e2) for e0, _, e2 in ev])
+    upd_ms = np.array([ev[3 * k].elapsed_time(ev[3 * k + 1]) for k in range(K)])
+    spmv_ms = np.array([ev[3 * k + 1].elapsed_time(ev[3 * k + 2]) for k in range(K)])
+    step_ms = np.array([ev_b[k].elapsed_time(ev_b[k + 1]) for k in range(K)])
     trip = K * args.batch
     n_ins = sum(s["last_inserted"] for s in stats)
     nnz_after = [s["nnz"] for s in stats]
@@ -329,6 +345,8 @@ def bench_powerlaw(args):
          mean_update_ms=float(upd_ms.mean()),
          mean_spmv_ms=float(spmv_ms.mean()),
          update_call="DCSR_UPDATE_NOSYNC, status read once after the timed steps",
+         split_pass_ms_per_step=elapsed_split * 1e3 / K,
+         compactions_in_timed_steps=compactions,
          sync_call_triples_per_s=trip / elapsed_sync,
          sync_call_ms_per_step=elapsed_sync * 1e3 / K,
          roofline=roof,

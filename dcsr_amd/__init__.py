@@ -66,7 +66,8 @@ class DcsrInfo(ctypes.Structure):
         "rows", "cols", "nnz", "pool_cap", "pool_used", "garbage", "hub_rows",
         "last_inserted", "last_added", "last_deleted", "last_touched_rows")] + \
         [("canonical", ctypes.c_int32), ("max_segments", ctypes.c_int32),
-         ("scan_fallback", ctypes.c_int32), ("spmv_paneled", ctypes.c_int32)]
+         ("scan_fallback", ctypes.c_int32), ("spmv_paneled", ctypes.c_int32),
+         ("compactions", ctypes.c_int64)]
 
     def as_dict(self):
         return {n: getattr(self, n) for n, _ in self._fields_}

@@ -68,6 +68,12 @@ enum WsSlot {
     WS_COUNT
 };
 
+// A NOSYNC update's state snapshot, stored by its write kernel (seq last).
+struct HostSnap {
+    uint64_t nnz, bump, seq;
+};
+constexpr int SNAP_RING = 64;
+
 struct Workspace {
     void *ptr[WS_COUNT];
     size_t bytes[WS_COUNT];
@@ -102,16 +108,24 @@ struct dcsr_matrix_s {
     // entries and claims at most its claim bound): they size SpMV grids and
     // decide pool room without reading the device state back
     int64_t nnz_ub, bump_ub;
-    // asynchronous state snapshot taken after each NOSYNC update: once its
-    // event has completed, the bounds restart from its exact values plus what
-    // was launched after it (nnz_since, bump_since)
-    DevState *snap;          // pinned
-    hipEvent_t snap_ev;
-    bool snap_pending;       // a copy is in flight (at most one at a time)
-    bool snap_stale;         // the state changed outside the counters since
-    int64_t nnz_since, bump_since;
+    // state snapshot of each NOSYNC update: its write kernel stores the live
+    // count and bump pointer it leaves, then the update's sequence number,
+    // into coherent host memory (no copy, no event).  The host keeps the
+    // running sums of the bounds it added per update (cum_*, and their
+    // values after update k in ring slot k % SNAP_RING), so a landed
+    // snapshot k bounds the current state by its values plus what was added
+    // after k.  Snapshots before snap_valid_from predate a change the sums
+    // do not count (a read-back, load or compaction) and are ignored.
+    HostSnap *hsnap;         // host view (pinned, coherent)
+    HostSnap *dsnap;         // the same memory as the device addresses it
+    uint64_t snap_seq;       // last sequence number handed to an update
+    uint64_t snap_valid_from;
+    uint64_t snap_used;      // last snapshot folded into the bounds
+    int64_t cum_add, cum_claim;
+    int64_t ring_add[SNAP_RING], ring_claim[SNAP_RING];
     bool canonical;
     bool hubs_dirty;
+    int64_t ncompact;        // compactions and growths (dcsr_query reports it)
     bool flat_dirty;         // flat row offsets (SpMV binning) need a rebuild
     bool state_known;        // host mirrors valid (no NOSYNC update since last sync)
     // column-paneled copy for SpMV on matrices whose x exceeds the L2 (see
@@ -160,7 +174,8 @@ void state_unknown(dcsr_handle h, int64_t added, int64_t claimed);
 void state_exact(dcsr_handle h);
 // Enqueue an asynchronous copy of the device state (NOSYNC paths), and fold
 // a completed one into the bounds (never waits).
-dcsr_status snapshot_state(dcsr_handle h, hipStream_t s);
+uint64_t snapshot_next(dcsr_handle h);
+void snapshot_note(dcsr_handle h, uint64_t seq);
 void refresh_bounds(dcsr_handle h);
 // Upper bounds of the live entries and of the bump pointer (exact when the
 // state is known).

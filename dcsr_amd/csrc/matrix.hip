@@ -408,7 +408,7 @@ dcsr_status pull_state(dcsr_handle h, hipStream_t s)
 void state_exact(dcsr_handle h)
 {
     h->state_known = true;
-    h->snap_stale = true;
+    h->snap_valid_from = h->snap_seq + 1;
 }
 
 void state_unknown(dcsr_handle h, int64_t added, int64_t claimed)
@@ -418,37 +418,40 @@ void state_unknown(dcsr_handle h, int64_t added, int64_t claimed)
     // rejects a batch whose claims would
     h->nnz_ub = nnz < h->pool_cap ? nnz : h->pool_cap;
     h->bump_ub = bump < h->pool_cap ? bump : h->pool_cap;
-    h->nnz_since += added;
-    h->bump_since += claimed;
+    h->cum_add += added;
+    h->cum_claim += claimed;
     h->state_known = false;
     h->hubs_dirty = true;
 }
 
-// At most one snapshot copy is in flight: a new one is enqueued only after
-// the last one's event completed, so the buffer is never written twice at
-// once, whatever streams the calls use.
-dcsr_status snapshot_state(dcsr_handle h, hipStream_t s)
+// The next NOSYNC update's sequence number (its write kernel stores the
+// snapshot under it).
+uint64_t snapshot_next(dcsr_handle h) { return ++h->snap_seq; }
+
+// After update seq's launches and its state_unknown: the running sums that
+// its snapshot's values are to be read against.
+void snapshot_note(dcsr_handle h, uint64_t seq)
 {
-    if (h->snap_pending) return DCSR_OK;
-    if (!h->snap_ev) DCSR_HIP(hipEventCreateWithFlags(&h->snap_ev, hipEventDisableTiming));
-    DCSR_HIP(hipMemcpyAsync(h->snap, h->ds, sizeof(DevState), hipMemcpyDeviceToHost, s));
-    DCSR_HIP(hipEventRecord(h->snap_ev, s));
-    h->snap_pending = true;
-    h->snap_stale = false;
-    h->nnz_since = 0;
-    h->bump_since = 0;
-    return DCSR_OK;
+    h->ring_add[seq % SNAP_RING] = h->cum_add;
+    h->ring_claim[seq % SNAP_RING] = h->cum_claim;
 }
 
-// The snapshot's values plus the NOSYNC launches counted since it bound the
-// current state, unless the state changed in some other way meanwhile
-// (stale): then the snapshot is only retired.
+// The newest landed snapshot k bounds the state: its values plus the sums
+// added after update k.  The write kernel stores the values before k (a
+// release at system scope), so values read after k was seen are those of k
+// or of a later update j, whose sums after j are at least k's: the bound
+// holds either way.  A snapshot older than the ring, or than the last change
+// the sums do not count, is skipped.
 void refresh_bounds(dcsr_handle h)
 {
-    if (!h->snap_pending || hipEventQuery(h->snap_ev) != hipSuccess) return;
-    h->snap_pending = false;
-    if (h->snap_stale || h->state_known) return;
-    const int64_t nnz = (int64_t)h->snap->nnz + h->nnz_since, bump = (int64_t)h->snap->bump + h->bump_since;
+    const uint64_t k = __atomic_load_n(&h->hsnap->seq, __ATOMIC_ACQUIRE);
+    if (k == h->snap_used || k < h->snap_valid_from || k > h->snap_seq || k + SNAP_RING <= h->snap_seq) return;
+    h->snap_used = k;
+    if (h->state_known) return;
+    const int64_t nnz = (int64_t)__atomic_load_n(&h->hsnap->nnz, __ATOMIC_RELAXED) +
+                        (h->cum_add - h->ring_add[k % SNAP_RING]);
+    const int64_t bump = (int64_t)__atomic_load_n(&h->hsnap->bump, __ATOMIC_RELAXED) +
+                         (h->cum_claim - h->ring_claim[k % SNAP_RING]);
     if (nnz < h->nnz_ub) h->nnz_ub = nnz;
     if (bump < h->bump_ub) h->bump_ub = bump;
 }
@@ -531,11 +534,17 @@ dcsr_status dcsr_create(dcsr_handle *out, int64_t rows, int64_t cols, const dcsr
     if (st == DCSR_OK) st = alloc_pool(h, h->cfg.min_pool);
     if (st == DCSR_OK && hipMalloc(&h->ds, sizeof(DevState)) != hipSuccess) st = DCSR_ERR_OOM;
     if (st == DCSR_OK && hipHostMalloc(&h->hs, sizeof(DevState), hipHostMallocDefault) != hipSuccess) st = DCSR_ERR_OOM;
-    if (st == DCSR_OK && hipHostMalloc(&h->snap, sizeof(DevState), hipHostMallocDefault) != hipSuccess)
+    // the NOSYNC snapshot: coherent, so the write kernel's stores reach the
+    // host without a copy
+    if (st == DCSR_OK &&
+        hipHostMalloc(&h->hsnap, sizeof(HostSnap), hipHostMallocMapped | hipHostMallocCoherent) != hipSuccess)
         st = DCSR_ERR_OOM;
+    if (st == DCSR_OK) {
+        memset(h->hsnap, 0, sizeof(HostSnap));
+        h->snap_valid_from = 1;
+        if (hipHostGetDevicePointer((void **)&h->dsnap, h->hsnap, 0) != hipSuccess) st = DCSR_ERR_HIP;
+    }
     if (st == DCSR_OK && hipMemset(h->ds, 0, sizeof(DevState)) != hipSuccess) st = DCSR_ERR_HIP;
-    // the NOSYNC snapshot's event, made here so that no update call creates one
-    if (st == DCSR_OK && hipEventCreateWithFlags(&h->snap_ev, hipEventDisableTiming) != hipSuccess) st = DCSR_ERR_HIP;
     if (st == DCSR_OK) {
         // all rows empty
         k_init_rows<<<grid_for(rows, TB), TB>>>(rows, 0, h->row_start, h->row_len, h->row_cap, h->ds);
@@ -564,8 +573,7 @@ dcsr_status dcsr_destroy(dcsr_handle h)
     release_retired(h);
     (void)hipFree(h->ds);
     if (h->hs) (void)hipHostFree(h->hs);
-    if (h->snap) (void)hipHostFree(h->snap);
-    if (h->snap_ev) (void)hipEventDestroy(h->snap_ev);
+    if (h->hsnap) (void)hipHostFree(h->hsnap);
     for (int i = 0; i < WS_COUNT; ++i)
         if (h->ws.ptr[i]) (void)hipFree(h->ws.ptr[i]);
     free(h);
@@ -828,6 +836,7 @@ static dcsr_status compact_into(dcsr_handle h, int64_t new_cap, hipStream_t s)
         h->spare_cap = new_cap;
     }
     launch_copy_rows(off, R, nnz, h->row_start, h->pool_col, h->pool_val, h->spare_col, h->spare_val, s);
+    ++h->ncompact;
     if (R > 0) k_set_compacted<<<grid_for(R, TB), TB, 0, s>>>(R, off, h->row_start, h->row_len, h->row_cap, h->ds, off + R);
     else k_set_compacted<<<1, 1, 0, s>>>(0, off, h->row_start, h->row_len, h->row_cap, h->ds, off);
     DCSR_HIP(hipGetLastError());
@@ -847,7 +856,7 @@ static dcsr_status compact_into(dcsr_handle h, int64_t new_cap, hipStream_t s)
         // the device's bump pointer is now its live count; a snapshot in
         // flight predates the swap
         h->bump_ub = h->nnz_ub;
-        h->snap_stale = true;
+        h->snap_valid_from = h->snap_seq + 1;
     }
     return DCSR_OK;
 }
@@ -946,6 +955,7 @@ dcsr_status dcsr_query(dcsr_handle h, dcsr_info *info)
     info->max_segments = 1;
     info->scan_fallback = (h->hs->status & ST_SCAN_FALLBACK) ? 1 : 0;
     info->spmv_paneled = h->pnl_valid ? 1 : 0;
+    info->compactions = h->ncompact;
     return DCSR_OK;
 }
 

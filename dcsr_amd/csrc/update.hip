@@ -1240,6 +1240,16 @@ __global__ __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(4))) void k_
     }
 }
 
+// A NOSYNC update's state snapshot into coherent host memory: the values,
+// then the sequence number with a release at system scope, so a host that
+// reads seq k then reads values of update k or later.
+__device__ __forceinline__ void snap_publish(HostSnap *p, unsigned long long nnz, unsigned long long bump, uint64_t seq)
+{
+    __hip_atomic_store(&p->nnz, (uint64_t)nnz, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+    __hip_atomic_store(&p->bump, (uint64_t)bump, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+    __hip_atomic_store(&p->seq, seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+}
+
 // Inserted entries, in-place value updates, row-table commit.  A rejected
 // batch writes nothing; with `report` (a caller that does not read the
 // decision back) its cause becomes a sticky status bit.  The per-coordinate
@@ -1255,7 +1265,8 @@ __global__ __launch_bounds__(TB) void k_upd_write(DevState *ds, uint32_t report,
                                                   const uint32_t *__restrict__ ucode,
                                                   const uint32_t *__restrict__ upre, int32_t *__restrict__ pcol,
                                                   float *__restrict__ pval, uint32_t *__restrict__ row_start,
-                                                  int32_t *__restrict__ row_len, int32_t *__restrict__ row_cap)
+                                                  int32_t *__restrict__ row_len, int32_t *__restrict__ row_cap,
+                                                  HostSnap *snap, uint64_t seq)
 {
     const int64_t u = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     uint32_t t = 0, code = C_NOP, pos = 0, ui = 0, ud = 0;
@@ -1272,12 +1283,15 @@ __global__ __launch_bounds__(TB) void k_upd_write(DevState *ds, uint32_t report,
     }
     if (!ds->go) {
         if (report && u == 0) ds->status |= ds->ubad_last ? ST_BAD_INDEX : ST_POOL_EXHAUSTED;
+        if (snap && u == 0) snap_publish(snap, ds->nnz, ds->bump, seq);
         return;
     }
     const uint32_t nu = ds->nuniq;
     if (u == 0) {
-        ds->nnz += ds->d_nnz;
+        const long long nnz = ds->nnz + ds->d_nnz;
+        ds->nnz = nnz;
         ds->garbage += ds->d_garbage;
+        if (snap) snap_publish(snap, (unsigned long long)nnz, ds->bump, seq);
     }
     if (u >= nu) return;
     const uint32_t mode = tmode[t];
@@ -1518,7 +1532,7 @@ dcsr_status prepare_launch(dcsr_handle h, const int32_t *rows, const int32_t *co
 // past the device's count exit at once).
 constexpr int64_t MV_BLOCKS = 1024;
 dcsr_status commit_launch(dcsr_handle h, const UpdBufs &b, int64_t n, int64_t move_bound, bool report,
-                          hipStream_t s)
+                          hipStream_t s, uint64_t seq = 0)
 {
     const int64_t waves = (move_bound + MV_SPAN - 1) / MV_SPAN;
     int64_t blocks = (waves + 3) / 4;
@@ -1528,7 +1542,7 @@ dcsr_status commit_launch(dcsr_handle h, const UpdBufs &b, int64_t n, int64_t mo
                                                h->pool_val, b.mspan);
     k_upd_write<<<grid_for(n, TB), TB, 0, s>>>(h->ds, report ? 1u : 0u, n, b.ut, b.trow, b.tubeg, b.tpre, b.tnewl, b.tmode, b.ucol, b.uval,
                                                 b.upos, b.ucode, b.upre, h->pool_col, h->pool_val, h->row_start,
-                                                h->row_len, h->row_cap);
+                                                h->row_len, h->row_cap, seq ? h->dsnap : nullptr, seq);
     DCSR_HIP(hipGetLastError());
     h->flat_dirty = true;
     return DCSR_OK;
@@ -1536,10 +1550,11 @@ dcsr_status commit_launch(dcsr_handle h, const UpdBufs &b, int64_t n, int64_t mo
 
 // Sort through commit of a packed batch of n triples (the commit kernels
 // skip themselves when the plan rejected the batch; `report`: see k_upd_write).
-dcsr_status pipeline_launch(dcsr_handle h, int64_t n, UpdBufs &b, bool report, hipStream_t s)
+// seq: a NOSYNC update's snapshot sequence number (0: no snapshot).
+dcsr_status pipeline_launch(dcsr_handle h, int64_t n, UpdBufs &b, bool report, hipStream_t s, uint64_t seq = 0)
 {
     DCSR_TRY(sort_plan_launch(h, n, b, s));
-    return commit_launch(h, b, n, h->pool_cap, report, s);
+    return commit_launch(h, b, n, h->pool_cap, report, s, seq);
 }
 
 
@@ -1710,9 +1725,11 @@ static dcsr_status update_nosync(dcsr_handle h, const int32_t *rows, const int32
     DCSR_TRY(upd_bufs(h, n, b, s));
     launch_pack(h, rows, cols, vals, n, b, s);
     DCSR_HIP(hipGetLastError());
-    DCSR_TRY(pipeline_launch(h, n, b, true, s));
+    const uint64_t seq = snapshot_next(h);
+    DCSR_TRY(pipeline_launch(h, n, b, true, s, seq));
     state_unknown(h, n, claimed);
-    return snapshot_state(h, s);
+    snapshot_note(h, seq);
+    return DCSR_OK;
 }
 
 namespace {
@@ -1751,10 +1768,13 @@ dcsr_status update_fixed_scratch(dcsr_handle h, const int32_t *rows, const int32
     if (nosync) refresh_bounds(h);
     launch_pack(h, rows, cols, vals, n, b, s);
     DCSR_HIP(hipGetLastError());
-    DCSR_TRY(pipeline_launch(h, n, b, nosync, s));
+    const uint64_t seq = nosync ? snapshot_next(h) : 0;
+    const int64_t claimed = claims_bound(h, n);
+    DCSR_TRY(pipeline_launch(h, n, b, nosync, s, seq));
     if (nosync) {
-        state_unknown(h, n, claims_bound(h, n));
-        return snapshot_state(h, s);
+        state_unknown(h, n, claimed);
+        snapshot_note(h, seq);
+        return DCSR_OK;
     }
     DCSR_TRY(pull_state(h, s));
     if (h->hs->ubad_last) return DCSR_ERR_BAD_INDEX;

@@ -93,6 +93,8 @@ typedef struct dcsr_info {
                               (slow) fallback: results stay correct; 0 in a
                               healthy run */
     int32_t spmv_paneled; /* 1 while SpMV runs over the column-paneled copy */
+    int64_t compactions;  /* pool compactions and growths on this handle, the
+                             explicit dcsr_compact included */
 } dcsr_info;
 
 const char *dcsr_status_string(dcsr_status s);
