@@ -64,7 +64,7 @@ enum WsSlot {
     WS_SG_HPRE, WS_SG_EA, WS_SG_APOS, WS_SG_KEY0, WS_SG_KEY1, WS_SG_VAL0, WS_SG_VAL1, WS_SG_FLAG, WS_SG_EPOS,
     WS_CANON, WS_TMP0, WS_TMP1, WS_TMP2,
     WS_PNL_OFF, WS_PNL_BND, WS_PNL_COL, WS_PNL_VAL, WS_PNL_MAP, WS_PNL_PART,
-    WS_TRS, WS_TLEN, WS_PSPAN, WS_MSPAN,
+    WS_TRS, WS_TLEN, WS_PSPAN, WS_MSPAN, WS_TCAP,
     WS_COUNT
 };
 

@@ -20,8 +20,8 @@
 namespace {
 
 constexpr int64_t NOALLOC_MAX_TRIPLES = int64_t(1) << 16;
-// covers every shape at NOALLOC_MAX_TRIPLES (about 6.6 MB are used: 96 bytes
-// per triple plus the row-group tables); checked per call
+// covers every shape at NOALLOC_MAX_TRIPLES (about 6.9 MB are used: 100
+// bytes per triple plus the row-group tables); checked per call
 constexpr size_t NOALLOC_SCRATCH_BYTES = size_t(8) << 20;
 constexpr int NOALLOC_MAX_DEVICES = 64;
 

@@ -55,7 +55,10 @@ __global__ void k_rowmap(uint32_t R, const uint32_t *__restrict__ foff, uint32_t
 // matrix of a few hundred thousand rows has under a hundred tiles, so the
 // cross-tile look-back (lookback.h) resolves in one or two rounds.  Measured
 // on 325K rows against 8 and 32 rows per thread: 55 vs 57 and 59 us per
-// SpMV after an update.
+// SpMV after an update.  The tile's granules are filled by the whole
+// workgroup, one granule per thread and a search of the tile's offsets in
+// LDS, so that a long row does not leave one thread storing its granules
+// one by one.
 constexpr int SR_IPT = 16;
 constexpr int SR_TILE = TB * SR_IPT;
 __global__ __launch_bounds__(TB) void k_scan_rows(uint32_t R, const int32_t *__restrict__ len,
@@ -63,6 +66,7 @@ __global__ __launch_bounds__(TB) void k_scan_rows(uint32_t R, const int32_t *__r
                                                   uint32_t *__restrict__ part, unsigned int *fault, int limit)
 {
     __shared__ uint32_t lds4[4], s_tk[2], s_pre;
+    __shared__ uint32_t s_off[SR_TILE + 1];
     unsigned int *ctl = part;
     uint64_t *st = (uint64_t *)(part + 4);
     uint32_t tag;
@@ -102,14 +106,34 @@ __global__ __launch_bounds__(TB) void k_scan_rows(uint32_t R, const int32_t *__r
         }
     }
     __syncthreads();
+    const int64_t tr0 = tile * SR_TILE;
+    const uint32_t n = (uint32_t)((int64_t)R - tr0 < SR_TILE ? (int64_t)R - tr0 : SR_TILE);  // rows in the tile
     uint32_t a = s_pre + pre;
     uint32_t o[SR_IPT];
 #pragma unroll
     for (int j = 0; j < SR_IPT; ++j) {
         o[j] = a;
-        const uint32_t b = a + v[j];
-        for (uint32_t g = a / GRAN + 1; g <= b / GRAN; ++g) rowmap[g] = (uint32_t)(r0 + j);
-        a = b;
+        const uint32_t lj = threadIdx.x * SR_IPT + j;
+        if (lj < n) s_off[lj] = a;
+        a += v[j];
+    }
+    if (threadIdx.x == 0) s_off[n] = s_pre + tot;
+    __syncthreads();
+    {
+        // granules g with s_off[0] < g*GRAN <= s_off[n]: the tile's row i
+        // with s_off[i] < g*GRAN <= s_off[i+1] (the last i with s_off[i] <
+        // g*GRAN, found by bisection)
+        const uint32_t g0 = s_off[0] / GRAN + 1, g1 = s_off[n] / GRAN;
+        for (uint32_t g = g0 + threadIdx.x; g <= g1; g += TB) {
+            const uint32_t x = g * GRAN;
+            uint32_t l = 0, hgh = n - 1;
+            while (l < hgh) {
+                const uint32_t mid = (l + hgh + 1) >> 1;
+                if (s_off[mid] < x) l = mid;
+                else hgh = mid - 1;
+            }
+            rowmap[g] = (uint32_t)(tr0 + l);
+        }
     }
     if (full) {
         uint4 *q = (uint4 *)(foff + r0);

@@ -303,7 +303,7 @@ __global__ __launch_bounds__(TB) void k_upd_heads_count(const uint64_t *__restri
 
 // Pass 2 of the heads: every head writes its compacted record: ucol/usum/
 // ut for coordinate u, trow/tubeg/tlpre (probe-length offset), the row's
-// pool start and length (trs, tlen) and zeroed tins/tdel for touched row t,
+// pool start, length and capacity (trs, tlen, tcap) and zeroed tins/tdel for touched row t,
 // and (pspan) touched row t for every probe span that starts in it.  Sums run left to right from 0 in batch order, as the host path
 // accumulates B_row[j]; a run is summed from the thread's registers and
 // only a run that continues past the thread's items reads memory.  A tile's
@@ -317,12 +317,14 @@ __global__ __launch_bounds__(TB) void k_upd_heads_emit(const uint64_t *__restric
                                                        uint32_t *__restrict__ ut, uint32_t *__restrict__ trow,
                                                        uint32_t *__restrict__ tubeg, const int32_t *__restrict__ row_len,
                                                        const uint32_t *__restrict__ row_start,
+                                                       const int32_t *__restrict__ row_cap,
                                                        const uint32_t *__restrict__ tp, uint32_t *__restrict__ tlpre,
                                                        uint32_t *__restrict__ trs, uint32_t *__restrict__ tlen,
+                                                       uint32_t *__restrict__ tcap,
                                                        uint32_t *__restrict__ tins, uint32_t *__restrict__ tdel,
                                                        uint32_t *__restrict__ pspan)
 {
-    __shared__ uint32_t lds4[4];
+    __shared__ uint32_t lds3[4][3];
     __shared__ int32_t s_ucol[TILE];
     __shared__ float s_usum[TILE];
     __shared__ uint32_t s_ut[TILE], s_trow[TILE], s_tub[TILE], s_tl[TILE];
@@ -352,27 +354,32 @@ __global__ __launch_bounds__(TB) void k_upd_heads_emit(const uint64_t *__restric
     for (int j = 0; j < IPT; ++j) vb[j] = val[i0 + j < n ? i0 + j : n - 1];
     const HeadFlags f = head_flags(k, prev, cbits);
     int32_t lj[IPT];
-    uint32_t rsj[IPT];
+    uint32_t rsj[IPT], rcj[IPT];
 #pragma unroll
-    for (int j = 0; j < IPT; ++j) rsj[j] = (f.r >> j) & 1u ? row_start[(uint32_t)(k[j] >> cbits)] : 0u;
+    for (int j = 0; j < IPT; ++j) {
+        const bool rh = (f.r >> j) & 1u;
+        rsj[j] = rh ? row_start[(uint32_t)(k[j] >> cbits)] : 0u;
+        rcj[j] = rh ? (uint32_t)row_cap[(uint32_t)(k[j] >> cbits)] : 0u;
+    }
     const uint32_t ls = head_rows_len(k, f.r, cbits, row_len, lj);
     if (!table) {
-        uint32_t sh = 0, sr = 0, sl = 0;
+        uint32_t sb[3] = {0, 0, 0};
 #pragma unroll
         for (int q = 0; q < HEADS_DIRECT_TILES / TB; ++q) {
             const bool in = (int64_t)threadIdx.x + q * TB < tile;
-            sh += in ? ch[q] : 0u;
-            sr += in ? cr[q] : 0u;
-            sl += in ? cl[q] : 0u;
+            sb[0] += in ? ch[q] : 0u;
+            sb[1] += in ? cr[q] : 0u;
+            sb[2] += in ? cl[q] : 0u;
         }
-        ubase = block_sum_u32(sh, lds4);
-        tbase = block_sum_u32(sr, lds4);
-        lbase = block_sum_u32(sl, lds4);
+        block_sum_n<3>(sb, lds3);
+        ubase = sb[0];
+        tbase = sb[1];
+        lbase = sb[2];
     }
-    uint32_t htot, rtot, ltot;
-    uint32_t lu = block_excl_scan_256(__popc(f.h), lds4, &htot);
-    uint32_t lt = block_excl_scan_256(__popc(f.r), lds4, &rtot);
-    uint32_t ll = block_excl_scan_256(ls, lds4, &ltot) + lbase;
+    uint32_t sc[3] = {(uint32_t)__popc(f.h), (uint32_t)__popc(f.r), ls}, stot[3];
+    block_excl_scan_n<3>(sc, lds3, stot);
+    const uint32_t htot = stot[0], rtot = stot[1], ltot = stot[2];
+    uint32_t lu = sc[0], lt = sc[1], ll = sc[2] + lbase;
 #pragma unroll
     for (int j = 0; j < IPT; ++j) {
         if (!((f.h >> j) & 1u)) continue;
@@ -402,6 +409,7 @@ __global__ __launch_bounds__(TB) void k_upd_heads_emit(const uint64_t *__restric
             ll += probe_len(lj[j]);
             trs[tbase + lt] = rsj[j];
             tlen[tbase + lt] = (uint32_t)lj[j];
+            tcap[tbase + lt] = rcj[j];
             ++lt;
         }
         ++lu;
@@ -786,7 +794,11 @@ __device__ void plan_totals_decide(DevState *ds, unsigned long long pool_cap, co
 // Per touched row: new length, mode (values in place / rewrite), chunk
 // claim and old entries to move, per-tile sums to the batch tile table and per-tile counters to
 // pplan; per tile of coordinates, the probe's insert / add / delete counts
-// to the probe table.  The last workgroup to finish then decides the batch: it makes the
+// to the probe table.  fresh: the row lengths and capacities the heads pass
+// recorded (tlen, tcap) still hold, so every load is one coalesced round
+// trip; otherwise (a re-plan after the pool was compacted or grown) they
+// and the row's pool start are re-read from the row table and trs is
+// rewritten.  The last workgroup to finish then decides the batch: it makes the
 // claim / moved / insert / delete columns exclusive offsets, totals every
 // counter, rejects the batch (no-op) on a bad index or when the claims do
 // not fit the pool, and otherwise advances the bump pointer.
@@ -800,10 +812,11 @@ __global__ __launch_bounds__(TB) void k_upd_plan(DevState *ds, unsigned long lon
                                                  uint32_t *__restrict__ tmode, uint32_t *__restrict__ tmoved,
                                                  uint32_t *__restrict__ tt,
                                                  uint32_t *__restrict__ tp, long long *__restrict__ pplan,
-                                                 unsigned int *fin, bool table)
+                                                 unsigned int *fin, bool table, bool fresh,
+                                                 const uint32_t *__restrict__ tlen, const uint32_t *__restrict__ tcap)
 {
     __shared__ long long lds[4][8];
-    __shared__ uint32_t lds4[4], s_flag;
+    __shared__ uint32_t lds4[4], lds44[4][4], s_flag;
     const int64_t tile = blockIdx.x;
     const int64_t t0 = tile * TILE + (int64_t)threadIdx.x * IPT;
     const uint32_t nt = ds->ntouched, nu = ds->nuniq;
@@ -812,24 +825,30 @@ __global__ __launch_bounds__(TB) void k_upd_plan(DevState *ds, unsigned long lon
     // loads in two phases (row ids, counts and the tile's codes, then row
     // lengths), so each phase is one memory round trip for all eight rows
     uint32_t rr[IPT], ti[IPT], td[IPT], rl[IPT], rc[IPT], cd[IPT];
-#pragma unroll
-    for (int j = 0; j < IPT; ++j) {
-        rr[j] = ti[j] = td[j] = 0;
-        cd[j] = t0 + j < nu ? ucode[t0 + j] : (uint32_t)C_NOP;
-        if (t0 + j < nt) {
-            rr[j] = trow[t0 + j];
-            ti[j] = tins[t0 + j];
-            td[j] = tdel[t0 + j];
-        }
-    }
     uint32_t rsv[IPT];
 #pragma unroll
     for (int j = 0; j < IPT; ++j) {
-        rl[j] = rc[j] = rsv[j] = 0;
+        rr[j] = ti[j] = td[j] = rl[j] = rc[j] = rsv[j] = 0;
+        cd[j] = t0 + j < nu ? ucode[t0 + j] : (uint32_t)C_NOP;
         if (t0 + j < nt) {
-            rl[j] = (uint32_t)row_len[rr[j]];
-            rc[j] = (uint32_t)row_cap[rr[j]];
-            rsv[j] = row_start[rr[j]];
+            ti[j] = tins[t0 + j];
+            td[j] = tdel[t0 + j];
+            if (fresh) {
+                rl[j] = tlen[t0 + j];
+                rc[j] = tcap[t0 + j];
+            } else {
+                rr[j] = trow[t0 + j];
+            }
+        }
+    }
+    if (!fresh) {
+#pragma unroll
+        for (int j = 0; j < IPT; ++j) {
+            if (t0 + j < nt) {
+                rl[j] = (uint32_t)row_len[rr[j]];
+                rc[j] = (uint32_t)row_cap[rr[j]];
+                rsv[j] = row_start[rr[j]];
+            }
         }
     }
 #pragma unroll
@@ -854,20 +873,20 @@ __global__ __launch_bounds__(TB) void k_upd_plan(DevState *ds, unsigned long lon
         }
         tnewl[t] = nl;
         tmode[t] = mode;
-        trs[t] = rsv[j];
+        if (!fresh) trs[t] = rsv[j];
         tmoved[t] = mode == M_REWRITE ? L : 0u;
     }
-    csum = block_sum_u32(csum, lds4);
-    msum = block_sum_u32(msum, lds4);
-    // inserts | adds << 16 (each at most TILE), deletes
-    uint32_t cia = 0, cdl = 0;
+    // claims, moved, inserts | adds << 16 (each at most TILE), deletes
+    uint32_t sums[4] = {csum, msum, 0u, 0u};
 #pragma unroll
     for (int j = 0; j < IPT; ++j) {
-        cia += (cd[j] == C_INS ? 1u : 0u) + (cd[j] == C_ADD ? 0x10000u : 0u);
-        cdl += cd[j] == C_DEL ? 1u : 0u;
+        sums[2] += (cd[j] == C_INS ? 1u : 0u) + (cd[j] == C_ADD ? 0x10000u : 0u);
+        sums[3] += cd[j] == C_DEL ? 1u : 0u;
     }
-    cia = block_sum_u32(cia, lds4);
-    cdl = block_sum_u32(cdl, lds4);
+    block_sum_n<4>(sums, lds44);
+    csum = sums[0];
+    msum = sums[1];
+    const uint32_t cia = sums[2], cdl = sums[3];
 #pragma unroll
     for (int i = 0; i < NPLAN; ++i) {
         const long long w = wave_sum_i64(v[i]);
@@ -912,7 +931,7 @@ __global__ __launch_bounds__(TB) void k_upd_offsets(DevState *__restrict__ ds,
                                                     const long long *__restrict__ pplan, unsigned long long pool_cap,
                                                     bool table, uint32_t *__restrict__ mspan)
 {
-    __shared__ uint32_t lds4[4];
+    __shared__ uint32_t lds44[4][4];
     __shared__ long long lds[4][8];
     const int64_t tile = blockIdx.x;
     const int64_t e0 = tile * TILE + (int64_t)threadIdx.x * IPT;
@@ -967,25 +986,28 @@ __global__ __launch_bounds__(TB) void k_upd_offsets(DevState *__restrict__ ds,
         bc = tt[tile * TT_WORDS + TT_CX];
         bm = tt[tile * TT_WORDS + TT_MX];
     } else {
-        uint32_t a0 = 0, a1 = 0, a2 = 0, a3 = 0;
+        uint32_t a[4] = {0, 0, 0, 0};
 #pragma unroll
         for (int q = 0; q < QT; ++q) {
             const bool in = (int64_t)threadIdx.x + q * TB < tile;
-            a0 += in ? qi[q] : 0u;
-            a1 += in ? qd[q] : 0u;
-            a2 += in ? qc[q] : 0u;
-            a3 += in ? qm[q] : 0u;
+            a[0] += in ? qi[q] : 0u;
+            a[1] += in ? qd[q] : 0u;
+            a[2] += in ? qc[q] : 0u;
+            a[3] += in ? qm[q] : 0u;
         }
-        bi = block_sum_u32(a0, lds4);
-        bd = block_sum_u32(a1, lds4);
-        bc = block_sum_u32(a2, lds4);
-        bm = block_sum_u32(a3, lds4);
+        block_sum_n<4>(a, lds44);
+        bi = a[0];
+        bd = a[1];
+        bc = a[2];
+        bm = a[3];
     }
-    uint32_t ti, td, tc, tm;
-    si = block_excl_scan_256(si, lds4, &ti) + bi;
-    sd = block_excl_scan_256(sd, lds4, &td) + bd;
-    sc = block_excl_scan_256(sc, lds4, &tc) + bc;
-    sm = block_excl_scan_256(sm, lds4, &tm) + bm;
+    uint32_t sv[4] = {si, sd, sc, sm}, stot[4];
+    block_excl_scan_n<4>(sv, lds44, stot);
+    const uint32_t ti = stot[0], td = stot[1], tc = stot[2], tm = stot[3];
+    si = sv[0] + bi;
+    sd = sv[1] + bd;
+    sc = sv[2] + bc;
+    sm = sv[3] + bm;
 #pragma unroll
     for (int j = 0; j < IPT; ++j) {
         const int64_t e = e0 + j;
@@ -1325,7 +1347,7 @@ struct UpdBufs {
     int32_t *ucol;
     float *usum, *uval;
     uint32_t *ut, *upos, *ucode, *upre;
-    uint32_t *trow, *tubeg, *tlpre, *trs, *tlen, *tins, *tdel, *tnewl, *tmode, *tmoved, *tpre;
+    uint32_t *trow, *tubeg, *tlpre, *trs, *tlen, *tcap, *tins, *tdel, *tnewl, *tmode, *tmoved, *tpre;
     uint32_t *tt, *tp, *fin;
     // first touched row of every probe / move span (null: searched for)
     uint32_t *pspan, *mspan;
@@ -1419,6 +1441,7 @@ dcsr_status upd_bufs_src(BufSrc &src, int64_t n, UpdBufs &b)
     DCSR_TRY(take(src, WS_UROW, n4 + 4, (void **)&b.tlpre));
     DCSR_TRY(take(src, WS_TRS, n4, (void **)&b.trs));
     DCSR_TRY(take(src, WS_TLEN, n4, (void **)&b.tlen));
+    DCSR_TRY(take(src, WS_TCAP, n4, (void **)&b.tcap));
     DCSR_TRY(take(src, WS_TINS, n4, (void **)&b.tins));
     DCSR_TRY(take(src, WS_TDEL, n4, (void **)&b.tdel));
     DCSR_TRY(take(src, WS_TNEWL, n4, (void **)&b.tnewl));
@@ -1459,12 +1482,14 @@ static void launch_offsets(dcsr_handle h, UpdBufs &b, int64_t n, hipStream_t s)
 
 // Plan and decision.  With tile tables the plan's last workgroup decides;
 // without them the offsets pass does (its last tile), so it runs here.
-dcsr_status plan_and_check(dcsr_handle h, UpdBufs &b, int64_t n, hipStream_t s)
+// fresh: straight after the heads pass (false for a re-plan after the pool
+// was compacted or grown).
+dcsr_status plan_and_check(dcsr_handle h, UpdBufs &b, int64_t n, hipStream_t s, bool fresh)
 {
     const bool table = use_tables(n);
     k_upd_plan<<<ptiles(n), TB, 0, s>>>(h->ds, (unsigned long long)h->pool_cap, b.trow, b.tins, b.tdel, b.ucode, h->row_len,
                                         h->row_cap, h->row_start, b.trs, b.tnewl, b.tmode, b.tmoved, b.tt, b.tp,
-                                        b.pplan, b.fin, table);
+                                        b.pplan, b.fin, table, fresh, b.tlen, b.tcap);
     if (!table) launch_offsets(h, b, n, s);
     return hipGetLastError() == hipSuccess ? DCSR_OK : DCSR_ERR_HIP;
 }
@@ -1497,13 +1522,13 @@ dcsr_status sort_plan_launch(dcsr_handle h, int64_t n, UpdBufs &b, hipStream_t s
     const bool table = ht > HEADS_DIRECT_TILES;
     k_upd_heads_count<<<ht, TB, 0, s>>>(key, n, h->cbits, h->row_len, b.tt, b.tp, b.fin, h->ds, table);
     k_upd_heads_emit<<<ht, TB, 0, s>>>(key, val, n, h->cbits, b.tt, h->ds, table, b.ucol, b.usum, b.ut, b.trow,
-                                       b.tubeg, h->row_len, h->row_start, b.tp, b.tlpre, b.trs, b.tlen, b.tins,
-                                       b.tdel, b.pspan);
+                                       b.tubeg, h->row_len, h->row_start, h->row_cap, b.tp, b.tlpre, b.trs, b.tlen,
+                                       b.tcap, b.tins, b.tdel, b.pspan);
     k_upd_probe<<<probe_blocks(h, n), TB, 0, s>>>(h->ds, h->row_start, h->row_len, h->pool_col, h->pool_val,
                                                   b.tubeg, b.tlpre, b.trs, b.tlen, b.ucol, b.usum, b.uval, b.upos,
                                                   b.ucode, b.tins, b.tdel, b.pspan);
     DCSR_HIP(hipGetLastError());
-    DCSR_TRY(plan_and_check(h, b, n, s));
+    DCSR_TRY(plan_and_check(h, b, n, s, true));
     if (use_tables(n)) launch_offsets(h, b, n, s);
     DCSR_HIP(hipGetLastError());
     return DCSR_OK;
@@ -1606,7 +1631,7 @@ dcsr_status update_replan(dcsr_handle h, int64_t n, hipStream_t s, bool *go)
 {
     UpdBufs b;
     DCSR_TRY(upd_bufs(h, n, b, s));
-    DCSR_TRY(plan_and_check(h, b, n, s));
+    DCSR_TRY(plan_and_check(h, b, n, s, false));
     DCSR_TRY(pull_state(h, s));
     *go = h->hs->go != 0;
     return DCSR_OK;

@@ -118,6 +118,53 @@ __device__ __forceinline__ uint32_t block_excl_scan_256(uint32_t v, uint32_t *ld
     return carry + inc - v;
 }
 
+// block_excl_scan_256 for N columns at once (one LDS exchange and one pair
+// of barriers for all of them): v[i] becomes the exclusive prefix of column
+// i over the workgroup, tot[i] its total.  lds: [4][N] words.
+template <int N>
+__device__ __forceinline__ void block_excl_scan_n(uint32_t (&v)[N], uint32_t (*lds)[N], uint32_t (&tot)[N])
+{
+    const int w = threadIdx.x >> 6;
+    uint32_t inc[N];
+#pragma unroll
+    for (int i = 0; i < N; ++i) inc[i] = wave_incl_scan_u32(v[i]);
+    if (lane_id() == 63) {
+#pragma unroll
+        for (int i = 0; i < N; ++i) lds[w][i] = inc[i];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < N; ++i) {
+        uint32_t carry = 0, t = 0;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const uint32_t x = lds[q][i];
+            if (q < w) carry += x;
+            t += x;
+        }
+        tot[i] = t;
+        v[i] = carry + inc[i] - v[i];
+    }
+    __syncthreads();
+}
+
+// Workgroup totals of N columns at once: v[i] becomes column i's total.
+template <int N>
+__device__ __forceinline__ void block_sum_n(uint32_t (&v)[N], uint32_t (*lds)[N])
+{
+    uint32_t t[N];
+#pragma unroll
+    for (int i = 0; i < N; ++i) t[i] = wave_read_lane_u32(wave_incl_scan_u32(v[i]), 63);
+    if (lane_id() == 0) {
+#pragma unroll
+        for (int i = 0; i < N; ++i) lds[threadIdx.x >> 6][i] = t[i];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < N; ++i) v[i] = (lds[0][i] + lds[1][i]) + (lds[2][i] + lds[3][i]);
+    __syncthreads();
+}
+
 // Fixed-order float sum over the wave (tree over lane index).
 __device__ __forceinline__ float wave_sum_f32(float v)
 {

@@ -161,6 +161,43 @@ def test_nosync_update_chain_and_spmv(torch, pool_factor):
     sync.close()
 
 
+def test_nosync_snapshots_keep_the_bounds_tight(torch):
+    """Each NOSYNC update's write kernel stores the live count and bump
+    pointer it leaves into host memory; the next call folds the newest one
+    into the host's bounds.  Batches that only add to stored coordinates claim
+    nothing, but the host must assume each may claim the whole live matrix
+    plus the batch: without landed snapshots the bump bound would pass the
+    pool within a few calls and force an in-order compaction.  With the
+    stream synchronized between calls every snapshot has landed, so no
+    compaction may happen; the results still match the oracle."""
+    from dcsr_amd import DcsrMatrix, UPDATE_NOSYNC
+    R, C, nnz, n = 2000, 2000, 20000, 4000
+    rng = np.random.default_rng(33)
+    r, c, v = random_coo(rng, R, C, nnz, 0.0, 0.0, "pos")  # positive: no sum below cancels
+    # distinct coordinates: the matrix is canonical at load, so the first
+    # NOSYNC call does not take the synchronous canonicalizing path
+    _, first = np.unique(r.astype(np.int64) * C + c, return_index=True)
+    r, c, v = r[first], c[first], v[first]
+    A = O.coo_to_csr(R, C, r, c, v)
+    # pool 4x the entries: the host's bound grows by ~nnz + n + 7 * touched
+    # (~50K slots) per call without a snapshot, past the 80K pool at call 2
+    m = DcsrMatrix(R, C, pool_factor=4.0)
+    m.LoadMatrix(r, c, v)
+    rows_of = np.repeat(np.arange(R), np.diff(A.off)).astype(np.int32)
+    c0 = m.info()["compactions"]
+    for k in range(12):
+        pick = rng.integers(0, A.nnz, n)
+        br, bc = rows_of[pick], A.col[pick].astype(np.int32)
+        bv = rng.integers(1, 4, n).astype(np.float32)
+        m.UpdateMatrix(br, bc, bv, flags=UPDATE_NOSYNC)
+        A = O.update(A, br, bc, bv)
+        torch.cuda.synchronize()
+    assert m.status() == 0
+    assert m.info()["compactions"] == c0
+    assert_same_structure(to_host_csr(m), A)
+    m.close()
+
+
 def test_powerlaw_with_hub_rows(torch):
     """Rows longer than the hub threshold go through the split SpMV path."""
     from dcsr_amd import DcsrMatrix
