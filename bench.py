@@ -540,6 +540,7 @@ def bench_spgemm(args):
 def bench_serving(args):
     import torch
     import dcsr_amd
+    from dcsr_amd import workloads as W
 
     R, N = args.serving_rows, args.serving_nnz
     nreq = args.steps

@@ -219,8 +219,19 @@ __device__ void lds_radix_pass(const uint32_t *ks, const float *vs, uint32_t *kd
     __syncthreads();
 }
 
+// Counters of the matrices one workgroup updated, kept by its thread 0 and
+// added to the device state once when the workgroup is done (one atomic per
+// counter and workgroup instead of per matrix).
+struct BatchAcc {
+    long long dnnz, dgarbage;
+    unsigned long long ins, add, del;
+    uint32_t touched;
+};
+
 // Update of matrix m inside one workgroup.  Returns nothing; status bits go
-// to mstat[m].
+// to mstat[m].  reserved: the host has made room in the pool for every
+// matrix's largest claim, so claims take a plain atomic add; otherwise a
+// compare-and-swap that advances the bump pointer only when the claim fits.
 __device__ void batch_update(uint32_t m, const uint32_t *__restrict__ row_base, const uint32_t *__restrict__ col_base,
                              const int64_t *__restrict__ tri_off, int64_t ntri, const int32_t *__restrict__ trow,
                              const int32_t *__restrict__ tcol, const float *__restrict__ tval,
@@ -228,7 +239,7 @@ __device__ void batch_update(uint32_t m, const uint32_t *__restrict__ row_base, 
                              int32_t *__restrict__ row_cap, int32_t *__restrict__ pcol, float *__restrict__ pval,
                              unsigned long long pool_cap, DevState *ds, uint32_t *__restrict__ mstat, uint32_t *sc,
                              uint32_t (*skb)[BMAX], float (*svb)[BMAX], uint32_t (*wcnt)[256], uint32_t *lw,
-                             uint32_t *sm)
+                             uint32_t *sm, bool reserved, BatchAcc &acc)
 {
     const uint32_t t = threadIdx.x;
     const uint32_t gbase = row_base[m];
@@ -420,7 +431,9 @@ __device__ void batch_update(uint32_t m, const uint32_t *__restrict__ row_base, 
         // (smaller) matrices of the same step
         unsigned long long base = 0ull;
         bool ok = true;
-        if (claim_tot) {
+        if (claim_tot && reserved) {
+            base = atomicAdd(&ds->bump, (unsigned long long)claim_tot);
+        } else if (claim_tot) {
             unsigned long long cur = __hip_atomic_load(&ds->bump, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             for (;;) {
                 if (cur + claim_tot > pool_cap) {
@@ -439,12 +452,12 @@ __device__ void batch_update(uint32_t m, const uint32_t *__restrict__ row_base, 
         sm[2] = (uint32_t)base;
         if (!ok) mstat[m] |= BST_EXHAUSTED;
         if (ok) {
-            atomicAdd((unsigned long long *)&ds->nnz, (unsigned long long)(long long)(int32_t)dn_lo);
-            atomicAdd((unsigned long long *)&ds->garbage, (unsigned long long)(long long)(int32_t)dg_lo);
-            atomicAdd((unsigned long long *)&ds->n_ins, (unsigned long long)nins);
-            atomicAdd((unsigned long long *)&ds->n_add, (unsigned long long)nadd);
-            atomicAdd((unsigned long long *)&ds->n_del, (unsigned long long)ndel);
-            atomicAdd(&ds->ntouched, nt);
+            acc.dnnz += (long long)(int32_t)dn_lo;
+            acc.dgarbage += (long long)(int32_t)dg_lo;
+            acc.ins += nins;
+            acc.add += nadd;
+            acc.del += ndel;
+            acc.touched += nt;
         }
     }
     __syncthreads();
@@ -597,7 +610,7 @@ __global__ __launch_bounds__(BT) void k_batch_step(uint32_t M, const uint32_t *_
                                                    float *__restrict__ pval, unsigned long long pool_cap, DevState *ds,
                                                    uint32_t *__restrict__ mstat, uint32_t *__restrict__ scratch,
                                                    const float *__restrict__ x, float *__restrict__ y, uint32_t thresh,
-                                                   BatchState *bs, int do_update, int do_spmv)
+                                                   BatchState *bs, int do_update, int do_spmv, int reserved)
 {
     __shared__ uint32_t sk[2][BMAX];
     __shared__ float sv[2][BMAX];
@@ -605,6 +618,7 @@ __global__ __launch_bounds__(BT) void k_batch_step(uint32_t M, const uint32_t *_
     __shared__ uint32_t lw[BT / 64];
     __shared__ uint32_t sm[4];
     uint32_t *sc = scratch + (uint64_t)blockIdx.x * S_WORDS;
+    BatchAcc acc{0, 0, 0, 0, 0, 0};
     for (;;) {
         if (threadIdx.x == 0) sm[3] = atomicAdd(&bs->work, 1u);
         __syncthreads();
@@ -613,15 +627,24 @@ __global__ __launch_bounds__(BT) void k_batch_step(uint32_t M, const uint32_t *_
         if (m >= M) break;
         if (do_update)
             batch_update(m, row_base, col_base, tri_off, ntri, trow, tcol, tval, row_start, row_len, row_cap, pcol, pval,
-                         pool_cap, ds, mstat, sc, sk, sv, wcnt, lw, sm);
+                         pool_cap, ds, mstat, sc, sk, sv, wcnt, lw, sm, reserved != 0, acc);
         if (do_spmv) batch_spmv<TEAM>(m, row_base, col_base, row_start, row_len, pcol, pval, x, y, thresh);
         __syncthreads();
     }
-    // the last workgroup out resets the counters (no memset: replay-safe).
-    // No fence: both counters are atomics, every workgroup's last work claim
-    // returned before its finish count, and the kernel boundary publishes the
-    // reset (a release fence would write back the XCD's whole L2 here)
+    // the workgroup's counters, then: the last workgroup out resets the work
+    // counters (no memset: replay-safe).  No fence: both counters are
+    // atomics, every workgroup's last work claim returned before its finish
+    // count, and the kernel boundary publishes the reset (a release fence
+    // would write back the XCD's whole L2 here)
     if (threadIdx.x == 0) {
+        if (acc.touched) {
+            atomicAdd((unsigned long long *)&ds->nnz, (unsigned long long)acc.dnnz);
+            atomicAdd((unsigned long long *)&ds->garbage, (unsigned long long)acc.dgarbage);
+            atomicAdd((unsigned long long *)&ds->n_ins, acc.ins);
+            atomicAdd((unsigned long long *)&ds->n_add, acc.add);
+            atomicAdd((unsigned long long *)&ds->n_del, acc.del);
+            atomicAdd(&ds->ntouched, acc.touched);
+        }
         if (atomicAdd(&bs->done, 1u) == gridDim.x - 1) {
             bs->work = 0;
             bs->done = 0;
@@ -802,38 +825,47 @@ dcsr_status dcsr_batch_load_coo(dcsr_batch b, const int64_t *tri_off, const int3
 
 template <int TEAM>
 static void launch_step(dcsr_batch b, const int64_t *tri_off, int64_t n, const int32_t *rows, const int32_t *cols,
-                        const float *vals, const float *x, float *y, int do_update, int do_spmv, hipStream_t s)
+                        const float *vals, const float *x, float *y, int do_update, int do_spmv, int reserved,
+                        hipStream_t s)
 {
     dcsr_handle h = b->h;
     k_batch_step<TEAM><<<b->grid, BT, 0, s>>>(b->M, b->row_base, b->col_base, tri_off, n, rows, cols, vals, h->row_start,
                                               h->row_len, h->row_cap, h->pool_col, h->pool_val,
                                               (unsigned long long)h->pool_cap, h->ds, b->mstat, b->scratch, x, y,
-                                              (uint32_t)b->thresh, b->bs, do_update, do_spmv);
+                                              (uint32_t)b->thresh, b->bs, do_update, do_spmv, reserved);
 }
 
 static void launch_any(dcsr_batch b, const int64_t *tri_off, int64_t n, const int32_t *rows, const int32_t *cols,
-                       const float *vals, const float *x, float *y, int do_update, int do_spmv, hipStream_t s)
+                       const float *vals, const float *x, float *y, int do_update, int do_spmv, int reserved,
+                       hipStream_t s)
 {
     switch (b->team) {
-    case 4: launch_step<4>(b, tri_off, n, rows, cols, vals, x, y, do_update, do_spmv, s); break;
-    case 16: launch_step<16>(b, tri_off, n, rows, cols, vals, x, y, do_update, do_spmv, s); break;
-    case 32: launch_step<32>(b, tri_off, n, rows, cols, vals, x, y, do_update, do_spmv, s); break;
-    default: launch_step<8>(b, tri_off, n, rows, cols, vals, x, y, do_update, do_spmv, s); break;
+    case 4: launch_step<4>(b, tri_off, n, rows, cols, vals, x, y, do_update, do_spmv, reserved, s); break;
+    case 16: launch_step<16>(b, tri_off, n, rows, cols, vals, x, y, do_update, do_spmv, reserved, s); break;
+    case 32: launch_step<32>(b, tri_off, n, rows, cols, vals, x, y, do_update, do_spmv, reserved, s); break;
+    default: launch_step<8>(b, tri_off, n, rows, cols, vals, x, y, do_update, do_spmv, reserved, s); break;
     }
 }
 
 // Pool headroom for one step: every matrix may rewrite all its rows plus its
-// batch.  Compacts/grows ahead of the launch so the kernel never runs out.
-// With allow_grow = 0 the pool keeps its capacity (retired chunks are
-// reclaimed first): a matrix whose claim then does not fit fails alone with
-// its exhausted bit and leaves room for the others.
-static dcsr_status batch_reserve(dcsr_batch b, int64_t n, hipStream_t s)
+// batch.  Compacts/grows ahead of the launch so the kernel never runs out
+// (*reserved = 1: the claims need no fit check).  With allow_grow = 0 the
+// pool keeps its capacity (retired chunks are reclaimed first): if the room
+// may still fall short (*reserved = 0), a matrix whose claim does not fit
+// fails alone with its exhausted bit and leaves room for the others.
+static dcsr_status batch_reserve(dcsr_batch b, int64_t n, hipStream_t s, int *reserved)
 {
     dcsr_handle h = b->h;
     if (!h->state_known) DCSR_TRY(pull_state(h, s));
     const int64_t need = h->nnz_h + n + 4 * (int64_t)h->rows + 64;
+    *reserved = 1;
     if (h->pool_cap - h->bump_h >= need) return DCSR_OK;
-    if (!h->cfg.allow_grow) return h->bump_h > h->nnz_h ? compact_in_order(h, s) : DCSR_OK;
+    if (!h->cfg.allow_grow) {
+        if (h->bump_h > h->nnz_h) DCSR_TRY(compact_in_order(h, s));
+        // after a compaction the bump pointer is the live count
+        *reserved = h->pool_cap - h->nnz_h >= need ? 1 : 0;
+        return DCSR_OK;
+    }
     return pool_reserve(h, need, s, nullptr);
 }
 
@@ -850,11 +882,13 @@ dcsr_status dcsr_batch_step(dcsr_batch b, const int64_t *tri_off, const int32_t 
     dcsr_handle h = b->h;
     h->last_stream = s;
     if (!h->canonical) DCSR_TRY(canonicalize(h, s));
-    if (do_update) DCSR_TRY(batch_reserve(b, n, s));
+    int reserved = 0;
+    if (do_update) DCSR_TRY(batch_reserve(b, n, s, &reserved));
     k_batch_zero_stat<<<grid_for(b->M, 256), 256, 0, s>>>(b->mstat, b->M, h->ds);
     if (flags & DCSR_BATCH_GRAPH) {
         const void *key[13] = {tri_off, rows, cols, vals, x, y, h->pool_col, (const void *)(intptr_t)b->team,
-                               (const void *)(intptr_t)b->thresh, (const void *)(intptr_t)(do_update * 2 + do_spmv),
+                               (const void *)(intptr_t)b->thresh,
+                               (const void *)(intptr_t)(reserved * 4 + do_update * 2 + do_spmv),
                                (const void *)(intptr_t)h->pool_cap, (const void *)(intptr_t)n, h->row_start};
         bool same = b->gexec != nullptr;
         for (int i = 0; i < 13 && same; ++i) same = key[i] == b->gkey[i];
@@ -866,14 +900,14 @@ dcsr_status dcsr_batch_step(dcsr_batch b, const int64_t *tri_off, const int32_t 
             // capture on the batch's own stream (the caller's may be the null
             // stream, which cannot be captured); nothing runs during capture
             DCSR_HIP(hipStreamBeginCapture(b->cs, hipStreamCaptureModeThreadLocal));
-            launch_any(b, tri_off, n, rows, cols, vals, x, y, do_update, do_spmv, b->cs);
+            launch_any(b, tri_off, n, rows, cols, vals, x, y, do_update, do_spmv, reserved, b->cs);
             DCSR_HIP(hipStreamEndCapture(b->cs, &b->graph));
             DCSR_HIP(hipGraphInstantiate(&b->gexec, b->graph, nullptr, nullptr, 0));
             for (int i = 0; i < 13; ++i) b->gkey[i] = key[i];
         }
         DCSR_HIP(hipGraphLaunch(b->gexec, s));
     } else {
-        launch_any(b, tri_off, n, rows, cols, vals, x, y, do_update, do_spmv, s);
+        launch_any(b, tri_off, n, rows, cols, vals, x, y, do_update, do_spmv, reserved, s);
     }
     DCSR_HIP(hipGetLastError());
     if (do_update) {
@@ -940,9 +974,9 @@ dcsr_status dcsr_batch_autotune(dcsr_batch b, const float *x, float *y, int32_t 
             for (int hi = 0; hi < 3; ++hi) {
                 b->team = teams[ti];
                 b->thresh = threshs[hi];
-                launch_any(b, nullptr, 0, nullptr, nullptr, nullptr, x, y, 0, 1, s);  // warm
+                launch_any(b, nullptr, 0, nullptr, nullptr, nullptr, x, y, 0, 1, 0, s);  // warm
                 DCSR_HIP(hipEventRecord(e0, s));
-                for (int rep = 0; rep < 3; ++rep) launch_any(b, nullptr, 0, nullptr, nullptr, nullptr, x, y, 0, 1, s);
+                for (int rep = 0; rep < 3; ++rep) launch_any(b, nullptr, 0, nullptr, nullptr, nullptr, x, y, 0, 1, 0, s);
                 DCSR_HIP(hipEventRecord(e1, s));
                 DCSR_HIP(hipEventSynchronize(e1));
                 float ms = 0.0f;
