@@ -64,7 +64,7 @@ enum WsSlot {
     WS_SG_HPRE, WS_SG_EA, WS_SG_APOS, WS_SG_KEY0, WS_SG_KEY1, WS_SG_VAL0, WS_SG_VAL1, WS_SG_FLAG, WS_SG_EPOS,
     WS_CANON, WS_TMP0, WS_TMP1, WS_TMP2,
     WS_PNL_OFF, WS_PNL_BND, WS_PNL_COL, WS_PNL_VAL, WS_PNL_MAP, WS_PNL_PART,
-    WS_TRS, WS_TLEN, WS_PSPAN, WS_MSPAN, WS_TCAP,
+    WS_TRS, WS_TLEN, WS_PSPAN, WS_MSPAN, WS_TCAP, WS_ROWT,
     WS_COUNT
 };
 
@@ -126,6 +126,7 @@ struct dcsr_matrix_s {
     bool canonical;
     bool hubs_dirty;
     int64_t ncompact;        // compactions and growths (dcsr_query reports it)
+    uint32_t upd_tag;        // tag of the last update's row -> touched-row map
     bool flat_dirty;         // flat row offsets (SpMV binning) need a rebuild
     bool state_known;        // host mirrors valid (no NOSYNC update since last sync)
     // column-paneled copy for SpMV on matrices whose x exceeds the L2 (see
@@ -182,6 +183,14 @@ void refresh_bounds(dcsr_handle h);
 static inline int64_t live_bound(dcsr_handle h) { return h->state_known ? h->nnz_h : h->nnz_ub; }
 static inline int64_t bump_bound(dcsr_handle h) { return h->state_known ? h->bump_h : h->bump_ub; }
 dcsr_status ensure_flat(dcsr_handle h, hipStream_t s);  // spmv.hip
+// The SpMV's flat offsets, row map and row-scan look-back block (sized for
+// the current pool), and the bookkeeping once a stream-ordered scan has
+// rebuilt them (the update's move kernel can run that scan).
+struct FlatBufs {
+    uint32_t *foff, *rowmap, *part;
+};
+dcsr_status flat_buffers(dcsr_handle h, hipStream_t s, FlatBufs *fb);  // spmv.hip
+void flat_rebuilt(dcsr_handle h);                                      // spmv.hip
 dcsr_status pool_reserve(dcsr_handle h, int64_t min_free, hipStream_t s, const int32_t *reserve_len);
 dcsr_status canonicalize(dcsr_handle h, hipStream_t s);
 // Compaction at the current capacity, in stream order, without a read-back.

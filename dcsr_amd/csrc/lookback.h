@@ -90,15 +90,18 @@ __device__ __forceinline__ uint32_t lb_ticket(unsigned int *ctl, uint32_t *lds2,
 // stores read only within this launch (all before the last finish), and
 // the kernel boundary publishes the reset to the next launch.  (A release
 // fence here writes back the XCD's whole L2, dirty lines of earlier kernels
-// included, in every workgroup.)
-__device__ __forceinline__ void lb_finish(unsigned int *ctl)
+// included, in every workgroup.)  n: the workgroups of the launch that take
+// part in the scan.
+__device__ __forceinline__ void lb_finish_n(unsigned int *ctl, uint32_t n)
 {
-    if (atomicAdd(&ctl[1], 1u) == gridDim.x - 1) {
+    if (atomicAdd(&ctl[1], 1u) == n - 1) {
         ctl[3] = ctl[3] + 1u;
         ctl[0] = 0u;
         ctl[1] = 0u;
     }
 }
+
+__device__ __forceinline__ void lb_finish(unsigned int *ctl) { lb_finish_n(ctl, gridDim.x); }
 
 // Exclusive prefixes of tile `tile` (> 0) for NC components whose status
 // words sit at st[j * NC + c] (j < tile), computed by one whole wave: 64

@@ -20,6 +20,7 @@
 #include "dcsr_internal.h"
 #include "lookback.h"
 #include "prims.h"
+#include "rowscan.h"
 #include "segments.h"
 
 namespace {
@@ -27,7 +28,7 @@ namespace {
 constexpr int TB = 256;
 constexpr int ILP = 4;
 constexpr uint32_t NONE = 0xffffffffu;
-constexpr uint32_t GRAN = 64;  // row-sample granule (nonzeros)
+constexpr uint32_t GRAN = RS_GRAN;  // row-map granule (nonzeros)
 
 // rowmap[i] = first row not entirely before flat position i*GRAN, i.e. the
 // smallest r with foff[r+1] >= i*GRAN.  Lets any GRAN-aligned range find its
@@ -47,108 +48,33 @@ __global__ void k_rowmap(uint32_t R, const uint32_t *__restrict__ foff, uint32_t
     rowmap[i] = lo;
 }
 
-// Flat offsets and row map of the row lengths in one single-pass scan:
-// foff[r] = len[0] + ... + len[r-1] (foff[R] = total) and rowmap[g] as
-// k_rowmap defines it, for g <= total / GRAN: row r owns the granules g with
-// foff[r] < g*GRAN <= foff[r+1], and granule 0 belongs to row 0.  Tiles of
-// 4096 rows (16 contiguous rows per thread, vector loads and stores): a
-// matrix of a few hundred thousand rows has under a hundred tiles, so the
-// cross-tile look-back (lookback.h) resolves in one or two rounds.  Measured
-// on 325K rows against 8 and 32 rows per thread: 55 vs 57 and 59 us per
-// SpMV after an update.  The tile's granules are filled by the whole
-// workgroup, one granule per thread and a search of the tile's offsets in
-// LDS, so that a long row does not leave one thread storing its granules
-// one by one.
-constexpr int SR_IPT = 16;
-constexpr int SR_TILE = TB * SR_IPT;
+// Flat offsets and row map of the row lengths (rowscan.h), one tile per
+// workgroup.  Measured on 325K rows against 8 and 32 rows per thread: 55 vs
+// 57 and 59 us per SpMV after an update.
 __global__ __launch_bounds__(TB) void k_scan_rows(uint32_t R, const int32_t *__restrict__ len,
                                                   uint32_t *__restrict__ foff, uint32_t *__restrict__ rowmap,
                                                   uint32_t *__restrict__ part, unsigned int *fault, int limit)
 {
-    __shared__ uint32_t lds4[4], s_tk[2], s_pre;
-    __shared__ uint32_t s_off[SR_TILE + 1];
-    unsigned int *ctl = part;
-    uint64_t *st = (uint64_t *)(part + 4);
-    uint32_t tag;
-    const int64_t tile = lb_ticket(ctl, s_tk, &tag);
-    const int64_t r0 = tile * SR_TILE + (int64_t)threadIdx.x * SR_IPT;
-    const bool full = r0 + SR_IPT <= (int64_t)R;
-    uint32_t v[SR_IPT];
-    if (full) {
-        const int4 *p = (const int4 *)(len + r0);
+    __shared__ RowScanLds L;
+    auto lens = [&](int64_t r0, bool full, uint32_t(&v)[RS_IPT]) {
+        if (full) {
+            const int4 *p = (const int4 *)(len + r0);
 #pragma unroll
-        for (int q = 0; q < SR_IPT / 4; ++q) {
-            const int4 t = p[q];
-            v[4 * q] = (uint32_t)t.x;
-            v[4 * q + 1] = (uint32_t)t.y;
-            v[4 * q + 2] = (uint32_t)t.z;
-            v[4 * q + 3] = (uint32_t)t.w;
-        }
-    } else {
-#pragma unroll
-        for (int j = 0; j < SR_IPT; ++j) v[j] = r0 + j < (int64_t)R ? (uint32_t)len[r0 + j] : 0u;
-    }
-    uint32_t sum = 0;
-#pragma unroll
-    for (int j = 0; j < SR_IPT; ++j) sum += v[j];
-    uint32_t tot;
-    const uint32_t pre = block_excl_scan_256(sum, lds4, &tot);
-    if (threadIdx.x == 0) lb_publish(&st[tile], tag, tile == 0 ? LB_INC : LB_AGG, tot);
-    if (threadIdx.x < 64) {
-        uint32_t p[1] = {0u};
-        // fallback: the lengths of every earlier tile's rows
-        if (tile > 0)
-            lb_wave_lookback<1>(st, tile, tag, ctl, fault, limit,
-                                [&](int) { return lb_wave_sum_prefix((const uint32_t *)len, tile * SR_TILE, 1); }, p);
-        if (threadIdx.x == 0) {
-            s_pre = p[0];
-            if (tile > 0) lb_publish(&st[tile], tag, LB_INC, p[0] + tot);
-        }
-    }
-    __syncthreads();
-    const int64_t tr0 = tile * SR_TILE;
-    const uint32_t n = (uint32_t)((int64_t)R - tr0 < SR_TILE ? (int64_t)R - tr0 : SR_TILE);  // rows in the tile
-    uint32_t a = s_pre + pre;
-    uint32_t o[SR_IPT];
-#pragma unroll
-    for (int j = 0; j < SR_IPT; ++j) {
-        o[j] = a;
-        const uint32_t lj = threadIdx.x * SR_IPT + j;
-        if (lj < n) s_off[lj] = a;
-        a += v[j];
-    }
-    if (threadIdx.x == 0) s_off[n] = s_pre + tot;
-    __syncthreads();
-    {
-        // granules g with s_off[0] < g*GRAN <= s_off[n]: the tile's row i
-        // with s_off[i] < g*GRAN <= s_off[i+1] (the last i with s_off[i] <
-        // g*GRAN, found by bisection)
-        const uint32_t g0 = s_off[0] / GRAN + 1, g1 = s_off[n] / GRAN;
-        for (uint32_t g = g0 + threadIdx.x; g <= g1; g += TB) {
-            const uint32_t x = g * GRAN;
-            uint32_t l = 0, hgh = n - 1;
-            while (l < hgh) {
-                const uint32_t mid = (l + hgh + 1) >> 1;
-                if (s_off[mid] < x) l = mid;
-                else hgh = mid - 1;
+            for (int q = 0; q < RS_IPT / 4; ++q) {
+                const int4 t = p[q];
+                v[4 * q] = (uint32_t)t.x;
+                v[4 * q + 1] = (uint32_t)t.y;
+                v[4 * q + 2] = (uint32_t)t.z;
+                v[4 * q + 3] = (uint32_t)t.w;
             }
-            rowmap[g] = (uint32_t)(tr0 + l);
+        } else {
+#pragma unroll
+            for (int j = 0; j < RS_IPT; ++j) v[j] = r0 + j < (int64_t)R ? (uint32_t)len[r0 + j] : 0u;
         }
-    }
-    if (full) {
-        uint4 *q = (uint4 *)(foff + r0);
-#pragma unroll
-        for (int k = 0; k < SR_IPT / 4; ++k) q[k] = make_uint4(o[4 * k], o[4 * k + 1], o[4 * k + 2], o[4 * k + 3]);
-    } else {
-#pragma unroll
-        for (int j = 0; j < SR_IPT; ++j)
-            if (r0 + j < (int64_t)R) foff[r0 + j] = o[j];
-    }
-    // the thread holding row R-1 stores the total
-    if (r0 < (int64_t)R && r0 + SR_IPT >= (int64_t)R) foff[R] = a;
-    if (tile == 0 && threadIdx.x == 0) rowmap[0] = 0u;
-    __syncthreads();
-    if (threadIdx.x == 0) lb_finish(ctl);
+    };
+    // fallback: the lengths of every earlier tile's rows
+    auto prefix = [&](int64_t n) { return lb_wave_sum_prefix((const uint32_t *)len, n, 1); };
+    rowscan_tile(R, lens, prefix, foff, rowmap, part, fault, limit, gridDim.x, L);
 }
 
 // An entry load; NT: non-temporal, so that streaming the matrix does not
@@ -887,30 +813,40 @@ static size_t carry_slots(dcsr_handle h)
 
 // Flat offsets of the live row lengths (natural row order), rebuilt lazily
 // after a structural change.  This is the engine's "binning" step.
-dcsr_status ensure_flat(dcsr_handle h, hipStream_t s)
+dcsr_status flat_buffers(dcsr_handle h, hipStream_t s, FlatBufs *fb)
 {
-    if (!h->flat_dirty) return DCSR_OK;
-    h->pnl_valid = false;
-    if (h->pnl_calls > 0) h->pnl_prev_calls = h->pnl_calls;
-    h->pnl_calls = 0;
     const int64_t R = h->rows;
-    uint32_t *foff, *part;
-    DCSR_TRY(ws_get(h, WS_FOFF, (size_t)(R + 1) * 4, (void **)&foff));
+    DCSR_TRY(ws_get(h, WS_FOFF, (size_t)(R + 1) * 4, (void **)&fb->foff));
     // look-back block: 4 control words and one 64-bit status word per tile
-    const int64_t tiles = (R + SR_TILE - 1) / SR_TILE;
-    DCSR_TRY(ws_get_zeroed(h, WS_PART, (size_t)(4 + 2 * tiles) * 4, (void **)&part, s));
+    DCSR_TRY(ws_get_zeroed(h, WS_PART, rowscan_part_bytes(R), (void **)&fb->part, s));
     // the row map covers the live entries, which the pool bounds: sized by
     // the pool, it needs no reallocation while the matrix grows within it
     // (no read-back, also after NOSYNC updates)
     const int64_t live = live_bound(h);
-    uint32_t *rowmap;
     const int64_t ngcap = (h->pool_cap > live ? h->pool_cap : live) / GRAN + 1;
-    DCSR_TRY(ws_get(h, WS_ROWMAP, (size_t)ngcap * 4, (void **)&rowmap));
-    if (R > 0)
-        k_scan_rows<<<(unsigned)tiles, TB, 0, s>>>((uint32_t)R, h->row_len, foff, rowmap, part, fault_word(h),
-                                                   lb_spin_limit());
-    DCSR_HIP(hipGetLastError());
+    DCSR_TRY(ws_get(h, WS_ROWMAP, (size_t)ngcap * 4, (void **)&fb->rowmap));
+    return DCSR_OK;
+}
+
+void flat_rebuilt(dcsr_handle h)
+{
+    h->pnl_valid = false;
+    if (h->pnl_calls > 0) h->pnl_prev_calls = h->pnl_calls;
+    h->pnl_calls = 0;
     h->flat_dirty = false;
+}
+
+dcsr_status ensure_flat(dcsr_handle h, hipStream_t s)
+{
+    if (!h->flat_dirty) return DCSR_OK;
+    const int64_t R = h->rows;
+    FlatBufs fb;
+    DCSR_TRY(flat_buffers(h, s, &fb));
+    if (R > 0)
+        k_scan_rows<<<(unsigned)rowscan_tiles(R), TB, 0, s>>>((uint32_t)R, h->row_len, fb.foff, fb.rowmap, fb.part,
+                                                              fault_word(h), lb_spin_limit());
+    DCSR_HIP(hipGetLastError());
+    flat_rebuilt(h);
     return DCSR_OK;
 }
 

@@ -38,6 +38,7 @@
 // cannot rule out exhaustion.
 #include "dcsr_internal.h"
 #include "prims.h"
+#include "rowscan.h"
 #include "segments.h"
 
 namespace {
@@ -304,7 +305,10 @@ __global__ __launch_bounds__(TB) void k_upd_heads_count(const uint64_t *__restri
 // Pass 2 of the heads: every head writes its compacted record: ucol/usum/
 // ut for coordinate u, trow/tubeg/tlpre (probe-length offset), the row's
 // pool start, length and capacity (trs, tlen, tcap) and zeroed tins/tdel for touched row t,
-// and (pspan) touched row t for every probe span that starts in it.  Sums run left to right from 0 in batch order, as the host path
+// and (pspan) touched row t for every probe span that starts in it; (rowt)
+// rowt[row] = rtag << 32 | t, the update's tag beside the row's touched
+// index, for the move kernel's scan of the new row lengths (entries of
+// earlier updates carry older tags).  Sums run left to right from 0 in batch order, as the host path
 // accumulates B_row[j]; a run is summed from the thread's registers and
 // only a run that continues past the thread's items reads memory.  A tile's
 // records are contiguous, so they are staged in LDS and written out
@@ -322,7 +326,8 @@ __global__ __launch_bounds__(TB) void k_upd_heads_emit(const uint64_t *__restric
                                                        uint32_t *__restrict__ trs, uint32_t *__restrict__ tlen,
                                                        uint32_t *__restrict__ tcap,
                                                        uint32_t *__restrict__ tins, uint32_t *__restrict__ tdel,
-                                                       uint32_t *__restrict__ pspan)
+                                                       uint32_t *__restrict__ pspan, uint64_t *__restrict__ rowt,
+                                                       uint32_t rtag)
 {
     __shared__ uint32_t lds3[4][3];
     __shared__ int32_t s_ucol[TILE];
@@ -410,6 +415,7 @@ __global__ __launch_bounds__(TB) void k_upd_heads_emit(const uint64_t *__restric
             trs[tbase + lt] = rsj[j];
             tlen[tbase + lt] = (uint32_t)lj[j];
             tcap[tbase + lt] = rcj[j];
+            if (rowt) rowt[r] = ((uint64_t)rtag << 32) | (uint64_t)(tbase + lt);
             ++lt;
         }
         ++lu;
@@ -1047,6 +1053,69 @@ __global__ __launch_bounds__(TB) void k_upd_offsets(DevState *__restrict__ ds,
 // of its records before the cut, located by a search on their positions.
 constexpr uint32_t MV_PER = MV_SPAN / 64;  // entries per lane
 constexpr int MV_RU = 4;                   // record rounds of 64 in flight
+// The SpMV's flat row scan (rowscan.h) run by the first `ntiles`
+// workgroups of the move kernel over the row lengths the update commits:
+// row r's new length is tnewl[t] when rowt[r] carries this update's tag
+// (touched row t) and the batch goes ahead, else row_len[r].  ntiles = 0:
+// no scan (the SpMV scans on its own).
+struct RowScanArgs {
+    const int32_t *row_len;
+    const uint64_t *rowt;
+    const uint32_t *tnewl;
+    uint32_t *foff, *rowmap, *part;
+    unsigned int *fault;
+    uint32_t R, rtag, ntiles;
+    int limit;
+};
+
+__device__ void move_rowscan(const DevState *__restrict__ ds, const RowScanArgs &a, RowScanLds &L)
+{
+    const bool go = ds->go != 0;
+    const uint32_t R = a.R;
+    auto lens = [&](int64_t r0, bool full, uint32_t(&v)[RS_IPT]) {
+        uint64_t x[RS_IPT];
+        uint32_t l[RS_IPT];
+        if (full) {
+            const int4 *p = (const int4 *)(a.row_len + r0);
+            const ulonglong2 *q = (const ulonglong2 *)(a.rowt + r0);
+#pragma unroll
+            for (int k = 0; k < RS_IPT / 4; ++k) {
+                const int4 t = p[k];
+                l[4 * k] = (uint32_t)t.x;
+                l[4 * k + 1] = (uint32_t)t.y;
+                l[4 * k + 2] = (uint32_t)t.z;
+                l[4 * k + 3] = (uint32_t)t.w;
+            }
+#pragma unroll
+            for (int k = 0; k < RS_IPT / 2; ++k) {
+                const ulonglong2 t = q[k];
+                x[2 * k] = t.x;
+                x[2 * k + 1] = t.y;
+            }
+        } else {
+#pragma unroll
+            for (int j = 0; j < RS_IPT; ++j) {
+                const bool in = r0 + j < (int64_t)R;
+                l[j] = in ? (uint32_t)a.row_len[r0 + j] : 0u;
+                x[j] = in ? a.rowt[r0 + j] : 0ull;
+            }
+        }
+#pragma unroll
+        for (int j = 0; j < RS_IPT; ++j)
+            v[j] = go && (uint32_t)(x[j] >> 32) == a.rtag ? a.tnewl[(uint32_t)x[j]] : l[j];
+    };
+    // fallback: the new lengths of every earlier tile's rows
+    auto prefix = [&](int64_t n) {
+        uint32_t s = 0;
+        for (int64_t i = lane_id(); i < n; i += 64) {
+            const uint64_t x = a.rowt[i];
+            s += go && (uint32_t)(x >> 32) == a.rtag ? a.tnewl[(uint32_t)x] : (uint32_t)a.row_len[i];
+        }
+        return wave_sum_u32(s);
+    };
+    rowscan_tile(R, lens, prefix, a.foff, a.rowmap, a.part, a.fault, a.limit, a.ntiles, L);
+}
+
 struct MoveLds {
     int32_t mk[MV_SPAN];             // marks
     uint32_t tomb[MV_SPAN / 32];     // deleted entries
@@ -1064,17 +1133,27 @@ __global__ __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(4))) void k_
                                                  const uint32_t *__restrict__ tubeg,
                                                  const uint32_t *__restrict__ ucode, const uint32_t *__restrict__ upos,
                                                  const uint32_t *__restrict__ upre, int32_t *__restrict__ pcol,
-                                                 float *__restrict__ pval, const uint32_t *__restrict__ mspan)
+                                                 float *__restrict__ pval, const uint32_t *__restrict__ mspan,
+                                                 RowScanArgs rsa)
 {
-    __shared__ MoveLds sh[TB / 64];
+    __shared__ union {
+        MoveLds mv[TB / 64];
+        RowScanLds rs;
+    } sh;
+    // the first rsa.ntiles workgroups scan the new row lengths for the SpMV
+    // (whether or not the batch goes ahead); the others move entries
+    if (blockIdx.x < rsa.ntiles) {
+        move_rowscan(ds, rsa, sh.rs);
+        return;
+    }
     if (!ds->go) return;
-    MoveLds &S = sh[threadIdx.x >> 6];
+    MoveLds &S = sh.mv[threadIdx.x >> 6];
     const uint32_t total = ds->nrewrite_old;
     const uint32_t nt = ds->ntouched;
     const uint64_t base = ds->bump_saved;
     const uint32_t lane = (uint32_t)lane_id();
-    const uint64_t nwaves = (uint64_t)gridDim.x * (TB / 64);
-    for (uint64_t wave = (uint64_t)blockIdx.x * (TB / 64) + (threadIdx.x >> 6); wave * MV_SPAN < total;
+    const uint64_t nwaves = (uint64_t)(gridDim.x - rsa.ntiles) * (TB / 64);
+    for (uint64_t wave = (uint64_t)(blockIdx.x - rsa.ntiles) * (TB / 64) + (threadIdx.x >> 6); wave * MV_SPAN < total;
          wave += nwaves) {
         const uint32_t e0 = (uint32_t)(wave * MV_SPAN);
         const uint32_t e1 = total - e0 < MV_SPAN ? total : e0 + MV_SPAN;
@@ -1341,6 +1420,12 @@ __global__ __launch_bounds__(TB) void k_upd_write(DevState *ds, uint32_t report,
     }
 }
 
+__global__ void k_fill_u64(uint64_t *p, int64_t n, uint64_t v)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
 struct UpdBufs {
     uint64_t *k0, *k1;
     uint32_t *v0, *v1, *hist, *part;
@@ -1351,6 +1436,9 @@ struct UpdBufs {
     uint32_t *tt, *tp, *fin;
     // first touched row of every probe / move span (null: searched for)
     uint32_t *pspan, *mspan;
+    // tagged row -> touched-row map for the move kernel's row scan (null: no
+    // scan there)
+    uint64_t *rowt;
     long long *pplan;
     // sort: row groups (lb = gshift + cbits > 0, ndig = groups + 1) or, for
     // batches past GS_MAX_N triples, the LSD radix sort (lb = 0)
@@ -1456,6 +1544,7 @@ dcsr_status upd_bufs_src(BufSrc &src, int64_t n, UpdBufs &b)
     DCSR_TRY(take(src, WS_TOLDS, 64, (void **)&b.fin));
     // span tables grow with the matrix, so a fixed region goes without them
     b.pspan = b.mspan = nullptr;
+    b.rowt = nullptr;
     if (!src.base && !src.measure) {
         const int64_t live = live_bound(h);
         DCSR_TRY(take(src, WS_PSPAN, (size_t)((live + n) / PR_SPAN + 2) * 4, (void **)&b.pspan));
@@ -1467,7 +1556,11 @@ dcsr_status upd_bufs_src(BufSrc &src, int64_t n, UpdBufs &b)
 dcsr_status upd_bufs(dcsr_handle h, int64_t n, UpdBufs &b, hipStream_t s)
 {
     BufSrc src{h, nullptr, 0, 0, false, s};
-    return upd_bufs_src(src, n, b);
+    DCSR_TRY(upd_bufs_src(src, n, b));
+    // zeroed when first allocated: no entry carries a tag before the first
+    // update's
+    if (h->rows > 0) DCSR_TRY(ws_get_zeroed(h, WS_ROWT, (size_t)h->rows * 8, (void **)&b.rowt, s));
+    return DCSR_OK;
 }
 
 // Tile tables scanned by a last workgroup (table) only for batches of more
@@ -1520,10 +1613,15 @@ dcsr_status sort_plan_launch(dcsr_handle h, int64_t n, UpdBufs &b, hipStream_t s
     const uint32_t *val = which ? b.v1 : b.v0;
     const unsigned ht = (unsigned)((n + TILE - 1) / TILE);
     const bool table = ht > HEADS_DIRECT_TILES;
+    if (b.rowt && ++h->upd_tag == 0u) {
+        // tags wrapped: clear the map, restart at 1
+        k_fill_u64<<<grid_for(h->rows, TB), TB, 0, s>>>(b.rowt, h->rows, 0ull);
+        h->upd_tag = 1u;
+    }
     k_upd_heads_count<<<ht, TB, 0, s>>>(key, n, h->cbits, h->row_len, b.tt, b.tp, b.fin, h->ds, table);
     k_upd_heads_emit<<<ht, TB, 0, s>>>(key, val, n, h->cbits, b.tt, h->ds, table, b.ucol, b.usum, b.ut, b.trow,
                                        b.tubeg, h->row_len, h->row_start, h->row_cap, b.tp, b.tlpre, b.trs, b.tlen,
-                                       b.tcap, b.tins, b.tdel, b.pspan);
+                                       b.tcap, b.tins, b.tdel, b.pspan, b.rowt, h->upd_tag);
     k_upd_probe<<<probe_blocks(h, n), TB, 0, s>>>(h->ds, h->row_start, h->row_len, h->pool_col, h->pool_val,
                                                   b.tubeg, b.tlpre, b.trs, b.tlen, b.ucol, b.usum, b.uval, b.upos,
                                                   b.ucode, b.tins, b.tdel, b.pspan);
@@ -1563,13 +1661,24 @@ dcsr_status commit_launch(dcsr_handle h, const UpdBufs &b, int64_t n, int64_t mo
     int64_t blocks = (waves + 3) / 4;
     if (blocks > MV_BLOCKS) blocks = MV_BLOCKS;
     if (blocks < 1) blocks = 1;
-    k_upd_move<<<(unsigned)blocks, TB, 0, s>>>(h->ds, b.tpre, b.trs, b.tubeg, b.ucode, b.upos, b.upre, h->pool_col,
-                                               h->pool_val, b.mspan);
+    // the SpMV's row scan rides along in the move launch (its own first
+    // workgroups) when the handle's workspace has the tagged row map
+    RowScanArgs rsa{};
+    const bool scan = b.rowt != nullptr && h->rows > 0;
+    if (scan) {
+        FlatBufs fb;
+        DCSR_TRY(flat_buffers(h, s, &fb));
+        rsa = RowScanArgs{h->row_len, b.rowt, b.tnewl, fb.foff, fb.rowmap, fb.part, fault_word(h),
+                          (uint32_t)h->rows, h->upd_tag, (uint32_t)rowscan_tiles(h->rows), lb_spin_limit()};
+    }
+    k_upd_move<<<(unsigned)(blocks + rsa.ntiles), TB, 0, s>>>(h->ds, b.tpre, b.trs, b.tubeg, b.ucode, b.upos, b.upre,
+                                                              h->pool_col, h->pool_val, b.mspan, rsa);
     k_upd_write<<<grid_for(n, TB), TB, 0, s>>>(h->ds, report ? 1u : 0u, n, b.ut, b.trow, b.tubeg, b.tpre, b.tnewl, b.tmode, b.ucol, b.uval,
                                                 b.upos, b.ucode, b.upre, h->pool_col, h->pool_val, h->row_start,
                                                 h->row_len, h->row_cap, seq ? h->dsnap : nullptr, seq);
     DCSR_HIP(hipGetLastError());
-    h->flat_dirty = true;
+    if (scan) flat_rebuilt(h);
+    else h->flat_dirty = true;
     return DCSR_OK;
 }
 

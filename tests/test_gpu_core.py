@@ -325,12 +325,17 @@ def test_nosync_errors_are_sticky_and_leave_matrix_unchanged(torch):
     say("nosync bad index")
     m.UpdateMatrix(np.array([3, R + 5], np.int32), np.array([1, 1], np.int32), np.array([1.0, 2.0], np.float32),
                    flags=UPDATE_NOSYNC)
+    x = rng.uniform(-1, 1, R).astype(np.float32)
+    # the rejected batch's launch still rebuilt the SpMV's row offsets, from
+    # the unchanged lengths
+    fp64_spmv_check(A, x, spmv_host(m, x))
     assert m.status() == 2  # DCSR_ERR_BAD_INDEX
     assert m.status() == 0  # read clears it
     assert_same_structure(to_host_csr(m), A)
     say("nosync pool exhaustion without growth")
     br, bc, bv = random_coo(rng, R, R, 6000, 0.0, 0.0, "sym")
     m.UpdateMatrix(br, bc, bv, flags=UPDATE_NOSYNC | UPDATE_NOGROW)
+    fp64_spmv_check(A, x, spmv_host(m, x))
     assert m.status() == 3  # DCSR_ERR_POOL_EXHAUSTED
     assert_same_structure(to_host_csr(m), A)
     say("nosync batch that fits, after the rejected ones")
@@ -340,6 +345,7 @@ def test_nosync_errors_are_sticky_and_leave_matrix_unchanged(torch):
     ur, uc, uv = rows_of[pick].astype(np.int32), sc[pick].astype(np.int32), (0.5 * sv[pick]).astype(np.float32)
     m.UpdateMatrix(ur, uc, uv, flags=UPDATE_NOSYNC)  # adds only: values in place, no claims
     A = O.update(A, ur, uc, uv)
+    fp64_spmv_check(A, x, spmv_host(m, x))
     assert m.status() == 0
     assert_same_structure(to_host_csr(m), A)
     m.close()
