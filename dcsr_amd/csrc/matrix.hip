@@ -530,6 +530,8 @@ dcsr_status dcsr_create(dcsr_handle *out, int64_t rows, int64_t cols, const dcsr
     h->pnl_min_nnz = 16ll << 20;
     if (const char *e = getenv("DCSR_SPMV_PANEL_COLS")) h->pnl_cols = atoll(e) > 0 ? atoll(e) : h->pnl_cols;
     if (const char *e = getenv("DCSR_SPMV_PANEL_MIN_NNZ")) h->pnl_min_nnz = atoll(e) >= 0 ? atoll(e) : h->pnl_min_nnz;
+    // the update tag's starting value (a test of its wrap-around)
+    if (const char *e = getenv("DCSR_UPDATE_TAG_START")) h->upd_tag = (uint32_t)strtoul(e, nullptr, 0);
     dcsr_status st = alloc_rows(h);
     if (st == DCSR_OK) st = alloc_pool(h, h->cfg.min_pool);
     if (st == DCSR_OK && hipMalloc(&h->ds, sizeof(DevState)) != hipSuccess) st = DCSR_ERR_OOM;

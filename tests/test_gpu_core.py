@@ -198,6 +198,36 @@ def test_nosync_snapshots_keep_the_bounds_tight(torch):
     m.close()
 
 
+def test_update_tag_wraps_and_spmv_offsets_stay_right(torch, monkeypatch):
+    """Each update tags the rows it touches (tag << 32 | touched index) so
+    that its move launch can scan the new row lengths for the next SpMV.
+    Started two updates before the 32-bit tag wraps, the chain crosses the
+    wrap (the map is cleared, tags restart at 1); every SpMV after an update,
+    and the structure, must match the oracle."""
+    from dcsr_amd import DcsrMatrix, UPDATE_NOSYNC
+    monkeypatch.setenv("DCSR_UPDATE_TAG_START", str(0xFFFFFFFE))
+    R, C = 3000, 2000
+    rng = np.random.default_rng(41)
+    r, c, v = random_coo(rng, R, C, 30000, 0.0, 0.0, "sym")
+    m = DcsrMatrix(R, C, pool_factor=4.0)
+    m.LoadMatrix(r, c, v)
+    A = O.coo_to_csr(R, C, r, c, v)
+    x = rng.uniform(-1, 1, C).astype(np.float32)
+    for k in range(5):
+        # rows drawn from a shifting window: some rows touched by one update
+        # and not the next
+        br = rng.integers(k * 400, k * 400 + 1200, 3000).astype(np.int32)
+        bc = rng.integers(0, C, 3000).astype(np.int32)
+        bv = rng.uniform(-1, 1, 3000).astype(np.float32)
+        say(f"tag wrap update {k}")
+        m.UpdateMatrix(br, bc, bv, flags=UPDATE_NOSYNC if k % 2 else 0)
+        A = O.update(A, br, bc, bv)
+        fp64_spmv_check(A, x, spmv_host(m, x))
+    assert m.status() == 0
+    assert_same_structure(to_host_csr(m), A)
+    m.close()
+
+
 def test_powerlaw_with_hub_rows(torch):
     """Rows longer than the hub threshold go through the split SpMV path."""
     from dcsr_amd import DcsrMatrix
