@@ -77,6 +77,9 @@ def test_batch_steps_match_per_matrix_oracle(dev, graph):
         code, per = b.status()
         assert code == 0 and not per.any()
         A = [O.update(Am, *bt) for Am, bt in zip(A, batches)]
+        # the live count the step's workgroups summed up (one atomic per
+        # workgroup, not per matrix)
+        assert b.info()["nnz"] == sum(Am.nnz for Am in A)
         got = split_csr(b, *[t.cpu().numpy() for t in b.to_csr()])
         y = d_y.cpu().numpy()
         for m in range(count):
