@@ -84,23 +84,40 @@ __global__ __launch_bounds__(TB) void k_sg_count(uint32_t R, const uint32_t *__r
     s_st[w][lane] = st;
     s_acc[w][lane] = 0;
     __builtin_amdgcn_wave_barrier();
-    for (uint32_t b = 0; b < tot; b += 64) {
-        const uint32_t e = b + lane;
-        uint32_t j = 64, p = 0;
-        if (e < tot) {
-            j = (uint32_t)find_seg64(s_ex[w], e);
-            p = (uint32_t)b_len[a_col[(uint64_t)s_st[w][j] + (e - s_ex[w][j])]];
+    // four 64-entry chunks per round: their A-column loads, then their B-row
+    // length loads, each issued together (two round trips per 256 entries)
+    constexpr int CU = 4;
+    for (uint32_t b = 0; b < tot; b += 64 * CU) {
+        uint32_t j[CU], p[CU];
+        int32_t c[CU];
+#pragma unroll
+        for (int u = 0; u < CU; ++u) {
+            const uint32_t e = b + (uint32_t)u * 64u + lane;
+            j[u] = 64;
+            c[u] = -1;
+            if (e < tot) {
+                j[u] = (uint32_t)find_seg64(s_ex[w], e);
+                c[u] = a_col[(uint64_t)s_st[w][j[u]] + (e - s_ex[w][j[u]])];
+            }
         }
-        p = wave_seg_incl_scan_u32(p, j);
-        const uint32_t jn = __shfl_down(j, 1);
-        if (j < 64 && (lane == 63 || jn != j)) s_acc[w][j] += p;
-        __builtin_amdgcn_wave_barrier();
+#pragma unroll
+        for (int u = 0; u < CU; ++u) p[u] = c[u] >= 0 ? (uint32_t)b_len[c[u]] : 0u;
+#pragma unroll
+        for (int u = 0; u < CU; ++u) {
+            const uint32_t q = wave_seg_incl_scan_u32(p[u], j[u]);
+            const uint32_t jn = __shfl_down(j[u], 1);
+            if (j[u] < 64 && (lane == 63 || jn != j[u])) s_acc[w][j[u]] += q;
+            __builtin_amdgcn_wave_barrier();
+        }
     }
     if (r < R) P[r] = s_acc[w][lane];
 }
 
 // Row classes (list appends aggregated per workgroup: one global atomic per
-// class per workgroup) and per-block 64-bit P sums.
+// class per workgroup of SG_CR * TB rows) and per-block 64-bit P sums.  Each
+// thread classifies SG_CR rows; with 256 rows per workgroup the class
+// counters took ~60K contended atomics per product (94 us on 2M rows).
+constexpr int SG_CR = 8;
 __global__ __launch_bounds__(TB) void k_sg_classify(uint32_t R, const uint32_t *__restrict__ P,
                                                     uint32_t *__restrict__ lists, uint32_t list_stride,
                                                     SgState *st, uint32_t *__restrict__ cnt,
@@ -110,39 +127,51 @@ __global__ __launch_bounds__(TB) void k_sg_classify(uint32_t R, const uint32_t *
     __shared__ uint32_t s_cnt[NCLS], s_base[NCLS];
     if (threadIdx.x < NCLS) s_cnt[threadIdx.x] = 0;
     __syncthreads();
-    const uint32_t r = blockIdx.x * TB + threadIdx.x;
-    uint32_t p = 0;
-    int cls = -1;
-    if (r < R) {
-        p = P[r];
-        // team rows in classes by rank-sort width (2, 4, 6 or 8 keys per
-        // lane), so the four teams of a wave run the same sort over similar
-        // row sizes
-        cls = p == 0                         ? -1
-              : (p <= TEAM_MAX && team_ok)   ? CL_TEAM + (int)((p - 1) / (2 * TEAM))
-              : p <= WAVE_MAX                ? CL_WAVE
-              : p <= BLOCK_MAX               ? CL_BLOCK
-                                             : CL_HUB;
-        if (p == 0) cnt[r] = 0;
-    }
-    // slot of this row inside its workgroup's share of its class list
-    uint32_t slot = 0;
+    const uint32_t r0 = blockIdx.x * (TB * SG_CR) + threadIdx.x;
+    uint32_t p[SG_CR], slot[SG_CR];
+    int cls[SG_CR];
 #pragma unroll
-    for (int c = 0; c < NCLS; ++c) {
-        const uint64_t m = __ballot(cls == c);
-        if (!m) continue;
-        const int leader = __ffsll((long long)m) - 1;
-        uint32_t base = 0;
-        if (lane_id() == leader) base = atomicAdd(&s_cnt[c], (uint32_t)__popcll(m));
-        base = __shfl(base, leader);
-        if (cls == c) slot = base + (uint32_t)__popcll(m & lanemask_lt());
+    for (int q = 0; q < SG_CR; ++q) {
+        const uint32_t r = r0 + (uint32_t)q * TB;
+        p[q] = r < R ? P[r] : 0u;
+    }
+    unsigned long long v = 0;
+#pragma unroll
+    for (int q = 0; q < SG_CR; ++q) {
+        const uint32_t r = r0 + (uint32_t)q * TB;
+        cls[q] = -1;
+        slot[q] = 0;
+        if (r < R) {
+            // team rows in classes by rank-sort width (2, 4, 6 or 8 keys per
+            // lane), so the four teams of a wave run the same sort over similar
+            // row sizes
+            cls[q] = p[q] == 0                         ? -1
+                     : (p[q] <= TEAM_MAX && team_ok)   ? CL_TEAM + (int)((p[q] - 1) / (2 * TEAM))
+                     : p[q] <= WAVE_MAX                ? CL_WAVE
+                     : p[q] <= BLOCK_MAX               ? CL_BLOCK
+                                                       : CL_HUB;
+            if (p[q] == 0) cnt[r] = 0;
+        }
+        v += p[q];
+        // slot of this row inside its workgroup's share of its class list
+#pragma unroll
+        for (int c = 0; c < NCLS; ++c) {
+            const uint64_t m = __ballot(cls[q] == c);
+            if (!m) continue;
+            const int leader = __ffsll((long long)m) - 1;
+            uint32_t base = 0;
+            if (lane_id() == leader) base = atomicAdd(&s_cnt[c], (uint32_t)__popcll(m));
+            base = __shfl(base, leader);
+            if (cls[q] == c) slot[q] = base + (uint32_t)__popcll(m & lanemask_lt());
+        }
     }
     __syncthreads();
     if (threadIdx.x < NCLS && s_cnt[threadIdx.x])
         s_base[threadIdx.x] = atomicAdd(&st->count[threadIdx.x], s_cnt[threadIdx.x]);
     __syncthreads();
-    if (cls >= 0) lists[(uint64_t)cls * list_stride + s_base[cls] + slot] = r;
-    unsigned long long v = p;
+#pragma unroll
+    for (int q = 0; q < SG_CR; ++q)
+        if (cls[q] >= 0) lists[(uint64_t)cls[q] * list_stride + s_base[cls[q]] + slot[q]] = r0 + (uint32_t)q * TB;
 #pragma unroll
     for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o);
     if (lane_id() == 0) ws[threadIdx.x >> 6] = v;
@@ -377,13 +406,20 @@ __device__ void sg_row(const SgRowInfo &ri, const int32_t *__restrict__ a_col, c
         gbase += tot;
         team_sync<NT>();
     }
-    // 2. sort by (column, generation): rank sort for quarter-wave teams,
-    // bitonic network over LDS for wider ones (padding sorts last)
+    // 2. sort by (column, generation): rank sort for quarter-wave teams and
+    // for wave rows of up to 256 partials, bitonic network over LDS for the
+    // others (padding sorts last)
     if constexpr (NT < 64 && KPL != 0) {
         team_rank_sort<NT, KPL, K>(key, val, P);
     } else if constexpr (NT < 64) {
         if (P <= 4 * NT) team_rank_sort<NT, 4, K>(key, val, P);
         else team_rank_sort<NT, 8, K>(key, val, P);
+    } else if (NT == 64 && P <= 4 * NT) {
+        // wave rows of up to 256 partials: the rank sort's P * 4 compares per
+        // lane cost less than the bitonic network's LDS passes (k_sg_wave
+        // 808 -> 538 us on the 2M-row Poisson(8) product); up to 512 with 8
+        // keys per lane measured the same as the network
+        team_rank_sort<NT, 4, K>(key, val, P);
     } else {
         const uint32_t n2 = next_pow2(P < 2 * NT ? 2 * NT : P);
         for (uint32_t p = P + t; p < n2; p += NT) key[p] = (K)~0ull;
@@ -693,7 +729,7 @@ dcsr_status dcsr_spgemm(dcsr_handle A, dcsr_handle B, dcsr_handle C, dcsr_stream
     uint32_t *P, *Poff, *cnt, *coff, *lists, *part;
     SgState *st;
     unsigned long long *bsum;
-    const unsigned nbR = grid_for(R, TB);
+    const unsigned nbR = grid_for(R, TB * SG_CR);  // classify workgroups
     DCSR_TRY(ws_get(C, WS_SG_P, (size_t)(R + 1) * 4, (void **)&P));
     DCSR_TRY(ws_get(C, WS_SG_POFF, (size_t)(R + 1) * 4, (void **)&Poff));
     DCSR_TRY(ws_get(C, WS_SG_CNT, (size_t)(R + 1) * 4, (void **)&cnt));
