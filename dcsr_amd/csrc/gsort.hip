@@ -15,8 +15,10 @@
 //   3. k_gs_sort: one workgroup per group sorts the group's pairs by their
 //      low lb key bits (all of its keys share the bits above), a stable LSD
 //      sort with 8- or 9-bit digits (whichever takes fewer passes); groups of
-//      up to GS_CAP pairs sort in LDS,
-//      larger ones out of global memory with the same ranking.
+//      up to GS_CAP pairs sort in LDS (one 32-bit word per pair where the
+//      local key's unranked bits and the pair's position fit one, else a
+//      key and a position array), larger ones out of global memory with the
+//      same ranking.
 //
 // The result is exactly the stable sort of the batch by key, which the LSD
 // radix sort (prims.hip) gives in one count/scan/scatter round per 8 key
@@ -221,9 +223,9 @@ __global__ __launch_bounds__(GT) void k_gs_scatter(const uint64_t *__restrict__ 
 // ---------------------------------------------------------------------------
 constexpr int SJ = GS_CAP / GT;  // items per lane in a full LDS group
 
-template <int J, int DB = 8>
+template <int J, int DB = 8, typename C = uint32_t>
 __device__ __forceinline__ void rank_chunk(const uint32_t (&dg)[J], const bool (&real)[J], int jmax,
-                                           uint32_t (&rnk)[J], uint32_t (*wcnt)[1 << DB], uint32_t *loff,
+                                           uint32_t (&rnk)[J], C (*wcnt)[1 << DB], uint32_t *loff,
                                            uint32_t *lds4)
 {
     constexpr int ND = 1 << DB;     // digits
@@ -250,7 +252,7 @@ __device__ __forceinline__ void rank_chunk(const uint32_t (&dg)[J], const bool (
         uint32_t c = 0;
         if (lane == leader) {
             c = wcnt[w][dg[j]];
-            wcnt[w][dg[j]] = c + (uint32_t)__popcll(peers);
+            wcnt[w][dg[j]] = (C)(c + (uint32_t)__popcll(peers));
         }
         c = __shfl(c, leader);
         rnk[j] = c + (uint32_t)__popcll(peers & lt);
@@ -267,7 +269,7 @@ __device__ __forceinline__ void rank_chunk(const uint32_t (&dg)[J], const bool (
 #pragma unroll
             for (int i = 0; i < GT / 64; ++i) {
                 const uint32_t x = wcnt[i][d];
-                wcnt[i][d] = s;
+                wcnt[i][d] = (C)s;
                 s += x;
             }
             sd[k] = s;
@@ -366,6 +368,98 @@ __device__ void group_sort_lds(GsLds<K, DB> &S, const uint64_t *__restrict__ kin
     }
 }
 
+// LDS sort of cnt <= GS_CAP pairs of a group whose local keys have at most
+// GS_PK_BITS + DB bits: one 32-bit word per pair and pass, the key bits not
+// yet ranked above the pair's position in the group (GS_PK_POS bits).  The
+// first pass ranks the digits straight from the loaded keys; the last one
+// leaves the positions in sorted order, and each pair's key and value are
+// gathered from the group's input by position.  Half the LDS of the
+// two-array sort (16-bit per-wave digit counts too): five workgroups fit a
+// CU, so the bench-sized batch of ~1,270 groups runs in one round.
+constexpr int GS_PK_POS = 12;                 // position bits (GS_CAP <= 4096)
+constexpr int GS_PK_BITS = 32 - GS_PK_POS;    // key bits a word carries
+static_assert(GS_CAP <= (1 << GS_PK_POS), "a position fits its field");
+static_assert(GS_CAP <= 65535, "16-bit digit counts");
+
+template <int DB>
+struct GsPacked {
+    uint32_t w[2][GS_CAP];
+    uint16_t wcnt[GT / 64][1 << DB];
+    uint32_t loff[1 << DB];
+    uint32_t lds4[4];
+};
+
+template <int DB>
+__device__ void group_sort_packed(GsPacked<DB> &S, const uint64_t *__restrict__ kin,
+                                  const uint32_t *__restrict__ vin, uint64_t *__restrict__ kout,
+                                  uint32_t *__restrict__ vout, int64_t s0, int cnt, int lb)
+{
+    constexpr uint32_t DM = (1u << DB) - 1u;
+    constexpr uint32_t PM = (1u << GS_PK_POS) - 1u;
+    const int w = threadIdx.x >> 6, lane = lane_id();
+    const uint64_t lmask = (1ull << lb) - 1ull;
+    // positions per wave: a multiple of 64 covering cnt / 4; item j of a
+    // lane sits at w * pw + 64 j + lane in every pass
+    const int jmax = (cnt + GT - 1) / GT;
+    const int pw = jmax * 64;
+    uint32_t dg[SJ], rnk[SJ], wd[SJ];
+    bool real[SJ];
+    {
+        uint64_t kl[SJ];
+#pragma unroll
+        for (int j = 0; j < SJ; ++j) {
+            const int p = w * pw + j * 64 + lane;
+            real[j] = j < jmax && p < cnt;
+            kl[j] = real[j] ? kin[s0 + p] : 0ull;
+        }
+#pragma unroll
+        for (int j = 0; j < SJ; ++j) {
+            const uint64_t lk = kl[j] & lmask;
+            dg[j] = (uint32_t)lk & DM;
+            wd[j] = ((uint32_t)(lk >> DB) << GS_PK_POS) | (uint32_t)(w * pw + j * 64 + lane);
+        }
+    }
+    int cur = 0;
+    for (int sh = 0;; sh += DB) {
+        rank_chunk<SJ, DB, uint16_t>(dg, real, jmax, rnk, S.wcnt, S.loff, S.lds4);
+#pragma unroll
+        for (int j = 0; j < SJ; ++j) {
+            if (!real[j]) continue;
+            S.w[cur][S.loff[dg[j]] + S.wcnt[w][dg[j]] + rnk[j]] = wd[j];
+        }
+        __syncthreads();
+        if (sh + DB >= lb) break;
+        // the next digit: the lowest key bits the words still carry
+#pragma unroll
+        for (int j = 0; j < SJ; ++j) {
+            const int p = w * pw + j * 64 + lane;
+            wd[j] = real[j] ? S.w[cur][p] : 0u;
+            dg[j] = (wd[j] >> GS_PK_POS) & DM;
+            wd[j] = (((wd[j] >> GS_PK_POS) >> DB) << GS_PK_POS) | (wd[j] & PM);
+        }
+        cur ^= 1;
+    }
+    // S.w[cur]: the positions in sorted order; the key and value gathers of
+    // every item in flight before the first store
+    uint64_t kk[SJ];
+    uint32_t vv[SJ];
+#pragma unroll
+    for (int j = 0; j < SJ; ++j) {
+        const int i = j * GT + threadIdx.x;
+        const uint32_t src = i < cnt ? (S.w[cur][i] & PM) : 0u;
+        kk[j] = kin[s0 + src];
+        vv[j] = vin[s0 + src];
+    }
+#pragma unroll
+    for (int j = 0; j < SJ; ++j) {
+        const int i = j * GT + threadIdx.x;
+        if (i < cnt) {
+            kout[s0 + i] = kk[j];
+            vout[s0 + i] = vv[j];
+        }
+    }
+}
+
 // A group larger than GS_CAP: stable LSD passes out of global memory by this
 // workgroup alone, chunk after chunk (each chunk ranked as above, each
 // digit's running offset carried in LDS).  Ping-pongs between the group's
@@ -381,8 +475,8 @@ __device__ void group_sort_big(GsBig &S, uint64_t *__restrict__ a_k, uint32_t *_
                                uint64_t *__restrict__ b_k, uint32_t *__restrict__ b_v, int64_t s0, int64_t cnt,
                                int lb)
 {
-    constexpr int BJ = 4;                 // items per lane per chunk
-    constexpr int CH = GT * BJ;           // 1024 pairs per chunk
+    constexpr int BJ = 2;                 // items per lane per chunk
+    constexpr int CH = GT * BJ;           // 512 pairs per chunk
     const int w = threadIdx.x >> 6, lane = lane_id();
     uint64_t *sk = a_k, *dk = b_k;
     uint32_t *sv = a_v, *dv = b_v;
@@ -474,6 +568,34 @@ __global__ __launch_bounds__(GT) void k_gs_sort(uint64_t *__restrict__ kin, uint
     }
 }
 
+// The same with the one-word LDS sort (lb <= GS_PK_BITS + DB).
+template <int DB>
+__global__ __launch_bounds__(GT) __attribute__((amdgpu_waves_per_eu(5))) void k_gs_sort_packed(
+    uint64_t *__restrict__ kin, uint32_t *__restrict__ vin, uint64_t *__restrict__ kout, uint32_t *__restrict__ vout,
+    int lb, int ndig, const uint32_t *__restrict__ gbase)
+{
+    __shared__ union {
+        GsPacked<DB> pk;
+        GsBig big;
+    } S;
+    const int g = blockIdx.x;
+    const int64_t s0 = gbase[g];
+    const int64_t cnt = (int64_t)gbase[g + 1] - s0;
+    if (cnt <= 0) return;
+    if (g == ndig - 1) {
+        for (int64_t i = threadIdx.x; i < cnt; i += GT) {
+            kout[s0 + i] = kin[s0 + i];
+            vout[s0 + i] = vin[s0 + i];
+        }
+        return;
+    }
+    if (cnt <= GS_CAP) {
+        group_sort_packed<DB>(S.pk, kin, vin, kout, vout, s0, (int)cnt, lb);
+    } else {
+        group_sort_big(S.big, kin, vin, kout, vout, s0, cnt, lb);
+    }
+}
+
 }  // namespace
 
 size_t gs_hist_words(int64_t n, int ndig)
@@ -498,9 +620,12 @@ hipError_t group_sort_pairs(uint64_t *k0, uint32_t *v0, uint64_t *k1, uint32_t *
     k_gs_scatter<<<(unsigned)nb, GT, 0, s>>>(k0, v0, k1, v1, n, lb, ndig, dbits, hist, nb, dtot, gbase);
     // 9-bit digits where they save a pass over the lb local bits
     const bool nine = (lb + 8) / 9 < (lb + 7) / 8;
-    if (lb <= 32) {
-        if (nine) k_gs_sort<uint32_t, 9><<<ndig, GT, 0, s>>>(k1, v1, k0, v0, lb, ndig, gbase);
-        else k_gs_sort<uint32_t, 8><<<ndig, GT, 0, s>>>(k1, v1, k0, v0, lb, ndig, gbase);
+    if (lb <= GS_PK_BITS + (nine ? 9 : 8)) {
+        if (nine) k_gs_sort_packed<9><<<ndig, GT, 0, s>>>(k1, v1, k0, v0, lb, ndig, gbase);
+        else k_gs_sort_packed<8><<<ndig, GT, 0, s>>>(k1, v1, k0, v0, lb, ndig, gbase);
+    } else if (lb <= 32) {
+        // lb in 29..32 here, where 9-bit digits take as many passes as 8-bit
+        k_gs_sort<uint32_t, 8><<<ndig, GT, 0, s>>>(k1, v1, k0, v0, lb, ndig, gbase);
     } else {
         if (nine) k_gs_sort<uint64_t, 9><<<ndig, GT, 0, s>>>(k1, v1, k0, v0, lb, ndig, gbase);
         else k_gs_sort<uint64_t, 8><<<ndig, GT, 0, s>>>(k1, v1, k0, v0, lb, ndig, gbase);

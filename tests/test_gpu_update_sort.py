@@ -3,8 +3,10 @@
 Batches of up to 2^21 triples are sorted by row group (rows split into groups
 of 2^gshift consecutive rows: one scatter, then one workgroup sorts each
 group; gsort.hip); larger batches use the LSD radix sort.  A group of up to
-4,096 triples sorts in LDS with 32-bit local keys when gshift + cbits <= 32
-and 64-bit ones otherwise; a larger group sorts out of global memory.  Each
+3,072 triples sorts in LDS: in one 32-bit word per triple when its local key
+(gshift + cbits bits) has at most 20 bits beyond the first digit (28 with
+8-bit digits, 29 with 9-bit ones), else with 32-bit local keys up to 32 bits
+and 64-bit ones beyond; a larger group sorts out of global memory.  Each
 case below is built so that one of these paths carries it; structure and
 update values are compared bit-exactly with the oracle, SpMV within 1e-5 of
 FP64 (conditioned)."""
@@ -76,6 +78,28 @@ def test_groups_in_lds_powerlaw(torch):
         batches.append((np.concatenate([br, br[dup]]), np.concatenate([bc, bc[dup]]),
                         np.concatenate([bv, bv[dup]])))
     run_chain(R, R, (r, c, v), batches, 13)
+
+
+@pytest.mark.parametrize("R,n,lb", [(100_000, 300_000, 25), (1 << 20, 600_000, 30)])
+def test_group_key_widths(torch, R, n, lb):
+    """Groups in LDS at both key widths of the 32-bit sort: 100,000 rows and
+    300,000 triples give groups of 2^8 rows and 25-bit local keys (9-bit
+    digits, one word per triple); 2^20 rows and 600,000 triples give groups
+    of 2^10 rows and 30-bit local keys (8-bit digits, key and position
+    arrays)."""
+    cbits = (R - 1).bit_length()
+    want = max(n // 512, 1)
+    gshift = 0
+    while (R + (1 << gshift) - 1) >> gshift > want:
+        gshift += 1
+    assert gshift + cbits == lb
+    say(f"key width R={R} n={n} lb={lb}")
+    rng = np.random.default_rng(R % 1009 + n)
+    base = random_coo(rng, R, R, 400_000, 0.0, 0.0, "sym")
+    batches = [random_coo(rng, R, R, n, 0.2, 0.01, "sym") for _ in range(2)]
+    br, bc, bv = batches[1]
+    br[:5000], bc[:5000] = base[0][:5000], base[1][:5000]
+    run_chain(R, R, base, batches, 7)
 
 
 def test_oversized_groups_sort_in_global_memory(torch):
