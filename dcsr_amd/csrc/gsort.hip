@@ -151,12 +151,10 @@ __global__ __launch_bounds__(GT) void k_gs_scatter(const uint64_t *__restrict__ 
         const uint32_t d = real ? group_digit(key[j], lb, ndig) : (uint32_t)(ndig - 1);
         const uint64_t peers = wave_match_bits(d, dbits);
         const int leader = __ffsll((long long)peers) - 1;
-        uint32_t b = 0;
-        if (lane == leader) {
-            b = wcnt[w][d];
-            wcnt[w][d] = b + (uint32_t)__popcll(peers);
-        }
-        b = __shfl(b, leader);
+        // every lane of the digit reads its count before the leader adds to
+        // it (a wave's LDS accesses complete in order): no broadcast needed
+        const uint32_t b = wcnt[w][d];
+        if (lane == leader) wcnt[w][d] = b + (uint32_t)__popcll(peers);
         rnk[j] = b + (uint32_t)__popcll(peers & lt);
         dg[j] = d;
     }
@@ -249,12 +247,10 @@ __device__ __forceinline__ void rank_chunk(const uint32_t (&dg)[J], const bool (
             peers &= ((dg[j] >> b) & 1u) ? m : ~m;
         }
         const int leader = __ffsll((long long)peers) - 1;
-        uint32_t c = 0;
-        if (lane == leader) {
-            c = wcnt[w][dg[j]];
-            wcnt[w][dg[j]] = (C)(c + (uint32_t)__popcll(peers));
-        }
-        c = __shfl(c, leader);
+        // every lane of the digit reads its count before the leader adds to
+        // it (a wave's LDS accesses complete in order): no broadcast needed
+        const uint32_t c = wcnt[w][dg[j]];
+        if (lane == leader) wcnt[w][dg[j]] = (C)(c + (uint32_t)__popcll(peers));
         rnk[j] = c + (uint32_t)__popcll(peers & lt);
     }
     __syncthreads();

@@ -217,12 +217,10 @@ __global__ __launch_bounds__(RS_THREADS) void k_rs_scatter(const uint64_t *__res
         uint32_t d = (uint32_t)(key[j] >> shift) & 255u;
         uint64_t peers = wave_match_digit8(d, ~0ull);
         int leader = __ffsll((long long)peers) - 1;
-        uint32_t b = 0;
-        if (lane == leader) {
-            b = wcnt[w][d];
-            wcnt[w][d] = b + (uint32_t)__popcll(peers);
-        }
-        b = __shfl(b, leader);
+        // every lane of the digit reads its count before the leader adds to
+        // it (a wave's LDS accesses complete in order): no broadcast needed
+        const uint32_t b = wcnt[w][d];
+        if (lane == leader) wcnt[w][d] = b + (uint32_t)__popcll(peers);
         rnk[j] = b + (uint32_t)__popcll(peers & lt);
     }
     __syncthreads();
