@@ -176,12 +176,10 @@ __device__ void lds_radix_pass(const uint32_t *ks, const float *vs, uint32_t *kd
             const uint64_t act = __ballot(ok);
             const uint64_t peers = wave_match_digit8(d, act);
             const int leader = __ffsll((long long)peers) - 1;
-            uint32_t b = 0;
-            if (ok && lane == leader) {
-                b = wcnt[w][d];
-                wcnt[w][d] = b + (uint32_t)__popcll(peers);
-            }
-            b = __shfl(b, leader < 0 ? 0 : leader);
+            // every lane of the digit reads its count before the leader adds
+            // to it (a wave's LDS accesses complete in order): no broadcast
+            const uint32_t b = wcnt[w][d];
+            if (ok && lane == leader) wcnt[w][d] = b + (uint32_t)__popcll(peers);
             rnk[j] = b + (uint32_t)__popcll(peers & lt);
         }
     }
