@@ -120,13 +120,15 @@ __global__ __launch_bounds__(GT) void k_gs_scatter(const uint64_t *__restrict__ 
     const int64_t base = tile * GTILE;
     const int64_t wbase = base + (int64_t)w * (GIPT * 64);
     // every load first: the tile's offsets and the digit totals of this
-    // thread's digits, then the pairs
+    // thread's digits (dpt contiguous digits per thread, spread over every
+    // thread), then the pairs
     uint32_t ho[DPT], dt[DPT];
-    const int d0 = (int)threadIdx.x * DPT;
+    const int dpt = (ndig + GT - 1) / GT;
+    const int d0 = (int)threadIdx.x * dpt;
 #pragma unroll
     for (int q = 0; q < DPT; ++q) {
         const int d = d0 + q;
-        const bool in = d < ndig;
+        const bool in = q < dpt && d < ndig;
         ho[q] = in ? hist[gs_hist_index(d, tile, nb)] : 0u;
         dt[q] = in ? dtot[d] : 0u;
     }
@@ -166,7 +168,7 @@ __global__ __launch_bounds__(GT) void k_gs_scatter(const uint64_t *__restrict__ 
         for (int q = 0; q < DPT; ++q) {
             const int d = d0 + q;
             uint32_t c = 0;
-            if (d < ndig) {
+            if (q < dpt && d < ndig) {
 #pragma unroll
                 for (int i = 0; i < GT / 64; ++i) {
                     const uint32_t x = wcnt[i][d];
@@ -185,7 +187,7 @@ __global__ __launch_bounds__(GT) void k_gs_scatter(const uint64_t *__restrict__ 
 #pragma unroll
         for (int q = 0; q < DPT; ++q) {
             const int d = d0 + q;
-            if (d < ndig) {
+            if (q < dpt && d < ndig) {
                 lofs[d] = lp + cs[q];
                 gofs[d] = gp + gs[q] + ho[q];
                 if (tile == 0) gbase_out[d] = gp + gs[q];
@@ -202,12 +204,16 @@ __global__ __launch_bounds__(GT) void k_gs_scatter(const uint64_t *__restrict__ 
     }
     __syncthreads();
     const int64_t valid = (n - base) < GTILE ? (n - base) : GTILE;
-    for (int p = threadIdx.x; p < valid; p += GT) {
-        const uint64_t k = skey[p];
-        const uint32_t d = group_digit(k, lb, ndig);
-        const int64_t o = (int64_t)gofs[d] + (p - (int64_t)lofs[d]);
-        kout[o] = k;
-        vout[o] = sval[p];
+#pragma unroll
+    for (int j = 0; j < GIPT; ++j) {
+        const int p = j * GT + (int)threadIdx.x;
+        if (p < valid) {
+            const uint64_t k = skey[p];
+            const uint32_t d = group_digit(k, lb, ndig);
+            const int64_t o = (int64_t)gofs[d] + (p - (int64_t)lofs[d]);
+            kout[o] = k;
+            vout[o] = sval[p];
+        }
     }
 }
 
