@@ -1356,6 +1356,7 @@ __device__ __forceinline__ void snap_publish(HostSnap *p, unsigned long long nnz
 // decision back) its cause becomes a sticky status bit.  The per-coordinate
 // loads (in bounds for every u < n) are issued before the decision is read,
 // then the per-row ones: two round trips before the writes.
+constexpr int WR_PER = 4;  // coordinates per thread of the write pass
 __global__ __launch_bounds__(TB) void k_upd_write(DevState *ds, uint32_t report, int64_t n,
                                                   const uint32_t *__restrict__ ut,
                                                   const uint32_t *__restrict__ trow, const uint32_t *__restrict__ tubeg,
@@ -1369,54 +1370,91 @@ __global__ __launch_bounds__(TB) void k_upd_write(DevState *ds, uint32_t report,
                                                   int32_t *__restrict__ row_len, int32_t *__restrict__ row_cap,
                                                   HostSnap *snap, uint64_t seq)
 {
-    const int64_t u = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    uint32_t t = 0, code = C_NOP, pos = 0, ui = 0, ud = 0;
-    int32_t col = 0;
-    float v = 0.0f;
-    if (u < n) {
-        t = ut[u];
-        code = ucode[u];
-        pos = upos[u];
-        v = uval[u];
-        col = ucol[u];
-        ui = upre[2 * u];
-        ud = upre[2 * u + 1];
+    // WR_PER coordinates per thread, TB apart, so that each of the pass's
+    // three rounds of loads (the records, their rows' constants, the row
+    // starts or prefixes) has every coordinate's loads in flight at once
+    const int64_t u0 = (int64_t)blockIdx.x * (TB * WR_PER) + threadIdx.x;
+    uint32_t t[WR_PER], code[WR_PER], pos[WR_PER], ui[WR_PER], ud[WR_PER];
+    int32_t col[WR_PER];
+    float v[WR_PER];
+#pragma unroll
+    for (int k = 0; k < WR_PER; ++k) {
+        const int64_t u = u0 + (int64_t)k * TB;
+        t[k] = code[k] = pos[k] = ui[k] = ud[k] = 0;
+        code[k] = C_NOP;
+        col[k] = 0;
+        v[k] = 0.0f;
+        if (u < n) {
+            t[k] = ut[u];
+            code[k] = ucode[u];
+            pos[k] = upos[u];
+            v[k] = uval[u];
+            col[k] = ucol[u];
+            ui[k] = upre[2 * u];
+            ud[k] = upre[2 * u + 1];
+        }
     }
     if (!ds->go) {
-        if (report && u == 0) ds->status |= ds->ubad_last ? ST_BAD_INDEX : ST_POOL_EXHAUSTED;
-        if (snap && u == 0) snap_publish(snap, ds->nnz, ds->bump, seq);
+        if (report && u0 == 0) ds->status |= ds->ubad_last ? ST_BAD_INDEX : ST_POOL_EXHAUSTED;
+        if (snap && u0 == 0) snap_publish(snap, ds->nnz, ds->bump, seq);
         return;
     }
     const uint32_t nu = ds->nuniq;
-    if (u == 0) {
+    if (u0 == 0) {
         const long long nnz = ds->nnz + ds->d_nnz;
         ds->nnz = nnz;
         ds->garbage += ds->d_garbage;
         if (snap) snap_publish(snap, (unsigned long long)nnz, ds->bump, seq);
     }
-    if (u >= nu) return;
-    const uint32_t mode = tmode[t];
-    const uint32_t r = trow[t];
-    const uint32_t ub = tubeg[t];
-    const uint32_t tp = tpre[2 * t];
-    const uint32_t nl = tnewl[t];
-    if (mode == M_VALUES) {
-        if (code == C_ADD) pval[(uint64_t)row_start[r] + pos] = v;
-        return;
+    if (u0 >= nu) return;
+    const uint64_t bsaved = ds->bump_saved;
+    bool in[WR_PER];
+    uint32_t mode[WR_PER], r[WR_PER], ub[WR_PER], tp[WR_PER], nl[WR_PER];
+#pragma unroll
+    for (int k = 0; k < WR_PER; ++k) {
+        in[k] = u0 + (int64_t)k * TB < (int64_t)nu;
+        const uint32_t tk = in[k] ? t[k] : 0u;
+        mode[k] = tmode[tk];
+        r[k] = trow[tk];
+        ub[k] = tubeg[tk];
+        tp[k] = tpre[2 * tk];
+        nl[k] = tnewl[tk];
     }
-    const uint64_t dest = ds->bump_saved + tp;
-    if (code == C_INS || code == C_ADD) {
-        // the inserts / deletes before it in the row shift it (an added-to
-        // entry was moved there with its old value)
-        const int64_t shift = (int64_t)(ui - upre[2 * ub]) - (int64_t)(ud - upre[2 * ub + 1]);
-        const uint64_t d = dest + (uint64_t)((int64_t)pos + shift);
-        if (code == C_INS) pcol[d] = col;
-        pval[d] = v;
+    // values in place need the row's pool start, a rewritten row's entries
+    // the inserts / deletes before the row
+    uint32_t a0[WR_PER], a1[WR_PER];
+#pragma unroll
+    for (int k = 0; k < WR_PER; ++k) {
+        a0[k] = a1[k] = 0;
+        if (!in[k]) continue;
+        if (mode[k] == M_VALUES) {
+            if (code[k] == C_ADD) a0[k] = row_start[r[k]];
+        } else {
+            a0[k] = upre[2 * ub[k]];
+            a1[k] = upre[2 * ub[k] + 1];
+        }
     }
-    if (u == ub) {
-        row_start[r] = (uint32_t)dest;
-        row_len[r] = (int32_t)nl;
-        row_cap[r] = (int32_t)chunk_round(nl);
+#pragma unroll
+    for (int k = 0; k < WR_PER; ++k) {
+        if (!in[k]) continue;
+        if (mode[k] == M_VALUES) {
+            if (code[k] == C_ADD) pval[(uint64_t)a0[k] + pos[k]] = v[k];
+            continue;
+        }
+        const uint64_t dest = bsaved + tp[k];
+        if (code[k] == C_INS || code[k] == C_ADD) {
+            // the inserts / deletes before it in the row shift it (an
+            // added-to entry was moved there with its old value)
+            const int64_t shift = (int64_t)(ui[k] - a0[k]) - (int64_t)(ud[k] - a1[k]);
+            const uint64_t d = dest + (uint64_t)((int64_t)pos[k] + shift);
+            if (code[k] == C_INS) pcol[d] = col[k];
+            pval[d] = v[k];
+        }
+        if (u0 + (int64_t)k * TB == (int64_t)ub[k]) {
+            row_start[r[k]] = (uint32_t)dest;
+            row_len[r[k]] = (int32_t)nl[k];
+            row_cap[r[k]] = (int32_t)chunk_round(nl[k]);
+        }
     }
 }
 
@@ -1673,7 +1711,7 @@ dcsr_status commit_launch(dcsr_handle h, const UpdBufs &b, int64_t n, int64_t mo
     }
     k_upd_move<<<(unsigned)(blocks + rsa.ntiles), TB, 0, s>>>(h->ds, b.tpre, b.trs, b.tubeg, b.ucode, b.upos, b.upre,
                                                               h->pool_col, h->pool_val, b.mspan, rsa);
-    k_upd_write<<<grid_for(n, TB), TB, 0, s>>>(h->ds, report ? 1u : 0u, n, b.ut, b.trow, b.tubeg, b.tpre, b.tnewl, b.tmode, b.ucol, b.uval,
+    k_upd_write<<<grid_for(n, TB * WR_PER), TB, 0, s>>>(h->ds, report ? 1u : 0u, n, b.ut, b.trow, b.tubeg, b.tpre, b.tnewl, b.tmode, b.ucol, b.uval,
                                                 b.upos, b.ucode, b.upre, h->pool_col, h->pool_val, h->row_start,
                                                 h->row_len, h->row_cap, seq ? h->dsnap : nullptr, seq);
     DCSR_HIP(hipGetLastError());
