@@ -140,7 +140,7 @@ constexpr int IPT = 8;
 constexpr int TILE = TB * IPT;
 // Positions of the probe space (touched rows end to end) and of the move
 // space (old entries of rewritten rows end to end) owned by one wave.
-constexpr uint32_t PR_SPAN = 2048;
+constexpr uint32_t PR_SPAN = 1024;
 constexpr uint32_t MV_SPAN = 1024;
 // batch tile table: heads / row heads (count, offset), claims / moved (sum, offset)
 enum { TT_H = 0, TT_R, TT_HX, TT_RX, TT_C, TT_M, TT_CX, TT_MX, TT_WORDS };
