@@ -141,7 +141,7 @@ constexpr int TILE = TB * IPT;
 // Positions of the probe space (touched rows end to end) and of the move
 // space (old entries of rewritten rows end to end) owned by one wave.
 constexpr uint32_t PR_SPAN = 1024;
-constexpr uint32_t MV_SPAN = 1024;
+constexpr uint32_t MV_SPAN = 512;
 // batch tile table: heads / row heads (count, offset), claims / moved (sum, offset)
 enum { TT_H = 0, TT_R, TT_HX, TT_RX, TT_C, TT_M, TT_CX, TT_MX, TT_WORDS };
 // probe tile table: inserts, adds, deletes; insert / delete offsets; probe
@@ -1691,7 +1691,7 @@ dcsr_status prepare_launch(dcsr_handle h, const int32_t *rows, const int32_t *co
 // Move + write launches.  move_bound bounds the old entries to move: the
 // move kernel strides over them with at most MV_BLOCKS workgroups (waves
 // past the device's count exit at once).
-constexpr int64_t MV_BLOCKS = 1024;
+constexpr int64_t MV_BLOCKS = 1280;
 dcsr_status commit_launch(dcsr_handle h, const UpdBufs &b, int64_t n, int64_t move_bound, bool report,
                           hipStream_t s, uint64_t seq = 0)
 {
